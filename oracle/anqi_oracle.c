@@ -1,0 +1,722 @@
+/*
+ * anqi_oracle.c -- CPU restatement of the ANQI approximate reverse-kNN path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  The product never links it.
+ * Compiled with -ffp-contract=off: every fused multiply-add that the
+ * reference's GCC build contracts (-ffp-contract=fast is GCC's default, the
+ * reference Makefile does not change it) is spelled out with fmaf()/fma().
+ *
+ * Pinning (what checks each function; tests/test_oracle_*.py):
+ *   or_ip/or_l2/or_norm/or_d2     float bits vs the reference's own functions
+ *                                 compiled from include/distance.h (harness in
+ *                                 oracle/ref_harness.cpp), random vectors,
+ *                                 d in {1..300}; closed form on integer data.
+ *   or_exact_topk                 dataset_gt golden (_baseknn_gt.bin) bit-exact;
+ *                                 brute force sort on tiny n.
+ *   or_rknn_membership            dataset_gt golden CSR for k=10/50/100.
+ *   or_heap_push                  harness HeapList::push_nolock on random
+ *                                 streams (same storage order).
+ *   or_queue_*                    harness NeighborPriorityQueue streams.
+ *   or_tree_entry, or_range_search  harness nndindex::range_search on the
+ *                                 reference graph (per-query candidate lists,
+ *                                 identical order) and #dist.
+ *   or_lift / or_lift_query       reference _approxlift file + graph meta
+ *                                 (bit-exact, single-thread reduction).
+ *   or_recheck_key                harness recheck key bits.
+ *   or_lrq_*                      reference ANQIVF2 artifact bytes (coeff and
+ *                                 codes) and harness materialize_k thresholds.
+ *   or_warm_start, or_nndescent_seq, or_gather, or_pool_topk, or_prune,
+ *   or_commit                     harness calls into the reference's
+ *                                 HeapList/nndindex methods on identical inputs
+ *                                 (1 thread), bit-exact.
+ *   or_nndescent_snapshot         not a reference schedule (the reference
+ *                                 LocalJoin is lock-order dependent); pinned
+ *                                 by invariants: every applied push obeys the
+ *                                 heap rule, heaps only improve, recall vs the
+ *                                 sequential schedule within tolerance.
+ */
+#include "anqi_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_EPS 1e-6f   /* include/distance.h:14 */
+
+/* GCC _mm512_reduce_add_ps tree (avx512fintrin.h, used by distance.h):
+ * T3[i]=a[i+8]+a[i]; T6[i]=T3[i+4]+T3[i]; T8=[T6[0]+T6[2], T6[1]+T6[3]];
+ * result T8[0]+T8[1]. */
+static inline float tree16(const float* a) {
+    float t3[8], t6[4], t80, t81;
+    for (int i = 0; i < 8; i++) t3[i] = a[i + 8] + a[i];
+    for (int i = 0; i < 4; i++) t6[i] = t3[i + 4] + t3[i];
+    t80 = t6[0] + t6[2];
+    t81 = t6[1] + t6[3];
+    return t80 + t81;
+}
+
+/* distance.h:22-49 / 89-115: 16 FMA lanes over full chunks, tree reduce,
+ * then the masked tail reduced (products, no FMA) and added. */
+float or_ip(const float* x, const float* y, uint32_t d) {
+    float acc[16];
+    for (int l = 0; l < 16; l++) acc[l] = 0.0f;
+    uint32_t m = d / 16, r = d % 16;
+    for (uint32_t t = 0; t < m; t++)
+        for (int l = 0; l < 16; l++) acc[l] = fmaf(x[16 * t + l], y[16 * t + l], acc[l]);
+    float res = tree16(acc);
+    if (r) {
+        float p[16];
+        for (int l = 0; l < 16; l++) p[l] = 0.0f;
+        for (uint32_t l = 0; l < r; l++) p[l] = x[16 * m + l] * y[16 * m + l];
+        res += tree16(p);
+    }
+    return res;
+}
+
+/* distance.h:58-86 */
+float or_l2(const float* x, const float* y, uint32_t d) {
+    float acc[16];
+    for (int l = 0; l < 16; l++) acc[l] = 0.0f;
+    uint32_t m = d / 16, r = d % 16;
+    for (uint32_t t = 0; t < m; t++)
+        for (int l = 0; l < 16; l++) {
+            float df = x[16 * t + l] - y[16 * t + l];
+            acc[l] = fmaf(df, df, acc[l]);
+        }
+    float res = tree16(acc);
+    if (r) {
+        float p[16];
+        for (int l = 0; l < 16; l++) p[l] = 0.0f;
+        for (uint32_t l = 0; l < r; l++) {
+            float df = x[16 * m + l] - y[16 * m + l];
+            p[l] = df * df;
+        }
+        res += tree16(p);
+    }
+    return res;
+}
+
+/* distance.h:143-161 (same structure as ip(x, x)). */
+float or_norm(const float* x, uint32_t d) { return or_ip(x, x, d); }
+
+/* index.h:172-176.  na + nb - 2*ip: 2*ip is exact, so the contracted
+ * fnmadd and the separate subtract give the same bits. */
+float or_d2(const float* x, const float* y, float nx, float ny, uint32_t d) {
+    float ip = or_ip(x, y, d);
+    return fmaf(-2.0f, ip, nx + ny);
+}
+
+void or_norms(const float* x, uint64_t n, uint32_t d, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) out[i] = or_norm(x + (size_t)i * d, d);
+}
+
+/* ---------------------------------------------------------------------- */
+static int pair_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+static void sift(float* d, uint32_t* id, uint32_t m, uint32_t root) {
+    for (;;) {
+        uint32_t l = 2 * root + 1, big = root;
+        if (l < m && pair_less(d[big], id[big], d[l], id[l])) big = l;
+        if (l + 1 < m && pair_less(d[big], id[big], d[l + 1], id[l + 1])) big = l + 1;
+        if (big == root) return;
+        float td = d[root]; d[root] = d[big]; d[big] = td;
+        uint32_t ti = id[root]; id[root] = id[big]; id[big] = ti;
+        root = big;
+    }
+}
+
+/* Heap sort by (dist, id) ascending; deterministic, no allocation. */
+void or_sort_pairs(float* d, uint32_t* id, uint32_t m) {
+    if (m < 2) return;
+    for (int64_t r = (int64_t)m / 2 - 1; r >= 0; r--) sift(d, id, m, (uint32_t)r);
+    for (uint32_t end = m - 1; end > 0; end--) {
+        float td = d[0]; d[0] = d[end]; d[end] = td;
+        uint32_t ti = id[0]; id[0] = id[end]; id[end] = ti;
+        sift(d, id, end, 0);
+    }
+}
+
+/* dataset_gt.cpp:206-245: key = fnmadd(2, ip, bn_i + bn_j); strict-< max-heap
+ * over ascending j gives the K smallest (key, id) pairs. */
+int or_exact_topk(const float* base, uint32_t n, uint32_t d, uint32_t K,
+                  uint32_t* ids, float* keys, int threads) {
+    if (K >= n) return -1;
+    float* bn = (float*)malloc(sizeof(float) * n);
+    or_norms(base, n, d, bn);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel
+    {
+        float* kd = (float*)malloc(sizeof(float) * (K + 1));
+        uint32_t* ki = (uint32_t*)malloc(sizeof(uint32_t) * (K + 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            uint32_t i = (uint32_t)ii, cnt = 0;
+            const float* xi = base + (size_t)i * d;
+            /* bounded sorted insertion of (key, j) keeping the K smallest */
+            for (uint32_t j = 0; j < n; j++) {
+                if (j == i) continue;
+                float ip = or_ip(xi, base + (size_t)j * d, d);
+                float key = fmaf(-2.0f, ip, bn[i] + bn[j]);
+                if (cnt == K && !pair_less(key, j, kd[K - 1], ki[K - 1])) continue;
+                uint32_t p = cnt < K ? cnt : K - 1;
+                while (p > 0 && pair_less(key, j, kd[p - 1], ki[p - 1])) {
+                    kd[p] = kd[p - 1]; ki[p] = ki[p - 1]; p--;
+                }
+                kd[p] = key; ki[p] = j;
+                if (cnt < K) cnt++;
+            }
+            memcpy(ids + (size_t)i * K, ki, sizeof(uint32_t) * K);
+            memcpy(keys + (size_t)i * K, kd, sizeof(float) * K);
+        }
+        free(kd); free(ki);
+    }
+    free(bn);
+    return 0;
+}
+
+/* dataset_gt.cpp:283-313: hit iff qn + bn_j - 2 ip <= rk2[j]; rows sorted by
+ * (key, id). */
+uint64_t or_rknn_membership(const float* base, uint32_t n, const float* query,
+                            uint32_t nq, uint32_t d, const float* rk2,
+                            uint32_t* off, uint32_t* ids, float* keys, int threads) {
+    float* bn = (float*)malloc(sizeof(float) * n);
+    or_norms(base, n, d, bn);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    uint32_t* cnt = (uint32_t*)calloc(nq, sizeof(uint32_t));
+#pragma omp parallel
+    {
+        float* hd = (float*)malloc(sizeof(float) * n);
+        uint32_t* hi = (uint32_t*)malloc(sizeof(uint32_t) * n);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t qq = 0; qq < (int64_t)nq; qq++) {
+            const float* q = query + (size_t)qq * d;
+            float qn = or_norm(q, d);
+            uint32_t c = 0;
+            for (uint32_t j = 0; j < n; j++) {
+                float ip = or_ip(q, base + (size_t)j * d, d);
+                float key = fmaf(-2.0f, ip, qn + bn[j]);
+                if (key <= rk2[j]) { hd[c] = key; hi[c] = j; c++; }
+            }
+            cnt[qq] = c;
+            if (ids) {
+                or_sort_pairs(hd, hi, c);
+                memcpy(ids + off[qq], hi, sizeof(uint32_t) * c);
+                memcpy(keys + off[qq], hd, sizeof(float) * c);
+            }
+        }
+        free(hd); free(hi);
+    }
+    uint64_t total = 0;
+    if (!ids) {
+        off[0] = 0;
+        for (uint32_t q = 0; q < nq; q++) { total += cnt[q]; off[q + 1] = (uint32_t)total; }
+    } else {
+        total = off[nq];
+    }
+    free(cnt); free(bn);
+    return total;
+}
+
+/* heap.h:83-186 (push_nolock == check_push with update_reverse=false). */
+int or_heap_push(uint32_t* idx, float* val, uint32_t cap, uint32_t node, float value) {
+    if (value > val[0]) return 0;
+    for (uint32_t i = 0; i < cap; i++)
+        if (idx[i] == node) return 0;
+    uint32_t cur = 0, sw;
+    for (;;) {
+        uint32_t l = 2 * cur + 1, r = l + 1;
+        if (l >= cap) break;
+        else if (r >= cap) { if (val[l] > value) sw = l; else break; }
+        else if (val[l] >= val[r]) { if (val[l] > value) sw = l; else break; }
+        else { if (val[r] > value) sw = r; else break; }
+        idx[cur] = idx[sw]; val[cur] = val[sw]; cur = sw;
+    }
+    idx[cur] = node; val[cur] = value;
+    return 1;
+}
+
+/* neighbor.h:43-141 */
+void or_queue_init(or_queue* q, float* dist, uint32_t* pid, uint32_t cap) {
+    q->dist = dist; q->pid = pid; q->size = 0; q->cap = cap; q->cur = 0;
+}
+
+#define QID(p) ((p) & 0x7FFFFFFFu)
+
+int or_queue_insert(or_queue* q, uint32_t id, float dist) {
+    if (q->cap == 0) return 0;
+    if (q->size == q->cap &&
+        pair_less(q->dist[q->size - 1], QID(q->pid[q->size - 1]), dist, id)) return 0;
+    uint32_t lo = 0, hi = q->size;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pair_less(dist, id, q->dist[mid], QID(q->pid[mid]))) hi = mid;
+        else if (QID(q->pid[mid]) == id) return 0;
+        else lo = mid + 1;
+    }
+    if (lo < q->cap) {
+        memmove(q->dist + lo + 1, q->dist + lo, sizeof(float) * (q->size - lo));
+        memmove(q->pid + lo + 1, q->pid + lo, sizeof(uint32_t) * (q->size - lo));
+    }
+    q->dist[lo] = dist; q->pid[lo] = id;
+    if (q->size < q->cap) q->size++;
+    if (lo < q->cur) q->cur = lo;
+    return 1;
+}
+
+uint32_t or_queue_pop_closest(or_queue* q) {
+    q->pid[q->cur] |= 0x80000000u;
+    uint32_t pre = q->cur;
+    while (q->cur < q->size && (q->pid[q->cur] >> 31)) q->cur++;
+    return QID(q->pid[pre]);
+}
+
+/* rptrees.h:108-123.  Ties (|margin| < EPS) go left here; the reference
+ * draws a random side, a measure-zero event on continuous data. */
+uint32_t or_tree_entry(const or_tree* t, const float* q) {
+    uint64_t k = t->nodes - 1;
+    while (t->left[k] != (uint64_t)UINT32_MAX) {
+        float margin = or_ip(t->hyper + (size_t)k * t->d, q, t->d) - t->offset[k];
+        if (margin < -OR_EPS) k = t->left[k];
+        else if (margin > OR_EPS) k = t->right[k];
+        else k = t->left[k];
+    }
+    if (t->idx_off[k + 1] == t->idx_off[k]) return 0;
+    return t->idx[t->idx_off[k]];
+}
+
+/* index.h:1535-1652 with radj == NULL, eps = 0, CENTER score: the collect and
+ * expand thresholds are both R2, compared as double(float D) <= R2. */
+uint32_t or_range_search(const float* lifted, uint32_t n, uint32_t dl,
+                         const uint32_t* deg, const uint32_t* links, uint32_t M,
+                         uint32_t entry, const float* q, double R2, uint32_t L,
+                         uint32_t* out, uint32_t cap, uint64_t* ndist) {
+    uint8_t* vis = (uint8_t*)calloc(n, 1);
+    float* qd = (float*)malloc(sizeof(float) * (L + 1));
+    uint32_t* qp = (uint32_t*)malloc(sizeof(uint32_t) * (L + 1));
+    uint32_t* flood = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t fsz = 0, fhead = 0, nres = 0;
+    uint64_t nd = 0;
+    or_queue beam;
+    or_queue_init(&beam, qd, qp, L);
+#define PUSH_RES(id) do { if (nres < cap) out[nres] = (id); nres++; } while (0)
+    uint32_t ep = entry;
+    float dep = or_l2(lifted + (size_t)ep * dl, q, dl); nd++;
+    vis[ep] = 1;
+    or_queue_insert(&beam, ep, dep);
+    if ((double)dep <= R2) flood[fsz++] = ep;
+    if ((double)dep <= R2) PUSH_RES(ep);
+    while (beam.cur < beam.size) {
+        uint32_t cur = or_queue_pop_closest(&beam);
+        for (uint32_t j = 0; j < deg[cur]; j++) {
+            uint32_t nb = links[(size_t)cur * M + j];
+            if (vis[nb]) continue;
+            vis[nb] = 1;
+            float dd = or_l2(lifted + (size_t)nb * dl, q, dl); nd++;
+            or_queue_insert(&beam, nb, dd);
+            if ((double)dd <= R2) flood[fsz++] = nb;
+            if ((double)dd <= R2) PUSH_RES(nb);
+        }
+    }
+    while (fhead < fsz) {
+        uint32_t cur = flood[fhead++];
+        for (uint32_t j = 0; j < deg[cur]; j++) {
+            uint32_t nb = links[(size_t)cur * M + j];
+            if (vis[nb]) continue;
+            vis[nb] = 1;
+            float dd = or_l2(lifted + (size_t)nb * dl, q, dl); nd++;
+            if ((double)dd <= R2) PUSH_RES(nb);
+            if ((double)dd <= R2) flood[fsz++] = nb;
+        }
+    }
+#undef PUSH_RES
+    free(vis); free(qd); free(qp); free(flood);
+    if (ndist) *ndist = nd;
+    return nres;
+}
+
+/* rknn_query.cpp:2719-2730 (GRAPH_RANKM == 0, L2). */
+void or_lift_query(const float* q, uint32_t d, double scale, float* qb) {
+    for (uint32_t j = 0; j < d; j++) qb[j] = (float)((double)q[j] * scale);
+    qb[d] = 1.0f;
+    qb[d + 1] = 0.0f;
+}
+
+/* rknn_query.cpp:2555-2561 */
+float or_recheck_key(const float* o, const float* q, uint32_t d) {
+    float ip = or_ip(o, q, d);
+    float on = or_ip(o, o, d);
+    float qn = or_ip(q, q, d);
+    return fmaf(-2.0f, ip, on + qn);
+}
+
+/* rknn_query.cpp:482-513, policy "floor": upper_bound on float(value). */
+uint32_t or_lrq_encode_value(double v, const float* centers, uint32_t code_size) {
+    float fv = (float)v;
+    uint32_t lo = 0, hi = code_size;   /* first index with centers[i] > fv */
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (fv < centers[mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo == 0 ? 0 : lo - 1;
+}
+
+/* rknn_query.cpp:679-720 (bits 8).  GCC contracts the products into fma. */
+void or_lrq_encode(const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
+                   const float* mu, const float* basis, const float* codebook,
+                   uint32_t code_size, float* coeff, uint8_t* codes) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        const float* rk = cutoffs + (size_t)ii * nk;
+        float* ci = coeff + (size_t)ii * rankm;
+        for (uint32_t m = 0; m < rankm; m++) {
+            const float* gm = basis + (size_t)m * nk;
+            double c = 0.0;
+            for (uint32_t k = 0; k < nk; k++)
+                c = fma((double)rk[k] - (double)mu[k], (double)gm[k], c);
+            ci[m] = (float)c;
+        }
+        for (uint32_t k = 0; k < nk; k++) {
+            double recon = (double)mu[k];
+            for (uint32_t m = 0; m < rankm; m++)
+                recon = fma((double)ci[m], (double)basis[(size_t)m * nk + k], recon);
+            double residual = (double)rk[k] - recon;
+            codes[(size_t)ii * nk + k] =
+                (uint8_t)or_lrq_encode_value(residual, codebook + (size_t)k * code_size, code_size);
+        }
+    }
+}
+
+/* rknn_query.cpp:722-753 (bits 8, L2). */
+void or_lrq_materialize(const float* coeff, const uint8_t* codes, uint64_t n,
+                        uint32_t nk, uint32_t rankm, const float* mu,
+                        const float* basis, const float* codebook,
+                        uint32_t code_size, uint32_t k_one_based, double slack,
+                        float* thr) {
+    uint32_t k = k_one_based - 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        const float* ci = coeff + (size_t)ii * rankm;
+        double value = (double)mu[k];
+        for (uint32_t m = 0; m < rankm; m++)
+            value = fma((double)ci[m], (double)basis[(size_t)m * nk + k], value);
+        value += (double)codebook[(size_t)k * code_size + codes[(size_t)ii * nk + k]];
+        double scaled = slack * value;
+        thr[ii] = (float)(0.0 < scaled ? scaled : 0.0);
+    }
+}
+
+/* rknn_query.cpp:2203-2236 (GRAPH_RANKM == 0, L2, postfilter horizon). */
+void or_lift(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
+             float* lifted, double* mstar2, double* scale) {
+    double sum_n2 = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+        const float* o = base + i * d;
+        double s = 0.0;
+        for (uint32_t j = 0; j < d; j++) s = fma((double)o[j], (double)o[j], s);
+        sum_n2 += s;
+    }
+    double s_scale = 1.0 / sqrt(sum_n2 / (double)n + 1e-12), sc2 = s_scale * s_scale;
+    uint32_t dl = d + 1, dout = d + 2;
+    double maxn2 = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+        const float* o = base + i * d;
+        float* h = lifted + i * dout;
+        double nr2 = 0.0;
+        for (uint32_t j = 0; j < d; j++) {
+            double v = (double)o[j] * s_scale;
+            h[j] = (float)v;
+            nr2 = fma(v, v, nr2);
+        }
+        h[d] = (float)(0.5 * fma((double)rk_horizon[i], sc2, -nr2));
+        double hn2 = fma((double)h[d], (double)h[d], nr2);
+        if (hn2 > maxn2) maxn2 = hn2;
+    }
+    double M2 = maxn2 * 1.0001;
+    for (uint64_t i = 0; i < n; i++) {
+        float* h = lifted + i * dout;
+        double hn2 = 0.0;
+        for (uint32_t j = 0; j < dl; j++) hn2 = fma((double)h[j], (double)h[j], hn2);
+        double pad = M2 - hn2;
+        if (pad < 0) pad = 0;
+        h[dl] = (float)sqrt(pad);
+    }
+    *mstar2 = M2;
+    *scale = s_scale;
+}
+
+/* index.h:1204-1218 (flat warm start). */
+void or_warm_start(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                   const uint32_t* ids, uint32_t width, uint32_t cap,
+                   uint32_t* hidx, float* hval) {
+    for (size_t t = 0; t < (size_t)n * cap; t++) { hidx[t] = UINT32_MAX; hval[t] = FLT_MAX; }
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        uint32_t i = (uint32_t)ii;
+        for (uint32_t k = 0; k < width; k++) {
+            uint32_t j = ids[(size_t)i * width + k];
+            if (j != i && j < n)
+                or_heap_push(hidx + (size_t)i * cap, hval + (size_t)i * cap, cap, j,
+                             or_d2(x + (size_t)i * d, x + (size_t)j * d, nrm[i], nrm[j], d));
+        }
+    }
+}
+
+static uint32_t take_sample(const uint32_t* row, uint32_t cap, int sample, uint32_t* loc) {
+    uint32_t c = 0;
+    for (uint32_t s = 0; s < cap; s++) {
+        if (row[s] != UINT32_MAX) { loc[c++] = row[s]; if ((int)c >= sample) break; }
+    }
+    return c;
+}
+
+/* index.h:1468-1487 with one thread. */
+int or_nndescent_seq(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                     uint32_t cap, uint32_t* hidx, float* hval, int iters,
+                     int sample, int64_t* upd) {
+    uint32_t loc[128];
+    if (sample > 120) sample = 120;
+    int it;
+    for (it = 0; it < iters; it++) {
+        int64_t u_cnt = 0;
+        for (uint32_t u = 0; u < n; u++) {
+            uint32_t cnt = take_sample(hidx + (size_t)u * cap, cap, sample, loc);
+            for (uint32_t a = 0; a < cnt; a++)
+                for (uint32_t b = a + 1; b < cnt; b++) {
+                    uint32_t v = loc[a], w = loc[b];
+                    float dd = or_d2(x + (size_t)v * d, x + (size_t)w * d, nrm[v], nrm[w], d);
+                    u_cnt += or_heap_push(hidx + (size_t)v * cap, hval + (size_t)v * cap, cap, w, dd);
+                    u_cnt += or_heap_push(hidx + (size_t)w * cap, hval + (size_t)w * cap, cap, v, dd);
+                }
+        }
+        if (upd) upd[it] = u_cnt;
+        if (u_cnt < (int64_t)(n / 200)) { it++; break; }
+    }
+    return it;
+}
+
+typedef struct { uint32_t row, id; float d; } or_prop;
+
+static int prop_cmp(const void* a, const void* b) {
+    const or_prop* x = (const or_prop*)a;
+    const or_prop* y = (const or_prop*)b;
+    if (x->row != y->row) return x->row < y->row ? -1 : 1;
+    if (x->d != y->d) return x->d < y->d ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return 0;
+}
+
+int or_nndescent_snapshot(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                          uint32_t cap, uint32_t* hidx, float* hval, int iters,
+                          int sample, int64_t* upd) {
+    if (sample > 120) sample = 120;
+    uint32_t* samp = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * sample);
+    uint32_t* scnt = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    int it;
+    for (it = 0; it < iters; it++) {
+        size_t np = 0;
+        for (uint32_t u = 0; u < n; u++) {
+            scnt[u] = take_sample(hidx + (size_t)u * cap, cap, sample, samp + (size_t)u * sample);
+            np += (size_t)scnt[u] * (scnt[u] - 1);
+        }
+        or_prop* P = (or_prop*)malloc(sizeof(or_prop) * (np ? np : 1));
+        size_t k = 0;
+        for (uint32_t u = 0; u < n; u++) {
+            const uint32_t* loc = samp + (size_t)u * sample;
+            for (uint32_t a = 0; a < scnt[u]; a++)
+                for (uint32_t b = a + 1; b < scnt[u]; b++) {
+                    uint32_t v = loc[a], w = loc[b];
+                    float dd = or_d2(x + (size_t)v * d, x + (size_t)w * d, nrm[v], nrm[w], d);
+                    P[k].row = v; P[k].id = w; P[k].d = dd; k++;
+                    P[k].row = w; P[k].id = v; P[k].d = dd; k++;
+                }
+        }
+        qsort(P, np, sizeof(or_prop), prop_cmp);
+        int64_t u_cnt = 0;
+        for (size_t t = 0; t < np; t++)
+            u_cnt += or_heap_push(hidx + (size_t)P[t].row * cap, hval + (size_t)P[t].row * cap,
+                                  cap, P[t].id, P[t].d);
+        free(P);
+        if (upd) upd[it] = u_cnt;
+        if (u_cnt < (int64_t)(n / 200)) { it++; break; }
+    }
+    free(samp); free(scnt);
+    return it;
+}
+
+/* heap.h:190-207 in row order (the single-thread order). */
+void or_reverse_csr(const uint32_t* hidx, uint32_t n, uint32_t cap,
+                    uint32_t* roff, uint32_t* rids) {
+    memset(roff, 0, sizeof(uint32_t) * (n + 1));
+    for (size_t t = 0; t < (size_t)n * cap; t++)
+        if (hidx[t] < n) roff[hidx[t] + 1]++;
+    for (uint32_t i = 0; i < n; i++) roff[i + 1] += roff[i];
+    uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    memcpy(cur, roff, sizeof(uint32_t) * n);
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t s = 0; s < cap; s++) {
+            uint32_t j = hidx[(size_t)i * cap + s];
+            if (j < n) rids[cur[j]++] = i;
+        }
+    free(cur);
+}
+
+/* index.h:1491-1525 */
+uint32_t or_gather(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                   const uint32_t* hidx, const float* hval, uint32_t cap,
+                   const uint32_t* roff, const uint32_t* rids, uint32_t i,
+                   uint32_t explore, uint32_t pool_cap, float* pool_d,
+                   uint32_t* pool_id, uint32_t pool_max) {
+    uint8_t* vis = (uint8_t*)calloc(n, 1);
+    float* qd = (float*)malloc(sizeof(float) * (explore + 1));
+    uint32_t* qp = (uint32_t*)malloc(sizeof(uint32_t) * (explore + 1));
+    or_queue cq;
+    or_queue_init(&cq, qd, qp, explore);
+    uint32_t m = 0;
+#define PUSHP(id_, d_) do { if (m < pool_max) { pool_d[m] = (d_); pool_id[m] = (id_); } m++; } while (0)
+    const float* xi = x + (size_t)i * d;
+    for (uint32_t s = 0; s < cap; s++) {
+        uint32_t id = hidx[(size_t)i * cap + s];
+        if (id == UINT32_MAX || id >= n || vis[id]) continue;
+        vis[id] = 1;
+        float dv = hval[(size_t)i * cap + s];
+        or_queue_insert(&cq, id, dv);
+        PUSHP(id, dv);
+    }
+    while (cq.cur < cq.size) {
+        uint32_t cur = or_queue_pop_closest(&cq);
+        for (uint32_t s = 0; s < cap; s++) {
+            uint32_t nb = hidx[(size_t)cur * cap + s];
+            if (nb == UINT32_MAX || nb >= n || vis[nb]) continue;
+            vis[nb] = 1;
+            float dv = or_d2(xi, x + (size_t)nb * d, nrm[i], nrm[nb], d);
+            or_queue_insert(&cq, nb, dv);
+            PUSHP(nb, dv);
+        }
+        for (uint32_t p = roff[cur]; p < roff[cur + 1]; p++) {
+            uint32_t nb = rids[p];
+            if (nb == UINT32_MAX || nb >= n || vis[nb]) continue;
+            vis[nb] = 1;
+            float dv = or_d2(xi, x + (size_t)nb * d, nrm[i], nrm[nb], d);
+            or_queue_insert(&cq, nb, dv);
+            PUSHP(nb, dv);
+        }
+        if (m >= pool_cap) break;
+    }
+#undef PUSHP
+    free(vis); free(qd); free(qp);
+    return m;
+}
+
+uint32_t or_pool_topk(uint32_t self, const float* pool_d, const uint32_t* pool_id,
+                      uint32_t m, uint32_t k, uint32_t* ids, float* keys) {
+    float* d = (float*)malloc(sizeof(float) * (m ? m : 1));
+    uint32_t* id = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+    memcpy(d, pool_d, sizeof(float) * m);
+    memcpy(id, pool_id, sizeof(uint32_t) * m);
+    or_sort_pairs(d, id, m);
+    uint32_t c = 0;
+    for (uint32_t t = 0; t < m && c < k; t++) {
+        if (id[t] == self) continue;
+        ids[c] = id[t]; keys[c] = d[t]; c++;
+    }
+    free(d); free(id);
+    return c;
+}
+
+/* index.h:225-248 + 267-281 + default branch of 772-829. */
+uint32_t or_prune(const float* x, const float* nrm, uint32_t d, uint32_t base,
+                  uint32_t n, float* pool_d, uint32_t* pool_id, uint32_t m,
+                  uint32_t M, float alpha, uint32_t* out) {
+    /* drop sentinel/out-of-range/self, keep min dist per id */
+    uint32_t w = 0;
+    for (uint32_t t = 0; t < m; t++) {
+        uint32_t id = pool_id[t];
+        if (id == UINT32_MAX || id >= n || id == base) continue;
+        pool_id[w] = id; pool_d[w] = pool_d[t]; w++;
+    }
+    /* sort by (id, dist), keep the minimum distance per id */
+    {
+        or_prop* P = (or_prop*)malloc(sizeof(or_prop) * (w ? w : 1));
+        for (uint32_t t = 0; t < w; t++) { P[t].row = pool_id[t]; P[t].id = 0; P[t].d = pool_d[t]; }
+        qsort(P, w, sizeof(or_prop), prop_cmp);   /* (row=id, d) order */
+        uint32_t u = 0;
+        for (uint32_t r = 0; r < w;) {
+            uint32_t e = r + 1;
+            while (e < w && P[e].row == P[r].row) e++;
+            pool_id[u] = P[r].row; pool_d[u] = P[r].d; u++;   /* first = min d */
+            r = e;
+        }
+        free(P);
+        w = u;
+    }
+    or_sort_pairs(pool_d, pool_id, w);
+    uint32_t deg = 0;
+    for (int pass = 0; pass < 2; pass++) {   /* rng_extend(pool, M) twice */
+        for (uint32_t t = 0; t < w; t++) {
+            if (deg >= M) break;
+            uint32_t cid = pool_id[t];
+            float d2c = pool_d[t];
+            int sel = 0;
+            for (uint32_t s = 0; s < deg; s++) if (out[s] == cid) { sel = 1; break; }
+            if (sel) continue;
+            int occ = 0;
+            for (uint32_t s = 0; s < deg; s++) {
+                uint32_t sid = out[s];
+                float dsc = or_d2(x + (size_t)sid * d, x + (size_t)cid * d, nrm[sid], nrm[cid], d);
+                if (alpha * dsc < d2c) { occ = 1; break; }
+            }
+            if (!occ) out[deg++] = cid;
+        }
+    }
+    return deg;
+}
+
+/* index.h:907-930 */
+void or_commit(const float* x, const float* nrm, uint32_t n, uint32_t d,
+               const uint32_t* O, const uint32_t* ocnt, uint32_t M, float alpha,
+               uint32_t* links, uint32_t* deg) {
+    uint32_t* roff = (uint32_t*)calloc(n + 1, sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t t = 0; t < ocnt[i]; t++) roff[O[(size_t)i * M + t] + 1]++;
+    for (uint32_t i = 0; i < n; i++) roff[i + 1] += roff[i];
+    uint32_t* rids = (uint32_t*)malloc(sizeof(uint32_t) * (roff[n] ? roff[n] : 1));
+    uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    memcpy(cur, roff, sizeof(uint32_t) * n);
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t t = 0; t < ocnt[i]; t++) { uint32_t j = O[(size_t)i * M + t]; rids[cur[j]++] = i; }
+#pragma omp parallel
+    {
+        float* pd = (float*)malloc(sizeof(float) * (M + n));
+        uint32_t* pi = (uint32_t*)malloc(sizeof(uint32_t) * (M + n));
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            uint32_t i = (uint32_t)ii, m = 0;
+            for (uint32_t t = 0; t < ocnt[i]; t++) {
+                uint32_t j = O[(size_t)i * M + t];
+                pd[m] = or_d2(x + (size_t)i * d, x + (size_t)j * d, nrm[i], nrm[j], d); pi[m] = j; m++;
+            }
+            for (uint32_t p = roff[i]; p < roff[i + 1]; p++) {
+                uint32_t j = rids[p];
+                pd[m] = or_d2(x + (size_t)i * d, x + (size_t)j * d, nrm[i], nrm[j], d); pi[m] = j; m++;
+            }
+            deg[i] = or_prune(x, nrm, d, i, n, pd, pi, m, M, alpha, links + (size_t)i * M);
+        }
+        free(pd); free(pi);
+    }
+    free(roff); free(rids); free(cur);
+}

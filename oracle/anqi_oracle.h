@@ -1,0 +1,151 @@
+/*
+ * anqi_oracle.h -- CPU restatement of the ANQI approximate reverse-kNN path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
+ * command-line tools, bench.py's timed region) may link, load or call this
+ * code.  tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it
+ * as the checker.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Pinning status of each function (which
+ * golden vector / known-answer test / invariant checks it) is listed in the
+ * header comment of anqi_oracle.c.
+ */
+#ifndef ANQI_ORACLE_H
+#define ANQI_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- FP32 distance contract (include/distance.h) ------------------------ */
+float or_ip(const float* x, const float* y, uint32_t d);      /* distance.h:22-49, 89-115 */
+float or_l2(const float* x, const float* y, uint32_t d);      /* distance.h:58-86 */
+float or_norm(const float* x, uint32_t d);                    /* distance.h:143-161 */
+/* nndindex::d2, include/index.h:172-176: na + nb - 2*ip (contracted). */
+float or_d2(const float* x, const float* y, float nx, float ny, uint32_t d);
+void or_norms(const float* x, uint64_t n, uint32_t d, float* out);
+
+/* ---- exact top-K and RkNN ground truth (tools/dataset_gt.cpp) ------------ */
+/* dataset_gt.cpp:206-245.  ids/keys: n*K, rows sorted by (key, id). */
+int or_exact_topk(const float* base, uint32_t n, uint32_t d, uint32_t K,
+                  uint32_t* ids, float* keys, int threads);
+/* dataset_gt.cpp:283-313.  Returns total hits; off has nq+1 entries.  Call
+ * with ids==NULL to size, then again with buffers of off[nq] entries. */
+uint64_t or_rknn_membership(const float* base, uint32_t n, const float* query,
+                            uint32_t nq, uint32_t d, const float* rk2,
+                            uint32_t* off, uint32_t* ids, float* keys, int threads);
+
+/* ---- bounded max-heap rows (include/heap.h) ------------------------------ */
+/* HeapList::push_nolock / check_push(update_reverse=false), heap.h:83-186.
+ * idx/val: one row of capacity cap (sentinel UINT32_MAX / FLT_MAX). */
+int or_heap_push(uint32_t* idx, float* val, uint32_t cap, uint32_t node, float value);
+
+/* ---- sorted candidate queue (include/neighbor.h) ------------------------- */
+typedef struct {
+    float* dist;
+    uint32_t* pid;   /* bit31 = expanded */
+    uint32_t size, cap, cur;
+} or_queue;
+void or_queue_init(or_queue* q, float* dist, uint32_t* pid, uint32_t cap);
+int or_queue_insert(or_queue* q, uint32_t id, float dist);          /* neighbor.h:61-102 */
+uint32_t or_queue_pop_closest(or_queue* q);                         /* neighbor.h:105-111 */
+
+/* ---- RP tree (include/rptrees.h) ----------------------------------------- */
+/* A tree in flat form: node k is internal if left[k] != UINT32_MAX.
+ * hyper: nodes*d floats (only internal nodes meaningful); leaf_off/leaf_ids
+ * give each leaf's indices in stored order.  root = nodes-1 (post-order). */
+typedef struct {
+    uint32_t nodes, d;
+    const uint64_t* left;
+    const uint64_t* right;
+    const float* offset;
+    const float* hyper;
+    const uint64_t* idx_off;   /* nodes+1 */
+    const uint32_t* idx;       /* leaf indices */
+} or_tree;
+/* rptree::get_leave_mediod, rptrees.h:108-123 (ties |margin|<EPS go left). */
+uint32_t or_tree_entry(const or_tree* t, const float* q);
+
+/* ---- range search (include/index.h:1535-1652, CENTER mode, radj==NULL) --- */
+/* lifted: n*dl, deg: n, links: n*M.  Writes up to cap candidate ids in
+ * discovery order; returns count (may exceed cap: truncated).  ndist out. */
+uint32_t or_range_search(const float* lifted, uint32_t n, uint32_t dl,
+                         const uint32_t* deg, const uint32_t* links, uint32_t M,
+                         uint32_t entry, const float* q, double R2, uint32_t L,
+                         uint32_t* out, uint32_t cap, uint64_t* ndist);
+
+/* Query transform, src/rknn_query.cpp:2719-2730 (anyk_lift, L2). */
+void or_lift_query(const float* q, uint32_t d, double scale, float* qb);
+
+/* Postfilter key, src/rknn_query.cpp:2555-2561. */
+float or_recheck_key(const float* o, const float* q, uint32_t d);
+
+/* ---- LRQ verifier (src/rknn_query.cpp:603-839), FP64 on the host --------- */
+uint32_t or_lrq_encode_value(double v, const float* centers, uint32_t code_size); /* :482-513 floor */
+void or_lrq_encode(const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
+                   const float* mu, const float* basis, const float* codebook,
+                   uint32_t code_size, float* coeff, uint8_t* codes);            /* :679-720 */
+void or_lrq_materialize(const float* coeff, const uint8_t* codes, uint64_t n,
+                        uint32_t nk, uint32_t rankm, const float* mu,
+                        const float* basis, const float* codebook,
+                        uint32_t code_size, uint32_t k_one_based, double slack,
+                        float* thr);                                              /* :722-753 */
+
+/* ---- lift (src/rknn_query.cpp:2203-2236), FP64 on the host --------------- */
+/* Writes n*(d+2) lifted floats; returns M*^2 and scale through pointers.
+ * sum of squared norms is accumulated in row order (the reference uses an
+ * OpenMP reduction; identical with one thread). */
+void or_lift(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
+             float* lifted, double* mstar2, double* scale);
+
+/* ---- graph build pieces (include/index.h) -------------------------------- */
+/* Warm start, index.h:1204-1218: push width AKNN ids with lifted d2 into rows
+ * of capacity cap (heap arrays n*cap, initialised here). */
+void or_warm_start(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                   const uint32_t* ids, uint32_t width, uint32_t cap,
+                   uint32_t* hidx, float* hval);
+/* nndescent_refine, index.h:1468-1487, single-threaded order (u ascending,
+ * updates visible immediately).  Returns passes run; upd per pass in upd. */
+int or_nndescent_seq(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                     uint32_t cap, uint32_t* hidx, float* hval, int iters,
+                     int sample, int64_t* upd);
+/* Deterministic snapshot schedule used by the GPU: each pass reads every row's
+ * sample at pass start, then applies each row's proposals in ascending
+ * (dist, id) order.  Same heap semantics. */
+int or_nndescent_snapshot(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                          uint32_t cap, uint32_t* hidx, float* hval, int iters,
+                          int sample, int64_t* upd);
+/* HeapList::push_reverse, heap.h:190-207, in row order (1 thread). */
+void or_reverse_csr(const uint32_t* hidx, uint32_t n, uint32_t cap,
+                    uint32_t* roff, uint32_t* rids);
+/* gather_candidates, index.h:1491-1525.  pool_d/pool_id receive the pool in
+ * push order (capacity pool_max); returns pool size. */
+uint32_t or_gather(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                   const uint32_t* hidx, const float* hval, uint32_t cap,
+                   const uint32_t* roff, const uint32_t* rids, uint32_t i,
+                   uint32_t explore, uint32_t pool_cap, float* pool_d,
+                   uint32_t* pool_id, uint32_t pool_max);
+/* build_approx_knn tail, index.h:1348-1366: sort pool by (dist,id), drop self,
+ * keep k.  Returns kept count. */
+uint32_t or_pool_topk(uint32_t self, const float* pool_d, const uint32_t* pool_id,
+                      uint32_t m, uint32_t k, uint32_t* ids, float* keys);
+/* prune_pool_to_ids default branch, index.h:772-829 with sort_unique_candidates
+ * (:225-248) and rng_extend (:267-281). Returns degree. */
+uint32_t or_prune(const float* x, const float* nrm, uint32_t d, uint32_t base,
+                  uint32_t n, float* pool_d, uint32_t* pool_id, uint32_t m,
+                  uint32_t M, float alpha, uint32_t* out);
+/* commit_edges, index.h:907-930.  O: n*M with counts ocnt; writes links/deg. */
+void or_commit(const float* x, const float* nrm, uint32_t n, uint32_t d,
+               const uint32_t* O, const uint32_t* ocnt, uint32_t M, float alpha,
+               uint32_t* links, uint32_t* deg);
+
+/* ---- small helpers shared by tests -------------------------------------- */
+void or_sort_pairs(float* d, uint32_t* id, uint32_t m);   /* (dist, id) ascending */
+
+#ifdef __cplusplus
+}
+#endif
+#endif
