@@ -1,0 +1,959 @@
+// abi.hip -- C ABI (include/anqi.h) over the HIP kernels: context, memory,
+// and the host orchestration of the AKNN source build, the lifted-graph
+// build, the verifier and the RkNN query.  Errors are returned as status
+// codes with a message in the context; nothing throws across the boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/anqi.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+struct anqi_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cus = 0;
+};
+
+static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int anqi_fail_hip(anqi_ctx* ctx, hipError_t e, const char* what, int line) {
+    return anqi_set_err(ctx, ANQI_E_DEVICE, "HIP error %d (%s) at abi.hip:%d: %s", (int)e,
+                        hipGetErrorString(e), line, what);
+}
+
+#define HCHK(expr) HIP_CHECK_RET(expr)
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// RAII device buffer
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        return hipMalloc(&p, b ? b : 16);
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+static uint32_t next_pow2(uint64_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+extern "C" {
+
+const char* anqi_build_info(void) {
+    return "libanqi_hip: hipcc --offload-arch=gfx950, FP32 distance contract, soft-binary64 verifier";
+}
+
+int anqi_ctx_create(int device, anqi_ctx** out) {
+    if (!out) return ANQI_E_USAGE;
+    *out = nullptr;
+    anqi_ctx* ctx = new anqi_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) { delete ctx; return ANQI_E_DEVICE; }
+    e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete ctx; return ANQI_E_DEVICE; }
+    ctx->stream = ctx->own;
+    hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
+    *out = ctx;
+    return ANQI_OK;
+}
+
+void anqi_ctx_destroy(anqi_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->own) { (void)hipStreamSynchronize(ctx->own); (void)hipStreamDestroy(ctx->own); }
+    delete ctx;
+}
+
+const char* anqi_last_error(const anqi_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int anqi_ctx_sync(anqi_ctx* ctx) {
+    HCHK(hipStreamSynchronize(ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_ctx_set_stream(anqi_ctx* ctx, void* stream) {
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+    return ANQI_OK;
+}
+
+void* anqi_ctx_stream(anqi_ctx* ctx) { return (void*)ctx->stream; }
+
+int anqi_dmalloc(anqi_ctx* ctx, size_t bytes, void** out) {
+    HCHK(hipMalloc(out, bytes ? bytes : 16));
+    return ANQI_OK;
+}
+int anqi_dfree(anqi_ctx* ctx, void* p) {
+    if (p) HCHK(hipFree(p));
+    return ANQI_OK;
+}
+int anqi_h2d(anqi_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    HCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    return ANQI_OK;
+}
+int anqi_d2h(anqi_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    HCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_norms(anqi_ctx* ctx, const float* x, uint64_t n, uint32_t d, float* out) {
+    if (d == 0 || d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
+    HCHK(launch_norms(x, n, d, out, ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_pair_d2(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t d, const uint32_t* a,
+                 const uint32_t* b, uint64_t m, float* out) {
+    HCHK(launch_pair_d2(x, nrm, d, a, b, m, out, ctx->stream));
+    return ANQI_OK;
+}
+
+// ---- exact -----------------------------------------------------------------------------
+int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t K, int precision,
+                   uint32_t* ids, float* keys) {
+    if (K == 0 || K >= n || K > 128) return anqi_set_err(ctx, ANQI_E_USAGE, "exact_knn: need 0 < K < n, K <= 128");
+    if (d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
+    DBuf nrm;
+    HCHK(nrm.alloc(sizeof(float) * (size_t)n));
+    HCHK(launch_norms(x, n, d, nrm.as<float>(), ctx->stream));
+    HCHK(launch_exact_topk(x, nrm.as<float>(), n, d, K, 0, n, precision, nullptr, 0.0f, ids, keys, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_rknn_membership(anqi_ctx* ctx, const float* base, uint32_t n, const float* q, uint32_t nq,
+                         uint32_t d, const float* rk2, uint32_t* off, uint64_t* total, uint32_t* ids,
+                         float* keys) {
+    DBuf nrm, cnt;
+    HCHK(nrm.alloc(sizeof(float) * (size_t)n));
+    HCHK(cnt.alloc(sizeof(uint32_t) * (size_t)nq));
+    HCHK(launch_norms(base, n, d, nrm.as<float>(), ctx->stream));
+    if (!ids) {
+        HCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * nq, ctx->stream));
+        HCHK(launch_rknn_count(base, nrm.as<float>(), n, q, nq, d, rk2, cnt.as<uint32_t>(), ctx->stream));
+        std::vector<uint32_t> c(nq);
+        HCHK(hipMemcpyAsync(c.data(), cnt.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ctx->stream));
+        HCHK(hipStreamSynchronize(ctx->stream));
+        uint64_t t = 0;
+        off[0] = 0;
+        for (uint32_t i = 0; i < nq; i++) { t += c[i]; off[i + 1] = (uint32_t)t; }
+        *total = t;
+        return ANQI_OK;
+    }
+    DBuf doff;
+    HCHK(doff.alloc(sizeof(uint32_t) * (size_t)(nq + 1)));
+    HCHK(hipMemcpyAsync(doff.p, off, sizeof(uint32_t) * (nq + 1), hipMemcpyHostToDevice, ctx->stream));
+    HCHK(launch_rknn_fill(base, nrm.as<float>(), n, q, nq, d, rk2, doff.as<uint32_t>(), ids, keys, ctx->stream));
+    // order each row by (key, id) on the host (rows are short)
+    uint64_t tot = off[nq];
+    std::vector<uint32_t> hid(tot);
+    std::vector<float> hk(tot);
+    HCHK(hipMemcpyAsync(hid.data(), ids, 4 * tot, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipMemcpyAsync(hk.data(), keys, 4 * tot, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<std::pair<float, uint32_t>> row;
+    for (uint32_t i = 0; i < nq; i++) {
+        row.clear();
+        for (uint32_t t = off[i]; t < off[i + 1]; t++) row.emplace_back(hk[t], hid[t]);
+        std::sort(row.begin(), row.end());
+        for (uint32_t t = off[i]; t < off[i + 1]; t++) { hk[t] = row[t - off[i]].first; hid[t] = row[t - off[i]].second; }
+    }
+    HCHK(hipMemcpyAsync(ids, hid.data(), 4 * tot, hipMemcpyHostToDevice, ctx->stream));
+    HCHK(hipMemcpyAsync(keys, hk.data(), 4 * tot, hipMemcpyHostToDevice, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    if (total) *total = tot;
+    return ANQI_OK;
+}
+
+// ---- verifier ------------------------------------------------------------------------------
+int anqi_lrq_encode(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
+                    const float* mu, const float* basis, const float* codebook, uint32_t code_size,
+                    float* coeff, uint8_t* codes, uint8_t* codes_kmajor) {
+    if (code_size != 256) return anqi_set_err(ctx, ANQI_E_INPUT, "only u8 codebooks (256 levels) are supported");
+    HCHK(launch_lrq_encode(cutoffs, n, nk, rankm, mu, basis, codebook, code_size, coeff, codes, codes_kmajor,
+                           ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_transpose_codes(anqi_ctx* ctx, const uint8_t* codes, uint64_t n, uint32_t nk, uint8_t* codes_kmajor) {
+    HCHK(launch_transpose_codes(codes, n, nk, codes_kmajor, ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
+                         uint32_t nk, uint32_t rankm, const float* mu, const float* basis,
+                         const float* codebook, uint32_t code_size, uint32_t k_one_based, double slack,
+                         float* thr) {
+    if (k_one_based < 1 || k_one_based > nk)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "LRQ verifier is not ready for k=%u", k_one_based);
+    uint64_t sb;
+    std::memcpy(&sb, &slack, 8);
+    HCHK(launch_lrq_materialize(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, k_one_based,
+                                sb, thr, ctx->stream));
+    return ANQI_OK;
+}
+
+// ---- lift (host FP64; src/rknn_query.cpp:2203-2236) ------------------------------------------
+int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
+                   double* mstar2, double* scale) {
+    // squared norms summed in row order (the reference's OpenMP reduction
+    // with one thread); GCC contracts the multiply-adds into fma.
+    double sum_n2 = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+        const float* o = base + i * d;
+        double s = 0.0;
+        for (uint32_t j = 0; j < d; j++) s = std::fma((double)o[j], (double)o[j], s);
+        sum_n2 += s;
+    }
+    double s_scale = 1.0 / std::sqrt(sum_n2 / (double)n + 1e-12), sc2 = s_scale * s_scale;
+    uint32_t dl = d + 1, dout = d + 2;
+    double maxn2 = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+        const float* o = base + i * d;
+        float* h = lifted + i * dout;
+        double nr2 = 0.0;
+        for (uint32_t j = 0; j < d; j++) {
+            double v = (double)o[j] * s_scale;
+            h[j] = (float)v;
+            nr2 = std::fma(v, v, nr2);
+        }
+        h[d] = (float)(0.5 * std::fma((double)rk_horizon[i], sc2, -nr2));
+        double hn2 = std::fma((double)h[d], (double)h[d], nr2);
+        if (hn2 > maxn2) maxn2 = hn2;
+    }
+    double M2 = maxn2 * 1.0001;
+    for (uint64_t i = 0; i < n; i++) {
+        float* h = lifted + i * dout;
+        double hn2 = 0.0;
+        for (uint32_t j = 0; j < dl; j++) hn2 = std::fma((double)h[j], (double)h[j], hn2);
+        double pad = M2 - hn2;
+        if (pad < 0) pad = 0;
+        h[dl] = (float)std::sqrt(pad);
+    }
+    *mstar2 = M2;
+    *scale = s_scale;
+    return ANQI_OK;
+}
+
+// ---- graph pieces -----------------------------------------------------------------------------
+int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* ids,
+                    uint32_t width, uint32_t cap, uint32_t* hidx, float* hval) {
+    HCHK(launch_warm_start(x, nrm, n, d, ids, width, cap, hidx, hval, ctx->stream));
+    return ANQI_OK;
+}
+
+int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t cap,
+                   uint32_t* hidx, float* hval, int iters, int sample, int64_t* upd, int* passes) {
+    if (sample > 120) sample = 120;
+    if (sample < 2 || iters <= 0) { if (passes) *passes = 0; return ANQI_OK; }
+    hipStream_t s = ctx->stream;
+    uint64_t maxp = (uint64_t)n * sample * (sample - 1);
+    DBuf samp, scnt, k0, k1, v0, v1, cnt, off, updd, tmp;
+    HCHK(samp.alloc(4ull * n * sample));
+    HCHK(scnt.alloc(4ull * n));
+    HCHK(k0.alloc(8 * maxp));
+    HCHK(k1.alloc(8 * maxp));
+    HCHK(v0.alloc(4 * maxp));
+    HCHK(v1.alloc(4 * maxp));
+    HCHK(cnt.alloc(8));
+    HCHK(updd.alloc(8));
+    HCHK(off.alloc(4ull * (n + 1)));
+    int end_bit = 32;
+    while (end_bit < 64 && ((uint64_t)n >> (end_bit - 32))) end_bit++;
+    size_t tb = 0;
+    HCHK(prim_sort_pairs_u64_u32(nullptr, nullptr, nullptr, nullptr, maxp, end_bit, nullptr, tb, s));
+    HCHK(tmp.alloc(tb));
+    int it = 0;
+    for (; it < iters; it++) {
+        HCHK(launch_nnd_sample(hidx, n, cap, (uint32_t)sample, samp.as<uint32_t>(), scnt.as<uint32_t>(), s));
+        HCHK(hipMemsetAsync(cnt.p, 0, 8, s));
+        HCHK(hipMemsetAsync(updd.p, 0, 8, s));
+        HCHK(launch_nnd_propose(x, nrm, n, d, samp.as<uint32_t>(), scnt.as<uint32_t>(), (uint32_t)sample, hval, cap,
+                                k0.as<uint64_t>(), v0.as<uint32_t>(), cnt.as<unsigned long long>(), maxp, s));
+        unsigned long long m = 0;
+        HCHK(hipMemcpyAsync(&m, cnt.p, 8, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        if (m > maxp) return anqi_set_err(ctx, ANQI_E_DEVICE, "LocalJoin proposal overflow");
+        if (m == 0) {   // nothing to apply: same stopping rule with zero updates
+            if (upd) upd[it] = 0;
+            if (0 < (int64_t)(n / 200)) { it++; break; }
+            continue;
+        }
+        size_t tbm = tb;
+        HCHK(prim_sort_pairs_u64_u32(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), m,
+                                     end_bit, tmp.p, tbm, s));
+        HCHK(launch_row_bounds(k1.as<uint64_t>(), m, n, off.as<uint32_t>(), s));
+        HCHK(launch_nnd_apply(k1.as<uint64_t>(), v1.as<uint32_t>(), m, off.as<uint32_t>(), n, hidx, hval, cap,
+                              updd.as<unsigned long long>(), s));
+        unsigned long long u = 0;
+        HCHK(hipMemcpyAsync(&u, updd.p, 8, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        if (upd) upd[it] = (int64_t)u;
+        if ((int64_t)u < (int64_t)(n / 200)) { it++; break; }
+    }
+    if (passes) *passes = it;
+    return ANQI_OK;
+}
+
+// reverse CSR; also reports the maximum list length
+static int reverse_csr(anqi_ctx* ctx, const uint32_t* ids, uint32_t rows, uint32_t width, const uint32_t* rowcnt,
+                       uint32_t n, uint32_t* roff, uint32_t* rids, uint32_t* maxlen) {
+    hipStream_t s = ctx->stream;
+    DBuf cnt, tmp;
+    HCHK(cnt.alloc(4ull * (n + 1)));
+    HCHK(hipMemsetAsync(cnt.p, 0, 4ull * (n + 1), s));
+    if (rowcnt) {
+        // counts = cursors after a scatter into a scratch array
+        DBuf scratch;
+        HCHK(scratch.alloc(4ull * rows * width));
+        HCHK(launch_scatter_rev(ids, rows, width, rowcnt, n, cnt.as<uint32_t>(), scratch.as<uint32_t>(), s));
+        HCHK(hipStreamSynchronize(s));
+    } else {
+        HCHK(launch_count_refs(ids, (uint64_t)rows * width, n, cnt.as<uint32_t>(), s));
+    }
+    size_t tb = 0;
+    HCHK(prim_exclusive_scan_u32(cnt.as<uint32_t>(), roff, (uint64_t)n + 1, nullptr, tb, s));
+    HCHK(tmp.alloc(tb));
+    HCHK(prim_exclusive_scan_u32(cnt.as<uint32_t>(), roff, (uint64_t)n + 1, tmp.p, tb, s));
+    if (maxlen) {
+        std::vector<uint32_t> c(n);
+        HCHK(hipMemcpyAsync(c.data(), cnt.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        *maxlen = n ? *std::max_element(c.begin(), c.end()) : 0;
+    }
+    // cursors start at roff
+    HCHK(hipMemcpyAsync(cnt.p, roff, 4ull * n, hipMemcpyDeviceToDevice, s));
+    HCHK(launch_scatter_rev(ids, rows, width, rowcnt, n, cnt.as<uint32_t>(), rids, s));
+    return ANQI_OK;
+}
+
+int anqi_reverse_csr(anqi_ctx* ctx, const uint32_t* ids, uint32_t rows, uint32_t width, const uint32_t* rowcnt,
+                     uint32_t n, uint32_t* roff, uint32_t* rids) {
+    return reverse_csr(ctx, ids, rows, width, rowcnt, n, roff, rids, nullptr);
+}
+
+// rows whose status is non-zero
+static int failed_rows(anqi_ctx* ctx, const uint32_t* status, uint32_t n, std::vector<uint32_t>& rows) {
+    std::vector<uint32_t> st(n);
+    HCHK(hipMemcpyAsync(st.data(), status, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    rows.clear();
+    for (uint32_t i = 0; i < n; i++)
+        if (st[i]) rows.push_back(i);
+    return ANQI_OK;
+}
+
+static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev, uint32_t* overflow_rows) {
+    hipStream_t s = ctx->stream;
+    DBuf status;
+    HCHK(status.alloc(4ull * P.n));
+    HCHK(hipMemsetAsync(status.p, 0, 4ull * P.n, s));
+    P.status = status.as<uint32_t>();
+    uint64_t need = (uint64_t)P.pool_cap + P.cap + maxrev;   // visits / pool entries bound
+    // fast path: LDS work area
+    P.hmask = std::min<uint32_t>(graph ? 4096 : 8192, next_pow2(need * 3 / 2 + 64)) - 1;
+    P.pmax = graph ? std::min<uint32_t>(2048, next_pow2(need + 64)) : 0;
+    P.rows = nullptr;
+    P.nrows = P.n;
+    HCHK(launch_gather(P, graph, false, s));
+    std::vector<uint32_t> bad;
+    int rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
+    if (rc) return rc;
+    if (overflow_rows) *overflow_rows = (uint32_t)bad.size();
+    if (bad.empty()) return ANQI_OK;
+    // slow path: global work area sized for the bound
+    GatherParams Q = P;
+    Q.hmask = next_pow2(need * 2 + 64) - 1;
+    Q.pmax = graph ? next_pow2(need + 64) : 0;
+    size_t words = (gather_lds_bytes(Q, graph) + 3) / 4;
+    DBuf rows, ws;
+    HCHK(rows.alloc(4 * bad.size()));
+    HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
+    HCHK(hipMemsetAsync(status.p, 0, 4ull * P.n, s));
+    const uint32_t chunk = 4096;
+    HCHK(ws.alloc(4 * words * std::min<size_t>(chunk, bad.size())));
+    for (size_t b = 0; b < bad.size(); b += chunk) {
+        Q.rows = rows.as<uint32_t>() + b;
+        Q.nrows = (uint32_t)std::min<size_t>(chunk, bad.size() - b);
+        Q.gscratch = ws.as<uint32_t>();
+        Q.gscratch_words = words;
+        HCHK(launch_gather(Q, graph, true, s));
+    }
+    rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
+    if (rc) return rc;
+    if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "gather: %zu rows exceed the work area", bad.size());
+    return ANQI_OK;
+}
+
+int anqi_gather_topk(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx,
+                     const float* hval, uint32_t cap, const uint32_t* roff, const uint32_t* rids, uint32_t explore,
+                     uint32_t pool_cap, uint32_t K, uint32_t* ids, float* keys, uint32_t* cnt) {
+    if (K > 256) return anqi_set_err(ctx, ANQI_E_USAGE, "K <= 256");
+    std::vector<uint32_t> ro(2);
+    uint32_t maxrev = 0;
+    {
+        std::vector<uint32_t> h(n + 1);
+        HCHK(hipMemcpyAsync(h.data(), roff, 4ull * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+        HCHK(hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < n; i++) maxrev = std::max(maxrev, h[i + 1] - h[i]);
+    }
+    GatherParams P{};
+    P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.hidx = hidx; P.hval = hval; P.cap = cap; P.roff = roff; P.rids = rids;
+    P.explore = explore; P.pool_cap = pool_cap; P.K = K; P.out_ids = ids; P.out_keys = keys; P.out_cnt = cnt;
+    return gather_run(ctx, P, false, maxrev, nullptr);
+}
+
+int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx,
+                      const float* hval, uint32_t cap, const uint32_t* roff, const uint32_t* rids, uint32_t explore,
+                      uint32_t pool_cap, uint32_t M, float alpha, uint32_t* O, uint32_t* ocnt) {
+    if (M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "graph degree <= 64");
+    uint32_t maxrev = 0;
+    {
+        std::vector<uint32_t> h(n + 1);
+        HCHK(hipMemcpyAsync(h.data(), roff, 4ull * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+        HCHK(hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < n; i++) maxrev = std::max(maxrev, h[i + 1] - h[i]);
+    }
+    GatherParams P{};
+    P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.hidx = hidx; P.hval = hval; P.cap = cap; P.roff = roff; P.rids = rids;
+    P.explore = explore; P.pool_cap = pool_cap; P.M = M; P.alpha = alpha; P.out_ids = O; P.out_cnt = ocnt;
+    return gather_run(ctx, P, true, maxrev, nullptr);
+}
+
+int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* O,
+                const uint32_t* ocnt, uint32_t M, float alpha, uint32_t* links, uint32_t* deg) {
+    if (M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "graph degree <= 64");
+    hipStream_t s = ctx->stream;
+    DBuf roff, rids, status;
+    HCHK(roff.alloc(4ull * (n + 1)));
+    HCHK(rids.alloc(4ull * n * M));
+    HCHK(status.alloc(4ull * n));
+    uint32_t maxin = 0;
+    int rc = reverse_csr(ctx, O, n, M, ocnt, n, roff.as<uint32_t>(), rids.as<uint32_t>(), &maxin);
+    if (rc) return rc;
+    HCHK(hipMemsetAsync(status.p, 0, 4ull * n, s));
+    PruneParams P{};
+    P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.O = O; P.ocnt = ocnt; P.roff = roff.as<uint32_t>();
+    P.rids = rids.as<uint32_t>(); P.M = M; P.alpha = alpha; P.links = links; P.deg = deg;
+    P.status = status.as<uint32_t>(); P.nrows = n;
+    P.pmax = std::min<uint32_t>(2048, next_pow2((uint64_t)M + maxin + 64));
+    HCHK(launch_commit(P, false, s));
+    std::vector<uint32_t> bad;
+    rc = failed_rows(ctx, status.as<uint32_t>(), n, bad);
+    if (rc) return rc;
+    if (bad.empty()) return ANQI_OK;
+    PruneParams Q = P;
+    Q.pmax = next_pow2((uint64_t)M + maxin + 64);
+    size_t words = (commit_lds_bytes(Q) + 3) / 4;
+    DBuf rows, ws;
+    HCHK(rows.alloc(4 * bad.size()));
+    HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
+    HCHK(hipMemsetAsync(status.p, 0, 4ull * n, s));
+    HCHK(ws.alloc(4 * words * bad.size()));
+    Q.rows = rows.as<uint32_t>();
+    Q.nrows = (uint32_t)bad.size();
+    Q.gscratch = ws.as<uint32_t>();
+    Q.gscratch_words = words;
+    HCHK(launch_commit(Q, true, s));
+    rc = failed_rows(ctx, status.as<uint32_t>(), n, bad);
+    if (rc) return rc;
+    if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "commit: %zu rows exceed the work area", bad.size());
+    return ANQI_OK;
+}
+
+}  // extern "C"
+
+// ---- RP forest ---------------------------------------------------------------------------------
+struct Forest {
+    DBuf perm;                               // T x n (positions t*n + local)
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> leaves;   // per tree (start, size)
+    // node structure for a single tree (want_nodes)
+    std::vector<uint32_t> left, right, entry_pos;
+    std::vector<float> offset;
+    DBuf hyper;                              // nodes x d
+    std::vector<uint64_t> leaf_off;
+    std::vector<uint32_t> leaf_pos_start, leaf_pos_size;
+    uint32_t nodes = 0;
+};
+
+static int build_forest(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t T, uint32_t leaf,
+                        uint32_t max_depth, uint64_t seed, bool want_nodes, Forest& F) {
+    hipStream_t s = ctx->stream;
+    uint64_t total = (uint64_t)T * n;
+    DBuf perm2, seg_of, side, flags, lscan, tmp;
+    HCHK(F.perm.alloc(4 * total));
+    HCHK(perm2.alloc(4 * total));
+    HCHK(seg_of.alloc(4 * total));
+    HCHK(side.alloc(total));
+    HCHK(flags.alloc(4 * total));
+    HCHK(lscan.alloc(4 * total));
+    {
+        std::vector<uint32_t> iota(n);
+        for (uint32_t i = 0; i < n; i++) iota[i] = i;
+        for (uint32_t t = 0; t < T; t++)
+            HCHK(hipMemcpyAsync(F.perm.as<uint32_t>() + (size_t)t * n, iota.data(), 4ull * n, hipMemcpyHostToDevice, s));
+        HCHK(hipStreamSynchronize(s));
+    }
+    size_t tb = 0;
+    HCHK(prim_exclusive_scan_u32(nullptr, nullptr, total, nullptr, tb, s));
+    HCHK(tmp.alloc(tb));
+    struct Seg { uint32_t start, size, tree, node; };
+    std::vector<Seg> cur;
+    F.leaves.assign(T, {});
+    for (uint32_t t = 0; t < T; t++) cur.push_back({(uint32_t)((uint64_t)t * n), n, t, 0});
+    // node bookkeeping (single tree)
+    if (want_nodes) { F.left.assign(1, U32_NONE); F.right.assign(1, U32_NONE); F.offset.assign(1, F32_BIG); }
+    std::vector<std::unique_ptr<DBuf>> level_hyper;
+    std::vector<std::vector<uint32_t>> level_nodes;
+    std::vector<std::pair<uint32_t, Seg>> leaf_nodes;
+    for (uint32_t depth = 0;; depth++) {
+        std::vector<Seg> active;
+        for (auto& g : cur) {
+            if (g.size <= leaf) {
+                F.leaves[g.tree].push_back({g.start, g.size});
+                if (want_nodes) leaf_nodes.push_back({g.node, g});
+            } else if (depth >= max_depth) {
+                Seg h = g;
+                h.size = std::min(leaf, g.size);   // rptrees.h:131-135
+                F.leaves[g.tree].push_back({h.start, h.size});
+                if (want_nodes) leaf_nodes.push_back({g.node, h});
+            } else {
+                active.push_back(g);
+            }
+        }
+        if (active.empty()) break;
+        uint32_t ns = (uint32_t)active.size();
+        std::vector<uint32_t> hs(ns), hz(ns), hp(2 * ns);
+        for (uint32_t k = 0; k < ns; k++) {
+            hs[k] = active[k].start;
+            hz[k] = active[k].size;
+            uint64_t key = ((uint64_t)active[k].tree << 40) ^ ((uint64_t)depth << 32) ^ active[k].start;
+            uint32_t r0 = (uint32_t)(rng_at(seed, key, 0) % active[k].size);
+            uint32_t r1 = (uint32_t)(rng_at(seed, key, 1) % active[k].size);
+            if (r0 == r1) r0 = (r1 + 1) % active[k].size;   // rptrees.h:33-35
+            hp[2 * k] = r0;
+            hp[2 * k + 1] = r1;
+        }
+        DBuf dstart, dsize, dpick, dleft, ddegen, doff;
+        auto hyp = std::make_unique<DBuf>();
+        HCHK(dstart.alloc(4ull * ns));
+        HCHK(dsize.alloc(4ull * ns));
+        HCHK(dpick.alloc(8ull * ns));
+        HCHK(dleft.alloc(4ull * ns));
+        HCHK(ddegen.alloc(ns));
+        HCHK(doff.alloc(4ull * ns));
+        HCHK(hyp->alloc(4ull * ns * d));
+        HCHK(hipMemcpyAsync(dstart.p, hs.data(), 4ull * ns, hipMemcpyHostToDevice, s));
+        HCHK(hipMemcpyAsync(dsize.p, hz.data(), 4ull * ns, hipMemcpyHostToDevice, s));
+        HCHK(hipMemcpyAsync(dpick.p, hp.data(), 8ull * ns, hipMemcpyHostToDevice, s));
+        HCHK(launch_fill_u32(seg_of.as<uint32_t>(), U32_NONE, total, s));
+        HCHK(launch_seg_fill(seg_of.as<uint32_t>(), dstart.as<uint32_t>(), dsize.as<uint32_t>(), ns, s));
+        ForestLevel L{};
+        L.x = x; L.n = n; L.d = d; L.perm = F.perm.as<uint32_t>(); L.perm_out = perm2.as<uint32_t>();
+        L.seg_of = seg_of.as<uint32_t>(); L.seg_start = dstart.as<uint32_t>(); L.seg_size = dsize.as<uint32_t>();
+        L.seg_pick = dpick.as<uint32_t>(); L.seg_hyper = hyp->as<float>(); L.seg_off = doff.as<float>();
+        L.side = side.as<uint8_t>(); L.nseg = ns; L.total = total;
+        HCHK(launch_split_planes(L, s));
+        HCHK(launch_split_sides(L, seed ^ ((uint64_t)depth * 0x100000001ull), s));
+        HCHK(hipMemsetAsync(dleft.p, 0, 4ull * ns, s));
+        HCHK(launch_seg_count_left(side.as<uint8_t>(), seg_of.as<uint32_t>(), total, dleft.as<uint32_t>(), s));
+        std::vector<uint32_t> hl(ns);
+        HCHK(hipMemcpyAsync(hl.data(), dleft.p, 4ull * ns, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        std::vector<uint8_t> dg(ns, 0);
+        bool anyd = false;
+        for (uint32_t k = 0; k < ns; k++)
+            if (hl[k] == 0 || hl[k] == hz[k]) { dg[k] = 1; anyd = true; }
+        if (anyd) {
+            HCHK(hipMemcpyAsync(ddegen.p, dg.data(), ns, hipMemcpyHostToDevice, s));
+            HCHK(launch_seg_redeal(side.as<uint8_t>(), seg_of.as<uint32_t>(), ddegen.as<uint8_t>(), total,
+                                   seed ^ 0xA5A5A5A5ull ^ depth, s));
+            HCHK(hipMemsetAsync(dleft.p, 0, 4ull * ns, s));
+            HCHK(launch_seg_count_left(side.as<uint8_t>(), seg_of.as<uint32_t>(), total, dleft.as<uint32_t>(), s));
+            HCHK(hipMemcpyAsync(hl.data(), dleft.p, 4ull * ns, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+        }
+        HCHK(launch_left_flags(side.as<uint8_t>(), seg_of.as<uint32_t>(), total, flags.as<uint32_t>(), s));
+        size_t tbb = tb;
+        HCHK(prim_exclusive_scan_u32(flags.as<uint32_t>(), lscan.as<uint32_t>(), total, tmp.p, tbb, s));
+        HCHK(launch_seg_partition(F.perm.as<uint32_t>(), perm2.as<uint32_t>(), side.as<uint8_t>(),
+                                  seg_of.as<uint32_t>(), dstart.as<uint32_t>(), dleft.as<uint32_t>(),
+                                  lscan.as<uint32_t>(), total, s));
+        std::swap(F.perm.p, perm2.p);
+        std::vector<float> hoff;
+        if (want_nodes) {
+            hoff.resize(ns);
+            HCHK(hipMemcpyAsync(hoff.data(), doff.p, 4ull * ns, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+        }
+        std::vector<Seg> next;
+        std::vector<uint32_t> lvl_nodes(ns);
+        for (uint32_t k = 0; k < ns; k++) {
+            Seg g = active[k];
+            uint32_t L0 = hl[k];
+            uint32_t ln = 0, rn = 0;
+            if (want_nodes) {
+                ln = (uint32_t)F.left.size();
+                rn = ln + 1;
+                F.left.push_back(U32_NONE); F.right.push_back(U32_NONE); F.offset.push_back(F32_BIG);
+                F.left.push_back(U32_NONE); F.right.push_back(U32_NONE); F.offset.push_back(F32_BIG);
+                F.left[g.node] = ln;
+                F.right[g.node] = rn;
+                F.offset[g.node] = hoff[k];
+                lvl_nodes[k] = g.node;
+            }
+            next.push_back({g.start, L0, g.tree, ln});
+            next.push_back({g.start + L0, g.size - L0, g.tree, rn});
+        }
+        if (want_nodes) {
+            level_nodes.push_back(lvl_nodes);
+            level_hyper.push_back(std::move(hyp));
+        }
+        cur.swap(next);
+    }
+    if (want_nodes) {
+        F.nodes = (uint32_t)F.left.size();
+        HCHK(F.hyper.alloc(4ull * F.nodes * d));
+        HCHK(hipMemsetAsync(F.hyper.p, 0, 4ull * F.nodes * d, s));
+        for (size_t lv = 0; lv < level_nodes.size(); lv++)
+            for (size_t k = 0; k < level_nodes[lv].size(); k++)
+                HCHK(hipMemcpyAsync(F.hyper.as<float>() + (size_t)level_nodes[lv][k] * d,
+                                    level_hyper[lv]->as<float>() + k * d, 4ull * d, hipMemcpyDeviceToDevice, s));
+        F.entry_pos.assign(F.nodes, 0);
+        F.leaf_pos_start.assign(F.nodes, 0);
+        F.leaf_pos_size.assign(F.nodes, 0);
+        for (auto& ln : leaf_nodes) {
+            F.leaf_pos_start[ln.first] = ln.second.start;
+            F.leaf_pos_size[ln.first] = ln.second.size;
+        }
+        HCHK(hipStreamSynchronize(s));
+    }
+    return ANQI_OK;
+}
+
+extern "C" {
+
+int anqi_tree_free(anqi_ctx* ctx, anqi_tree* t) {
+    (void)ctx;
+    if (!t) return ANQI_OK;
+    if (t->left) (void)hipFree(t->left);
+    if (t->right) (void)hipFree(t->right);
+    if (t->offset) (void)hipFree(t->offset);
+    if (t->hyper) (void)hipFree(t->hyper);
+    if (t->entry) (void)hipFree(t->entry);
+    delete[] t->leaf_off;
+    delete[] t->leaf_ids;
+    std::memset(t, 0, sizeof(*t));
+    return ANQI_OK;
+}
+
+int anqi_tree_upload(anqi_ctx* ctx, uint32_t nodes, uint32_t root, uint32_t d, const uint32_t* left,
+                     const uint32_t* right, const float* offset, const float* hyper, const uint32_t* entry,
+                     anqi_tree* out) {
+    hipStream_t s = ctx->stream;
+    std::memset(out, 0, sizeof(*out));
+    out->nodes = nodes; out->root = root; out->d = d;
+    HCHK(hipMalloc((void**)&out->left, 4ull * nodes));
+    HCHK(hipMalloc((void**)&out->right, 4ull * nodes));
+    HCHK(hipMalloc((void**)&out->offset, 4ull * nodes));
+    HCHK(hipMalloc((void**)&out->hyper, 4ull * nodes * d));
+    HCHK(hipMalloc((void**)&out->entry, 4ull * nodes));
+    HCHK(hipMemcpyAsync(out->left, left, 4ull * nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->right, right, 4ull * nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->offset, offset, 4ull * nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->hyper, hyper, 4ull * nodes * d, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->entry, entry, 4ull * nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipStreamSynchronize(s));
+    return ANQI_OK;
+}
+
+int anqi_rp_tree(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t leaf, uint32_t max_depth,
+                 uint64_t seed, anqi_tree* out) {
+    Forest F;
+    int rc = build_forest(ctx, x, n, d, 1, leaf, max_depth, seed, true, F);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    std::vector<uint32_t> perm(n);
+    HCHK(hipMemcpyAsync(perm.data(), F.perm.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    std::memset(out, 0, sizeof(*out));
+    out->nodes = F.nodes; out->root = 0; out->d = d;
+    std::vector<uint32_t> entry(F.nodes, 0);
+    out->leaf_off = new uint64_t[F.nodes + 1];
+    std::vector<uint32_t> ids;
+    out->leaf_off[0] = 0;
+    for (uint32_t k = 0; k < F.nodes; k++) {
+        if (F.left[k] == U32_NONE) {
+            uint32_t st = F.leaf_pos_start[k], sz = F.leaf_pos_size[k];
+            entry[k] = sz ? perm[st] : 0;
+            for (uint32_t t = 0; t < sz; t++) ids.push_back(perm[st + t]);
+        }
+        out->leaf_off[k + 1] = ids.size();
+    }
+    out->leaf_ids = new uint32_t[ids.size() ? ids.size() : 1];
+    std::memcpy(out->leaf_ids, ids.data(), 4 * ids.size());
+    HCHK(hipMalloc((void**)&out->left, 4ull * F.nodes));
+    HCHK(hipMalloc((void**)&out->right, 4ull * F.nodes));
+    HCHK(hipMalloc((void**)&out->offset, 4ull * F.nodes));
+    HCHK(hipMalloc((void**)&out->hyper, 4ull * F.nodes * d));
+    HCHK(hipMalloc((void**)&out->entry, 4ull * F.nodes));
+    HCHK(hipMemcpyAsync(out->left, F.left.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->right, F.right.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->offset, F.offset.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(out->hyper, F.hyper.p, 4ull * F.nodes * d, hipMemcpyDeviceToDevice, s));
+    HCHK(hipMemcpyAsync(out->entry, entry.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
+    HCHK(hipStreamSynchronize(s));
+    return ANQI_OK;
+}
+
+// ---- AKNN source -------------------------------------------------------------------------------
+int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const anqi_aknn_params* p,
+                    uint32_t* ids, float* keys, anqi_aknn_stats* st) {
+    if (!p || p->K == 0 || p->K >= n || p->K > 256) return anqi_set_err(ctx, ANQI_E_INPUT, "invalid base file or K");
+    if (d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
+    hipStream_t s = ctx->stream;
+    anqi_aknn_stats S{};
+    double t0 = now_s();
+    const uint32_t K = p->K;
+    DBuf nrm, lid, lkey, lcnt;
+    HCHK(nrm.alloc(4ull * n));
+    HCHK(launch_norms(x, n, d, nrm.as<float>(), s));
+    Forest F;
+    int rc = build_forest(ctx, x, n, d, p->trees ? p->trees : 16, std::max<uint32_t>(p->leaf ? p->leaf : K, 10),
+                          p->max_depth ? p->max_depth : 100, p->seed, false, F);
+    if (rc) return rc;
+    HCHK(hipStreamSynchronize(s));
+    double t1 = now_s();
+    S.forest_s = t1 - t0;
+    // leaf all-pairs (init_knn)
+    HCHK(lid.alloc(4ull * n * K));
+    HCHK(lkey.alloc(4ull * n * K));
+    HCHK(lcnt.alloc(4ull * n));
+    HCHK(hipMemsetAsync(lcnt.p, 0, 4ull * n, s));
+    HCHK(launch_fill_u32(lid.as<uint32_t>(), U32_NONE, (uint64_t)n * K, s));
+    uint32_t T = (uint32_t)F.leaves.size();
+    for (uint32_t t = 0; t < T; t++) {
+        auto& lv = F.leaves[t];
+        std::vector<uint32_t> hs(lv.size()), hz(lv.size());
+        uint32_t mx = 0;
+        for (size_t k = 0; k < lv.size(); k++) {
+            hs[k] = lv[k].first - t * n;
+            hz[k] = lv[k].second;
+            mx = std::max(mx, hz[k]);
+            S.leaf_pairs += (uint64_t)hz[k] * (hz[k] - (hz[k] ? 1 : 0)) / 2;
+        }
+        S.leaves += (uint32_t)lv.size();
+        S.max_leaf = std::max(S.max_leaf, mx);
+        DBuf ds, dz;
+        HCHK(ds.alloc(4 * hs.size()));
+        HCHK(dz.alloc(4 * hz.size()));
+        HCHK(hipMemcpyAsync(ds.p, hs.data(), 4 * hs.size(), hipMemcpyHostToDevice, s));
+        HCHK(hipMemcpyAsync(dz.p, hz.data(), 4 * hz.size(), hipMemcpyHostToDevice, s));
+        LeafParams LP{};
+        LP.x = x; LP.nrm = nrm.as<float>(); LP.n = n; LP.d = d; LP.perm = F.perm.as<uint32_t>() + (size_t)t * n;
+        LP.leaf_start = ds.as<uint32_t>(); LP.leaf_size = dz.as<uint32_t>(); LP.nleaves = (uint32_t)lv.size();
+        LP.K = K; LP.ids = lid.as<uint32_t>(); LP.keys = lkey.as<float>(); LP.cnt = lcnt.as<uint32_t>();
+        LP.max_leaf = mx;
+        HCHK(launch_leaf_knn(LP, p->precision != 0, s));
+        HCHK(hipStreamSynchronize(s));   // keeps ds/dz alive until the launch ends
+    }
+    HCHK(launch_random_fill(x, nrm.as<float>(), n, d, K, p->seed ^ 0x5EEDull, lid.as<uint32_t>(), lkey.as<float>(),
+                            lcnt.as<uint32_t>(), s));
+    HCHK(hipStreamSynchronize(s));
+    double t2 = now_s();
+    S.leaf_s = t2 - t1;
+    F.perm.alloc(16);
+    // gather (build_approx_knn tail)
+    DBuf roff, rids, ocnt;
+    HCHK(roff.alloc(4ull * (n + 1)));
+    HCHK(rids.alloc(4ull * n * K));
+    HCHK(ocnt.alloc(4ull * n));
+    rc = reverse_csr(ctx, lid.as<uint32_t>(), n, K, nullptr, n, roff.as<uint32_t>(), rids.as<uint32_t>(), nullptr);
+    if (rc) return rc;
+    uint32_t ef = p->ef ? p->ef : 100;
+    uint32_t pool_cap = std::max<uint32_t>(K, (p->ef_alpha ? p->ef_alpha : 8) * ef);
+    uint32_t maxrev = 0;
+    {
+        std::vector<uint32_t> h(n + 1);
+        HCHK(hipMemcpyAsync(h.data(), roff.p, 4ull * (n + 1), hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < n; i++) maxrev = std::max(maxrev, h[i + 1] - h[i]);
+    }
+    GatherParams G{};
+    G.x = x; G.nrm = nrm.as<float>(); G.n = n; G.d = d; G.hidx = lid.as<uint32_t>(); G.hval = lkey.as<float>();
+    G.cap = K; G.roff = roff.as<uint32_t>(); G.rids = rids.as<uint32_t>(); G.explore = ef; G.pool_cap = pool_cap;
+    G.K = K; G.out_ids = ids; G.out_keys = keys; G.out_cnt = ocnt.as<uint32_t>();
+    rc = gather_run(ctx, G, false, maxrev, &S.overflow_rows);
+    if (rc) return rc;
+    std::vector<uint32_t> oc(n);
+    HCHK(hipMemcpyAsync(oc.data(), ocnt.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < n; i++) S.short_rows += oc[i] < K;
+    double t3 = now_s();
+    S.gather_s = t3 - t2;
+    S.total_s = t3 - t0;
+    if (st) *st = S;
+    return ANQI_OK;
+}
+
+// ---- graph -------------------------------------------------------------------------------------
+int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl, const uint32_t* warm_ids,
+                     uint32_t width, const anqi_graph_params* p, uint32_t* links, uint32_t* deg, anqi_tree* tree,
+                     anqi_graph_stats* st) {
+    if (!p || p->M == 0 || p->M > 64 || p->nn_k == 0 || p->nn_k > 256)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "invalid graph parameters");
+    hipStream_t s = ctx->stream;
+    anqi_graph_stats S{};
+    double t0 = now_s();
+    DBuf nrm, hidx, hval, roff, rids, O, ocnt;
+    HCHK(nrm.alloc(4ull * n));
+    HCHK(launch_norms(lifted, n, dl, nrm.as<float>(), s));
+    int rc = anqi_rp_tree(ctx, lifted, n, dl, p->leaf ? p->leaf : 100, 100, p->seed, tree);
+    if (rc) return rc;
+    double t1 = now_s();
+    S.tree_s = t1 - t0;
+    const uint32_t cap = p->nn_k;
+    HCHK(hidx.alloc(4ull * n * cap));
+    HCHK(hval.alloc(4ull * n * cap));
+    HCHK(launch_warm_start(lifted, nrm.as<float>(), n, dl, warm_ids, width, cap, hidx.as<uint32_t>(),
+                           hval.as<float>(), s));
+    HCHK(hipStreamSynchronize(s));
+    double t2 = now_s();
+    S.warm_s = t2 - t1;
+    int passes = 0;
+    rc = anqi_nndescent(ctx, lifted, nrm.as<float>(), n, dl, cap, hidx.as<uint32_t>(), hval.as<float>(),
+                        std::min(p->iters, 16), p->sample, S.nnd_upd, &passes);
+    if (rc) return rc;
+    S.nnd_passes = passes;
+    HCHK(hipStreamSynchronize(s));
+    double t3 = now_s();
+    S.nnd_s = t3 - t2;
+    HCHK(roff.alloc(4ull * (n + 1)));
+    HCHK(rids.alloc(4ull * n * cap));
+    uint32_t maxrev = 0;
+    rc = reverse_csr(ctx, hidx.as<uint32_t>(), n, cap, nullptr, n, roff.as<uint32_t>(), rids.as<uint32_t>(), &maxrev);
+    if (rc) return rc;
+    HCHK(O.alloc(4ull * n * p->M));
+    HCHK(ocnt.alloc(4ull * n));
+    GatherParams G{};
+    G.x = lifted; G.nrm = nrm.as<float>(); G.n = n; G.d = dl; G.hidx = hidx.as<uint32_t>(); G.hval = hval.as<float>();
+    G.cap = cap; G.roff = roff.as<uint32_t>(); G.rids = rids.as<uint32_t>(); G.explore = p->explore ? p->explore : 100;
+    G.pool_cap = std::max<uint32_t>(p->M, (p->ef_alpha ? p->ef_alpha : 8) * G.explore);
+    G.M = p->M; G.alpha = p->alpha; G.out_ids = O.as<uint32_t>(); G.out_cnt = ocnt.as<uint32_t>();
+    rc = gather_run(ctx, G, true, maxrev, &S.overflow_rows);
+    if (rc) return rc;
+    HCHK(hipStreamSynchronize(s));
+    double t4 = now_s();
+    S.gather_s = t4 - t3;
+    rc = anqi_commit(ctx, lifted, nrm.as<float>(), n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), p->M, p->alpha,
+                     links, deg);
+    if (rc) return rc;
+    HCHK(hipStreamSynchronize(s));
+    double t5 = now_s();
+    S.commit_s = t5 - t4;
+    S.total_s = t5 - t0;
+    if (st) *st = S;
+    return ANQI_OK;
+}
+
+// ---- search ------------------------------------------------------------------------------------
+static float round_down_float(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t L,
+                     const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx, uint32_t out_cap,
+                     uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt,
+                     uint32_t* status, int mode) {
+    if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
+    if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
+    if (nq == 0) return ANQI_OK;
+    hipStream_t s = ctx->stream;
+    SearchParams P{};
+    P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
+    P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
+    P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
+    P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
+    double sc = ix->scale;
+    std::memcpy(&P.scale_bits, &sc, 8);
+    P.R2f = round_down_float(ix->mstar2 + 1.0);
+    P.queries = queries; P.nq = nq; P.L = L; P.Lq = Lq; P.Lmax = Lq ? std::max(Lmax, 1u) : std::max(L, 1u);
+    P.thr = thr; P.kidx = kidx;
+    P.hmask = 8191; P.rcap = 2048; P.out_cap = out_cap;
+    P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
+    P.status = status;
+    if (P.Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "L <= 1024");
+    DBuf counter;
+    if (mode == 1) {
+        HCHK(counter.alloc(4));
+        HCHK(hipMemsetAsync(counter.p, 0, 4, s));
+        P.work_counter = counter.as<uint32_t>();
+        int occ = search_persistent_occupancy(search_lds_bytes(P));
+        uint32_t grid = (uint32_t)std::max(1, occ) * (uint32_t)std::max(1, ctx->cus);
+        HCHK(launch_range_search(P, true, grid, s));
+    } else {
+        HCHK(launch_range_search(P, false, 0, s));
+    }
+    std::vector<uint32_t> hst(nq);
+    HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    std::vector<uint32_t> redo;
+    for (uint32_t i = 0; i < nq; i++)
+        if (hst[i] == 1 || hst[i] == 2) redo.push_back(i);
+    if (!redo.empty()) {
+        SearchParams Q = P;
+        Q.hmask = 16383;
+        Q.rcap = 16384;
+        DBuf ql;
+        HCHK(ql.alloc(4 * redo.size()));
+        HCHK(hipMemcpyAsync(ql.p, redo.data(), 4 * redo.size(), hipMemcpyHostToDevice, s));
+        Q.qlist = ql.as<uint32_t>();
+        Q.nq = (uint32_t)redo.size();
+        HCHK(launch_range_search(Q, false, 0, s));
+        HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        for (uint32_t i : redo)
+            if (hst[i] == 1 || hst[i] == 2)
+                return anqi_set_err(ctx, ANQI_E_DEVICE, "query %u exceeds the search work area", i);
+    }
+    return ANQI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,680 @@
+// k_graph.hip -- lifted-graph and AKNN construction kernels.
+//
+//   warm start      include/index.h:1204-1218  (per-row sequential heap pushes)
+//   LocalJoin       include/index.h:1468-1487  (snapshot schedule, see below)
+//   reverse lists   include/heap.h:190-207     (count / scan / scatter)
+//   gather          include/index.h:1491-1525  (one wavefront per row)
+//   AKNN top-K      include/index.h:1348-1366  (pool sorted by (dist, id), self dropped)
+//   prune           include/index.h:225-281, 772-829 (alpha-RNG to M)
+//   commit          include/index.h:907-930
+//
+// Heap rows use the reference HeapList rule (include/heap.h:83-186): reject if
+// value > root, reject duplicates, replace the root and sift down, so row
+// storage order (which LocalJoin samples from) matches the reference rule.
+//
+// LocalJoin schedule.  The reference applies pushes under per-row locks in
+// thread-arrival order, so its result depends on timing.  Here each pass
+// takes every row's sample (first `sample` non-sentinel slots in storage
+// order) at pass start, computes all pair distances, keeps proposals with
+// d <= root of the target row (a pass-start snapshot; roots only decrease,
+// so nothing that could be accepted is dropped), sorts them by
+// (row, dist, id) and applies each row's proposals in that order with the
+// heap rule.  The result is deterministic.
+#include "common.hpp"
+#include "kernels.hpp"
+
+// ---- heap rule (one thread) ---------------------------------------------------
+__device__ __forceinline__ int heap_push(uint32_t* idx, float* val, uint32_t cap, uint32_t node,
+                                         float value) {
+    if (value > val[0]) return 0;
+    for (uint32_t s = 0; s < cap; s++)
+        if (idx[s] == node) return 0;
+    uint32_t cur = 0, sw;
+    for (;;) {
+        uint32_t l = 2 * cur + 1, r = l + 1;
+        if (l >= cap) break;
+        else if (r >= cap) { if (val[l] > value) sw = l; else break; }
+        else if (val[l] >= val[r]) { if (val[l] > value) sw = l; else break; }
+        else { if (val[r] > value) sw = r; else break; }
+        idx[cur] = idx[sw];
+        val[cur] = val[sw];
+        cur = sw;
+    }
+    idx[cur] = node;
+    val[cur] = value;
+    return 1;
+}
+
+// d2(i, ids[i][t]) for every (row, slot); one 16-lane group per pair.
+__global__ void __launch_bounds__(256) k_rows_d2(const float* __restrict__ x, const float* __restrict__ nrm,
+                                                 uint32_t n, uint32_t d, const uint32_t* __restrict__ ids,
+                                                 uint32_t width, float* __restrict__ out) {
+    uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    int l = threadIdx.x & 15;
+    if (t >= (uint64_t)n * width) return;
+    uint32_t i = (uint32_t)(t / width), j = ids[t];
+    if (j >= n || j == i) { if (l == 0) out[t] = F32_BIG; return; }
+    float ip = g16_ip(x + (size_t)i * d, x + (size_t)j * d, (int)d, l);
+    if (l == 0) out[t] = d2_combine(ip, nrm[i], nrm[j]);
+}
+
+__global__ void k_warm_push(const uint32_t* __restrict__ ids, const float* __restrict__ dv, uint32_t n,
+                            uint32_t width, uint32_t cap, uint32_t* __restrict__ hidx,
+                            float* __restrict__ hval) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t* hi = hidx + (size_t)i * cap;
+    float* hv = hval + (size_t)i * cap;
+    for (uint32_t s = 0; s < cap; s++) { hi[s] = U32_NONE; hv[s] = F32_BIG; }
+    for (uint32_t t = 0; t < width; t++) {
+        uint32_t j = ids[(size_t)i * width + t];
+        if (j != i && j < n) heap_push(hi, hv, cap, j, dv[(size_t)i * width + t]);
+    }
+}
+
+hipError_t launch_warm_start(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                             const uint32_t* ids, uint32_t width, uint32_t cap,
+                             uint32_t* hidx, float* hval, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    float* dv = nullptr;
+    hipError_t e = hipMallocAsync((void**)&dv, sizeof(float) * (size_t)n * width, s);
+    if (e != hipSuccess) return e;
+    uint64_t pairs = (uint64_t)n * width;
+    hipLaunchKernelGGL(k_rows_d2, dim3((unsigned)((pairs * 16 + 255) / 256)), dim3(256), 0, s, x, nrm, n, d,
+                       ids, width, dv);
+    hipLaunchKernelGGL(k_warm_push, dim3((n + 127) / 128), dim3(128), 0, s, ids, dv, n, width, cap, hidx,
+                       hval);
+    e = hipGetLastError();
+    hipFreeAsync(dv, s);
+    return e;
+}
+
+// ---- LocalJoin ----------------------------------------------------------------------
+__global__ void k_nnd_sample(const uint32_t* __restrict__ hidx, uint32_t n, uint32_t cap,
+                             uint32_t sample, uint32_t* __restrict__ samp, uint32_t* __restrict__ scnt) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    uint32_t c = 0;
+    for (uint32_t s = 0; s < cap && c < sample; s++) {
+        uint32_t v = hidx[(size_t)u * cap + s];
+        if (v != U32_NONE) samp[(size_t)u * sample + c++] = v;
+    }
+    scnt[u] = c;
+}
+
+// One wavefront per row u: sample vectors staged in LDS, all pairs (a < b).
+__global__ void __launch_bounds__(64) k_nnd_propose(const float* __restrict__ x, const float* __restrict__ nrm,
+                                                    uint32_t n, uint32_t d, const uint32_t* __restrict__ samp,
+                                                    const uint32_t* __restrict__ scnt, uint32_t sample,
+                                                    const float* __restrict__ hval, uint32_t cap,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                    unsigned long long* __restrict__ count, uint64_t max_props) {
+    extern __shared__ __align__(16) char lds[];
+    float* sv = (float*)lds;                              // sample x d
+    uint32_t* sid = (uint32_t*)(sv + (size_t)sample * d);  // sample
+    float* snrm = (float*)(sid + sample);
+    float* sroot = snrm + sample;
+    uint32_t u = blockIdx.x;
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    uint32_t c = scnt[u];
+    if (c < 2) return;
+    for (uint32_t t = lane; t < c; t += 64) {
+        uint32_t v = samp[(size_t)u * sample + t];
+        sid[t] = v;
+        snrm[t] = nrm[v];
+        sroot[t] = hval[(size_t)v * cap];
+    }
+    __syncthreads();
+    for (uint32_t e = lane; e < c * d; e += 64) {
+        uint32_t t = e / d, k = e % d;
+        sv[e] = x[(size_t)sid[t] * d + k];
+    }
+    __syncthreads();
+    uint32_t npairs = c * (c - 1) / 2;
+    for (uint32_t base = 0; base < npairs; base += 4) {
+        uint32_t p = base + (uint32_t)g;
+        int ok_vw = 0, ok_wv = 0;
+        uint32_t a = 0, b = 0;
+        float dd = 0.0f;
+        if (p < npairs) {
+            // p -> (a, b), a < b, row-major over the upper triangle
+            uint32_t rem = p;
+            a = 0;
+            while (rem >= c - 1 - a) { rem -= c - 1 - a; a++; }
+            b = a + 1 + rem;
+            float ip = g16_ip(sv + (size_t)a * d, sv + (size_t)b * d, (int)d, l);
+            dd = d2_combine(ip, snrm[a], snrm[b]);
+            ok_vw = (l == 0) && !(dd > sroot[a]);   // push w into row v
+            ok_wv = (l == 0) && !(dd > sroot[b]);   // push v into row w
+        }
+        uint64_t m1 = ballot(ok_vw), m2 = ballot(ok_wv);
+        int tot = popc64(m1) + popc64(m2);
+        if (tot == 0) continue;
+        unsigned long long basep = 0;
+        if (lane == 0) basep = atomicAdd(count, (unsigned long long)tot);
+        basep = (unsigned long long)shfl64((uint64_t)basep, 0);
+        if (ok_vw) {
+            uint64_t pos = basep + lanes_below(m1);
+            if (pos < max_props) { keys[pos] = ((uint64_t)sid[a] << 32) | fkey(dd); vals[pos] = sid[b]; }
+        }
+        if (ok_wv) {
+            uint64_t pos = basep + popc64(m1) + lanes_below(m2);
+            if (pos < max_props) { keys[pos] = ((uint64_t)sid[b] << 32) | fkey(dd); vals[pos] = sid[a]; }
+        }
+    }
+}
+
+__global__ void k_row_bounds(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n,
+                             uint32_t* __restrict__ row_off) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    uint64_t target = (uint64_t)r << 32;
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    row_off[r] = (uint32_t)lo;
+}
+
+// Applies each row's sorted proposals; equal distances go in ascending id order.
+__global__ void k_nnd_apply(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                            const uint32_t* __restrict__ row_off, uint32_t n, uint32_t* __restrict__ hidx,
+                            float* __restrict__ hval, uint32_t cap, unsigned long long* __restrict__ upd) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t s = row_off[r], e = row_off[r + 1];
+    if (s == e) return;
+    uint32_t* hi = hidx + (size_t)r * cap;
+    float* hv = hval + (size_t)r * cap;
+    unsigned long long acc = 0;
+    uint32_t t = s;
+    while (t < e) {
+        uint32_t run = t + 1;
+        while (run < e && keys[run] == keys[t]) run++;
+        float dd = fkey_inv((uint32_t)keys[t]);
+        if (run == t + 1) {
+            acc += heap_push(hi, hv, cap, vals[t], dd);
+        } else {
+            uint32_t last = 0;
+            bool first = true;
+            for (;;) {
+                uint32_t best = U32_NONE;
+                for (uint32_t q = t; q < run; q++) {
+                    uint32_t v = vals[q];
+                    if ((first || v > last) && v < best) best = v;
+                }
+                if (best == U32_NONE) break;
+                acc += heap_push(hi, hv, cap, best, dd);
+                last = best;
+                first = false;
+            }
+        }
+        t = run;
+    }
+    if (acc) atomicAdd(upd, acc);
+}
+
+hipError_t launch_nnd_sample(const uint32_t* hidx, uint32_t n, uint32_t cap, uint32_t sample,
+                             uint32_t* samp, uint32_t* scnt, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nnd_sample, dim3((n + 255) / 256), dim3(256), 0, s, hidx, n, cap, sample, samp, scnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_nnd_propose(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                              const uint32_t* samp, const uint32_t* scnt, uint32_t sample,
+                              const float* hval, uint32_t cap, uint64_t* keys, uint32_t* vals,
+                              unsigned long long* count, uint64_t max_props, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t lds = sizeof(float) * (size_t)sample * d + sizeof(uint32_t) * sample + 2 * sizeof(float) * sample;
+    hipFuncSetAttribute((const void*)k_nnd_propose, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_nnd_propose, dim3(n), dim3(64), lds, s, x, nrm, n, d, samp, scnt, sample, hval, cap,
+                       keys, vals, count, max_props);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_bounds(const uint64_t* keys, uint64_t m, uint32_t n, uint32_t* row_off,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_row_bounds, dim3((n + 1 + 255) / 256), dim3(256), 0, s, keys, m, n, row_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_nnd_apply(const uint64_t* keys, const uint32_t* vals, uint64_t m,
+                            const uint32_t* row_off, uint32_t n, uint32_t* hidx, float* hval,
+                            uint32_t cap, unsigned long long* upd, hipStream_t s) {
+    (void)m;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nnd_apply, dim3((n + 127) / 128), dim3(128), 0, s, keys, vals, row_off, n, hidx, hval,
+                       cap, upd);
+    return hipGetLastError();
+}
+
+// ---- reverse lists ---------------------------------------------------------------
+__global__ void k_count_refs(const uint32_t* __restrict__ ids, uint64_t m, uint32_t n,
+                             uint32_t* __restrict__ cnt) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    uint32_t j = ids[t];
+    if (j < n) atomicAdd(&cnt[j], 1u);
+}
+
+// rows x width id table whose row r has rowcnt[r] valid leading entries
+// (rowcnt == nullptr: sentinel-filtered full rows).
+__global__ void k_scatter_rev(const uint32_t* __restrict__ ids, uint32_t rows, uint32_t width,
+                              const uint32_t* __restrict__ rowcnt, uint32_t n,
+                              uint32_t* __restrict__ cursor, uint32_t* __restrict__ rids) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)rows * width) return;
+    uint32_t r = (uint32_t)(t / width), s = (uint32_t)(t % width);
+    if (rowcnt && s >= rowcnt[r]) return;
+    uint32_t j = ids[t];
+    if (j >= n) return;
+    uint32_t p = atomicAdd(&cursor[j], 1u);
+    rids[p] = r;
+}
+
+hipError_t launch_count_refs(const uint32_t* ids, uint64_t m, uint32_t n, uint32_t* cnt, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_refs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, ids, m, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_rev(const uint32_t* ids, uint32_t rows, uint32_t width,
+                              const uint32_t* rowcnt, uint32_t n, uint32_t* cursor,
+                              uint32_t* rids, hipStream_t s) {
+    uint64_t m = (uint64_t)rows * width;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_rev, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, ids, rows, width,
+                       rowcnt, n, cursor, rids);
+    return hipGetLastError();
+}
+
+// ---- sorted-list merge (one wavefront) ----------------------------------------------
+// cs[0..kc) sorted candidate keys (LDS); (ad, ap) current list of size `size`
+// (bit31 of ap = expanded).  Writes the first `cap` of the merge to (od, op).
+__device__ __forceinline__ uint32_t merge_into(const uint64_t* cs, int kc, const float* ad,
+                                               const uint32_t* ap, uint32_t size, uint32_t cap,
+                                               float* od, uint32_t* op) {
+    const int lane = __lane_id();
+    for (uint32_t i = lane; i < size; i += 64) {
+        uint64_t bk = pkey(ad[i], ap[i] & 0x7fffffffu);
+        int lo = 0, hi = kc;
+        while (lo < hi) { int mid = (lo + hi) >> 1; if (cs[mid] < bk) lo = mid + 1; else hi = mid; }
+        uint32_t pos = i + (uint32_t)lo;
+        if (pos < cap) { od[pos] = ad[i]; op[pos] = ap[i]; }
+    }
+    if (lane < kc) {
+        uint64_t key = cs[lane];
+        uint32_t lo = 0, hi = size;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (pkey(ad[mid], ap[mid] & 0x7fffffffu) < key) lo = mid + 1; else hi = mid;
+        }
+        uint32_t pos = (uint32_t)lane + lo;
+        if (pos < cap) { od[pos] = pkey_d(key); op[pos] = pkey_id(key); }
+    }
+    return min(cap, size + (uint32_t)kc);
+}
+
+// ---- gather (+ top-K or + prune) -------------------------------------------------------
+struct GatherLds {
+    float* qd[2];
+    uint32_t* qp[2];
+    uint32_t* hash;
+    float* pd;        // pool (graph) or top-K (2 buffers in AKNN mode)
+    uint32_t* pi;
+    float* pd2;
+    uint32_t* pi2;
+    uint32_t* tid;
+    float* tdv;
+    uint64_t* cs;
+    float* selv;      // prune: selected vectors (aliases the hash region)
+    uint32_t* sel;
+};
+
+__host__ __device__ __forceinline__ size_t rnd16(size_t b) { return (b + 15) & ~size_t(15); }
+
+__host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P, bool graph) {
+    // hash + queues; in graph mode also the selected vectors of the prune step
+    size_t a = rnd16(4 * (size_t)(P.hmask + 1)) + 4 * rnd16(4 * (size_t)P.explore);
+    size_t b = graph ? rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) : 0;
+    return a > b ? a : b;
+}
+
+size_t gather_lds_bytes(const GatherParams& P, bool graph) {
+    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : 4 * rnd16(4 * (size_t)P.K);
+    return gather_region0(P, graph) + pool + rnd16(4 * 64) + rnd16(4 * 64) + rnd16(8 * 64);
+}
+
+__device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams& P, bool graph) {
+    GatherLds s;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { char* p = base + o; o += rnd16(bytes); return p; };
+    s.hash = (uint32_t*)take(4 * (size_t)(P.hmask + 1));
+    s.qd[0] = (float*)take(4 * (size_t)P.explore);
+    s.qd[1] = (float*)take(4 * (size_t)P.explore);
+    s.qp[0] = (uint32_t*)take(4 * (size_t)P.explore);
+    s.qp[1] = (uint32_t*)take(4 * (size_t)P.explore);
+    s.selv = (float*)base;
+    s.sel = (uint32_t*)(base + rnd16(4 * (size_t)P.M * P.d));
+    o = gather_region0(P, graph);
+    if (graph) {
+        s.pd = (float*)take(4 * (size_t)P.pmax);
+        s.pi = (uint32_t*)take(4 * (size_t)P.pmax);
+        s.pd2 = nullptr;
+        s.pi2 = nullptr;
+    } else {
+        s.pd = (float*)take(4 * (size_t)P.K);
+        s.pi = (uint32_t*)take(4 * (size_t)P.K);
+        s.pd2 = (float*)take(4 * (size_t)P.K);
+        s.pi2 = (uint32_t*)take(4 * (size_t)P.K);
+    }
+    s.tid = (uint32_t*)take(4 * 64);
+    s.tdv = (float*)take(4 * 64);
+    s.cs = (uint64_t*)take(8 * 64);
+    return s;
+}
+
+// Bitonic sort of (pd, pi) [0, m) by (dist, id) in place; m <= cap2 (power of 2).
+__device__ void lds_sort_pairs(float* pd, uint32_t* pi, uint32_t m, uint32_t cap2) {
+    const int lane = __lane_id();
+    for (uint32_t t = m + lane; t < cap2; t += 64) { pd[t] = F32_BIG; pi[t] = U32_NONE; }
+    __syncthreads();
+    for (uint32_t k = 2; k <= cap2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = lane; t < cap2; t += 64) {
+                uint32_t p = t ^ j;
+                if (p > t) {
+                    uint64_t a = pkey(pd[t], pi[t]), b = pkey(pd[p], pi[p]);
+                    bool up = (t & k) == 0;
+                    if ((a > b) == up) {
+                        float fd = pd[t]; pd[t] = pd[p]; pd[p] = fd;
+                        uint32_t fi = pi[t]; pi[t] = pi[p]; pi[p] = fi;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// alpha-RNG selection over a (dist, id)-sorted, duplicate-free pool
+// (include/index.h:267-281); selected vectors are staged in LDS.
+__device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uint32_t* pi, uint32_t m,
+                               float* selv, uint32_t* sel, uint32_t* out) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const int d = (int)P.d;
+    uint32_t deg = 0;
+    __shared__ float selnrm[64];
+    for (uint32_t t = 0; t < m && deg < P.M; t++) {
+        uint32_t cid = pi[t];
+        float d2c = pd[t];
+        QRow QC;
+        qrow_load(QC, P.x + (size_t)cid * d, d, l);
+        float nc = P.nrm[cid];
+        int occ = 0;
+        for (uint32_t s0 = 0; s0 < deg; s0 += 4) {
+            uint32_t s = s0 + (uint32_t)g;
+            int o = 0;
+            if (s < deg) {
+                float ip = qrow_ip(QC, selv + (size_t)s * d, d, l);
+                float dsc = d2_combine(ip, selnrm[s], nc);
+                o = (P.alpha * dsc < d2c);
+            }
+            if (__any(o)) { occ = 1; break; }
+        }
+        if (!occ) {
+            // stage the vector: group 0 writes its chunks
+            if (g == 0) {
+                int mm = d >> 4, r = d & 15;
+#pragma unroll
+                for (int c = 0; c < QMAX_CHUNKS; c++)
+                    if (c < mm) selv[(size_t)deg * d + 16 * c + l] = QC.v[c];
+                if (l < r) selv[(size_t)deg * d + 16 * mm + l] = QC.tail;
+            }
+            if (lane == 0) { sel[deg] = cid; selnrm[deg] = nc; out[deg] = cid; }
+            deg++;
+            __syncthreads();
+        }
+    }
+    return deg;
+}
+
+template <bool GRAPH, bool GWS>
+__global__ void __launch_bounds__(64) k_gather(GatherParams P) {
+    extern __shared__ __align__(16) char lds_raw[];
+    uint32_t bi = blockIdx.x;
+    if (bi >= P.nrows) return;
+    uint32_t i = P.rows ? P.rows[bi] : bi;
+    char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
+    GatherLds S = gather_carve(base, P, GRAPH);
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const int d = (int)P.d;
+    for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    __syncthreads();
+    QRow QI;
+    qrow_load(QI, P.x + (size_t)i * d, d, l);
+    const float ni = P.nrm[i];
+    uint32_t qsize = 0, psize = 0, pcount = 0;
+    int qb = 0, pb = 0, status = 0;
+
+    // Offers a chunk of candidate ids (lane-held, in reference order) to the
+    // visited set, pool and queue.  seed_d: stored distances for the seeds.
+    auto offer = [&](uint32_t id, int valid, bool seeded, float seed_d) -> int {
+        int fresh = 0;
+        if (valid) {
+            int r = hset_insert(S.hash, P.hmask, id);
+            if (r < 0) status = 1;
+            fresh = (r == 1);
+        }
+        if (__any(status)) { status = 1; return 0; }
+        uint64_t fm = ballot(fresh);
+        int k = popc64(fm);
+        if (k == 0) return 1;
+        if (fresh) { S.tid[lanes_below(fm)] = id; S.tdv[lanes_below(fm)] = seed_d; }
+        __syncthreads();
+        if (!seeded) {
+            for (int s = g; s < k; s += 4) {
+                uint32_t c = S.tid[s];
+                float ip = qrow_ip(QI, P.x + (size_t)c * d, d, l);
+                if (l == 0) S.tdv[s] = d2_combine(ip, ni, P.nrm[c]);
+            }
+            __syncthreads();
+        }
+        float dv = lane < k ? S.tdv[lane] : F32_BIG;
+        uint32_t cid = lane < k ? S.tid[lane] : U32_NONE;
+        // pool
+        if (GRAPH) {
+            if (psize + (uint32_t)k > P.pmax) { status = 2; return 0; }
+            if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
+            psize += (uint32_t)k;
+        } else {
+            uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
+            float* ad = pb ? S.pd2 : S.pd;
+            uint32_t* ap = pb ? S.pi2 : S.pi;
+            if (psize == P.K && key != ~0ull && !(key < pkey(ad[P.K - 1], ap[P.K - 1]))) key = ~0ull;
+            key = wave_sort64(key);
+            int kc = popc64(ballot(key != ~0ull));
+            if (kc) {
+                S.cs[lane] = key;
+                __syncthreads();
+                psize = merge_into(S.cs, kc, ad, ap, psize, P.K, pb ? S.pd : S.pd2, pb ? S.pi : S.pi2);
+                pb ^= 1;
+                __syncthreads();
+            }
+        }
+        pcount += (uint32_t)k;
+        // queue
+        {
+            uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
+            float* ad = S.qd[qb];
+            uint32_t* ap = S.qp[qb];
+            if (qsize == P.explore && key != ~0ull &&
+                !(key < pkey(ad[P.explore - 1], ap[P.explore - 1] & 0x7fffffffu)))
+                key = ~0ull;
+            key = wave_sort64(key);
+            int kc = popc64(ballot(key != ~0ull));
+            if (kc) {
+                S.cs[lane] = key;
+                __syncthreads();
+                qsize = merge_into(S.cs, kc, ad, ap, qsize, P.explore, S.qd[qb ^ 1], S.qp[qb ^ 1]);
+                qb ^= 1;
+            }
+            __syncthreads();
+        }
+        return 1;
+    };
+
+    // seeds: the row's own slots in storage order, with stored values
+    for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
+        uint32_t s = s0 + lane;
+        uint32_t id = s < P.cap ? P.hidx[(size_t)i * P.cap + s] : U32_NONE;
+        float dv = s < P.cap ? P.hval[(size_t)i * P.cap + s] : F32_BIG;
+        offer(id, id != U32_NONE && id < P.n, true, dv);
+    }
+    // beam over the neighbour heaps and their reverse lists
+    while (!status) {
+        uint32_t cur = U32_NONE;
+        for (uint32_t b = 0; b < qsize; b += 64) {
+            uint32_t t = b + lane;
+            uint64_t m = ballot(t < qsize && !(S.qp[qb][t] >> 31));
+            if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
+        }
+        if (cur == U32_NONE) break;
+        uint32_t c = S.qp[qb][cur];
+        __syncthreads();
+        if (lane == 0) S.qp[qb][cur] = c | 0x80000000u;
+        __syncthreads();
+        for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
+            uint32_t s = s0 + lane;
+            uint32_t id = s < P.cap ? P.hidx[(size_t)c * P.cap + s] : U32_NONE;
+            offer(id, id != U32_NONE && id < P.n, false, 0.0f);
+        }
+        uint32_t r0 = P.roff[c], r1 = P.roff[c + 1];
+        for (uint32_t s0 = r0; s0 < r1 && !status; s0 += 64) {
+            uint32_t s = s0 + lane;
+            uint32_t id = s < r1 ? P.rids[s] : U32_NONE;
+            offer(id, id != U32_NONE && id < P.n, false, 0.0f);
+        }
+        if (pcount >= P.pool_cap) break;
+    }
+    if (status) {
+        if (lane == 0) { P.status[i] = (uint32_t)status; P.out_cnt[i] = 0; }
+        return;
+    }
+    if (!GRAPH) {
+        const float* ad = pb ? S.pd2 : S.pd;
+        const uint32_t* ap = pb ? S.pi2 : S.pi;
+        for (uint32_t t = lane; t < P.K; t += 64) {
+            P.out_ids[(size_t)i * P.K + t] = t < psize ? ap[t] : U32_NONE;
+            P.out_keys[(size_t)i * P.K + t] = t < psize ? ad[t] : F32_BIG;
+        }
+        if (lane == 0) { P.out_cnt[i] = psize; P.status[i] = 0; }
+        return;
+    }
+    // graph mode: drop self, sort by (dist, id), alpha-RNG to M
+    uint32_t w = 0;
+    for (uint32_t t0 = 0; t0 < psize; t0 += 64) {
+        uint32_t t = t0 + lane;
+        int keep = t < psize && S.pi[t] != i;
+        float dv = keep ? S.pd[t] : 0.0f;
+        uint32_t id = keep ? S.pi[t] : 0u;
+        uint64_t km = ballot(keep);
+        __syncthreads();
+        if (keep) { S.pd[w + lanes_below(km)] = dv; S.pi[w + lanes_below(km)] = id; }
+        w += (uint32_t)popc64(km);
+        __syncthreads();
+    }
+    uint32_t cap2 = 64;
+    while (cap2 < w) cap2 <<= 1;
+    if (cap2 > P.pmax) { if (lane == 0) { P.status[i] = 2; P.out_cnt[i] = 0; } return; }
+    lds_sort_pairs(S.pd, S.pi, w, cap2);
+    uint32_t deg = rng_select(P, S.pd, S.pi, w, S.selv, S.sel, P.out_ids + (size_t)i * P.M);
+    if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
+}
+
+hipError_t launch_gather(const GatherParams& P, bool graph, bool gws, hipStream_t s) {
+    if (P.nrows == 0) return hipSuccess;
+    size_t lds = gws ? 0 : gather_lds_bytes(P, graph);
+    const void* f;
+    if (graph) f = gws ? (const void*)k_gather<true, true> : (const void*)k_gather<true, false>;
+    else f = gws ? (const void*)k_gather<false, true> : (const void*)k_gather<false, false>;
+    if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (graph) {
+        if (gws) hipLaunchKernelGGL((k_gather<true, true>), dim3(P.nrows), dim3(64), lds, s, P);
+        else hipLaunchKernelGGL((k_gather<true, false>), dim3(P.nrows), dim3(64), lds, s, P);
+    } else {
+        if (gws) hipLaunchKernelGGL((k_gather<false, true>), dim3(P.nrows), dim3(64), lds, s, P);
+        else hipLaunchKernelGGL((k_gather<false, false>), dim3(P.nrows), dim3(64), lds, s, P);
+    }
+    return hipGetLastError();
+}
+
+// ---- commit: cand = O_i u R_i, prune to M (include/index.h:907-930) -----------------
+size_t commit_lds_bytes(const PruneParams& P) {
+    return 2 * rnd16(4 * (size_t)P.pmax) + rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) +
+           rnd16(4 * 64);
+}
+
+template <bool GWS>
+__global__ void __launch_bounds__(64) k_commit(PruneParams P) {
+    extern __shared__ __align__(16) char lds_raw[];
+    uint32_t bi = blockIdx.x;
+    if (bi >= P.nrows) return;
+    uint32_t i = P.rows ? P.rows[bi] : bi;
+    char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
+    float* pd = (float*)base;
+    uint32_t* pi = (uint32_t*)(base + rnd16(4 * (size_t)P.pmax));
+    float* selv = (float*)(base + 2 * rnd16(4 * (size_t)P.pmax));
+    uint32_t* sel = (uint32_t*)((char*)selv + rnd16(4 * (size_t)P.M * P.d));
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const int d = (int)P.d;
+    uint32_t no = P.ocnt[i], r0 = P.roff[i], r1 = P.roff[i + 1];
+    uint32_t m = no + (r1 - r0);
+    uint32_t cap2 = 64;
+    while (cap2 < m) cap2 <<= 1;
+    if (cap2 > P.pmax) { if (lane == 0) { P.status[i] = 2; P.deg[i] = 0; } return; }
+    for (uint32_t t = lane; t < m; t += 64)
+        pi[t] = t < no ? P.O[(size_t)i * P.M + t] : P.rids[r0 + (t - no)];
+    __syncthreads();
+    QRow QI;
+    qrow_load(QI, P.x + (size_t)i * d, d, l);
+    float ni = P.nrm[i];
+    for (uint32_t s = g; s < m; s += 4) {
+        uint32_t j = pi[s];
+        float ip = qrow_ip(QI, P.x + (size_t)j * d, d, l);
+        if (l == 0) pd[s] = d2_combine(ip, ni, P.nrm[j]);
+    }
+    __syncthreads();
+    lds_sort_pairs(pd, pi, m, cap2);
+    // drop self and duplicates (same id -> same distance -> adjacent)
+    uint32_t w = 0;
+    for (uint32_t t0 = 0; t0 < m; t0 += 64) {
+        uint32_t t = t0 + lane;
+        int keep = t < m && pi[t] != i && pi[t] < P.n && !(t > 0 && pi[t - 1] == pi[t]);
+        float dv = keep ? pd[t] : 0.0f;
+        uint32_t id = keep ? pi[t] : 0u;
+        uint64_t km = ballot(keep);
+        __syncthreads();
+        if (keep) { pd[w + lanes_below(km)] = dv; pi[w + lanes_below(km)] = id; }
+        w += (uint32_t)popc64(km);
+        __syncthreads();
+    }
+    GatherParams G{};
+    G.x = P.x; G.nrm = P.nrm; G.n = P.n; G.d = P.d; G.M = P.M; G.alpha = P.alpha;
+    uint32_t dg = rng_select(G, pd, pi, w, selv, sel, P.links + (size_t)i * P.M);
+    if (lane == 0) { P.deg[i] = dg; P.status[i] = 0; }
+}
+
+hipError_t launch_commit(const PruneParams& P, bool gws, hipStream_t s) {
+    if (P.nrows == 0) return hipSuccess;
+    size_t lds = gws ? 0 : commit_lds_bytes(P);
+    if (gws) {
+        hipLaunchKernelGGL((k_commit<true>), dim3(P.nrows), dim3(64), 0, s, P);
+    } else {
+        hipFuncSetAttribute((const void*)k_commit<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((k_commit<false>), dim3(P.nrows), dim3(64), lds, s, P);
+    }
+    return hipGetLastError();
+}

@@ -1,0 +1,284 @@
+// k_search.hip -- RkNN range search over the lifted graph plus the fused
+// verifier postfilter.  One wavefront per query.
+//
+// Semantics follow nndindex::range_search (include/index.h:1535-1652) in the
+// pinned profile (CENTER score, radj == nullptr, eps = 0): a beam of width L
+// keyed on lifted L2^2 from the search-tree entry, then an unbounded flood over
+// every node with D <= R2; collected candidates are the nodes with D <= R2 in
+// discovery order.  The comparison double(float D) <= R2 is done as
+// D <= R2f where R2f is the largest float <= R2 (computed on the host).
+// The postfilter is src/rknn_query.cpp:2762-2766: key = on + qn - 2 ip in base
+// space (each term by the FP32 contract), accept iff key <= t_k[o].
+//
+// Batch semantics: merging one expansion's (up to 64) neighbours into the
+// sorted beam at once equals inserting them one by one (total (dist, id)
+// order, ids unique thanks to the visited set).  The flood is a closure, so
+// its visiting order does not change the collected set; it is processed in
+// FIFO order anyway, so candidate lists match the reference's order.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+struct SearchLds {
+    float* qlift;      // dl
+    float* qbase;      // d
+    float* bd[2];      // beam distances
+    uint32_t* bp[2];   // beam ids, bit31 = expanded
+    uint32_t* hash;    // visited set
+    uint32_t* res;     // collected ids (FIFO)
+    uint32_t* tid;     // 64 new ids
+    float* tdv;        // 64 distances
+    uint64_t* cs;      // 64 sorted candidate keys
+};
+
+__device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
+    SearchLds s;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { char* p = base + o; o += (bytes + 15) & ~size_t(15); return p; };
+    s.qlift = (float*)take(sizeof(float) * (P.dl + 16));
+    s.qbase = (float*)take(sizeof(float) * (P.d + 16));
+    s.bd[0] = (float*)take(sizeof(float) * P.Lmax);
+    s.bd[1] = (float*)take(sizeof(float) * P.Lmax);
+    s.bp[0] = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+    s.bp[1] = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+    s.hash = (uint32_t*)take(sizeof(uint32_t) * (P.hmask + 1));
+    s.res = (uint32_t*)take(sizeof(uint32_t) * P.rcap);
+    s.tid = (uint32_t*)take(sizeof(uint32_t) * 64);
+    s.tdv = (float*)take(sizeof(float) * 64);
+    s.cs = (uint64_t*)take(sizeof(uint64_t) * 64);
+    return s;
+}
+
+// Expands node c: visits its unvisited neighbours, computes D, appends
+// collected ones to the FIFO.  In beam mode also merges them into the beam.
+// Returns false on a capacity overflow.
+template <bool BEAM>
+__device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, const QRow& QL,
+                                       uint32_t c, uint32_t& rcnt, uint32_t& nd,
+                                       int& buf, uint32_t& bsize, uint32_t L, int& ovf) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    uint32_t dg = P.deg[c];
+    uint32_t nb = U32_NONE;
+    int fresh = 0;
+    if ((uint32_t)lane < dg) {
+        nb = P.links[(size_t)c * P.M + lane];
+        int r = hset_insert(S.hash, P.hmask, nb);
+        if (r < 0) ovf = 1;
+        fresh = (r == 1);
+    }
+    if (__any(ovf)) { ovf = 1; return false; }
+    uint64_t fm = ballot(fresh);
+    int k = popc64(fm);
+    if (fresh) S.tid[lanes_below(fm)] = nb;
+    __syncthreads();
+    // distances, four pairs per step (one per 16-lane group)
+    for (int s = g; s < k; s += 4) {
+        uint32_t id = S.tid[s];
+        float dv = qrow_l2(QL, P.lifted + (size_t)id * P.dl, P.dl, l);
+        if (l == 0) S.tdv[s] = dv;
+    }
+    __syncthreads();
+    nd += (uint32_t)k;
+    float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
+    uint32_t myid = (lane < k) ? S.tid[lane] : U32_NONE;
+    int hit = (lane < k) && (dv <= P.R2f);
+    uint64_t hm = ballot(hit);
+    int nh = popc64(hm);
+    if (rcnt + (uint32_t)nh > P.rcap) { ovf = 2; return false; }
+    if (hit) S.res[rcnt + lanes_below(hm)] = myid;
+    rcnt += (uint32_t)nh;
+    if (BEAM) {
+        // candidate keys; drop those that cannot enter a full beam
+        uint64_t key = (lane < k) ? pkey(dv, myid) : ~0ull;
+        float* bd = S.bd[buf];
+        uint32_t* bp = S.bp[buf];
+        if (bsize == L && lane < k) {
+            uint64_t last = pkey(bd[L - 1], bp[L - 1] & 0x7fffffffu);
+            if (!(key < last)) key = ~0ull;
+        }
+        key = wave_sort64(key);
+        uint64_t km = ballot(key != ~0ull);
+        int kc = popc64(km);
+        if (kc == 0) return true;
+        S.cs[lane] = key;
+        __syncthreads();
+        float* od = S.bd[buf ^ 1];
+        uint32_t* op = S.bp[buf ^ 1];
+        // beam elements move right by the number of smaller candidates
+        for (uint32_t i = lane; i < bsize; i += 64) {
+            uint64_t bk = pkey(bd[i], bp[i] & 0x7fffffffu);
+            int lo = 0, hi = kc;
+            while (lo < hi) { int mid = (lo + hi) >> 1; if (S.cs[mid] < bk) lo = mid + 1; else hi = mid; }
+            uint32_t pos = i + (uint32_t)lo;
+            if (pos < L) { od[pos] = bd[i]; op[pos] = bp[i]; }
+        }
+        if (lane < kc) {
+            uint32_t lo = 0, hi = bsize;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (pkey(bd[mid], bp[mid] & 0x7fffffffu) < key) lo = mid + 1; else hi = mid;
+            }
+            uint32_t pos = (uint32_t)lane + lo;
+            if (pos < L) { od[pos] = pkey_d(key); op[pos] = pkey_id(key); }
+        }
+        bsize = min(L, bsize + (uint32_t)kc);
+        buf ^= 1;
+        __syncthreads();
+    }
+    return true;
+}
+
+__device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
+    const int lane = __lane_id(), l = lane & 15;
+    const uint32_t L = P.Lq ? P.Lq[qi] : P.L;
+    const float* thr = P.thr + (P.kidx ? (size_t)P.kidx[qi] * P.n : 0);
+    // query transform (rknn_query.cpp:2719-2730), binary64 product rounded to float
+    const float* q = P.queries + (size_t)qi * P.d;
+    for (uint32_t j = lane; j < P.d; j += 64) {
+        float v = q[j];
+        S.qbase[j] = v;
+        S.qlift[j] = sd_to_float(sd_mul(sd_from_float(v), P.scale_bits));
+    }
+    if (lane == 0) { S.qlift[P.d] = 1.0f; S.qlift[P.d + 1] = 0.0f; }
+    for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    __syncthreads();
+    QRow QL;
+    qrow_load(QL, S.qlift, P.dl, l);
+    // entry point: search-tree descent (rptrees.h:108-123), ties go left
+    uint32_t node = P.t_root;
+    while (P.t_left[node] != U32_NONE) {
+        float mg = qrow_ip(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l) - P.t_off[node];
+        node = (mg < -1e-6f) ? P.t_left[node] : (mg > 1e-6f ? P.t_right[node] : P.t_left[node]);
+    }
+    uint32_t ep = P.t_entry[node];
+    float dep = qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
+    uint32_t nd = 1, rcnt = 0, bsize = 1;
+    int buf = 0, ovf = 0;
+    if (lane == 0) {
+        S.hash[hslot(ep, P.hmask)] = ep + 1u;
+        S.bd[0][0] = dep;
+        S.bp[0][0] = ep;
+        if (dep <= P.R2f) S.res[0] = ep;
+    }
+    if (dep <= P.R2f) rcnt = 1;
+    __syncthreads();
+    // (1) beam descent
+    for (;;) {
+        uint32_t cur = U32_NONE;
+        for (uint32_t b = 0; b < bsize; b += 64) {
+            uint32_t i = b + lane;
+            uint64_t m = ballot(i < bsize && !(S.bp[buf][i] >> 31));
+            if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
+        }
+        if (cur == U32_NONE) break;
+        uint32_t c = S.bp[buf][cur];
+        __syncthreads();
+        if (lane == 0) S.bp[buf][cur] = c | 0x80000000u;
+        __syncthreads();
+        if (!expand<true>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+    }
+    // (2) flood over collected nodes, FIFO order
+    uint32_t head = 0;
+    while (!ovf && head < rcnt) {
+        uint32_t c = S.res[head++];
+        if (!expand<false>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+    }
+    if (ovf) {
+        if (lane == 0) { P.status[qi] = (uint32_t)ovf; P.cand_cnt[qi] = rcnt; P.ndist[qi] = nd; P.pred_cnt[qi] = 0; }
+        return;
+    }
+    // (3) postfilter in base space
+    QRow QB;
+    qrow_load(QB, S.qbase, P.d, l);
+    float qn = qrow_ip(QB, S.qbase, P.d, l);
+    uint32_t npred = 0;
+    uint32_t* out = P.cand_ids + (size_t)qi * P.out_cap;
+    uint8_t* acc = P.accept + (size_t)qi * P.out_cap;
+    for (uint32_t b = 0; b < rcnt; b += 4) {
+        uint32_t j = b + (uint32_t)(lane >> 4);
+        int ok = 0;
+        if (j < rcnt) {
+            uint32_t o = S.res[j];
+            float ip = qrow_ip(QB, P.base + (size_t)o * P.d, P.d, l);
+            float key = fmaf(-2.0f, ip, P.bnorm[o] + qn);
+            ok = key <= thr[o];
+            if (l == 0 && j < P.out_cap) { out[j] = o; acc[j] = (uint8_t)ok; }
+        }
+        npred += (uint32_t)popc64(ballot(ok && l == 0));
+    }
+    if (lane == 0) {
+        P.status[qi] = (rcnt > P.out_cap) ? 3u : 0u;
+        P.cand_cnt[qi] = rcnt;
+        P.ndist[qi] = nd;          // graph distances (the verifier adds rcnt)
+        P.pred_cnt[qi] = npred;
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(64) k_range_search(SearchParams P) {
+    extern __shared__ __align__(16) char lds[];
+    SearchLds S = carve(lds, P);
+    uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
+    if (blockIdx.x >= P.nq) return;
+    search_one(P, S, qi);
+}
+
+// Persistent form: a fixed grid pulls query indices from a device counter.
+// Every wave exits once the counter passes nq.
+__global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) {
+    extern __shared__ __align__(16) char lds[];
+    SearchLds S = carve(lds, P);
+    __shared__ uint32_t next;
+    for (;;) {
+        if (__lane_id() == 0) next = atomicAdd(P.work_counter, 1u);
+        __syncthreads();
+        uint32_t t = next;
+        __syncthreads();
+        if (t >= P.nq) break;
+        uint32_t qi = P.qlist ? P.qlist[t] : t;
+        search_one(P, S, qi);
+        __syncthreads();
+    }
+}
+
+size_t search_lds_bytes(const SearchParams& P) {
+    auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
+    return r(4 * (P.dl + 16)) + r(4 * (P.d + 16)) + 2 * r(4 * P.Lmax) + 2 * r(4 * P.Lmax) +
+           r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) + r(4 * 64) + r(4 * 64) + r(8 * 64);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s) {
+    size_t lds = search_lds_bytes(P);
+    if (persistent) {
+        hipFuncSetAttribute((const void*)k_range_search_persistent,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_range_search_persistent, dim3(grid), dim3(64), lds, s, P);
+    } else {
+        if (P.nq == 0) return hipSuccess;
+        hipFuncSetAttribute((const void*)k_range_search,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_range_search, dim3(P.nq), dim3(64), lds, s, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+int search_persistent_occupancy(size_t lds) {
+    int occ = 0;
+    hipFuncSetAttribute((const void*)k_range_search_persistent, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_range_search_persistent, 64, lds) != hipSuccess)
+        occ = 1;
+    return occ;
+}

@@ -1,0 +1,205 @@
+// kernels.hpp -- parameter blocks and host launchers shared between the
+// kernel translation units and the C-ABI layer (abi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+// ---- range search + postfilter (k_search.hip) ------------------------------
+struct SearchParams {
+    const float* base;      // n x d
+    const float* bnorm;     // n
+    const float* lifted;    // n x dl
+    const uint32_t* deg;    // n
+    const uint32_t* links;  // n x M
+    uint32_t n, d, dl, M;
+    const uint32_t* t_left;
+    const uint32_t* t_right;
+    const float* t_off;
+    const float* t_hyper;   // nodes x dl
+    const uint32_t* t_entry;
+    uint32_t t_root;
+    uint64_t scale_bits;    // binary64 lift scale
+    float R2f;              // largest float <= M*^2 + 1
+    const float* queries;   // nq x d (base space)
+    uint32_t nq;
+    const uint32_t* qlist;  // optional query index list
+    uint32_t L, Lmax;
+    const uint32_t* Lq;     // optional per-query L
+    const float* thr;       // thresholds (one table, or tables x n)
+    const uint32_t* kidx;   // optional per-query table index
+    uint32_t hmask;         // visited hash size - 1
+    uint32_t rcap;          // LDS FIFO capacity
+    uint32_t out_cap;       // per-query output capacity
+    uint32_t* cand_ids;     // nq x out_cap
+    uint8_t* accept;        // nq x out_cap
+    uint32_t* cand_cnt;     // nq
+    uint32_t* ndist;        // nq (graph distance evaluations)
+    uint32_t* pred_cnt;     // nq
+    uint32_t* status;       // nq: 0 ok, 1 hash full, 2 FIFO full, 3 output truncated
+    uint32_t* work_counter; // persistent form
+};
+size_t search_lds_bytes(const SearchParams& P);
+int search_persistent_occupancy(size_t lds);
+hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+// ---- verifier (k_lrq.hip) ---------------------------------------------------
+hipError_t launch_lrq_encode(const float* cut, uint64_t n, uint32_t nk, uint32_t rankm,
+                             const float* mu, const float* basis, const float* cb,
+                             uint32_t code_size, float* coeff, uint8_t* codes,
+                             uint8_t* codes_kmajor, hipStream_t s);
+hipError_t launch_lrq_materialize(const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
+                                  uint32_t nk, uint32_t rankm, const float* mu,
+                                  const float* basis, const float* cb, uint32_t code_size,
+                                  uint32_t k_one_based, uint64_t slack_bits, float* thr,
+                                  hipStream_t s);
+hipError_t launch_transpose_codes(const uint8_t* codes, uint64_t n, uint32_t nk,
+                                  uint8_t* codes_kmajor, hipStream_t s);
+
+// ---- distances (k_dist.hip) ---------------------------------------------------
+hipError_t launch_norms(const float* x, uint64_t n, uint32_t d, float* out, hipStream_t s);
+hipError_t launch_pair_d2(const float* x, const float* nrm, uint32_t d, const uint32_t* a,
+                          const uint32_t* b, uint64_t m, float* out, hipStream_t s);
+
+// ---- graph construction (k_graph.hip) ------------------------------------------
+hipError_t launch_warm_start(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                             const uint32_t* ids, uint32_t width, uint32_t cap,
+                             uint32_t* hidx, float* hval, hipStream_t s);
+hipError_t launch_nnd_sample(const uint32_t* hidx, uint32_t n, uint32_t cap, uint32_t sample,
+                             uint32_t* samp, uint32_t* scnt, hipStream_t s);
+hipError_t launch_nnd_propose(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                              const uint32_t* samp, const uint32_t* scnt, uint32_t sample,
+                              const float* hval, uint32_t cap, uint64_t* keys, uint32_t* vals,
+                              unsigned long long* count, uint64_t max_props, hipStream_t s);
+hipError_t launch_nnd_apply(const uint64_t* keys, const uint32_t* vals, uint64_t m,
+                            const uint32_t* row_off, uint32_t n, uint32_t* hidx, float* hval,
+                            uint32_t cap, unsigned long long* upd, hipStream_t s);
+hipError_t launch_row_bounds(const uint64_t* keys, uint64_t m, uint32_t n, uint32_t* row_off,
+                             hipStream_t s);
+hipError_t launch_count_refs(const uint32_t* ids, uint64_t m, uint32_t n, uint32_t* cnt,
+                             hipStream_t s);
+hipError_t launch_scatter_rev(const uint32_t* ids, uint32_t rows, uint32_t width,
+                              const uint32_t* rowcnt, uint32_t n, uint32_t* cursor,
+                              uint32_t* rids, hipStream_t s);
+
+struct GatherParams {
+    const float* x;
+    const float* nrm;
+    uint32_t n, d;
+    const uint32_t* hidx;   // n x cap
+    const float* hval;
+    uint32_t cap;
+    const uint32_t* roff;   // n+1
+    const uint32_t* rids;
+    uint32_t explore;       // queue capacity
+    uint32_t pool_cap;      // stop once the pool holds this many
+    uint32_t K;             // top-K output (AKNN mode)
+    uint32_t M;             // prune degree (graph mode)
+    float alpha;
+    uint32_t hmask;         // visited hash size - 1
+    uint32_t pmax;          // pool capacity in the work area (graph mode)
+    const uint32_t* rows;   // optional row list
+    uint32_t nrows;
+    uint32_t* out_ids;      // AKNN: n x K ; graph: n x M
+    float* out_keys;        // AKNN: n x K
+    uint32_t* out_cnt;      // n
+    uint32_t* status;       // n: 0 ok, 1 hash full, 2 pool full
+    uint32_t* gscratch;     // optional global work area (per block)
+    size_t gscratch_words;  // words per block
+};
+size_t gather_lds_bytes(const GatherParams& P, bool graph_mode);
+hipError_t launch_gather(const GatherParams& P, bool graph_mode, bool global_ws, hipStream_t s);
+
+struct PruneParams {
+    const float* x;
+    const float* nrm;
+    uint32_t n, d;
+    const uint32_t* O;      // n x M (first ocnt[i] valid)
+    const uint32_t* ocnt;
+    const uint32_t* roff;   // reverse CSR of O
+    const uint32_t* rids;
+    uint32_t M;
+    float alpha;
+    uint32_t pmax;          // pool capacity
+    uint32_t* links;        // n x M
+    uint32_t* deg;
+    uint32_t* status;
+    const uint32_t* rows;
+    uint32_t nrows;
+    uint32_t* gscratch;
+    size_t gscratch_words;
+};
+size_t commit_lds_bytes(const PruneParams& P);
+hipError_t launch_commit(const PruneParams& P, bool global_ws, hipStream_t s);
+
+// ---- RP forest + leaf brute force (k_forest.hip) --------------------------------
+struct ForestLevel {
+    const float* x;
+    uint32_t n, d;
+    uint32_t* perm;        // trees x n
+    uint32_t* perm_out;
+    const uint32_t* seg_of; // trees x n: segment id of each position (UINT32_MAX: settled)
+    const uint32_t* seg_start;
+    const uint32_t* seg_size;
+    const uint32_t* seg_pick;  // 2 per segment: picked positions
+    float* seg_hyper;       // segments x d
+    float* seg_off;
+    uint8_t* side;          // trees x n
+    uint32_t nseg;
+    uint64_t total;         // trees * n
+};
+hipError_t launch_split_planes(const ForestLevel& F, hipStream_t s);
+hipError_t launch_split_sides(const ForestLevel& F, uint64_t seed, hipStream_t s);
+
+struct LeafParams {
+    const float* x;
+    const float* nrm;
+    uint32_t n, d;
+    const uint32_t* perm;       // the tree's permutation
+    const uint32_t* leaf_start;
+    const uint32_t* leaf_size;
+    uint32_t nleaves;
+    uint32_t K;
+    uint32_t* ids;              // n x K sorted by (dist, id), UINT32_MAX padded
+    float* keys;
+    uint32_t* cnt;              // n
+    uint32_t max_leaf;
+};
+size_t leaf_lds_bytes(const LeafParams& P);
+hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s);
+hipError_t launch_random_fill(const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t K,
+                              uint64_t seed, uint32_t* ids, float* keys, uint32_t* cnt,
+                              hipStream_t s);
+
+// ---- exact kNN / RkNN membership (k_exact.hip) ------------------------------------
+hipError_t launch_exact_topk(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                             uint32_t K, uint32_t row0, uint32_t rows, int mfma,
+                             const uint16_t* xbf, float margin_scale, uint32_t* ids,
+                             float* keys, hipStream_t s);
+hipError_t launch_rknn_count(const float* base, const float* bnorm, uint32_t n,
+                             const float* q, uint32_t nq, uint32_t d, const float* rk2,
+                             uint32_t* counts, hipStream_t s);
+hipError_t launch_rknn_fill(const float* base, const float* bnorm, uint32_t n,
+                            const float* q, uint32_t nq, uint32_t d, const float* rk2,
+                            const uint32_t* off, uint32_t* ids, float* keys, hipStream_t s);
+hipError_t launch_to_bf16(const float* x, uint64_t m, uint16_t* out, hipStream_t s);
+
+hipError_t launch_seg_fill(uint32_t* seg_of, const uint32_t* seg_start, const uint32_t* seg_size,
+                           uint32_t nseg, hipStream_t s);
+hipError_t launch_seg_count_left(const uint8_t* side, const uint32_t* seg_of, uint64_t total,
+                                 uint32_t* left, hipStream_t s);
+hipError_t launch_seg_redeal(uint8_t* side, const uint32_t* seg_of, const uint8_t* degen, uint64_t total,
+                             uint64_t seed, hipStream_t s);
+hipError_t launch_left_flags(const uint8_t* side, const uint32_t* seg_of, uint64_t total, uint32_t* flags,
+                             hipStream_t s);
+hipError_t launch_seg_partition(const uint32_t* perm, uint32_t* perm_out, const uint8_t* side,
+                                const uint32_t* seg_of, const uint32_t* seg_start, const uint32_t* left,
+                                const uint32_t* lscan, uint64_t total, hipStream_t s);
+
+// ---- device-wide primitives (k_prims.hip, hipCUB) ----------------------------------
+hipError_t prim_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t m, void* tmp,
+                                   size_t& tmp_bytes, hipStream_t s);
+hipError_t prim_sort_pairs_u64_u32(const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                                   uint32_t* vout, uint64_t m, int end_bit, void* tmp,
+                                   size_t& tmp_bytes, hipStream_t s);

@@ -1,0 +1,340 @@
+"""ctypes bindings for libanqi_hip.so (include/anqi.h).
+
+PyTorch provides device memory and streams only: tensors are passed to the C
+ABI as raw pointers.  Loading fails loudly when the library is missing; there
+is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libanqi_hip.so")
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int
+F32 = ctypes.c_float
+F64 = ctypes.c_double
+
+
+class AknnParams(ctypes.Structure):
+    _fields_ = [("K", U32), ("leaf", U32), ("trees", U32), ("ef", U32), ("ef_alpha", U32),
+                ("max_depth", U32), ("seed", U64), ("precision", I32)]
+
+
+class AknnStats(ctypes.Structure):
+    _fields_ = [("forest_s", F64), ("leaf_s", F64), ("gather_s", F64), ("total_s", F64),
+                ("leaf_pairs", U64), ("leaves", U32), ("max_leaf", U32), ("short_rows", U32),
+                ("overflow_rows", U32)]
+
+
+class Tree(ctypes.Structure):
+    _fields_ = [("nodes", U32), ("root", U32), ("d", U32), ("left", P), ("right", P), ("offset", P),
+                ("hyper", P), ("entry", P), ("leaf_off", ctypes.POINTER(U64)),
+                ("leaf_ids", ctypes.POINTER(U32))]
+
+
+class GraphParams(ctypes.Structure):
+    _fields_ = [("M", U32), ("nn_k", U32), ("leaf", U32), ("explore", U32), ("ef_alpha", U32),
+                ("iters", I32), ("sample", I32), ("alpha", F32), ("seed", U64)]
+
+
+class GraphStats(ctypes.Structure):
+    _fields_ = [("tree_s", F64), ("warm_s", F64), ("nnd_s", F64), ("gather_s", F64), ("commit_s", F64),
+                ("total_s", F64), ("nnd_passes", I32), ("nnd_upd", ctypes.c_int64 * 16),
+                ("overflow_rows", U32)]
+
+
+class Index(ctypes.Structure):
+    _fields_ = [("base", P), ("bnorm", P), ("lifted", P), ("deg", P), ("links", P), ("n", U32), ("d", U32),
+                ("dl", U32), ("M", U32), ("tree", ctypes.POINTER(Tree)), ("mstar2", F64), ("scale", F64)]
+
+
+SIGS = {
+    "anqi_ctx_create": (I32, [I32, ctypes.POINTER(P)]),
+    "anqi_ctx_destroy": (None, [P]),
+    "anqi_last_error": (ctypes.c_char_p, [P]),
+    "anqi_ctx_sync": (I32, [P]),
+    "anqi_ctx_set_stream": (I32, [P, P]),
+    "anqi_ctx_stream": (P, [P]),
+    "anqi_dmalloc": (I32, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+    "anqi_dfree": (I32, [P, P]),
+    "anqi_h2d": (I32, [P, P, P, ctypes.c_size_t]),
+    "anqi_d2h": (I32, [P, P, P, ctypes.c_size_t]),
+    "anqi_norms": (I32, [P, P, U64, U32, P]),
+    "anqi_pair_d2": (I32, [P, P, P, U32, P, P, U64, P]),
+    "anqi_exact_knn": (I32, [P, P, U32, U32, U32, I32, P, P]),
+    "anqi_rknn_membership": (I32, [P, P, U32, P, U32, U32, P, P, ctypes.POINTER(U64), P, P]),
+    "anqi_aknn_build": (I32, [P, P, U32, U32, ctypes.POINTER(AknnParams), P, P, ctypes.POINTER(AknnStats)]),
+    "anqi_lrq_encode": (I32, [P, P, U64, U32, U32, P, P, P, U32, P, P, P]),
+    "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
+    "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
+    "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
+    "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
+    "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32)]),
+    "anqi_reverse_csr": (I32, [P, P, U32, U32, P, U32, P, P]),
+    "anqi_gather_topk": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, P, P, P]),
+    "anqi_gather_prune": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, F32, P, P]),
+    "anqi_commit": (I32, [P, P, P, U32, U32, P, P, U32, F32, P, P]),
+    "anqi_rp_tree": (I32, [P, P, U32, U32, U32, U32, U64, ctypes.POINTER(Tree)]),
+    "anqi_tree_upload": (I32, [P, U32, U32, U32, P, P, P, P, P, ctypes.POINTER(Tree)]),
+    "anqi_tree_free": (I32, [P, ctypes.POINTER(Tree)]),
+    "anqi_graph_build": (I32, [P, P, U32, U32, P, U32, ctypes.POINTER(GraphParams), P, P, ctypes.POINTER(Tree),
+                               ctypes.POINTER(GraphStats)]),
+    "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, I32]),
+    "anqi_build_info": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (make -C anqi-implementation_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class AnqiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"anqi status {code}: {msg}")
+        self.code = code
+
+
+def dp(t):
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    return P(t.data_ptr())
+
+
+def hp(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+class Context:
+    """One anqi_ctx bound to a device; runs on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        self.torch = torch
+        self.L = lib()
+        self.ctx = P()
+        rc = self.L.anqi_ctx_create(device, ctypes.byref(self.ctx))
+        if rc != 0:
+            raise AnqiError(rc, "anqi_ctx_create failed")
+        self.device = device
+        self.use_torch_stream()
+
+    def use_torch_stream(self):
+        s = self.torch.cuda.current_stream(self.device)
+        self.L.anqi_ctx_set_stream(self.ctx, P(s.cuda_stream))
+
+    def close(self):
+        if self.ctx:
+            self.L.anqi_ctx_destroy(self.ctx)
+            self.ctx = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            raise AnqiError(rc, self.L.anqi_last_error(self.ctx).decode())
+
+    def call(self, name, *args):
+        self.check(getattr(self.L, name)(self.ctx, *args))
+
+    def sync(self):
+        self.call("anqi_ctx_sync")
+
+    # ---- helpers returning torch tensors -------------------------------------
+    def empty(self, shape, dtype):
+        return self.torch.empty(shape, dtype=dtype, device=f"cuda:{self.device}")
+
+    def to_dev(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to(f"cuda:{self.device}")
+
+    def norms(self, x):
+        n, d = x.shape
+        out = self.empty((n,), self.torch.float32)
+        self.call("anqi_norms", dp(x), n, d, dp(out))
+        return out
+
+    def exact_knn(self, x, K, precision=0):
+        n, d = x.shape
+        ids = self.empty((n, K), self.torch.int32)
+        keys = self.empty((n, K), self.torch.float32)
+        self.call("anqi_exact_knn", dp(x), n, d, K, precision, dp(ids), dp(keys))
+        return ids, keys
+
+    def rknn_membership(self, base, q, rk2):
+        n, d = base.shape
+        nq = q.shape[0]
+        off = np.zeros(nq + 1, np.uint32)
+        tot = U64(0)
+        self.call("anqi_rknn_membership", dp(base), n, dp(q), nq, d, dp(rk2), hp(off), ctypes.byref(tot), None, None)
+        ids = self.empty((max(1, tot.value),), self.torch.int32)
+        keys = self.empty((max(1, tot.value),), self.torch.float32)
+        self.call("anqi_rknn_membership", dp(base), n, dp(q), nq, d, dp(rk2), hp(off), ctypes.byref(tot), dp(ids),
+                  dp(keys))
+        return off, ids[:tot.value], keys[:tot.value]
+
+    def aknn_build(self, x, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, precision=0, max_depth=100):
+        n, d = x.shape
+        ids = self.empty((n, K), self.torch.int32)
+        keys = self.empty((n, K), self.torch.float32)
+        p = AknnParams(K, leaf, trees, ef, ef_alpha, max_depth, seed, precision)
+        st = AknnStats()
+        self.call("anqi_aknn_build", dp(x), n, d, ctypes.byref(p), dp(ids), dp(keys), ctypes.byref(st))
+        return ids, keys, st
+
+    def lrq_encode(self, cut, mu, basis, cb):
+        n, nk = cut.shape
+        rankm = basis.shape[0]
+        coeff = self.empty((n, rankm), self.torch.float32)
+        codes = self.empty((n, nk), self.torch.uint8)
+        codes_km = self.empty((nk, n), self.torch.uint8)
+        self.call("anqi_lrq_encode", dp(cut), n, nk, rankm, dp(mu), dp(basis), dp(cb), cb.shape[1], dp(coeff),
+                  dp(codes), dp(codes_km))
+        return coeff, codes, codes_km
+
+    def transpose_codes(self, codes):
+        n, nk = codes.shape
+        out = self.empty((nk, n), self.torch.uint8)
+        self.call("anqi_transpose_codes", dp(codes), n, nk, dp(out))
+        return out
+
+    def lrq_materialize(self, coeff, codes_km, mu, basis, cb, k, slack=1.0, out=None):
+        n, rankm = coeff.shape
+        nk = mu.shape[0]
+        thr = out if out is not None else self.empty((n,), self.torch.float32)
+        self.call("anqi_lrq_materialize", dp(coeff), dp(codes_km), n, nk, rankm, dp(mu), dp(basis), dp(cb),
+                  cb.shape[1], k, slack, dp(thr))
+        return thr
+
+    def warm_start(self, x, nrm, ids, cap):
+        n, d = x.shape
+        width = ids.shape[1]
+        hidx = self.empty((n, cap), self.torch.int32)
+        hval = self.empty((n, cap), self.torch.float32)
+        self.call("anqi_warm_start", dp(x), dp(nrm), n, d, dp(ids), width, cap, dp(hidx), dp(hval))
+        return hidx, hval
+
+    def nndescent(self, x, nrm, hidx, hval, iters=6, sample=20):
+        n, d = x.shape
+        cap = hidx.shape[1]
+        upd = np.zeros(max(iters, 1), np.int64)
+        passes = I32(0)
+        self.call("anqi_nndescent", dp(x), dp(nrm), n, d, cap, dp(hidx), dp(hval), iters, sample, hp(upd),
+                  ctypes.byref(passes))
+        return passes.value, upd[:passes.value]
+
+    def reverse_csr(self, ids, n, rowcnt=None):
+        rows, width = ids.shape
+        roff = self.empty((n + 1,), self.torch.int32)
+        rids = self.empty((max(1, rows * width),), self.torch.int32)
+        self.call("anqi_reverse_csr", dp(ids), rows, width, dp(rowcnt), n, dp(roff), dp(rids))
+        return roff, rids
+
+    def gather_topk(self, x, nrm, hidx, hval, roff, rids, explore, pool_cap, K):
+        n, d = x.shape
+        ids = self.empty((n, K), self.torch.int32)
+        keys = self.empty((n, K), self.torch.float32)
+        cnt = self.empty((n,), self.torch.int32)
+        self.call("anqi_gather_topk", dp(x), dp(nrm), n, d, dp(hidx), dp(hval), hidx.shape[1], dp(roff), dp(rids),
+                  explore, pool_cap, K, dp(ids), dp(keys), dp(cnt))
+        return ids, keys, cnt
+
+    def gather_prune(self, x, nrm, hidx, hval, roff, rids, explore, pool_cap, M, alpha):
+        n, d = x.shape
+        O = self.empty((n, M), self.torch.int32)
+        ocnt = self.empty((n,), self.torch.int32)
+        self.call("anqi_gather_prune", dp(x), dp(nrm), n, d, dp(hidx), dp(hval), hidx.shape[1], dp(roff), dp(rids),
+                  explore, pool_cap, M, alpha, dp(O), dp(ocnt))
+        return O, ocnt
+
+    def commit(self, x, nrm, O, ocnt, M, alpha):
+        n, d = x.shape
+        links = self.empty((n, M), self.torch.int32)
+        deg = self.empty((n,), self.torch.int32)
+        self.call("anqi_commit", dp(x), dp(nrm), n, d, dp(O), dp(ocnt), M, alpha, dp(links), dp(deg))
+        return links, deg
+
+    def rp_tree(self, x, leaf=100, max_depth=100, seed=10):
+        n, d = x.shape
+        t = Tree()
+        self.call("anqi_rp_tree", dp(x), n, d, leaf, max_depth, seed, ctypes.byref(t))
+        return t
+
+    def tree_upload(self, tree):
+        """tree: dict of host arrays (nodes, root, d, left, right, offset, hyper, entry)."""
+        t = Tree()
+        keep = {k: np.ascontiguousarray(v) for k, v in tree.items() if isinstance(v, np.ndarray)}
+        self.call("anqi_tree_upload", tree["nodes"], tree["root"], tree["d"], hp(keep["left"]), hp(keep["right"]),
+                  hp(keep["offset"]), hp(keep["hyper"]), hp(keep["entry"]), ctypes.byref(t))
+        return t
+
+    def tree_free(self, t):
+        self.call("anqi_tree_free", ctypes.byref(t))
+
+    def graph_build(self, lifted, warm_ids, M=48, nn_k=50, leaf=100, explore=100, ef_alpha=8, iters=6, sample=20,
+                    alpha=1.2, seed=10):
+        n, dl = lifted.shape
+        links = self.empty((n, M), self.torch.int32)
+        deg = self.empty((n,), self.torch.int32)
+        t = Tree()
+        st = GraphStats()
+        p = GraphParams(M, nn_k, leaf, explore, ef_alpha, iters, sample, alpha, seed)
+        self.call("anqi_graph_build", dp(lifted), n, dl, dp(warm_ids), warm_ids.shape[1], ctypes.byref(p),
+                  dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
+        return links, deg, t, st
+
+    def make_index(self, base, bnorm, lifted, deg, links, tree, mstar2, scale):
+        ix = Index(dp(base), dp(bnorm), dp(lifted), dp(deg), dp(links), base.shape[0], base.shape[1],
+                   lifted.shape[1], links.shape[1], ctypes.pointer(tree), mstar2, scale)
+        ix._keep = (base, bnorm, lifted, deg, links, tree)
+        return ix
+
+    def search_buffers(self, nq, out_cap):
+        T = self.torch
+        return dict(cand=self.empty((nq, out_cap), T.int32), acc=self.empty((nq, out_cap), T.uint8),
+                    cnt=self.empty((nq,), T.int32), nd=self.empty((nq,), T.int32),
+                    pred=self.empty((nq,), T.int32), status=self.empty((nq,), T.int32))
+
+    def rknn_search(self, ix, queries, L, thr, out_cap=2048, mode=0, Lq=None, Lmax=0, kidx=None, bufs=None):
+        nq = queries.shape[0]
+        b = bufs if bufs is not None else self.search_buffers(nq, out_cap)
+        self.call("anqi_rknn_search", ctypes.byref(ix), dp(queries), nq, L, dp(Lq), Lmax, dp(thr), dp(kidx),
+                  b["cand"].shape[1], dp(b["cand"]), dp(b["acc"]), dp(b["cnt"]), dp(b["nd"]), dp(b["pred"]),
+                  dp(b["status"]), mode)
+        return b
+
+
+def lift_host(base: np.ndarray, rk_horizon: np.ndarray):
+    base = np.ascontiguousarray(base, np.float32)
+    rk = np.ascontiguousarray(rk_horizon, np.float32)
+    n, d = base.shape
+    out = np.zeros((n, d + 2), np.float32)
+    m2, sc = F64(), F64()
+    rc = lib().anqi_lift_host(hp(base), n, d, hp(rk), hp(out), ctypes.byref(m2), ctypes.byref(sc))
+    if rc != 0:
+        raise AnqiError(rc, "lift failed")
+    return out, m2.value, sc.value

@@ -1,0 +1,206 @@
+/*
+ * anqi.h -- C ABI of the MI355X ANQI reverse-kNN library (libanqi_hip.so).
+ *
+ * The reference exposes no library API: its boundary is the processes
+ * build_aknn_source and rknn_query (argv, ANQI_* environment, on-disk files,
+ * stdout lines, exit codes).  This ABI is the layer the drop-in command-line
+ * tools (anqi-implementation_amd/cli/) call; each entry point names the
+ * reference code it replaces.  Conventions:
+ *   - plain pointers and sizes; "device" arguments are HIP device pointers,
+ *     "host" arguments are host pointers;
+ *   - every call returns an int status (ANQI_OK or one of the codes below,
+ *     which the tools map to the reference exit codes) and never throws;
+ *     anqi_last_error() describes the last failure;
+ *   - work is enqueued on the context's stream; calls that must read results
+ *     on the host (counts, overflow checks) synchronise internally;
+ *   - a context is used by one host thread at a time.
+ */
+#ifndef ANQI_H
+#define ANQI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANQI_OK 0
+#define ANQI_E_USAGE 2     /* bad arguments (reference exit 2) */
+#define ANQI_E_INPUT 3     /* bad input / format (exit 3) */
+#define ANQI_E_ROWS 4      /* row-count mismatch (exit 4) */
+#define ANQI_E_INVALID 5   /* invalid output row (exit 5) */
+#define ANQI_E_IO 6        /* write failure (exit 6) */
+#define ANQI_E_DEVICE 7    /* HIP / device failure */
+
+typedef struct anqi_ctx anqi_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+int anqi_ctx_create(int device, anqi_ctx** out);
+void anqi_ctx_destroy(anqi_ctx* ctx);
+const char* anqi_last_error(const anqi_ctx* ctx);
+int anqi_ctx_sync(anqi_ctx* ctx);
+/* Use an external hipStream_t (e.g. the caller's); NULL restores the own one. */
+int anqi_ctx_set_stream(anqi_ctx* ctx, void* stream);
+void* anqi_ctx_stream(anqi_ctx* ctx);
+int anqi_dmalloc(anqi_ctx* ctx, size_t bytes, void** out);
+int anqi_dfree(anqi_ctx* ctx, void* p);
+int anqi_h2d(anqi_ctx* ctx, void* dst, const void* src, size_t bytes);
+int anqi_d2h(anqi_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- FP32 distance contract (include/distance.h:22-161, index.h:172-176) -- */
+/* out[i] = <x_i, x_i>.  device x (n*d), out (n). */
+int anqi_norms(anqi_ctx* ctx, const float* x, uint64_t n, uint32_t d, float* out);
+/* out[t] = nrm[a] + nrm[b] - 2 <x_a, x_b>.  all device. */
+int anqi_pair_d2(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t d,
+                 const uint32_t* a, const uint32_t* b, uint64_t m, float* out);
+
+/* ---- exact ground truth (tools/dataset_gt.cpp:206-313) ------------------- */
+/* Self-excluded top-K by (key, id).  precision 0: FP32 tiles; 1: bf16 MFMA
+ * tiles with exact FP32 re-scoring (same output).  device x, ids, keys (n*K). */
+int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t K,
+                   int precision, uint32_t* ids, float* keys);
+/* Reverse-kNN membership key <= rk2[j].  host off (nq+1) and *total out;
+ * ids/keys are device buffers of *total entries, each row sorted by (key, id).
+ * Call with ids == NULL to size. */
+int anqi_rknn_membership(anqi_ctx* ctx, const float* base, uint32_t n, const float* q,
+                         uint32_t nq, uint32_t d, const float* rk2, uint32_t* off,
+                         uint64_t* total, uint32_t* ids, float* keys);
+
+/* ---- AKNN source (build_aknn_source, include/index.h:1300-1461) ----------- */
+typedef struct {
+    uint32_t K;          /* 100 */
+    uint32_t leaf;       /* 500 */
+    uint32_t trees;      /* 64 */
+    uint32_t ef;         /* explore range, 600 */
+    uint32_t ef_alpha;   /* pool cap = max(K, ef_alpha * ef), 8 */
+    uint32_t max_depth;  /* 100 */
+    uint64_t seed;       /* 10 */
+    int precision;       /* 0 FP32 leaf tiles, 1 bf16 MFMA filtered */
+} anqi_aknn_params;
+typedef struct {
+    double forest_s, leaf_s, gather_s, total_s;
+    uint64_t leaf_pairs;
+    uint32_t leaves, max_leaf, short_rows, overflow_rows;
+} anqi_aknn_stats;
+/* device x (n*d); device ids/keys (n*K) rows sorted by (dist, id), self
+ * excluded; keys are the raw d2 (the tool clamps at 0 when writing .rec). */
+int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d,
+                    const anqi_aknn_params* p, uint32_t* ids, float* keys,
+                    anqi_aknn_stats* stats);
+
+/* ---- LRQ verifier (src/rknn_query.cpp:603-753, policy floor, u8) ---------- */
+/* device everything.  codes row-major (n*nk, file order); codes_kmajor
+ * (nk*n) is the device layout materialize reads (may be NULL). */
+int anqi_lrq_encode(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk,
+                    uint32_t rankm, const float* mu, const float* basis,
+                    const float* codebook, uint32_t code_size, float* coeff,
+                    uint8_t* codes, uint8_t* codes_kmajor);
+int anqi_transpose_codes(anqi_ctx* ctx, const uint8_t* codes, uint64_t n, uint32_t nk,
+                         uint8_t* codes_kmajor);
+/* thr[i] = float(max(0, slack * (mu_k + sum_m c_m G_mk + cb_k[code])))  */
+int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor,
+                         uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
+                         const float* basis, const float* codebook, uint32_t code_size,
+                         uint32_t k_one_based, double slack, float* thr);
+
+/* ---- lift (src/rknn_query.cpp:2203-2236), host FP64 ----------------------- */
+/* host base (n*d), rk_horizon (n); host lifted (n*(d+2)). */
+int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
+                   float* lifted, double* mstar2, double* scale);
+
+/* ---- graph construction pieces (include/index.h:1071-1295) ---------------- */
+int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
+                    const uint32_t* ids, uint32_t width, uint32_t cap, uint32_t* hidx,
+                    float* hval);
+/* LocalJoin, snapshot schedule; upd (host, iters entries) receives updates
+ * per pass; returns the pass count through *passes. */
+int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
+                   uint32_t cap, uint32_t* hidx, float* hval, int iters, int sample,
+                   int64_t* upd, int* passes);
+/* Reverse lists of a rows x width id table (rowcnt NULL: sentinel-filtered). */
+int anqi_reverse_csr(anqi_ctx* ctx, const uint32_t* ids, uint32_t rows, uint32_t width,
+                     const uint32_t* rowcnt, uint32_t n, uint32_t* roff, uint32_t* rids);
+int anqi_gather_topk(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
+                     const uint32_t* hidx, const float* hval, uint32_t cap,
+                     const uint32_t* roff, const uint32_t* rids, uint32_t explore,
+                     uint32_t pool_cap, uint32_t K, uint32_t* ids, float* keys,
+                     uint32_t* cnt);
+int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
+                      const uint32_t* hidx, const float* hval, uint32_t cap,
+                      const uint32_t* roff, const uint32_t* rids, uint32_t explore,
+                      uint32_t pool_cap, uint32_t M, float alpha, uint32_t* O,
+                      uint32_t* ocnt);
+int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
+                const uint32_t* O, const uint32_t* ocnt, uint32_t M, float alpha,
+                uint32_t* links, uint32_t* deg);
+
+/* ---- search tree (include/rptrees.h:108-184) ------------------------------- */
+typedef struct {
+    uint32_t nodes, root, d;
+    uint32_t* left;    /* device; UINT32_MAX for leaves */
+    uint32_t* right;
+    float* offset;
+    float* hyper;      /* nodes * d */
+    uint32_t* entry;   /* leaf: first member (indices[0]) */
+    uint64_t* leaf_off;  /* host: nodes+1 offsets into leaf_ids (export) */
+    uint32_t* leaf_ids;  /* host: members of every leaf in stored order */
+} anqi_tree;
+int anqi_rp_tree(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t leaf,
+                 uint32_t max_depth, uint64_t seed, anqi_tree* out);
+/* Upload a tree given in host arrays (e.g. read from a reference graph file). */
+int anqi_tree_upload(anqi_ctx* ctx, uint32_t nodes, uint32_t root, uint32_t d,
+                     const uint32_t* left, const uint32_t* right, const float* offset,
+                     const float* hyper, const uint32_t* entry, anqi_tree* out);
+int anqi_tree_free(anqi_ctx* ctx, anqi_tree* t);
+
+typedef struct {
+    uint32_t M;        /* graph degree 48 */
+    uint32_t nn_k;     /* LIFTNNK 50 */
+    uint32_t leaf;     /* search tree leaf 100 */
+    uint32_t explore;  /* 100 */
+    uint32_t ef_alpha; /* 8 */
+    int iters;         /* GRAPHNND 6 */
+    int sample;        /* NNSAMPLE 20 */
+    float alpha;       /* 1.2 */
+    uint64_t seed;     /* 10 */
+} anqi_graph_params;
+typedef struct {
+    double tree_s, warm_s, nnd_s, gather_s, commit_s, total_s;
+    int nnd_passes;
+    int64_t nnd_upd[16];
+    uint32_t overflow_rows;
+} anqi_graph_stats;
+/* device lifted (n*dl), warm_ids (n*width); device links (n*M), deg (n). */
+int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl,
+                     const uint32_t* warm_ids, uint32_t width, const anqi_graph_params* p,
+                     uint32_t* links, uint32_t* deg, anqi_tree* tree, anqi_graph_stats* st);
+
+/* ---- RkNN query (index.h:1535-1652 + rknn_query.cpp:2692-2824) ----------- */
+typedef struct {
+    const float* base;      /* device n*d */
+    const float* bnorm;     /* device n */
+    const float* lifted;    /* device n*dl */
+    const uint32_t* deg;    /* device n */
+    const uint32_t* links;  /* device n*M */
+    uint32_t n, d, dl, M;
+    const anqi_tree* tree;
+    double mstar2, scale;
+} anqi_index;
+/* queries (device nq*d, base space).  thr: device thresholds for k (n floats)
+ * or, with kidx, tables (ntables*n) picked per query.  Lq/kidx optional
+ * device per-query arrays.  Outputs (device): cand_ids/accept nq*out_cap,
+ * cand_cnt, ndist (graph distances), pred_cnt, status (0 ok, 3 truncated).
+ * mode 0: one wavefront per query; 1: persistent work-queue kernel. */
+int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq,
+                     uint32_t L, const uint32_t* Lq, uint32_t Lmax, const float* thr,
+                     const uint32_t* kidx, uint32_t out_cap, uint32_t* cand_ids,
+                     uint8_t* accept, uint32_t* cand_cnt, uint32_t* ndist,
+                     uint32_t* pred_cnt, uint32_t* status, int mode);
+
+/* Library build description (compiler, arch). */
+const char* anqi_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
