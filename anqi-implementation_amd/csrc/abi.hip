@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -44,6 +45,20 @@ int anqi_fail_hip(anqi_ctx* ctx, hipError_t e, const char* what, int line) {
 }
 
 #define HCHK(expr) HIP_CHECK_RET(expr)
+
+// Stage progress on stderr when ANQI_PROGRESS is set (long builds).
+static void progress(const char* fmt, ...) {
+    static int on = -1;
+    if (on < 0) on = std::getenv("ANQI_PROGRESS") != nullptr;
+    if (!on) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::fprintf(stderr, "[anqi] ");
+    std::vfprintf(stderr, fmt, ap);
+    std::fprintf(stderr, "\n");
+    va_end(ap);
+    std::fflush(stderr);
+}
 
 static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -106,7 +121,12 @@ int anqi_ctx_sync(anqi_ctx* ctx) {
 }
 
 int anqi_ctx_set_stream(anqi_ctx* ctx, void* stream) {
-    ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+    ctx->stream = (hipStream_t)stream;   // NULL: the default (null) stream
+    return ANQI_OK;
+}
+
+int anqi_ctx_reset_stream(anqi_ctx* ctx) {
+    ctx->stream = ctx->own;
     return ANQI_OK;
 }
 
@@ -324,6 +344,7 @@ int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, 
         HCHK(hipMemcpyAsync(&u, updd.p, 8, hipMemcpyDeviceToHost, s));
         HCHK(hipStreamSynchronize(s));
         if (upd) upd[it] = (int64_t)u;
+        progress("localjoin pass %d: %llu proposals, %llu updates", it, (unsigned long long)m, u);
         if ((int64_t)u < (int64_t)(n / 200)) { it++; break; }
     }
     if (passes) *passes = it;
@@ -558,6 +579,7 @@ static int build_forest(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, u
         }
         if (active.empty()) break;
         uint32_t ns = (uint32_t)active.size();
+        progress("forest depth %u: %u segments to split", depth, ns);
         std::vector<uint32_t> hs(ns), hz(ns), hp(2 * ns);
         for (uint32_t k = 0; k < ns; k++) {
             hs[k] = active[k].start;
@@ -791,6 +813,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
         LP.max_leaf = mx;
         HCHK(launch_leaf_knn(LP, p->precision != 0, s));
         HCHK(hipStreamSynchronize(s));   // keeps ds/dz alive until the launch ends
+        if ((t & 7) == 7 || t + 1 == T) progress("leaf all-pairs: tree %u/%u", t + 1, T);
     }
     HCHK(launch_random_fill(x, nrm.as<float>(), n, d, K, p->seed ^ 0x5EEDull, lid.as<uint32_t>(), lkey.as<float>(),
                             lcnt.as<uint32_t>(), s));
@@ -805,6 +828,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     HCHK(ocnt.alloc(4ull * n));
     rc = reverse_csr(ctx, lid.as<uint32_t>(), n, K, nullptr, n, roff.as<uint32_t>(), rids.as<uint32_t>(), nullptr);
     if (rc) return rc;
+    progress("aknn: forest %.2fs leaves %.2fs; gathering", S.forest_s, S.leaf_s);
     uint32_t ef = p->ef ? p->ef : 100;
     uint32_t pool_cap = std::max<uint32_t>(K, (p->ef_alpha ? p->ef_alpha : 8) * ef);
     uint32_t maxrev = 0;
@@ -880,6 +904,7 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     HCHK(hipStreamSynchronize(s));
     double t4 = now_s();
     S.gather_s = t4 - t3;
+    progress("graph: gather+prune %.2fs (overflow rows %u); commit", S.gather_s, S.overflow_rows);
     rc = anqi_commit(ctx, lifted, nrm.as<float>(), n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), p->M, p->alpha,
                      links, deg);
     if (rc) return rc;
@@ -901,7 +926,7 @@ static float round_down_float(double v) {
 int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t L,
                      const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx, uint32_t out_cap,
                      uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt,
-                     uint32_t* status, int mode) {
+                     uint32_t* nexp, uint32_t* status, int mode) {
     if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
     if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
     if (nq == 0) return ANQI_OK;
@@ -918,6 +943,7 @@ int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, 
     P.thr = thr; P.kidx = kidx;
     P.hmask = 8191; P.rcap = 2048; P.out_cap = out_cap;
     P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
+    P.nexp = nexp;
     P.status = status;
     if (P.Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "L <= 1024");
     DBuf counter;

@@ -153,7 +153,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     }
     uint32_t ep = P.t_entry[node];
     float dep = qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
-    uint32_t nd = 1, rcnt = 0, bsize = 1;
+    uint32_t nd = 1, rcnt = 0, bsize = 1, nx = 0;
     int buf = 0, ovf = 0;
     if (lane == 0) {
         S.hash[hslot(ep, P.hmask)] = ep + 1u;
@@ -176,12 +176,14 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         __syncthreads();
         if (lane == 0) S.bp[buf][cur] = c | 0x80000000u;
         __syncthreads();
+        nx++;
         if (!expand<true>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
     }
     // (2) flood over collected nodes, FIFO order
     uint32_t head = 0;
     while (!ovf && head < rcnt) {
         uint32_t c = S.res[head++];
+        nx++;
         if (!expand<false>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
     }
     if (ovf) {
@@ -212,6 +214,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         P.cand_cnt[qi] = rcnt;
         P.ndist[qi] = nd;          // graph distances (the verifier adds rcnt)
         P.pred_cnt[qi] = npred;
+        if (P.nexp) P.nexp[qi] = nx;
     }
 }
 

@@ -36,6 +36,7 @@ struct SearchParams {
     uint32_t* cand_cnt;     // nq
     uint32_t* ndist;        // nq (graph distance evaluations)
     uint32_t* pred_cnt;     // nq
+    uint32_t* nexp;         // nq: expanded nodes (optional)
     uint32_t* status;       // nq: 0 ok, 1 hash full, 2 FIFO full, 3 output truncated
     uint32_t* work_counter; // persistent form
 };

@@ -62,6 +62,7 @@ SIGS = {
     "anqi_ctx_sync": (I32, [P]),
     "anqi_ctx_set_stream": (I32, [P, P]),
     "anqi_ctx_stream": (P, [P]),
+    "anqi_ctx_reset_stream": (I32, [P]),
     "anqi_dmalloc": (I32, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "anqi_dfree": (I32, [P, P]),
     "anqi_h2d": (I32, [P, P, P, ctypes.c_size_t]),
@@ -86,7 +87,7 @@ SIGS = {
     "anqi_tree_free": (I32, [P, ctypes.POINTER(Tree)]),
     "anqi_graph_build": (I32, [P, P, U32, U32, P, U32, ctypes.POINTER(GraphParams), P, P, ctypes.POINTER(Tree),
                                ctypes.POINTER(GraphStats)]),
-    "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, I32]),
+    "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, P, I32]),
     "anqi_build_info": (ctypes.c_char_p, []),
 }
 
@@ -317,14 +318,15 @@ class Context:
         T = self.torch
         return dict(cand=self.empty((nq, out_cap), T.int32), acc=self.empty((nq, out_cap), T.uint8),
                     cnt=self.empty((nq,), T.int32), nd=self.empty((nq,), T.int32),
-                    pred=self.empty((nq,), T.int32), status=self.empty((nq,), T.int32))
+                    pred=self.empty((nq,), T.int32), nexp=self.empty((nq,), T.int32),
+                    status=self.empty((nq,), T.int32))
 
     def rknn_search(self, ix, queries, L, thr, out_cap=2048, mode=0, Lq=None, Lmax=0, kidx=None, bufs=None):
         nq = queries.shape[0]
         b = bufs if bufs is not None else self.search_buffers(nq, out_cap)
         self.call("anqi_rknn_search", ctypes.byref(ix), dp(queries), nq, L, dp(Lq), Lmax, dp(thr), dp(kidx),
                   b["cand"].shape[1], dp(b["cand"]), dp(b["acc"]), dp(b["cnt"]), dp(b["nd"]), dp(b["pred"]),
-                  dp(b["status"]), mode)
+                  dp(b["nexp"]), dp(b["status"]), mode)
         return b
 
 

@@ -39,8 +39,10 @@ int anqi_ctx_create(int device, anqi_ctx** out);
 void anqi_ctx_destroy(anqi_ctx* ctx);
 const char* anqi_last_error(const anqi_ctx* ctx);
 int anqi_ctx_sync(anqi_ctx* ctx);
-/* Use an external hipStream_t (e.g. the caller's); NULL restores the own one. */
+/* Enqueue on an external hipStream_t (e.g. the caller's); NULL selects the
+ * default (null) stream.  anqi_ctx_reset_stream restores the context's own. */
 int anqi_ctx_set_stream(anqi_ctx* ctx, void* stream);
+int anqi_ctx_reset_stream(anqi_ctx* ctx);
 void* anqi_ctx_stream(anqi_ctx* ctx);
 int anqi_dmalloc(anqi_ctx* ctx, size_t bytes, void** out);
 int anqi_dfree(anqi_ctx* ctx, void* p);
@@ -189,13 +191,14 @@ typedef struct {
 /* queries (device nq*d, base space).  thr: device thresholds for k (n floats)
  * or, with kidx, tables (ntables*n) picked per query.  Lq/kidx optional
  * device per-query arrays.  Outputs (device): cand_ids/accept nq*out_cap,
- * cand_cnt, ndist (graph distances), pred_cnt, status (0 ok, 3 truncated).
+ * cand_cnt, ndist (graph distances), pred_cnt, nexp (expanded nodes,
+ * optional), status (0 ok, 3 truncated).
  * mode 0: one wavefront per query; 1: persistent work-queue kernel. */
 int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq,
                      uint32_t L, const uint32_t* Lq, uint32_t Lmax, const float* thr,
                      const uint32_t* kidx, uint32_t out_cap, uint32_t* cand_ids,
                      uint8_t* accept, uint32_t* cand_cnt, uint32_t* ndist,
-                     uint32_t* pred_cnt, uint32_t* status, int mode);
+                     uint32_t* pred_cnt, uint32_t* nexp, uint32_t* status, int mode);
 
 /* Library build description (compiler, arch). */
 const char* anqi_build_info(void);

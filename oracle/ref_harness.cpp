@@ -262,3 +262,52 @@ uint32_t ref_lrq_encode_value(double v, const float* centers, uint32_t code_size
 }
 
 }  // extern "C"
+
+// ---- CPU baseline: the reference's query loop body (rknn_query.cpp:2705-2776)
+// over an index loaded into the harness: lift q, range_search, postfilter
+// into an unordered_set.  Ground-truth evaluation (also inside the
+// reference's timer) is not included.  Returns elapsed seconds.
+extern "C" double ref_rknn_batch(void* h, const float* base, uint32_t d, const float* q, uint32_t nq,
+                                 double scale, double R2, uint64_t L, const float* thr, int threads,
+                                 uint32_t repeats, uint64_t* tot_cand, uint64_t* tot_pred,
+                                 uint64_t* tot_dist) {
+    nndindex* ix = static_cast<RefIndex*>(h)->idx;
+    const size_t n = ix->n, dl = ix->dim;
+    omp_set_num_threads(threads);
+    uint64_t c = 0, p = 0, dd = 0;
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel reduction(+ : c, p, dd)
+    {
+        std::vector<uint16_t> mark(n, 0);
+        uint16_t ver = 0;
+        std::vector<uint32_t> cand;
+        std::vector<float> qb(dl);
+#pragma omp for schedule(dynamic, 16)
+        for (size_t task = 0; task < (size_t)nq * repeats; task++) {
+            const size_t i = task % nq;
+            const float* qq = q + i * d;
+            for (size_t j = 0; j < d; j++) qb[j] = (float)((double)qq[j] * scale);
+            qb[d] = 1.0f;
+            qb[d + 1] = 0.0f;
+            if (++ver == 0) { std::fill(mark.begin(), mark.end(), 0); ver = 1; }
+            size_t nd = 0;
+            ix->range_search(qb.data(), R2, 0.0f, L, cand, nd, mark.data(), ver, nullptr, 0.5f, false);
+            std::unordered_set<uint32_t> pred;
+            for (uint32_t id : cand) {
+                const float* o = base + (size_t)id * d;
+                float ip = DistCal::InnerProductSIMD16ExtAVX512_(o, qq, d);
+                float on = DistCal::InnerProductSIMD16ExtAVX512_(o, o, d);
+                float qn = DistCal::InnerProductSIMD16ExtAVX512_(qq, qq, d);
+                if (on + qn - 2.0f * ip <= (double)thr[id]) pred.insert(id);
+            }
+            c += cand.size();
+            p += pred.size();
+            dd += nd + cand.size();
+        }
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *tot_cand = c;
+    *tot_pred = p;
+    *tot_dist = dd;
+    return secs;
+}
