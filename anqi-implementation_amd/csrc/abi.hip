@@ -927,6 +927,15 @@ int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, 
                      const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx, uint32_t out_cap,
                      uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt,
                      uint32_t* nexp, uint32_t* status, int mode) {
+    return anqi_rknn_search_entries(ctx, ix, queries, nq, L, Lq, Lmax, thr, kidx, nullptr, out_cap, cand_ids,
+                                    accept, cand_cnt, ndist, pred_cnt, nexp, status, mode);
+}
+
+int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t L,
+                             const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx,
+                             const uint32_t* entries, uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept,
+                             uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
+                             uint32_t* status, int mode) {
     if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
     if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
     if (nq == 0) return ANQI_OK;
@@ -936,6 +945,7 @@ int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, 
     P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
     P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
     P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
+    P.entries = entries;
     double sc = ix->scale;
     std::memcpy(&P.scale_bits, &sc, 8);
     P.R2f = round_down_float(ix->mstar2 + 1.0);

@@ -63,7 +63,21 @@ __global__ void __launch_bounds__(256) k_exact_topk(const float* __restrict__ x,
     __syncthreads();
     for (uint32_t c0 = 0; c0 < n; c0 += EX_C) {
         uint32_t cn = min((uint32_t)EX_C, n - c0);
-        for (uint32_t e = tid; e < cn * d; e += 256) tile[e] = x[(size_t)c0 * d + e];
+        // 16 independent loads in flight per thread before the LDS stores
+        const uint32_t tot = cn * d;
+        for (uint32_t b0 = 0; b0 < tot; b0 += 256 * 16) {
+            float r[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                uint32_t e = b0 + (uint32_t)u * 256 + (uint32_t)tid;
+                r[u] = e < tot ? x[(size_t)c0 * d + e] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                uint32_t e = b0 + (uint32_t)u * 256 + (uint32_t)tid;
+                if (e < tot) tile[e] = r[u];
+            }
+        }
         for (uint32_t e = tid; e < cn; e += 256) tnrm[e] = nrm[c0 + e];
         __syncthreads();
         uint64_t th[EX_RPG];

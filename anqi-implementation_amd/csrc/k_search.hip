@@ -146,12 +146,17 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     QRow QL;
     qrow_load(QL, S.qlift, P.dl, l);
     // entry point: search-tree descent (rptrees.h:108-123), ties go left
-    uint32_t node = P.t_root;
-    while (P.t_left[node] != U32_NONE) {
-        float mg = qrow_ip(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l) - P.t_off[node];
-        node = (mg < -1e-6f) ? P.t_left[node] : (mg > 1e-6f ? P.t_right[node] : P.t_left[node]);
+    uint32_t ep;
+    if (P.entries) {
+        ep = P.entries[qi];
+    } else {
+        uint32_t node = P.t_root;
+        while (P.t_left[node] != U32_NONE) {
+            float mg = qrow_ip(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l) - P.t_off[node];
+            node = (mg < -1e-6f) ? P.t_left[node] : (mg > 1e-6f ? P.t_right[node] : P.t_left[node]);
+        }
+        ep = P.t_entry[node];
     }
-    uint32_t ep = P.t_entry[node];
     float dep = qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
     uint32_t nd = 1, rcnt = 0, bsize = 1, nx = 0;
     int buf = 0, ovf = 0;

@@ -19,6 +19,7 @@ struct SearchParams {
     const float* t_hyper;   // nodes x dl
     const uint32_t* t_entry;
     uint32_t t_root;
+    const uint32_t* entries;   // optional per-query entry points (tree bypassed)
     uint64_t scale_bits;    // binary64 lift scale
     float R2f;              // largest float <= M*^2 + 1
     const float* queries;   // nq x d (base space)

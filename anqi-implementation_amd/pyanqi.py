@@ -88,6 +88,8 @@ SIGS = {
     "anqi_graph_build": (I32, [P, P, U32, U32, P, U32, ctypes.POINTER(GraphParams), P, P, ctypes.POINTER(Tree),
                                ctypes.POINTER(GraphStats)]),
     "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, P, I32]),
+    "anqi_rknn_search_entries": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, P, U32, P, P, P, P, P, P,
+                                       P, I32]),
     "anqi_build_info": (ctypes.c_char_p, []),
 }
 
@@ -321,12 +323,13 @@ class Context:
                     pred=self.empty((nq,), T.int32), nexp=self.empty((nq,), T.int32),
                     status=self.empty((nq,), T.int32))
 
-    def rknn_search(self, ix, queries, L, thr, out_cap=2048, mode=0, Lq=None, Lmax=0, kidx=None, bufs=None):
+    def rknn_search(self, ix, queries, L, thr, out_cap=2048, mode=0, Lq=None, Lmax=0, kidx=None, bufs=None,
+                    entries=None):
         nq = queries.shape[0]
         b = bufs if bufs is not None else self.search_buffers(nq, out_cap)
-        self.call("anqi_rknn_search", ctypes.byref(ix), dp(queries), nq, L, dp(Lq), Lmax, dp(thr), dp(kidx),
-                  b["cand"].shape[1], dp(b["cand"]), dp(b["acc"]), dp(b["cnt"]), dp(b["nd"]), dp(b["pred"]),
-                  dp(b["nexp"]), dp(b["status"]), mode)
+        self.call("anqi_rknn_search_entries", ctypes.byref(ix), dp(queries), nq, L, dp(Lq), Lmax, dp(thr), dp(kidx),
+                  dp(entries), b["cand"].shape[1], dp(b["cand"]), dp(b["acc"]), dp(b["cnt"]), dp(b["nd"]),
+                  dp(b["pred"]), dp(b["nexp"]), dp(b["status"]), mode)
         return b
 
 

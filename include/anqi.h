@@ -199,6 +199,15 @@ int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, 
                      const uint32_t* kidx, uint32_t out_cap, uint32_t* cand_ids,
                      uint8_t* accept, uint32_t* cand_cnt, uint32_t* ndist,
                      uint32_t* pred_cnt, uint32_t* nexp, uint32_t* status, int mode);
+/* Same, with explicit per-query entry points (device, nq) in place of the
+ * search-tree descent (the reference's ANQI_QUERY_SINGLE_ENTRY analogue,
+ * index.h:1597-1598); used to isolate graph quality from entry quality. */
+int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq,
+                             uint32_t L, const uint32_t* Lq, uint32_t Lmax, const float* thr,
+                             const uint32_t* kidx, const uint32_t* entries, uint32_t out_cap,
+                             uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt,
+                             uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
+                             uint32_t* status, int mode);
 
 /* Library build description (compiler, arch). */
 const char* anqi_build_info(void);

@@ -55,8 +55,9 @@ def test_norms_and_pair_d2_bits(ctx):
         b = rng.integers(0, 700, 5000).astype(np.uint32)
         out = ctx.empty((5000,), ctx.torch.float32)
         import pyanqi
-        ctx.call("anqi_pair_d2", pyanqi.dp(xd), pyanqi.dp(ctx.to_dev(ref)), d, pyanqi.dp(ctx.to_dev(a.view(np.int32))),
-                 pyanqi.dp(ctx.to_dev(b.view(np.int32))), 5000, pyanqi.dp(out))
+        keep = [ctx.to_dev(ref), ctx.to_dev(a.view(np.int32)), ctx.to_dev(b.view(np.int32))]   # alive until used
+        ctx.call("anqi_pair_d2", pyanqi.dp(xd), pyanqi.dp(keep[0]), d, pyanqi.dp(keep[1]), pyanqi.dp(keep[2]), 5000,
+                 pyanqi.dp(out))
         got = host(out)
         exp = np.array([O.or_d2(ptr(x[i]), ptr(x[j]), ref[i], ref[j], d) for i, j in zip(a, b)], np.float32)
         assert np.array_equal(bits(got), bits(exp))
