@@ -42,9 +42,14 @@ def env():
 
 def test_aknn_source_recall(env):
     ctx, ix = env["ctx"], env["ix"]
-    eids, _ = ctx.exact_knn(ix.base, 100)
+    eids, ekeys = ctx.exact_knn(ix.base, 100)
     a = ix.aknn_ids.cpu().numpy()
     e = eids.cpu().numpy()
+    ek = ekeys.cpu().numpy()
+    ak = ix.cut.cpu().numpy()
+    below = int((ak[:, 99] < np.maximum(ek[:, 99], 0) ).sum())
+    say("rows whose AKNN 100-th key is below the exact 100-th key:", below)
+    assert below == 0
     n = a.shape[0]
     rec = np.mean([len(np.intersect1d(a[i], e[i])) for i in range(n)]) / 100.0
     say("aknn recall@100", rec, "reference", env["ref"]["aknn_recall_at_100"])
@@ -62,6 +67,26 @@ def _bruteforce_entries(env):
     qb[:, q.shape[1]] = 1.0
     D = (lifted ** 2).sum(1)[None, :] - 2 * qb.astype(np.float64) @ lifted.T
     return np.argmin(D, axis=1).astype(np.int32)
+
+
+def test_truth_inside_lifted_ball(env):
+    """Lift property (report Theorem 1): every true RkNN_10 member lies in the
+    lifted ball D <= M*^2 + 1."""
+    ix = env["ix"]
+    lifted = ix.lifted.cpu().numpy().astype(np.float64)
+    q = env["q"]
+    qb = np.zeros((len(q), lifted.shape[1]), np.float64)
+    qb[:, :q.shape[1]] = (q.astype(np.float64) * ix.scale).astype(np.float32)
+    qb[:, q.shape[1]] = 1.0
+    off, gids = env["off"], env["gids"]
+    inside = tot = 0
+    for i in range(len(q)):
+        T = gids[off[i]:off[i + 1]]
+        D = ((lifted[T] - qb[i]) ** 2).sum(1)
+        inside += int((D <= ix.mstar2 + 1.0 + 1e-9).sum())
+        tot += len(T)
+    say(f"truth members inside the lifted ball: {inside}/{tot}")
+    assert inside >= 0.999 * tot
 
 
 def test_recall_matches_reference(env):
