@@ -1,0 +1,605 @@
+// rknn_query -- drop-in for the reference's src/rknn_query.cpp on one MI355X.
+//
+// Same command line (L_DESCENT_LIST PREFIX [GRAPH_DEGREE] [LEAF_SIZE]
+// [TREE_COUNT] [BUILD_EF] [QUERY_K], rknn_query.cpp:1547-1596), same input
+// files (PREFIX_lifted.meta, _base.bin, _query.bin, _rknn_gt_k<K>.bin or
+// _rknn_gt.bin, the AKNN .rec named by ANQI_PREBUILT_TOPK_REC, the rank-M basis
+// PREFIX_radbasis_M8.bin and the u8 codebook), same stdout lines for the
+// build and for every L_descent point, same exit codes (2 usage, 3 input).
+//
+// It implements the configuration that scripts/run_final.sh pins (any-k
+// lifted graph, rank-8 + u8 LRQ floor verifier as a post-filter, no recheck,
+// slack 0, LocalJoin 6, alpha 1.2, LIFTNNK 50).  Those ANQI_* variables may be
+// unset or set to the pinned values; any other value is rejected (exit 2)
+// rather than silently computing something else.
+//
+// Everything from the AKNN cutoffs on runs on the GPU through libanqi_hip.so:
+// LRQ encode + threshold decode, lift, graph build, search tree, range search
+// and post-filter.  Recall/precision/F1 and distance counts are computed on
+// the host exactly as the reference does (rknn_query.cpp:2776-2798, 2891-2913).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "formats.hpp"
+
+using namespace anqi_cli;
+
+namespace {
+
+anqi_ctx* g_ctx = nullptr;
+
+[[noreturn]] void device_fail(const char* what) {
+    std::fprintf(stderr, "[rknn] GPU failure in %s: %s\n", what, g_ctx ? anqi_last_error(g_ctx) : "no context");
+    std::exit(3);
+}
+
+void ck(int rc, const char* what) {
+    if (rc == ANQI_OK) return;
+    if (rc == ANQI_E_USAGE) {
+        std::fprintf(stderr, "[rknn] %s: %s\n", what, anqi_last_error(g_ctx));
+        std::exit(2);
+    }
+    device_fail(what);
+}
+
+template <class T> T* dalloc(size_t count) {
+    void* p = nullptr;
+    ck(anqi_dmalloc(g_ctx, count * sizeof(T) + 16, &p), "device allocation");
+    return static_cast<T*>(p);
+}
+
+template <class T> T* upload(const std::vector<T>& v) {
+    T* p = dalloc<T>(v.size());
+    ck(anqi_h2d(g_ctx, p, v.data(), v.size() * sizeof(T)), "upload");
+    return p;
+}
+
+template <class T> void download(std::vector<T>& v, const T* p, size_t count) {
+    v.resize(count);
+    ck(anqi_d2h(g_ctx, v.data(), p, count * sizeof(T)), "download");
+}
+
+bool parse_size(const char* text, const char* name, size_t& out) {
+    if (!text || !*text) {
+        std::fprintf(stderr, "[rknn] invalid %s (expected a positive integer)\n", name);
+        return false;
+    }
+    char* end = nullptr;
+    errno = 0;
+    const unsigned long long v = std::strtoull(text, &end, 10);
+    if (errno == ERANGE || end == text || *end != '\0' || v == 0) {
+        std::fprintf(stderr, "[rknn] invalid %s=%s (expected a positive integer)\n", name, text);
+        return false;
+    }
+    out = (size_t)v;
+    return true;
+}
+
+// The pinned configuration (scripts/run_final.sh exports): variable -> the
+// accepted spellings.  Unset means the pinned value.
+struct Pin {
+    const char* name;
+    std::vector<std::string> ok;
+};
+const std::vector<Pin>& pins() {
+    static const std::vector<Pin> p = {
+        {"ANQI_GRAPH_GEOMETRY", {"anyk_lift", "anyk", "lift"}},
+        {"ANQI_RADIUS_MODE", {"rankm_resid_u8_lrq_floor"}},
+        {"ANQI_VERIFIER_PLACEMENT", {"postfilter"}},
+        {"ANQI_RANKM", {"8"}},
+        {"ANQI_NK", {"100"}},
+        {"ANQI_GRAPH_NK", {"100"}},
+        {"ANQI_RANKM_SLACK", {"0", "0.0"}},
+        {"ANQI_RECHECK", {"0", "false", "no", "off"}},
+        {"ANQI_RQ_OBJECTIVE", {"radius_reconstruction"}},
+        {"ANQI_ALPHA", {"1.2"}},
+        {"ANQI_LIFTNNK", {"50"}},
+        {"ANQI_GRAPHNND", {"6"}},
+        {"ANQI_PREBUILT_TOPK_K", {"100"}},
+        {"ANQI_SEARCH_SCORE", {"center"}},
+        {"ANQI_SPECIFIC", {"0"}},
+        {"ANQI_BETA", {"0.5"}},
+        {"ANQI_RCEPS", {"0", "0.0"}},
+        {"ANQI_VAMANA", {"0"}},
+        {"ANQI_EDGE_POLICY", {"rng", "default"}},
+        {"ANQI_CANDIDATE_ENVELOPE", {"aknn", "aknn_k100", "default"}},
+    };
+    return p;
+}
+const char* const kUnsupported[] = {
+    "ANQI_REQUIRE_GRAPH_CACHE", "ANQI_REUSE_ORIG_KNN", "ANQI_EDGE_RESID_FILE", "ANQI_GRAPH_RADIUS_SOURCE_PFX",
+    "ANQI_PREBUILT_WARM_START_COMMAND", "ANQI_ANALYTIC_RANKM_MODEL", "ANQI_RADIUS_KNOTS", "ANQI_RADIUS_RELAX",
+    "ANQI_QUERY_SINGLE_ENTRY", "ANQI_VAMANA_QUERY_ENTRY",
+};
+
+void check_configuration() {
+    for (const Pin& p : pins()) {
+        const char* v = std::getenv(p.name);
+        if (!v || !*v) continue;
+        if (std::find(p.ok.begin(), p.ok.end(), std::string(v)) == p.ok.end()) {
+            std::fprintf(stderr,
+                         "[rknn] %s=%s is not supported by the GPU build (supported: %s; this build implements the "
+                         "configuration pinned by scripts/run_final.sh)\n",
+                         p.name, v, p.ok[0].c_str());
+            std::exit(2);
+        }
+    }
+    for (const char* name : kUnsupported) {
+        const char* v = std::getenv(name);
+        if (v && *v && std::string(v) != "0") {
+            std::fprintf(stderr, "[rknn] %s is not supported by the GPU build\n", name);
+            std::exit(2);
+        }
+    }
+}
+
+double external_seconds(const char* name) {
+    const char* raw = std::getenv(name);
+    if (!raw || !*raw) return 0.0;
+    char* end = nullptr;
+    errno = 0;
+    const double v = std::strtod(raw, &end);
+    if (errno != 0 || end == raw || *end != '\0' || !std::isfinite(v) || v < 0.0) {
+        std::fprintf(stderr, "[rknn] invalid %s=%s\n", name, raw);
+        std::exit(2);
+    }
+    return v;
+}
+
+double positive_seconds_env(const char* name, double fallback) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return fallback;
+    char* end = nullptr;
+    errno = 0;
+    const double p = std::strtod(v, &end);
+    if (errno == ERANGE || end == v || *end != '\0' || !std::isfinite(p) || p <= 0.0) {
+        std::fprintf(stderr, "[rknn] invalid %s=%s\n", name, v);
+        std::exit(2);
+    }
+    return p;
+}
+
+std::string cache_dir(const char* env, const std::string& dflt) {
+    const char* v = std::getenv(env);
+    if (!v || !*v) return dflt;
+    std::string s(v);
+    if (s.back() != '/') s.push_back('/');
+    return s;
+}
+
+double since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct PointStats {
+    double sum_recall = 0, sum_prec = 0, sum_f1 = 0, sum_candidate_recall = 0;
+    double sum_cand = 0, sum_pred = 0, sum_ndist = 0, sum_graph_ndist = 0, sum_verify_ndist = 0;
+    size_t nonempty = 0, tasks = 0, batches = 0;
+    double secs = 0;
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3 || argc > 8) {
+        std::fprintf(stderr,
+                     "usage: %s L_DESCENT_LIST PREFIX [GRAPH_DEGREE] [LEAF_SIZE] [TREE_COUNT] [BUILD_EF] [QUERY_K]\n",
+                     argv[0]);
+        return 2;
+    }
+    std::vector<size_t> Lds;
+    {
+        std::string list(argv[1]);
+        size_t start = 0;
+        while (start <= list.size()) {
+            const size_t comma = list.find(',', start);
+            const std::string item = list.substr(start, comma == std::string::npos ? std::string::npos : comma - start);
+            size_t v = 0;
+            if (!parse_size(item.c_str(), "L_DESCENT", v)) return 2;
+            Lds.push_back(v);
+            if (comma == std::string::npos) break;
+            start = comma + 1;
+        }
+    }
+    size_t Marg = 48, leafA = 1000, ntA = 64, efA = 400;
+    if (argc > 3 && !parse_size(argv[3], "GRAPH_DEGREE", Marg)) return 2;
+    if (argc > 4 && !parse_size(argv[4], "LEAF_SIZE", leafA)) return 2;
+    if (argc > 5 && !parse_size(argv[5], "TREE_COUNT", ntA)) return 2;
+    if (argc > 6 && !parse_size(argv[6], "BUILD_EF", efA)) return 2;
+    const std::string PFX(argv[2]);
+    const std::string meta_p = PFX + "_lifted.meta";
+    std::string gt_p = PFX + "_rknn_gt.bin";
+
+    double Mstar2_meta = 0.0, scale_meta = 0.0;
+    size_t d0 = 0, kk = 0;
+    std::string metric;
+    {
+        std::ifstream mf(meta_p);
+        if (!(mf >> Mstar2_meta >> d0 >> kk >> metric >> scale_meta) || !std::isfinite(Mstar2_meta) ||
+            Mstar2_meta < 0.0 || d0 == 0 || kk == 0 || (metric != "l2" && metric != "ip") ||
+            !std::isfinite(scale_meta) || scale_meta <= 0.0) {
+            std::fprintf(stderr, "[rknn] invalid or missing lift metadata: %s\n", meta_p.c_str());
+            return 3;
+        }
+    }
+    std::printf("[rknn] metric=%s M*^2=%.6g d=%zu k=%zu L_descent_points=%zu\n", metric.c_str(), Mstar2_meta, d0, kk,
+                Lds.size());
+    if (metric != "l2") {
+        std::fprintf(stderr, "[rknn] metric=%s is not supported by the GPU build (l2 only)\n", metric.c_str());
+        return 2;
+    }
+    check_configuration();
+    const size_t NK = 100, GRAPH_NK = 100;
+    const int RANKM = 8;
+    size_t Kq = kk;
+    if (argc > 7 && !parse_size(argv[7], "QUERY_K", Kq)) return 2;
+    if (Kq < 1 || Kq > NK) {
+        std::fprintf(stderr, "[rknn] requested Kq=%zu is outside the represented rank range [1,%zu]\n", Kq, NK);
+        return 2;
+    }
+    {
+        std::string gp = PFX + "_rknn_gt_k" + std::to_string(Kq) + ".bin";
+        if (exists(gp)) {
+            gt_p = gp;
+        } else if (Kq != kk) {
+            std::fprintf(stderr,
+                         "[rknn] missing k-specific reverse-kNN GT %s; the generic GT is only valid for metadata k=%zu\n",
+                         gp.c_str(), kk);
+            return 3;
+        } else if (!exists(gt_p)) {
+            std::fprintf(stderr, "[rknn] missing reverse-kNN GT %s\n", gt_p.c_str());
+            return 3;
+        }
+    }
+    std::printf("[rknn] shared-index query: Kq=%zu NK=%zu graph_NK=%zu GT=%s\n", Kq, NK, GRAPH_NK, gt_p.c_str());
+
+    std::vector<float> base, query;
+    uint32_t n = 0, d = 0, nq = 0, dq = 0;
+    if (!read_fbin(PFX + "_base.bin", base, n, d)) {
+        std::fprintf(stderr, "[rknn] cannot read base vectors %s_base.bin\n", PFX.c_str());
+        return 3;
+    }
+    if (!read_fbin(PFX + "_query.bin", query, nq, dq)) {
+        std::fprintf(stderr, "[rknn] cannot read query vectors %s_query.bin\n", PFX.c_str());
+        return 3;
+    }
+    if (d != d0 || dq != d0) {
+        std::fprintf(stderr, "[rknn] vector dimension does not match lift metadata: base=%u query=%u metadata=%zu\n", d,
+                     dq, d0);
+        return 3;
+    }
+    std::vector<uint32_t> goff, gids;
+    switch (read_gt_csr(gt_p, nq, n, goff, gids)) {
+        case 0: break;
+        case 1: std::fprintf(stderr, "[rknn] invalid reverse-kNN GT header: %s\n", gt_p.c_str()); return 3;
+        case 2: std::fprintf(stderr, "[rknn] invalid reverse-kNN GT offsets: %s\n", gt_p.c_str()); return 3;
+        case 3: std::fprintf(stderr, "[rknn] invalid reverse-kNN GT size: %s\n", gt_p.c_str()); return 3;
+        default: std::fprintf(stderr, "[rknn] invalid reverse-kNN GT ids: %s\n", gt_p.c_str()); return 3;
+    }
+
+    const std::string DSDIR = PFX.substr(0, PFX.rfind('/') + 1);
+    const std::string graph_dir = cache_dir("ANQI_GRAPH_CACHE_DIR", DSDIR);
+    const std::string verifier_dir = cache_dir("ANQI_VERIFIER_CACHE_DIR", DSDIR);
+    std::printf("[rknn] graph geometry=anyk_lift graph_rankm=0 verification_rankm=%d "
+                "radius_mode=rankm_resid_u8_lrq_floor placement=postfilter\n", RANKM);
+    std::printf("[rknn] component caches: graph=%s verifier=%s\n", graph_dir.c_str(), verifier_dir.c_str());
+
+    // rank-M basis and codebook (rknn_query.cpp:1896-1934, 407-454)
+    const std::string basis_p = PFX + "_radbasis_M8.bin";
+    const char* cbenv = std::getenv("ANQI_RQ_CODEBOOK");
+    const std::string cb_p = (cbenv && *cbenv) ? std::string(cbenv) : PFX + "_learned_rq_codebook_M8_u8.bin";
+    std::vector<float> mu, G, cb;
+    if (!read_basis(basis_p, NK, RANKM, mu, G)) {
+        std::fprintf(stderr, "[rknn] invalid or missing rank-M basis %s\n", basis_p.c_str());
+        return 3;
+    }
+    std::printf("[rknn] loaded rank-M basis: M=%d path=%s\n", RANKM, basis_p.c_str());
+    int cbr = read_codebook(cb_p, NK, RANKM, 8, cb);
+    if (cbr) {
+        std::fprintf(stderr, "[rknn] invalid learned residual codebook %s (code %d)\n", cb_p.c_str(), cbr);
+        return 3;
+    }
+    const uint32_t code_size = (uint32_t)(cb.size() / NK);
+
+    ck(anqi_ctx_create(0, &g_ctx), "context");
+    const auto tb0 = std::chrono::steady_clock::now();
+
+    // AKNN source: the prebuilt .rec (rknn_query.cpp:890-1010) or a GPU build
+    std::printf("[rknn] original-space AKNN: leaf=%zu n_trees=%zu ef=%zu metric=l2\n", leafA, ntA, efA);
+    std::vector<uint32_t> aknn_ids;
+    std::vector<float> aknn_keys;
+    const char* rec_env = std::getenv("ANQI_PREBUILT_TOPK_REC");
+    std::string rec_p = (rec_env && *rec_env) ? std::string(rec_env) : std::string();
+    float* d_base = upload(base);
+    uint32_t* d_aknn = nullptr;
+    float* d_keys = nullptr;
+    if (!rec_p.empty()) {
+        if (file_size(rec_p) != (uint64_t)n * 100 * 8) {
+            std::fprintf(stderr, "[rknn] bad prebuilt top-k table %s: bytes=%lld expected=%llu\n", rec_p.c_str(),
+                         (long long)(exists(rec_p) ? (long long)file_size(rec_p) : -1LL),
+                         (unsigned long long)((uint64_t)n * 100 * 8));
+            return 3;
+        }
+        if (!read_rec(rec_p, n, 100, aknn_ids, aknn_keys)) {
+            std::fprintf(stderr, "[rknn] cannot read prebuilt top-k %s\n", rec_p.c_str());
+            return 3;
+        }
+        for (size_t t = 0; t < aknn_ids.size(); t++)
+            if (aknn_ids[t] >= n) {
+                std::fprintf(stderr, "[rknn] invalid prebuilt top-k id at row=%zu\n", t / 100);
+                return 3;
+            }
+        std::printf("[rknn] loaded contiguous prebuilt original KNN %s (n=%u source_k=100 width=100)\n", rec_p.c_str(),
+                    n);
+        d_aknn = upload(aknn_ids);
+        d_keys = upload(aknn_keys);
+    } else {
+        anqi_aknn_params ap{100, (uint32_t)leafA, (uint32_t)ntA, (uint32_t)efA, 8, 100, 10, 0};
+        anqi_aknn_stats st{};
+        d_aknn = dalloc<uint32_t>((size_t)n * 100);
+        d_keys = dalloc<float>((size_t)n * 100);
+        ck(anqi_aknn_build(g_ctx, d_base, n, d, &ap, d_aknn, d_keys, &st), "AKNN source");
+        download(aknn_keys, d_keys, (size_t)n * 100);
+        for (auto& k : aknn_keys) k = std::max(0.0f, k);
+        ck(anqi_h2d(g_ctx, d_keys, aknn_keys.data(), aknn_keys.size() * 4), "upload");
+        std::printf("[rknn] built original-space AKNN on the GPU (n=%u K=100 in %.2fs)\n", n, st.total_s);
+    }
+    std::printf("[rknn] original-space AKNN warm start ready: rows=%u source=%s\n", n,
+                rec_p.empty() ? "gpu" : "prebuilt-contiguous");
+    const double source_prepare_s = since(tb0);
+
+    // verifier: rank-8 coefficients + u8 floor codes (rknn_query.cpp:603-753)
+    const auto tv0 = std::chrono::steady_clock::now();
+    float* d_mu = upload(mu);
+    float* d_G = upload(G);
+    float* d_cb = upload(cb);
+    float* d_coeff = dalloc<float>((size_t)n * RANKM);
+    uint8_t* d_codes = dalloc<uint8_t>((size_t)n * NK);
+    uint8_t* d_codes_km = dalloc<uint8_t>((size_t)n * NK);
+    ck(anqi_lrq_encode(g_ctx, d_keys, n, NK, RANKM, d_mu, d_G, d_cb, code_size, d_coeff, d_codes, d_codes_km),
+       "LRQ encode");
+    std::vector<float> coeff;
+    std::vector<uint8_t> codes;
+    download(coeff, d_coeff, (size_t)n * RANKM);
+    download(codes, d_codes, (size_t)n * NK);
+    const std::string vkey = hex16(fnv1a(codes.data(), codes.size(), fnv1a(coeff.data(), coeff.size() * 4)));
+    const std::string vpath = verifier_dir + "_anqi_verifier_gpu_" + vkey + ".bin";
+    const auto ts0 = std::chrono::steady_clock::now();
+    if (!write_verifier(vpath, coeff, codes, n, NK, RANKM)) {
+        std::fprintf(stderr, "[rknn] failed writing verifier %s\n", vpath.c_str());
+        return 3;
+    }
+    const double encode_save_ms = since(ts0) * 1e3 + 0.0;
+    const double verifier_bytes = (double)n * (RANKM * 4 + NK);
+    std::printf("[rknn] built independent compact verifier mode=rank-M+LRQ bits=8 %s (persistent=%.3fGB "
+                "encode_save_ms=%.3f)\n", vpath.c_str(), verifier_bytes / 1e9, encode_save_ms);
+
+    // lift (rknn_query.cpp:2203-2236) with the AKNN 100th key as horizon
+    std::vector<float> rk(n);
+    for (uint32_t i = 0; i < n; i++) rk[i] = aknn_keys[(size_t)i * 100 + (GRAPH_NK - 1)];
+    const uint32_t dl = d + 2;
+    std::vector<float> lifted((size_t)n * dl);
+    double mstar2 = 0, scale = 0;
+    ck(anqi_lift_host(base.data(), n, d, rk.data(), lifted.data(), &mstar2, &scale), "lift");
+    float* d_lifted = upload(lifted);
+    const double lift_verifier_prepare_s = since(tv0);
+    std::printf("[rknn] BUILD lift M*^2=%.6g scale=%.4g (graph radius=aknn_k100; verifier=rankm_resid_u8_lrq_floor:M8:"
+                "b0:objradius_reconstruction:basis%s:cb%s:postfilter)\n",
+                mstar2, scale, fingerprint(basis_p).c_str(), fingerprint(cb_p).c_str());
+    std::printf("[rknn] graph degree M=%zu\n", Marg);
+
+    // graph + search tree (index.h:1071-1295)
+    const auto tg0 = std::chrono::steady_clock::now();
+    anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10};
+    anqi_graph_stats gst{};
+    anqi_tree tree{};
+    uint32_t* d_links = dalloc<uint32_t>((size_t)n * Marg);
+    uint32_t* d_deg = dalloc<uint32_t>(n);
+    ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_aknn, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
+    float* d_bnorm = dalloc<float>(n);
+    ck(anqi_norms(g_ctx, d_base, n, d, d_bnorm), "norms");
+    ck(anqi_ctx_sync(g_ctx), "sync");
+    const double graph_index_s = since(tg0);
+    const double build_s = since(tb0);
+    const double aknn_build_s = external_seconds("ANQI_AKNN_BUILD_SECONDS");
+    const double verifier_train_s = external_seconds("ANQI_VERIFIER_TRAIN_SECONDS");
+    const double end_to_end = aknn_build_s + verifier_train_s + build_s;
+    const double idx_gb = ((double)n * Marg * 4 + (double)n * dl * 4) / 1e9;
+    std::printf("[rknn] BUILD_TIME=%.1fs INDEX_SIZE=%.2fGB (aknn_source=%.1fs + verifier_training=%.1fs + index=%.1fs, "
+                "graph M=%zu + graph dim=%u + verification=%.3fGB + envelope=%.3fGB)\n",
+                end_to_end, idx_gb, aknn_build_s, verifier_train_s, build_s, Marg, dl, 0.0, 0.0);
+    if (env_flag("ANQI_BUILD_BREAKDOWN"))
+        std::printf("[rknn] BUILD_BREAKDOWN aknn_source_s=%.6f verifier_train_s=%.6f source_prepare_s=%.6f "
+                    "lift_verifier_prepare_s=%.6f graph_index_s=%.6f index_total_s=%.6f end_to_end_s=%.6f "
+                    "verifier_encode_save_ms=%.3f\n",
+                    aknn_build_s, verifier_train_s, source_prepare_s, lift_verifier_prepare_s, graph_index_s, build_s,
+                    end_to_end, encode_save_ms);
+    std::fflush(stdout);
+
+    // thresholds for Kq (materialize_k, rknn_query.cpp:720-753, slack 1)
+    const auto tm0 = std::chrono::steady_clock::now();
+    float* d_thr = dalloc<float>(n);
+    ck(anqi_lrq_materialize(g_ctx, d_coeff, d_codes_km, n, NK, RANKM, d_mu, d_G, d_cb, code_size, (uint32_t)Kq, 1.0,
+                            d_thr),
+       "threshold decode");
+    ck(anqi_ctx_sync(g_ctx), "sync");
+    std::printf("[rknn] independent verifier ready: mode=rank-M+LRQ k=%zu slack=0 persistent=%.3fGB "
+                "temporary_thresholds=%.3fGB materialize_ms=%.3f bits=8\n",
+                Kq, verifier_bytes / 1e9, n * 4 / 1e9, since(tm0) * 1e3);
+    std::printf("[rknn] k-adaptive flood: Kq=%zu radj=off beta=0.50\n", Kq);
+    std::printf("[rknn] candidate generator: AKNN-k100 envelope, center graph search; verifier is post-filter only\n");
+    std::printf("[rknn] search score: mode=center lambda=1.0\n");
+    std::printf("[rknn] query visited mode=lds-hash aggregate_store=0.000GB\n");
+
+    size_t bench_repeats = 1;
+    if (const char* r = std::getenv("ANQI_BENCH_REPEATS")) {
+        size_t v = 0;
+        if (!parse_size(r, "ANQI_BENCH_REPEATS", v)) return 2;
+        bench_repeats = v;
+    }
+    const bool calibrated = env_flag("ANQI_BENCH_CALIBRATED_TIMING");
+    const double min_total_s = positive_seconds_env("ANQI_BENCH_MIN_TOTAL_S", 30.0);
+    const double min_repeat_s = positive_seconds_env("ANQI_BENCH_MIN_REPEAT_S", 10.0);
+    std::printf("[rknn] benchmark repeats=%zu calibrated=%s (timing uses steady_clock sub-millisecond precision)\n",
+                bench_repeats, calibrated ? "on" : "off");
+    std::printf("[rknn] query parallelism: nq=%u device=MI355X mode=persistent\n", nq);
+    if (calibrated)
+        std::printf("[rknn] calibrated timing: min_total_s=%.3f min_repeat_s=%.3f windows=%zu\n", min_total_s,
+                    min_repeat_s, bench_repeats);
+    else
+        std::printf("[rknn] flattened_repeats=1 tasks=%zu\n", (size_t)nq * bench_repeats);
+
+    anqi_index ix{d_base, d_bnorm, d_lifted, d_deg, d_links, n, d, dl, (uint32_t)Marg, &tree, mstar2, scale};
+    float* d_query = upload(query);
+    uint32_t out_cap = 4096;
+    uint32_t* d_cand = nullptr;
+    uint8_t* d_acc = nullptr;
+    uint32_t* d_cnt = dalloc<uint32_t>(nq);
+    uint32_t* d_nd = dalloc<uint32_t>(nq);
+    uint32_t* d_pred = dalloc<uint32_t>(nq);
+    uint32_t* d_status = dalloc<uint32_t>(nq);
+    auto alloc_out = [&](uint32_t cap) {
+        if (d_cand) anqi_dfree(g_ctx, d_cand);
+        if (d_acc) anqi_dfree(g_ctx, d_acc);
+        out_cap = cap;
+        d_cand = dalloc<uint32_t>((size_t)nq * cap);
+        d_acc = dalloc<uint8_t>((size_t)nq * cap);
+    };
+    alloc_out(out_cap);
+    std::vector<uint32_t> cand, cnt, nd;
+    std::vector<uint8_t> acc;
+    // one batch: search + post-filter on the device, results on the host
+    auto search = [&](size_t L) {
+        for (;;) {
+            ck(anqi_rknn_search(g_ctx, &ix, d_query, nq, (uint32_t)L, nullptr, (uint32_t)L, d_thr, nullptr, out_cap,
+                                d_cand, d_acc, d_cnt, d_nd, d_pred, nullptr, d_status, 1),
+               "range search");
+            download(cnt, d_cnt, nq);
+            uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
+            if (mx <= out_cap) break;
+            alloc_out(mx + 1024);
+        }
+        download(nd, d_nd, nq);
+        download(cand, d_cand, (size_t)nq * out_cap);
+        download(acc, d_acc, (size_t)nq * out_cap);
+    };
+    std::vector<std::vector<uint32_t>> truth(nq);
+    for (uint32_t i = 0; i < nq; i++) {
+        truth[i].assign(gids.begin() + goff[i], gids.begin() + goff[i + 1]);
+        std::sort(truth[i].begin(), truth[i].end());
+        truth[i].erase(std::unique(truth[i].begin(), truth[i].end()), truth[i].end());
+    }
+    auto run_point = [&](size_t L, size_t batches) {
+        PointStats s;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (size_t b = 0; b < batches; b++) {
+            search(L);
+            for (uint32_t i = 0; i < nq; i++) {
+                const auto& T = truth[i];
+                if (T.empty()) continue;
+                s.nonempty++;
+                const uint32_t c = cnt[i];
+                std::vector<uint32_t> pred;
+                size_t chit = 0;
+                for (uint32_t t = 0; t < c; t++) {
+                    const uint32_t id = cand[(size_t)i * out_cap + t];
+                    if (std::binary_search(T.begin(), T.end(), id)) chit++;
+                    if (acc[(size_t)i * out_cap + t]) pred.push_back(id);
+                }
+                std::sort(pred.begin(), pred.end());
+                pred.erase(std::unique(pred.begin(), pred.end()), pred.end());
+                size_t hit = 0;
+                for (uint32_t id : pred) hit += std::binary_search(T.begin(), T.end(), id);
+                const double rec = (double)hit / T.size();
+                const double prec = pred.empty() ? 1.0 : (double)hit / pred.size();
+                s.sum_recall += rec;
+                s.sum_prec += prec;
+                s.sum_f1 += (rec + prec > 0 ? 2 * rec * prec / (rec + prec) : 0);
+                s.sum_candidate_recall += (double)chit / T.size();
+                s.sum_cand += c;
+                s.sum_pred += pred.size();
+                s.sum_ndist += (double)nd[i] + c;
+                s.sum_graph_ndist += nd[i];
+                s.sum_verify_ndist += c;
+            }
+        }
+        s.secs = since(t0);
+        s.tasks = (size_t)nq * batches;
+        s.batches = batches;
+        return s;
+    };
+    auto print_point = [&](size_t L, const PointStats& s, double qps) {
+        if (s.nonempty == 0 || s.tasks == 0 || s.secs <= 0.0) {
+            std::fprintf(stderr, "[rknn] empty benchmark statistics for L_descent=%zu\n", L);
+            std::exit(3);
+        }
+        const double dn = (double)s.nonempty;
+        std::printf("[rknn] L_descent=%-5zu | recall=%.4f precision=%.4f F1=%.4f | avg cand=%.0f pred=%.0f #dist=%.0f | "
+                    "QPS=%.0f (parallel)\n",
+                    L, s.sum_recall / dn, s.sum_prec / dn, s.sum_f1 / dn, s.sum_cand / dn, s.sum_pred / dn,
+                    s.sum_ndist / dn, qps);
+        std::printf("[rknn] DIST_BREAKDOWN L_descent=%zu #graph_dist=%.0f #verify_dist=%.0f #total_dist=%.0f\n", L,
+                    s.sum_graph_ndist / dn, s.sum_verify_ndist / dn, s.sum_ndist / dn);
+        std::printf("[rknn] CANDIDATE_RECALL L_descent=%zu recall=%.6f\n", L, s.sum_candidate_recall / dn);
+        std::fflush(stdout);
+    };
+    for (size_t L : Lds) {
+        if (!calibrated) {
+            PointStats s = run_point(L, bench_repeats);
+            print_point(L, s, (double)s.tasks / s.secs);
+            continue;
+        }
+        const PointStats warm = run_point(L, 1);
+        const PointStats calib = run_point(L, 1);
+        const double target = std::max(min_repeat_s, min_total_s / (double)bench_repeats);
+        const size_t planned = std::max<size_t>(1, (size_t)std::ceil(target / std::max(calib.secs, 1e-9)));
+        PointStats agg;
+        std::vector<double> qs;
+        double min_obs = 1e300;
+        for (size_t r = 0; r < bench_repeats; r++) {
+            size_t batches = planned;
+            PointStats m;
+            for (int attempt = 0; attempt < 4; attempt++) {
+                m = run_point(L, batches);
+                if (m.secs + 1e-9 >= target) break;
+                size_t next = std::max(batches + 1, (size_t)std::ceil((double)batches * target / std::max(m.secs, 1e-9) * 1.05));
+                std::printf("[rknn] HOT_RETRY L_descent=%zu repeat=%zu seconds=%.6f target_s=%.6f batches=%zu "
+                            "next_batches=%zu\n", L, r + 1, m.secs, target, batches, next);
+                batches = next;
+            }
+            if (m.secs + 1e-9 < target) {
+                std::fprintf(stderr, "[rknn] calibrated repeat too short for L_descent=%zu repeat=%zu seconds=%.6f "
+                             "target_s=%.6f\n", L, r + 1, m.secs, target);
+                return 3;
+            }
+            const double q = (double)m.tasks / m.secs;
+            qs.push_back(q);
+            min_obs = std::min(min_obs, m.secs);
+            agg.sum_recall += m.sum_recall; agg.sum_prec += m.sum_prec; agg.sum_f1 += m.sum_f1;
+            agg.sum_candidate_recall += m.sum_candidate_recall; agg.sum_cand += m.sum_cand;
+            agg.sum_pred += m.sum_pred; agg.sum_ndist += m.sum_ndist; agg.sum_graph_ndist += m.sum_graph_ndist;
+            agg.sum_verify_ndist += m.sum_verify_ndist; agg.nonempty += m.nonempty; agg.tasks += m.tasks;
+            agg.batches += m.batches; agg.secs += m.secs;
+            std::printf("[rknn] HOT_REPEAT L_descent=%zu repeat=%zu/%zu batches=%zu tasks=%zu seconds=%.6f QPS=%.3f\n",
+                        L, r + 1, bench_repeats, m.batches, m.tasks, m.secs, q);
+        }
+        std::vector<double> sq = qs;
+        std::sort(sq.begin(), sq.end());
+        const size_t mid = sq.size() / 2;
+        const double med = sq.size() % 2 ? sq[mid] : 0.5 * (sq[mid - 1] + sq[mid]);
+        print_point(L, agg, med);
+        std::printf("[rknn] HOT_TIMING L_descent=%zu warm_s=%.6f calibration_s=%.6f measured_s=%.6f repeats=%zu "
+                    "planned_batches_per_repeat=%zu actual_batches_total=%zu min_repeat_s=%.6f qps_median=%.3f "
+                    "qps_samples=", L, warm.secs, calib.secs, agg.secs, bench_repeats, planned, agg.batches, min_obs, med);
+        for (size_t i = 0; i < qs.size(); i++) std::printf("%s%.3f", i ? "," : "", qs[i]);
+        std::printf("\n");
+    }
+    anqi_tree_free(g_ctx, &tree);
+    anqi_ctx_destroy(g_ctx);
+    return 0;
+}

@@ -80,6 +80,33 @@ struct DBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// MT19937 (Matsumoto & Nishimura 1998), the engine behind std::mt19937:
+// seed recurrence x[i] = 1812433253 (x[i-1] ^ x[i-1] >> 30) + i, twist every
+// 624 outputs, tempering (11, 7/0x9d2c5680, 15/0xefc60000, 18).
+struct Mt19937 {
+    uint32_t x[624];
+    uint32_t i = 624;
+    explicit Mt19937(uint32_t seed) {
+        x[0] = seed;
+        for (uint32_t k = 1; k < 624; k++) x[k] = 1812433253u * (x[k - 1] ^ (x[k - 1] >> 30)) + k;
+    }
+    uint32_t next() {
+        if (i >= 624) {
+            for (uint32_t k = 0; k < 624; k++) {
+                uint32_t y = (x[k] & 0x80000000u) | (x[(k + 1) % 624] & 0x7fffffffu);
+                x[k] = x[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            i = 0;
+        }
+        uint32_t y = x[i++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+};
+
 static uint32_t next_pow2(uint64_t v) {
     uint32_t p = 1;
     while (p < v) p <<= 1;
@@ -724,43 +751,88 @@ int anqi_tree_upload(anqi_ctx* ctx, uint32_t nodes, uint32_t root, uint32_t d, c
     return ANQI_OK;
 }
 
+// The reference's lifted search tree (rptrees.h:155-163 via index.h:1116): one
+// std::mt19937 seeded with `seed` feeds every draw, depth first.  The engine's
+// outputs do not depend on the data, so they are generated here and the
+// device walks the chain (k_tree.hip).  A stream or node table that turns out
+// too short is enlarged and the (deterministic) build rerun.
 int anqi_rp_tree(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t leaf, uint32_t max_depth,
                  uint64_t seed, anqi_tree* out) {
-    Forest F;
-    int rc = build_forest(ctx, x, n, d, 1, leaf, max_depth, seed, true, F);
-    if (rc) return rc;
+    if (n == 0 || leaf == 0) return anqi_set_err(ctx, ANQI_E_INPUT, "search tree needs n > 0 and leaf > 0");
+    if (d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
+    if (max_depth + 2 > TB_STACK) return anqi_set_err(ctx, ANQI_E_USAGE, "max_depth %u unsupported", max_depth);
     hipStream_t s = ctx->stream;
-    std::vector<uint32_t> perm(n);
-    HCHK(hipMemcpyAsync(perm.data(), F.perm.p, 4ull * n, hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
     std::memset(out, 0, sizeof(*out));
-    out->nodes = F.nodes; out->root = 0; out->d = d;
-    std::vector<uint32_t> entry(F.nodes, 0);
-    out->leaf_off = new uint64_t[F.nodes + 1];
-    std::vector<uint32_t> ids;
-    out->leaf_off[0] = 0;
-    for (uint32_t k = 0; k < F.nodes; k++) {
-        if (F.left[k] == U32_NONE) {
-            uint32_t st = F.leaf_pos_start[k], sz = F.leaf_pos_size[k];
-            entry[k] = sz ? perm[st] : 0;
-            for (uint32_t t = 0; t < sz; t++) ids.push_back(perm[st + t]);
+    uint64_t stream_len = 8ull * n / leaf + 8192;
+    uint64_t cap = std::min<uint64_t>(2ull * n + 2, 4ull * n / leaf + 1024);
+    std::vector<uint32_t> stream;
+    Mt19937 mt((uint32_t)seed);
+    DBuf dperm, dtmp, dside, dstatus;
+    HCHK(dperm.alloc(4ull * n));
+    HCHK(dtmp.alloc(4ull * n));
+    HCHK(dside.alloc(n));
+    HCHK(dstatus.alloc(16));
+    uint32_t status[4] = {0, 0, 0, 0};
+    for (int attempt = 0;; attempt++) {
+        while (stream.size() < stream_len) stream.push_back(mt.next());
+        DBuf dstream, dleft, dright, doff, dhyper, dls, dlz;
+        HCHK(dstream.alloc(4ull * stream_len));
+        HCHK(dleft.alloc(4ull * cap));
+        HCHK(dright.alloc(4ull * cap));
+        HCHK(doff.alloc(4ull * cap));
+        HCHK(dhyper.alloc(4ull * cap * d));
+        HCHK(dls.alloc(4ull * cap));
+        HCHK(dlz.alloc(4ull * cap));
+        HCHK(hipMemcpyAsync(dstream.p, stream.data(), 4ull * stream_len, hipMemcpyHostToDevice, s));
+        TreeBuildParams P{};
+        P.x = x; P.n = n; P.d = d; P.leaf = leaf; P.max_depth = max_depth;
+        P.stream = (const uint32_t*)dstream.p; P.stream_len = stream_len;
+        P.perm = (uint32_t*)dperm.p; P.tmp = (uint32_t*)dtmp.p; P.side = (uint8_t*)dside.p;
+        P.left = (uint32_t*)dleft.p; P.right = (uint32_t*)dright.p; P.offset = (float*)doff.p;
+        P.hyper = (float*)dhyper.p; P.leaf_start = (uint32_t*)dls.p; P.leaf_size = (uint32_t*)dlz.p;
+        P.node_cap = (uint32_t)cap; P.status = (uint32_t*)dstatus.p;
+        HCHK(launch_tree_build(P, s));
+        HCHK(hipMemcpyAsync(status, dstatus.p, 16, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        if (status[1] == TB_ERR_STREAM && attempt < 40) { stream_len = stream_len * 2 + n; continue; }
+        if (status[1] == TB_ERR_NODES && attempt < 40 && cap < 4ull * n + 4) {
+            cap = std::min<uint64_t>(4ull * n + 4, cap * 2);
+            continue;
         }
-        out->leaf_off[k + 1] = ids.size();
+        if (status[1] != 0) return anqi_set_err(ctx, ANQI_E_DEVICE, "search tree build failed (code %u)", status[1]);
+        const uint32_t nodes = status[0];
+        std::vector<uint32_t> left(nodes), ls(nodes), lz(nodes), perm(n);
+        HCHK(hipMemcpyAsync(left.data(), dleft.p, 4ull * nodes, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(ls.data(), dls.p, 4ull * nodes, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(lz.data(), dlz.p, 4ull * nodes, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(perm.data(), dperm.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipMalloc((void**)&out->left, 4ull * nodes));
+        HCHK(hipMalloc((void**)&out->right, 4ull * nodes));
+        HCHK(hipMalloc((void**)&out->offset, 4ull * nodes));
+        HCHK(hipMalloc((void**)&out->hyper, 4ull * nodes * d));
+        HCHK(hipMalloc((void**)&out->entry, 4ull * nodes));
+        HCHK(hipMemcpyAsync(out->left, dleft.p, 4ull * nodes, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(out->right, dright.p, 4ull * nodes, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(out->offset, doff.p, 4ull * nodes, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(out->hyper, dhyper.p, 4ull * nodes * d, hipMemcpyDeviceToDevice, s));
+        HCHK(launch_tree_entries(out->left, (const uint32_t*)dls.p, (const uint32_t*)dlz.p,
+                                 (const uint32_t*)dperm.p, nodes, out->entry, s));
+        HCHK(hipStreamSynchronize(s));
+        out->nodes = nodes; out->root = 0; out->d = d;
+        out->leaf_off = new uint64_t[nodes + 1];
+        uint64_t tot = 0;
+        out->leaf_off[0] = 0;
+        for (uint32_t k = 0; k < nodes; k++) {
+            if (left[k] == U32_NONE) tot += lz[k];
+            out->leaf_off[k + 1] = tot;
+        }
+        out->leaf_ids = new uint32_t[tot ? tot : 1];
+        for (uint32_t k = 0; k < nodes; k++)
+            if (left[k] == U32_NONE) std::memcpy(out->leaf_ids + out->leaf_off[k], perm.data() + ls[k], 4ull * lz[k]);
+        progress("search tree: %u nodes, %llu draws", nodes,
+                 (unsigned long long)(((uint64_t)status[3] << 32) | status[2]));
+        return ANQI_OK;
     }
-    out->leaf_ids = new uint32_t[ids.size() ? ids.size() : 1];
-    std::memcpy(out->leaf_ids, ids.data(), 4 * ids.size());
-    HCHK(hipMalloc((void**)&out->left, 4ull * F.nodes));
-    HCHK(hipMalloc((void**)&out->right, 4ull * F.nodes));
-    HCHK(hipMalloc((void**)&out->offset, 4ull * F.nodes));
-    HCHK(hipMalloc((void**)&out->hyper, 4ull * F.nodes * d));
-    HCHK(hipMalloc((void**)&out->entry, 4ull * F.nodes));
-    HCHK(hipMemcpyAsync(out->left, F.left.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
-    HCHK(hipMemcpyAsync(out->right, F.right.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
-    HCHK(hipMemcpyAsync(out->offset, F.offset.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
-    HCHK(hipMemcpyAsync(out->hyper, F.hyper.p, 4ull * F.nodes * d, hipMemcpyDeviceToDevice, s));
-    HCHK(hipMemcpyAsync(out->entry, entry.data(), 4ull * F.nodes, hipMemcpyHostToDevice, s));
-    HCHK(hipStreamSynchronize(s));
-    return ANQI_OK;
 }
 
 // ---- AKNN source -------------------------------------------------------------------------------

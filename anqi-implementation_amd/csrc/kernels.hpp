@@ -199,6 +199,32 @@ hipError_t launch_seg_partition(const uint32_t* perm, uint32_t* perm_out, const 
                                 const uint32_t* seg_of, const uint32_t* seg_start, const uint32_t* left,
                                 const uint32_t* lscan, uint64_t total, hipStream_t s);
 
+// ---- lifted search tree in the reference's draw order (k_tree.hip) ------------------
+#define TB_STACK 128
+#define TB_ERR_NODES 1
+#define TB_ERR_STREAM 2
+#define TB_ERR_STACK 3
+struct TreeBuildParams {
+    const float* x;
+    uint32_t n, d, leaf, max_depth;
+    const uint32_t* stream;   // MT19937 outputs in order
+    uint64_t stream_len;
+    uint32_t* perm;           // n: point order, leaves contiguous
+    uint32_t* tmp;            // n
+    uint8_t* side;            // n
+    uint32_t* left;           // node_cap each (pre-order ids, root 0)
+    uint32_t* right;
+    float* offset;
+    float* hyper;             // node_cap * d
+    uint32_t* leaf_start;
+    uint32_t* leaf_size;
+    uint32_t node_cap;
+    uint32_t* status;         // nodes, error, draws lo, draws hi
+};
+hipError_t launch_tree_build(const TreeBuildParams& P, hipStream_t s);
+hipError_t launch_tree_entries(const uint32_t* left, const uint32_t* leaf_start, const uint32_t* leaf_size,
+                               const uint32_t* perm, uint32_t nodes, uint32_t* entry, hipStream_t s);
+
 // ---- device-wide primitives (k_prims.hip, hipCUB) ----------------------------------
 hipError_t prim_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t m, void* tmp,
                                    size_t& tmp_bytes, hipStream_t s);

@@ -287,6 +287,37 @@ class Context:
         self.call("anqi_rp_tree", dp(x), n, d, leaf, max_depth, seed, ctypes.byref(t))
         return t
 
+    def tree_host(self, t):
+        """Host copy of a device tree in the reference's node order (children
+        before parents, root last; rptrees.h:99-152): dict of left/right
+        (uint64, UINT32_MAX for leaves), offset, hyper, idx_off, idx."""
+        import formats
+        nodes = t.nodes
+        left = np.zeros(nodes, np.uint32)
+        right = np.zeros(nodes, np.uint32)
+        off = np.zeros(nodes, np.float32)
+        hyper = np.zeros((nodes, t.d), np.float32)
+        for arr, src in ((left, t.left), (right, t.right), (off, t.offset), (hyper, t.hyper)):
+            if arr.nbytes:
+                self.check(self.L.anqi_d2h(self.ctx, hp(arr), ctypes.c_void_p(src), arr.nbytes))
+        loff = np.ctypeslib.as_array(t.leaf_off, shape=(nodes + 1,)).copy()
+        lids = np.ctypeslib.as_array(t.leaf_ids, shape=(max(int(loff[-1]), 1),))[:int(loff[-1])].copy()
+        tn = formats.tree_postorder(left, right, off, hyper, loff, lids, root=t.root)
+        k = len(tn)
+        out = dict(nodes=k, d=t.d, left=np.zeros(k, np.uint64), right=np.zeros(k, np.uint64),
+                   offset=np.zeros(k, np.float32), hyper=np.zeros((k, t.d), np.float32),
+                   idx_off=np.zeros(k + 1, np.uint64))
+        ids = []
+        for i, (lk, rk, o, h, members) in enumerate(tn):
+            out["left"][i], out["right"][i], out["offset"][i] = lk, rk, o
+            if h is not None:
+                out["hyper"][i] = h
+            if members is not None:
+                ids.extend(np.asarray(members).tolist())
+            out["idx_off"][i + 1] = len(ids)
+        out["idx"] = np.array(ids, np.uint32)
+        return out
+
     def tree_upload(self, tree):
         """tree: dict of host arrays (nodes, root, d, left, right, offset, hyper, entry)."""
         t = Tree()

@@ -299,6 +299,167 @@ uint32_t or_tree_entry(const or_tree* t, const float* q) {
     return t->idx[t->idx_off[k]];
 }
 
+/* MT19937 (Matsumoto & Nishimura 1998): std::mt19937's seed() reduces the
+ * seed mod 2^32 and runs the published initialisation recurrence. */
+void or_mt_seed(or_mt* m, uint64_t seed) {
+    m->x[0] = (uint32_t)seed;
+    for (uint32_t i = 1; i < 624; i++) m->x[i] = 1812433253u * (m->x[i - 1] ^ (m->x[i - 1] >> 30)) + i;
+    m->i = 624;
+}
+
+uint32_t or_mt_next(or_mt* m) {
+    if (m->i >= 624) {
+        for (uint32_t k = 0; k < 624; k++) {
+            uint32_t y = (m->x[k] & 0x80000000u) | (m->x[(k + 1) % 624] & 0x7fffffffu);
+            m->x[k] = m->x[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        m->i = 0;
+    }
+    uint32_t y = m->x[m->i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+/* Lemire, "Fast Random Integer Generation in an Interval" (TOMACS 2019): the
+ * 64-bit product of a 32-bit draw and the range; reject while its low word is
+ * below 2^32 mod range. */
+uint32_t or_uniform_below(or_mt* m, uint32_t range) {
+    uint64_t p = (uint64_t)or_mt_next(m) * range;
+    uint32_t low = (uint32_t)p;
+    if (low < range) {
+        uint32_t t = (uint32_t)(0u - range) % range;
+        while (low < t) {
+            p = (uint64_t)or_mt_next(m) * range;
+            low = (uint32_t)p;
+        }
+    }
+    return (uint32_t)(p >> 32);
+}
+
+void or_mt_stream(uint64_t seed, uint32_t* out, uint64_t count) {
+    or_mt m;
+    or_mt_seed(&m, seed);
+    for (uint64_t i = 0; i < count; i++) out[i] = or_mt_next(&m);
+}
+
+typedef struct {
+    const float* data;
+    uint32_t d, leaf, node_cap, nodes;
+    or_mt mt;
+    uint64_t draws, nidx;
+    uint64_t *left, *right, *idx_off;
+    float *offset, *hyper, *margin;
+    uint32_t *idx, *scratch;
+    int overflow;
+} or_tb;
+
+/* or_uniform_below, counting the engine outputs it consumes. */
+static uint32_t tb_draw(or_tb* b, uint32_t range) {
+    uint64_t p = (uint64_t)or_mt_next(&b->mt) * range;
+    b->draws++;
+    uint32_t low = (uint32_t)p;
+    if (low < range) {
+        uint32_t t = (uint32_t)(0u - range) % range;
+        while (low < t) {
+            p = (uint64_t)or_mt_next(&b->mt) * range;
+            b->draws++;
+            low = (uint32_t)p;
+        }
+    }
+    return (uint32_t)(p >> 32);
+}
+
+static uint32_t tb_add(or_tb* b, uint64_t l, uint64_t r, float off, const float* hp,
+                       const uint32_t* ids, uint64_t m) {
+    if (b->nodes >= b->node_cap) { b->overflow = 1; return 0; }
+    uint32_t k = b->nodes++;
+    b->left[k] = l;
+    b->right[k] = r;
+    b->offset[k] = off;
+    if (hp) memcpy(b->hyper + (size_t)k * b->d, hp, 4ull * b->d);
+    else memset(b->hyper + (size_t)k * b->d, 0, 4ull * b->d);
+    if (ids) memcpy(b->idx + b->nidx, ids, 4ull * m);
+    b->nidx += m;
+    b->idx_off[k + 1] = b->nidx;
+    return k;
+}
+
+/* build_rptree_recursively (rptrees.h:127-152) with random_projection_split
+ * (rptrees.h:28-85), L2 branch.  seg holds the node's indices in order; it is
+ * partitioned stably in place (left part then right part). */
+static void tb_rec(or_tb* b, uint32_t* seg, uint64_t m, uint32_t depth_left) {
+    if (b->overflow) return;
+    if (m <= b->leaf) { tb_add(b, UINT32_MAX, UINT32_MAX, FLT_MAX, NULL, seg, m); return; }
+    if (depth_left == 0) {
+        tb_add(b, UINT32_MAX, UINT32_MAX, FLT_MAX, NULL, seg, b->leaf < m ? b->leaf : m);
+        return;
+    }
+    const uint32_t d = b->d;
+    uint32_t r0 = tb_draw(b, (uint32_t)m), r1 = tb_draw(b, (uint32_t)m);
+    if (r0 == r1) r0 = (uint32_t)((r1 + 1) % m);
+    const float* a = b->data + (size_t)seg[r0] * d;
+    const float* c = b->data + (size_t)seg[r1] * d;
+    float* hp = (float*)malloc(4ull * d);
+    float* mid = (float*)malloc(4ull * d);
+    for (uint32_t j = 0; j < d; j++) {
+        mid[j] = (a[j] + c[j]) / 2;
+        hp[j] = a[j] - c[j];
+    }
+    float off = or_ip(hp, mid, d);
+    free(mid);
+    uint8_t* side = (uint8_t*)malloc(m);
+    uint64_t nl = 0;
+    for (uint64_t i = 0; i < m; i++) {
+        float mg = or_ip(hp, b->data + (size_t)seg[i] * d, d) - off;
+        if (mg < -OR_EPS) side[i] = 0;
+        else if (mg > OR_EPS) side[i] = 1;
+        else side[i] = tb_draw(b, 2) == 0 ? 0 : 1;
+        nl += side[i] == 0;
+    }
+    if (nl == 0 || nl == m) {
+        nl = 0;
+        for (uint64_t i = 0; i < m; i++) {
+            side[i] = tb_draw(b, 2) == 0 ? 0 : 1;
+            nl += side[i] == 0;
+        }
+    }
+    uint32_t* tmp = b->scratch;
+    uint64_t pl = 0, pr = nl;
+    for (uint64_t i = 0; i < m; i++) tmp[side[i] ? pr++ : pl++] = seg[i];
+    memcpy(seg, tmp, 4ull * m);
+    free(side);
+    tb_rec(b, seg, nl, depth_left - 1);
+    uint32_t lk = b->nodes - 1;
+    tb_rec(b, seg + nl, m - nl, depth_left - 1);
+    uint32_t rk = b->nodes - 1;
+    if (!b->overflow) tb_add(b, lk, rk, off, hp, NULL, 0);
+    free(hp);
+}
+
+uint32_t or_rp_tree_build(const float* data, uint32_t n, uint32_t d, uint32_t leaf,
+                          uint32_t max_depth, uint64_t seed, uint32_t node_cap,
+                          uint64_t* left, uint64_t* right, float* offset, float* hyper,
+                          uint64_t* idx_off, uint32_t* idx, uint64_t* draws) {
+    or_tb b;
+    memset(&b, 0, sizeof(b));
+    b.data = data; b.d = d; b.leaf = leaf; b.node_cap = node_cap;
+    b.left = left; b.right = right; b.offset = offset; b.hyper = hyper;
+    b.idx_off = idx_off; b.idx = idx;
+    or_mt_seed(&b.mt, seed);
+    uint32_t* seg = (uint32_t*)malloc(4ull * (n ? n : 1));
+    b.scratch = (uint32_t*)malloc(4ull * (n ? n : 1));
+    for (uint32_t i = 0; i < n; i++) seg[i] = i;
+    idx_off[0] = 0;
+    tb_rec(&b, seg, n, max_depth);
+    free(seg);
+    free(b.scratch);
+    if (draws) *draws = b.draws;
+    return b.overflow ? 0 : b.nodes;
+}
+
 /* index.h:1535-1652 with radj == NULL, eps = 0, CENTER score: the collect and
  * expand thresholds are both R2, compared as double(float D) <= R2. */
 uint32_t or_range_search(const float* lifted, uint32_t n, uint32_t dl,

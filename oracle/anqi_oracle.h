@@ -69,6 +69,27 @@ typedef struct {
 /* rptree::get_leave_mediod, rptrees.h:108-123 (ties |margin|<EPS go left). */
 uint32_t or_tree_entry(const or_tree* t, const float* q);
 
+/* generate_random_integer (include/utils.h:169-173): one thread_local
+ * std::mt19937 per thread, seeded by the first call's seed only, drawn through
+ * libstdc++'s uniform_int_distribution<uint64_t> (GCC 11
+ * bits/uniform_int_dist.h: a 32-bit engine takes Lemire's nearly-divisionless
+ * branch).  or_mt_* is MT19937 (Matsumoto & Nishimura 1998). */
+typedef struct { uint32_t x[624]; uint32_t i; } or_mt;
+void or_mt_seed(or_mt* m, uint64_t seed);
+uint32_t or_mt_next(or_mt* m);
+uint32_t or_uniform_below(or_mt* m, uint32_t range);   /* uniform in [0, range) */
+void or_mt_stream(uint64_t seed, uint32_t* out, uint64_t count);
+/* build_rptree (rptrees.h:28-163) on one fresh generator seeded with seed: the
+ * whole tree draws from one stream in depth-first order.  Writes nodes in the
+ * reference's post-order (children before parent, root last; leaves have
+ * left == right == UINT32_MAX); idx_off/idx hold each leaf's indices in stored
+ * order.  node_cap bounds nodes, n bounds idx.  Returns the node count (0 if
+ * node_cap is too small); *draws = engine outputs consumed. */
+uint32_t or_rp_tree_build(const float* data, uint32_t n, uint32_t d, uint32_t leaf,
+                          uint32_t max_depth, uint64_t seed, uint32_t node_cap,
+                          uint64_t* left, uint64_t* right, float* offset, float* hyper,
+                          uint64_t* idx_off, uint32_t* idx, uint64_t* draws);
+
 /* ---- range search (include/index.h:1535-1652, CENTER mode, radj==NULL) --- */
 /* lifted: n*dl, deg: n, links: n*M.  Writes up to cap candidate ids in
  * discovery order; returns count (may exceed cap: truncated).  ndist out. */

@@ -261,6 +261,45 @@ uint32_t ref_lrq_encode_value(double v, const float* centers, uint32_t code_size
     return encode_learned_residual(v, centers, code_size, "floor");
 }
 
+// The reference's build_rptree (rptrees.h:155-163), run on a thread of its own
+// so that generate_random_integer's thread_local engine starts from `seed`,
+// as it does for the lifted search tree (index.h:1116 builds one tree inside a
+// parallel region on a worker whose engine is still unseeded).
+void* ref_tree_build(const float* data, uint64_t n, uint64_t dim, uint64_t leaf, uint64_t max_depth,
+                     uint64_t seed) {
+    auto* t = new nndgraph::rptree();
+    std::thread th([&] {
+        *t = nndgraph::build_rptree(const_cast<float*>(data), n, dim, leaf, false, max_depth, seed);
+    });
+    th.join();
+    return t;
+}
+
+uint64_t ref_tree_nodes(void* h, uint64_t* total_idx) {
+    auto* t = static_cast<nndgraph::rptree*>(h);
+    uint64_t s = 0;
+    for (const auto& nd : t->nodes) s += nd.indices.size();
+    *total_idx = s;
+    return t->nodes.size();
+}
+
+void ref_tree_export(void* h, uint64_t* left, uint64_t* right, float* offset, float* hyper, uint64_t dim,
+                     uint64_t* idx_off, uint32_t* idx) {
+    auto* t = static_cast<nndgraph::rptree*>(h);
+    idx_off[0] = 0;
+    for (size_t k = 0; k < t->nodes.size(); k++) {
+        const auto& nd = t->nodes[k];
+        left[k] = nd.left;
+        right[k] = nd.right;
+        offset[k] = nd.offset;
+        for (uint64_t j = 0; j < dim; j++) hyper[k * dim + j] = nd.hyperplane.empty() ? 0.f : nd.hyperplane[j];
+        std::memcpy(idx + idx_off[k], nd.indices.data(), 4 * nd.indices.size());
+        idx_off[k + 1] = idx_off[k] + nd.indices.size();
+    }
+}
+
+void ref_tree_free(void* h) { delete static_cast<nndgraph::rptree*>(h); }
+
 }  // extern "C"
 
 // ---- CPU baseline: the reference's query loop body (rknn_query.cpp:2705-2776)

@@ -64,6 +64,8 @@ ORACLE_SIGS = {
     "or_prune": (U32, [P, P, U32, U32, U32, P, P, U32, U32, F32, P]),
     "or_commit": (None, [P, P, U32, U32, P, P, U32, F32, P, P]),
     "or_sort_pairs": (None, [P, P, U32]),
+    "or_mt_stream": (None, [U64, P, U64]),
+    "or_rp_tree_build": (U32, [P, U32, U32, U32, U32, U64, U32, P, P, P, P, P, P, P]),
 }
 
 REF_SIGS = {
@@ -90,6 +92,10 @@ REF_SIGS = {
     "ref_lrq_build": (None, [U64, U32, U32, P, P, P, U32, P, P, P]),
     "ref_lrq_materialize": (None, [U64, U32, U32, P, P, P, U32, P, P, U32, F64, P]),
     "ref_lrq_encode_value": (U32, [F64, P, U32]),
+    "ref_tree_build": (P, [P, U64, U64, U64, U64, U64]),
+    "ref_tree_nodes": (U64, [P, P]),
+    "ref_tree_export": (None, [P, P, P, P, P, U64, P, P]),
+    "ref_tree_free": (None, [P]),
 }
 
 _oracle = None
@@ -246,3 +252,68 @@ class Sample:
         self.deg, self.links, self.tree = read_graph(g + "_graph.bin", self.n, 48)
         self.h = dict(np.load(g + "_harness.npz"))
         self.prefix = g
+
+
+def oracle_rp_tree(x, leaf=100, seed=10, max_depth=100):
+    """or_rp_tree_build on host data -> dict in the reference's node order."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    cap = 4 * n + 16
+    t = dict(left=np.zeros(cap, np.uint64), right=np.zeros(cap, np.uint64), offset=np.zeros(cap, np.float32),
+             hyper=np.zeros((cap, d), np.float32), idx_off=np.zeros(cap + 1, np.uint64), idx=np.zeros(n, np.uint32))
+    draws = ctypes.c_uint64()
+    k = oracle().or_rp_tree_build(ptr(x), n, d, leaf, max_depth, seed, cap, ptr(t["left"]), ptr(t["right"]),
+                                  ptr(t["offset"]), ptr(t["hyper"]), ptr(t["idx_off"]), ptr(t["idx"]),
+                                  ctypes.byref(draws))
+    assert k > 0
+    return dict(nodes=k, d=d, left=t["left"][:k], right=t["right"][:k], offset=t["offset"][:k],
+                hyper=t["hyper"][:k], idx_off=t["idx_off"][:k + 1], idx=t["idx"][:int(t["idx_off"][k])],
+                draws=draws.value)
+
+
+def ref_rp_tree(x, leaf=100, seed=10, max_depth=100):
+    """The reference's build_rptree through the harness (fresh engine)."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    R = ref()
+    h = R.ref_tree_build(ptr(x), n, d, leaf, max_depth, seed)
+    tot = ctypes.c_uint64()
+    k = R.ref_tree_nodes(h, ctypes.byref(tot))
+    t = dict(nodes=k, d=d, left=np.zeros(k, np.uint64), right=np.zeros(k, np.uint64),
+             offset=np.zeros(k, np.float32), hyper=np.zeros((k, d), np.float32),
+             idx_off=np.zeros(k + 1, np.uint64), idx=np.zeros(max(tot.value, 1), np.uint32))
+    R.ref_tree_export(h, ptr(t["left"]), ptr(t["right"]), ptr(t["offset"]), ptr(t["hyper"]), d,
+                      ptr(t["idx_off"]), ptr(t["idx"]))
+    R.ref_tree_free(h)
+    t["idx"] = t["idx"][:tot.value]
+    return t
+
+
+def tree_diff(a, b):
+    """'' when two trees in the reference's node order are identical (float
+    fields compared as bits), else the name of the first differing field."""
+    if int(a["nodes"]) != int(b["nodes"]):
+        return f"nodes {a['nodes']} != {b['nodes']}"
+    for f in ("left", "right", "idx_off", "idx"):
+        if not np.array_equal(np.asarray(a[f], np.uint64), np.asarray(b[f], np.uint64)):
+            return f
+    for f in ("offset", "hyper"):
+        if not np.array_equal(np.asarray(a[f], np.float32).view(np.uint32),
+                              np.asarray(b[f], np.float32).view(np.uint32)):
+            return f
+    return ""
+
+
+def tree_datasets():
+    """Inputs that exercise every branch of the tree build: lifted sample,
+    mixture rows, integer rows with many ties, a two-point set that splits
+    degenerately, and a depth cap."""
+    import workload
+    rng = np.random.default_rng(0)
+    ties = np.floor(rng.random((4000, 20)) * 3).astype(np.float32)
+    degen = np.zeros((500, 8), np.float32)
+    degen[::2] = 1
+    mix = workload.million_base(20000) * np.float32(0.01)
+    return [("mixture-20000x128-leaf100", mix, 100, 100), ("mixture-5000-leaf7", mix[:5000], 7, 100),
+            ("ties-4000x20-leaf10", ties, 10, 100), ("degenerate-500x8-leaf10", degen, 10, 100),
+            ("depthcap-500x8-depth3", degen, 10, 3)]

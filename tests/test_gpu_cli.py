@@ -1,0 +1,79 @@
+"""Drop-in tools on the GPU against the reference's own outputs for the
+sample inputs (tests/golden/sample): rknn_query given the reference's AKNN
+source writes a verifier artifact byte-identical to the reference's, the same
+lift (M*^2, scale) and, per L_descent, the reference's recall and precision;
+build_aknn_source writes a valid .rec whose recall against the exact top-100
+is not below the reference source's."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from anqi_testlib import GOLDEN, REPO, Sample, read_rec
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(REPO, "anqi-implementation_amd", "bin")
+PIN_ENV = {
+    "ANQI_GRAPH_GEOMETRY": "anyk_lift", "ANQI_RADIUS_MODE": "rankm_resid_u8_lrq_floor",
+    "ANQI_VERIFIER_PLACEMENT": "postfilter", "ANQI_RANKM": "8", "ANQI_NK": "100", "ANQI_GRAPH_NK": "100",
+    "ANQI_RANKM_SLACK": "0", "ANQI_RECHECK": "0", "ANQI_RQ_OBJECTIVE": "radius_reconstruction",
+    "ANQI_ALPHA": "1.2", "ANQI_LIFTNNK": "50", "ANQI_GRAPHNND": "6", "ANQI_BUILD_BREAKDOWN": "1",
+    "ANQI_PREBUILT_TOPK_K": "100", "ANQI_BENCH_CALIBRATED_TIMING": "0",
+}
+POINT = re.compile(r"\[rknn\] L_descent=(\d+)\s+\| recall=([\d.]+) precision=([\d.]+) F1=([\d.]+) \| "
+                   r"avg cand=(\d+) pred=(\d+) #dist=(\d+)")
+LIFT = re.compile(r"BUILD lift M\*\^2=([\d.e+-]+) scale=([\d.e+-]+)")
+
+
+def say(*a):
+    print("[gpu-cli]", *a, flush=True)
+
+
+def points(text):
+    return {int(m.group(1)): tuple(float(g) for g in m.groups()[1:]) for m in POINT.finditer(text)}
+
+
+def test_rknn_query_on_reference_source(tmp_path):
+    pfx = os.path.join(GOLDEN, "sample", "sample")
+    env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
+    env.update(PIN_ENV)
+    env["ANQI_PREBUILT_TOPK_REC"] = pfx + "_baseknn_aknn.rec"
+    env["ANQI_GRAPH_CACHE_DIR"] = str(tmp_path / "graph")
+    env["ANQI_VERIFIER_CACHE_DIR"] = str(tmp_path / "verifier")
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    say("rknn_query sample k=10")
+    r = subprocess.run([os.path.join(BIN, "rknn_query"), "16,32,64,128", pfx, "48", "500", "64", "600", "10"],
+                       env=env, capture_output=True, text=True, timeout=300)
+    say(r.stdout[-3000:])
+    assert r.returncode == 0, r.stderr
+    ref = open(pfx + "_rknn_stdout_k10.txt").read()
+    mine, theirs = points(r.stdout), points(ref)
+    assert sorted(mine) == sorted(theirs)
+    for L in theirs:
+        assert abs(mine[L][0] - theirs[L][0]) <= 0.005 and abs(mine[L][1] - theirs[L][1]) <= 0.005, L
+    assert LIFT.search(r.stdout).groups() == LIFT.search(ref).groups()
+    art = [f for f in os.listdir(env["ANQI_VERIFIER_CACHE_DIR"]) if f.endswith(".bin")]
+    assert len(art) == 1
+    a = open(os.path.join(env["ANQI_VERIFIER_CACHE_DIR"], art[0]), "rb").read()
+    assert a == open(pfx + "_verifier.anqivf2", "rb").read()
+
+
+def test_build_aknn_source_sample(tmp_path):
+    S = Sample()
+    out = str(tmp_path / "s")
+    say("build_aknn_source sample")
+    r = subprocess.run([os.path.join(BIN, "build_aknn_source"), S.prefix + "_base.bin", out, "100", "500", "64",
+                        "600", "8", "10", "l2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "[aknn] DONE n=256 d=128 K=100" in r.stdout
+    ids, keys = read_rec(out + "_baseknn_aknn.rec", S.n)
+    assert (keys >= 0).all() and (np.diff(keys, axis=1) >= 0).all()
+    rec = np.mean([len(set(ids[i]) & set(S.gt_ids[i])) for i in range(S.n)]) / 100
+    ref_rec = np.mean([len(set(S.rec_ids[i]) & set(S.gt_ids[i])) for i in range(S.n)]) / 100
+    say("aknn recall", rec, "reference source", ref_rec)
+    assert rec >= ref_rec - 0.005
+    meta = open(out + "_aknn_build_time.meta").read().split("\n")
+    assert meta[0] == "ANQI_AKNN_BUILD_V1" and meta[1].startswith("seconds ")

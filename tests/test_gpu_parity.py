@@ -272,20 +272,24 @@ def test_gather_topk_matches_oracle(ctx, S):
         assert np.array_equal(bits(keys[i, :c]), bits(rk[:c]))
 
 
-def test_rp_tree_partitions_rows(ctx, S):
-    say("rp tree")
-    t = ctx.rp_tree(ctx.to_dev(S.lifted), leaf=20)
-    nodes = t.nodes
-    off = np.ctypeslib.as_array(t.leaf_off, shape=(nodes + 1,))
-    ids = np.ctypeslib.as_array(t.leaf_ids, shape=(int(off[-1]),))
-    assert sorted(ids.tolist()) == list(range(S.n))
-    sizes = np.diff(off)
-    left = np.zeros(nodes, np.uint32)
-    import pyanqi
-    ctx.check(ctx.L.anqi_d2h(ctx.ctx, pyanqi.hp(left), ctypes.c_void_p(t.left), 4 * nodes))
-    leaves = left == U32MAX
-    assert (sizes[~leaves] == 0).all() and (sizes[leaves] <= 20).all() and (sizes[leaves] > 0).all()
+def test_search_tree_matches_reference_draws(ctx, S):
+    """The device-built search tree equals the oracle's restatement of the
+    reference's build (one MT19937 stream, depth-first) node for node: the
+    sample tree as the reference wrote it, and every tree dataset (ties,
+    degenerate deals, depth cap, stream and node-table regrowth)."""
+    from anqi_testlib import oracle_rp_tree, tree_diff, tree_datasets
+    say("search tree: sample")
+    t = ctx.rp_tree(ctx.to_dev(S.lifted), leaf=100)
+    h = ctx.tree_host(t)
     ctx.tree_free(t)
+    assert tree_diff(h, dict(S.tree)) == ""
+    for name, x, leaf, depth in tree_datasets():
+        say("search tree:", name)
+        t = ctx.rp_tree(ctx.to_dev(x), leaf=leaf, max_depth=depth)
+        h = ctx.tree_host(t)
+        ctx.tree_free(t)
+        o = oracle_rp_tree(x, leaf, 10, depth)
+        assert tree_diff(h, o) == "", name
 
 
 def test_aknn_build_sample_recall(ctx, S):
