@@ -25,6 +25,7 @@ struct anqi_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int cus = 0;
+    uint32_t* counter = nullptr;   // device work counter of the persistent search
 };
 
 static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
@@ -137,6 +138,8 @@ int anqi_ctx_create(int device, anqi_ctx** out) {
 void anqi_ctx_destroy(anqi_ctx* ctx) {
     if (!ctx) return;
     if (ctx->own) { (void)hipStreamSynchronize(ctx->own); (void)hipStreamDestroy(ctx->own); }
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->counter) (void)hipFree(ctx->counter);
     delete ctx;
 }
 
@@ -1023,17 +1026,20 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     P.R2f = round_down_float(ix->mstar2 + 1.0);
     P.queries = queries; P.nq = nq; P.L = L; P.Lq = Lq; P.Lmax = Lq ? std::max(Lmax, 1u) : std::max(L, 1u);
     P.thr = thr; P.kidx = kidx;
-    P.hmask = 8191; P.rcap = 2048; P.out_cap = out_cap;
+    // LDS work area sized to the beam: visited-set slots ~ 4 L + 1536 (at least
+    // twice the typical visited count), results FIFO 1024.  Queries that
+    // overflow it are rerun below with 16K-entry areas.
+    P.hmask = std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * std::max(L, Lmax) + 1536))) - 1;
+    P.rcap = 1024; P.out_cap = out_cap;
     P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
     P.nexp = nexp;
     P.status = status;
     if (P.Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "L <= 1024");
-    DBuf counter;
     if (mode == 1) {
-        HCHK(counter.alloc(4));
-        HCHK(hipMemsetAsync(counter.p, 0, 4, s));
-        P.work_counter = counter.as<uint32_t>();
-        int occ = search_persistent_occupancy(search_lds_bytes(P));
+        if (!ctx->counter) HCHK(hipMalloc((void**)&ctx->counter, 256));
+        HCHK(launch_fill_u32(ctx->counter, 0u, 1, s));
+        P.work_counter = ctx->counter;
+        int occ = search_persistent_occupancy(P);
         uint32_t grid = (uint32_t)std::max(1, occ) * (uint32_t)std::max(1, ctx->cus);
         HCHK(launch_range_search(P, true, grid, s));
     } else {

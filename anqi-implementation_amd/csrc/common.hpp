@@ -131,6 +131,66 @@ __device__ __forceinline__ float qrow_l2(const QRow& q, const float* __restrict_
     return res;
 }
 
+// Batched row distances: for rows ids[s], s < k, group g (16 lanes) takes
+// s = g, g+4, g+8, g+12 of each 16-row pass and issues all four rows' loads
+// before reducing any, so the fetches overlap.  Same per-lane FMA order and
+// reduction tree as qrow_l2 / qrow_ip, hence the same bits.  MC = number of
+// full 16-float chunks of a row of X (dim = 16 MC + r, r < 16).
+//   MODE 0: out[s] = ||q - x||^2
+//   MODE 1: out[s] = d2_combine(<q, x>, nq, nrm[id])   (nndindex::d2)
+template <int MC, int MODE>
+__device__ __forceinline__ void rows_batch(const QRow& q, const float* __restrict__ X, uint32_t dim,
+                                           const float* __restrict__ nrm, float nq,
+                                           const uint32_t* ids, int k, int l, int g, float* out) {
+    const int r = (int)dim - 16 * MC;
+    for (int s0 = 0; s0 < k; s0 += 16) {
+        float x[4][MC];
+        float xt[4], nb[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int s = s0 + g + 4 * j;
+            if (s < k) {
+                const uint32_t id = ids[s];
+                const float* row = X + (size_t)id * dim;
+#pragma unroll
+                for (int t = 0; t < MC; t++) x[j][t] = row[16 * t + l];
+                xt[j] = (l < r) ? row[16 * MC + l] : 0.0f;
+                nb[j] = MODE == 1 ? nrm[id] : 0.0f;
+            } else {
+#pragma unroll
+                for (int t = 0; t < MC; t++) x[j][t] = 0.0f;
+                xt[j] = 0.0f;
+                nb[j] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float acc = 0.0f;
+            float res;
+            if (MODE == 0) {
+#pragma unroll
+                for (int t = 0; t < MC; t++) {
+                    float a = q.v[t] - x[j][t];
+                    acc = fmaf(a, a, acc);
+                }
+                res = tree16(acc);
+                if (r) {
+                    float a = q.tail - xt[j];
+                    res += tree16(a * a);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < MC; t++) acc = fmaf(q.v[t], x[j][t], acc);
+                res = tree16(acc);
+                if (r) res += tree16(q.tail * xt[j]);
+                res = fmaf(-2.0f, res, nq + nb[j]);
+            }
+            const int s = s0 + g + 4 * j;
+            if (l == 0 && s < k) out[s] = res;
+        }
+    }
+}
+
 // nndindex::d2 (include/index.h:172-176): na + nb - 2 ip.
 __device__ __forceinline__ float d2_combine(float ip, float na, float nb) {
     return fmaf(-2.0f, ip, na + nb);
@@ -203,7 +263,8 @@ __host__ __device__ __forceinline__ uint64_t rng_at(uint64_t seed, uint64_t a, u
 
 // ---- LDS open-addressing id set (key = id + 1, 0 = empty) --------------------
 __device__ __forceinline__ uint32_t hslot(uint32_t id, uint32_t mask) {
-    return (id * 0x9E3779B1u) & mask;
+    uint32_t h = id * 0x9E3779B1u;
+    return (h ^ (h >> 16)) & mask;
 }
 // Returns 1 if id was inserted (first visit), 0 if present, -1 if table full.
 __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_t id) {

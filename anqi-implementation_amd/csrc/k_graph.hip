@@ -319,8 +319,12 @@ __device__ __forceinline__ uint32_t merge_into(const uint64_t* cs, int kc, const
 
 // ---- gather (+ top-K or + prune) -------------------------------------------------------
 struct GatherLds {
-    float* qd[2];
-    uint32_t* qp[2];
+    float* qd0;       // beam queue, double buffered (selected by value so the
+    float* qd1;       // struct stays in registers)
+    uint32_t* qp0;
+    uint32_t* qp1;
+    __device__ float* qd(int b) const { return b ? qd1 : qd0; }
+    __device__ uint32_t* qp(int b) const { return b ? qp1 : qp0; }
     uint32_t* hash;
     float* pd;        // pool (graph) or top-K (2 buffers in AKNN mode)
     uint32_t* pi;
@@ -352,10 +356,10 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base + o; o += rnd16(bytes); return p; };
     s.hash = (uint32_t*)take(4 * (size_t)(P.hmask + 1));
-    s.qd[0] = (float*)take(4 * (size_t)P.explore);
-    s.qd[1] = (float*)take(4 * (size_t)P.explore);
-    s.qp[0] = (uint32_t*)take(4 * (size_t)P.explore);
-    s.qp[1] = (uint32_t*)take(4 * (size_t)P.explore);
+    s.qd0 = (float*)take(4 * (size_t)P.explore);
+    s.qd1 = (float*)take(4 * (size_t)P.explore);
+    s.qp0 = (uint32_t*)take(4 * (size_t)P.explore);
+    s.qp1 = (uint32_t*)take(4 * (size_t)P.explore);
     s.selv = (float*)base;
     s.sel = (uint32_t*)(base + rnd16(4 * (size_t)P.M * P.d));
     o = gather_region0(P, graph);
@@ -441,7 +445,7 @@ __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uin
     return deg;
 }
 
-template <bool GRAPH, bool GWS>
+template <bool GRAPH, bool GWS, int MC>
 __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     extern __shared__ __align__(16) char lds_raw[];
     uint32_t bi = blockIdx.x;
@@ -475,10 +479,14 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
         if (fresh) { S.tid[lanes_below(fm)] = id; S.tdv[lanes_below(fm)] = seed_d; }
         __syncthreads();
         if (!seeded) {
-            for (int s = g; s < k; s += 4) {
-                uint32_t c = S.tid[s];
-                float ip = qrow_ip(QI, P.x + (size_t)c * d, d, l);
-                if (l == 0) S.tdv[s] = d2_combine(ip, ni, P.nrm[c]);
+            if constexpr (MC > 0) {
+                rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, k, l, g, S.tdv);
+            } else {
+                for (int s = g; s < k; s += 4) {
+                    uint32_t c = S.tid[s];
+                    float ip = qrow_ip(QI, P.x + (size_t)c * d, d, l);
+                    if (l == 0) S.tdv[s] = d2_combine(ip, ni, P.nrm[c]);
+                }
             }
             __syncthreads();
         }
@@ -508,8 +516,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
         // queue
         {
             uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
-            float* ad = S.qd[qb];
-            uint32_t* ap = S.qp[qb];
+            float* ad = S.qd(qb);
+            uint32_t* ap = S.qp(qb);
             if (qsize == P.explore && key != ~0ull &&
                 !(key < pkey(ad[P.explore - 1], ap[P.explore - 1] & 0x7fffffffu)))
                 key = ~0ull;
@@ -518,7 +526,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
             if (kc) {
                 S.cs[lane] = key;
                 __syncthreads();
-                qsize = merge_into(S.cs, kc, ad, ap, qsize, P.explore, S.qd[qb ^ 1], S.qp[qb ^ 1]);
+                qsize = merge_into(S.cs, kc, ad, ap, qsize, P.explore, S.qd(qb ^ 1), S.qp(qb ^ 1));
                 qb ^= 1;
             }
             __syncthreads();
@@ -538,13 +546,13 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
         uint32_t cur = U32_NONE;
         for (uint32_t b = 0; b < qsize; b += 64) {
             uint32_t t = b + lane;
-            uint64_t m = ballot(t < qsize && !(S.qp[qb][t] >> 31));
+            uint64_t m = ballot(t < qsize && !(S.qp(qb)[t] >> 31));
             if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
         }
         if (cur == U32_NONE) break;
-        uint32_t c = S.qp[qb][cur];
+        uint32_t c = S.qp(qb)[cur];
         __syncthreads();
-        if (lane == 0) S.qp[qb][cur] = c | 0x80000000u;
+        if (lane == 0) S.qp(qb)[cur] = c | 0x80000000u;
         __syncthreads();
         for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
             uint32_t s = s0 + lane;
@@ -594,21 +602,28 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
 }
 
+template <bool GRAPH, bool GWS, int MC>
+static hipError_t launch_gather_t(const GatherParams& P, size_t lds, hipStream_t s) {
+    const void* f = (const void*)k_gather<GRAPH, GWS, MC>;
+    if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC>), dim3(P.nrows), dim3(64), lds, s, P);
+    return hipGetLastError();
+}
+
+template <bool GRAPH, bool GWS>
+static hipError_t launch_gather_mc(const GatherParams& P, size_t lds, hipStream_t s) {
+    switch (P.d >> 4) {
+        case 8: return launch_gather_t<GRAPH, GWS, 8>(P, lds, s);
+        case 6: return launch_gather_t<GRAPH, GWS, 6>(P, lds, s);
+        default: return launch_gather_t<GRAPH, GWS, 0>(P, lds, s);
+    }
+}
+
 hipError_t launch_gather(const GatherParams& P, bool graph, bool gws, hipStream_t s) {
     if (P.nrows == 0) return hipSuccess;
     size_t lds = gws ? 0 : gather_lds_bytes(P, graph);
-    const void* f;
-    if (graph) f = gws ? (const void*)k_gather<true, true> : (const void*)k_gather<true, false>;
-    else f = gws ? (const void*)k_gather<false, true> : (const void*)k_gather<false, false>;
-    if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (graph) {
-        if (gws) hipLaunchKernelGGL((k_gather<true, true>), dim3(P.nrows), dim3(64), lds, s, P);
-        else hipLaunchKernelGGL((k_gather<true, false>), dim3(P.nrows), dim3(64), lds, s, P);
-    } else {
-        if (gws) hipLaunchKernelGGL((k_gather<false, true>), dim3(P.nrows), dim3(64), lds, s, P);
-        else hipLaunchKernelGGL((k_gather<false, false>), dim3(P.nrows), dim3(64), lds, s, P);
-    }
-    return hipGetLastError();
+    if (graph) return gws ? launch_gather_mc<true, true>(P, lds, s) : launch_gather_mc<true, false>(P, lds, s);
+    return gws ? launch_gather_mc<false, true>(P, lds, s) : launch_gather_mc<false, false>(P, lds, s);
 }
 
 // ---- commit: cand = O_i u R_i, prune to M (include/index.h:907-930) -----------------

@@ -23,8 +23,13 @@ namespace {
 struct SearchLds {
     float* qlift;      // dl
     float* qbase;      // d
-    float* bd[2];      // beam distances
-    uint32_t* bp[2];   // beam ids, bit31 = expanded
+    float* bd0;        // beam distances (double buffer 0 / 1)
+    float* bd1;
+    uint32_t* bp0;     // beam ids, bit31 = expanded
+    uint32_t* bp1;
+    // buffer selection by value, so the struct stays in registers
+    __device__ float* bd(int b) const { return b ? bd1 : bd0; }
+    __device__ uint32_t* bp(int b) const { return b ? bp1 : bp0; }
     uint32_t* hash;    // visited set
     uint32_t* res;     // collected ids (FIFO)
     uint32_t* tid;     // 64 new ids
@@ -38,10 +43,10 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
     auto take = [&](size_t bytes) { char* p = base + o; o += (bytes + 15) & ~size_t(15); return p; };
     s.qlift = (float*)take(sizeof(float) * (P.dl + 16));
     s.qbase = (float*)take(sizeof(float) * (P.d + 16));
-    s.bd[0] = (float*)take(sizeof(float) * P.Lmax);
-    s.bd[1] = (float*)take(sizeof(float) * P.Lmax);
-    s.bp[0] = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
-    s.bp[1] = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+    s.bd0 = (float*)take(sizeof(float) * P.Lmax);
+    s.bd1 = (float*)take(sizeof(float) * P.Lmax);
+    s.bp0 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+    s.bp1 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
     s.hash = (uint32_t*)take(sizeof(uint32_t) * (P.hmask + 1));
     s.res = (uint32_t*)take(sizeof(uint32_t) * P.rcap);
     s.tid = (uint32_t*)take(sizeof(uint32_t) * 64);
@@ -53,16 +58,35 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
 // Expands node c: visits its unvisited neighbours, computes D, appends
 // collected ones to the FIFO.  In beam mode also merges them into the beam.
 // Returns false on a capacity overflow.
-template <bool BEAM>
+// D for the k new neighbours tid[0..k).  Group g (16 lanes) takes rows g,
+// g+4, g+8, g+12 of each 16-row pass and loads all four before reducing any,
+// so their fetches overlap.  MC = number of full 16-float chunks of a lifted
+// row (d' = 16 MC + r, r < 16); MC == 0 selects the generic loop.
+template <int MC>
+__device__ __forceinline__ void batch_l2(const SearchParams& P, const QRow& QL, const uint32_t* tid,
+                                         float* tdv, int k, int l, int g) {
+    if constexpr (MC > 0) {
+        rows_batch<MC, 0>(QL, P.lifted, P.dl, nullptr, 0.0f, tid, k, l, g, tdv);
+    } else {
+        for (int s = g; s < k; s += 4) {
+            float dv = qrow_l2(QL, P.lifted + (size_t)tid[s] * P.dl, P.dl, l);
+            if (l == 0) tdv[s] = dv;
+        }
+    }
+}
+
+template <bool BEAM, int MC>
 __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, const QRow& QL,
                                        uint32_t c, uint32_t& rcnt, uint32_t& nd,
                                        int& buf, uint32_t& bsize, uint32_t L, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
-    uint32_t dg = P.deg[c];
+    // degree and neighbour row fetched together (slots past deg are ignored)
+    const uint32_t dg = P.deg[c];
+    const uint32_t nbr = ((uint32_t)lane < P.M) ? P.links[(size_t)c * P.M + lane] : U32_NONE;
     uint32_t nb = U32_NONE;
     int fresh = 0;
     if ((uint32_t)lane < dg) {
-        nb = P.links[(size_t)c * P.M + lane];
+        nb = nbr;
         int r = hset_insert(S.hash, P.hmask, nb);
         if (r < 0) ovf = 1;
         fresh = (r == 1);
@@ -72,12 +96,7 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     int k = popc64(fm);
     if (fresh) S.tid[lanes_below(fm)] = nb;
     __syncthreads();
-    // distances, four pairs per step (one per 16-lane group)
-    for (int s = g; s < k; s += 4) {
-        uint32_t id = S.tid[s];
-        float dv = qrow_l2(QL, P.lifted + (size_t)id * P.dl, P.dl, l);
-        if (l == 0) S.tdv[s] = dv;
-    }
+    batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
     __syncthreads();
     nd += (uint32_t)k;
     float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
@@ -91,8 +110,8 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     if (BEAM) {
         // candidate keys; drop those that cannot enter a full beam
         uint64_t key = (lane < k) ? pkey(dv, myid) : ~0ull;
-        float* bd = S.bd[buf];
-        uint32_t* bp = S.bp[buf];
+        float* bd = S.bd(buf);
+        uint32_t* bp = S.bp(buf);
         if (bsize == L && lane < k) {
             uint64_t last = pkey(bd[L - 1], bp[L - 1] & 0x7fffffffu);
             if (!(key < last)) key = ~0ull;
@@ -103,8 +122,8 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
         if (kc == 0) return true;
         S.cs[lane] = key;
         __syncthreads();
-        float* od = S.bd[buf ^ 1];
-        uint32_t* op = S.bp[buf ^ 1];
+        float* od = S.bd(buf ^ 1);
+        uint32_t* op = S.bp(buf ^ 1);
         // beam elements move right by the number of smaller candidates
         for (uint32_t i = lane; i < bsize; i += 64) {
             uint64_t bk = pkey(bd[i], bp[i] & 0x7fffffffu);
@@ -129,6 +148,7 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     return true;
 }
 
+template <int MC>
 __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     const int lane = __lane_id(), l = lane & 15;
     const uint32_t L = P.Lq ? P.Lq[qi] : P.L;
@@ -162,8 +182,8 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     int buf = 0, ovf = 0;
     if (lane == 0) {
         S.hash[hslot(ep, P.hmask)] = ep + 1u;
-        S.bd[0][0] = dep;
-        S.bp[0][0] = ep;
+        S.bd0[0] = dep;
+        S.bp0[0] = ep;
         if (dep <= P.R2f) S.res[0] = ep;
     }
     if (dep <= P.R2f) rcnt = 1;
@@ -173,23 +193,23 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         uint32_t cur = U32_NONE;
         for (uint32_t b = 0; b < bsize; b += 64) {
             uint32_t i = b + lane;
-            uint64_t m = ballot(i < bsize && !(S.bp[buf][i] >> 31));
+            uint64_t m = ballot(i < bsize && !(S.bp(buf)[i] >> 31));
             if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
         }
         if (cur == U32_NONE) break;
-        uint32_t c = S.bp[buf][cur];
+        uint32_t c = S.bp(buf)[cur];
         __syncthreads();
-        if (lane == 0) S.bp[buf][cur] = c | 0x80000000u;
+        if (lane == 0) S.bp(buf)[cur] = c | 0x80000000u;
         __syncthreads();
         nx++;
-        if (!expand<true>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+        if (!expand<true, MC>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
     }
     // (2) flood over collected nodes, FIFO order
     uint32_t head = 0;
     while (!ovf && head < rcnt) {
         uint32_t c = S.res[head++];
         nx++;
-        if (!expand<false>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+        if (!expand<false, MC>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
     }
     if (ovf) {
         if (lane == 0) { P.status[qi] = (uint32_t)ovf; P.cand_cnt[qi] = rcnt; P.ndist[qi] = nd; P.pred_cnt[qi] = 0; }
@@ -202,17 +222,65 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     uint32_t npred = 0;
     uint32_t* out = P.cand_ids + (size_t)qi * P.out_cap;
     uint8_t* acc = P.accept + (size_t)qi * P.out_cap;
-    for (uint32_t b = 0; b < rcnt; b += 4) {
-        uint32_t j = b + (uint32_t)(lane >> 4);
-        int ok = 0;
-        if (j < rcnt) {
-            uint32_t o = S.res[j];
-            float ip = qrow_ip(QB, P.base + (size_t)o * P.d, P.d, l);
-            float key = fmaf(-2.0f, ip, P.bnorm[o] + qn);
-            ok = key <= thr[o];
-            if (l == 0 && j < P.out_cap) { out[j] = o; acc[j] = (uint8_t)ok; }
+    if constexpr (MC == 0) {
+        for (uint32_t b = 0; b < rcnt; b += 4) {
+            uint32_t j = b + (uint32_t)(lane >> 4);
+            int ok = 0;
+            if (j < rcnt) {
+                uint32_t o = S.res[j];
+                float ip = qrow_ip(QB, P.base + (size_t)o * P.d, P.d, l);
+                float key = fmaf(-2.0f, ip, P.bnorm[o] + qn);
+                ok = key <= thr[o];
+                if (l == 0 && j < P.out_cap) { out[j] = o; acc[j] = (uint8_t)ok; }
+            }
+            npred += (uint32_t)popc64(ballot(ok && l == 0));
         }
-        npred += (uint32_t)popc64(ballot(ok && l == 0));
+    } else {
+        // four candidates per group per pass, loads issued before the reductions
+        constexpr int M1 = MC;
+        const int g = lane >> 4;
+        const int rb = (int)P.d - 16 * M1;
+        for (uint32_t b = 0; b < rcnt; b += 16) {
+            float x[4][M1], xt[4], bn[4], th[4];
+            uint32_t oid[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t s = b + (uint32_t)(g + 4 * j);
+                oid[j] = U32_NONE;
+                if (s < rcnt) {
+                    const uint32_t o = S.res[s];
+                    oid[j] = o;
+                    const float* row = P.base + (size_t)o * P.d;
+#pragma unroll
+                    for (int t = 0; t < M1; t++) x[j][t] = row[16 * t + l];
+                    xt[j] = (l < rb) ? row[16 * M1 + l] : 0.0f;
+                    bn[j] = P.bnorm[o];
+                    th[j] = thr[o];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < M1; t++) x[j][t] = 0.0f;
+                    xt[j] = 0.0f;
+                    bn[j] = 0.0f;
+                    th[j] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float a = 0.0f;
+#pragma unroll
+                for (int t = 0; t < M1; t++) a = fmaf(QB.v[t], x[j][t], a);
+                float ip = tree16(a);
+                if (rb) ip += tree16(QB.tail * xt[j]);
+                const uint32_t s = b + (uint32_t)(g + 4 * j);
+                int ok = 0;
+                if (s < rcnt) {
+                    float key = fmaf(-2.0f, ip, bn[j] + qn);
+                    ok = key <= th[j];
+                    if (l == 0 && s < P.out_cap) { out[s] = oid[j]; acc[s] = (uint8_t)ok; }
+                }
+                npred += (uint32_t)popc64(ballot(ok && l == 0));
+            }
+        }
     }
     if (lane == 0) {
         P.status[qi] = (rcnt > P.out_cap) ? 3u : 0u;
@@ -225,16 +293,18 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
 
 }  // namespace
 
+template <int MC>
 __global__ void __launch_bounds__(64) k_range_search(SearchParams P) {
     extern __shared__ __align__(16) char lds[];
     SearchLds S = carve(lds, P);
-    uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
     if (blockIdx.x >= P.nq) return;
-    search_one(P, S, qi);
+    uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
+    search_one<MC>(P, S, qi);
 }
 
 // Persistent form: a fixed grid pulls query indices from a device counter.
 // Every wave exits once the counter passes nq.
+template <int MC>
 __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) {
     extern __shared__ __align__(16) char lds[];
     SearchLds S = carve(lds, P);
@@ -246,9 +316,16 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
         __syncthreads();
         if (t >= P.nq) break;
         uint32_t qi = P.qlist ? P.qlist[t] : t;
-        search_one(P, S, qi);
+        search_one<MC>(P, S, qi);
         __syncthreads();
     }
+}
+
+// Chunk count shared by the lifted rows (d + 2) and the base rows (d).
+static int search_chunks(const SearchParams& P) {
+    const int m = (int)(P.dl >> 4);
+    if ((int)(P.d >> 4) != m) return 0;
+    return (m == 8 || m == 6) ? m : 0;
 }
 
 size_t search_lds_bytes(const SearchParams& P) {
@@ -262,19 +339,28 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     if (i < n) p[i] = v;
 }
 
-hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s) {
+template <int MC>
+static hipError_t launch_search_mc(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s) {
     size_t lds = search_lds_bytes(P);
     if (persistent) {
-        hipFuncSetAttribute((const void*)k_range_search_persistent,
+        hipFuncSetAttribute((const void*)k_range_search_persistent<MC>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_range_search_persistent, dim3(grid), dim3(64), lds, s, P);
+        hipLaunchKernelGGL(k_range_search_persistent<MC>, dim3(grid), dim3(64), lds, s, P);
     } else {
         if (P.nq == 0) return hipSuccess;
-        hipFuncSetAttribute((const void*)k_range_search,
+        hipFuncSetAttribute((const void*)k_range_search<MC>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_range_search, dim3(P.nq), dim3(64), lds, s, P);
+        hipLaunchKernelGGL(k_range_search<MC>, dim3(P.nq), dim3(64), lds, s, P);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s) {
+    switch (search_chunks(P)) {
+        case 8: return launch_search_mc<8>(P, persistent, grid, s);
+        case 6: return launch_search_mc<6>(P, persistent, grid, s);
+        default: return launch_search_mc<0>(P, persistent, grid, s);
+    }
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
@@ -283,10 +369,20 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-int search_persistent_occupancy(size_t lds) {
+template <int MC>
+static int occupancy_mc(size_t lds) {
     int occ = 0;
-    hipFuncSetAttribute((const void*)k_range_search_persistent, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_range_search_persistent, 64, lds) != hipSuccess)
-        occ = 1;
+    const void* f = (const void*)k_range_search_persistent<MC>;
+    hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 64, lds) != hipSuccess) occ = 1;
     return occ;
+}
+
+int search_persistent_occupancy(const SearchParams& P) {
+    const size_t lds = search_lds_bytes(P);
+    switch (search_chunks(P)) {
+        case 8: return occupancy_mc<8>(lds);
+        case 6: return occupancy_mc<6>(lds);
+        default: return occupancy_mc<0>(lds);
+    }
 }

@@ -42,7 +42,7 @@ struct SearchParams {
     uint32_t* work_counter; // persistent form
 };
 size_t search_lds_bytes(const SearchParams& P);
-int search_persistent_occupancy(size_t lds);
+int search_persistent_occupancy(const SearchParams& P);
 hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 
