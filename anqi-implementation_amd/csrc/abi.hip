@@ -198,11 +198,32 @@ int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32
                    uint32_t* ids, float* keys) {
     if (K == 0 || K >= n || K > 128) return anqi_set_err(ctx, ANQI_E_USAGE, "exact_knn: need 0 < K < n, K <= 128");
     if (d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
+    hipStream_t s = ctx->stream;
     DBuf nrm;
     HCHK(nrm.alloc(sizeof(float) * (size_t)n));
-    HCHK(launch_norms(x, n, d, nrm.as<float>(), ctx->stream));
-    HCHK(launch_exact_topk(x, nrm.as<float>(), n, d, K, 0, n, precision, nullptr, 0.0f, ids, keys, ctx->stream));
-    HCHK(hipStreamSynchronize(ctx->stream));
+    HCHK(launch_norms(x, n, d, nrm.as<float>(), s));
+    if (precision == 1 && exact_mfma_supported(d, K)) {
+        // bf16 sweep on the matrix cores, exact FP32 re-scoring inside a
+        // per-row error bound: same ids and keys as the FP32 sweep
+        DBuf xb, nx, ex, marg;
+        HCHK(xb.alloc(2ull * n * d));
+        HCHK(nx.alloc(4ull * n));
+        HCHK(ex.alloc(4ull * n));
+        HCHK(marg.alloc(4ull * n));
+        HCHK(launch_to_bf16(x, (uint64_t)n * d, xb.as<uint16_t>(), s));
+        HCHK(launch_exact_norms2(x, nrm.as<float>(), n, d, nx.as<float>(), ex.as<float>(), s));
+        std::vector<float> hn(n), he(n);
+        HCHK(hipMemcpyAsync(hn.data(), nx.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(he.data(), ex.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        float Y = 0.0f, E = 0.0f;
+        for (uint32_t i = 0; i < n; i++) { Y = std::max(Y, hn[i]); E = std::max(E, he[i]); }
+        HCHK(launch_exact_margin(nx.as<float>(), ex.as<float>(), n, d, Y, E, marg.as<float>(), s));
+        HCHK(launch_exact_mfma(x, xb.as<uint16_t>(), nrm.as<float>(), marg.as<float>(), n, d, K, 0, n, ids, keys, s));
+    } else {
+        HCHK(launch_exact_topk(x, nrm.as<float>(), n, d, K, 0, n, 0, nullptr, 0.0f, ids, keys, s));
+    }
+    HCHK(hipStreamSynchronize(s));
     return ANQI_OK;
 }
 

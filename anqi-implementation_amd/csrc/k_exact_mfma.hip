@@ -1,0 +1,341 @@
+// k_exact_mfma.hip -- exact self-excluded top-K (tools/dataset_gt.cpp:206-245)
+// with the pair sweep on the matrix cores.
+//
+// Output is identical to the FP32 kernel (k_exact.hip): the K smallest
+// (key, id) pairs with key = bn_i + bn_j - 2 <x_i, x_j> under the reference's
+// FP32 contract.  The MFMA pass only decides which pairs need that exact key:
+//
+//   approx_ij = fmaf(-2, ip_bf16_ij, bn_i + bn_j)     (v_mfma_f32_32x32x16_bf16)
+//   a pair can enter row i's top-K only if exact_ij <= tau_i (the row's
+//   current K-th key), and |exact_ij - approx_ij| <= M_i, so pairs with
+//   approx_ij > tau_i + M_i are skipped; the rest are re-scored exactly.
+//
+// M_i bounds the bf16 rounding of both operands, the FP32 accumulation error
+// of the MFMA sum and of the reference's own FP32 dot product, and the final
+// roundings (computed by k_exact_margin; see there).
+//
+// Tiling: a block owns 64 rows (4 wavefronts).  Column tiles of 128 rows of a
+// bf16 copy of the base are staged in LDS (the next tile is prefetched into
+// registers while the current one is swept).  Wavefront w computes rows
+// 32 (w >> 1) .. +32 against tile columns 64 (w & 1) .. +64 as two 32x32 MFMA
+// accumulators; the operand fragments follow the instruction's lane layout
+// (A: row l % 32, k 8 (l / 32) .. +8; B: column l % 32, same k; C element v:
+// row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  Surviving pairs go to a
+// per-row list; the owning wavefront re-scores them with the FP32 contract and
+// merges those beating the row's K-th pair into its sorted list.
+#include "common.hpp"
+#include "kernels.hpp"
+
+typedef __bf16 xm_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float xm_f32x16 __attribute__((ext_vector_type(16)));
+
+#define XM_R 64                      // rows per block
+#define XM_C 128                     // columns per tile
+#define XM_KS 8                      // 16-wide k steps: d <= 128, zero padded
+#define XM_BS (16 * XM_KS + 8)       // bf16 per staged tile row (272 bytes)
+#define XM_VEC ((XM_C * 16 * XM_KS / 8) / 256)   // 16-byte vectors per thread per tile
+
+struct XmLds {
+    uint16_t* btile;   // XM_C x XM_BS bf16
+    float* bnc;        // XM_C column norms
+    float* ld;         // XM_R x K
+    uint32_t* li;      // XM_R x K
+    uint8_t* surv;     // XM_R x XM_C column offsets
+    uint32_t* scnt;    // XM_R
+    uint32_t* lcnt;    // XM_R
+    uint64_t* thr;     // XM_R: K-th (key, id) pair, ~0 while the list is short
+    float* thrm;       // XM_R: K-th key + M_i (filter bound), +inf while short
+    float* sd;         // 4 x K merge scratch
+    uint32_t* si;      // 4 x K
+    uint64_t* cs;      // 4 x 64 sorted new pairs
+    uint32_t* xid;     // 4 x 64 column ids under exact evaluation
+    float* xd;         // 4 x 64 exact keys
+};
+
+__host__ __device__ inline size_t xm_r16(size_t b) { return (b + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t xm_carve(char* base, uint32_t K, XmLds* s) {
+    size_t o = 0;
+    auto take = [&](size_t bytes) { char* p = base ? base + o : nullptr; o += xm_r16(bytes); return p; };
+    char* p;
+    p = take(2ull * XM_C * XM_BS); if (s) s->btile = (uint16_t*)p;
+    p = take(4ull * XM_C); if (s) s->bnc = (float*)p;
+    p = take(4ull * XM_R * K); if (s) s->ld = (float*)p;
+    p = take(4ull * XM_R * K); if (s) s->li = (uint32_t*)p;
+    p = take(1ull * XM_R * XM_C); if (s) s->surv = (uint8_t*)p;
+    p = take(4ull * XM_R); if (s) s->scnt = (uint32_t*)p;
+    p = take(4ull * XM_R); if (s) s->lcnt = (uint32_t*)p;
+    p = take(8ull * XM_R); if (s) s->thr = (uint64_t*)p;
+    p = take(4ull * XM_R); if (s) s->thrm = (float*)p;
+    p = take(4ull * 4 * K); if (s) s->sd = (float*)p;
+    p = take(4ull * 4 * K); if (s) s->si = (uint32_t*)p;
+    p = take(8ull * 4 * 64); if (s) s->cs = (uint64_t*)p;
+    p = take(4ull * 4 * 64); if (s) s->xid = (uint32_t*)p;
+    p = take(4ull * 4 * 64); if (s) s->xd = (float*)p;
+    return o;
+}
+
+#define XM_WSYNC()                                              \
+    do {                                                        \
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  \
+        __builtin_amdgcn_wave_barrier();                        \
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  \
+    } while (0)
+
+// Merges the wave's sorted new pairs (key per lane, bc of them, ascending,
+// all below the row's K-th pair) into row r's sorted list.
+__device__ __forceinline__ void xm_merge(XmLds& S, int w, int r, uint32_t K, uint64_t key, int bc,
+                                         float mrow) {
+    const int lane = __lane_id();
+    uint64_t* csw = S.cs + w * 64;
+    csw[lane] = key;
+    XM_WSYNC();
+    float* rd = S.ld + (size_t)r * K;
+    uint32_t* ri = S.li + (size_t)r * K;
+    float* od = S.sd + (size_t)w * K;
+    uint32_t* oi = S.si + (size_t)w * K;
+    const uint32_t size = S.lcnt[r];
+    for (uint32_t i = lane; i < size; i += 64) {
+        uint64_t bk = pkey(rd[i], ri[i]);
+        int lo = 0, hi = bc;
+        while (lo < hi) { int mid = (lo + hi) >> 1; if (csw[mid] < bk) lo = mid + 1; else hi = mid; }
+        uint32_t pos = i + (uint32_t)lo;
+        if (pos < K) { od[pos] = rd[i]; oi[pos] = ri[i]; }
+    }
+    if (lane < bc) {
+        uint32_t lo = 0, hi = size;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (pkey(rd[mid], ri[mid]) < key) lo = mid + 1; else hi = mid;
+        }
+        uint32_t pos = (uint32_t)lane + lo;
+        if (pos < K) { od[pos] = pkey_d(key); oi[pos] = pkey_id(key); }
+    }
+    XM_WSYNC();
+    const uint32_t ns = min(K, size + (uint32_t)bc);
+    for (uint32_t i = lane; i < ns; i += 64) { rd[i] = od[i]; ri[i] = oi[i]; }
+    XM_WSYNC();
+    if (lane == 0) {
+        S.lcnt[r] = ns;
+        if (ns >= K) {
+            S.thr[r] = pkey(rd[K - 1], ri[K - 1]);
+            S.thrm[r] = rd[K - 1] + mrow;
+        }
+    }
+    XM_WSYNC();
+}
+
+template <int MC>
+__global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x, const uint16_t* __restrict__ xb,
+                                                    const float* __restrict__ nrm, const float* __restrict__ marg,
+                                                    uint32_t n, uint32_t d, uint32_t K, uint32_t row0,
+                                                    uint32_t rows, uint32_t* __restrict__ out_ids,
+                                                    float* __restrict__ out_keys) {
+    extern __shared__ __align__(16) char lds[];
+    XmLds S;
+    xm_carve(lds, K, &S);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l = lane & 15, g = lane >> 4;
+    const uint32_t rb = row0 + blockIdx.x * XM_R;
+    const uint32_t rend = min(row0 + rows, rb + XM_R);
+    for (int r = tid; r < XM_R; r += 256) {
+        S.scnt[r] = 0;
+        S.lcnt[r] = 0;
+        S.thr[r] = ~0ull;
+        S.thrm[r] = __builtin_huge_valf();
+    }
+    // A fragments: this wave's 32 rows, bf16, zero padded past d
+    const int h = w >> 1, ch = w & 1;
+    const uint32_t arow = rb + 32 * h + (lane & 31);
+    xm_bf16x8 afr[XM_KS];
+#pragma unroll
+    for (int s = 0; s < XM_KS; s++) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            uint32_t k = 16 * s + 8 * (lane >> 5) + e;
+            uint16_t b = (arow < rend && k < d) ? xb[(size_t)arow * d + k] : (uint16_t)0;
+            afr[s][e] = __builtin_bit_cast(__bf16, b);
+        }
+    }
+    // per-lane accumulator rows: v -> row 8 (v / 4) + 4 (l / 32) + v % 4 of the half
+    float bnr[16];
+    uint32_t rid[16];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        uint32_t rl = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        rid[v] = rb + rl;
+        bnr[v] = rid[v] < rend ? nrm[rid[v]] : 0.0f;
+    }
+    // prefetch of tile 0
+    const uint32_t vrow = d / 8;              // 16-byte vectors per base row
+    uint4 pf[XM_VEC];
+    auto fetch = [&](uint32_t c0) {
+#pragma unroll
+        for (int u = 0; u < XM_VEC; u++) {
+            uint32_t e = (uint32_t)u * 256 + (uint32_t)tid;     // vector index in the tile
+            uint32_t c = e / (16 * XM_KS / 8), kv = e % (16 * XM_KS / 8);
+            uint4 z = make_uint4(0, 0, 0, 0);
+            if (c0 + c < n && kv < vrow) z = *(const uint4*)(xb + (size_t)(c0 + c) * d + 8 * kv);
+            pf[u] = z;
+        }
+    };
+    fetch(0);
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < n; c0 += XM_C) {
+        // stage the prefetched tile, then prefetch the next one
+#pragma unroll
+        for (int u = 0; u < XM_VEC; u++) {
+            uint32_t e = (uint32_t)u * 256 + (uint32_t)tid;
+            uint32_t c = e / (16 * XM_KS / 8), kv = e % (16 * XM_KS / 8);
+            *(uint4*)(S.btile + (size_t)c * XM_BS + 8 * kv) = pf[u];
+        }
+        for (uint32_t c = tid; c < XM_C; c += 256) S.bnc[c] = (c0 + c < n) ? nrm[c0 + c] : 0.0f;
+        __syncthreads();
+        if (c0 + XM_C < n) fetch(c0 + XM_C);
+        // sweep: two 32x32 accumulators per wave
+        float thrm[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) thrm[v] = S.thrm[32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int cl = 64 * ch + 32 * t + (lane & 31);     // tile column of this lane
+            xm_f32x16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+            const uint16_t* bcol = S.btile + (size_t)cl * XM_BS + 8 * (lane >> 5);
+#pragma unroll
+            for (int s = 0; s < XM_KS; s++) {
+                xm_bf16x8 bfr = *(const xm_bf16x8*)(bcol + 16 * s);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[s], bfr, acc, 0, 0, 0);
+            }
+            const uint32_t cid = c0 + (uint32_t)cl;
+            const float bc = S.bnc[cl];
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                float approx = fmaf(-2.0f, acc[v], bnr[v] + bc);
+                if (cid < n && rid[v] < rend && cid != rid[v] && approx <= thrm[v]) {
+                    int rl = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    uint32_t p = atomicAdd(&S.scnt[rl], 1u);
+                    S.surv[(size_t)rl * XM_C + p] = (uint8_t)cl;
+                }
+            }
+        }
+        __syncthreads();
+        // exact re-scoring and merge: wave w owns rows 16 w .. 16 w + 15
+        for (int rr = 0; rr < 16; rr++) {
+            const int r = w * 16 + rr;
+            const uint32_t cnt = S.scnt[r];
+            if (cnt == 0) continue;
+            const uint32_t row = rb + (uint32_t)r;
+            QRow QR;
+            qrow_load(QR, x + (size_t)row * d, (int)d, l);
+            const float nr = nrm[row];
+            const float mrow = marg[row];
+            uint32_t* xid = S.xid + w * 64;
+            float* xd = S.xd + w * 64;
+            for (uint32_t b0 = 0; b0 < cnt; b0 += 64) {
+                const int k = (int)min(64u, cnt - b0);
+                if (lane < k) xid[lane] = c0 + S.surv[(size_t)r * XM_C + b0 + lane];
+                XM_WSYNC();
+                rows_batch<MC, 1>(QR, x, d, nrm, nr, xid, k, l, g, xd);
+                XM_WSYNC();
+                const uint64_t th = S.thr[r];
+                uint64_t key = ~0ull;
+                if (lane < k) {
+                    uint64_t pk = pkey(xd[lane], xid[lane]);
+                    if (pk < th) key = pk;
+                }
+                key = wave_sort64(key);
+                const int bc = popc64(ballot(key != ~0ull));
+                if (bc) xm_merge(S, w, r, K, key, bc, mrow);
+            }
+            if (lane == 0) S.scnt[r] = 0;
+        }
+        __syncthreads();
+    }
+    for (int r = w * 16; r < w * 16 + 16; r++) {
+        uint32_t row = rb + r;
+        if (row >= rend) continue;
+        for (uint32_t i = lane; i < K; i += 64) {
+            out_ids[(size_t)(row - row0) * K + i] = i < S.lcnt[r] ? S.li[(size_t)r * K + i] : U32_NONE;
+            out_keys[(size_t)(row - row0) * K + i] = i < S.lcnt[r] ? S.ld[(size_t)r * K + i] : F32_BIG;
+        }
+    }
+}
+
+// Per-row filter margin M_i (see the header), evaluated in FP32 with every
+// term rounded up by generous factors:
+//   |approx - exact| <= 2 [ g ((|x|+e_x)(Y+E) + |x| Y) + e_x (Y+E) + |x| E ]
+//                       + 4 u (|x| + Y)^2 + tiny
+// with u = 2^-24, g = d u / (1 - d u) (any summation order), e_x = |x - bf16(x)|,
+// Y = max |y|, E = max e_y over the columns.
+__global__ void k_exact_norms2(const float* __restrict__ x, const float* __restrict__ nrm, uint32_t n, uint32_t d,
+                               float* __restrict__ nx, float* __restrict__ ex) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = x + (size_t)i * d;
+    float e2 = 0.0f;
+    for (uint32_t k = 0; k < d; k++) {
+        float v = r[k];
+        uint32_t u = __float_as_uint(v);
+        float b = __uint_as_float(((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) << 16);
+        float df = v - b;
+        e2 = fmaf(df, df, e2);
+    }
+    nx[i] = sqrtf(fmaxf(nrm[i], 0.0f)) * 1.0001f + 1e-30f;
+    ex[i] = sqrtf(e2) * 1.001f + 1e-30f;
+}
+
+__global__ void k_exact_margin(const float* __restrict__ nx, const float* __restrict__ ex, uint32_t n, uint32_t d,
+                               float Y, float E, float* __restrict__ marg) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float u = 5.9604645e-08f;
+    const float gm = (float)d * u / (1.0f - (float)d * u) * 1.01f;
+    const float a = nx[i], e = ex[i];
+    float m = 2.0f * (gm * ((a + e) * (Y + E) + a * Y) + e * (Y + E) + a * E) + 4.0f * u * (a + Y) * (a + Y);
+    marg[i] = m * 1.01f + 1e-30f + 2.0f * (float)d * 1.2e-38f;
+}
+
+size_t exact_mfma_lds(uint32_t K) { return xm_carve(nullptr, K, nullptr); }
+
+bool exact_mfma_supported(uint32_t d, uint32_t K) {
+    const uint32_t mc = d >> 4;
+    return d <= 16 * XM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && K <= 128;
+}
+
+template <int MC>
+static hipError_t launch_exact_mfma_t(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
+                                      uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
+                                      uint32_t* ids, float* keys, hipStream_t s) {
+    size_t lds = exact_mfma_lds(K);
+    hipFuncSetAttribute((const void*)k_exact_mfma<MC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_exact_mfma<MC>, dim3((rows + XM_R - 1) / XM_R), dim3(256), lds, s, x, xb, nrm, marg, n,
+                       d, K, row0, rows, ids, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nrm, const float* marg, uint32_t n,
+                             uint32_t d, uint32_t K, uint32_t row0, uint32_t rows, uint32_t* ids, float* keys,
+                             hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    if (!exact_mfma_supported(d, K)) return hipErrorInvalidValue;
+    switch (d >> 4) {
+        case 8: return launch_exact_mfma_t<8>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        case 7: return launch_exact_mfma_t<7>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        default: return launch_exact_mfma_t<6>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+    }
+}
+
+hipError_t launch_exact_norms2(const float* x, const float* nrm, uint32_t n, uint32_t d, float* nx, float* ex,
+                               hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_exact_norms2, dim3((n + 255) / 256), dim3(256), 0, s, x, nrm, n, d, nx, ex);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_margin(const float* nx, const float* ex, uint32_t n, uint32_t d, float Y, float E,
+                               float* marg, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_exact_margin, dim3((n + 255) / 256), dim3(256), 0, s, nx, ex, n, d, Y, E, marg);
+    return hipGetLastError();
+}

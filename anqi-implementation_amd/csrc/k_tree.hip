@@ -74,6 +74,55 @@ __device__ __forceinline__ void tb_scan2(int fa, int fb, uint32_t* wsum, uint32_
     tb = xb;
 }
 
+// side[start + i] for the m points of a node: each 16-lane group takes four
+// points per pass (i = g, g + 64, g + 128, g + 192 of the pass) and issues all
+// their loads before reducing, so one workgroup keeps many rows in flight.
+// Same FMA order and reduction tree as qrow_ip.
+template <int MC>
+__device__ __forceinline__ void tb_margins(const TreeBuildParams& P, const QRow& hq, float off, uint32_t start,
+                                           uint32_t m, uint32_t g, uint32_t l) {
+    constexpr uint32_t G = TB_THREADS / 16;
+    const uint32_t d = P.d;
+    if constexpr (MC == 0) {
+        for (uint32_t i = g; i < m; i += G) {
+            const float* xr = P.x + (size_t)P.perm[start + i] * d;
+            float mg = qrow_ip(hq, xr, (int)d, (int)l) - off;
+            if (l == 0) P.side[start + i] = mg < -TB_EPS ? 0 : (mg > TB_EPS ? 1 : 2);
+        }
+    } else {
+        const int r = (int)d - 16 * MC;
+        for (uint32_t i0 = 0; i0 < m; i0 += 4 * G) {
+            float xv[4][MC], xt[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t i = i0 + g + G * (uint32_t)j;
+                if (i < m) {
+                    const float* xr = P.x + (size_t)P.perm[start + i] * d;
+#pragma unroll
+                    for (int t = 0; t < MC; t++) xv[j][t] = xr[16 * t + l];
+                    xt[j] = ((int)l < r) ? xr[16 * MC + l] : 0.0f;
+                } else {
+#pragma unroll
+                    for (int t = 0; t < MC; t++) xv[j][t] = 0.0f;
+                    xt[j] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < MC; t++) acc = fmaf(hq.v[t], xv[j][t], acc);
+                float ip = tree16(acc);
+                if (r) ip += tree16(hq.tail * xt[j]);
+                const float mg = ip - off;
+                const uint32_t i = i0 + g + G * (uint32_t)j;
+                if (l == 0 && i < m) P.side[start + i] = mg < -TB_EPS ? 0 : (mg > TB_EPS ? 1 : 2);
+            }
+        }
+    }
+}
+
+template <int MC>
 __global__ __launch_bounds__(TB_THREADS) void k_tree_build(TreeBuildParams P) {
     extern __shared__ float tb_dyn[];
     float* hyp = tb_dyn;
@@ -167,11 +216,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_tree_build(TreeBuildParams P) {
         QRow hq;
         qrow_load(hq, hyp, (int)d, (int)l);
         // margins (rptrees.h:64-76); 2 marks a tie to be drawn in index order
-        for (uint32_t i = g; i < f.m; i += TB_THREADS / 16) {
-            const float* xr = P.x + (size_t)P.perm[f.start + i] * d;
-            float mg = qrow_ip(hq, xr, (int)d, (int)l) - off;
-            if (l == 0) P.side[f.start + i] = mg < -TB_EPS ? 0 : (mg > TB_EPS ? 1 : 2);
-        }
+        tb_margins<MC>(P, hq, off, f.start, f.m, g, l);
         __syncthreads();
         // resolve ties with consecutive draws; count the left side
         uint32_t nl = 0, ties = 0;
@@ -285,7 +330,11 @@ hipError_t launch_tree_build(const TreeBuildParams& P, hipStream_t s) {
     if (P.d > 16 * QMAX_CHUNKS + 15 || P.max_depth + 2 > TB_STACK) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_iota_u32, dim3((P.n + 255) / 256), dim3(256), 0, s, P.perm, P.n);
     size_t lds = 2ull * P.d * sizeof(float);
-    hipLaunchKernelGGL(k_tree_build, dim3(1), dim3(TB_THREADS), lds, s, P);
+    switch (P.d >> 4) {
+        case 8: hipLaunchKernelGGL(k_tree_build<8>, dim3(1), dim3(TB_THREADS), lds, s, P); break;
+        case 6: hipLaunchKernelGGL(k_tree_build<6>, dim3(1), dim3(TB_THREADS), lds, s, P); break;
+        default: hipLaunchKernelGGL(k_tree_build<0>, dim3(1), dim3(TB_THREADS), lds, s, P); break;
+    }
     return hipGetLastError();
 }
 

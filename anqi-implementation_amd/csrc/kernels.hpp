@@ -186,6 +186,16 @@ hipError_t launch_rknn_fill(const float* base, const float* bnorm, uint32_t n,
                             const float* q, uint32_t nq, uint32_t d, const float* rk2,
                             const uint32_t* off, uint32_t* ids, float* keys, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, uint64_t m, uint16_t* out, hipStream_t s);
+// MFMA-filtered exact top-K (k_exact_mfma.hip); same output as launch_exact_topk.
+bool exact_mfma_supported(uint32_t d, uint32_t K);
+size_t exact_mfma_lds(uint32_t K);
+hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nrm, const float* marg, uint32_t n,
+                             uint32_t d, uint32_t K, uint32_t row0, uint32_t rows, uint32_t* ids, float* keys,
+                             hipStream_t s);
+hipError_t launch_exact_norms2(const float* x, const float* nrm, uint32_t n, uint32_t d, float* nx, float* ex,
+                               hipStream_t s);
+hipError_t launch_exact_margin(const float* nx, const float* ex, uint32_t n, uint32_t d, float Y, float E,
+                               float* marg, hipStream_t s);
 
 hipError_t launch_seg_fill(uint32_t* seg_of, const uint32_t* seg_start, const uint32_t* seg_size,
                            uint32_t nseg, hipStream_t s);

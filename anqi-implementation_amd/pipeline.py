@@ -93,7 +93,7 @@ def ground_truth(ctx, base, queries, k):
     """Exact RkNN CSR (tools/dataset_gt.cpp): exact top-k thresholds, then
     membership key <= r_k."""
     n = base.shape[0]
-    ids, keys = ctx.exact_knn(base, k)
+    ids, keys = ctx.exact_knn(base, k, precision=1)   # bf16 MFMA sweep, FP32-exact output
     rk2 = keys[:, k - 1].contiguous()
     off, gids, gkeys = ctx.rknn_membership(base, queries, rk2)
     return off, gids.cpu().numpy().view(np.uint32), rk2

@@ -63,18 +63,40 @@ def test_norms_and_pair_d2_bits(ctx):
         assert np.array_equal(bits(got), bits(exp))
 
 
-def test_exact_knn_matches_dataset_gt(ctx, S):
-    say("exact knn sample")
-    ids, keys = ctx.exact_knn(ctx.to_dev(S.base), 100)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_exact_knn_matches_dataset_gt(ctx, S, precision):
+    say("exact knn sample, precision", precision)
+    ids, keys = ctx.exact_knn(ctx.to_dev(S.base), 100, precision=precision)
     assert np.array_equal(host(ids).view(np.uint32), S.gt_ids)
     assert np.array_equal(bits(host(keys)), bits(S.gt_keys))
 
 
-@pytest.mark.parametrize("n,d,K", [(3000, 128, 100), (1500, 130, 50), (700, 96, 10)])
-def test_exact_knn_matches_oracle(ctx, n, d, K):
-    say("exact knn", n, d, K)
-    x = workload.mixture(n, d, 50, 20.0, seed=7)   # integer data: many distance ties
-    ids, keys = ctx.exact_knn(ctx.to_dev(x), K)
+def _exact_cases():
+    rng = np.random.default_rng(3)
+    wide = (rng.standard_normal((2000, 128)) * np.exp(rng.uniform(-3, 3, (2000, 1)))).astype(np.float32)
+    small = (rng.standard_normal((1200, 112)) * 1e-3 + 5.0).astype(np.float32)   # large offset, tiny spread
+    return {
+        "mixture-3000x128-K100": (lambda: workload.mixture(3000, 128, 50, 20.0, seed=7), 100),
+        "mixture-1500x130-K50": (lambda: workload.mixture(1500, 130, 50, 20.0, seed=7), 50),
+        "mixture-700x96-K10": (lambda: workload.mixture(700, 96, 50, 20.0, seed=7), 10),
+        "gaussian-2500x128-K100": (lambda: workload.gaussian(2500, 128, seed=5), 100),
+        "wide-norms-2000x128-K64": (lambda: wide, 64),
+        "offset-1200x112-K128": (lambda: small, 128),
+    }
+
+
+@pytest.mark.parametrize("case", list(_exact_cases()))
+@pytest.mark.parametrize("precision", [0, 1])
+def test_exact_knn_matches_oracle(ctx, case, precision):
+    """Both sweeps (FP32 tiles; bf16 MFMA filter + FP32 re-score) return the
+    oracle's ids and key bits: integer data with many key ties, Gaussian
+    data, rows whose norms span e^6, and rows far from the origin whose keys
+    are small differences of large norms (the bf16 bound is widest there)."""
+    make, K = _exact_cases()[case]
+    x = np.ascontiguousarray(make(), np.float32)
+    n, d = x.shape
+    say("exact knn", case, "precision", precision)
+    ids, keys = ctx.exact_knn(ctx.to_dev(x), K, precision=precision)
     ri = np.zeros((n, K), np.uint32)
     rk = np.zeros((n, K), np.float32)
     oracle().or_exact_topk(ptr(x), n, d, K, ptr(ri), ptr(rk), 8)
