@@ -885,6 +885,29 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     HCHK(hipMemsetAsync(lcnt.p, 0, 4ull * n, s));
     HCHK(launch_fill_u32(lid.as<uint32_t>(), U32_NONE, (uint64_t)n * K, s));
     uint32_t T = (uint32_t)F.leaves.size();
+    // bf16 copy and per-row error bounds for the MFMA leaf sweep
+    uint32_t max_leaf_all = 0;
+    for (auto& lv : F.leaves)
+        for (auto& pr : lv) max_leaf_all = std::max(max_leaf_all, pr.second);
+    const bool use_mfma = p->precision == 1 && leaf_mfma_supported(d, max_leaf_all);
+    DBuf xb, lmarg;
+    if (use_mfma) {
+        DBuf nx, ex;
+        HCHK(xb.alloc(2ull * n * d));
+        HCHK(nx.alloc(4ull * n));
+        HCHK(ex.alloc(4ull * n));
+        HCHK(lmarg.alloc(4ull * n));
+        HCHK(launch_to_bf16(x, (uint64_t)n * d, xb.as<uint16_t>(), s));
+        HCHK(launch_exact_norms2(x, nrm.as<float>(), n, d, nx.as<float>(), ex.as<float>(), s));
+        std::vector<float> hn(n), he(n);
+        HCHK(hipMemcpyAsync(hn.data(), nx.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(he.data(), ex.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        float Y = 0.0f, E = 0.0f;
+        for (uint32_t i = 0; i < n; i++) { Y = std::max(Y, hn[i]); E = std::max(E, he[i]); }
+        HCHK(launch_exact_margin(nx.as<float>(), ex.as<float>(), n, d, Y, E, lmarg.as<float>(), s));
+        HCHK(hipStreamSynchronize(s));
+    }
     for (uint32_t t = 0; t < T; t++) {
         auto& lv = F.leaves[t];
         std::vector<uint32_t> hs(lv.size()), hz(lv.size());
@@ -907,7 +930,8 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
         LP.leaf_start = ds.as<uint32_t>(); LP.leaf_size = dz.as<uint32_t>(); LP.nleaves = (uint32_t)lv.size();
         LP.K = K; LP.ids = lid.as<uint32_t>(); LP.keys = lkey.as<float>(); LP.cnt = lcnt.as<uint32_t>();
         LP.max_leaf = mx;
-        HCHK(launch_leaf_knn(LP, p->precision != 0, s));
+        if (use_mfma) HCHK(launch_leaf_mfma(LP, xb.as<uint16_t>(), lmarg.as<float>(), s));
+        else HCHK(launch_leaf_knn(LP, false, s));
         HCHK(hipStreamSynchronize(s));   // keeps ds/dz alive until the launch ends
         if ((t & 7) == 7 || t + 1 == T) progress("leaf all-pairs: tree %u/%u", t + 1, T);
     }

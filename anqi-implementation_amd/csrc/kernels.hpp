@@ -170,6 +170,10 @@ struct LeafParams {
 };
 size_t leaf_lds_bytes(const LeafParams& P);
 hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s);
+// bf16 MFMA leaf sweep (k_leaf_mfma.hip); same lists as launch_leaf_knn.
+bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf);
+size_t leaf_mfma_lds(uint32_t K);
+hipError_t launch_leaf_mfma(const LeafParams& P, const uint16_t* xb, const float* marg, hipStream_t s);
 hipError_t launch_random_fill(const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t K,
                               uint64_t seed, uint32_t* ids, float* keys, uint32_t* cnt,
                               hipStream_t s);

@@ -327,3 +327,24 @@ def test_aknn_build_sample_recall(ctx, S):
     rec_ref = np.mean([len(set(S.rec_ids[i]) & set(S.gt_ids[i])) / 100 for i in range(n)])
     say("aknn recall", rec, "reference", rec_ref)
     assert rec >= rec_ref - 0.005
+
+
+@pytest.mark.parametrize("case", ["sample-256x128", "mixture-5000x128", "gaussian-3000x96", "wide-norms-2000x128"])
+def test_aknn_mfma_leaf_sweep_equals_fp32(ctx, S, case):
+    """The bf16 MFMA leaf sweep (precision 1) leaves every row's list exactly
+    as the FP32 sweep does: identical AKNN ids and key bits after the whole
+    build (leaf all-pairs over every tree, then the gather)."""
+    rng = np.random.default_rng(11)
+    x = {"sample-256x128": lambda: S.base,
+         "mixture-5000x128": lambda: workload.mixture(5000, 128, 40, 20.0, seed=3),
+         "gaussian-3000x96": lambda: workload.gaussian(3000, 96, seed=4),
+         "wide-norms-2000x128": lambda: (rng.standard_normal((2000, 128)) *
+                                         np.exp(rng.uniform(-3, 3, (2000, 1)))).astype(np.float32)}[case]()
+    x = np.ascontiguousarray(x, np.float32)
+    say("aknn mfma vs fp32:", case)
+    xd = ctx.to_dev(x)
+    K = 100 if x.shape[0] > 300 else 50
+    a_ids, a_keys, _ = ctx.aknn_build(xd, K=K, leaf=200, trees=8, ef=64, precision=0)
+    b_ids, b_keys, _ = ctx.aknn_build(xd, K=K, leaf=200, trees=8, ef=64, precision=1)
+    assert np.array_equal(host(a_ids), host(b_ids))
+    assert np.array_equal(bits(host(a_keys)), bits(host(b_keys)))
