@@ -5,9 +5,21 @@ queries from the same centres), with the GPU build seconds of the index.
 
 A step is one pass of the query path (lift, beam + flood range search over
 the G48 lifted graph, LRQ postfilter) over the 10,000-query batch, with the
-index, thresholds and queries resident in HBM.  The beam width L is the
-smallest of a sweep whose recall against the exact RkNN ground truth is
->= 0.95.  Prints one JSON line.
+index, thresholds and queries resident in HBM.
+
+Forms: "chosen" = bf16-MFMA build sweeps (exact top-K filter, RP-leaf) +
+persistent work-queue query launch; "plain" = FP32 build sweeps + one
+launch per batch with one block per query.  Both produce the same index
+bit for bit (checked here), so recall is shared; build seconds and QPS are
+measured for each.
+
+Fixed recall: the beam width L is the smallest swept L whose recall against
+the exact RkNN ground truth is >= 0.95; when no swept L reaches 0.95 (the
+search-tree entry limits recall on this clustered workload, for the
+reference as well), L is the smallest one within 0.5 points of the recall at
+the largest swept L.  The reference's query loop on the same graph (cpu
+baseline) returns the same candidates, so its recall at that L is the same.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -80,14 +92,47 @@ def cpu_baseline(ctx, ix, queries_np, thr, L, seconds_target=12.0):
         probe = min(nq, 1000)
         s = R.ref_rknn_batch(h, base.ctypes.data, base.shape[1], q.ctypes.data, probe, ix.scale, R2, L,
                              thr_h.ctypes.data, cores, 1, ctypes.byref(c), ctypes.byref(p), ctypes.byref(dd))
+        # per-query check on the same graph: candidate / accepted / distance
+        # counts, and the tree ties (|margin| < EPS) where the reference draws
+        # a random side from its thread's engine and this build goes left
+        det = {k: np.zeros(nq, np.uint32) for k in ("cnt", "pred", "nd", "entry", "ties")}
+        R.ref_rknn_detail.restype = None
+        R.ref_rknn_detail(h, base.ctypes.data, base.shape[1], q.ctypes.data, nq, ctypes.c_double(ix.scale),
+                          ctypes.c_double(R2), ctypes.c_uint64(L), thr_h.ctypes.data, cores,
+                          *(det[k].ctypes.data for k in ("cnt", "pred", "nd", "entry", "ties")))
         reps = max(1, int(seconds_target / max(s * nq / probe, 1e-3)))
         s = R.ref_rknn_batch(h, base.ctypes.data, base.shape[1], q.ctypes.data, nq, ix.scale, R2, L,
                              thr_h.ctypes.data, cores, reps, ctypes.byref(c), ctypes.byref(p), ctypes.byref(dd))
         R.ref_index_close(h)
+        tot = nq * reps
         return {"value": nq * reps / s, "unit": "queries/s", "cores": cores, "kind": "reference",
+                "_totals": {"ref_avg_cand": c.value / tot, "ref_avg_pred": p.value / tot,
+                            "ref_avg_dist": dd.value / tot},
+                "_detail": det,
                 "sample": f"{nq} queries x {reps} passes ({s:.1f} s), reference range_search + postfilter "
                           f"(rknn_query.cpp:2705-2776, ground-truth evaluation excluded) on the GPU-built graph, "
                           f"L={L}, {cores} OpenMP threads"}
+
+
+def index_digest(ix):
+    """Digest of the built index (graph rows, lift meta, verifier codes)."""
+    import hashlib
+    h = hashlib.sha256()
+    for t in (ix.links, ix.deg, ix.codes, ix.coeff, ix.lifted):
+        h.update(t.cpu().numpy().tobytes())
+    h.update(np.float64([ix.mstar2, ix.scale]).tobytes())
+    h.update(np.uint32([ix.tree.nodes]).tobytes())
+    return h.hexdigest()[:16]
+
+
+def load_traffic(kernel, workload_tag):
+    """HBM bytes per launch measured with rocprofv3 PMC counters for this
+    kernel and workload (profiles/traffic_<kernel>_<workload>.json), or None."""
+    path = os.path.join(HERE, "profiles", f"traffic_{kernel}_{workload_tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def main():
@@ -99,9 +144,12 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--L", type=int, default=0, help="fixed beam width (0: recall sweep)")
-    ap.add_argument("--form", choices=["chosen", "plain"], default="chosen")
+    ap.add_argument("--form", choices=["chosen", "plain"], default="chosen", help="form reported as value")
+    ap.add_argument("--no-plain-build", action="store_true", help="skip the plain-form build timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--detail", default="")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed region, one 1 GiB device-to-device copy (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported by this build "
@@ -119,11 +167,28 @@ def main():
     t = time.perf_counter()
     base = workload.million_base(a.n)
     queries = workload.million_query(a.nq)
-    log(f"data n={a.n} nq={a.nq} in {time.perf_counter() - t:.1f}s sha256 base={workload.sha256(base)[:16]}")
-    precision = 1 if a.form == "chosen" else 0
-    mode = 1 if a.form == "chosen" else 0
-    ix = pipeline.build(ctx, base, precision=precision, log=log)
-    log("build", json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in ix.times.items()}))
+    sha = workload.sha256(base)
+    log(f"data n={a.n} nq={a.nq} in {time.perf_counter() - t:.1f}s sha256 base={sha[:16]}")
+    # one-time device/library initialisation (code objects, solver handles)
+    # outside the timed builds
+    wix = pipeline.build(ctx, base[:20000], precision=1, log=lambda *x: None)
+    ctx.tree_free(wix.tree)
+    del wix
+    plain_build = None
+    plain_digest = None
+    if not a.no_plain_build:
+        ixp = pipeline.build(ctx, base, precision=0, log=log)
+        plain_build = ixp.times
+        plain_digest = index_digest(ixp)
+        ctx.tree_free(ixp.tree)
+        del ixp
+        torch.cuda.empty_cache()
+        log("plain build", json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in plain_build.items()}))
+    ix = pipeline.build(ctx, base, precision=1, log=log)
+    chosen_digest = index_digest(ix)
+    log("chosen build", json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in ix.times.items()}))
+    if plain_digest is not None and plain_digest != chosen_digest:
+        raise SystemExit(f"plain and chosen builds differ: {plain_digest} vs {chosen_digest}")
     qd = ctx.to_dev(queries)
     t = time.perf_counter()
     off, gids, _ = pipeline.ground_truth(ctx, ix.base, qd, a.k)
@@ -132,55 +197,82 @@ def main():
     out_cap = 4096
     bufs = ctx.search_buffers(a.nq, out_cap)
     sweep = []
-    chosen_L = a.L
-    for L in ([a.L] if a.L else [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256]):
-        ctx.rknn_search(ix.handle, qd, L, thr, mode=mode, bufs=bufs)
+    Ls = [a.L] if a.L else [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256]
+    for L in Ls:
+        ctx.rknn_search(ix.handle, qd, L, thr, mode=1, bufs=bufs)
         m = pipeline.metrics(bufs, off, gids)
         m["L"] = L
         sweep.append(m)
         log("sweep", json.dumps({k: round(v, 4) if isinstance(v, float) else v for k, v in m.items()}))
-        if not a.L and m["recall"] >= TARGET_RECALL:
-            chosen_L = L
-            break
-    if not chosen_L:
-        chosen_L = max(sweep, key=lambda m: m["recall"])["L"]
+    if a.L:
+        chosen_L, rule = a.L, "fixed by --L"
+    elif any(m["recall"] >= TARGET_RECALL for m in sweep):
+        chosen_L = min(m["L"] for m in sweep if m["recall"] >= TARGET_RECALL)
+        rule = f"smallest L with recall >= {TARGET_RECALL}"
+    else:
+        best = sweep[-1]["recall"]
+        chosen_L = min(m["L"] for m in sweep if m["recall"] >= best - 0.005)
+        rule = f"recall {TARGET_RECALL} unreachable; smallest L within 0.5 pt of recall {best:.4f} at L={sweep[-1]['L']}"
     point = [m for m in sweep if m["L"] == chosen_L][0]
-    # timed region
-    for _ in range(a.warmup):
-        ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=mode, bufs=bufs)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    kms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        ev0.record()
-        ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=mode, bufs=bufs)
-        ev1.record()
-        ev1.synchronize()
-        kms += ev0.elapsed_time(ev1)
-    torch.cuda.synchronize()
-    secs = time.perf_counter() - t0
-    qps = a.nq * a.steps / secs
-    launch_ms = kms / a.steps
+    ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=1, bufs=bufs)
     B = query_bytes(ix, bufs)
-    achieved = B / (launch_ms * 1e-3) / 1e9
-    other = {}
-    if a.form == "chosen":
-        # plain form on the same index: one wavefront per query, per-batch launch
-        for _ in range(2):
-            ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=0, bufs=bufs)
+
+    def timed(mode):
+        for _ in range(a.warmup):
+            ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=mode, bufs=bufs)
         torch.cuda.synchronize()
-        tp = time.perf_counter()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        kms = 0.0
+        t0 = time.perf_counter()
         for _ in range(a.steps):
-            ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=0, bufs=bufs)
+            ev0.record()
+            ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=mode, bufs=bufs)
+            ev1.record()
+            ev1.synchronize()
+            kms += ev0.elapsed_time(ev1)
         torch.cuda.synchronize()
-        other["plain_query_qps"] = a.nq * a.steps / (time.perf_counter() - tp)
+        secs = time.perf_counter() - t0
+        return a.nq * a.steps / secs, secs, kms / a.steps
+
+    qps_c, secs_c, launch_c = timed(1)
+    qps_p, secs_p, launch_p = timed(0)
+    if a.calib_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
+    if a.form == "chosen":
+        qps, secs, launch_ms, mode, kern = qps_c, secs_c, launch_c, 1, "k_range_search_persistent"
+    else:
+        qps, secs, launch_ms, mode, kern = qps_p, secs_p, launch_p, 0, "k_range_search"
+    achieved = B / (launch_ms * 1e-3) / 1e9
+    tag = f"mix{a.n // 1000}k-L{chosen_L}" if a.n % 1_000_000 else f"mix{a.n // 1_000_000}m-L{chosen_L}"
+    tr = load_traffic(kern, tag)
     cpu = None
     if not a.no_cpu:
         try:
             cpu = cpu_baseline(ctx, ix, queries, thr, chosen_L)
+            if cpu is not None:
+                det = cpu.pop("_detail")
+                gc, gp, gn = (bufs[k].cpu().numpy().astype(np.int64) for k in ("cnt", "pred", "nd"))
+                diff = (gc != det["cnt"]) | (gp != det["pred"]) | (gn != det["nd"])
+                cpu["same_graph_check"] = {
+                    "gpu_avg_cand": float(gc.mean()), "gpu_avg_pred": float(gp.mean()),
+                    "gpu_avg_dist": float((gn + gc).mean()),
+                    **cpu.pop("_totals"),
+                    "queries_differing": int(diff.sum()),
+                    "queries_differing_with_tree_tie": int((diff & (det["ties"] > 0)).sum()),
+                    "queries_with_tree_tie": int((det["ties"] > 0).sum())}
         except Exception as e:   # reported, never fatal for the GPU measurement
             log("cpu baseline failed:", repr(e))
+    build_c = ix.times["build_s"]
+    other = {"chosen_query_qps": qps_c, "plain_query_qps": qps_p, "query_qps_ratio": qps_c / qps_p,
+             "chosen_build_s": build_c}
+    if plain_build is not None:
+        other.update({"plain_build_s": plain_build["build_s"], "build_s_ratio": build_c / plain_build["build_s"],
+                      "plain_build_breakdown_s": {k: plain_build[k] for k in ("aknn_source_s", "verifier_train_s",
+                                                                             "encode_s", "lift_s", "graph_s")},
+                      "index_digest": chosen_digest, "plain_index_identical": plain_digest == chosen_digest})
     res = {
         "metric": "RkNN QPS @k=10 at fixed recall (chosen vs plain GPU form); 1Mx128 build s",
         "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -188,21 +280,22 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"gaussian-mixture {a.n}x128 (1000 centres, sigma 20, rounded, clip 0..255), "
                                f"{a.nq} queries, k={a.k}, AKNN 100/500/64/600, G48, LocalJoin 6, M8+u8 LRQ",
-                   "form": a.form, "L": chosen_L, "recall": point["recall"], "precision": point["precision"],
-                   "F1": point["F1"], "avg_cand": point["cand"], "avg_dist": point["dist"],
-                   "build_s": ix.times["build_s"],
+                   "base_sha256": sha, "form": a.form, "L": chosen_L, "L_rule": rule,
+                   "recall": point["recall"], "precision": point["precision"], "F1": point["F1"],
+                   "avg_cand": point["cand"], "avg_dist": point["dist"], "build_s": build_c,
                    "build_breakdown_s": {k: ix.times[k] for k in ("aknn_source_s", "verifier_train_s", "encode_s",
                                                                   "lift_s", "graph_s")},
                    **other},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_range_search_persistent" if mode == 1 else "k_range_search",
-                     "launch_ms": launch_ms, "bytes_per_launch": B},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tr["bytes_per_launch"] if tr else None,
+                     "kernel": kern, "launch_ms": launch_ms, "bytes_per_launch": B,
+                     **({"traffic_source": tr["source"]} if tr else {})},
         "cpu_baseline": cpu,
     }
     if a.detail:
         with open(a.detail, "w") as f:
-            json.dump({"result": res, "sweep": sweep, "times": ix.times}, f, indent=1)
+            json.dump({"result": res, "sweep": sweep, "times": ix.times, "plain_times": plain_build}, f, indent=1)
     print(json.dumps(res), flush=True)
 
 

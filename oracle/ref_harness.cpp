@@ -350,3 +350,58 @@ extern "C" double ref_rknn_batch(void* h, const float* base, uint32_t d, const f
     *tot_dist = dd;
     return secs;
 }
+
+// Per-query form of ref_rknn_batch (one pass): candidate, accepted and
+// distance counts, the entry point, and the number of |margin| < EPS ties met
+// on the search-tree descent (where the reference draws a random side from
+// the calling thread's engine, rptrees.h:116-118).
+extern "C" void ref_rknn_detail(void* h, const float* base, uint32_t d, const float* q, uint32_t nq,
+                                double scale, double R2, uint64_t L, const float* thr, int threads,
+                                uint32_t* cnt, uint32_t* pred, uint32_t* ndist, uint32_t* entry,
+                                uint32_t* ties) {
+    nndindex* ix = static_cast<RefIndex*>(h)->idx;
+    const size_t n = ix->n, dl = ix->dim;
+    omp_set_num_threads(threads);
+#pragma omp parallel
+    {
+        std::vector<uint16_t> mark(n, 0);
+        uint16_t ver = 0;
+        std::vector<uint32_t> cand;
+        std::vector<float> qb(dl);
+#pragma omp for schedule(dynamic, 16)
+        for (size_t i = 0; i < nq; i++) {
+            const float* qq = q + i * d;
+            for (size_t j = 0; j < d; j++) qb[j] = (float)((double)qq[j] * scale);
+            qb[d] = 1.0f;
+            qb[d + 1] = 0.0f;
+            uint32_t nt = 0;
+            {
+                const auto& T = ix->search_tree;
+                size_t k = T.nodes.size() - 1;
+                while (T.nodes[k].left != UINT32_MAX) {
+                    float m = DistCal::InnerProductSIMD16ExtAVX512_(T.nodes[k].hyperplane.data(), qb.data(),
+                                                                    (uint32_t)dl) - T.nodes[k].offset;
+                    if (m < -EPS) k = T.nodes[k].left;
+                    else if (m > EPS) k = T.nodes[k].right;
+                    else { nt++; k = T.nodes[k].left; }
+                }
+                entry[i] = T.nodes[k].indices.empty() ? 0 : T.nodes[k].indices[0];
+            }
+            ties[i] = nt;
+            if (++ver == 0) { std::fill(mark.begin(), mark.end(), 0); ver = 1; }
+            size_t nd = 0;
+            ix->range_search(qb.data(), R2, 0.0f, L, cand, nd, mark.data(), ver, nullptr, 0.5f, false);
+            std::unordered_set<uint32_t> pr;
+            for (uint32_t id : cand) {
+                const float* o = base + (size_t)id * d;
+                float ip = DistCal::InnerProductSIMD16ExtAVX512_(o, qq, d);
+                float on = DistCal::InnerProductSIMD16ExtAVX512_(o, o, d);
+                float qn = DistCal::InnerProductSIMD16ExtAVX512_(qq, qq, d);
+                if (on + qn - 2.0f * ip <= (double)thr[id]) pr.insert(id);
+            }
+            cnt[i] = (uint32_t)cand.size();
+            pred[i] = (uint32_t)pr.size();
+            ndist[i] = (uint32_t)nd;
+        }
+    }
+}

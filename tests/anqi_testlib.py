@@ -96,6 +96,7 @@ REF_SIGS = {
     "ref_tree_nodes": (U64, [P, P]),
     "ref_tree_export": (None, [P, P, P, P, P, U64, P, P]),
     "ref_tree_free": (None, [P]),
+    "ref_rknn_detail": (None, [P, P, U32, P, U32, F64, F64, U64, P, I32, P, P, P, P, P]),
 }
 
 _oracle = None
