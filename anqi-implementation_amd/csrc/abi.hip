@@ -300,6 +300,43 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
 }
 
 // ---- lift (host FP64; src/rknn_query.cpp:2203-2236) ------------------------------------------
+int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
+                   const float* basis, const float* codebook, uint32_t code_size, float tau, float grad_scale,
+                   double* loss, float* grad_basis, float* grad_codebook) {
+    if (n == 0 || n > 0xffffffffull || nk == 0 || nk > 128 || rankm == 0 || rankm > 16 || code_size == 0 ||
+        code_size > 1024 || !(tau > 0.0f))
+        return anqi_set_err(ctx, ANQI_E_USAGE, "lrq_relax: need n > 0, nk <= 128, rankm <= 16, tau > 0");
+    hipStream_t s = ctx->stream;
+    DBuf work, rows;
+    const size_t nkz = (size_t)n * nk, nm = (size_t)n * rankm;
+    HCHK(work.alloc(4 * (5 * nkz + 2 * nm)));
+    HCHK(rows.alloc(4 * (size_t)n));
+    float* w = work.as<float>();
+    RelaxParams P{};
+    P.cut = cutoffs; P.n = (uint32_t)n; P.nk = nk; P.rankm = rankm; P.ncode = code_size;
+    P.mu = mu; P.G = basis; P.cb = codebook;
+    P.inv_tau = 1.0f / tau;
+    P.gscale = grad_scale / (float)((double)n * nk);
+    P.e = w; P.q = w + nkz; P.mx = w + 2 * nkz; P.sm = w + 3 * nkz; P.ge = w + 4 * nkz;
+    P.c = w + 5 * nkz; P.gc = w + 5 * nkz + nm;
+    P.loss_rows = rows.as<float>();
+    P.gG = grad_basis; P.gcb = grad_codebook;
+    if (grad_basis) HCHK(launch_fill_u32((uint32_t*)grad_basis, 0u, (uint64_t)rankm * nk, s));
+    if (grad_codebook) HCHK(launch_fill_u32((uint32_t*)grad_codebook, 0u, (uint64_t)nk * code_size, s));
+    HCHK(launch_relax(P, s));
+    if (loss) {
+        std::vector<float> h(n);
+        HCHK(hipMemcpyAsync(h.data(), rows.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        double t = 0.0;
+        for (float v : h) t += v;
+        *loss = t / ((double)n * nk);
+    } else {
+        HCHK(hipStreamSynchronize(s));
+    }
+    return ANQI_OK;
+}
+
 int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
                    double* mstar2, double* scale) {
     // squared norms summed in row order (the reference's OpenMP reduction

@@ -239,6 +239,22 @@ hipError_t launch_tree_build(const TreeBuildParams& P, hipStream_t s);
 hipError_t launch_tree_entries(const uint32_t* left, const uint32_t* leaf_start, const uint32_t* leaf_size,
                                const uint32_t* perm, uint32_t nodes, uint32_t* entry, hipStream_t s);
 
+// ---- relaxed LRQ objective, forward + backward (k_lrq_relax.hip) ----------------------
+struct RelaxParams {
+    const float* cut;      // n x nk
+    uint32_t n, nk, rankm, ncode;
+    const float* mu;       // nk
+    const float* G;        // rankm x nk
+    const float* cb;       // nk x ncode
+    float inv_tau, gscale;
+    float *c, *gc;         // n x rankm (work)
+    float *e, *q, *mx, *sm, *ge;   // n x nk (work)
+    float* loss_rows;      // n
+    float* gG;             // rankm x nk (nullable, accumulated)
+    float* gcb;            // nk x ncode (nullable, accumulated)
+};
+hipError_t launch_relax(const RelaxParams& P, hipStream_t s);
+
 // ---- device-wide primitives (k_prims.hip, hipCUB) ----------------------------------
 hipError_t prim_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t m, void* tmp,
                                    size_t& tmp_bytes, hipStream_t s);

@@ -76,6 +76,7 @@ SIGS = {
     "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
+    "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, F32, ctypes.POINTER(F64), P, P]),
     "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
     "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32)]),
     "anqi_reverse_csr": (I32, [P, P, U32, U32, P, U32, P, P]),
@@ -286,6 +287,18 @@ class Context:
         t = Tree()
         self.call("anqi_rp_tree", dp(x), n, d, leaf, max_depth, seed, ctypes.byref(t))
         return t
+
+    def lrq_relax(self, cut, mu, basis, cb, tau, grad_scale=1.0, want_loss=True, grads=True):
+        """Relaxed radius-reconstruction loss and, optionally, its gradients
+        (hand-written HIP backward).  Returns (loss or None, gG or None, gcb or None)."""
+        n, nk = cut.shape
+        rankm = basis.shape[0]
+        gG = self.empty((rankm, nk), self.torch.float32) if grads else None
+        gcb = self.empty(tuple(cb.shape), self.torch.float32) if grads else None
+        loss = F64(0.0)
+        self.call("anqi_lrq_relax", dp(cut), n, nk, rankm, dp(mu), dp(basis), dp(cb), cb.shape[1], float(tau),
+                  float(grad_scale), ctypes.byref(loss) if want_loss else None, dp(gG), dp(gcb))
+        return (loss.value if want_loss else None), gG, gcb
 
     def tree_host(self, t):
         """Host copy of a device tree in the reference's node order (children

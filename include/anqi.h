@@ -105,6 +105,20 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
                          const float* basis, const float* codebook, uint32_t code_size,
                          uint32_t k_one_based, double slack, float* thr);
 
+/* ---- relaxed verifier fit (tools/train_final_m8_u8.py:81-156 objective) --
+ * Radius reconstruction with the codeword choice relaxed to a softmax over
+ * the code_size codewords of each rank (temperature tau):
+ *   loss = mean_ik (r_ik - mu_k - sum_m c_im G_mk - sum_j w_ikj cb_kj)^2,
+ *   c_i = (r_i - mu) G^T,  w_ik = softmax_j(-(e_ik - cb_kj)^2 / tau).
+ * All pointers device except *loss (host, may be NULL).  grad_basis
+ * (rankm x nk) and grad_codebook (nk x code_size) receive grad_scale times
+ * the gradient of the mean loss (either may be NULL).  Hand-written backward;
+ * the exact floor decode (anqi_lrq_encode/materialize) judges the result. */
+int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
+                   const float* mu, const float* basis, const float* codebook, uint32_t code_size,
+                   float tau, float grad_scale, double* loss, float* grad_basis,
+                   float* grad_codebook);
+
 /* ---- lift (src/rknn_query.cpp:2203-2236), host FP64 ----------------------- */
 /* host base (n*d), rk_horizon (n); host lifted (n*(d+2)). */
 int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,

@@ -1,0 +1,64 @@
+"""The backward formulas of the relaxed LRQ objective (the ones the HIP
+kernels csrc/k_lrq_relax.hip implement) restated in numpy FP64 and checked
+against torch autograd of the forward on the CPU, and against central finite
+differences.  Pins the derivation independently of the GPU."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+
+def manual_grads(cut, mu, G, cb, tau):
+    y = cut - mu
+    c = y @ G.T
+    e = y - c @ G
+    s = -(e[:, :, None] - cb[None]) ** 2 / tau
+    s -= s.max(axis=2, keepdims=True)
+    w = np.exp(s)
+    w /= w.sum(axis=2, keepdims=True)
+    q = (w * cb[None]).sum(axis=2)
+    var = (w * cb[None] ** 2).sum(axis=2) - q ** 2
+    z = e - q
+    scale = 1.0 / z.size
+    g = 2 * z * scale
+    ge = g * (1 - 2 / tau * var)
+    gcb = (-g[:, :, None] * w * (1 + 2 / tau * (cb[None] - q[:, :, None]) * (e[:, :, None] - cb[None]))).sum(0)
+    gp = -ge
+    gc = gp @ G.T
+    gG = c.T @ gp + gc.T @ y
+    return (z ** 2).mean(), gG, gcb
+
+
+def test_manual_backward_equals_autograd_and_finite_differences():
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "anqi-implementation_amd"))
+    from lrq_relax import relaxed_loss_reference
+    rng = np.random.default_rng(0)
+    n, nk, m, nc = 40, 100, 8, 256
+    cut = np.cumsum(rng.gamma(2.0, 50.0, (n, nk)), axis=1)      # increasing radii per row, like cutoffs
+    cut /= np.sqrt((cut ** 2).mean())                             # the fit works in RMS-normalised units
+    mu = cut.mean(0)
+    u, _, vt = np.linalg.svd(cut - mu, full_matrices=False)
+    G = vt[:m].copy()
+    resid = (cut - mu) - ((cut - mu) @ G.T) @ G
+    cb = np.sort(np.quantile(resid, (np.arange(nc) + 0.5) / nc, axis=0).T, axis=1) + rng.normal(0, 1e-3, (nk, nc))
+    tau = float(np.median(np.diff(cb, axis=1))) ** 2 * 4
+    L, gG, gcb = manual_grads(cut, mu, G, cb, tau)
+    Gt = torch.tensor(G, requires_grad=True)
+    cbt = torch.tensor(cb, requires_grad=True)
+    Lt = relaxed_loss_reference(torch.tensor(cut), torch.tensor(mu), Gt, cbt, tau)
+    Lt.backward()
+    assert abs(L - Lt.item()) <= 1e-12 * abs(L)
+    assert np.abs(gG - Gt.grad.numpy()).max() <= 1e-9 * np.abs(gG).max()
+    assert np.abs(gcb - cbt.grad.numpy()).max() <= 1e-9 * np.abs(gcb).max()
+    eps = 1e-4
+    for (a, b) in [(0, 0), (3, 17), (7, 99), (5, 50)]:
+        P = G.copy(); P[a, b] += eps
+        M_ = G.copy(); M_[a, b] -= eps
+        fd = (manual_grads(cut, mu, P, cb, tau)[0] - manual_grads(cut, mu, M_, cb, tau)[0]) / (2 * eps)
+        assert abs(fd - gG[a, b]) <= 1e-5 * max(1.0, abs(gG[a, b])), (a, b, fd, gG[a, b])
+    for (a, b) in [(0, 0), (10, 128), (99, 255), (50, 3)]:
+        P = cb.copy(); P[a, b] += eps
+        M_ = cb.copy(); M_[a, b] -= eps
+        fd = (manual_grads(cut, mu, G, P, tau)[0] - manual_grads(cut, mu, G, M_, tau)[0]) / (2 * eps)
+        assert abs(fd - gcb[a, b]) <= 1e-5 * max(1.0, abs(gcb[a, b])), (a, b, fd, gcb[a, b])
