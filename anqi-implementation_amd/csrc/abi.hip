@@ -301,8 +301,9 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
 
 // ---- lift (host FP64; src/rknn_query.cpp:2203-2236) ------------------------------------------
 int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
-                   const float* basis, const float* codebook, uint32_t code_size, float tau, float grad_scale,
-                   double* loss, float* grad_basis, float* grad_codebook) {
+                   const float* basis, const float* codebook, uint32_t code_size, float tau, int mode,
+                   float grad_scale, double* loss, float* grad_basis, float* grad_codebook) {
+    if (mode != 0 && mode != 1) return anqi_set_err(ctx, ANQI_E_USAGE, "lrq_relax: mode 0 or 1");
     if (n == 0 || n > 0xffffffffull || nk == 0 || nk > 128 || rankm == 0 || rankm > 16 || code_size == 0 ||
         code_size > 1024 || !(tau > 0.0f))
         return anqi_set_err(ctx, ANQI_E_USAGE, "lrq_relax: need n > 0, nk <= 128, rankm <= 16, tau > 0");
@@ -313,7 +314,7 @@ int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk,
     HCHK(rows.alloc(4 * (size_t)n));
     float* w = work.as<float>();
     RelaxParams P{};
-    P.cut = cutoffs; P.n = (uint32_t)n; P.nk = nk; P.rankm = rankm; P.ncode = code_size;
+    P.cut = cutoffs; P.n = (uint32_t)n; P.nk = nk; P.rankm = rankm; P.ncode = code_size; P.mode = mode;
     P.mu = mu; P.G = basis; P.cb = codebook;
     P.inv_tau = 1.0f / tau;
     P.gscale = grad_scale / (float)((double)n * nk);
@@ -390,22 +391,35 @@ int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, 
     if (sample > 120) sample = 120;
     if (sample < 2 || iters <= 0) { if (passes) *passes = 0; return ANQI_OK; }
     hipStream_t s = ctx->stream;
-    uint64_t maxp = (uint64_t)n * sample * (sample - 1);
+    // proposal buffers: the bound n s (s - 1) is ~6x the count seen in
+    // practice, so start from a fraction of it and regrow (rerunning the
+    // pass, whose proposals depend only on the pass-start heaps) if a pass
+    // reports more
+    const uint64_t bound = (uint64_t)n * sample * (sample - 1);
+    uint64_t maxp = std::min<uint64_t>(bound, std::max<uint64_t>((uint64_t)n * 96, 1u << 20));
     DBuf samp, scnt, k0, k1, v0, v1, cnt, off, updd, tmp;
     HCHK(samp.alloc(4ull * n * sample));
     HCHK(scnt.alloc(4ull * n));
-    HCHK(k0.alloc(8 * maxp));
-    HCHK(k1.alloc(8 * maxp));
-    HCHK(v0.alloc(4 * maxp));
-    HCHK(v1.alloc(4 * maxp));
     HCHK(cnt.alloc(8));
     HCHK(updd.alloc(8));
     HCHK(off.alloc(4ull * (n + 1)));
     int end_bit = 32;
     while (end_bit < 64 && ((uint64_t)n >> (end_bit - 32))) end_bit++;
     size_t tb = 0;
-    HCHK(prim_sort_pairs_u64_u32(nullptr, nullptr, nullptr, nullptr, maxp, end_bit, nullptr, tb, s));
-    HCHK(tmp.alloc(tb));
+    auto grow = [&](uint64_t cap) -> hipError_t {
+        hipError_t e;
+        maxp = cap;
+        if ((e = k0.alloc(8 * maxp)) != hipSuccess) return e;
+        if ((e = k1.alloc(8 * maxp)) != hipSuccess) return e;
+        if ((e = v0.alloc(4 * maxp)) != hipSuccess) return e;
+        if ((e = v1.alloc(4 * maxp)) != hipSuccess) return e;
+        tb = 0;
+        if ((e = prim_sort_pairs_u64_u32(nullptr, nullptr, nullptr, nullptr, maxp, end_bit, nullptr, tb, s)) !=
+            hipSuccess)
+            return e;
+        return tmp.alloc(tb);
+    };
+    HCHK(grow(maxp));
     int it = 0;
     for (; it < iters; it++) {
         HCHK(launch_nnd_sample(hidx, n, cap, (uint32_t)sample, samp.as<uint32_t>(), scnt.as<uint32_t>(), s));
@@ -416,7 +430,12 @@ int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, 
         unsigned long long m = 0;
         HCHK(hipMemcpyAsync(&m, cnt.p, 8, hipMemcpyDeviceToHost, s));
         HCHK(hipStreamSynchronize(s));
-        if (m > maxp) return anqi_set_err(ctx, ANQI_E_DEVICE, "LocalJoin proposal overflow");
+        if (m > maxp) {
+            if (m > bound) return anqi_set_err(ctx, ANQI_E_DEVICE, "LocalJoin proposal count above its bound");
+            HCHK(grow(std::min<uint64_t>(bound, m + m / 8)));
+            it--;
+            continue;
+        }
         if (m == 0) {   // nothing to apply: same stopping rule with zero updates
             if (upd) upd[it] = 0;
             if (0 < (int64_t)(n / 200)) { it++; break; }
@@ -1149,6 +1168,255 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
             if (hst[i] == 1 || hst[i] == 2)
                 return anqi_set_err(ctx, ANQI_E_DEVICE, "query %u exceeds the search work area", i);
     }
+    return ANQI_OK;
+}
+
+}  // extern "C"
+
+// ---- captured query batches (mixed k and L) ---------------------------------------------
+// The index arrays, the k threshold tables and the batch buffers are fixed;
+// the batch size lives in device memory, so one instantiated graph
+// {counter reset -> persistent search -> scan -> compaction} serves every
+// batch up to max_batch.  Search work areas and the persistent grid come from
+// an anqi_search_tuning (anqi_search_autotune measures candidates).
+
+static void fill_search_params(SearchParams& P, const anqi_index* ix) {
+    P = SearchParams{};
+    P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
+    P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
+    P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
+    P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
+    double sc = ix->scale;
+    std::memcpy(&P.scale_bits, &sc, 8);
+    P.R2f = round_down_float(ix->mstar2 + 1.0);
+}
+
+struct anqi_batcher {
+    anqi_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    SearchParams P{};
+    anqi_index ix{};
+    uint32_t max_batch = 0, out_cap = 0, Lmax = 0, grid = 0;
+    DBuf q, Lq, kidx, nq, cand, acc, cnt, nd, pred, status, off, ids, counter;
+    float* hq = nullptr;
+    uint32_t *hL = nullptr, *hk = nullptr, *hnq = nullptr, *hstatus = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~anqi_batcher() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+        if (hq) (void)hipHostFree(hq);
+        if (hL) (void)hipHostFree(hL);
+        if (hk) (void)hipHostFree(hk);
+        if (hnq) (void)hipHostFree(hnq);
+        if (hstatus) (void)hipHostFree(hstatus);
+    }
+};
+
+static hipError_t batch_enqueue(anqi_batcher* b) {
+    hipError_t e;
+    if ((e = launch_fill_u32(b->counter.as<uint32_t>(), 0u, 1, b->s)) != hipSuccess) return e;
+    if ((e = launch_range_search(b->P, true, b->grid, b->s)) != hipSuccess) return e;
+    if ((e = launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(),
+                               b->s)) != hipSuccess)
+        return e;
+    return launch_batch_compact(b->cand.as<uint32_t>(), b->acc.as<uint8_t>(), b->cnt.as<uint32_t>(),
+                                b->off.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->out_cap,
+                                b->ids.as<uint32_t>(), b->s);
+}
+
+extern "C" {
+
+int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables, uint32_t ntables,
+                        uint32_t max_batch, uint32_t Lmax, uint32_t out_cap, const anqi_search_tuning* tune,
+                        anqi_batcher** out) {
+    if (!ix || !ix->tree || ix->M > 64 || ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
+    if (max_batch == 0 || Lmax == 0 || Lmax > 1024 || out_cap == 0 || ntables == 0)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "batcher: need max_batch, 0 < Lmax <= 1024, out_cap, ntables > 0");
+    std::unique_ptr<anqi_batcher> b(new anqi_batcher());
+    b->ctx = ctx;
+    b->ix = *ix;
+    b->max_batch = max_batch; b->out_cap = out_cap; b->Lmax = Lmax;
+    HCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+    const size_t mb = max_batch;
+    HCHK(b->q.alloc(4 * mb * ix->d));
+    HCHK(b->Lq.alloc(4 * mb));
+    HCHK(b->kidx.alloc(4 * mb));
+    HCHK(b->nq.alloc(256));
+    HCHK(b->cand.alloc(4 * mb * out_cap));
+    HCHK(b->acc.alloc(mb * out_cap));
+    HCHK(b->cnt.alloc(4 * mb));
+    HCHK(b->nd.alloc(4 * mb));
+    HCHK(b->pred.alloc(4 * mb));
+    HCHK(b->status.alloc(4 * mb));
+    HCHK(b->off.alloc(4 * (mb + 1)));
+    HCHK(b->ids.alloc(4 * mb * out_cap));
+    HCHK(b->counter.alloc(256));
+    HCHK(hipHostMalloc((void**)&b->hq, 4 * mb * ix->d, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void**)&b->hL, 4 * mb, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void**)&b->hk, 4 * mb, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void**)&b->hnq, 256, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void**)&b->hstatus, 4 * mb, hipHostMallocDefault));
+    SearchParams& P = b->P;
+    fill_search_params(P, ix);
+    P.queries = b->q.as<float>(); P.nq = max_batch; P.nq_dev = b->nq.as<uint32_t>();
+    P.L = Lmax; P.Lmax = Lmax; P.Lq = b->Lq.as<uint32_t>();
+    P.thr = thr_tables; P.kidx = b->kidx.as<uint32_t>();
+    anqi_search_tuning t{0, 0, 0};
+    if (tune) t = *tune;
+    const uint32_t slots = t.visited_slots ? next_pow2(t.visited_slots)
+                                           : std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * Lmax + 1536)));
+    P.hmask = slots - 1;
+    P.rcap = t.result_cap ? t.result_cap : 1024;
+    P.out_cap = out_cap;
+    P.cand_ids = b->cand.as<uint32_t>(); P.accept = b->acc.as<uint8_t>(); P.cand_cnt = b->cnt.as<uint32_t>();
+    P.ndist = b->nd.as<uint32_t>(); P.pred_cnt = b->pred.as<uint32_t>(); P.nexp = nullptr;
+    P.status = b->status.as<uint32_t>(); P.work_counter = b->counter.as<uint32_t>();
+    const int occ = std::max(1, search_persistent_occupancy(P));
+    const uint32_t wpc = t.waves_per_cu ? std::min<uint32_t>(t.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
+    b->grid = wpc * (uint32_t)std::max(1, ctx->cus);
+    // one uncaptured pass (sets kernel attributes, checks the launch), then capture
+    b->hnq[0] = 0;
+    HCHK(hipMemcpyAsync(b->nq.p, b->hnq, 4, hipMemcpyHostToDevice, b->s));
+    HCHK(batch_enqueue(b.get()));
+    HCHK(hipStreamSynchronize(b->s));
+    HCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = batch_enqueue(b.get());
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(b->s, &g);
+    if (ce != hipSuccess) { if (g) (void)hipGraphDestroy(g); return anqi_fail_hip(ctx, ce, "batch capture", __LINE__); }
+    HCHK(ee);
+    b->graph = g;
+    HCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    *out = b.release();
+    return ANQI_OK;
+}
+
+void anqi_batcher_destroy(anqi_batcher* b) { delete b; }
+
+int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_L, const uint32_t* h_kidx,
+                     uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total) {
+    anqi_ctx* ctx = b->ctx;
+    if (nq > b->max_batch) return anqi_set_err(ctx, ANQI_E_USAGE, "batch of %u exceeds capacity %u", nq, b->max_batch);
+    const uint32_t d = b->ix.d;
+    for (uint32_t i = 0; i < nq; i++)
+        if (h_L[i] == 0 || h_L[i] > b->Lmax) return anqi_set_err(ctx, ANQI_E_USAGE, "query %u: L outside 1..Lmax", i);
+    std::memcpy(b->hq, h_queries, 4ull * nq * d);
+    std::memcpy(b->hL, h_L, 4ull * nq);
+    std::memcpy(b->hk, h_kidx, 4ull * nq);
+    b->hnq[0] = nq;
+    hipStream_t s = b->s;
+    HCHK(hipMemcpyAsync(b->q.p, b->hq, 4ull * nq * d, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(b->Lq.p, b->hL, 4ull * nq, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(b->kidx.p, b->hk, 4ull * nq, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(b->nq.p, b->hnq, 4, hipMemcpyHostToDevice, s));
+    HCHK(hipGraphLaunch(b->exec, s));
+    HCHK(hipMemcpyAsync(b->hstatus, b->status.p, 4ull * nq, hipMemcpyDeviceToHost, s));
+    HCHK(hipMemcpyAsync(h_off, b->off.p, 4ull * (nq + 1), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    std::vector<uint32_t> redo;
+    for (uint32_t i = 0; i < nq; i++)
+        if (b->hstatus[i] != 0) redo.push_back(i);
+    if (!redo.empty()) {
+        // queries that outgrew the tuned work areas: rerun with 16K-entry areas
+        // (uncaptured), then rebuild the answer lists
+        SearchParams Q = b->P;
+        Q.nq_dev = nullptr;
+        Q.hmask = 16383;
+        Q.rcap = 16384;
+        DBuf ql;
+        HCHK(ql.alloc(4 * redo.size()));
+        HCHK(hipMemcpyAsync(ql.p, redo.data(), 4 * redo.size(), hipMemcpyHostToDevice, s));
+        Q.qlist = ql.as<uint32_t>();
+        Q.nq = (uint32_t)redo.size();
+        HCHK(launch_range_search(Q, false, 0, s));
+        HCHK(launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(), s));
+        HCHK(launch_batch_compact(b->cand.as<uint32_t>(), b->acc.as<uint8_t>(), b->cnt.as<uint32_t>(),
+                                  b->off.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->out_cap,
+                                  b->ids.as<uint32_t>(), s));
+        HCHK(hipMemcpyAsync(b->hstatus, b->status.p, 4ull * nq, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(h_off, b->off.p, 4ull * (nq + 1), hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        for (uint32_t i : redo)
+            if (b->hstatus[i] != 0)
+                return anqi_set_err(ctx, ANQI_E_DEVICE, "query %u exceeds the search work area", i);
+    }
+    *total = h_off[nq];
+    if (*total > ids_cap) return anqi_set_err(ctx, ANQI_E_USAGE, "answers need %llu ids", (unsigned long long)*total);
+    if (*total) {
+        HCHK(hipMemcpyAsync(h_ids, b->ids.p, 4ull * *total, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+    }
+    return ANQI_OK;
+}
+
+int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables, const float* queries,
+                         uint32_t nq, const uint32_t* Lq, const uint32_t* kidx, uint32_t Lmax,
+                         anqi_search_tuning* out, double* best_ms) {
+    if (!ix || !ix->tree || nq == 0 || Lmax == 0 || Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "autotune: bad input");
+    hipStream_t s = ctx->stream;
+    DBuf cand, acc, cnt, nd, pred, status, counter;
+    const uint32_t out_cap = 4096;
+    HCHK(cand.alloc(4ull * nq * out_cap));
+    HCHK(acc.alloc(1ull * nq * out_cap));
+    HCHK(cnt.alloc(4ull * nq));
+    HCHK(nd.alloc(4ull * nq));
+    HCHK(pred.alloc(4ull * nq));
+    HCHK(status.alloc(4ull * nq));
+    HCHK(counter.alloc(256));
+    SearchParams P;
+    fill_search_params(P, ix);
+    P.queries = queries; P.nq = nq; P.L = Lmax; P.Lmax = Lmax; P.Lq = Lq; P.thr = thr_tables; P.kidx = kidx;
+    P.out_cap = out_cap;
+    P.cand_ids = cand.as<uint32_t>(); P.accept = acc.as<uint8_t>(); P.cand_cnt = cnt.as<uint32_t>();
+    P.ndist = nd.as<uint32_t>(); P.pred_cnt = pred.as<uint32_t>(); P.status = status.as<uint32_t>();
+    P.work_counter = counter.as<uint32_t>();
+    hipEvent_t e0, e1;
+    HCHK(hipEventCreate(&e0));
+    HCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    anqi_search_tuning bt{0, 0, 0};
+    const uint32_t slot_opts[3] = {2048, 4096, 8192};
+    const uint32_t rcap_opts[2] = {512, 1024};
+    std::vector<uint32_t> st(nq);
+    for (uint32_t so : slot_opts) {
+        if (so < 2 * Lmax) continue;
+        for (uint32_t rc : rcap_opts) {
+            P.hmask = so - 1;
+            P.rcap = rc;
+            const int occ = std::max(1, search_persistent_occupancy(P));
+            for (int frac = 2; frac <= 4; frac++) {             // 1/2, 3/4, 4/4 of the resident waves
+                uint32_t wpc = std::max<uint32_t>(1, (uint32_t)(occ * frac / 4));
+                if (frac < 4 && wpc == (uint32_t)occ) continue;
+                const uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
+                double tot = 0.0;
+                for (int rep = 0; rep < 3; rep++) {
+                    HCHK(launch_fill_u32(counter.as<uint32_t>(), 0u, 1, s));
+                    HCHK(hipEventRecord(e0, s));
+                    HCHK(launch_range_search(P, true, grid, s));
+                    HCHK(hipEventRecord(e1, s));
+                    HCHK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep) tot += ms;            // first pass warms the caches
+                }
+                HCHK(hipMemcpyAsync(st.data(), status.p, 4ull * nq, hipMemcpyDeviceToHost, s));
+                HCHK(hipStreamSynchronize(s));
+                uint32_t bad = 0;
+                for (uint32_t v : st) bad += v != 0;
+                // overflowing queries are rerun uncaptured; charge them
+                const double cost = tot / 2.0 * (1.0 + 50.0 * (double)bad / nq);
+                progress("autotune: slots %u fifo %u waves/CU %u -> %.3f ms (%u overflow)", so, rc, wpc, tot / 2.0, bad);
+                if (cost < best) { best = cost; bt = anqi_search_tuning{wpc, so, rc}; }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *out = bt;
+    if (best_ms) *best_ms = best;
     return ANQI_OK;
 }
 

@@ -309,12 +309,13 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
     extern __shared__ __align__(16) char lds[];
     SearchLds S = carve(lds, P);
     __shared__ uint32_t next;
+    const uint32_t nq = P.nq_dev ? *P.nq_dev : P.nq;
     for (;;) {
         if (__lane_id() == 0) next = atomicAdd(P.work_counter, 1u);
         __syncthreads();
         uint32_t t = next;
         __syncthreads();
-        if (t >= P.nq) break;
+        if (t >= nq) break;
         uint32_t qi = P.qlist ? P.qlist[t] : t;
         search_one<MC>(P, S, qi);
         __syncthreads();

@@ -40,11 +40,20 @@ struct SearchParams {
     uint32_t* nexp;         // nq: expanded nodes (optional)
     uint32_t* status;       // nq: 0 ok, 1 hash full, 2 FIFO full, 3 output truncated
     uint32_t* work_counter; // persistent form
+    const uint32_t* nq_dev; // optional: batch size read on the device (captured batches)
 };
 size_t search_lds_bytes(const SearchParams& P);
 int search_persistent_occupancy(const SearchParams& P);
 hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// Batch result compaction (k_batch.hip): off = exclusive scan of pred_cnt over
+// *nq_dev queries (one block), then each query's accepted ids in candidate
+// order into ids[off[q] ..].  Capturable (no host reads).
+hipError_t launch_batch_scan(const uint32_t* pred_cnt, const uint32_t* nq_dev, uint32_t max_nq, uint32_t* off,
+                             hipStream_t s);
+hipError_t launch_batch_compact(const uint32_t* cand_ids, const uint8_t* accept, const uint32_t* cand_cnt,
+                                const uint32_t* off, const uint32_t* nq_dev, uint32_t max_nq, uint32_t out_cap,
+                                uint32_t* ids, hipStream_t s);
 
 // ---- verifier (k_lrq.hip) ---------------------------------------------------
 hipError_t launch_lrq_encode(const float* cut, uint64_t n, uint32_t nk, uint32_t rankm,
@@ -243,12 +252,13 @@ hipError_t launch_tree_entries(const uint32_t* left, const uint32_t* leaf_start,
 struct RelaxParams {
     const float* cut;      // n x nk
     uint32_t n, nk, rankm, ncode;
+    int mode;              // 0 softmax (nearest), 1 sigmoid (floor)
     const float* mu;       // nk
     const float* G;        // rankm x nk
     const float* cb;       // nk x ncode
     float inv_tau, gscale;
     float *c, *gc;         // n x rankm (work)
-    float *e, *q, *mx, *sm, *ge;   // n x nk (work)
+    float *e, *q, *mx, *sm, *ge;   // n x nk (work; q holds z = e - q)
     float* loss_rows;      // n
     float* gG;             // rankm x nk (nullable, accumulated)
     float* gcb;            // nk x ncode (nullable, accumulated)

@@ -50,6 +50,10 @@ class GraphStats(ctypes.Structure):
                 ("overflow_rows", U32)]
 
 
+class SearchTuning(ctypes.Structure):
+    _fields_ = [("waves_per_cu", U32), ("visited_slots", U32), ("result_cap", U32)]
+
+
 class Index(ctypes.Structure):
     _fields_ = [("base", P), ("bnorm", P), ("lifted", P), ("deg", P), ("links", P), ("n", U32), ("d", U32),
                 ("dl", U32), ("M", U32), ("tree", ctypes.POINTER(Tree)), ("mstar2", F64), ("scale", F64)]
@@ -76,7 +80,7 @@ SIGS = {
     "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
-    "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, F32, ctypes.POINTER(F64), P, P]),
+    "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, I32, F32, ctypes.POINTER(F64), P, P]),
     "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
     "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32)]),
     "anqi_reverse_csr": (I32, [P, P, U32, U32, P, U32, P, P]),
@@ -91,6 +95,12 @@ SIGS = {
     "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, P, I32]),
     "anqi_rknn_search_entries": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, P, U32, P, P, P, P, P, P,
                                        P, I32]),
+    "anqi_batcher_create": (I32, [P, ctypes.POINTER(Index), P, U32, U32, U32, U32, ctypes.POINTER(SearchTuning),
+                                  ctypes.POINTER(P)]),
+    "anqi_batcher_run": (I32, [P, P, P, P, U32, P, P, U64, ctypes.POINTER(U64)]),
+    "anqi_batcher_destroy": (None, [P]),
+    "anqi_search_autotune": (I32, [P, ctypes.POINTER(Index), P, P, U32, P, P, U32, ctypes.POINTER(SearchTuning),
+                                   ctypes.POINTER(F64)]),
     "anqi_build_info": (ctypes.c_char_p, []),
 }
 
@@ -288,17 +298,28 @@ class Context:
         self.call("anqi_rp_tree", dp(x), n, d, leaf, max_depth, seed, ctypes.byref(t))
         return t
 
-    def lrq_relax(self, cut, mu, basis, cb, tau, grad_scale=1.0, want_loss=True, grads=True):
-        """Relaxed radius-reconstruction loss and, optionally, its gradients
-        (hand-written HIP backward).  Returns (loss or None, gG or None, gcb or None)."""
+    def lrq_relax(self, cut, mu, basis, cb, tau, mode=0, grad_scale=1.0, want_loss=True, grads=True):
+        """Relaxed radius-reconstruction loss (mode 0 softmax/nearest, 1
+        sigmoid/floor) and, optionally, its gradients (hand-written HIP
+        backward).  Returns (loss or None, gG or None, gcb or None)."""
         n, nk = cut.shape
         rankm = basis.shape[0]
         gG = self.empty((rankm, nk), self.torch.float32) if grads else None
         gcb = self.empty(tuple(cb.shape), self.torch.float32) if grads else None
         loss = F64(0.0)
         self.call("anqi_lrq_relax", dp(cut), n, nk, rankm, dp(mu), dp(basis), dp(cb), cb.shape[1], float(tau),
-                  float(grad_scale), ctypes.byref(loss) if want_loss else None, dp(gG), dp(gcb))
+                  int(mode), float(grad_scale), ctypes.byref(loss) if want_loss else None, dp(gG), dp(gcb))
         return (loss.value if want_loss else None), gG, gcb
+
+    def search_autotune(self, ix, thr_tables, queries, Lq, kidx, Lmax):
+        t = SearchTuning()
+        ms = F64(0.0)
+        self.call("anqi_search_autotune", ctypes.byref(ix), dp(thr_tables), dp(queries), queries.shape[0], dp(Lq),
+                  dp(kidx), Lmax, ctypes.byref(t), ctypes.byref(ms))
+        return t, ms.value
+
+    def batcher(self, ix, thr_tables, max_batch, Lmax, out_cap=4096, tuning=None):
+        return Batcher(self, ix, thr_tables, max_batch, Lmax, out_cap, tuning)
 
     def tree_host(self, t):
         """Host copy of a device tree in the reference's node order (children
@@ -387,3 +408,50 @@ def lift_host(base: np.ndarray, rk_horizon: np.ndarray):
     if rc != 0:
         raise AnqiError(rc, "lift failed")
     return out, m2.value, sc.value
+
+
+class Batcher:
+    """anqi_batcher: one captured graph serving query batches of mixed k
+    (threshold table index) and L, answers returned as host CSR."""
+
+    def __init__(self, ctx, ix, thr_tables, max_batch, Lmax, out_cap=4096, tuning=None):
+        self.ctx = ctx
+        self.keep = (ix, thr_tables)
+        self.h = P()
+        self.max_batch = max_batch
+        self.out_cap = out_cap
+        ctx.check(ctx.L.anqi_batcher_create(ctx.ctx, ctypes.byref(ix), dp(thr_tables), thr_tables.shape[0],
+                                            max_batch, Lmax, out_cap,
+                                            ctypes.byref(tuning) if tuning is not None else None,
+                                            ctypes.byref(self.h)))
+        self.ids = np.zeros(max_batch * 256, np.uint32)
+
+    def run(self, queries, L, kidx):
+        """queries: host (nq, d) float32; L, kidx: host uint32 (nq,).
+        Returns (off[nq+1], ids[off[-1]])."""
+        q = np.ascontiguousarray(queries, np.float32)
+        Lh = np.ascontiguousarray(L, np.uint32)
+        kh = np.ascontiguousarray(kidx, np.uint32)
+        nq = q.shape[0]
+        off = np.zeros(nq + 1, np.uint32)
+        tot = U64(0)
+        for _ in range(2):
+            rc = self.ctx.L.anqi_batcher_run(self.h, hp(q), hp(Lh), hp(kh), nq, hp(off), hp(self.ids),
+                                             self.ids.size, ctypes.byref(tot))
+            if rc == 2 and tot.value > self.ids.size:       # answers outgrew the host buffer
+                self.ids = np.zeros(int(tot.value) * 2, np.uint32)
+                continue
+            self.ctx.check(rc)
+            break
+        return off, self.ids[:int(tot.value)].copy()
+
+    def close(self):
+        if self.h:
+            self.ctx.L.anqi_batcher_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -110,13 +110,15 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
  * the code_size codewords of each rank (temperature tau):
  *   loss = mean_ik (r_ik - mu_k - sum_m c_im G_mk - sum_j w_ikj cb_kj)^2,
  *   c_i = (r_i - mu) G^T,  w_ik = softmax_j(-(e_ik - cb_kj)^2 / tau).
+ * mode 1 relaxes the floor decode itself (sorted codewords):
+ *   q = cb_0 + sum_{j>=1} logistic((e - cb_j)/tau) (cb_j - cb_{j-1}).
  * All pointers device except *loss (host, may be NULL).  grad_basis
  * (rankm x nk) and grad_codebook (nk x code_size) receive grad_scale times
  * the gradient of the mean loss (either may be NULL).  Hand-written backward;
  * the exact floor decode (anqi_lrq_encode/materialize) judges the result. */
 int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
                    const float* mu, const float* basis, const float* codebook, uint32_t code_size,
-                   float tau, float grad_scale, double* loss, float* grad_basis,
+                   float tau, int mode, float grad_scale, double* loss, float* grad_basis,
                    float* grad_codebook);
 
 /* ---- lift (src/rknn_query.cpp:2203-2236), host FP64 ----------------------- */
@@ -222,6 +224,41 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
                              uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt,
                              uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
                              uint32_t* status, int mode);
+
+/* ---- captured query batches (configuration 5: mixed k and L) -------------
+ * No counterpart in the reference (its query driver loops over queries on
+ * CPU threads, src/rknn_query.cpp:2692-2824): this is the batch form of the
+ * same range search + post-filter, for serving.
+ * A batcher owns device buffers for up to max_batch queries and an
+ * instantiated hipGraph {counter reset, persistent search, answer scan,
+ * answer compaction}; the batch size is read on the device, so the same graph
+ * serves every batch.  thr_tables: device, ntables x n thresholds (one table
+ * per k, from anqi_lrq_materialize); a query's kidx selects its table and its
+ * L (1..Lmax) its beam width.  anqi_batcher_run copies host queries in, runs
+ * the graph, reruns (uncaptured) the rare queries that outgrow the tuned work
+ * areas, and returns the accepted ids of every query as host CSR
+ * (h_off[nq+1], h_ids[*total], candidate order). The index and tables must
+ * outlive the batcher; one thread per batcher. */
+typedef struct {
+    uint32_t waves_per_cu;    /* persistent grid = waves_per_cu x CUs (0: occupancy) */
+    uint32_t visited_slots;   /* LDS visited-set slots (0: from Lmax) */
+    uint32_t result_cap;      /* LDS candidate FIFO entries (0: 1024) */
+} anqi_search_tuning;
+typedef struct anqi_batcher anqi_batcher;
+int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables, uint32_t ntables,
+                        uint32_t max_batch, uint32_t Lmax, uint32_t out_cap,
+                        const anqi_search_tuning* tune, anqi_batcher** out);
+int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_L, const uint32_t* h_kidx,
+                     uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total);
+void anqi_batcher_destroy(anqi_batcher* b);
+/* Times the persistent search on a sample batch (device queries, Lq, kidx)
+ * for visited-set sizes {2048, 4096, 8192}, FIFO sizes {512, 1024} and
+ * 1/2, 3/4, 4/4 of the resident waves per CU; queries that overflow are
+ * charged as reruns.  Writes the fastest tuning and its time. */
+int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables,
+                         const float* queries, uint32_t nq, const uint32_t* Lq,
+                         const uint32_t* kidx, uint32_t Lmax, anqi_search_tuning* out,
+                         double* best_ms);
 
 /* Library build description (compiler, arch). */
 const char* anqi_build_info(void);

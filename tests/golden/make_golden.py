@@ -17,7 +17,7 @@ nothing else).  Outputs are data only (inputs and expected outputs):
             reference recall numbers only (AKNN recall vs exact, rknn_query
             recall/precision per L at k=10) -- the recall-parity targets.
 
-Usage: python3 tests/golden/make_golden.py [--only sample|mix20k]
+Usage: python3 tests/golden/make_golden.py [--only sample|mix20k|mix1m] [--work DIR]
 """
 from __future__ import annotations
 
@@ -269,9 +269,45 @@ def make_mix(out_dir, n=20000, nq=200):
     print("[golden] mix20k reference numbers written to", out_dir, res["aknn_recall_at_100"])
 
 
+def make_mix1m(out_dir, work_root=None):
+    """The reference's own pipeline on the full 1,000,000 x 128 mixture
+    (10,000 queries, k = 10): recall / precision / candidate / distance
+    numbers per L, lift meta and build seconds.  About 30 minutes on 8 host
+    threads; work_root must hold ~3 GB."""
+    import re
+    os.makedirs(out_dir, exist_ok=True)
+    base = workload.million_base(1_000_000)
+    query = workload.million_query(10_000)
+    with tempfile.TemporaryDirectory(dir=work_root) as work:
+        pfx = os.path.join(work, "mix")
+        thr, runroot, outs = pipeline(pfx, base, query, ["100", "500", "64", "600", "8", "10", "l2"],
+                                      8, [8, 16, 32, 64, 128, 256, 1024], [10], work)
+        out = outs[10]
+        pts = []
+        for m in re.finditer(r"\[rknn\] L_descent=(\d+)\s+\| recall=([\d.]+) precision=([\d.]+) F1=([\d.]+) \| "
+                             r"avg cand=(\d+) pred=(\d+) #dist=(\d+) \| QPS=(\d+)", out):
+            L, r, p, f, c, pr, d, q = m.groups()
+            pts.append(dict(L=int(L), recall=float(r), precision=float(p), F1=float(f), cand=int(c), pred=int(pr),
+                            dist=int(d), qps_8_threads=int(q)))
+        lift = re.search(r"BUILD lift M\*\^2=([\d.]+) scale=([\d.]+)", out).groups()
+        meta = open(pfx + "_aknn_build_time.meta").read().split("\n")
+        res = dict(n=1_000_000, nq=10_000, k=10, base_sha256=workload.sha256(base),
+                   query_sha256=workload.sha256(query), aknn_args=["100", "500", "64", "600", "8", "10", "l2"],
+                   threads=8,
+                   aknn_build_seconds_8_threads=float([l for l in meta if l.startswith("seconds")][0].split()[1]),
+                   lift_mstar2=float(lift[0]), lift_scale=float(lift[1]),
+                   build_breakdown=re.search(r"BUILD_BREAKDOWN .*", out).group(0), points_k10=pts,
+                   note="reference tools compiled from the reference sources (oracle/Makefile 'ref'), run by "
+                        "pipeline() with the run_final.sh environment pins; 8 host threads")
+        with open(os.path.join(out_dir, "mix1m_reference.json"), "w") as f:
+            json.dump(res, f, indent=1)
+    print("[golden] mix1m reference numbers written to", out_dir)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", choices=["sample", "mix20k"])
+    ap.add_argument("--only", choices=["sample", "mix20k", "mix1m"])
+    ap.add_argument("--work", default=None, help="scratch directory for the 1M run")
     a = ap.parse_args()
     if not os.path.exists(os.path.join(REF_DIR, "libref_harness.so")):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -279,6 +315,8 @@ def main():
         make_sample(os.path.join(HERE, "sample"))
     if a.only in (None, "mix20k"):
         make_mix(os.path.join(HERE, "mix20k"))
+    if a.only == "mix1m":
+        make_mix1m(os.path.join(HERE, "mix1m"), a.work)
 
 
 if __name__ == "__main__":

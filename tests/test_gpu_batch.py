@@ -1,0 +1,84 @@
+"""Captured query batches (configuration 5 on a small index): batches of
+mixed k (threshold table per query) and uneven L, each served by one replay
+of the instantiated graph, return exactly the accepted ids of the ordinary
+per-call search; batches of different sizes replay back to back with eager
+work in between; the autotuned work areas give the same answers."""
+import numpy as np
+import pytest
+
+import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-batch]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    assert torch.cuda.is_available()
+    import pipeline
+    import pyanqi
+    ctx = pyanqi.Context(0)
+    base = workload.mixture(30000, 96, 120, 15.0, seed=21)
+    ix = pipeline.build(ctx, base, log=say)
+    tables = torch.stack([pipeline.thresholds(ctx, ix, k) for k in (10, 50, 100)]).contiguous()
+    q = workload.mixture(20000, 96, 120, 15.0, seed=22)
+    yield dict(ctx=ctx, ix=ix, tables=tables, q=q, torch=torch)
+    ctx.tree_free(ix.tree)
+    ctx.close()
+
+
+def expected(env, q, L, kidx):
+    ctx, torch = env["ctx"], env["torch"]
+    nq = q.shape[0]
+    Lt = ctx.to_dev(L.astype(np.int32))
+    kt = ctx.to_dev(kidx.astype(np.int32))
+    b = ctx.rknn_search(env["ix"].handle, ctx.to_dev(q), int(L.max()), env["tables"], out_cap=4096, mode=1,
+                        Lq=Lt, Lmax=int(L.max()), kidx=kt)
+    cnt, cand, acc = b["cnt"].cpu().numpy(), b["cand"].cpu().numpy().view(np.uint32), b["acc"].cpu().numpy()
+    off = [0]
+    ids = []
+    for i in range(nq):
+        c = int(cnt[i])
+        ids.extend(cand[i, :c][acc[i, :c] != 0].tolist())
+        off.append(len(ids))
+    return np.array(off, np.uint32), np.array(ids, np.uint32)
+
+
+def test_batches_match_per_call_search(env):
+    ctx = env["ctx"]
+    rng = np.random.default_rng(3)
+    bat = ctx.batcher(env["ix"].handle, env["tables"], max_batch=4096, Lmax=512)
+    pos = 0
+    for size in (4096, 512, 1777, 3000, 600):
+        q = env["q"][pos:pos + size]
+        pos += size
+        L = rng.integers(64, 513, size).astype(np.uint32)
+        kidx = rng.integers(0, 3, size).astype(np.uint32)
+        off, ids = bat.run(q, L, kidx)
+        eoff, eids = expected(env, q, L, kidx)
+        say(f"batch {size}: {len(ids)} answers")
+        assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+        env["torch"].cuda.synchronize()
+        _ = (env["tables"] * 2).sum().item()        # eager work between replays
+    bat.close()
+
+
+def test_autotuned_batcher_same_answers(env):
+    ctx = env["ctx"]
+    rng = np.random.default_rng(4)
+    n = 2048
+    q = env["q"][:n]
+    L = rng.integers(64, 513, n).astype(np.uint32)
+    kidx = rng.integers(0, 3, n).astype(np.uint32)
+    tune, ms = ctx.search_autotune(env["ix"].handle, env["tables"], ctx.to_dev(q), ctx.to_dev(L.astype(np.int32)),
+                                   ctx.to_dev(kidx.astype(np.int32)), 512)
+    say(f"autotune: waves/CU {tune.waves_per_cu} slots {tune.visited_slots} fifo {tune.result_cap} ({ms:.3f} ms)")
+    bat = ctx.batcher(env["ix"].handle, env["tables"], max_batch=n, Lmax=512, tuning=tune)
+    off, ids = bat.run(q, L, kidx)
+    eoff, eids = expected(env, q, L, kidx)
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+    bat.close()

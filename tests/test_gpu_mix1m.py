@@ -1,0 +1,47 @@
+"""The headline workload end to end: the chosen GPU pipeline on the
+1,000,000 x 128 mixture (10,000 queries, k = 10) against the reference's own
+pipeline on the same inputs (tests/golden/mix1m, reference tools compiled from
+the reference sources): recall and precision within 0.5 points at every
+swept L, and the lift meta within rounding of the reference's."""
+import json
+import os
+
+import pytest
+
+from anqi_testlib import GOLDEN
+import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-mix1m]", *a, flush=True)
+
+
+def test_recall_precision_per_L_match_reference():
+    import torch
+    import pipeline
+    import pyanqi
+    ref = json.load(open(os.path.join(GOLDEN, "mix1m", "mix1m_reference.json")))
+    base = workload.million_base(ref["n"])
+    q = workload.million_query(ref["nq"])
+    assert workload.sha256(base) == ref["base_sha256"] and workload.sha256(q) == ref["query_sha256"]
+    ctx = pyanqi.Context(0)
+    ix = pipeline.build(ctx, base, precision=1, log=say)
+    say(f"lift M*^2 {ix.mstar2:.6g} (reference {ref['lift_mstar2']}), scale {ix.scale:.4g} "
+        f"(reference {ref['lift_scale']})")
+    assert abs(ix.mstar2 - ref["lift_mstar2"]) <= 0.01 * ref["lift_mstar2"]
+    assert abs(ix.scale - ref["lift_scale"]) <= 1e-6
+    qd = ctx.to_dev(q)
+    off, gids, _ = pipeline.ground_truth(ctx, ix.base, qd, ref["k"])
+    thr = pipeline.thresholds(ctx, ix, ref["k"])
+    bufs = ctx.search_buffers(ref["nq"], 4096)
+    for p in ref["points_k10"]:
+        ctx.rknn_search(ix.handle, qd, p["L"], thr, mode=1, bufs=bufs)
+        m = pipeline.metrics(bufs, off, gids)
+        say(f"L={p['L']}: recall {m['recall']:.4f} (reference {p['recall']:.4f}), precision {m['precision']:.4f} "
+            f"({p['precision']:.4f}), cand {m['cand']:.1f} ({p['cand']}), dist {m['dist']:.0f} ({p['dist']})")
+        assert abs(m["recall"] - p["recall"]) <= 0.005, p["L"]
+        assert abs(m["precision"] - p["precision"]) <= 0.005, p["L"]
+    ctx.tree_free(ix.tree)
+    ctx.close()

@@ -32,24 +32,33 @@ def env():
     ctx.close()
 
 
-def test_hip_backward_matches_fp64_reference_and_finite_differences(env):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_hip_backward_matches_fp64_reference_and_finite_differences(env, mode):
     from lrq_relax import default_tau, relaxed_loss_reference
     torch, ctx = env["torch"], env["ctx"]
     sc = float(torch.sqrt((env["cut"].float() ** 2).mean()))
     sub = (env["cut"][:1000] / sc).contiguous()
     mu, G, cb = (env["mu"] / sc).contiguous(), env["G"], (env["cb"] / sc).contiguous()
-    tau = 4.0 * default_tau(cb)
-    loss, gG, gcb = ctx.lrq_relax(sub, mu, G, cb, tau)
+    # radii in RMS units; tau = 1e-4 (softmax length scale 0.01, the start of
+    # the annealing schedule) keeps eps = 1e-4 differences inside the smooth
+    # range; evaluated away from the Lloyd/PCA optimum, where the gradient
+    # is not a small difference of large terms
+    tau = 1e-4 if mode == 0 else 1e-2       # softmax: tau = width^2; floor: logistic width
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    G = (G + 0.02 * torch.randn(G.shape, generator=gen).to(G.device)).contiguous()
+    cb = torch.sort(cb + 0.5 * default_tau(cb) ** 0.5 * torch.randn(cb.shape, generator=gen).to(cb.device),
+                    dim=1).values.contiguous()
+    loss, gG, gcb = ctx.lrq_relax(sub, mu, G, cb, tau, mode=mode)
     S64, mu64 = sub.double().cpu(), mu.double().cpu()
     G64 = G.double().cpu().requires_grad_(True)
     cb64 = cb.double().cpu().requires_grad_(True)
-    L = relaxed_loss_reference(S64, mu64, G64, cb64, tau)
+    L = relaxed_loss_reference(S64, mu64, G64, cb64, tau, mode=mode)
     L.backward()
     rg = float((gG.double().cpu() - G64.grad).norm() / G64.grad.norm())
     rc = float((gcb.double().cpu() - cb64.grad).norm() / cb64.grad.norm())
-    say(f"loss hip {loss:.9g} fp64 {L.item():.9g}; grad rel err basis {rg:.2e} codebook {rc:.2e}")
+    say(f"mode {mode}: loss hip {loss:.9g} fp64 {L.item():.9g}; grad rel err basis {rg:.2e} codebook {rc:.2e}")
     assert abs(loss - L.item()) <= 1e-5 * L.item()
-    assert rg <= 1e-4 and rc <= 1e-4
+    assert rg <= 2e-4 and rc <= 2e-4
     rng = np.random.default_rng(1)
     eps = 1e-4
     worst = 0.0
@@ -62,8 +71,8 @@ def test_hip_backward_matches_fp64_reference_and_finite_differences(env):
                 Pm[idx] -= eps
                 args_p = (Pp, cb64.detach()) if which == "G" else (G64.detach(), Pp)
                 args_m = (Pm, cb64.detach()) if which == "G" else (G64.detach(), Pm)
-                fd = (relaxed_loss_reference(S64, mu64, *args_p, tau) -
-                      relaxed_loss_reference(S64, mu64, *args_m, tau)).item() / (2 * eps)
+                fd = (relaxed_loss_reference(S64, mu64, *args_p, tau, mode=mode) -
+                      relaxed_loss_reference(S64, mu64, *args_m, tau, mode=mode)).item() / (2 * eps)
                 g = float(grad[idx])
                 scale = float(grad.abs().max())
                 err = abs(fd - g) / scale
@@ -77,12 +86,12 @@ def test_relaxed_fit_then_exact_floor_decode_matches_reference(env):
     torch, ctx = env["torch"], env["ctx"]
     cut, mu, G0, cb0 = env["cut"], env["mu"], env["G"], env["cb"]
     G, cb, hist, secs = fit_relaxed(ctx, cut, mu, G0, cb0, iters=120, log=say)
-    say(f"relaxed loss {hist[0]:.6g} -> {hist[-1]:.6g} in {secs:.2f}s")
-    assert hist[-1] < hist[0]
+    say(f"relaxed loss {hist[0]:.6g} -> min {min(hist):.6g} in {secs:.2f}s")
+    assert min(hist) < hist[0]
     r0 = floor_rmse(ctx, cut, mu, G0, cb0)
     r1 = floor_rmse(ctx, cut, mu, G, cb)
     say(f"floor-decoded radius RMSE: Lloyd fit {r0:.6g}, relaxed fit {r1:.6g}")
-    assert np.isfinite(r1) and r1 <= 1.25 * r0
+    assert np.isfinite(r1) and r1 <= r0 * (1 + 1e-6)
     assert bool((torch.diff(cb, dim=1) >= 0).all())
     if not ref_available():
         pytest.skip("reference harness not built")
