@@ -81,7 +81,8 @@ int main(int argc, char** argv) {
         std::cerr << "[aknn] device allocation failed: " << anqi_last_error(ctx) << "\n";
         return 3;
     }
-    anqi_aknn_params p{(uint32_t)k, (uint32_t)leaf, (uint32_t)trees, (uint32_t)ef, 8, 100, seed, 0};
+    anqi_aknn_params p{(uint32_t)k, (uint32_t)leaf, (uint32_t)trees, (uint32_t)ef, 8, 100, seed,
+                       anqi_cli::gpu_form_chosen() ? 1 : 0};
     anqi_aknn_stats st{};
     int rc = anqi_aknn_build(ctx, (const float*)dx, n, d, &p, (uint32_t*)dids, (float*)dkeys, &st);
     if (rc != ANQI_OK) {

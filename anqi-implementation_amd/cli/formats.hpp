@@ -19,6 +19,15 @@
 
 namespace anqi_cli {
 
+// GPU form of the kernels (same outputs either way): ANQI_GPU_FORM=plain
+// selects the FP32 sweeps, per-chunk gathers and per-query search launch;
+// anything else (default) the bf16-MFMA sweeps, batched gathers and
+// persistent register-beam search.
+inline bool gpu_form_chosen() {
+    const char* v = std::getenv("ANQI_GPU_FORM");
+    return !(v && std::strcmp(v, "plain") == 0);
+}
+
 [[noreturn]] inline void die(int code, const char* fmt, const char* a = "", const char* b = "") {
     std::fprintf(stderr, fmt, a, b);
     std::fprintf(stderr, "\n");

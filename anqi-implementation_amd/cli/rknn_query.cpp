@@ -395,7 +395,7 @@ int main(int argc, char** argv) {
 
     // graph + search tree (index.h:1071-1295)
     const auto tg0 = std::chrono::steady_clock::now();
-    anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10};
+    anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10, anqi_cli::gpu_form_chosen() ? 1 : 0};
     anqi_graph_stats gst{};
     anqi_tree tree{};
     uint32_t* d_links = dalloc<uint32_t>((size_t)n * Marg);
@@ -447,7 +447,8 @@ int main(int argc, char** argv) {
     const double min_repeat_s = positive_seconds_env("ANQI_BENCH_MIN_REPEAT_S", 10.0);
     std::printf("[rknn] benchmark repeats=%zu calibrated=%s (timing uses steady_clock sub-millisecond precision)\n",
                 bench_repeats, calibrated ? "on" : "off");
-    std::printf("[rknn] query parallelism: nq=%u device=MI355X mode=persistent\n", nq);
+    std::printf("[rknn] query parallelism: nq=%u device=MI355X mode=%s\n", nq,
+                anqi_cli::gpu_form_chosen() ? "persistent" : "per-query");
     if (calibrated)
         std::printf("[rknn] calibrated timing: min_total_s=%.3f min_repeat_s=%.3f windows=%zu\n", min_total_s,
                     min_repeat_s, bench_repeats);
@@ -477,7 +478,8 @@ int main(int argc, char** argv) {
     auto search = [&](size_t L) {
         for (;;) {
             ck(anqi_rknn_search(g_ctx, &ix, d_query, nq, (uint32_t)L, nullptr, (uint32_t)L, d_thr, nullptr, out_cap,
-                                d_cand, d_acc, d_cnt, d_nd, d_pred, nullptr, d_status, 1),
+                                d_cand, d_acc, d_cnt, d_nd, d_pred, nullptr, d_status,
+                                anqi_cli::gpu_form_chosen() ? 1 : 0),
                "range search");
             download(cnt, d_cnt, nq);
             uint32_t mx = *std::max_element(cnt.begin(), cnt.end());

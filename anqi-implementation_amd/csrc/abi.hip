@@ -26,6 +26,7 @@ struct anqi_ctx {
     std::string err;
     int cus = 0;
     uint32_t* counter = nullptr;   // device work counter of the persistent search
+    anqi_search_tuning tune{0, 0, 0};   // persistent search work areas / grid (0 = defaults)
 };
 
 static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
@@ -144,6 +145,11 @@ void anqi_ctx_destroy(anqi_ctx* ctx) {
 }
 
 const char* anqi_last_error(const anqi_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int anqi_ctx_set_search_tuning(anqi_ctx* ctx, const anqi_search_tuning* tune) {
+    ctx->tune = tune ? *tune : anqi_search_tuning{0, 0, 0};
+    return ANQI_OK;
+}
 
 int anqi_ctx_sync(anqi_ctx* ctx) {
     HCHK(hipStreamSynchronize(ctx->stream));
@@ -518,7 +524,23 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     P.pmax = graph ? std::min<uint32_t>(2048, next_pow2(need + 64)) : 0;
     P.rows = nullptr;
     P.nrows = P.n;
-    HCHK(launch_gather(P, graph, false, s));
+    DBuf vbits, vlist;
+    if (!graph && P.batched && need < (1u << 24)) {
+        // batched AKNN form: visited sets as per-block bitmaps in HBM, so LDS
+        // holds only the queue and the top-K and more rows run at once
+        const int occ = std::max(1, gather_gv_occupancy(P));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(P.n, (uint64_t)occ * std::max(1, ctx->cus));
+        P.vwords = (P.n + 31) / 32;
+        P.vcap = (uint32_t)need;
+        HCHK(vbits.alloc(4ull * grid * P.vwords));
+        HCHK(vlist.alloc(4ull * grid * P.vcap));
+        HCHK(hipMemsetAsync(vbits.p, 0, 4ull * grid * P.vwords, s));
+        P.vbits = vbits.as<uint32_t>();
+        P.vlist = vlist.as<uint32_t>();
+        HCHK(launch_gather_gv(P, grid, s));
+    } else {
+        HCHK(launch_gather(P, graph, false, s));
+    }
     std::vector<uint32_t> bad;
     int rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
     if (rc) return rc;
@@ -550,7 +572,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
 
 int anqi_gather_topk(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx,
                      const float* hval, uint32_t cap, const uint32_t* roff, const uint32_t* rids, uint32_t explore,
-                     uint32_t pool_cap, uint32_t K, uint32_t* ids, float* keys, uint32_t* cnt) {
+                     uint32_t pool_cap, uint32_t K, uint32_t* ids, float* keys, uint32_t* cnt, int form) {
     if (K > 256) return anqi_set_err(ctx, ANQI_E_USAGE, "K <= 256");
     std::vector<uint32_t> ro(2);
     uint32_t maxrev = 0;
@@ -563,12 +585,13 @@ int anqi_gather_topk(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n
     GatherParams P{};
     P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.hidx = hidx; P.hval = hval; P.cap = cap; P.roff = roff; P.rids = rids;
     P.explore = explore; P.pool_cap = pool_cap; P.K = K; P.out_ids = ids; P.out_keys = keys; P.out_cnt = cnt;
+    P.batched = form ? 1u : 0u;
     return gather_run(ctx, P, false, maxrev, nullptr);
 }
 
 int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx,
                       const float* hval, uint32_t cap, const uint32_t* roff, const uint32_t* rids, uint32_t explore,
-                      uint32_t pool_cap, uint32_t M, float alpha, uint32_t* O, uint32_t* ocnt) {
+                      uint32_t pool_cap, uint32_t M, float alpha, uint32_t* O, uint32_t* ocnt, int form) {
     if (M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "graph degree <= 64");
     uint32_t maxrev = 0;
     {
@@ -580,6 +603,7 @@ int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t 
     GatherParams P{};
     P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.hidx = hidx; P.hval = hval; P.cap = cap; P.roff = roff; P.rids = rids;
     P.explore = explore; P.pool_cap = pool_cap; P.M = M; P.alpha = alpha; P.out_ids = O; P.out_cnt = ocnt;
+    P.batched = form ? 1u : 0u;
     return gather_run(ctx, P, true, maxrev, nullptr);
 }
 
@@ -600,6 +624,7 @@ int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uin
     P.rids = rids.as<uint32_t>(); P.M = M; P.alpha = alpha; P.links = links; P.deg = deg;
     P.status = status.as<uint32_t>(); P.nrows = n;
     P.pmax = std::min<uint32_t>(2048, next_pow2((uint64_t)M + maxin + 64));
+    HCHK(launch_fill_u32(links, U32_NONE, (uint64_t)n * M, s));   // slots past deg[i]
     HCHK(launch_commit(P, false, s));
     std::vector<uint32_t> bad;
     rc = failed_rows(ctx, status.as<uint32_t>(), n, bad);
@@ -945,17 +970,36 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     uint32_t max_leaf_all = 0;
     for (auto& lv : F.leaves)
         for (auto& pr : lv) max_leaf_all = std::max(max_leaf_all, pr.second);
-    const bool use_mfma = p->precision == 1 && leaf_mfma_supported(d, max_leaf_all);
-    DBuf xb, lmarg;
+    if (p->precision < 0 || p->precision > 2) return anqi_set_err(ctx, ANQI_E_USAGE, "precision %d", p->precision);
+    const bool use_mfma = p->precision >= 1 && leaf_mfma_supported(d, max_leaf_all);
+    const bool use_fp8 = use_mfma && p->precision == 2;
+    DBuf xb, lmarg, nsurv;
+    float fp8_scale = 1.0f;
     if (use_mfma) {
         DBuf nx, ex;
-        HCHK(xb.alloc(2ull * n * d));
+        HCHK(xb.alloc((use_fp8 ? 1ull : 2ull) * n * d));
         HCHK(nx.alloc(4ull * n));
         HCHK(ex.alloc(4ull * n));
         HCHK(lmarg.alloc(4ull * n));
-        HCHK(launch_to_bf16(x, (uint64_t)n * d, xb.as<uint16_t>(), s));
-        HCHK(launch_exact_norms2(x, nrm.as<float>(), n, d, nx.as<float>(), ex.as<float>(), s));
+        HCHK(nsurv.alloc(8));
+        HCHK(hipMemsetAsync(nsurv.p, 0, 8, s));
         std::vector<float> hn(n), he(n);
+        if (use_fp8) {
+            // power-of-two scale putting every |x_k| <= max row norm under e4m3's 448
+            HCHK(launch_exact_norms2(x, nrm.as<float>(), n, d, nx.as<float>(), ex.as<float>(), s));
+            HCHK(hipMemcpyAsync(hn.data(), nx.p, 4ull * n, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+            float Y = 0.0f;
+            for (uint32_t i = 0; i < n; i++) Y = std::max(Y, hn[i]);
+            int e2 = 0;
+            std::frexp(std::max(Y, 1e-30f) / 448.0f, &e2);
+            fp8_scale = std::ldexp(1.0f, e2);
+            HCHK(launch_to_fp8(x, nrm.as<float>(), n, d, fp8_scale, xb.as<uint8_t>(), nx.as<float>(), ex.as<float>(),
+                               s));
+        } else {
+            HCHK(launch_to_bf16(x, (uint64_t)n * d, xb.as<uint16_t>(), s));
+            HCHK(launch_exact_norms2(x, nrm.as<float>(), n, d, nx.as<float>(), ex.as<float>(), s));
+        }
         HCHK(hipMemcpyAsync(hn.data(), nx.p, 4ull * n, hipMemcpyDeviceToHost, s));
         HCHK(hipMemcpyAsync(he.data(), ex.p, 4ull * n, hipMemcpyDeviceToHost, s));
         HCHK(hipStreamSynchronize(s));
@@ -986,7 +1030,9 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
         LP.leaf_start = ds.as<uint32_t>(); LP.leaf_size = dz.as<uint32_t>(); LP.nleaves = (uint32_t)lv.size();
         LP.K = K; LP.ids = lid.as<uint32_t>(); LP.keys = lkey.as<float>(); LP.cnt = lcnt.as<uint32_t>();
         LP.max_leaf = mx;
-        if (use_mfma) HCHK(launch_leaf_mfma(LP, xb.as<uint16_t>(), lmarg.as<float>(), s));
+        LP.nsurv = use_mfma ? nsurv.as<unsigned long long>() : nullptr;
+        if (use_fp8) HCHK(launch_leaf_mfma_fp8(LP, xb.as<uint8_t>(), fp8_scale, lmarg.as<float>(), s));
+        else if (use_mfma) HCHK(launch_leaf_mfma(LP, xb.as<uint16_t>(), lmarg.as<float>(), s));
         else HCHK(launch_leaf_knn(LP, false, s));
         HCHK(hipStreamSynchronize(s));   // keeps ds/dz alive until the launch ends
         if ((t & 7) == 7 || t + 1 == T) progress("leaf all-pairs: tree %u/%u", t + 1, T);
@@ -994,6 +1040,11 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     HCHK(launch_random_fill(x, nrm.as<float>(), n, d, K, p->seed ^ 0x5EEDull, lid.as<uint32_t>(), lkey.as<float>(),
                             lcnt.as<uint32_t>(), s));
     HCHK(hipStreamSynchronize(s));
+    if (use_mfma) {
+        HCHK(hipMemcpy(&S.leaf_survivors, nsurv.p, 8, hipMemcpyDeviceToHost));
+        progress("leaf sweep (%s): %llu of %llu ordered pairs passed the filter", use_fp8 ? "fp8" : "bf16",
+                 (unsigned long long)S.leaf_survivors, (unsigned long long)(2 * S.leaf_pairs));
+    }
     double t2 = now_s();
     S.leaf_s = t2 - t1;
     F.perm.alloc(16);
@@ -1018,6 +1069,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     G.x = x; G.nrm = nrm.as<float>(); G.n = n; G.d = d; G.hidx = lid.as<uint32_t>(); G.hval = lkey.as<float>();
     G.cap = K; G.roff = roff.as<uint32_t>(); G.rids = rids.as<uint32_t>(); G.explore = ef; G.pool_cap = pool_cap;
     G.K = K; G.out_ids = ids; G.out_keys = keys; G.out_cnt = ocnt.as<uint32_t>();
+    G.batched = p->precision >= 1 ? 1u : 0u;
     rc = gather_run(ctx, G, false, maxrev, &S.overflow_rows);
     if (rc) return rc;
     std::vector<uint32_t> oc(n);
@@ -1075,6 +1127,7 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     G.cap = cap; G.roff = roff.as<uint32_t>(); G.rids = rids.as<uint32_t>(); G.explore = p->explore ? p->explore : 100;
     G.pool_cap = std::max<uint32_t>(p->M, (p->ef_alpha ? p->ef_alpha : 8) * G.explore);
     G.M = p->M; G.alpha = p->alpha; G.out_ids = O.as<uint32_t>(); G.out_cnt = ocnt.as<uint32_t>();
+    G.batched = p->form ? 1u : 0u;
     rc = gather_run(ctx, G, true, maxrev, &S.overflow_rows);
     if (rc) return rc;
     HCHK(hipStreamSynchronize(s));
@@ -1132,6 +1185,8 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     // overflow it are rerun below with 16K-entry areas.
     P.hmask = std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * std::max(L, Lmax) + 1536))) - 1;
     P.rcap = 1024; P.out_cap = out_cap;
+    if (mode == 1 && ctx->tune.visited_slots >= 2 * P.Lmax) P.hmask = next_pow2(ctx->tune.visited_slots) - 1;
+    if (mode == 1 && ctx->tune.result_cap) P.rcap = ctx->tune.result_cap;
     P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
     P.nexp = nexp;
     P.status = status;
@@ -1140,8 +1195,9 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
         if (!ctx->counter) HCHK(hipMalloc((void**)&ctx->counter, 256));
         HCHK(launch_fill_u32(ctx->counter, 0u, 1, s));
         P.work_counter = ctx->counter;
-        int occ = search_persistent_occupancy(P);
-        uint32_t grid = (uint32_t)std::max(1, occ) * (uint32_t)std::max(1, ctx->cus);
+        int occ = std::max(1, search_persistent_occupancy(P));
+        uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
+        uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
         HCHK(launch_range_search(P, true, grid, s));
     } else {
         HCHK(launch_range_search(P, false, 0, s));
@@ -1379,7 +1435,7 @@ int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_t
     double best = 1e300;
     anqi_search_tuning bt{0, 0, 0};
     const uint32_t slot_opts[3] = {2048, 4096, 8192};
-    const uint32_t rcap_opts[2] = {512, 1024};
+    const uint32_t rcap_opts[2] = {128, 512};   // later FIFO entries spill to the output rows
     std::vector<uint32_t> st(nq);
     for (uint32_t so : slot_opts) {
         if (so < 2 * Lmax) continue;

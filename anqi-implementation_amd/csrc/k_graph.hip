@@ -346,9 +346,14 @@ __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P,
     return a > b ? a : b;
 }
 
+// Fresh-neighbour list of one expansion (batched form); a longer expansion is
+// processed in several lists.
+#define GX_CAP 256
+__host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P) { return P.batched ? GX_CAP : 64; }
+
 size_t gather_lds_bytes(const GatherParams& P, bool graph) {
     size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : 4 * rnd16(4 * (size_t)P.K);
-    return gather_region0(P, graph) + pool + rnd16(4 * 64) + rnd16(4 * 64) + rnd16(8 * 64);
+    return gather_region0(P, graph) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P)) + rnd16(8 * 64);
 }
 
 __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams& P, bool graph) {
@@ -374,8 +379,8 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
         s.pd2 = (float*)take(4 * (size_t)P.K);
         s.pi2 = (uint32_t*)take(4 * (size_t)P.K);
     }
-    s.tid = (uint32_t*)take(4 * 64);
-    s.tdv = (float*)take(4 * 64);
+    s.tid = (uint32_t*)take(4 * (size_t)gather_tcap(P));
+    s.tdv = (float*)take(4 * (size_t)gather_tcap(P));
     s.cs = (uint64_t*)take(8 * 64);
     return s;
 }
@@ -445,17 +450,18 @@ __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uin
     return deg;
 }
 
-template <bool GRAPH, bool GWS, int MC>
-__global__ void __launch_bounds__(64) k_gather(GatherParams P) {
-    extern __shared__ __align__(16) char lds_raw[];
-    uint32_t bi = blockIdx.x;
-    if (bi >= P.nrows) return;
-    uint32_t i = P.rows ? P.rows[bi] : bi;
-    char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
-    GatherLds S = gather_carve(base, P, GRAPH);
+// One row of the gather.  GV: the visited set is a bitmap in global memory
+// (vb, one bit per base row, owned by this wave) instead of the LDS hash;
+// first visits are listed in vl so the touched words are cleared at the end.
+template <bool GRAPH, bool GWS, int MC, bool GV>
+__device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, uint32_t i, uint32_t* vb,
+                                           uint32_t* vl) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const int d = (int)P.d;
-    for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    if constexpr (!GV) {
+        for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    }
+    uint32_t vcount = 0;
     __syncthreads();
     QRow QI;
     qrow_load(QI, P.x + (size_t)i * d, d, l);
@@ -463,26 +469,57 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     uint32_t qsize = 0, psize = 0, pcount = 0;
     int qb = 0, pb = 0, status = 0;
 
-    // Offers a chunk of candidate ids (lane-held, in reference order) to the
-    // visited set, pool and queue.  seed_d: stored distances for the seeds.
-    auto offer = [&](uint32_t id, int valid, bool seeded, float seed_d) -> int {
+    // visit: a chunk of candidate ids (lane-held, in reference order) goes
+    // through the visited set; first visits are appended to tid[nf..] (with
+    // seed_d, the stored distance, for the seeds).
+    // process: distances for tid[0..nf) (unless seeded), then pool and queue
+    // merges, 64 candidates at a time.  Merging a set of candidates in pieces
+    // or at once gives the same sorted, truncated pool and queue (total
+    // (dist, id) order, no duplicates), so one expansion's first visits can
+    // be processed together: the stop test (pool count) is only taken between
+    // expansions.
+    uint32_t nf = 0;
+    auto visit = [&](uint32_t id, int valid, float seed_d) __attribute__((always_inline)) -> int {
         int fresh = 0;
         if (valid) {
-            int r = hset_insert(S.hash, P.hmask, id);
-            if (r < 0) status = 1;
-            fresh = (r == 1);
+            if constexpr (GV) {
+                const uint32_t bit = 1u << (id & 31u);
+                fresh = !(atomicOr(&vb[id >> 5], bit) & bit);
+            } else {
+                int r = hset_insert(S.hash, P.hmask, id);
+                if (r < 0) status = 1;
+                fresh = (r == 1);
+            }
+        }
+        if constexpr (GV) {
+            // vcap bounds the first visits of a row (pool_cap + cap + longest
+            // reverse list); guarded anyway
+            const uint64_t vm = ballot(fresh);
+            if (vcount + (uint32_t)popc64(vm) > P.vcap) {
+                if (fresh) atomicAnd(&vb[id >> 5], ~(1u << (id & 31u)));
+                status = 1;
+                fresh = 0;
+            } else if (fresh) {
+                vl[vcount + lanes_below(vm)] = id;
+            }
+            vcount += (uint32_t)popc64(ballot(fresh));
         }
         if (__any(status)) { status = 1; return 0; }
         uint64_t fm = ballot(fresh);
-        int k = popc64(fm);
-        if (k == 0) return 1;
-        if (fresh) { S.tid[lanes_below(fm)] = id; S.tdv[lanes_below(fm)] = seed_d; }
+        if (fresh) { S.tid[nf + lanes_below(fm)] = id; S.tdv[nf + lanes_below(fm)] = seed_d; }
+        nf += (uint32_t)popc64(fm);
+        return 1;
+    };
+    auto process = [&](bool seeded) __attribute__((always_inline)) -> int {
+        const uint32_t kall = nf;
+        nf = 0;
+        if (kall == 0) return 1;
         __syncthreads();
         if (!seeded) {
             if constexpr (MC > 0) {
-                rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, k, l, g, S.tdv);
+                rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
             } else {
-                for (int s = g; s < k; s += 4) {
+                for (uint32_t s = g; s < kall; s += 4) {
                     uint32_t c = S.tid[s];
                     float ip = qrow_ip(QI, P.x + (size_t)c * d, d, l);
                     if (l == 0) S.tdv[s] = d2_combine(ip, ni, P.nrm[c]);
@@ -490,56 +527,60 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
             }
             __syncthreads();
         }
-        float dv = lane < k ? S.tdv[lane] : F32_BIG;
-        uint32_t cid = lane < k ? S.tid[lane] : U32_NONE;
-        // pool
-        if (GRAPH) {
-            if (psize + (uint32_t)k > P.pmax) { status = 2; return 0; }
-            if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
-            psize += (uint32_t)k;
-        } else {
-            uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
-            float* ad = pb ? S.pd2 : S.pd;
-            uint32_t* ap = pb ? S.pi2 : S.pi;
-            if (psize == P.K && key != ~0ull && !(key < pkey(ad[P.K - 1], ap[P.K - 1]))) key = ~0ull;
-            key = wave_sort64(key);
-            int kc = popc64(ballot(key != ~0ull));
-            if (kc) {
-                S.cs[lane] = key;
-                __syncthreads();
-                psize = merge_into(S.cs, kc, ad, ap, psize, P.K, pb ? S.pd : S.pd2, pb ? S.pi : S.pi2);
-                pb ^= 1;
+        for (uint32_t b = 0; b < kall; b += 64) {
+            const int k = (int)min(64u, kall - b);
+            float dv = lane < k ? S.tdv[b + lane] : F32_BIG;
+            uint32_t cid = lane < k ? S.tid[b + lane] : U32_NONE;
+            // pool
+            if (GRAPH) {
+                if (psize + (uint32_t)k > P.pmax) { status = 2; return 0; }
+                if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
+                psize += (uint32_t)k;
+            } else {
+                uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
+                float* ad = pb ? S.pd2 : S.pd;
+                uint32_t* ap = pb ? S.pi2 : S.pi;
+                if (psize == P.K && key != ~0ull && !(key < pkey(ad[P.K - 1], ap[P.K - 1]))) key = ~0ull;
+                int kc = popc64(ballot(key != ~0ull));
+                if (kc) {
+                    key = wave_sort64(key);
+                    S.cs[lane] = key;
+                    __syncthreads();
+                    psize = merge_into(S.cs, kc, ad, ap, psize, P.K, pb ? S.pd : S.pd2, pb ? S.pi : S.pi2);
+                    pb ^= 1;
+                    __syncthreads();
+                }
+            }
+            pcount += (uint32_t)k;
+            // queue
+            {
+                uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
+                float* ad = S.qd(qb);
+                uint32_t* ap = S.qp(qb);
+                if (qsize == P.explore && key != ~0ull &&
+                    !(key < pkey(ad[P.explore - 1], ap[P.explore - 1] & 0x7fffffffu)))
+                    key = ~0ull;
+                int kc = popc64(ballot(key != ~0ull));
+                if (kc) {
+                    key = wave_sort64(key);
+                    S.cs[lane] = key;
+                    __syncthreads();
+                    qsize = merge_into(S.cs, kc, ad, ap, qsize, P.explore, S.qd(qb ^ 1), S.qp(qb ^ 1));
+                    qb ^= 1;
+                }
                 __syncthreads();
             }
-        }
-        pcount += (uint32_t)k;
-        // queue
-        {
-            uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
-            float* ad = S.qd(qb);
-            uint32_t* ap = S.qp(qb);
-            if (qsize == P.explore && key != ~0ull &&
-                !(key < pkey(ad[P.explore - 1], ap[P.explore - 1] & 0x7fffffffu)))
-                key = ~0ull;
-            key = wave_sort64(key);
-            int kc = popc64(ballot(key != ~0ull));
-            if (kc) {
-                S.cs[lane] = key;
-                __syncthreads();
-                qsize = merge_into(S.cs, kc, ad, ap, qsize, P.explore, S.qd(qb ^ 1), S.qp(qb ^ 1));
-                qb ^= 1;
-            }
-            __syncthreads();
         }
         return 1;
     };
+    const uint32_t tcap = gather_tcap(P);
 
     // seeds: the row's own slots in storage order, with stored values
     for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
         uint32_t s = s0 + lane;
         uint32_t id = s < P.cap ? P.hidx[(size_t)i * P.cap + s] : U32_NONE;
         float dv = s < P.cap ? P.hval[(size_t)i * P.cap + s] : F32_BIG;
-        offer(id, id != U32_NONE && id < P.n, true, dv);
+        if (visit(id, id != U32_NONE && id < P.n, dv)) process(true);
     }
     // beam over the neighbour heaps and their reverse lists
     while (!status) {
@@ -557,15 +598,25 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
         for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
             uint32_t s = s0 + lane;
             uint32_t id = s < P.cap ? P.hidx[(size_t)c * P.cap + s] : U32_NONE;
-            offer(id, id != U32_NONE && id < P.n, false, 0.0f);
+            if (visit(id, id != U32_NONE && id < P.n, 0.0f) && (!P.batched || nf + 64 > tcap)) process(false);
         }
         uint32_t r0 = P.roff[c], r1 = P.roff[c + 1];
         for (uint32_t s0 = r0; s0 < r1 && !status; s0 += 64) {
             uint32_t s = s0 + lane;
             uint32_t id = s < r1 ? P.rids[s] : U32_NONE;
-            offer(id, id != U32_NONE && id < P.n, false, 0.0f);
+            if (visit(id, id != U32_NONE && id < P.n, 0.0f) && (!P.batched || nf + 64 > tcap)) process(false);
         }
+        // plain form: each chunk was processed on its own; batched form: the
+        // expansion's first visits together (in lists of up to tcap)
+        if (!status) process(false);
         if (pcount >= P.pool_cap) break;
+    }
+    if constexpr (GV) {
+        // clear the words this row set; the wave's next row starts from zero
+        __syncthreads();
+        for (uint32_t j = lane; j < vcount; j += 64) vb[vl[j] >> 5] = 0u;
+        __threadfence();
+        __syncthreads();
     }
     if (status) {
         if (lane == 0) { P.status[i] = (uint32_t)status; P.out_cnt[i] = 0; }
@@ -600,6 +651,75 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     lds_sort_pairs(S.pd, S.pi, w, cap2);
     uint32_t deg = rng_select(P, S.pd, S.pi, w, S.selv, S.sel, P.out_ids + (size_t)i * P.M);
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
+}
+
+template <bool GRAPH, bool GWS, int MC>
+__global__ void __launch_bounds__(64) k_gather(GatherParams P) {
+    extern __shared__ __align__(16) char lds_raw[];
+    uint32_t bi = blockIdx.x;
+    if (bi >= P.nrows) return;
+    uint32_t i = P.rows ? P.rows[bi] : bi;
+    char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
+    GatherLds S = gather_carve(base, P, GRAPH);
+    gather_row<GRAPH, GWS, MC, false>(P, S, i, nullptr, nullptr);
+}
+
+// AKNN gather, persistent: block b owns bitmap b and takes rows b, b + grid, ...
+template <int MC>
+__global__ void __launch_bounds__(64) k_gather_gv(GatherParams P) {
+    extern __shared__ __align__(16) char lds_raw[];
+    GatherLds S = gather_carve(lds_raw, P, false);
+    uint32_t* vb = P.vbits + (size_t)blockIdx.x * P.vwords;
+    uint32_t* vl = P.vlist + (size_t)blockIdx.x * P.vcap;
+    for (uint32_t bi = blockIdx.x; bi < P.nrows; bi += gridDim.x) {
+        const uint32_t i = P.rows ? P.rows[bi] : bi;
+        gather_row<false, false, MC, true>(P, S, i, vb, vl);
+        __syncthreads();
+    }
+}
+
+template <int MC>
+static int gather_gv_occ(size_t lds) {
+    int occ = 0;
+    const void* f = (const void*)k_gather_gv<MC>;
+    hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 64, lds) != hipSuccess) occ = 1;
+    return occ;
+}
+
+// Resident blocks per CU of the bitmap-visited AKNN gather (P.hmask ignored).
+int gather_gv_occupancy(const GatherParams& P0) {
+    GatherParams P = P0;
+    P.hmask = 0;
+    const size_t lds = gather_lds_bytes(P, false);
+    switch (P.d >> 4) {
+        case 8: return gather_gv_occ<8>(lds);
+        case 6: return gather_gv_occ<6>(lds);
+        default: return gather_gv_occ<0>(lds);
+    }
+}
+
+// P.vbits: grid x vwords zeroed words; P.vlist: grid x vcap.
+hipError_t launch_gather_gv(const GatherParams& P0, uint32_t grid, hipStream_t s) {
+    if (P0.nrows == 0) return hipSuccess;
+    GatherParams P = P0;
+    P.hmask = 0;
+    const size_t lds = gather_lds_bytes(P, false);
+    switch (P.d >> 4) {
+        case 8:
+            hipFuncSetAttribute((const void*)k_gather_gv<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k_gather_gv<8>, dim3(grid), dim3(64), lds, s, P);
+            break;
+        case 6:
+            hipFuncSetAttribute((const void*)k_gather_gv<6>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k_gather_gv<6>, dim3(grid), dim3(64), lds, s, P);
+            break;
+        default:
+            hipFuncSetAttribute((const void*)k_gather_gv<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k_gather_gv<0>, dim3(grid), dim3(64), lds, s, P);
+            break;
+    }
+    return hipGetLastError();
 }
 
 template <bool GRAPH, bool GWS, int MC>

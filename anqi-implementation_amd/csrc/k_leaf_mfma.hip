@@ -137,8 +137,38 @@ __device__ void lm_flush_row(const LeafParams& P, LmWave& W, const uint32_t* mem
     LM_WSYNC();
 }
 
-template <int MC>
-__global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const uint16_t* __restrict__ xb,
+// Operand fragments: 8 k-values per lane per 16-wide step, as bf16 (16 B) or,
+// in the FP8 trial, as OCP e4m3 bytes (8 B) of x / s with s a power of two
+// (the MFMA sum is scaled back by s^2).
+template <bool F8>
+struct LmFrag;
+template <>
+struct LmFrag<false> {
+    lm_bf16x8 v;
+    __device__ void load(const void* xq, uint32_t row, uint32_t d, uint32_t k0, bool ok) {
+        uint4 z = make_uint4(0, 0, 0, 0);
+        if (ok) z = *(const uint4*)((const uint16_t*)xq + (size_t)row * d + k0);
+        v = __builtin_bit_cast(lm_bf16x8, z);
+    }
+};
+template <>
+struct LmFrag<true> {
+    long v;
+    __device__ void load(const void* xq, uint32_t row, uint32_t d, uint32_t k0, bool ok) {
+        uint64_t z = 0;
+        if (ok) z = *(const uint64_t*)((const uint8_t*)xq + (size_t)row * d + k0);
+        v = (long)z;
+    }
+};
+__device__ __forceinline__ lm_f32x16 lm_mma(const LmFrag<false>& a, const LmFrag<false>& b, lm_f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v, b.v, c, 0, 0, 0);
+}
+__device__ __forceinline__ lm_f32x16 lm_mma(const LmFrag<true>& a, const LmFrag<true>& b, lm_f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a.v, b.v, c, 0, 0, 0);
+}
+
+template <int MC, bool F8>
+__global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const void* __restrict__ xq, float s2,
                                                    const float* __restrict__ marg) {
     extern __shared__ __align__(16) char lds[];
     const uint32_t leaf = blockIdx.x;
@@ -148,17 +178,16 @@ __global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const uint16_t*
     const uint32_t m = P.leaf_size[leaf], K = P.K, d = P.d;
     const uint32_t* mem = P.perm + P.leaf_start[leaf];
     const uint32_t nrb = (m + 31) / 32;
+    uint32_t nsv = 0;   // pairs this lane let through the filter
     for (uint32_t rb = (uint32_t)w; rb < nrb; rb += 4) {
         // A fragments of the block's 32 members
         const uint32_t am = rb * 32 + (lane & 31);
         const uint32_t arow = am < m ? mem[am] : U32_NONE;
-        lm_bf16x8 afr[LM_KS];
+        LmFrag<F8> afr[LM_KS];
 #pragma unroll
         for (int s = 0; s < LM_KS; s++) {
-            uint4 z = make_uint4(0, 0, 0, 0);
             const uint32_t k0 = 16 * s + 8 * (lane >> 5);
-            if (arow != U32_NONE && k0 < d) z = *(const uint4*)(xb + (size_t)arow * d + k0);
-            afr[s] = __builtin_bit_cast(lm_bf16x8, z);
+            afr[s].load(xq, arow, d, k0, arow != U32_NONE && k0 < d);
         }
         uint32_t rid[16];
         float bnr[16], thrm[16];
@@ -192,25 +221,24 @@ __global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const uint16_t*
             lm_f32x16 acc;
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[v] = 0.0f;
-            lm_bf16x8 bfr[LM_KS];
+            LmFrag<F8> bfr[LM_KS];
 #pragma unroll
             for (int s = 0; s < LM_KS; s++) {
-                uint4 z = make_uint4(0, 0, 0, 0);
                 const uint32_t k0 = 16 * s + 8 * (lane >> 5);
-                if (crow != U32_NONE && k0 < d) z = *(const uint4*)(xb + (size_t)crow * d + k0);
-                bfr[s] = __builtin_bit_cast(lm_bf16x8, z);
+                bfr[s].load(xq, crow, d, k0, crow != U32_NONE && k0 < d);
             }
 #pragma unroll
-            for (int s = 0; s < LM_KS; s++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[s], bfr[s], acc, 0, 0, 0);
+            for (int s = 0; s < LM_KS; s++) acc = lm_mma(afr[s], bfr[s], acc);
             const float bc = crow != U32_NONE ? P.nrm[crow] : 0.0f;
 #pragma unroll
             for (int v = 0; v < 16; v++) {
                 const uint32_t a = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const float approx = fmaf(-2.0f, acc[v], bnr[v] + bc);
+                const float approx = fmaf(-2.0f, acc[v] * s2, bnr[v] + bc);
                 if (a < m && cm < m && cm != a && approx <= thrm[v]) {
                     const int rl = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                     uint32_t p = atomicAdd(&W.svc[rl], 1u);
                     W.sv[rl * LM_SV + p] = (uint16_t)cm;
+                    nsv++;
                 }
             }
             LM_WSYNC();
@@ -218,6 +246,10 @@ __global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const uint16_t*
             if (__any(lane < 32 && W.svc[lane] > LM_SV - 32)) flush();
         }
         flush();
+    }
+    if (P.nsurv) {
+        for (int o = 32; o > 0; o >>= 1) nsv += __shfl_xor(nsv, o);
+        if (lane == 0 && nsv) atomicAdd(P.nsurv, (unsigned long long)nsv);
     }
 }
 
@@ -228,20 +260,71 @@ bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf) {
     return d <= 16 * LM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && max_leaf <= 65535;
 }
 
-template <int MC>
-static hipError_t launch_leaf_mfma_t(const LeafParams& P, const uint16_t* xb, const float* marg, hipStream_t s) {
+template <int MC, bool F8>
+static hipError_t launch_leaf_mfma_t(const LeafParams& P, const void* xq, float s2, const float* marg,
+                                     hipStream_t s) {
     size_t lds = leaf_mfma_lds(P.K);
-    hipFuncSetAttribute((const void*)k_leaf_mfma<MC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_leaf_mfma<MC>, dim3(P.nleaves), dim3(256), lds, s, P, xb, marg);
+    hipFuncSetAttribute((const void*)k_leaf_mfma<MC, F8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_leaf_mfma<MC, F8>), dim3(P.nleaves), dim3(256), lds, s, P, xq, s2, marg);
     return hipGetLastError();
+}
+
+template <bool F8>
+static hipError_t launch_leaf_mfma_f(const LeafParams& P, const void* xq, float s2, const float* marg,
+                                     hipStream_t s) {
+    switch (P.d >> 4) {
+        case 8: return launch_leaf_mfma_t<8, F8>(P, xq, s2, marg, s);
+        case 7: return launch_leaf_mfma_t<7, F8>(P, xq, s2, marg, s);
+        default: return launch_leaf_mfma_t<6, F8>(P, xq, s2, marg, s);
+    }
 }
 
 hipError_t launch_leaf_mfma(const LeafParams& P, const uint16_t* xb, const float* marg, hipStream_t s) {
     if (P.nleaves == 0) return hipSuccess;
     if (!leaf_mfma_supported(P.d, P.max_leaf)) return hipErrorInvalidValue;
-    switch (P.d >> 4) {
-        case 8: return launch_leaf_mfma_t<8>(P, xb, marg, s);
-        case 7: return launch_leaf_mfma_t<7>(P, xb, marg, s);
-        default: return launch_leaf_mfma_t<6>(P, xb, marg, s);
+    return launch_leaf_mfma_f<false>(P, xb, 1.0f, marg, s);
+}
+
+hipError_t launch_leaf_mfma_fp8(const LeafParams& P, const uint8_t* x8, float scale, const float* marg,
+                                hipStream_t s) {
+    if (P.nleaves == 0) return hipSuccess;
+    if (!leaf_mfma_supported(P.d, P.max_leaf)) return hipErrorInvalidValue;
+    return launch_leaf_mfma_f<true>(P, x8, scale * scale, marg, s);
+}
+
+// x8 = e4m3(x / scale) (OCP FP8, round to nearest even); err[i] = |x_i - scale * x8_i|
+// (rounded up), nx[i] = sqrt(nrm[i]) (rounded up): the inputs of the per-row
+// filter bound for the FP8 sweep.
+__global__ void k_to_fp8(const float* __restrict__ x, const float* __restrict__ nrm, uint32_t n, uint32_t d,
+                         float inv_scale, float scale, uint8_t* __restrict__ x8, float* __restrict__ nx,
+                         float* __restrict__ ex) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = x + (size_t)i * d;
+    uint8_t* o = x8 + (size_t)i * d;
+    float e2 = 0.0f;
+    for (uint32_t k = 0; k < d; k += 2) {
+        const float a = r[k] * inv_scale, b = k + 1 < d ? r[k + 1] * inv_scale : 0.0f;
+        const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+        o[k] = (uint8_t)(pk & 0xff);
+        if (k + 1 < d) o[k + 1] = (uint8_t)((pk >> 8) & 0xff);
+        const float qa = __builtin_amdgcn_cvt_f32_fp8(pk, 0) * scale;
+        const float da = r[k] - qa;
+        e2 = fmaf(da, da, e2);
+        if (k + 1 < d) {
+            const float qb = __builtin_amdgcn_cvt_f32_fp8(pk, 1) * scale;
+            const float db = r[k + 1] - qb;
+            e2 = fmaf(db, db, e2);
+        }
     }
+    nx[i] = sqrtf(fmaxf(nrm[i], 0.0f)) * 1.0001f + 1e-30f;
+    ex[i] = sqrtf(e2) * 1.001f + 1e-30f;
+}
+
+hipError_t launch_to_fp8(const float* x, const float* nrm, uint32_t n, uint32_t d, float scale, uint8_t* x8,
+                         float* nx, float* ex, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_fp8, dim3((n + 255) / 256), dim3(256), 0, s, x, nrm, n, d, 1.0f / scale, scale, x8, nx,
+                       ex);
+    return hipGetLastError();
 }

@@ -18,6 +18,10 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+// Flood nodes expanded together (their adjacency rows and all their new
+// neighbours' rows are fetched in one round).
+#define SR_FLOOD 4
+
 namespace {
 
 struct SearchLds {
@@ -31,11 +35,24 @@ struct SearchLds {
     __device__ float* bd(int b) const { return b ? bd1 : bd0; }
     __device__ uint32_t* bp(int b) const { return b ? bp1 : bp0; }
     uint32_t* hash;    // visited set
-    uint32_t* res;     // collected ids (FIFO)
+    uint32_t* res;     // collected ids (FIFO), first rcap entries
+    uint32_t* spill;   // the query's output row: FIFO entries from rcap on
     uint32_t* tid;     // 64 new ids
     float* tdv;        // 64 distances
     uint64_t* cs;      // 64 sorted candidate keys
 };
+
+// Collected-candidate FIFO: entries below rcap live in LDS, later ones in
+// the query's output row, at the position the final list gives them anyway
+// (the post-filter rewrites out[j] = fifo[j]).  Only more than
+// max(rcap, out_cap) candidates overflow.
+__device__ __forceinline__ uint32_t fifo_cap(const SearchParams& P) { return max(P.rcap, P.out_cap); }
+__device__ __forceinline__ void fifo_put(const SearchParams& P, const SearchLds& S, uint32_t j, uint32_t v) {
+    if (j < P.rcap) S.res[j] = v; else S.spill[j] = v;
+}
+__device__ __forceinline__ uint32_t fifo_get(const SearchParams& P, const SearchLds& S, uint32_t j) {
+    return j < P.rcap ? S.res[j] : S.spill[j];
+}
 
 __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
     SearchLds s;
@@ -43,15 +60,21 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
     auto take = [&](size_t bytes) { char* p = base + o; o += (bytes + 15) & ~size_t(15); return p; };
     s.qlift = (float*)take(sizeof(float) * (P.dl + 16));
     s.qbase = (float*)take(sizeof(float) * (P.d + 16));
-    s.bd0 = (float*)take(sizeof(float) * P.Lmax);
-    s.bd1 = (float*)take(sizeof(float) * P.Lmax);
-    s.bp0 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
-    s.bp1 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+    if (P.rbeam) {
+        // register beam: one (dist, id) key exchange area of Lmax entries at bd0
+        s.bd0 = (float*)take(sizeof(uint64_t) * P.Lmax);
+        s.bd1 = s.bd0; s.bp0 = nullptr; s.bp1 = nullptr; s.cs = nullptr;
+    } else {
+        s.bd0 = (float*)take(sizeof(float) * P.Lmax);
+        s.bd1 = (float*)take(sizeof(float) * P.Lmax);
+        s.bp0 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+        s.bp1 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
+        s.cs = (uint64_t*)take(sizeof(uint64_t) * 64);
+    }
     s.hash = (uint32_t*)take(sizeof(uint32_t) * (P.hmask + 1));
     s.res = (uint32_t*)take(sizeof(uint32_t) * P.rcap);
-    s.tid = (uint32_t*)take(sizeof(uint32_t) * 64);
-    s.tdv = (float*)take(sizeof(float) * 64);
-    s.cs = (uint64_t*)take(sizeof(uint64_t) * 64);
+    s.tid = (uint32_t*)take(sizeof(uint32_t) * SR_FLOOD * 64);
+    s.tdv = (float*)take(sizeof(float) * SR_FLOOD * 64);
     return s;
 }
 
@@ -104,8 +127,8 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     int hit = (lane < k) && (dv <= P.R2f);
     uint64_t hm = ballot(hit);
     int nh = popc64(hm);
-    if (rcnt + (uint32_t)nh > P.rcap) { ovf = 2; return false; }
-    if (hit) S.res[rcnt + lanes_below(hm)] = myid;
+    if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
+    if (hit) fifo_put(P, S, rcnt + lanes_below(hm), myid);
     rcnt += (uint32_t)nh;
     if (BEAM) {
         // candidate keys; drop those that cannot enter a full beam
@@ -148,10 +171,166 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     return true;
 }
 
+// Register-resident beam (persistent form): lane l holds beam positions
+// 64 j + l, j < NCH, as (dist, id) keys (~0 past the end) plus a bit per
+// chunk for "expanded".  Merging kc candidates needs no sort and no LDS
+// search: for each candidate (scalar loop) one ballot per chunk counts the
+// beam keys below it and every beam lane counts the candidates below its key;
+// the rank among candidates is one more ballot.  Keys then move to their new
+// positions through one LDS write and one read per chunk.
+#define RB_FLAG 0x80000000ull   // expanded bit, kept in bit 31 of the id while in LDS
+
+template <int NCH>
+struct RBeam {
+    uint64_t k[NCH];
+    uint32_t x;   // bit j: position 64 j + lane expanded
+};
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// dg, nbr: node c's degree and adjacency row (lane-held), fetched by the caller.
+template <int NCH, int MC>
+__device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t dg,
+                                          uint32_t nbr, uint32_t& rcnt, uint32_t& nd, RBeam<NCH>& B,
+                                          uint32_t& bsize, uint32_t L, int& ovf) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    uint32_t nb = U32_NONE;
+    int fresh = 0;
+    if ((uint32_t)lane < dg) {
+        nb = nbr;
+        int r = hset_insert(S.hash, P.hmask, nb);
+        if (r < 0) ovf = 1;
+        fresh = (r == 1);
+    }
+    if (__any(ovf)) { ovf = 1; return false; }
+    const uint64_t fm = ballot(fresh);
+    const int k = popc64(fm);
+    if (fresh) S.tid[lanes_below(fm)] = nb;
+    __syncthreads();
+    batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
+    __syncthreads();
+    nd += (uint32_t)k;
+    const float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
+    const uint32_t myid = (lane < k) ? S.tid[lane] : U32_NONE;
+    const int hit = (lane < k) && (dv <= P.R2f);
+    const uint64_t hm = ballot(hit);
+    const int nh = popc64(hm);
+    if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
+    if (hit) fifo_put(P, S, rcnt + lanes_below(hm), myid);
+    rcnt += (uint32_t)nh;
+    uint64_t key = (lane < k) ? pkey(dv, myid) : ~0ull;
+    if (bsize == L) {   // drop candidates that cannot enter the full beam
+        uint64_t last = 0;
+#pragma unroll
+        for (int j = 0; j < NCH; j++)
+            if ((uint32_t)j == ((L - 1) >> 6)) last = readlane64(B.k[j], (int)((L - 1) & 63));
+        if (!(key < last)) key = ~0ull;
+    }
+    const uint64_t km = ballot(key != ~0ull);
+    if (km == 0) return true;
+    uint32_t sc[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; j++) sc[j] = 0;
+    uint32_t mypos = U32_NONE;
+    for (uint64_t mm = km; mm; mm &= mm - 1) {
+        const int src = __ffsll((long long)mm) - 1;
+        const uint64_t cv = readlane64(key, src);
+        uint32_t pos = (uint32_t)popc64(ballot(key < cv));
+#pragma unroll
+        for (int j = 0; j < NCH; j++) {
+            pos += (uint32_t)popc64(ballot(B.k[j] < cv));
+            sc[j] += (cv < B.k[j]) ? 1u : 0u;
+        }
+        if (lane == src) mypos = pos;
+    }
+    uint64_t* kb = (uint64_t*)S.bd0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t np = 64u * j + (uint32_t)lane + sc[j];
+        if (B.k[j] != ~0ull && np < L) kb[np] = B.k[j] | (((B.x >> j) & 1u) ? RB_FLAG : 0ull);
+    }
+    if (mypos < L) kb[mypos] = key;
+    bsize = min(L, bsize + (uint32_t)popc64(km));
+    __syncthreads();
+    B.x = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t pos = 64u * j + (uint32_t)lane;
+        uint64_t v = ~0ull;
+        if (pos < bsize) {
+            v = kb[pos];
+            if (v & RB_FLAG) B.x |= 1u << j;
+            v &= ~RB_FLAG;
+        }
+        B.k[j] = v;
+    }
+    __syncthreads();
+    return true;
+}
+
+// Flood step over the nb (<= SR_FLOOD) FIFO entries at head: the visited-set
+// inserts run node by node in FIFO order, neighbours in adjacency order, so
+// every neighbour is attributed to the same first visitor as in a one-node-
+// at-a-time flood; collected nodes are appended in that same order.  Their
+// distances are computed in one batch.
 template <int MC>
+__device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t head,
+                                           uint32_t nb, uint32_t& rcnt, uint32_t& nd, int& ovf) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    uint32_t dg[SR_FLOOD], nbr[SR_FLOOD];
+#pragma unroll
+    for (int t = 0; t < SR_FLOOD; t++) {
+        dg[t] = 0;
+        nbr[t] = U32_NONE;
+        if ((uint32_t)t < nb) {
+            const uint32_t c = fifo_get(P, S, head + t);
+            dg[t] = P.deg[c];
+            nbr[t] = ((uint32_t)lane < P.M) ? P.links[(size_t)c * P.M + lane] : U32_NONE;
+        }
+    }
+    int k = 0;
+#pragma unroll
+    for (int t = 0; t < SR_FLOOD; t++) {
+        int fresh = 0;
+        if ((uint32_t)lane < dg[t]) {
+            int r = hset_insert(S.hash, P.hmask, nbr[t]);
+            if (r < 0) ovf = 1;
+            fresh = (r == 1);
+        }
+        const uint64_t fm = ballot(fresh);
+        if (fresh) S.tid[k + lanes_below(fm)] = nbr[t];
+        k += popc64(fm);
+    }
+    if (__any(ovf)) { ovf = 1; return false; }
+    __syncthreads();
+    batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
+    __syncthreads();
+    nd += (uint32_t)k;
+    for (int b = 0; b < k; b += 64) {
+        const int s = b + lane;
+        const int hit = s < k && S.tdv[s] <= P.R2f;
+        const uint64_t hm = ballot(hit);
+        const int nh = popc64(hm);
+        if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
+        if (hit) fifo_put(P, S, rcnt + lanes_below(hm), S.tid[s]);
+        rcnt += (uint32_t)nh;
+    }
+    __syncthreads();
+    return true;
+}
+
+// NCH == 0: beam sorted in LDS (plain form); NCH > 0: register beam of
+// 64 NCH positions (persistent form, L <= 64 NCH).
+template <int MC, int NCH>
 __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     const int lane = __lane_id(), l = lane & 15;
-    const uint32_t L = P.Lq ? P.Lq[qi] : P.L;
+    S.spill = P.cand_ids + (size_t)qi * P.out_cap;
+    // per-query beam width, clamped to 1..Lmax (the LDS beam areas' size)
+    const uint32_t L = max(1u, min(P.Lq ? P.Lq[qi] : P.L, P.Lmax));
     const float* thr = P.thr + (P.kidx ? (size_t)P.kidx[qi] * P.n : 0);
     // query transform (rknn_query.cpp:2719-2730), binary64 product rounded to float
     const float* q = P.queries + (size_t)qi * P.d;
@@ -182,14 +361,65 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     int buf = 0, ovf = 0;
     if (lane == 0) {
         S.hash[hslot(ep, P.hmask)] = ep + 1u;
-        S.bd0[0] = dep;
-        S.bp0[0] = ep;
-        if (dep <= P.R2f) S.res[0] = ep;
+        if constexpr (NCH == 0) {
+            S.bd0[0] = dep;
+            S.bp0[0] = ep;
+        }
+        if (dep <= P.R2f) fifo_put(P, S, 0, ep);
     }
     if (dep <= P.R2f) rcnt = 1;
     __syncthreads();
     // (1) beam descent
-    for (;;) {
+    if constexpr (NCH > 0) {
+        RBeam<NCH> B;
+#pragma unroll
+        for (int j = 0; j < NCH; j++) B.k[j] = (j == 0 && lane == 0) ? pkey(dep, ep) : ~0ull;
+        B.x = 0;
+        // first unexpanded beam entry (U32_NONE if none); marks it expanded
+        auto take_next = [&](bool mark) -> uint32_t {
+            int cj = -1, cl = 0;
+#pragma unroll
+            for (int j = 0; j < NCH; j++) {
+                if (cj < 0) {
+                    const uint64_t m = ballot(B.k[j] != ~0ull && !((B.x >> j) & 1u));
+                    if (m) { cj = j; cl = __ffsll((long long)m) - 1; }
+                }
+            }
+            if (cj < 0) return U32_NONE;
+            uint32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < NCH; j++) {
+                if (j == cj) {
+                    c = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)B.k[j], cl);
+                    if (mark && lane == cl) B.x |= 1u << j;
+                }
+            }
+            return c;
+        };
+        // The adjacency of the entry that is next in line once c is marked
+        // is fetched while c is expanded; it is used when no new neighbour
+        // of c overtakes it.
+        uint32_t pc = U32_NONE, pdg = 0, pnb = U32_NONE;
+        for (;;) {
+            const uint32_t c = take_next(true);
+            if (c == U32_NONE) break;
+            uint32_t dg, nbr;
+            if (c == pc) {
+                dg = pdg;
+                nbr = pnb;
+            } else {
+                dg = P.deg[c];
+                nbr = ((uint32_t)lane < P.M) ? P.links[(size_t)c * P.M + lane] : U32_NONE;
+            }
+            pc = take_next(false);
+            if (pc != U32_NONE) {
+                pdg = P.deg[pc];
+                pnb = ((uint32_t)lane < P.M) ? P.links[(size_t)pc * P.M + lane] : U32_NONE;
+            }
+            nx++;
+            if (!expand_rb<NCH, MC>(P, S, QL, dg, nbr, rcnt, nd, B, bsize, L, ovf)) break;
+        }
+    } else for (;;) {
         uint32_t cur = U32_NONE;
         for (uint32_t b = 0; b < bsize; b += 64) {
             uint32_t i = b + lane;
@@ -207,9 +437,10 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     // (2) flood over collected nodes, FIFO order
     uint32_t head = 0;
     while (!ovf && head < rcnt) {
-        uint32_t c = S.res[head++];
-        nx++;
-        if (!expand<false, MC>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+        const uint32_t nb = min((uint32_t)SR_FLOOD, rcnt - head);
+        if (!flood_step<MC>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
+        head += nb;
+        nx += nb;
     }
     if (ovf) {
         if (lane == 0) { P.status[qi] = (uint32_t)ovf; P.cand_cnt[qi] = rcnt; P.ndist[qi] = nd; P.pred_cnt[qi] = 0; }
@@ -227,7 +458,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
             uint32_t j = b + (uint32_t)(lane >> 4);
             int ok = 0;
             if (j < rcnt) {
-                uint32_t o = S.res[j];
+                uint32_t o = fifo_get(P, S, j);
                 float ip = qrow_ip(QB, P.base + (size_t)o * P.d, P.d, l);
                 float key = fmaf(-2.0f, ip, P.bnorm[o] + qn);
                 ok = key <= thr[o];
@@ -248,7 +479,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                 const uint32_t s = b + (uint32_t)(g + 4 * j);
                 oid[j] = U32_NONE;
                 if (s < rcnt) {
-                    const uint32_t o = S.res[s];
+                    const uint32_t o = fifo_get(P, S, s);
                     oid[j] = o;
                     const float* row = P.base + (size_t)o * P.d;
 #pragma unroll
@@ -299,12 +530,12 @@ __global__ void __launch_bounds__(64) k_range_search(SearchParams P) {
     SearchLds S = carve(lds, P);
     if (blockIdx.x >= P.nq) return;
     uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
-    search_one<MC>(P, S, qi);
+    search_one<MC, 0>(P, S, qi);
 }
 
 // Persistent form: a fixed grid pulls query indices from a device counter.
 // Every wave exits once the counter passes nq.
-template <int MC>
+template <int MC, int NCH>
 __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) {
     extern __shared__ __align__(16) char lds[];
     SearchLds S = carve(lds, P);
@@ -317,7 +548,7 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
         __syncthreads();
         if (t >= nq) break;
         uint32_t qi = P.qlist ? P.qlist[t] : t;
-        search_one<MC>(P, S, qi);
+        search_one<MC, NCH>(P, S, qi);
         __syncthreads();
     }
 }
@@ -329,10 +560,17 @@ static int search_chunks(const SearchParams& P) {
     return (m == 8 || m == 6) ? m : 0;
 }
 
+// Register-beam chunks for beams up to Lmax (0: Lmax > 512, LDS beam).
+static int search_nch(uint32_t Lmax) {
+    return Lmax <= 64 ? 1 : Lmax <= 128 ? 2 : Lmax <= 256 ? 4 : Lmax <= 512 ? 8 : 0;
+}
+
+// P.rbeam must already say which form runs (it changes the beam areas).
 size_t search_lds_bytes(const SearchParams& P) {
     auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
-    return r(4 * (P.dl + 16)) + r(4 * (P.d + 16)) + 2 * r(4 * P.Lmax) + 2 * r(4 * P.Lmax) +
-           r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) + r(4 * 64) + r(4 * 64) + r(8 * 64);
+    const size_t beam = P.rbeam ? r(8 * (size_t)P.Lmax) : 4 * r(4 * (size_t)P.Lmax) + r(8 * 64);
+    return r(4 * (P.dl + 16)) + r(4 * (P.d + 16)) + beam + r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) +
+           r(4 * SR_FLOOD * 64) + r(4 * SR_FLOOD * 64);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -340,13 +578,27 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     if (i < n) p[i] = v;
 }
 
+template <int MC, int NCH>
+static hipError_t launch_persistent_t(const SearchParams& P, uint32_t grid, size_t lds, hipStream_t s) {
+    hipFuncSetAttribute((const void*)k_range_search_persistent<MC, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+    hipLaunchKernelGGL((k_range_search_persistent<MC, NCH>), dim3(grid), dim3(64), lds, s, P);
+    return hipSuccess;
+}
+
 template <int MC>
-static hipError_t launch_search_mc(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s) {
+static hipError_t launch_search_mc(const SearchParams& P0, bool persistent, uint32_t grid, hipStream_t s) {
+    SearchParams P = P0;
+    P.rbeam = persistent && search_nch(P.Lmax) > 0;
     size_t lds = search_lds_bytes(P);
     if (persistent) {
-        hipFuncSetAttribute((const void*)k_range_search_persistent<MC>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_range_search_persistent<MC>, dim3(grid), dim3(64), lds, s, P);
+        switch (search_nch(P.Lmax)) {
+            case 1: launch_persistent_t<MC, 1>(P, grid, lds, s); break;
+            case 2: launch_persistent_t<MC, 2>(P, grid, lds, s); break;
+            case 4: launch_persistent_t<MC, 4>(P, grid, lds, s); break;
+            case 8: launch_persistent_t<MC, 8>(P, grid, lds, s); break;
+            default: launch_persistent_t<MC, 0>(P, grid, lds, s); break;
+        }
     } else {
         if (P.nq == 0) return hipSuccess;
         hipFuncSetAttribute((const void*)k_range_search<MC>,
@@ -370,20 +622,33 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int MC>
-static int occupancy_mc(size_t lds) {
+template <int MC, int NCH>
+static int occupancy_t(size_t lds) {
     int occ = 0;
-    const void* f = (const void*)k_range_search_persistent<MC>;
+    const void* f = (const void*)k_range_search_persistent<MC, NCH>;
     hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 64, lds) != hipSuccess) occ = 1;
     return occ;
 }
 
-int search_persistent_occupancy(const SearchParams& P) {
+template <int MC>
+static int occupancy_mc(const SearchParams& P, size_t lds) {
+    switch (search_nch(P.Lmax)) {
+        case 1: return occupancy_t<MC, 1>(lds);
+        case 2: return occupancy_t<MC, 2>(lds);
+        case 4: return occupancy_t<MC, 4>(lds);
+        case 8: return occupancy_t<MC, 8>(lds);
+        default: return occupancy_t<MC, 0>(lds);
+    }
+}
+
+int search_persistent_occupancy(const SearchParams& P0) {
+    SearchParams P = P0;
+    P.rbeam = search_nch(P.Lmax) > 0;
     const size_t lds = search_lds_bytes(P);
     switch (search_chunks(P)) {
-        case 8: return occupancy_mc<8>(lds);
-        case 6: return occupancy_mc<6>(lds);
-        default: return occupancy_mc<0>(lds);
+        case 8: return occupancy_mc<8>(P, lds);
+        case 6: return occupancy_mc<6>(P, lds);
+        default: return occupancy_mc<0>(P, lds);
     }
 }

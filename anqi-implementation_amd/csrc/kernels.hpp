@@ -41,6 +41,7 @@ struct SearchParams {
     uint32_t* status;       // nq: 0 ok, 1 hash full, 2 FIFO full, 3 output truncated
     uint32_t* work_counter; // persistent form
     const uint32_t* nq_dev; // optional: batch size read on the device (captured batches)
+    uint32_t rbeam;         // set by launch_range_search: register beam (persistent form)
 };
 size_t search_lds_bytes(const SearchParams& P);
 int search_persistent_occupancy(const SearchParams& P);
@@ -118,9 +119,18 @@ struct GatherParams {
     uint32_t* status;       // n: 0 ok, 1 hash full, 2 pool full
     uint32_t* gscratch;     // optional global work area (per block)
     size_t gscratch_words;  // words per block
+    uint32_t batched;       // 1: one expansion's first visits processed together
+    uint32_t* vbits;        // launch_gather_gv: per-block visited bitmaps (grid x vwords), zeroed
+    uint32_t vwords;        // ceil(n / 32)
+    uint32_t* vlist;        // per-block first-visit lists (grid x vcap)
+    uint32_t vcap;          // >= pool_cap + cap + longest reverse list
 };
 size_t gather_lds_bytes(const GatherParams& P, bool graph_mode);
 hipError_t launch_gather(const GatherParams& P, bool graph_mode, bool global_ws, hipStream_t s);
+// AKNN-mode gather with the visited sets as global bitmaps (LDS holds only
+// the queue and the top-K), persistent grid.
+int gather_gv_occupancy(const GatherParams& P);
+hipError_t launch_gather_gv(const GatherParams& P, uint32_t grid, hipStream_t s);
 
 struct PruneParams {
     const float* x;
@@ -176,6 +186,7 @@ struct LeafParams {
     float* keys;
     uint32_t* cnt;              // n
     uint32_t max_leaf;
+    unsigned long long* nsurv;  // MFMA sweeps: pairs passing the filter (may be null)
 };
 size_t leaf_lds_bytes(const LeafParams& P);
 hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s);
@@ -183,6 +194,11 @@ hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s);
 bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf);
 size_t leaf_mfma_lds(uint32_t K);
 hipError_t launch_leaf_mfma(const LeafParams& P, const uint16_t* xb, const float* marg, hipStream_t s);
+// FP8 (OCP e4m3) trial of the same sweep; x8 = e4m3(x / scale), scale a power of two.
+hipError_t launch_leaf_mfma_fp8(const LeafParams& P, const uint8_t* x8, float scale, const float* marg,
+                                hipStream_t s);
+hipError_t launch_to_fp8(const float* x, const float* nrm, uint32_t n, uint32_t d, float scale, uint8_t* x8,
+                         float* nx, float* ex, hipStream_t s);
 hipError_t launch_random_fill(const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t K,
                               uint64_t seed, uint32_t* ids, float* keys, uint32_t* cnt,
                               hipStream_t s);

@@ -46,7 +46,8 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     log(f"aknn source {t1 - t0:.2f}s")
     t["aknn_detail"] = dict(forest_s=st.forest_s, leaf_s=st.leaf_s, gather_s=st.gather_s,
                             leaf_pairs=int(st.leaf_pairs), leaves=int(st.leaves), max_leaf=int(st.max_leaf),
-                            short_rows=int(st.short_rows), overflow_rows=int(st.overflow_rows))
+                            short_rows=int(st.short_rows), overflow_rows=int(st.overflow_rows),
+                            leaf_survivors=int(st.leaf_survivors))
     mu, G, cb, fit_s = lrq_fit.fit(cut, log=lambda *a: None)
     _sync(ctx)
     t2 = time.perf_counter()
@@ -65,7 +66,8 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     t["lift_s"] = t4 - t3
     log(f"encode {t3 - t2:.2f}s lift {t4 - t3:.2f}s")
     links, deg, tree, gst = ctx.graph_build(lifted, ids, M=M, nn_k=nn_k, leaf=graph_leaf, explore=explore,
-                                            ef_alpha=ef_alpha, iters=iters, sample=sample, alpha=alpha, seed=seed)
+                                            ef_alpha=ef_alpha, iters=iters, sample=sample, alpha=alpha, seed=seed,
+                                            form=1 if precision >= 1 else 0)
     _sync(ctx)
     t5 = time.perf_counter()
     t["graph_s"] = t5 - t4

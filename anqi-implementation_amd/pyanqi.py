@@ -30,7 +30,7 @@ class AknnParams(ctypes.Structure):
 class AknnStats(ctypes.Structure):
     _fields_ = [("forest_s", F64), ("leaf_s", F64), ("gather_s", F64), ("total_s", F64),
                 ("leaf_pairs", U64), ("leaves", U32), ("max_leaf", U32), ("short_rows", U32),
-                ("overflow_rows", U32)]
+                ("overflow_rows", U32), ("leaf_survivors", U64)]
 
 
 class Tree(ctypes.Structure):
@@ -41,7 +41,7 @@ class Tree(ctypes.Structure):
 
 class GraphParams(ctypes.Structure):
     _fields_ = [("M", U32), ("nn_k", U32), ("leaf", U32), ("explore", U32), ("ef_alpha", U32),
-                ("iters", I32), ("sample", I32), ("alpha", F32), ("seed", U64)]
+                ("iters", I32), ("sample", I32), ("alpha", F32), ("seed", U64), ("form", I32)]
 
 
 class GraphStats(ctypes.Structure):
@@ -67,6 +67,7 @@ SIGS = {
     "anqi_ctx_set_stream": (I32, [P, P]),
     "anqi_ctx_stream": (P, [P]),
     "anqi_ctx_reset_stream": (I32, [P]),
+    "anqi_ctx_set_search_tuning": (I32, [P, P]),
     "anqi_dmalloc": (I32, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "anqi_dfree": (I32, [P, P]),
     "anqi_h2d": (I32, [P, P, P, ctypes.c_size_t]),
@@ -84,8 +85,8 @@ SIGS = {
     "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
     "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32)]),
     "anqi_reverse_csr": (I32, [P, P, U32, U32, P, U32, P, P]),
-    "anqi_gather_topk": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, P, P, P]),
-    "anqi_gather_prune": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, F32, P, P]),
+    "anqi_gather_topk": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, P, P, P, I32]),
+    "anqi_gather_prune": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, F32, P, P, I32]),
     "anqi_commit": (I32, [P, P, P, U32, U32, P, P, U32, F32, P, P]),
     "anqi_rp_tree": (I32, [P, P, U32, U32, U32, U32, U64, ctypes.POINTER(Tree)]),
     "anqi_tree_upload": (I32, [P, U32, U32, U32, P, P, P, P, P, ctypes.POINTER(Tree)]),
@@ -268,21 +269,21 @@ class Context:
         self.call("anqi_reverse_csr", dp(ids), rows, width, dp(rowcnt), n, dp(roff), dp(rids))
         return roff, rids
 
-    def gather_topk(self, x, nrm, hidx, hval, roff, rids, explore, pool_cap, K):
+    def gather_topk(self, x, nrm, hidx, hval, roff, rids, explore, pool_cap, K, form=0):
         n, d = x.shape
         ids = self.empty((n, K), self.torch.int32)
         keys = self.empty((n, K), self.torch.float32)
         cnt = self.empty((n,), self.torch.int32)
         self.call("anqi_gather_topk", dp(x), dp(nrm), n, d, dp(hidx), dp(hval), hidx.shape[1], dp(roff), dp(rids),
-                  explore, pool_cap, K, dp(ids), dp(keys), dp(cnt))
+                  explore, pool_cap, K, dp(ids), dp(keys), dp(cnt), form)
         return ids, keys, cnt
 
-    def gather_prune(self, x, nrm, hidx, hval, roff, rids, explore, pool_cap, M, alpha):
+    def gather_prune(self, x, nrm, hidx, hval, roff, rids, explore, pool_cap, M, alpha, form=0):
         n, d = x.shape
         O = self.empty((n, M), self.torch.int32)
         ocnt = self.empty((n,), self.torch.int32)
         self.call("anqi_gather_prune", dp(x), dp(nrm), n, d, dp(hidx), dp(hval), hidx.shape[1], dp(roff), dp(rids),
-                  explore, pool_cap, M, alpha, dp(O), dp(ocnt))
+                  explore, pool_cap, M, alpha, dp(O), dp(ocnt), form)
         return O, ocnt
 
     def commit(self, x, nrm, O, ocnt, M, alpha):
@@ -317,6 +318,9 @@ class Context:
         self.call("anqi_search_autotune", ctypes.byref(ix), dp(thr_tables), dp(queries), queries.shape[0], dp(Lq),
                   dp(kidx), Lmax, ctypes.byref(t), ctypes.byref(ms))
         return t, ms.value
+
+    def set_search_tuning(self, tuning=None):
+        self.call("anqi_ctx_set_search_tuning", ctypes.byref(tuning) if tuning is not None else None)
 
     def batcher(self, ix, thr_tables, max_batch, Lmax, out_cap=4096, tuning=None):
         return Batcher(self, ix, thr_tables, max_batch, Lmax, out_cap, tuning)
@@ -364,13 +368,13 @@ class Context:
         self.call("anqi_tree_free", ctypes.byref(t))
 
     def graph_build(self, lifted, warm_ids, M=48, nn_k=50, leaf=100, explore=100, ef_alpha=8, iters=6, sample=20,
-                    alpha=1.2, seed=10):
+                    alpha=1.2, seed=10, form=0):
         n, dl = lifted.shape
         links = self.empty((n, M), self.torch.int32)
         deg = self.empty((n,), self.torch.int32)
         t = Tree()
         st = GraphStats()
-        p = GraphParams(M, nn_k, leaf, explore, ef_alpha, iters, sample, alpha, seed)
+        p = GraphParams(M, nn_k, leaf, explore, ef_alpha, iters, sample, alpha, seed, form)
         self.call("anqi_graph_build", dp(lifted), n, dl, dp(warm_ids), warm_ids.shape[1], ctypes.byref(p),
                   dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
         return links, deg, t, st

@@ -216,6 +216,12 @@ def main():
     point = [m for m in sweep if m["L"] == chosen_L][0]
     ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=1, bufs=bufs)
     B = query_bytes(ix, bufs)
+    # chosen form: persistent launch with autotuned work areas and grid
+    tune, tune_ms = ctx.search_autotune(ix.handle, thr, qd, None, None, chosen_L)
+    ctx.set_search_tuning(tune)
+    tuning = {"waves_per_cu": tune.waves_per_cu, "visited_slots": tune.visited_slots,
+              "result_cap": tune.result_cap, "autotune_best_ms": tune_ms}
+    log("autotune", json.dumps(tuning))
 
     def timed(mode):
         for _ in range(a.warmup):
@@ -267,7 +273,7 @@ def main():
             log("cpu baseline failed:", repr(e))
     build_c = ix.times["build_s"]
     other = {"chosen_query_qps": qps_c, "plain_query_qps": qps_p, "query_qps_ratio": qps_c / qps_p,
-             "chosen_build_s": build_c}
+             "chosen_build_s": build_c, "chosen_search_tuning": tuning}
     if plain_build is not None:
         other.update({"plain_build_s": plain_build["build_s"], "build_s_ratio": build_c / plain_build["build_s"],
                       "plain_build_breakdown_s": {k: plain_build[k] for k in ("aknn_source_s", "verifier_train_s",

@@ -44,6 +44,17 @@ int anqi_ctx_sync(anqi_ctx* ctx);
 int anqi_ctx_set_stream(anqi_ctx* ctx, void* stream);
 int anqi_ctx_reset_stream(anqi_ctx* ctx);
 void* anqi_ctx_stream(anqi_ctx* ctx);
+/* Work areas and grid of the persistent search (mode 1 of anqi_rknn_search
+ * and the batcher); anqi_search_autotune measures candidates.  Zero fields
+ * keep the defaults. */
+typedef struct {
+    uint32_t waves_per_cu;    /* persistent grid = waves_per_cu x CUs (0: occupancy) */
+    uint32_t visited_slots;   /* LDS visited-set slots (0: from L) */
+    uint32_t result_cap;      /* LDS candidate FIFO entries (0: 1024); later
+                                 entries go to the query's output row */
+} anqi_search_tuning;
+/* Tuning used by later mode-1 searches on this context (NULL: defaults). */
+int anqi_ctx_set_search_tuning(anqi_ctx* ctx, const anqi_search_tuning* tune);
 int anqi_dmalloc(anqi_ctx* ctx, size_t bytes, void** out);
 int anqi_dfree(anqi_ctx* ctx, void* p);
 int anqi_h2d(anqi_ctx* ctx, void* dst, const void* src, size_t bytes);
@@ -77,12 +88,14 @@ typedef struct {
     uint32_t ef_alpha;   /* pool cap = max(K, ef_alpha * ef), 8 */
     uint32_t max_depth;  /* 100 */
     uint64_t seed;       /* 10 */
-    int precision;       /* 0 FP32 leaf tiles, 1 bf16 MFMA filtered */
+    int precision;       /* 0 FP32 leaf tiles, 1 bf16 MFMA filtered, 2 FP8-e4m3
+                            MFMA filtered (trial); every mode gives the same lists */
 } anqi_aknn_params;
 typedef struct {
     double forest_s, leaf_s, gather_s, total_s;
     uint64_t leaf_pairs;
     uint32_t leaves, max_leaf, short_rows, overflow_rows;
+    uint64_t leaf_survivors;  /* precision 1/2: ordered pairs the MFMA filter let through */
 } anqi_aknn_stats;
 /* device x (n*d); device ids/keys (n*K) rows sorted by (dist, id), self
  * excluded; keys are the raw d2 (the tool clamps at 0 when writing .rec). */
@@ -142,12 +155,12 @@ int anqi_gather_topk(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n
                      const uint32_t* hidx, const float* hval, uint32_t cap,
                      const uint32_t* roff, const uint32_t* rids, uint32_t explore,
                      uint32_t pool_cap, uint32_t K, uint32_t* ids, float* keys,
-                     uint32_t* cnt);
+                     uint32_t* cnt, int form);
 int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
                       const uint32_t* hidx, const float* hval, uint32_t cap,
                       const uint32_t* roff, const uint32_t* rids, uint32_t explore,
                       uint32_t pool_cap, uint32_t M, float alpha, uint32_t* O,
-                      uint32_t* ocnt);
+                      uint32_t* ocnt, int form);
 int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
                 const uint32_t* O, const uint32_t* ocnt, uint32_t M, float alpha,
                 uint32_t* links, uint32_t* deg);
@@ -181,6 +194,8 @@ typedef struct {
     int sample;        /* NNSAMPLE 20 */
     float alpha;       /* 1.2 */
     uint64_t seed;     /* 10 */
+    int form;          /* 0 plain (per-chunk gather), 1 chosen (per-expansion
+                          batched gather); both give the same graph */
 } anqi_graph_params;
 typedef struct {
     double tree_s, warm_s, nnd_s, gather_s, commit_s, total_s;
@@ -188,7 +203,8 @@ typedef struct {
     int64_t nnd_upd[16];
     uint32_t overflow_rows;
 } anqi_graph_stats;
-/* device lifted (n*dl), warm_ids (n*width); device links (n*M), deg (n). */
+/* device lifted (n*dl), warm_ids (n*width); device links (n*M), deg (n);
+ * link slots past deg[i] hold UINT32_MAX. */
 int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl,
                      const uint32_t* warm_ids, uint32_t width, const anqi_graph_params* p,
                      uint32_t* links, uint32_t* deg, anqi_tree* tree, anqi_graph_stats* st);
@@ -239,11 +255,6 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
  * areas, and returns the accepted ids of every query as host CSR
  * (h_off[nq+1], h_ids[*total], candidate order). The index and tables must
  * outlive the batcher; one thread per batcher. */
-typedef struct {
-    uint32_t waves_per_cu;    /* persistent grid = waves_per_cu x CUs (0: occupancy) */
-    uint32_t visited_slots;   /* LDS visited-set slots (0: from Lmax) */
-    uint32_t result_cap;      /* LDS candidate FIFO entries (0: 1024) */
-} anqi_search_tuning;
 typedef struct anqi_batcher anqi_batcher;
 int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables, uint32_t ntables,
                         uint32_t max_batch, uint32_t Lmax, uint32_t out_cap,
@@ -252,7 +263,7 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
                      uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total);
 void anqi_batcher_destroy(anqi_batcher* b);
 /* Times the persistent search on a sample batch (device queries, Lq, kidx)
- * for visited-set sizes {2048, 4096, 8192}, FIFO sizes {512, 1024} and
+ * for visited-set sizes {2048, 4096, 8192}, LDS FIFO sizes {128, 512} and
  * 1/2, 3/4, 4/4 of the resident waves per CU; queries that overflow are
  * charged as reruns.  Writes the fastest tuning and its time. */
 int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables,

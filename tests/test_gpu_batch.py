@@ -82,3 +82,40 @@ def test_autotuned_batcher_same_answers(env):
     eoff, eids = expected(env, q, L, kidx)
     assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
     bat.close()
+
+
+@pytest.mark.parametrize("L", [1, 2, 63, 64, 65, 100, 128, 129, 256, 257, 512, 513, 1024])
+def test_register_beam_equals_lds_beam(env, L):
+    """The persistent form keeps the beam in registers (64 positions per
+    chunk); the per-query form keeps it sorted in LDS.  Same candidate lists
+    in the same order, accept decisions, #dist and #expansions at every beam
+    width, including widths one past a chunk boundary and the LDS fallback
+    above 512."""
+    ctx = env["ctx"]
+    q = ctx.to_dev(env["q"][:1500])
+    thr = env["tables"][0].contiguous()
+    a = ctx.rknn_search(env["ix"].handle, q, L, thr, out_cap=4096, mode=0)
+    a = {k: v.cpu().numpy() for k, v in a.items()}
+    b = ctx.rknn_search(env["ix"].handle, q, L, thr, out_cap=4096, mode=1)
+    b = {k: v.cpu().numpy() for k, v in b.items()}
+    say(f"L={L}: mean cand {a['cnt'].mean():.1f}, dist {a['nd'].mean():.1f}, expansions {a['nexp'].mean():.1f}")
+    for f in ("cnt", "nd", "nexp", "pred", "status"):
+        assert np.array_equal(a[f], b[f]), f
+    for i in range(a["cnt"].shape[0]):
+        c = int(a["cnt"][i])
+        assert np.array_equal(a["cand"][i, :c], b["cand"][i, :c])
+        assert np.array_equal(a["acc"][i, :c], b["acc"][i, :c])
+
+
+def test_chosen_build_equals_plain(env):
+    """The chosen build (bf16 MFMA leaf sweep, batched gathers) yields the
+    plain build's index bit for bit: AKNN lists, cutoffs, LRQ codes, lifted
+    rows, graph links and degrees."""
+    import pipeline
+    ctx, a = env["ctx"], env["ix"]
+    b = pipeline.build(ctx, a.base_np, precision=1, log=say)
+    for f in ("aknn_ids", "cut", "codes", "lifted", "links", "deg"):
+        x, y = getattr(a, f).cpu().numpy(), getattr(b, f).cpu().numpy()
+        assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
+    say("plain vs chosen build s:", round(a.times["build_s"], 3), round(b.times["build_s"], 3))
+    ctx.tree_free(b.tree)

@@ -237,6 +237,9 @@ def _graph_steps(ctx, x, warm, tag):
     say(tag, "gather + prune")
     Og, oc = ctx.gather_prune(xd, nd_, hi, hv, roff, rids, 100, 800, 48, 1.2)
     Og, oc = host(Og).view(np.uint32), host(oc).view(np.uint32)
+    # the batched form (one expansion's first visits processed together)
+    Og1, oc1 = ctx.gather_prune(xd, nd_, hi, hv, roff, rids, 100, 800, 48, 1.2, form=1)
+    assert np.array_equal(host(oc1), oc) and np.array_equal(host(Og1).view(np.uint32), Og)
     O_o = np.zeros((n, 48), np.uint32)
     oc_o = np.zeros(n, np.uint32)
     pmax = 8 * n
@@ -261,7 +264,8 @@ def _graph_steps(ctx, x, warm, tag):
         assert links[i, :deg[i]].tolist() == l_o[i, :d_o[i]].tolist(), (tag, i)
 
 
-def test_gather_topk_matches_oracle(ctx, S):
+@pytest.mark.parametrize("form", [0, 1], ids=["per-chunk", "batched"])
+def test_gather_topk_matches_oracle(ctx, S, form):
     O = oracle()
     x = S.base
     n, d = x.shape
@@ -280,7 +284,8 @@ def test_gather_topk_matches_oracle(ctx, S):
     O.or_reverse_csr(ptr(hi), n, cap, ptr(roff), ptr(rids))
     say("gather topk")
     ids, keys, cnt = ctx.gather_topk(ctx.to_dev(x), ctx.to_dev(nrm), ctx.to_dev(hi.view(np.int32)), ctx.to_dev(hv),
-                                     ctx.to_dev(roff.view(np.int32)), ctx.to_dev(rids.view(np.int32)), 64, 512, 100)
+                                     ctx.to_dev(roff.view(np.int32)), ctx.to_dev(rids.view(np.int32)), 64, 512, 100,
+                                     form=form)
     ids, keys, cnt = host(ids).view(np.uint32), host(keys), host(cnt)
     for i in range(n):
         pd = np.zeros(8 * n, np.float32)
@@ -329,11 +334,14 @@ def test_aknn_build_sample_recall(ctx, S):
     assert rec >= rec_ref - 0.005
 
 
+@pytest.mark.parametrize("precision", [1, 2], ids=["bf16", "fp8"])
 @pytest.mark.parametrize("case", ["sample-256x128", "mixture-5000x128", "gaussian-3000x96", "wide-norms-2000x128"])
-def test_aknn_mfma_leaf_sweep_equals_fp32(ctx, S, case):
-    """The bf16 MFMA leaf sweep (precision 1) leaves every row's list exactly
-    as the FP32 sweep does: identical AKNN ids and key bits after the whole
-    build (leaf all-pairs over every tree, then the gather)."""
+def test_aknn_mfma_leaf_sweep_equals_fp32(ctx, S, case, precision):
+    """The MFMA leaf sweeps (precision 1: bf16, 2: FP8-e4m3 trial) leave
+    every row's list exactly as the FP32 sweep does: identical AKNN ids and
+    key bits after the whole build (leaf all-pairs over every tree, then the
+    gather).  Reports the fraction of ordered leaf pairs the filter let
+    through to the exact re-score."""
     rng = np.random.default_rng(11)
     x = {"sample-256x128": lambda: S.base,
          "mixture-5000x128": lambda: workload.mixture(5000, 128, 40, 20.0, seed=3),
@@ -345,6 +353,9 @@ def test_aknn_mfma_leaf_sweep_equals_fp32(ctx, S, case):
     xd = ctx.to_dev(x)
     K = 100 if x.shape[0] > 300 else 50
     a_ids, a_keys, _ = ctx.aknn_build(xd, K=K, leaf=200, trees=8, ef=64, precision=0)
-    b_ids, b_keys, _ = ctx.aknn_build(xd, K=K, leaf=200, trees=8, ef=64, precision=1)
+    b_ids, b_keys, st = ctx.aknn_build(xd, K=K, leaf=200, trees=8, ef=64, precision=precision)
+    say(f"  precision {precision}: {st.leaf_survivors} of {2 * st.leaf_pairs} ordered leaf pairs re-scored "
+        f"({st.leaf_survivors / max(1, 2 * st.leaf_pairs):.3f}), leaf sweep {st.leaf_s * 1e3:.1f} ms")
+    assert 0 < st.leaf_survivors <= 2 * st.leaf_pairs
     assert np.array_equal(host(a_ids), host(b_ids))
     assert np.array_equal(bits(host(a_keys)), bits(host(b_keys)))
