@@ -524,23 +524,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     P.pmax = graph ? std::min<uint32_t>(2048, next_pow2(need + 64)) : 0;
     P.rows = nullptr;
     P.nrows = P.n;
-    DBuf vbits, vlist;
-    if (!graph && P.batched && need < (1u << 24)) {
-        // batched AKNN form: visited sets as per-block bitmaps in HBM, so LDS
-        // holds only the queue and the top-K and more rows run at once
-        const int occ = std::max(1, gather_gv_occupancy(P));
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(P.n, (uint64_t)occ * std::max(1, ctx->cus));
-        P.vwords = (P.n + 31) / 32;
-        P.vcap = (uint32_t)need;
-        HCHK(vbits.alloc(4ull * grid * P.vwords));
-        HCHK(vlist.alloc(4ull * grid * P.vcap));
-        HCHK(hipMemsetAsync(vbits.p, 0, 4ull * grid * P.vwords, s));
-        P.vbits = vbits.as<uint32_t>();
-        P.vlist = vlist.as<uint32_t>();
-        HCHK(launch_gather_gv(P, grid, s));
-    } else {
-        HCHK(launch_gather(P, graph, false, s));
-    }
+    HCHK(launch_gather(P, graph, false, s));
     std::vector<uint32_t> bad;
     int rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
     if (rc) return rc;
@@ -550,7 +534,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     GatherParams Q = P;
     Q.hmask = next_pow2(need * 2 + 64) - 1;
     Q.pmax = graph ? next_pow2(need + 64) : 0;
-    size_t words = (gather_lds_bytes(Q, graph) + 3) / 4;
+    size_t words = (gather_lds_bytes(Q, graph, false) + 3) / 4;
     DBuf rows, ws;
     HCHK(rows.alloc(4 * bad.size()));
     HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
@@ -971,12 +955,17 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     for (auto& lv : F.leaves)
         for (auto& pr : lv) max_leaf_all = std::max(max_leaf_all, pr.second);
     if (p->precision < 0 || p->precision > 2) return anqi_set_err(ctx, ANQI_E_USAGE, "precision %d", p->precision);
-    const bool use_mfma = p->precision >= 1 && leaf_mfma_supported(d, max_leaf_all);
+    const bool use_mfma = p->precision >= 1 && leaf_mfma_supported(d, max_leaf_all, K);
     const bool use_fp8 = use_mfma && p->precision == 2;
-    DBuf xb, lmarg, nsurv;
+    DBuf xb, lmarg, nsurv, cbuf, ccnt;
     float fp8_scale = 1.0f;
     if (use_mfma) {
         DBuf nx, ex;
+        // candidate lists between the sweep and merge passes (a row's
+        // survivors in one tree are fewer than its leaf's size)
+        HCHK(cbuf.alloc(4ull * n * max_leaf_all));
+        HCHK(ccnt.alloc(4ull * n));
+        HCHK(hipMemsetAsync(ccnt.p, 0, 4ull * n, s));
         HCHK(xb.alloc((use_fp8 ? 1ull : 2ull) * n * d));
         HCHK(nx.alloc(4ull * n));
         HCHK(ex.alloc(4ull * n));
@@ -1031,9 +1020,11 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
         LP.K = K; LP.ids = lid.as<uint32_t>(); LP.keys = lkey.as<float>(); LP.cnt = lcnt.as<uint32_t>();
         LP.max_leaf = mx;
         LP.nsurv = use_mfma ? nsurv.as<unsigned long long>() : nullptr;
-        if (use_fp8) HCHK(launch_leaf_mfma_fp8(LP, xb.as<uint8_t>(), fp8_scale, lmarg.as<float>(), s));
-        else if (use_mfma) HCHK(launch_leaf_mfma(LP, xb.as<uint16_t>(), lmarg.as<float>(), s));
-        else HCHK(launch_leaf_knn(LP, false, s));
+        if (use_mfma)
+            HCHK(launch_leaf_mfma(LP, use_fp8, xb.p, fp8_scale, lmarg.as<float>(), cbuf.as<uint32_t>(),
+                                  ccnt.as<uint32_t>(), max_leaf_all, s));
+        else
+            HCHK(launch_leaf_knn(LP, false, s));
         HCHK(hipStreamSynchronize(s));   // keeps ds/dz alive until the launch ends
         if ((t & 7) == 7 || t + 1 == T) progress("leaf all-pairs: tree %u/%u", t + 1, T);
     }

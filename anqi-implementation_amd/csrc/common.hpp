@@ -248,6 +248,127 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
     return v;
 }
 
+// ---- register-resident sorted key lists -------------------------------------
+// A list of up to 64 NCH (dist, id) keys, sorted ascending: lane l holds
+// positions 64 j + l, j < NCH (~0 past the end), plus one bit per chunk (x)
+// for "expanded".  Merging kc candidate keys needs no sort and no LDS search:
+// for each candidate (scalar loop) one ballot per chunk counts the list keys
+// below it and every list lane counts the candidates below its key; the rank
+// among the candidates is one more ballot.  Keys then move to their new
+// positions through one LDS write and one read per chunk.  Single wavefront.
+#define RK_FLAG 0x80000000ull   // expanded bit, kept in bit 31 of the id while in LDS
+
+template <int NCH>
+struct RKeys {
+    uint64_t k[NCH];
+    uint32_t x;   // bit j: position 64 j + lane expanded
+};
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// LDS ordering between lanes of one wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+template <int NCH>
+__device__ __forceinline__ void rk_clear(RKeys<NCH>& B) {
+#pragma unroll
+    for (int j = 0; j < NCH; j++) B.k[j] = ~0ull;
+    B.x = 0;
+}
+
+// key at a (wave-uniform) position
+template <int NCH>
+__device__ __forceinline__ uint64_t rk_get(const RKeys<NCH>& B, uint32_t pos) {
+    uint64_t v = ~0ull;
+#pragma unroll
+    for (int j = 0; j < NCH; j++)
+        if ((uint32_t)j == (pos >> 6)) v = readlane64(B.k[j], (int)(pos & 63));
+    return v;
+}
+
+// Merges the lanes' candidate keys (~0 = none; none equal to a list key)
+// into the list of `size` entries, capacity cap <= 64 NCH: candidates that
+// cannot enter a full list are dropped first.  kb: LDS exchange area of cap
+// entries.  Returns the number of candidates that entered the merge.
+template <int NCH>
+__device__ __forceinline__ uint32_t rk_merge(RKeys<NCH>& B, uint64_t key, uint32_t& size, uint32_t cap,
+                                             uint64_t* kb) {
+    const int lane = __lane_id();
+    if (size == cap && !(key < rk_get(B, cap - 1))) key = ~0ull;
+    const uint64_t km = ballot(key != ~0ull);
+    if (km == 0) return 0;
+    uint32_t sc[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; j++) sc[j] = 0;
+    uint32_t mypos = U32_NONE;
+    for (uint64_t mm = km; mm; mm &= mm - 1) {
+        const int src = __ffsll((long long)mm) - 1;
+        const uint64_t cv = readlane64(key, src);
+        uint32_t pos = (uint32_t)popc64(ballot(key < cv));
+#pragma unroll
+        for (int j = 0; j < NCH; j++) {
+            pos += (uint32_t)popc64(ballot(B.k[j] < cv));
+            sc[j] += (cv < B.k[j]) ? 1u : 0u;
+        }
+        if (lane == src) mypos = pos;
+    }
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t np = 64u * j + (uint32_t)lane + sc[j];
+        if (B.k[j] != ~0ull && np < cap) kb[np] = B.k[j] | (((B.x >> j) & 1u) ? RK_FLAG : 0ull);
+    }
+    if (mypos < cap) kb[mypos] = key;
+    const uint32_t kc = (uint32_t)popc64(km);
+    size = min(cap, size + kc);
+    wave_lds_sync();
+    B.x = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t pos = 64u * j + (uint32_t)lane;
+        uint64_t v = ~0ull;
+        if (pos < size) {
+            v = kb[pos];
+            if (v & RK_FLAG) B.x |= 1u << j;
+            v &= ~RK_FLAG;
+        }
+        B.k[j] = v;
+    }
+    wave_lds_sync();
+    return kc;
+}
+
+// Id of the first unexpanded entry (U32_NONE if none); mark: flag it expanded.
+template <int NCH>
+__device__ __forceinline__ uint32_t rk_take(RKeys<NCH>& B, bool mark) {
+    const int lane = __lane_id();
+    int cj = -1, cl = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        if (cj < 0) {
+            const uint64_t m = ballot(B.k[j] != ~0ull && !((B.x >> j) & 1u));
+            if (m) { cj = j; cl = __ffsll((long long)m) - 1; }
+        }
+    }
+    if (cj < 0) return U32_NONE;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        if (j == cj) {
+            c = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)B.k[j], cl);
+            if (mark && lane == cl) B.x |= 1u << j;
+        }
+    }
+    return c;
+}
+
 // ---- counter-based RNG ------------------------------------------------------
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z ^= z >> 30;

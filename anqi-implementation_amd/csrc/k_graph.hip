@@ -339,9 +339,15 @@ struct GatherLds {
 
 __host__ __device__ __forceinline__ size_t rnd16(size_t b) { return (b + 15) & ~size_t(15); }
 
-__host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P, bool graph) {
-    // hash + queues; in graph mode also the selected vectors of the prune step
-    size_t a = rnd16(4 * (size_t)(P.hmask + 1)) + 4 * rnd16(4 * (size_t)P.explore);
+// Register-list form of the batched gather (queue <= 640 entries, AKNN
+// top-K <= 128): LDS then holds only one exchange area per list.
+__host__ __device__ __forceinline__ bool gather_rk(const GatherParams& P, bool graph) {
+    return P.batched && P.explore <= 640 && (graph || P.K <= 128);
+}
+
+__host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P, bool graph, bool rk) {
+    // hash + queue; in graph mode also the selected vectors of the prune step
+    size_t a = rnd16(4 * (size_t)(P.hmask + 1)) + (rk ? rnd16(8 * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
     size_t b = graph ? rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) : 0;
     return a > b ? a : b;
 }
@@ -351,28 +357,38 @@ __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P,
 #define GX_CAP 256
 __host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P) { return P.batched ? GX_CAP : 64; }
 
-size_t gather_lds_bytes(const GatherParams& P, bool graph) {
-    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : 4 * rnd16(4 * (size_t)P.K);
-    return gather_region0(P, graph) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P)) + rnd16(8 * 64);
+size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
+    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(8 * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
+    return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P)) + (rk ? 0 : rnd16(8 * 64));
 }
 
-__device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams& P, bool graph) {
+// rk: register lists; the queue's exchange area is qd0 (8 explore bytes), the
+// top-K's is pd (8 K bytes); the other list arrays are absent.
+__device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams& P, bool graph, bool rk) {
     GatherLds s;
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base + o; o += rnd16(bytes); return p; };
     s.hash = (uint32_t*)take(4 * (size_t)(P.hmask + 1));
-    s.qd0 = (float*)take(4 * (size_t)P.explore);
-    s.qd1 = (float*)take(4 * (size_t)P.explore);
-    s.qp0 = (uint32_t*)take(4 * (size_t)P.explore);
-    s.qp1 = (uint32_t*)take(4 * (size_t)P.explore);
+    if (rk) {
+        s.qd0 = (float*)take(8 * (size_t)P.explore);
+        s.qd1 = nullptr; s.qp0 = nullptr; s.qp1 = nullptr;
+    } else {
+        s.qd0 = (float*)take(4 * (size_t)P.explore);
+        s.qd1 = (float*)take(4 * (size_t)P.explore);
+        s.qp0 = (uint32_t*)take(4 * (size_t)P.explore);
+        s.qp1 = (uint32_t*)take(4 * (size_t)P.explore);
+    }
     s.selv = (float*)base;
     s.sel = (uint32_t*)(base + rnd16(4 * (size_t)P.M * P.d));
-    o = gather_region0(P, graph);
+    o = gather_region0(P, graph, rk);
     if (graph) {
         s.pd = (float*)take(4 * (size_t)P.pmax);
         s.pi = (uint32_t*)take(4 * (size_t)P.pmax);
         s.pd2 = nullptr;
         s.pi2 = nullptr;
+    } else if (rk) {
+        s.pd = (float*)take(8 * (size_t)P.K);
+        s.pi = nullptr; s.pd2 = nullptr; s.pi2 = nullptr;
     } else {
         s.pd = (float*)take(4 * (size_t)P.K);
         s.pi = (uint32_t*)take(4 * (size_t)P.K);
@@ -381,7 +397,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
     }
     s.tid = (uint32_t*)take(4 * (size_t)gather_tcap(P));
     s.tdv = (float*)take(4 * (size_t)gather_tcap(P));
-    s.cs = (uint64_t*)take(8 * 64);
+    s.cs = rk ? nullptr : (uint64_t*)take(8 * 64);
     return s;
 }
 
@@ -450,24 +466,24 @@ __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uin
     return deg;
 }
 
-// One row of the gather.  GV: the visited set is a bitmap in global memory
-// (vb, one bit per base row, owned by this wave) instead of the LDS hash;
-// first visits are listed in vl so the touched words are cleared at the end.
-template <bool GRAPH, bool GWS, int MC, bool GV>
-__device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, uint32_t i, uint32_t* vb,
-                                           uint32_t* vl) {
+// One row of the gather.  QCH > 0: the beam queue is a register list of
+// QCH chunks (explore <= 64 QCH); PCH > 0: so is the AKNN top-K (K <= 64 PCH).
+// Otherwise both live sorted in LDS (double buffered, binary-search merges).
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
+__device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, uint32_t i) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const int d = (int)P.d;
-    if constexpr (!GV) {
-        for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
-    }
-    uint32_t vcount = 0;
+    for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
     __syncthreads();
     QRow QI;
     qrow_load(QI, P.x + (size_t)i * d, d, l);
     const float ni = P.nrm[i];
     uint32_t qsize = 0, psize = 0, pcount = 0;
     int qb = 0, pb = 0, status = 0;
+    RKeys<(QCH > 0 ? QCH : 1)> Q;
+    RKeys<(PCH > 0 ? PCH : 1)> PL;
+    rk_clear(Q);
+    rk_clear(PL);
 
     // visit: a chunk of candidate ids (lane-held, in reference order) goes
     // through the visited set; first visits are appended to tid[nf..] (with
@@ -482,27 +498,9 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     auto visit = [&](uint32_t id, int valid, float seed_d) __attribute__((always_inline)) -> int {
         int fresh = 0;
         if (valid) {
-            if constexpr (GV) {
-                const uint32_t bit = 1u << (id & 31u);
-                fresh = !(atomicOr(&vb[id >> 5], bit) & bit);
-            } else {
-                int r = hset_insert(S.hash, P.hmask, id);
-                if (r < 0) status = 1;
-                fresh = (r == 1);
-            }
-        }
-        if constexpr (GV) {
-            // vcap bounds the first visits of a row (pool_cap + cap + longest
-            // reverse list); guarded anyway
-            const uint64_t vm = ballot(fresh);
-            if (vcount + (uint32_t)popc64(vm) > P.vcap) {
-                if (fresh) atomicAnd(&vb[id >> 5], ~(1u << (id & 31u)));
-                status = 1;
-                fresh = 0;
-            } else if (fresh) {
-                vl[vcount + lanes_below(vm)] = id;
-            }
-            vcount += (uint32_t)popc64(ballot(fresh));
+            int r = hset_insert(S.hash, P.hmask, id);
+            if (r < 0) status = 1;
+            fresh = (r == 1);
         }
         if (__any(status)) { status = 1; return 0; }
         uint64_t fm = ballot(fresh);
@@ -536,6 +534,8 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
                 if (psize + (uint32_t)k > P.pmax) { status = 2; return 0; }
                 if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
                 psize += (uint32_t)k;
+            } else if constexpr (PCH > 0) {
+                rk_merge(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint64_t*)S.pd);
             } else {
                 uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
                 float* ad = pb ? S.pd2 : S.pd;
@@ -553,7 +553,9 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
             }
             pcount += (uint32_t)k;
             // queue
-            {
+            if constexpr (QCH > 0) {
+                rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
+            } else {
                 uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
                 float* ad = S.qd(qb);
                 uint32_t* ap = S.qp(qb);
@@ -584,17 +586,23 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     }
     // beam over the neighbour heaps and their reverse lists
     while (!status) {
-        uint32_t cur = U32_NONE;
-        for (uint32_t b = 0; b < qsize; b += 64) {
-            uint32_t t = b + lane;
-            uint64_t m = ballot(t < qsize && !(S.qp(qb)[t] >> 31));
-            if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
+        uint32_t c;
+        if constexpr (QCH > 0) {
+            c = rk_take(Q, true);
+            if (c == U32_NONE) break;
+        } else {
+            uint32_t cur = U32_NONE;
+            for (uint32_t b = 0; b < qsize; b += 64) {
+                uint32_t t = b + lane;
+                uint64_t m = ballot(t < qsize && !(S.qp(qb)[t] >> 31));
+                if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
+            }
+            if (cur == U32_NONE) break;
+            c = S.qp(qb)[cur];
+            __syncthreads();
+            if (lane == 0) S.qp(qb)[cur] = c | 0x80000000u;
+            __syncthreads();
         }
-        if (cur == U32_NONE) break;
-        uint32_t c = S.qp(qb)[cur];
-        __syncthreads();
-        if (lane == 0) S.qp(qb)[cur] = c | 0x80000000u;
-        __syncthreads();
         for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
             uint32_t s = s0 + lane;
             uint32_t id = s < P.cap ? P.hidx[(size_t)c * P.cap + s] : U32_NONE;
@@ -611,23 +619,27 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         if (!status) process(false);
         if (pcount >= P.pool_cap) break;
     }
-    if constexpr (GV) {
-        // clear the words this row set; the wave's next row starts from zero
-        __syncthreads();
-        for (uint32_t j = lane; j < vcount; j += 64) vb[vl[j] >> 5] = 0u;
-        __threadfence();
-        __syncthreads();
-    }
     if (status) {
         if (lane == 0) { P.status[i] = (uint32_t)status; P.out_cnt[i] = 0; }
         return;
     }
     if (!GRAPH) {
-        const float* ad = pb ? S.pd2 : S.pd;
-        const uint32_t* ap = pb ? S.pi2 : S.pi;
-        for (uint32_t t = lane; t < P.K; t += 64) {
-            P.out_ids[(size_t)i * P.K + t] = t < psize ? ap[t] : U32_NONE;
-            P.out_keys[(size_t)i * P.K + t] = t < psize ? ad[t] : F32_BIG;
+        if constexpr (PCH > 0) {
+#pragma unroll
+            for (int j = 0; j < PCH; j++) {
+                const uint32_t t = 64u * j + (uint32_t)lane;
+                if (t < P.K) {
+                    P.out_ids[(size_t)i * P.K + t] = t < psize ? pkey_id(PL.k[j]) : U32_NONE;
+                    P.out_keys[(size_t)i * P.K + t] = t < psize ? pkey_d(PL.k[j]) : F32_BIG;
+                }
+            }
+        } else {
+            const float* ad = pb ? S.pd2 : S.pd;
+            const uint32_t* ap = pb ? S.pi2 : S.pi;
+            for (uint32_t t = lane; t < P.K; t += 64) {
+                P.out_ids[(size_t)i * P.K + t] = t < psize ? ap[t] : U32_NONE;
+                P.out_keys[(size_t)i * P.K + t] = t < psize ? ad[t] : F32_BIG;
+            }
         }
         if (lane == 0) { P.out_cnt[i] = psize; P.status[i] = 0; }
         return;
@@ -653,95 +665,49 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
 }
 
-template <bool GRAPH, bool GWS, int MC>
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
 __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     extern __shared__ __align__(16) char lds_raw[];
     uint32_t bi = blockIdx.x;
     if (bi >= P.nrows) return;
     uint32_t i = P.rows ? P.rows[bi] : bi;
     char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
-    GatherLds S = gather_carve(base, P, GRAPH);
-    gather_row<GRAPH, GWS, MC, false>(P, S, i, nullptr, nullptr);
+    GatherLds S = gather_carve(base, P, GRAPH, QCH > 0);
+    gather_row<GRAPH, GWS, MC, QCH, PCH>(P, S, i);
 }
 
-// AKNN gather, persistent: block b owns bitmap b and takes rows b, b + grid, ...
-template <int MC>
-__global__ void __launch_bounds__(64) k_gather_gv(GatherParams P) {
-    extern __shared__ __align__(16) char lds_raw[];
-    GatherLds S = gather_carve(lds_raw, P, false);
-    uint32_t* vb = P.vbits + (size_t)blockIdx.x * P.vwords;
-    uint32_t* vl = P.vlist + (size_t)blockIdx.x * P.vcap;
-    for (uint32_t bi = blockIdx.x; bi < P.nrows; bi += gridDim.x) {
-        const uint32_t i = P.rows ? P.rows[bi] : bi;
-        gather_row<false, false, MC, true>(P, S, i, vb, vl);
-        __syncthreads();
-    }
-}
-
-template <int MC>
-static int gather_gv_occ(size_t lds) {
-    int occ = 0;
-    const void* f = (const void*)k_gather_gv<MC>;
-    hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 64, lds) != hipSuccess) occ = 1;
-    return occ;
-}
-
-// Resident blocks per CU of the bitmap-visited AKNN gather (P.hmask ignored).
-int gather_gv_occupancy(const GatherParams& P0) {
-    GatherParams P = P0;
-    P.hmask = 0;
-    const size_t lds = gather_lds_bytes(P, false);
-    switch (P.d >> 4) {
-        case 8: return gather_gv_occ<8>(lds);
-        case 6: return gather_gv_occ<6>(lds);
-        default: return gather_gv_occ<0>(lds);
-    }
-}
-
-// P.vbits: grid x vwords zeroed words; P.vlist: grid x vcap.
-hipError_t launch_gather_gv(const GatherParams& P0, uint32_t grid, hipStream_t s) {
-    if (P0.nrows == 0) return hipSuccess;
-    GatherParams P = P0;
-    P.hmask = 0;
-    const size_t lds = gather_lds_bytes(P, false);
-    switch (P.d >> 4) {
-        case 8:
-            hipFuncSetAttribute((const void*)k_gather_gv<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(k_gather_gv<8>, dim3(grid), dim3(64), lds, s, P);
-            break;
-        case 6:
-            hipFuncSetAttribute((const void*)k_gather_gv<6>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(k_gather_gv<6>, dim3(grid), dim3(64), lds, s, P);
-            break;
-        default:
-            hipFuncSetAttribute((const void*)k_gather_gv<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(k_gather_gv<0>, dim3(grid), dim3(64), lds, s, P);
-            break;
-    }
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
+static hipError_t launch_gather_t(const GatherParams& P, size_t lds, hipStream_t s) {
+    const void* f = (const void*)k_gather<GRAPH, GWS, MC, QCH, PCH>;
+    if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC, QCH, PCH>), dim3(P.nrows), dim3(64), lds, s, P);
     return hipGetLastError();
 }
 
 template <bool GRAPH, bool GWS, int MC>
-static hipError_t launch_gather_t(const GatherParams& P, size_t lds, hipStream_t s) {
-    const void* f = (const void*)k_gather<GRAPH, GWS, MC>;
-    if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC>), dim3(P.nrows), dim3(64), lds, s, P);
-    return hipGetLastError();
+static hipError_t launch_gather_lists(const GatherParams& P, size_t lds, hipStream_t s) {
+    if constexpr (!GWS) {
+        if (gather_rk(P, GRAPH)) {
+            constexpr int PC = GRAPH ? 0 : 2;
+            if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC>(P, lds, s);
+            return launch_gather_t<GRAPH, GWS, MC, 10, PC>(P, lds, s);
+        }
+    }
+    return launch_gather_t<GRAPH, GWS, MC, 0, 0>(P, lds, s);
 }
 
 template <bool GRAPH, bool GWS>
 static hipError_t launch_gather_mc(const GatherParams& P, size_t lds, hipStream_t s) {
     switch (P.d >> 4) {
-        case 8: return launch_gather_t<GRAPH, GWS, 8>(P, lds, s);
-        case 6: return launch_gather_t<GRAPH, GWS, 6>(P, lds, s);
-        default: return launch_gather_t<GRAPH, GWS, 0>(P, lds, s);
+        case 8: return launch_gather_lists<GRAPH, GWS, 8>(P, lds, s);
+        case 6: return launch_gather_lists<GRAPH, GWS, 6>(P, lds, s);
+        default: return launch_gather_lists<GRAPH, GWS, 0>(P, lds, s);
     }
 }
 
 hipError_t launch_gather(const GatherParams& P, bool graph, bool gws, hipStream_t s) {
     if (P.nrows == 0) return hipSuccess;
-    size_t lds = gws ? 0 : gather_lds_bytes(P, graph);
+    size_t lds = gws ? 0 : gather_lds_bytes(P, graph, gather_rk(P, graph));
     if (graph) return gws ? launch_gather_mc<true, true>(P, lds, s) : launch_gather_mc<true, false>(P, lds, s);
     return gws ? launch_gather_mc<false, true>(P, lds, s) : launch_gather_mc<false, false>(P, lds, s);
 }

@@ -1,21 +1,25 @@
 // k_leaf_mfma.hip -- RP-forest leaf all-pairs (init_knn, include/index.h:
-// 1433-1449) with the pair sweep on the matrix cores.
+// 1433-1449) with the pair sweep on the matrix cores, two passes per tree.
 //
-// Same result as k_leaf_knn (the FP32 sweep): for every row of the leaf, the
-// leaf members whose exact (key, id) beats the row's current K-th pair are
-// merged into its sorted top-K list.  Exact keys use the reference's FP32
-// contract; the bf16 MFMA sweep only skips pairs that provably cannot beat
-// that threshold (approx > K-th key + M_r, with M_r the per-row bound of
-// k_exact_margin).  Survivors are re-scored exactly; several flushes per row
-// and tree are allowed because a candidate rejected by a tighter threshold
-// could not have stayed among the K best anyway.
+// Same result as k_leaf_knn (the FP32 sweep): after every tree, each row's
+// sorted top-K list holds the K best (key, id) pairs among the pairs seen so
+// far.  Exact keys use the reference's FP32 contract.
 //
-// One block (4 wavefronts) per leaf.  A wavefront takes 32-member row blocks
-// of the leaf; for each it sweeps the leaf's 32-member column tiles with
-// v_mfma_f32_32x32x16_bf16 (operands gathered from a bf16 copy of the base
-// through the tree permutation), keeps per-row survivor lists in its own LDS
-// region and flushes them (exact re-score, dedup against the list, merge)
-// when a list could overflow on the next tile, and at the end.
+// Pass 1 (k_leaf_sweep): one block (4 wavefronts) per leaf.  A wavefront
+// takes 32-member row blocks of the leaf and sweeps the leaf's 32-member
+// column tiles with v_mfma_f32_32x32x16_bf16 (operands gathered from a bf16
+// copy of the base through the tree permutation; the FP8 trial uses
+// v_mfma_f32_32x32x16_fp8_fp8 on e4m3(x / s)).  A pair survives unless its
+// approximate key exceeds the row's K-th key + M_r (M_r: the per-row bound of
+// k_exact_margin); survivors are appended to the row's candidate list in HBM
+// (the wave owns the row within this tree: every row sits in one leaf).
+// The thresholds are those before the tree; a skipped pair has exact key >
+// that K-th key >= every later K-th key, so it could never be listed.
+//
+// Pass 2 (k_leaf_merge): one wavefront per row with candidates: exact keys
+// (16 lanes per candidate, four candidates per group in flight), pairs
+// already listed dropped (identical pair -> identical key bits), the rest
+// merged into the row's top-K held as a register list, the count reset.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -23,119 +27,6 @@ typedef __bf16 lm_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float lm_f32x16 __attribute__((ext_vector_type(16)));
 
 #define LM_KS 8     // 16-wide k steps: d <= 128, zero padded
-#define LM_SV 64    // survivor slots per row
-
-__host__ __device__ inline size_t lm_r16(size_t b) { return (b + 15) & ~size_t(15); }
-
-// per-wave LDS region
-__host__ __device__ inline size_t lm_wave_bytes(uint32_t K) {
-    return lm_r16(2ull * 32 * LM_SV) + lm_r16(4ull * 32) + lm_r16(4ull * 64) + lm_r16(4ull * 64) +
-           4 * lm_r16(4ull * K) + lm_r16(8ull * 64);
-}
-
-#define LM_WSYNC()                                              \
-    do {                                                        \
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  \
-        __builtin_amdgcn_wave_barrier();                        \
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  \
-    } while (0)
-
-struct LmWave {
-    uint16_t* sv;    // 32 x LM_SV member indices
-    uint32_t* svc;   // 32 counts
-    uint32_t* xid;   // 64
-    float* xd;       // 64
-    float* ld0;      // K
-    uint32_t* li0;
-    float* ld1;
-    uint32_t* li1;
-    uint64_t* cs;    // 64
-};
-
-__device__ inline LmWave lm_carve(char* base, uint32_t K) {
-    LmWave s;
-    size_t o = 0;
-    auto take = [&](size_t b) { char* p = base + o; o += lm_r16(b); return p; };
-    s.sv = (uint16_t*)take(2ull * 32 * LM_SV);
-    s.svc = (uint32_t*)take(4ull * 32);
-    s.xid = (uint32_t*)take(4ull * 64);
-    s.xd = (float*)take(4ull * 64);
-    s.ld0 = (float*)take(4ull * K);
-    s.li0 = (uint32_t*)take(4ull * K);
-    s.ld1 = (float*)take(4ull * K);
-    s.li1 = (uint32_t*)take(4ull * K);
-    s.cs = (uint64_t*)take(8ull * 64);
-    return s;
-}
-
-// Exact re-score of row r's survivors, dedup against its list, merge.
-template <int MC>
-__device__ void lm_flush_row(const LeafParams& P, LmWave& W, const uint32_t* mem, int rl, uint32_t r) {
-    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
-    const uint32_t K = P.K, d = P.d;
-    const int k = (int)W.svc[rl];
-    if (lane < k) W.xid[lane] = mem[W.sv[rl * LM_SV + lane]];
-    LM_WSYNC();
-    QRow QR;
-    qrow_load(QR, P.x + (size_t)r * d, (int)d, l);
-    const float nr = P.nrm[r];
-    rows_batch<MC, 1>(QR, P.x, d, P.nrm, nr, W.xid, k, l, g, W.xd);
-    LM_WSYNC();
-    uint32_t* rid = P.ids + (size_t)r * K;
-    float* rkd = P.keys + (size_t)r * K;
-    const uint32_t cnt = P.cnt[r];
-    const uint64_t thr = cnt >= K ? pkey(rkd[K - 1], rid[K - 1]) : ~0ull;
-    uint64_t key = ~0ull;
-    if (lane < k) {
-        uint64_t pk = pkey(W.xd[lane], W.xid[lane]);
-        if (pk < thr) key = pk;
-    }
-    key = wave_sort64(key);
-    const int nc = popc64(ballot(key != ~0ull));
-    if (lane == 0) W.svc[rl] = 0;
-    if (nc == 0) { LM_WSYNC(); return; }
-    for (uint32_t t = lane; t < cnt; t += 64) { W.ld0[t] = rkd[t]; W.li0[t] = rid[t]; }
-    LM_WSYNC();
-    // drop candidates already present (identical pair -> identical bits)
-    int keep = 0;
-    if (lane < nc) {
-        uint32_t lo = 0, hi = cnt;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (pkey(W.ld0[mid], W.li0[mid]) < key) lo = mid + 1; else hi = mid;
-        }
-        keep = !(lo < cnt && W.li0[lo] == pkey_id(key));
-    }
-    const uint64_t km = ballot(keep);
-    const int kept = popc64(km);
-    if (keep) W.cs[lanes_below(km)] = key;
-    LM_WSYNC();
-    if (kept) {
-        for (uint32_t t = lane; t < cnt; t += 64) {
-            uint64_t bk = pkey(W.ld0[t], W.li0[t]);
-            int lo = 0, hi = kept;
-            while (lo < hi) { int mid = (lo + hi) >> 1; if (W.cs[mid] < bk) lo = mid + 1; else hi = mid; }
-            uint32_t pos = t + (uint32_t)lo;
-            if (pos < K) { W.ld1[pos] = W.ld0[t]; W.li1[pos] = W.li0[t]; }
-        }
-        if (lane < kept) {
-            const uint64_t ck = W.cs[lane];
-            uint32_t lo = 0, hi = cnt;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (pkey(W.ld0[mid], W.li0[mid]) < ck) lo = mid + 1; else hi = mid;
-            }
-            uint32_t pos = (uint32_t)lane + lo;
-            if (pos < K) { W.ld1[pos] = pkey_d(ck); W.li1[pos] = pkey_id(ck); }
-        }
-        LM_WSYNC();
-        const uint32_t ncnt = min(K, cnt + (uint32_t)kept);
-        for (uint32_t t = lane; t < ncnt; t += 64) { rkd[t] = W.ld1[t]; rid[t] = W.li1[t]; }
-        if (lane == 0) P.cnt[r] = ncnt;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-    LM_WSYNC();
-}
 
 // Operand fragments: 8 k-values per lane per 16-wide step, as bf16 (16 B) or,
 // in the FP8 trial, as OCP e4m3 bytes (8 B) of x / s with s a power of two
@@ -168,16 +59,17 @@ __device__ __forceinline__ lm_f32x16 lm_mma(const LmFrag<true>& a, const LmFrag<
 }
 
 template <int MC, bool F8>
-__global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const void* __restrict__ xq, float s2,
-                                                   const float* __restrict__ marg) {
-    extern __shared__ __align__(16) char lds[];
+__global__ void __launch_bounds__(256) k_leaf_sweep(LeafParams P, const void* __restrict__ xq, float s2,
+                                                    const float* __restrict__ marg, uint32_t* __restrict__ cbuf,
+                                                    uint32_t* __restrict__ ccnt, uint32_t capc) {
+    __shared__ uint32_t gcnt[4][32];
     const uint32_t leaf = blockIdx.x;
     if (leaf >= P.nleaves) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    LmWave W = lm_carve(lds + lm_wave_bytes(P.K) * w, P.K);
     const uint32_t m = P.leaf_size[leaf], K = P.K, d = P.d;
     const uint32_t* mem = P.perm + P.leaf_start[leaf];
     const uint32_t nrb = (m + 31) / 32;
+    uint32_t* gc = gcnt[w];
     uint32_t nsv = 0;   // pairs this lane let through the filter
     for (uint32_t rb = (uint32_t)w; rb < nrb; rb += 4) {
         // A fragments of the block's 32 members
@@ -191,30 +83,17 @@ __global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const void* __r
         }
         uint32_t rid[16];
         float bnr[16], thrm[16];
-        auto load_thr = [&]() {
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                float t = __builtin_huge_valf();
-                if (rid[v] != U32_NONE && P.cnt[rid[v]] >= K) t = P.keys[(size_t)rid[v] * K + K - 1] + marg[rid[v]];
-                thrm[v] = t;
-            }
-        };
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const uint32_t a = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
             rid[v] = a < m ? mem[a] : U32_NONE;
             bnr[v] = rid[v] != U32_NONE ? P.nrm[rid[v]] : 0.0f;
+            float t = __builtin_huge_valf();
+            if (rid[v] != U32_NONE && P.cnt[rid[v]] >= K) t = P.keys[(size_t)rid[v] * K + K - 1] + marg[rid[v]];
+            thrm[v] = t;
         }
-        load_thr();
-        if (lane < 32) W.svc[lane] = 0;
-        LM_WSYNC();
-        auto flush = [&]() {
-            for (int rl = 0; rl < 32; rl++) {
-                if (W.svc[rl] == 0) continue;
-                lm_flush_row<MC>(P, W, mem, rl, mem[rb * 32 + rl]);
-            }
-            load_thr();
-        };
+        if (lane < 32) gc[lane] = 0;
+        wave_lds_sync();
         for (uint32_t ct = 0; ct < nrb; ct++) {
             const uint32_t cm = ct * 32 + (lane & 31);
             const uint32_t crow = cm < m ? mem[cm] : U32_NONE;
@@ -236,16 +115,15 @@ __global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const void* __r
                 const float approx = fmaf(-2.0f, acc[v] * s2, bnr[v] + bc);
                 if (a < m && cm < m && cm != a && approx <= thrm[v]) {
                     const int rl = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                    uint32_t p = atomicAdd(&W.svc[rl], 1u);
-                    W.sv[rl * LM_SV + p] = (uint16_t)cm;
+                    const uint32_t slot = atomicAdd(&gc[rl], 1u);
+                    cbuf[(size_t)rid[v] * capc + slot] = crow;
                     nsv++;
                 }
             }
-            LM_WSYNC();
-            // a row gains at most 32 survivors per tile: flush before one could overflow
-            if (__any(lane < 32 && W.svc[lane] > LM_SV - 32)) flush();
         }
-        flush();
+        wave_lds_sync();
+        if (lane < 32 && rb * 32 + lane < m) ccnt[mem[rb * 32 + lane]] = gc[lane];
+        wave_lds_sync();
     }
     if (P.nsurv) {
         for (int o = 32; o > 0; o >>= 1) nsv += __shfl_xor(nsv, o);
@@ -253,43 +131,106 @@ __global__ void __launch_bounds__(256) k_leaf_mfma(LeafParams P, const void* __r
     }
 }
 
-size_t leaf_mfma_lds(uint32_t K) { return 4 * lm_wave_bytes(K); }
+// Drops the lanes' candidate keys that equal a key of the list.
+template <int NCH>
+__device__ __forceinline__ uint64_t rk_drop_listed(const RKeys<NCH>& B, uint64_t key) {
+    const int lane = __lane_id();
+    uint64_t drop = 0;
+    for (uint64_t mm = ballot(key != ~0ull); mm; mm &= mm - 1) {
+        const int src = __ffsll((long long)mm) - 1;
+        const uint64_t cv = readlane64(key, src);
+        uint64_t hit = 0;
+#pragma unroll
+        for (int j = 0; j < NCH; j++) hit |= ballot(B.k[j] == cv);
+        if (hit) drop |= 1ull << src;
+    }
+    return ((drop >> lane) & 1ull) ? ~0ull : key;
+}
 
-bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf) {
+// Pass 2: rows r with ccnt[r] > 0; one wavefront (block of 64) per row.
+template <int MC, int PCH>
+__global__ void __launch_bounds__(64) k_leaf_merge(LeafParams P, uint32_t* __restrict__ cbuf,
+                                                   uint32_t* __restrict__ ccnt, uint32_t capc) {
+    __shared__ uint32_t tid[64];
+    __shared__ float tdv[64];
+    __shared__ uint64_t kb[64 * PCH];
+    const uint32_t r = blockIdx.x;
+    if (r >= P.n) return;
+    const uint32_t c = ccnt[r];
+    if (c == 0) return;
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const uint32_t K = P.K, d = P.d;
+    QRow QR;
+    qrow_load(QR, P.x + (size_t)r * d, (int)d, l);
+    const float nr = P.nrm[r];
+    uint32_t* rid = P.ids + (size_t)r * K;
+    float* rkd = P.keys + (size_t)r * K;
+    uint32_t size = P.cnt[r];
+    RKeys<PCH> B;
+    B.x = 0;
+#pragma unroll
+    for (int j = 0; j < PCH; j++) {
+        const uint32_t t = 64u * j + (uint32_t)lane;
+        B.k[j] = t < size ? pkey(rkd[t], rid[t]) : ~0ull;
+    }
+    uint32_t added = 0;
+    for (uint32_t b = 0; b < c; b += 64) {
+        const int k = (int)min(64u, c - b);
+        if (lane < k) tid[lane] = cbuf[(size_t)r * capc + b + lane];
+        wave_lds_sync();
+        rows_batch<MC, 1>(QR, P.x, d, P.nrm, nr, tid, k, l, g, tdv);
+        wave_lds_sync();
+        uint64_t key = lane < k ? pkey(tdv[lane], tid[lane]) : ~0ull;
+        key = rk_drop_listed(B, key);
+        added += rk_merge(B, key, size, K, kb);
+    }
+    if (added) {
+#pragma unroll
+        for (int j = 0; j < PCH; j++) {
+            const uint32_t t = 64u * j + (uint32_t)lane;
+            if (t < size) { rkd[t] = pkey_d(B.k[j]); rid[t] = pkey_id(B.k[j]); }
+        }
+        if (lane == 0) P.cnt[r] = size;
+    }
+    if (lane == 0) ccnt[r] = 0;
+}
+
+bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf, uint32_t K) {
     const uint32_t mc = d >> 4;
-    return d <= 16 * LM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && max_leaf <= 65535;
+    return d <= 16 * LM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && max_leaf <= 65535 && K <= 256;
 }
 
 template <int MC, bool F8>
-static hipError_t launch_leaf_mfma_t(const LeafParams& P, const void* xq, float s2, const float* marg,
-                                     hipStream_t s) {
-    size_t lds = leaf_mfma_lds(P.K);
-    hipFuncSetAttribute((const void*)k_leaf_mfma<MC, F8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_leaf_mfma<MC, F8>), dim3(P.nleaves), dim3(256), lds, s, P, xq, s2, marg);
-    return hipGetLastError();
+static void leaf_sweep_t(const LeafParams& P, const void* xq, float s2, const float* marg, uint32_t* cbuf,
+                         uint32_t* ccnt, uint32_t capc, hipStream_t s) {
+    hipLaunchKernelGGL((k_leaf_sweep<MC, F8>), dim3(P.nleaves), dim3(256), 0, s, P, xq, s2, marg, cbuf, ccnt, capc);
 }
 
-template <bool F8>
-static hipError_t launch_leaf_mfma_f(const LeafParams& P, const void* xq, float s2, const float* marg,
-                                     hipStream_t s) {
+template <int MC, int PCH>
+static void leaf_merge_t(const LeafParams& P, uint32_t* cbuf, uint32_t* ccnt, uint32_t capc, hipStream_t s) {
+    hipLaunchKernelGGL((k_leaf_merge<MC, PCH>), dim3(P.n), dim3(64), 0, s, P, cbuf, ccnt, capc);
+}
+
+template <int MC>
+static void leaf_pair_mc(const LeafParams& P, bool f8, const void* xq, float s2, const float* marg, uint32_t* cbuf,
+                         uint32_t* ccnt, uint32_t capc, hipStream_t s) {
+    if (f8) leaf_sweep_t<MC, true>(P, xq, s2, marg, cbuf, ccnt, capc, s);
+    else leaf_sweep_t<MC, false>(P, xq, s2, marg, cbuf, ccnt, capc, s);
+    if (P.K <= 128) leaf_merge_t<MC, 2>(P, cbuf, ccnt, capc, s);
+    else leaf_merge_t<MC, 4>(P, cbuf, ccnt, capc, s);
+}
+
+hipError_t launch_leaf_mfma(const LeafParams& P, bool fp8, const void* xq, float scale, const float* marg,
+                            uint32_t* cbuf, uint32_t* ccnt, uint32_t capc, hipStream_t s) {
+    if (P.nleaves == 0) return hipSuccess;
+    if (!leaf_mfma_supported(P.d, P.max_leaf, P.K) || capc < P.max_leaf) return hipErrorInvalidValue;
+    const float s2 = fp8 ? scale * scale : 1.0f;
     switch (P.d >> 4) {
-        case 8: return launch_leaf_mfma_t<8, F8>(P, xq, s2, marg, s);
-        case 7: return launch_leaf_mfma_t<7, F8>(P, xq, s2, marg, s);
-        default: return launch_leaf_mfma_t<6, F8>(P, xq, s2, marg, s);
+        case 8: leaf_pair_mc<8>(P, fp8, xq, s2, marg, cbuf, ccnt, capc, s); break;
+        case 7: leaf_pair_mc<7>(P, fp8, xq, s2, marg, cbuf, ccnt, capc, s); break;
+        default: leaf_pair_mc<6>(P, fp8, xq, s2, marg, cbuf, ccnt, capc, s); break;
     }
-}
-
-hipError_t launch_leaf_mfma(const LeafParams& P, const uint16_t* xb, const float* marg, hipStream_t s) {
-    if (P.nleaves == 0) return hipSuccess;
-    if (!leaf_mfma_supported(P.d, P.max_leaf)) return hipErrorInvalidValue;
-    return launch_leaf_mfma_f<false>(P, xb, 1.0f, marg, s);
-}
-
-hipError_t launch_leaf_mfma_fp8(const LeafParams& P, const uint8_t* x8, float scale, const float* marg,
-                                hipStream_t s) {
-    if (P.nleaves == 0) return hipSuccess;
-    if (!leaf_mfma_supported(P.d, P.max_leaf)) return hipErrorInvalidValue;
-    return launch_leaf_mfma_f<true>(P, x8, scale * scale, marg, s);
+    return hipGetLastError();
 }
 
 // x8 = e4m3(x / scale) (OCP FP8, round to nearest even); err[i] = |x_i - scale * x8_i|

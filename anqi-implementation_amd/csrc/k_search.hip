@@ -171,31 +171,11 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     return true;
 }
 
-// Register-resident beam (persistent form): lane l holds beam positions
-// 64 j + l, j < NCH, as (dist, id) keys (~0 past the end) plus a bit per
-// chunk for "expanded".  Merging kc candidates needs no sort and no LDS
-// search: for each candidate (scalar loop) one ballot per chunk counts the
-// beam keys below it and every beam lane counts the candidates below its key;
-// the rank among candidates is one more ballot.  Keys then move to their new
-// positions through one LDS write and one read per chunk.
-#define RB_FLAG 0x80000000ull   // expanded bit, kept in bit 31 of the id while in LDS
-
-template <int NCH>
-struct RBeam {
-    uint64_t k[NCH];
-    uint32_t x;   // bit j: position 64 j + lane expanded
-};
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
-    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
-    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
-}
-
+// Register-resident beam (persistent form): an RKeys list (common.hpp).
 // dg, nbr: node c's degree and adjacency row (lane-held), fetched by the caller.
 template <int NCH, int MC>
 __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t dg,
-                                          uint32_t nbr, uint32_t& rcnt, uint32_t& nd, RBeam<NCH>& B,
+                                          uint32_t nbr, uint32_t& rcnt, uint32_t& nd, RKeys<NCH>& B,
                                           uint32_t& bsize, uint32_t L, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     uint32_t nb = U32_NONE;
@@ -222,53 +202,7 @@ __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, c
     if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
     if (hit) fifo_put(P, S, rcnt + lanes_below(hm), myid);
     rcnt += (uint32_t)nh;
-    uint64_t key = (lane < k) ? pkey(dv, myid) : ~0ull;
-    if (bsize == L) {   // drop candidates that cannot enter the full beam
-        uint64_t last = 0;
-#pragma unroll
-        for (int j = 0; j < NCH; j++)
-            if ((uint32_t)j == ((L - 1) >> 6)) last = readlane64(B.k[j], (int)((L - 1) & 63));
-        if (!(key < last)) key = ~0ull;
-    }
-    const uint64_t km = ballot(key != ~0ull);
-    if (km == 0) return true;
-    uint32_t sc[NCH];
-#pragma unroll
-    for (int j = 0; j < NCH; j++) sc[j] = 0;
-    uint32_t mypos = U32_NONE;
-    for (uint64_t mm = km; mm; mm &= mm - 1) {
-        const int src = __ffsll((long long)mm) - 1;
-        const uint64_t cv = readlane64(key, src);
-        uint32_t pos = (uint32_t)popc64(ballot(key < cv));
-#pragma unroll
-        for (int j = 0; j < NCH; j++) {
-            pos += (uint32_t)popc64(ballot(B.k[j] < cv));
-            sc[j] += (cv < B.k[j]) ? 1u : 0u;
-        }
-        if (lane == src) mypos = pos;
-    }
-    uint64_t* kb = (uint64_t*)S.bd0;
-#pragma unroll
-    for (int j = 0; j < NCH; j++) {
-        const uint32_t np = 64u * j + (uint32_t)lane + sc[j];
-        if (B.k[j] != ~0ull && np < L) kb[np] = B.k[j] | (((B.x >> j) & 1u) ? RB_FLAG : 0ull);
-    }
-    if (mypos < L) kb[mypos] = key;
-    bsize = min(L, bsize + (uint32_t)popc64(km));
-    __syncthreads();
-    B.x = 0;
-#pragma unroll
-    for (int j = 0; j < NCH; j++) {
-        const uint32_t pos = 64u * j + (uint32_t)lane;
-        uint64_t v = ~0ull;
-        if (pos < bsize) {
-            v = kb[pos];
-            if (v & RB_FLAG) B.x |= 1u << j;
-            v &= ~RB_FLAG;
-        }
-        B.k[j] = v;
-    }
-    __syncthreads();
+    rk_merge<NCH>(B, (lane < k) ? pkey(dv, myid) : ~0ull, bsize, L, (uint64_t*)S.bd0);
     return true;
 }
 
@@ -371,31 +305,10 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     __syncthreads();
     // (1) beam descent
     if constexpr (NCH > 0) {
-        RBeam<NCH> B;
-#pragma unroll
-        for (int j = 0; j < NCH; j++) B.k[j] = (j == 0 && lane == 0) ? pkey(dep, ep) : ~0ull;
-        B.x = 0;
-        // first unexpanded beam entry (U32_NONE if none); marks it expanded
-        auto take_next = [&](bool mark) -> uint32_t {
-            int cj = -1, cl = 0;
-#pragma unroll
-            for (int j = 0; j < NCH; j++) {
-                if (cj < 0) {
-                    const uint64_t m = ballot(B.k[j] != ~0ull && !((B.x >> j) & 1u));
-                    if (m) { cj = j; cl = __ffsll((long long)m) - 1; }
-                }
-            }
-            if (cj < 0) return U32_NONE;
-            uint32_t c = 0;
-#pragma unroll
-            for (int j = 0; j < NCH; j++) {
-                if (j == cj) {
-                    c = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)B.k[j], cl);
-                    if (mark && lane == cl) B.x |= 1u << j;
-                }
-            }
-            return c;
-        };
+        RKeys<NCH> B;
+        rk_clear(B);
+        if (lane == 0) B.k[0] = pkey(dep, ep);
+        auto take_next = [&](bool mark) -> uint32_t { return rk_take(B, mark); };
         // The adjacency of the entry that is next in line once c is marked
         // is fetched while c is expanded; it is used when no new neighbour
         // of c overtakes it.

@@ -120,17 +120,10 @@ struct GatherParams {
     uint32_t* gscratch;     // optional global work area (per block)
     size_t gscratch_words;  // words per block
     uint32_t batched;       // 1: one expansion's first visits processed together
-    uint32_t* vbits;        // launch_gather_gv: per-block visited bitmaps (grid x vwords), zeroed
-    uint32_t vwords;        // ceil(n / 32)
-    uint32_t* vlist;        // per-block first-visit lists (grid x vcap)
-    uint32_t vcap;          // >= pool_cap + cap + longest reverse list
 };
-size_t gather_lds_bytes(const GatherParams& P, bool graph_mode);
+// rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
+size_t gather_lds_bytes(const GatherParams& P, bool graph_mode, bool rk);
 hipError_t launch_gather(const GatherParams& P, bool graph_mode, bool global_ws, hipStream_t s);
-// AKNN-mode gather with the visited sets as global bitmaps (LDS holds only
-// the queue and the top-K), persistent grid.
-int gather_gv_occupancy(const GatherParams& P);
-hipError_t launch_gather_gv(const GatherParams& P, uint32_t grid, hipStream_t s);
 
 struct PruneParams {
     const float* x;
@@ -190,13 +183,13 @@ struct LeafParams {
 };
 size_t leaf_lds_bytes(const LeafParams& P);
 hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s);
-// bf16 MFMA leaf sweep (k_leaf_mfma.hip); same lists as launch_leaf_knn.
-bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf);
-size_t leaf_mfma_lds(uint32_t K);
-hipError_t launch_leaf_mfma(const LeafParams& P, const uint16_t* xb, const float* marg, hipStream_t s);
-// FP8 (OCP e4m3) trial of the same sweep; x8 = e4m3(x / scale), scale a power of two.
-hipError_t launch_leaf_mfma_fp8(const LeafParams& P, const uint8_t* x8, float scale, const float* marg,
-                                hipStream_t s);
+// MFMA-filtered leaf all-pairs (k_leaf_mfma.hip); same lists as launch_leaf_knn.
+// xq: bf16 copy of x, or (fp8) e4m3(x / scale) with scale a power of two;
+// marg: per-row filter bound; cbuf (n x capc, capc >= max_leaf) and ccnt (n,
+// zeroed) are the candidate lists between the two passes (left zeroed).
+bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf, uint32_t K);
+hipError_t launch_leaf_mfma(const LeafParams& P, bool fp8, const void* xq, float scale, const float* marg,
+                            uint32_t* cbuf, uint32_t* ccnt, uint32_t capc, hipStream_t s);
 hipError_t launch_to_fp8(const float* x, const float* nrm, uint32_t n, uint32_t d, float scale, uint8_t* x8,
                          float* nx, float* ex, hipStream_t s);
 hipError_t launch_random_fill(const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t K,

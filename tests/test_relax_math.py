@@ -129,3 +129,44 @@ def test_floor_relaxation_backward_equals_autograd_and_finite_differences():
         M_ = cb.copy(); M_[a, b] -= eps
         fd = (manual_grads_floor(cut, mu, G, P, beta)[0] - manual_grads_floor(cut, mu, G, M_, beta)[0]) / (2 * eps)
         assert abs(fd - gcb[a, b]) <= 1e-4 * max(1e-3, abs(gcb[a, b])), (a, b, fd, gcb[a, b])
+
+
+def test_relaxations_reduce_to_the_exact_decodes_as_the_width_vanishes():
+    """As the width goes to 0 the relaxed objective becomes the exact decode
+    it relaxes: mode 1 the floor decode (largest codeword <= e, the first
+    codeword below the range: src/rknn_query.cpp:638-720's floor policy),
+    mode 0 the nearest codeword.  Residuals are placed away from codeword
+    midpoints/edges so the limit is reached at a finite width."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "anqi-implementation_amd"))
+    from lrq_relax import relaxed_loss_reference
+    rng = np.random.default_rng(2)
+    n, nk, m, nc = 30, 20, 4, 16
+    cb = np.sort(rng.uniform(-10, 10, (nk, nc)), axis=1)
+    cb[:, 1:] = np.maximum(cb[:, 1:], cb[:, :-1] + 0.5)        # spacing >= 0.5
+    G = np.zeros((m, nk))          # no basis part: e = cut - mu exactly (the basis is pinned above)
+    # residuals e in a known place: 70% of the way from codeword j to j+1
+    # (floor picks j, nearest picks j+1; >= 0.15 from either)
+    j = rng.integers(0, nc - 1, (n, nk))
+    cbb = np.broadcast_to(cb, (n, nk, nc))
+    lo = np.take_along_axis(cbb, j[:, :, None], 2)[:, :, 0]
+    hi = np.take_along_axis(cbb, j[:, :, None] + 1, 2)[:, :, 0]
+    e = lo + 0.7 * (hi - lo)
+    e[:, :3] = cb[None, :3, 0] - 1.0                           # some below the first codeword
+    mu = rng.uniform(50, 60, nk)
+    cut = mu + e
+    T = lambda a: torch.tensor(a, dtype=torch.float64)
+    floor_q = np.empty_like(e)
+    near_q = np.empty_like(e)
+    for k in range(nk):
+        f = np.searchsorted(cb[k], e[:, k], side="right") - 1
+        floor_q[:, k] = cb[k, np.maximum(f, 0)]
+        near_q[:, k] = cb[k, np.abs(e[:, k][:, None] - cb[k][None]).argmin(1)]
+    exact_floor = ((e - floor_q) ** 2).mean()
+    exact_near = ((e - near_q) ** 2).mean()
+    l1 = float(relaxed_loss_reference(T(cut), T(mu), T(G), T(cb), 1e-4, mode=1))
+    l0 = float(relaxed_loss_reference(T(cut), T(mu), T(G), T(cb), 1e-6, mode=0))
+    assert abs(l1 - exact_floor) <= 1e-9 * max(1.0, exact_floor)
+    assert abs(l0 - exact_near) <= 1e-9 * max(1.0, exact_near)
+    # and they differ from each other here (the relaxations are not interchangeable)
+    assert abs(exact_floor - exact_near) > 1e-3
