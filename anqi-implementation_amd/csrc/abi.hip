@@ -393,7 +393,7 @@ int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n,
 }
 
 int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t cap,
-                   uint32_t* hidx, float* hval, int iters, int sample, int64_t* upd, int* passes) {
+                   uint32_t* hidx, float* hval, int iters, int sample, int64_t* upd, int* passes, int form) {
     if (sample > 120) sample = 120;
     if (sample < 2 || iters <= 0) { if (passes) *passes = 0; return ANQI_OK; }
     hipStream_t s = ctx->stream;
@@ -426,13 +426,36 @@ int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, 
         return tmp.alloc(tb);
     };
     HCHK(grow(maxp));
+    // chosen form: per-row bounds of the bf16 gram (k_exact_margin over the rows)
+    const bool mfma = form && nnd_propose_mfma_supported(d, (uint32_t)sample);
+    DBuf marg;
+    if (mfma) {
+        DBuf nx, ex;
+        HCHK(nx.alloc(4ull * n));
+        HCHK(ex.alloc(4ull * n));
+        HCHK(marg.alloc(4ull * n));
+        HCHK(launch_exact_norms2(x, nrm, n, d, nx.as<float>(), ex.as<float>(), s));
+        std::vector<float> hn(n), he(n);
+        HCHK(hipMemcpyAsync(hn.data(), nx.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipMemcpyAsync(he.data(), ex.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        float Y = 0.0f, E = 0.0f;
+        for (uint32_t i = 0; i < n; i++) { Y = std::max(Y, hn[i]); E = std::max(E, he[i]); }
+        HCHK(launch_exact_margin(nx.as<float>(), ex.as<float>(), n, d, Y, E, marg.as<float>(), s));
+        HCHK(hipStreamSynchronize(s));
+    }
     int it = 0;
     for (; it < iters; it++) {
         HCHK(launch_nnd_sample(hidx, n, cap, (uint32_t)sample, samp.as<uint32_t>(), scnt.as<uint32_t>(), s));
         HCHK(hipMemsetAsync(cnt.p, 0, 8, s));
         HCHK(hipMemsetAsync(updd.p, 0, 8, s));
-        HCHK(launch_nnd_propose(x, nrm, n, d, samp.as<uint32_t>(), scnt.as<uint32_t>(), (uint32_t)sample, hval, cap,
-                                k0.as<uint64_t>(), v0.as<uint32_t>(), cnt.as<unsigned long long>(), maxp, s));
+        if (mfma)
+            HCHK(launch_nnd_propose_mfma(x, nrm, marg.as<float>(), n, d, samp.as<uint32_t>(), scnt.as<uint32_t>(),
+                                         (uint32_t)sample, hval, cap, k0.as<uint64_t>(), v0.as<uint32_t>(),
+                                         cnt.as<unsigned long long>(), maxp, s));
+        else
+            HCHK(launch_nnd_propose(x, nrm, n, d, samp.as<uint32_t>(), scnt.as<uint32_t>(), (uint32_t)sample, hval,
+                                    cap, k0.as<uint64_t>(), v0.as<uint32_t>(), cnt.as<unsigned long long>(), maxp, s));
         unsigned long long m = 0;
         HCHK(hipMemcpyAsync(&m, cnt.p, 8, hipMemcpyDeviceToHost, s));
         HCHK(hipStreamSynchronize(s));
@@ -588,6 +611,7 @@ int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t 
     P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.hidx = hidx; P.hval = hval; P.cap = cap; P.roff = roff; P.rids = rids;
     P.explore = explore; P.pool_cap = pool_cap; P.M = M; P.alpha = alpha; P.out_ids = O; P.out_cnt = ocnt;
     P.batched = form ? 1u : 0u;
+    HCHK(launch_fill_u32(O, U32_NONE, (uint64_t)n * M, ctx->stream));   // slots past ocnt[i]
     return gather_run(ctx, P, true, maxrev, nullptr);
 }
 
@@ -1100,7 +1124,7 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     S.warm_s = t2 - t1;
     int passes = 0;
     rc = anqi_nndescent(ctx, lifted, nrm.as<float>(), n, dl, cap, hidx.as<uint32_t>(), hval.as<float>(),
-                        std::min(p->iters, 16), p->sample, S.nnd_upd, &passes);
+                        std::min(p->iters, 16), p->sample, S.nnd_upd, &passes, p->form);
     if (rc) return rc;
     S.nnd_passes = passes;
     HCHK(hipStreamSynchronize(s));

@@ -132,22 +132,22 @@ __device__ __forceinline__ float qrow_l2(const QRow& q, const float* __restrict_
 }
 
 // Batched row distances: for rows ids[s], s < k, group g (16 lanes) takes
-// s = g, g+4, g+8, g+12 of each 16-row pass and issues all four rows' loads
-// before reducing any, so the fetches overlap.  Same per-lane FMA order and
-// reduction tree as qrow_l2 / qrow_ip, hence the same bits.  MC = number of
-// full 16-float chunks of a row of X (dim = 16 MC + r, r < 16).
+// s = g, g+4, ..., g + 4 (RB - 1) of each 4 RB-row pass and issues all RB
+// rows' loads before reducing any, so the fetches overlap.  Same per-lane
+// FMA order and reduction tree as qrow_l2 / qrow_ip, hence the same bits.
+// MC = number of full 16-float chunks of a row of X (dim = 16 MC + r, r < 16).
 //   MODE 0: out[s] = ||q - x||^2
 //   MODE 1: out[s] = d2_combine(<q, x>, nq, nrm[id])   (nndindex::d2)
-template <int MC, int MODE>
+template <int MC, int MODE, int RB = 4>
 __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restrict__ X, uint32_t dim,
                                            const float* __restrict__ nrm, float nq,
                                            const uint32_t* ids, int k, int l, int g, float* out) {
     const int r = (int)dim - 16 * MC;
-    for (int s0 = 0; s0 < k; s0 += 16) {
-        float x[4][MC];
-        float xt[4], nb[4];
+    for (int s0 = 0; s0 < k; s0 += 4 * RB) {
+        float x[RB][MC];
+        float xt[RB], nb[RB];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < RB; j++) {
             const int s = s0 + g + 4 * j;
             if (s < k) {
                 const uint32_t id = ids[s];
@@ -164,7 +164,7 @@ __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restric
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < RB; j++) {
             float acc = 0.0f;
             float res;
             if (MODE == 0) {

@@ -164,6 +164,150 @@ __global__ void __launch_bounds__(64) k_nnd_propose(const float* __restrict__ x,
     }
 }
 
+// Chosen form of the LocalJoin proposals.  The sample's gram matrix (c <= 32
+// rows, d <= 144) is one 32 x 32 bf16 MFMA tile per row u (the A and B
+// fragments are the same rows); a pair is re-scored in FP32 (the plain
+// kernel's bits) unless its approximate distance exceeds both endpoints'
+// roots by more than the row bound marg (k_exact_margin over the lifted
+// rows).  Proposals are staged in LDS and written with one counter update per
+// row.  The proposal set equals the plain kernel's.
+typedef __bf16 np_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float np_f32x16 __attribute__((ext_vector_type(16)));
+#define NP_KS 9   // 16-wide k steps: d <= 144
+
+__device__ __forceinline__ uint16_t np_bf16(float v) {   // round to nearest even (as k_exact_norms2)
+    const uint32_t u = __float_as_uint(v);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__global__ void __launch_bounds__(64) k_nnd_propose_mfma(const float* __restrict__ x, const float* __restrict__ nrm,
+                                                         const float* __restrict__ marg, uint32_t n, uint32_t d,
+                                                         const uint32_t* __restrict__ samp,
+                                                         const uint32_t* __restrict__ scnt, uint32_t sample,
+                                                         const float* __restrict__ hval, uint32_t cap,
+                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                         unsigned long long* __restrict__ count, uint64_t max_props) {
+    extern __shared__ __align__(16) char lds[];
+    const uint32_t npc = sample * (sample - 1) / 2;        // pair capacity
+    uint64_t* pk = (uint64_t*)lds;                         // 2 npc proposals
+    uint32_t* pv = (uint32_t*)(pk + 2 * npc);
+    float* sv = (float*)(pv + 2 * npc);                    // sample x d
+    uint32_t* sid = (uint32_t*)(sv + (size_t)sample * d);  // 32
+    float* snrm = (float*)(sid + 32);
+    float* sroot = snrm + 32;
+    float* smg = sroot + 32;
+    uint32_t* npair = (uint32_t*)(smg + 32);               // [0] pairs, [1] proposals
+    uint32_t* pairs = npair + 2;                           // npc: a | b << 16
+    const uint32_t u = blockIdx.x;
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const uint32_t c = scnt[u];
+    if (c < 2) return;
+    if (lane < 32) {
+        uint32_t v = (uint32_t)lane < c ? samp[(size_t)u * sample + lane] : 0u;
+        sid[lane] = v;
+        snrm[lane] = (uint32_t)lane < c ? nrm[v] : 0.0f;
+        sroot[lane] = (uint32_t)lane < c ? hval[(size_t)v * cap] : 0.0f;
+        smg[lane] = (uint32_t)lane < c ? marg[v] : 0.0f;
+    }
+    if (lane < 2) npair[lane] = 0;
+    wave_lds_sync();
+    for (uint32_t t = 0; t < c; t++) {
+        const float* row = x + (size_t)sid[t] * d;
+        for (uint32_t k = lane; k < d; k += 64) sv[(size_t)t * d + k] = row[k];
+    }
+    wave_lds_sync();
+    // bf16 fragments of row l % 32 (A and B alike): k = 16 s + 8 (l / 32) .. +8
+    const uint32_t ar = (uint32_t)(lane & 31);
+    np_f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NP_KS; s++) {
+        const uint32_t k0 = 16 * s + 8 * (lane >> 5);
+        uint16_t h[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) h[e] = (ar < c && k0 + e < d) ? np_bf16(sv[(size_t)ar * d + k0 + e]) : (uint16_t)0;
+        uint4 z;
+        z.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+        z.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+        z.z = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
+        z.w = (uint32_t)h[6] | ((uint32_t)h[7] << 16);
+        const np_bf16x8 f = __builtin_bit_cast(np_bf16x8, z);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f, f, acc, 0, 0, 0);
+    }
+    // C element v of this lane: row a = 8 (v / 4) + 4 (l / 32) + v % 4, column b = l % 32
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const uint32_t a = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), b = ar;
+        if (a < b && b < c) {
+            const float approx = fmaf(-2.0f, acc[v], snrm[a] + snrm[b]);
+            if (approx <= fmaxf(sroot[a], sroot[b]) + smg[a]) {
+                const uint32_t slot = atomicAdd(&npair[0], 1u);
+                pairs[slot] = a | (b << 16);
+            }
+        }
+    }
+    wave_lds_sync();
+    const uint32_t np_ = npair[0];
+    for (uint32_t base = 0; base < np_; base += 4) {
+        const uint32_t p = base + (uint32_t)g;
+        uint32_t a = 0, b = 0;
+        float dd = 0.0f;
+        if (p < np_) {
+            a = pairs[p] & 0xffffu;
+            b = pairs[p] >> 16;
+            const float ip = g16_ip(sv + (size_t)a * d, sv + (size_t)b * d, (int)d, l);
+            dd = d2_combine(ip, snrm[a], snrm[b]);
+        }
+        const int ok_vw = p < np_ && l == 0 && !(dd > sroot[a]);   // push w into row v
+        const int ok_wv = p < np_ && l == 0 && !(dd > sroot[b]);   // push v into row w
+        const uint64_t m1 = ballot(ok_vw), m2 = ballot(ok_wv);
+        const uint32_t o = npair[1];
+        if (ok_vw) {
+            const uint32_t q = o + lanes_below(m1);
+            pk[q] = ((uint64_t)sid[a] << 32) | fkey(dd);
+            pv[q] = sid[b];
+        }
+        if (ok_wv) {
+            const uint32_t q = o + popc64(m1) + lanes_below(m2);
+            pk[q] = ((uint64_t)sid[b] << 32) | fkey(dd);
+            pv[q] = sid[a];
+        }
+        wave_lds_sync();
+        if (lane == 0) npair[1] = o + popc64(m1) + popc64(m2);
+        wave_lds_sync();
+    }
+    const uint32_t tot = npair[1];
+    if (tot == 0) return;
+    unsigned long long basep = 0;
+    if (lane == 0) basep = atomicAdd(count, (unsigned long long)tot);
+    basep = (unsigned long long)shfl64((uint64_t)basep, 0);
+    for (uint32_t q = lane; q < tot; q += 64) {
+        const uint64_t pos = basep + q;
+        if (pos < max_props) { keys[pos] = pk[q]; vals[pos] = pv[q]; }
+    }
+}
+
+size_t nnd_propose_mfma_lds(uint32_t d, uint32_t sample) {
+    const size_t npc = (size_t)sample * (sample - 1) / 2;
+    return 12 * 2 * npc + 4ull * sample * d + 4 * 32 * 4 + 4 * 2 + 4 * npc;
+}
+
+bool nnd_propose_mfma_supported(uint32_t d, uint32_t sample) { return d <= 16 * NP_KS && sample >= 2 && sample <= 32; }
+
+hipError_t launch_nnd_propose_mfma(const float* x, const float* nrm, const float* marg, uint32_t n, uint32_t d,
+                                   const uint32_t* samp, const uint32_t* scnt, uint32_t sample, const float* hval,
+                                   uint32_t cap, uint64_t* keys, uint32_t* vals, unsigned long long* count,
+                                   uint64_t max_props, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (!nnd_propose_mfma_supported(d, sample)) return hipErrorInvalidValue;
+    const size_t lds = nnd_propose_mfma_lds(d, sample);
+    hipFuncSetAttribute((const void*)k_nnd_propose_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_nnd_propose_mfma, dim3(n), dim3(64), lds, s, x, nrm, marg, n, d, samp, scnt, sample, hval,
+                       cap, keys, vals, count, max_props);
+    return hipGetLastError();
+}
+
 __global__ void k_row_bounds(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n,
                              uint32_t* __restrict__ row_off) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -515,7 +659,9 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         __syncthreads();
         if (!seeded) {
             if constexpr (MC > 0) {
-                rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
+                // register-list form: eight rows per group in flight
+                if constexpr (QCH > 0) rows_batch<MC, 1, 16>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
+                else rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
             } else {
                 for (uint32_t s = g; s < kall; s += 4) {
                     uint32_t c = S.tid[s];

@@ -84,6 +84,13 @@ hipError_t launch_nnd_propose(const float* x, const float* nrm, uint32_t n, uint
                               const uint32_t* samp, const uint32_t* scnt, uint32_t sample,
                               const float* hval, uint32_t cap, uint64_t* keys, uint32_t* vals,
                               unsigned long long* count, uint64_t max_props, hipStream_t s);
+// Chosen form (k_graph.hip): bf16 MFMA gram of the sample, FP32 re-score of
+// the pairs within the bound marg (k_exact_margin of the rows); same proposals.
+bool nnd_propose_mfma_supported(uint32_t d, uint32_t sample);
+hipError_t launch_nnd_propose_mfma(const float* x, const float* nrm, const float* marg, uint32_t n, uint32_t d,
+                                   const uint32_t* samp, const uint32_t* scnt, uint32_t sample, const float* hval,
+                                   uint32_t cap, uint64_t* keys, uint32_t* vals, unsigned long long* count,
+                                   uint64_t max_props, hipStream_t s);
 hipError_t launch_nnd_apply(const uint64_t* keys, const uint32_t* vals, uint64_t m,
                             const uint32_t* row_off, uint32_t n, uint32_t* hidx, float* hval,
                             uint32_t cap, unsigned long long* upd, hipStream_t s);

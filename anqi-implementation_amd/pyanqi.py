@@ -83,7 +83,7 @@ SIGS = {
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, I32, F32, ctypes.POINTER(F64), P, P]),
     "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
-    "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32)]),
+    "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32), I32]),
     "anqi_reverse_csr": (I32, [P, P, U32, U32, P, U32, P, P]),
     "anqi_gather_topk": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, P, P, P, I32]),
     "anqi_gather_prune": (I32, [P, P, P, U32, U32, P, P, U32, P, P, U32, U32, U32, F32, P, P, I32]),
@@ -253,13 +253,13 @@ class Context:
         self.call("anqi_warm_start", dp(x), dp(nrm), n, d, dp(ids), width, cap, dp(hidx), dp(hval))
         return hidx, hval
 
-    def nndescent(self, x, nrm, hidx, hval, iters=6, sample=20):
+    def nndescent(self, x, nrm, hidx, hval, iters=6, sample=20, form=0):
         n, d = x.shape
         cap = hidx.shape[1]
         upd = np.zeros(max(iters, 1), np.int64)
         passes = I32(0)
         self.call("anqi_nndescent", dp(x), dp(nrm), n, d, cap, dp(hidx), dp(hval), iters, sample, hp(upd),
-                  ctypes.byref(passes))
+                  ctypes.byref(passes), form)
         return passes.value, upd[:passes.value]
 
     def reverse_csr(self, ids, n, rowcnt=None):

@@ -144,10 +144,12 @@ int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n,
                     const uint32_t* ids, uint32_t width, uint32_t cap, uint32_t* hidx,
                     float* hval);
 /* LocalJoin, snapshot schedule; upd (host, iters entries) receives updates
- * per pass; returns the pass count through *passes. */
+ * per pass; returns the pass count through *passes.  form 1: the sample's
+ * pair distances as a bf16 MFMA tile with FP32 re-scoring near the roots
+ * (d <= 144, sample <= 32); same result as form 0. */
 int anqi_nndescent(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
                    uint32_t cap, uint32_t* hidx, float* hval, int iters, int sample,
-                   int64_t* upd, int* passes);
+                   int64_t* upd, int* passes, int form);
 /* Reverse lists of a rows x width id table (rowcnt NULL: sentinel-filtered). */
 int anqi_reverse_csr(anqi_ctx* ctx, const uint32_t* ids, uint32_t rows, uint32_t width,
                      const uint32_t* rowcnt, uint32_t n, uint32_t* roff, uint32_t* rids);
@@ -160,7 +162,7 @@ int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t 
                       const uint32_t* hidx, const float* hval, uint32_t cap,
                       const uint32_t* roff, const uint32_t* rids, uint32_t explore,
                       uint32_t pool_cap, uint32_t M, float alpha, uint32_t* O,
-                      uint32_t* ocnt, int form);
+                      uint32_t* ocnt, int form);   /* O slots past ocnt[i]: UINT32_MAX */
 int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
                 const uint32_t* O, const uint32_t* ocnt, uint32_t M, float alpha,
                 uint32_t* links, uint32_t* deg);

@@ -221,10 +221,15 @@ def _graph_steps(ctx, x, warm, tag):
     assert np.array_equal(host(hi).view(np.uint32), hi_o) and np.array_equal(bits(host(hv)), bits(hv_o))
     say(tag, "localjoin")
     upd_o = np.zeros(6, np.int64)
+    hi1, hv1 = hi.clone(), hv.clone()
     it_o = O.or_nndescent_snapshot(ptr(x), ptr(nrm), n, d, cap, ptr(hi_o), ptr(hv_o), 6, 20, ptr(upd_o))
     it, upd = ctx.nndescent(xd, nd_, hi, hv, 6, 20)
     assert it == it_o and upd.tolist() == upd_o[:it_o].tolist()
     assert np.array_equal(host(hi).view(np.uint32), hi_o) and np.array_equal(bits(host(hv)), bits(hv_o))
+    # the MFMA form (bf16 gram of the sample, FP32 re-score near the roots)
+    it1, upd1 = ctx.nndescent(xd, nd_, hi1, hv1, 6, 20, form=1)
+    assert it1 == it_o and upd1.tolist() == upd_o[:it_o].tolist()
+    assert np.array_equal(host(hi1).view(np.uint32), hi_o) and np.array_equal(bits(host(hv1)), bits(hv_o))
     say(tag, "reverse lists")
     roff, rids = ctx.reverse_csr(hi, n)
     roff_o = np.zeros(n + 1, np.uint32)

@@ -65,18 +65,26 @@ def test_graph_stages_match_oracle(env):
     hv_o = np.zeros((n, cap), np.float32)
     O.or_warm_start(ptr(x), ptr(nrm), n, d, ptr(ids), 100, cap, ptr(hi_o), ptr(hv_o))
     assert np.array_equal(hi.cpu().numpy().view(np.uint32), hi_o), "warm start"
+    hi1, hv1 = hi.clone(), hv.clone()
     passes, upd = ctx.nndescent(xd, nd, hi, hv, 6, 20)
     upd_o = np.zeros(6, np.int64)
     p_o = O.or_nndescent_snapshot(ptr(x), ptr(nrm), n, d, cap, ptr(hi_o), ptr(hv_o), 6, 20, ptr(upd_o))
     say("localjoin", passes, upd.tolist(), "oracle", p_o, upd_o[:p_o].tolist())
     assert passes == p_o and upd.tolist() == upd_o[:p_o].tolist()
     assert np.array_equal(hi.cpu().numpy().view(np.uint32), hi_o), "localjoin"
+    p1, upd1 = ctx.nndescent(xd, nd, hi1, hv1, 6, 20, form=1)      # MFMA form
+    assert p1 == p_o and upd1.tolist() == upd_o[:p_o].tolist()
+    assert np.array_equal(hi1.cpu().numpy(), hi.cpu().numpy()) and np.array_equal(
+        hv1.cpu().numpy().view(np.uint32), hv.cpu().numpy().view(np.uint32)), "localjoin (MFMA form)"
     roff, rids = ctx.reverse_csr(hi, n)
     ro = roff.cpu().numpy().view(np.uint32)
     rev_len = np.diff(ro)
     say("reverse list length max", int(rev_len.max()), "mean", float(rev_len.mean()))
     Og, oc = ctx.gather_prune(xd, nd, hi, hv, roff, rids, 100, 800, 48, 1.2)
+    Og1, oc1 = ctx.gather_prune(xd, nd, hi, hv, roff, rids, 100, 800, 48, 1.2, form=1)   # batched form
     Og, oc = Og.cpu().numpy().view(np.uint32), oc.cpu().numpy().view(np.uint32)
+    assert np.array_equal(oc1.cpu().numpy().view(np.uint32), oc) and np.array_equal(
+        Og1.cpu().numpy().view(np.uint32), Og), "gather + prune (batched form)"
     roff_o = np.zeros(n + 1, np.uint32)
     rids_o = np.zeros(n * cap, np.uint32)
     O.or_reverse_csr(ptr(hi_o), n, cap, ptr(roff_o), ptr(rids_o))
