@@ -23,11 +23,31 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ int grp_lane() { return __lane_id() & 15; }
 __device__ __forceinline__ int grp_id() { return __lane_id() >> 4; }
 
+// The butterfly partners are taken with DPP row rotations (row_ror 8, 4, 2,
+// 1 within each 16-lane row) instead of xor shuffles through the LDS
+// crossbar.  After the step with distance 8, lanes i and i^8 hold equal
+// values, after distance 4 also i^4, and so on; so lane (i + s) mod 16 holds
+// the same value as lane i ^ s at every step, and each add sees the operands
+// of the xor tree (IEEE addition is commutative): the bits are those of
+// _mm512_reduce_add_ps.  Whole 16-lane groups are active or inactive together
+// at every call site.
+__device__ __forceinline__ float dpp_ror16(float v, int s) {
+    const int x = __float_as_int(v);
+    int r;
+    switch (s) {
+        case 8: r = __builtin_amdgcn_mov_dpp(x, 0x128, 0xf, 0xf, false); break;
+        case 4: r = __builtin_amdgcn_mov_dpp(x, 0x124, 0xf, 0xf, false); break;
+        case 2: r = __builtin_amdgcn_mov_dpp(x, 0x122, 0xf, 0xf, false); break;
+        default: r = __builtin_amdgcn_mov_dpp(x, 0x121, 0xf, 0xf, false); break;
+    }
+    return __int_as_float(r);
+}
+
 __device__ __forceinline__ float tree16(float v) {
-    v = v + __shfl_xor(v, 8, 16);
-    v = v + __shfl_xor(v, 4, 16);
-    v = v + __shfl_xor(v, 2, 16);
-    v = v + __shfl_xor(v, 1, 16);
+    v = v + dpp_ror16(v, 8);
+    v = v + dpp_ror16(v, 4);
+    v = v + dpp_ror16(v, 2);
+    v = v + dpp_ror16(v, 1);
     return v;
 }
 

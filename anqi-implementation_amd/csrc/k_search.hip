@@ -25,8 +25,6 @@
 namespace {
 
 struct SearchLds {
-    float* qlift;      // dl
-    float* qbase;      // d
     float* bd0;        // beam distances (double buffer 0 / 1)
     float* bd1;
     uint32_t* bp0;     // beam ids, bit31 = expanded
@@ -58,8 +56,6 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
     SearchLds s;
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base + o; o += (bytes + 15) & ~size_t(15); return p; };
-    s.qlift = (float*)take(sizeof(float) * (P.dl + 16));
-    s.qbase = (float*)take(sizeof(float) * (P.d + 16));
     if (P.rbeam) {
         // register beam: one (dist, id) key exchange area of Lmax entries at bd0
         s.bd0 = (float*)take(sizeof(uint64_t) * P.Lmax);
@@ -266,18 +262,35 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     // per-query beam width, clamped to 1..Lmax (the LDS beam areas' size)
     const uint32_t L = max(1u, min(P.Lq ? P.Lq[qi] : P.L, P.Lmax));
     const float* thr = P.thr + (P.kidx ? (size_t)P.kidx[qi] * P.n : 0);
-    // query transform (rknn_query.cpp:2719-2730), binary64 product rounded to float
+    // query transform (rknn_query.cpp:2719-2730), binary64 product rounded to
+    // float, then the lift coordinates (1, 0); each lane forms the elements
+    // its register row holds (element 16 t + l, and the tail)
     const float* q = P.queries + (size_t)qi * P.d;
-    for (uint32_t j = lane; j < P.d; j += 64) {
-        float v = q[j];
-        S.qbase[j] = v;
-        S.qlift[j] = sd_to_float(sd_mul(sd_from_float(v), P.scale_bits));
-    }
-    if (lane == 0) { S.qlift[P.d] = 1.0f; S.qlift[P.d + 1] = 0.0f; }
     for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
-    __syncthreads();
     QRow QL;
-    qrow_load(QL, S.qlift, P.dl, l);
+    {
+        // lane holds elements lane + 64 k (k < 4, dl <= 256), moved into the
+        // register-row layout (element 16 t + l) by shuffles
+        float qv0 = 0.0f, qv1 = 0.0f, qv2 = 0.0f, qv3 = 0.0f;
+#pragma unroll 1
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t e = (uint32_t)lane + 64u * k;
+            float v = 0.0f;
+            if (e < P.d) v = sd_to_float(sd_mul(sd_from_float(q[e]), P.scale_bits));
+            else if (e == P.d) v = 1.0f;
+            if (k == 0) qv0 = v; else if (k == 1) qv1 = v; else if (k == 2) qv2 = v; else qv3 = v;
+        }
+        auto pick = [&](uint32_t blk) { return blk == 0 ? qv0 : blk == 1 ? qv1 : blk == 2 ? qv2 : qv3; };
+        const uint32_t m = P.dl >> 4, r = P.dl & 15;
+#pragma unroll
+        for (int t = 0; t < QMAX_CHUNKS; t++) {
+            const float v = __shfl(pick((uint32_t)t >> 2), (16 * t + l) & 63);
+            QL.v[t] = ((uint32_t)t < m) ? v : 0.0f;
+        }
+        const float tv = __shfl(pick(m >> 2), (int)((16u * m + (uint32_t)l) & 63u));
+        QL.tail = ((uint32_t)l < r) ? tv : 0.0f;
+    }
+    __syncthreads();
     // entry point: search-tree descent (rptrees.h:108-123), ties go left
     uint32_t ep;
     if (P.entries) {
@@ -361,8 +374,8 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     }
     // (3) postfilter in base space
     QRow QB;
-    qrow_load(QB, S.qbase, P.d, l);
-    float qn = qrow_ip(QB, S.qbase, P.d, l);
+    qrow_load(QB, q, P.d, l);
+    float qn = qrow_ip(QB, q, P.d, l);
     uint32_t npred = 0;
     uint32_t* out = P.cand_ids + (size_t)qi * P.out_cap;
     uint8_t* acc = P.accept + (size_t)qi * P.out_cap;
@@ -482,7 +495,7 @@ static int search_nch(uint32_t Lmax) {
 size_t search_lds_bytes(const SearchParams& P) {
     auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
     const size_t beam = P.rbeam ? r(8 * (size_t)P.Lmax) : 4 * r(4 * (size_t)P.Lmax) + r(8 * 64);
-    return r(4 * (P.dl + 16)) + r(4 * (P.d + 16)) + beam + r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) +
+    return beam + r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) +
            r(4 * SR_FLOOD * 64) + r(4 * SR_FLOOD * 64);
 }
 

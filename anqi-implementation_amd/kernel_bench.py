@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Per-kernel timing of the build and verifier kernels on the 1,000,000 x 128
 mixture, for the profiles: the chosen AKNN build (bf16 MFMA leaf sweep,
-batched gather), the bf16 MFMA exact top-K on a sub-base, and the LRQ
-threshold decode (the verifier's unpack-and-compare stage).
+batched gather), the bf16 MFMA exact top-K on a sub-base, the LRQ
+threshold decode (the verifier's unpack-and-compare stage) and the
+persistent range search at a fixed L (--what search).
 
 Prints one JSON line: per stage, its time and its algorithmic work (flop for
 the MFMA sweeps, bytes for the decode) and the achieved rate.  Meant to run
@@ -29,6 +30,9 @@ def main():
     ap.add_argument("--what", default="aknn,exact,lrq")
     ap.add_argument("--exact-n", type=int, default=262144, help="rows of the exact top-K sub-base")
     ap.add_argument("--lrq-reps", type=int, default=20)
+    ap.add_argument("--nq", type=int, default=10000)
+    ap.add_argument("--L", type=int, default=96)
+    ap.add_argument("--search-reps", type=int, default=5)
     a = ap.parse_args()
     import numpy as np
     import torch
@@ -44,8 +48,9 @@ def main():
     n, d = base.shape
     out = {"n": n, "d": d}
     ev = lambda: torch.cuda.Event(enable_timing=True)
-    ids, keys, st = ctx.aknn_build(x, K=100, leaf=500, trees=64, ef=600, precision=1)
-    torch.cuda.synchronize()
+    if what & {"aknn", "lrq"}:
+        ids, keys, st = ctx.aknn_build(x, K=100, leaf=500, trees=64, ef=600, precision=1)
+        torch.cuda.synchronize()
     if "aknn" in what:
         flop = 2.0 * d * 2 * st.leaf_pairs          # ordered pairs, one MFMA dot each
         out["aknn"] = {"forest_s": st.forest_s, "leaf_s": st.leaf_s, "gather_s": st.gather_s,
@@ -83,6 +88,28 @@ def main():
         byts = n * (4 * rankm + 1 + 4)                 # coefficients, one code byte, threshold out
         out["lrq_materialize"] = {"ms": ms, "bytes": byts, "GBps": byts / (ms * 1e-3) / 1e9}
         log("lrq", json.dumps(out["lrq_materialize"]))
+    if "search" in what:
+        import pipeline
+        ix = pipeline.build(ctx, base, precision=1, log=lambda *z: None)
+        thr = pipeline.thresholds(ctx, ix, 10)
+        q = ctx.to_dev(workload.million_query(a.nq))
+        tune, tms = ctx.search_autotune(ix.handle, thr, q, None, None, a.L)
+        ctx.set_search_tuning(tune)
+        bufs = ctx.search_buffers(a.nq, 4096)
+        ctx.rknn_search(ix.handle, q, a.L, thr, mode=1, bufs=bufs)
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record()
+        for _ in range(a.search_reps):
+            ctx.rknn_search(ix.handle, q, a.L, thr, mode=1, bufs=bufs)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / a.search_reps
+        out["search"] = {"L": a.L, "nq": a.nq, "ms_per_batch": ms, "qps": a.nq / (ms * 1e-3),
+                         "tuning": [tune.waves_per_cu, tune.visited_slots, tune.result_cap],
+                         "dist_per_query": float(bufs["nd"].double().mean() + bufs["cnt"].double().mean())}
+        log("search", json.dumps(out["search"]))
+        ctx.tree_free(ix.tree)
     print(json.dumps(out), flush=True)
     ctx.close()
 
