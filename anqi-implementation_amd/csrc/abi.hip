@@ -27,6 +27,8 @@ struct anqi_ctx {
     int cus = 0;
     uint32_t* counter = nullptr;   // device work counter of the persistent search
     anqi_search_tuning tune{0, 0, 0};   // persistent search work areas / grid (0 = defaults)
+    uint32_t* ent = nullptr;       // search-tree entries of the persistent search's batch
+    size_t ent_cap = 0;
 };
 
 static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
@@ -141,6 +143,7 @@ void anqi_ctx_destroy(anqi_ctx* ctx) {
     if (ctx->own) { (void)hipStreamSynchronize(ctx->own); (void)hipStreamDestroy(ctx->own); }
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->counter) (void)hipFree(ctx->counter);
+    if (ctx->ent) (void)hipFree(ctx->ent);
     delete ctx;
 }
 
@@ -1210,6 +1213,16 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
         if (!ctx->counter) HCHK(hipMalloc((void**)&ctx->counter, 256));
         HCHK(launch_fill_u32(ctx->counter, 0u, 1, s));
         P.work_counter = ctx->counter;
+        if (!P.entries) {
+            // the batch's tree descents first, four queries per wavefront
+            if (ctx->ent_cap < nq) {
+                if (ctx->ent) { HCHK(hipStreamSynchronize(s)); HCHK(hipFree(ctx->ent)); ctx->ent = nullptr; }
+                HCHK(hipMalloc((void**)&ctx->ent, 4ull * nq));
+                ctx->ent_cap = nq;
+            }
+            HCHK(launch_tree_entry_batch(P, nq, ctx->ent, s));
+            P.entries = ctx->ent;
+        }
         int occ = std::max(1, search_persistent_occupancy(P));
         uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
         uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
@@ -1268,7 +1281,7 @@ struct anqi_batcher {
     SearchParams P{};
     anqi_index ix{};
     uint32_t max_batch = 0, out_cap = 0, Lmax = 0, grid = 0;
-    DBuf q, Lq, kidx, nq, cand, acc, cnt, nd, pred, status, off, ids, counter;
+    DBuf q, Lq, kidx, nq, cand, acc, cnt, nd, pred, status, off, ids, counter, ent;
     float* hq = nullptr;
     uint32_t *hL = nullptr, *hk = nullptr, *hnq = nullptr, *hstatus = nullptr;
     hipGraph_t graph = nullptr;
@@ -1288,6 +1301,9 @@ struct anqi_batcher {
 static hipError_t batch_enqueue(anqi_batcher* b) {
     hipError_t e;
     if ((e = launch_fill_u32(b->counter.as<uint32_t>(), 0u, 1, b->s)) != hipSuccess) return e;
+    SearchParams T = b->P;
+    T.entries = nullptr;
+    if ((e = launch_tree_entry_batch(T, b->max_batch, b->ent.as<uint32_t>(), b->s)) != hipSuccess) return e;
     if ((e = launch_range_search(b->P, true, b->grid, b->s)) != hipSuccess) return e;
     if ((e = launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(),
                                b->s)) != hipSuccess)
@@ -1325,6 +1341,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     HCHK(b->off.alloc(4 * (mb + 1)));
     HCHK(b->ids.alloc(4 * mb * out_cap));
     HCHK(b->counter.alloc(256));
+    HCHK(b->ent.alloc(4 * mb));
     HCHK(hipHostMalloc((void**)&b->hq, 4 * mb * ix->d, hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&b->hL, 4 * mb, hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&b->hk, 4 * mb, hipHostMallocDefault));
@@ -1345,6 +1362,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     P.cand_ids = b->cand.as<uint32_t>(); P.accept = b->acc.as<uint8_t>(); P.cand_cnt = b->cnt.as<uint32_t>();
     P.ndist = b->nd.as<uint32_t>(); P.pred_cnt = b->pred.as<uint32_t>(); P.nexp = nullptr;
     P.status = b->status.as<uint32_t>(); P.work_counter = b->counter.as<uint32_t>();
+    P.entries = b->ent.as<uint32_t>();
     const int occ = std::max(1, search_persistent_occupancy(P));
     const uint32_t wpc = t.waves_per_cu ? std::min<uint32_t>(t.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
     b->grid = wpc * (uint32_t)std::max(1, ctx->cus);
@@ -1444,6 +1462,10 @@ int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_t
     P.cand_ids = cand.as<uint32_t>(); P.accept = acc.as<uint8_t>(); P.cand_cnt = cnt.as<uint32_t>();
     P.ndist = nd.as<uint32_t>(); P.pred_cnt = pred.as<uint32_t>(); P.status = status.as<uint32_t>();
     P.work_counter = counter.as<uint32_t>();
+    DBuf ent;   // the sample's tree entries, computed once as the persistent form does
+    HCHK(ent.alloc(4ull * nq));
+    HCHK(launch_tree_entry_batch(P, nq, ent.as<uint32_t>(), s));
+    P.entries = ent.as<uint32_t>();
     hipEvent_t e0, e1;
     HCHK(hipEventCreate(&e0));
     HCHK(hipEventCreate(&e1));

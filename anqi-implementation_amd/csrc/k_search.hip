@@ -307,7 +307,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     uint32_t nd = 1, rcnt = 0, bsize = 1, nx = 0;
     int buf = 0, ovf = 0;
     if (lane == 0) {
-        S.hash[hslot(ep, P.hmask)] = ep + 1u;
+        hset_first(S.hash, P.hmask, ep);
         if constexpr (NCH == 0) {
             S.bd0[0] = dep;
             S.bp0[0] = ep;
@@ -477,6 +477,50 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
         search_one<MC, NCH>(P, S, qi);
         __syncthreads();
     }
+}
+
+// Search-tree entries for a batch (persistent form): a 16-lane group per
+// query, four queries per wavefront, so the descent's dependent loads of
+// many queries overlap; per level the node's children, offset and
+// hyperplane are fetched together.  Same descent as search_one's
+// (rptrees.h:108-123, ties go left).  Queries beyond nq (or *nq_dev) exit.
+__global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_t* __restrict__ out) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const uint32_t nq = P.nq_dev ? *P.nq_dev : P.nq;
+    const uint32_t qi = blockIdx.x * 4u + (uint32_t)g;
+    if (qi >= nq) return;                       // whole 16-lane groups
+    const float* q = P.queries + (size_t)qi * P.d;
+    const uint32_t m = P.dl >> 4, r = P.dl & 15;
+    auto lifted_q = [&](uint32_t e) -> float {
+        if (e < P.d) return sd_to_float(sd_mul(sd_from_float(q[e]), P.scale_bits));
+        return e == P.d ? 1.0f : 0.0f;
+    };
+    QRow QL;
+#pragma unroll
+    for (int t = 0; t < QMAX_CHUNKS; t++) QL.v[t] = 0.0f;
+    for (uint32_t t = 0; t < m; t++) {
+        const float v = lifted_q(16u * t + (uint32_t)l);
+#pragma unroll
+        for (int u = 0; u < QMAX_CHUNKS; u++)
+            if ((uint32_t)u == t) QL.v[u] = v;
+    }
+    QL.tail = ((uint32_t)l < r) ? lifted_q(16u * m + (uint32_t)l) : 0.0f;
+    uint32_t node = P.t_root;
+    for (;;) {
+        const uint32_t left = P.t_left[node], right = P.t_right[node];
+        const float off = P.t_off[node];
+        const float ip = qrow_ip(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l);
+        if (left == U32_NONE) break;
+        const float mg = ip - off;
+        node = (mg < -1e-6f) ? left : (mg > 1e-6f ? right : left);
+    }
+    if (l == 0) out[qi] = P.t_entry[node];
+}
+
+hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint32_t* out, hipStream_t s) {
+    if (max_nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tree_entry_batch, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out);
+    return hipGetLastError();
 }
 
 // Chunk count shared by the lifted rows (d + 2) and the base rows (d).
