@@ -1203,7 +1203,7 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     // overflow it are rerun below with 16K-entry areas.
     P.hmask = std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * std::max(L, Lmax) + 1536))) - 1;
     P.rcap = 1024; P.out_cap = out_cap;
-    if (mode == 1 && ctx->tune.visited_slots >= 2 * P.Lmax) P.hmask = next_pow2(ctx->tune.visited_slots) - 1;
+    if (mode == 1 && ctx->tune.visited_slots >= 2 * P.Lmax) P.hmask = ((ctx->tune.visited_slots + 3u) & ~3u) - 1;
     if (mode == 1 && ctx->tune.result_cap) P.rcap = ctx->tune.result_cap;
     P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
     P.nexp = nexp;
@@ -1354,7 +1354,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     P.thr = thr_tables; P.kidx = b->kidx.as<uint32_t>();
     anqi_search_tuning t{0, 0, 0};
     if (tune) t = *tune;
-    const uint32_t slots = t.visited_slots ? next_pow2(t.visited_slots)
+    const uint32_t slots = t.visited_slots ? ((t.visited_slots + 3u) & ~3u)
                                            : std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * Lmax + 1536)));
     P.hmask = slots - 1;
     P.rcap = t.result_cap ? t.result_cap : 1024;
@@ -1471,7 +1471,7 @@ int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_t
     HCHK(hipEventCreate(&e1));
     double best = 1e300;
     anqi_search_tuning bt{0, 0, 0};
-    const uint32_t slot_opts[3] = {2048, 4096, 8192};
+    const uint32_t slot_opts[5] = {1536, 2048, 3072, 4096, 8192};
     const uint32_t rcap_opts[2] = {128, 512};   // later FIFO entries spill to the output rows
     std::vector<uint32_t> st(nq);
     for (uint32_t so : slot_opts) {

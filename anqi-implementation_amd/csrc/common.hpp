@@ -403,24 +403,28 @@ __host__ __device__ __forceinline__ uint64_t rng_at(uint64_t seed, uint64_t a, u
 }
 
 // ---- LDS open-addressing id set (key = id + 1, 0 = empty) --------------------
-// Buckets of four slots (16 bytes; mask + 1 slots, a power of two >= 4),
-// probed bucket by bucket: one 16-byte read answers "present" or finds a free
-// slot, so a lookup costs one LDS round trip unless its bucket is full.
-// Slots are never freed, so a key lives in the first bucket of its probe
-// sequence that had room when it was inserted; every bucket before it is still
-// full.  Lanes inserting different keys into one bucket race through the CAS;
-// a lane that loses moves to the next free slot.  (One call never holds a key
-// twice.)
-__device__ __forceinline__ uint32_t hslot(uint32_t id, uint32_t mask) {
+// mask + 1 slots (a multiple of 4) in buckets of four (16 bytes), probed
+// bucket by bucket: one 16-byte read answers "present" or finds a free slot,
+// so a lookup costs one LDS round trip unless its bucket is full.  Slots are
+// never freed, so a key lives in the first bucket of its probe sequence that
+// had room when it was inserted; every bucket before it is still full.  Lanes
+// inserting different keys into one bucket race through the CAS; a lane that
+// loses moves to the next free slot.  (One call never holds a key twice.)
+__device__ __forceinline__ uint32_t hmix(uint32_t id) {
     uint32_t h = id * 0x9E3779B1u;
-    return (h ^ (h >> 16)) & mask;
+    return h ^ (h >> 16);
+}
+__device__ __forceinline__ uint32_t hslot(uint32_t id, uint32_t mask) { return hmix(id) & mask; }
+// home bucket of id among nb buckets (any nb)
+__device__ __forceinline__ uint32_t hbucket(uint32_t id, uint32_t nb) {
+    return (uint32_t)(((uint64_t)hmix(id) * nb) >> 32);
 }
 // Returns 1 if id was inserted (first visit), 0 if present, -1 if table full.
 __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_t id) {
     const uint32_t key = id + 1u;
-    const uint32_t bmask = ((mask + 1u) >> 2) - 1u;
-    uint32_t b = hslot(id, bmask);
-    for (uint32_t probe = 0; probe <= bmask; probe++) {
+    const uint32_t nb = (mask + 1u) >> 2;
+    uint32_t b = hbucket(id, nb);
+    for (uint32_t probe = 0; probe < nb; probe++) {
         uint32_t* bk = tab + 4u * b;
         const uint4 v = *(const uint4*)bk;
         if (v.x == key || v.y == key || v.z == key || v.w == key) return 0;
@@ -430,13 +434,13 @@ __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_
             if (old == 0u) return 1;
             if (old == key) return 0;
         }
-        b = (b + 1u) & bmask;
+        b = (b + 1u == nb) ? 0u : b + 1u;
     }
     return -1;
 }
 // First key of an empty table (the search's entry point).
 __device__ __forceinline__ void hset_first(uint32_t* tab, uint32_t mask, uint32_t id) {
-    tab[4u * hslot(id, ((mask + 1u) >> 2) - 1u)] = id + 1u;
+    tab[4u * hbucket(id, (mask + 1u) >> 2)] = id + 1u;
 }
 
 #define HIP_CHECK_RET(expr)                                                   \

@@ -49,7 +49,7 @@ void* anqi_ctx_stream(anqi_ctx* ctx);
  * keep the defaults. */
 typedef struct {
     uint32_t waves_per_cu;    /* persistent grid = waves_per_cu x CUs (0: occupancy) */
-    uint32_t visited_slots;   /* LDS visited-set slots (0: from L) */
+    uint32_t visited_slots;   /* LDS visited-set slots, rounded up to a multiple of 4 (0: from L) */
     uint32_t result_cap;      /* LDS candidate FIFO entries (0: 1024); later
                                  entries go to the query's output row */
 } anqi_search_tuning;
@@ -265,7 +265,7 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
                      uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total);
 void anqi_batcher_destroy(anqi_batcher* b);
 /* Times the persistent search on a sample batch (device queries, Lq, kidx)
- * for visited-set sizes {2048, 4096, 8192}, LDS FIFO sizes {128, 512} and
+ * for visited-set sizes {1536, 2048, 3072, 4096, 8192}, LDS FIFO sizes {128, 512} and
  * 1/2, 3/4, 4/4 of the resident waves per CU; queries that overflow are
  * charged as reruns.  Writes the fastest tuning and its time. */
 int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables,
