@@ -455,7 +455,16 @@ int main(int argc, char** argv) {
     else
         std::printf("[rknn] flattened_repeats=1 tasks=%zu\n", (size_t)nq * bench_repeats);
 
-    anqi_index ix{d_base, d_bnorm, d_lifted, d_deg, d_links, n, d, dl, (uint32_t)Marg, &tree, mstar2, scale};
+    anqi_index ix{d_base, d_bnorm, d_lifted, d_deg, d_links, n, d, dl, (uint32_t)Marg, &tree, mstar2, scale,
+                  nullptr, nullptr};
+    if (anqi_cli::gpu_form_chosen() && d % 16 == 0) {
+        // persistent search: aligned main rows + precomputed lift terms
+        float* d_lmain = dalloc<float>((size_t)n * d);
+        float* d_ltail = dalloc<float>(n);
+        ck(anqi_lift_split(g_ctx, d_lifted, n, d, d_lmain, d_ltail), "lift split");
+        ix.lmain = d_lmain;
+        ix.ltail = d_ltail;
+    }
     float* d_query = upload(query);
     uint32_t out_cap = 4096;
     uint32_t* d_cand = nullptr;

@@ -308,6 +308,13 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
     return ANQI_OK;
 }
 
+int anqi_lift_split(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail) {
+    if (d == 0 || (d & 15u) || d > 16 * QMAX_CHUNKS)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "lift_split: d must be a multiple of 16, at most %d", 16 * QMAX_CHUNKS);
+    HCHK(launch_lift_split(lifted, n, d, lmain, ltail, ctx->stream));
+    return ANQI_OK;
+}
+
 // ---- lift (host FP64; src/rknn_query.cpp:2203-2236) ------------------------------------------
 int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
                    const float* basis, const float* codebook, uint32_t code_size, float tau, int mode,
@@ -1189,6 +1196,7 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     hipStream_t s = ctx->stream;
     SearchParams P{};
     P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
+    P.lmain = ix->lmain; P.ltail = ix->ltail;
     P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
     P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
     P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
@@ -1267,6 +1275,7 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
 static void fill_search_params(SearchParams& P, const anqi_index* ix) {
     P = SearchParams{};
     P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
+    P.lmain = ix->lmain; P.ltail = ix->ltail;
     P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
     P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
     P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;

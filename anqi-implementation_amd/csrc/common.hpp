@@ -158,6 +158,8 @@ __device__ __forceinline__ float qrow_l2(const QRow& q, const float* __restrict_
 // MC = number of full 16-float chunks of a row of X (dim = 16 MC + r, r < 16).
 //   MODE 0: out[s] = ||q - x||^2
 //   MODE 1: out[s] = d2_combine(<q, x>, nq, nrm[id])   (nndindex::d2)
+//   MODE 2: out[s] = ||q - x||^2 over the dim (= 16 MC) coordinates of X's rows
+//           + nrm[id] (the rest of the sum, already reduced: split lifted rows)
 template <int MC, int MODE, int RB = 4>
 __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restrict__ X, uint32_t dim,
                                            const float* __restrict__ nrm, float nq,
@@ -175,7 +177,7 @@ __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restric
 #pragma unroll
                 for (int t = 0; t < MC; t++) x[j][t] = row[16 * t + l];
                 xt[j] = (l < r) ? row[16 * MC + l] : 0.0f;
-                nb[j] = MODE == 1 ? nrm[id] : 0.0f;
+                nb[j] = MODE != 0 ? nrm[id] : 0.0f;
             } else {
 #pragma unroll
                 for (int t = 0; t < MC; t++) x[j][t] = 0.0f;
@@ -187,17 +189,18 @@ __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restric
         for (int j = 0; j < RB; j++) {
             float acc = 0.0f;
             float res;
-            if (MODE == 0) {
+            if (MODE == 0 || MODE == 2) {
 #pragma unroll
                 for (int t = 0; t < MC; t++) {
                     float a = q.v[t] - x[j][t];
                     acc = fmaf(a, a, acc);
                 }
                 res = tree16(acc);
-                if (r) {
+                if (MODE == 0 && r) {
                     float a = q.tail - xt[j];
                     res += tree16(a * a);
                 }
+                if (MODE == 2) res += nb[j];
             } else {
 #pragma unroll
                 for (int t = 0; t < MC; t++) acc = fmaf(q.v[t], x[j][t], acc);

@@ -81,11 +81,15 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
 // g+4, g+8, g+12 of each 16-row pass and loads all four before reducing any,
 // so their fetches overlap.  MC = number of full 16-float chunks of a lifted
 // row (d' = 16 MC + r, r < 16); MC == 0 selects the generic loop.
+// Split layout (P.lmain, d = 16 MC): the d leading coordinates come from the
+// aligned rows of lmain and the two lift coordinates' reduced term from
+// ltail; D = tree16(main) + tail, the same two terms the whole-row form adds.
 template <int MC>
 __device__ __forceinline__ void batch_l2(const SearchParams& P, const QRow& QL, const uint32_t* tid,
                                          float* tdv, int k, int l, int g) {
     if constexpr (MC > 0) {
-        rows_batch<MC, 0>(QL, P.lifted, P.dl, nullptr, 0.0f, tid, k, l, g, tdv);
+        if (P.lmain) rows_batch<MC, 2>(QL, P.lmain, P.d, P.ltail, 0.0f, tid, k, l, g, tdv);
+        else rows_batch<MC, 0>(QL, P.lifted, P.dl, nullptr, 0.0f, tid, k, l, g, tdv);
     } else {
         for (int s = g; s < k; s += 4) {
             float dv = qrow_l2(QL, P.lifted + (size_t)tid[s] * P.dl, P.dl, l);
@@ -303,7 +307,8 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         }
         ep = P.t_entry[node];
     }
-    float dep = qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
+    const float dep = (MC > 0 && P.lmain) ? qrow_l2(QL, P.lmain + (size_t)ep * P.d, P.d, l) + P.ltail[ep]
+                                          : qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
     uint32_t nd = 1, rcnt = 0, bsize = 1, nx = 0;
     int buf = 0, ovf = 0;
     if (lane == 0) {
@@ -523,6 +528,34 @@ hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint3
     return hipGetLastError();
 }
 
+// Split lifted layout: lmain[i] = the first d coordinates of lifted row i
+// (rows of d floats, so with d % 16 == 0 a row starts on a 64-byte boundary
+// and covers d / 32 128-byte lines instead of straddling one more), and
+// ltail[i] = the reduced term of its two lift coordinates against a query's
+// (1, 0): tree16 of the lane products ((1 - h0)^2, (0 - h1)^2, 0, ...), the
+// value the whole-row distance adds after its main reduction.  A 16-lane
+// group per row.
+__global__ void k_lift_split(const float* __restrict__ lifted, uint32_t n, uint32_t d, float* __restrict__ lmain,
+                             float* __restrict__ ltail) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const uint32_t i = blockIdx.x * 4u + (uint32_t)g;
+    if (i >= n) return;                          // whole 16-lane groups
+    const uint32_t dl = d + 2;
+    const float* row = lifted + (size_t)i * dl;
+    for (uint32_t e = (uint32_t)l; e < d; e += 16) lmain[(size_t)i * d + e] = row[e];
+    const float qt = (l == 0) ? 1.0f : 0.0f;
+    const float a = (l < 2) ? qt - row[d + l] : 0.0f;
+    const float t = tree16(a * a);
+    if (l == 0) ltail[i] = t;
+}
+
+hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lift_split, dim3((n + 3) / 4), dim3(64), 0, s, lifted, n, d, lmain, ltail);
+    return hipGetLastError();
+}
+
 // Chunk count shared by the lifted rows (d + 2) and the base rows (d).
 static int search_chunks(const SearchParams& P) {
     const int m = (int)(P.dl >> 4);
@@ -560,6 +593,8 @@ template <int MC>
 static hipError_t launch_search_mc(const SearchParams& P0, bool persistent, uint32_t grid, hipStream_t s) {
     SearchParams P = P0;
     P.rbeam = persistent && search_nch(P.Lmax) > 0;
+    // split rows: persistent form with whole 16-float chunks only
+    if (!persistent || MC == 0 || (P.d & 15u) || !P.ltail) P.lmain = nullptr;
     size_t lds = search_lds_bytes(P);
     if (persistent) {
         switch (search_nch(P.Lmax)) {

@@ -42,7 +42,13 @@ struct SearchParams {
     uint32_t* work_counter; // persistent form
     const uint32_t* nq_dev; // optional: batch size read on the device (captured batches)
     uint32_t rbeam;         // set by launch_range_search: register beam (persistent form)
+    // split lifted layout (persistent form, d % 16 == 0; null: lifted rows):
+    const float* lmain;     // n x d: the lifted rows' first d coordinates
+    const float* ltail;     // n: reduced term of each row's lift coordinates against a query's (1, 0)
 };
+// Split lifted layout of an index: lmain (n x d), ltail (n).
+hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail,
+                             hipStream_t s);
 size_t search_lds_bytes(const SearchParams& P);
 int search_persistent_occupancy(const SearchParams& P);
 hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s);

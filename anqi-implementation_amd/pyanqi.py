@@ -56,7 +56,8 @@ class SearchTuning(ctypes.Structure):
 
 class Index(ctypes.Structure):
     _fields_ = [("base", P), ("bnorm", P), ("lifted", P), ("deg", P), ("links", P), ("n", U32), ("d", U32),
-                ("dl", U32), ("M", U32), ("tree", ctypes.POINTER(Tree)), ("mstar2", F64), ("scale", F64)]
+                ("dl", U32), ("M", U32), ("tree", ctypes.POINTER(Tree)), ("mstar2", F64), ("scale", F64),
+                ("lmain", P), ("ltail", P)]
 
 
 SIGS = {
@@ -73,6 +74,7 @@ SIGS = {
     "anqi_h2d": (I32, [P, P, P, ctypes.c_size_t]),
     "anqi_d2h": (I32, [P, P, P, ctypes.c_size_t]),
     "anqi_norms": (I32, [P, P, U64, U32, P]),
+    "anqi_lift_split": (I32, [P, P, U32, U32, P, P]),
     "anqi_pair_d2": (I32, [P, P, P, U32, P, P, U64, P]),
     "anqi_exact_knn": (I32, [P, P, U32, U32, U32, I32, P, P]),
     "anqi_rknn_membership": (I32, [P, P, U32, P, U32, U32, P, P, ctypes.POINTER(U64), P, P]),
@@ -379,10 +381,23 @@ class Context:
                   dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
         return links, deg, t, st
 
-    def make_index(self, base, bnorm, lifted, deg, links, tree, mstar2, scale):
+    def lift_split(self, lifted):
+        """Split lifted layout (anqi_lift_split): (lmain n x d, ltail n)."""
+        n, dl = lifted.shape
+        d = dl - 2
+        lmain = self.empty((n, d), self.torch.float32)
+        ltail = self.empty((n,), self.torch.float32)
+        self.call("anqi_lift_split", dp(lifted), n, d, dp(lmain), dp(ltail))
+        return lmain, ltail
+
+    def make_index(self, base, bnorm, lifted, deg, links, tree, mstar2, scale, split=True):
+        """split: give the persistent search the split lifted layout (d % 16 == 0)."""
+        lmain = ltail = None
+        if split and base.shape[1] % 16 == 0:
+            lmain, ltail = self.lift_split(lifted)
         ix = Index(dp(base), dp(bnorm), dp(lifted), dp(deg), dp(links), base.shape[0], base.shape[1],
-                   lifted.shape[1], links.shape[1], ctypes.pointer(tree), mstar2, scale)
-        ix._keep = (base, bnorm, lifted, deg, links, tree)
+                   lifted.shape[1], links.shape[1], ctypes.pointer(tree), mstar2, scale, dp(lmain), dp(ltail))
+        ix._keep = (base, bnorm, lifted, deg, links, tree, lmain, ltail)
         return ix
 
     def search_buffers(self, nq, out_cap):

@@ -46,9 +46,12 @@ def log(*a):
 
 
 def query_bytes(ix, bufs):
-    """Algorithmic HBM bytes of one batch: 4 d' per graph distance, 4 (G+1)
-    per expanded node, (4 d + 4) per verified candidate, 4 d per query."""
+    """Algorithmic HBM bytes of one persistent-search batch: 4 d' per graph
+    distance (4 (d + 1) with the split lifted layout: main row + lift term),
+    4 (G+1) per expanded node, (4 d + 4) per verified candidate, 4 d per query."""
     d, dl, G = ix.base.shape[1], ix.lifted.shape[1], ix.links.shape[1]
+    if ix.handle.lmain:
+        dl = d + 1
     nd = bufs["nd"].double().sum().item()
     nx = bufs["nexp"].double().sum().item()
     nc = bufs["cnt"].double().sum().item()

@@ -221,7 +221,19 @@ typedef struct {
     uint32_t n, d, dl, M;
     const anqi_tree* tree;
     double mstar2, scale;
+    /* optional split lifted layout from anqi_lift_split (both NULL: the
+     * persistent search reads the lifted rows); used when d % 16 == 0 */
+    const float* lmain;     /* device n*d */
+    const float* ltail;     /* device n */
 } anqi_index;
+/* Split lifted layout for the persistent search: lmain = the first d
+ * coordinates of each lifted row (rows of d floats, so a row covers d/32
+ * whole 128-byte lines when d % 16 == 0), ltail = each row's lift-coordinate
+ * term of the lifted distance to a query's (1, 0), reduced as the whole-row
+ * distance reduces it.  D = main term + ltail[i] has the bits of the
+ * whole-row D.  No counterpart in the reference (a layout of
+ * rknn_query.cpp:2203-2236's lifted rows).  d % 16 == 0. */
+int anqi_lift_split(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail);
 /* queries (device nq*d, base space).  thr: device thresholds for k (n floats)
  * or, with kidx, tables (ntables*n) picked per query.  Lq/kidx optional
  * device per-query arrays.  Outputs (device): cand_ids/accept nq*out_cap,

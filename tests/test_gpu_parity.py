@@ -160,14 +160,16 @@ def _ref_tree_dev(ctx, S):
                                 entry=entry))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_range_search_and_postfilter_on_reference_graph(ctx, S, mode):
+@pytest.mark.parametrize("mode,split", [(0, False), (1, False), (1, True)])
+def test_range_search_and_postfilter_on_reference_graph(ctx, S, mode, split):
     """Candidate lists (identical order), #dist and accept decisions against
-    the reference's own range_search on the reference-built graph."""
+    the reference's own range_search on the reference-built graph; the
+    persistent form with whole lifted rows and with the split layout."""
     tree = _ref_tree_dev(ctx, S)
     base = ctx.to_dev(S.base)
+    assert S.base.shape[1] % 16 == 0
     ix = ctx.make_index(base, ctx.norms(base), ctx.to_dev(S.lifted), ctx.to_dev(S.deg.view(np.int32)),
-                        ctx.to_dev(S.links.view(np.int32)), tree, S.M2, S.scale)
+                        ctx.to_dev(S.links.view(np.int32)), tree, S.M2, S.scale, split=split)
     q = ctx.to_dev(S.query)
     for k in (10, 50, 100):
         thr = ctx.to_dev(S.h[f"thr_k{k}"])
@@ -187,6 +189,21 @@ def test_range_search_and_postfilter_on_reference_graph(ctx, S, mode):
                 if L == 128:
                     assert acc[i, :cnt[i]].tolist() == decs[o128[i]:o128[i + 1]].tolist()
     ctx.tree_free(tree)
+
+
+def test_lift_split_layout(ctx, S):
+    """lmain = the lifted rows' first d coordinates; ltail = the FP32 sum of
+    the two rounded squares (1 - h0)^2 and h1^2 (the lift coordinates' term
+    of the whole-row distance to a query's (1, 0))."""
+    n, dl = S.lifted.shape
+    d = dl - 2
+    lmain, ltail = ctx.lift_split(ctx.to_dev(S.lifted))
+    assert np.array_equal(bits(host(lmain)), bits(np.ascontiguousarray(S.lifted[:, :d])))
+    h0, h1 = S.lifted[:, d].astype(np.float32), S.lifted[:, d + 1].astype(np.float32)
+    a0 = np.float32(1.0) - h0
+    exp = (a0 * a0).astype(np.float32) + (h1 * h1).astype(np.float32)
+    assert exp.dtype == np.float32
+    assert np.array_equal(bits(host(ltail)), bits(exp))
 
 
 def test_graph_steps_match_oracle_on_sample(ctx, S):
