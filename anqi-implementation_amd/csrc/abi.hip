@@ -27,7 +27,7 @@ struct anqi_ctx {
     int cus = 0;
     uint32_t* counter = nullptr;   // device work counter of the persistent search
     anqi_search_tuning tune{0, 0, 0};   // persistent search work areas / grid (0 = defaults)
-    uint32_t* ent = nullptr;       // search-tree entries of the persistent search's batch
+    uint32_t* ent = nullptr;       // persistent search's batch: entries, keys, work order (3 x ent_cap)
     size_t ent_cap = 0;
 };
 
@@ -1222,14 +1222,19 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
         HCHK(launch_fill_u32(ctx->counter, 0u, 1, s));
         P.work_counter = ctx->counter;
         if (!P.entries) {
-            // the batch's tree descents first, four queries per wavefront
+            // the batch's tree descents first, four queries per wavefront,
+            // then the work order: nearest entries first
             if (ctx->ent_cap < nq) {
                 if (ctx->ent) { HCHK(hipStreamSynchronize(s)); HCHK(hipFree(ctx->ent)); ctx->ent = nullptr; }
-                HCHK(hipMalloc((void**)&ctx->ent, 4ull * nq));
+                HCHK(hipMalloc((void**)&ctx->ent, 12ull * nq));
                 ctx->ent_cap = nq;
             }
-            HCHK(launch_tree_entry_batch(P, nq, ctx->ent, s));
+            float* key = (float*)(ctx->ent + nq);
+            uint32_t* order = ctx->ent + 2ull * nq;
+            HCHK(launch_tree_entry_batch(P, nq, ctx->ent, key, s));
+            HCHK(launch_entry_order(P, key, order, s));
             P.entries = ctx->ent;
+            P.qlist = order;
         }
         int occ = std::max(1, search_persistent_occupancy(P));
         uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
@@ -1312,7 +1317,10 @@ static hipError_t batch_enqueue(anqi_batcher* b) {
     if ((e = launch_fill_u32(b->counter.as<uint32_t>(), 0u, 1, b->s)) != hipSuccess) return e;
     SearchParams T = b->P;
     T.entries = nullptr;
-    if ((e = launch_tree_entry_batch(T, b->max_batch, b->ent.as<uint32_t>(), b->s)) != hipSuccess) return e;
+    T.qlist = nullptr;
+    float* key = (float*)(b->ent.as<uint32_t>() + b->max_batch);
+    if ((e = launch_tree_entry_batch(T, b->max_batch, b->ent.as<uint32_t>(), key, b->s)) != hipSuccess) return e;
+    if ((e = launch_entry_order(T, key, b->ent.as<uint32_t>() + 2ull * b->max_batch, b->s)) != hipSuccess) return e;
     if ((e = launch_range_search(b->P, true, b->grid, b->s)) != hipSuccess) return e;
     if ((e = launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(),
                                b->s)) != hipSuccess)
@@ -1323,6 +1331,17 @@ static hipError_t batch_enqueue(anqi_batcher* b) {
 }
 
 extern "C" {
+
+int anqi_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t* entries) {
+    if (!ix || !ix->tree || ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
+    SearchParams P;
+    fill_search_params(P, ix);
+    P.queries = queries;
+    P.nq = nq;
+    HCHK(launch_tree_entry_batch(P, nq, entries, nullptr, ctx->stream));
+    return ANQI_OK;
+}
 
 int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables, uint32_t ntables,
                         uint32_t max_batch, uint32_t Lmax, uint32_t out_cap, const anqi_search_tuning* tune,
@@ -1350,7 +1369,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     HCHK(b->off.alloc(4 * (mb + 1)));
     HCHK(b->ids.alloc(4 * mb * out_cap));
     HCHK(b->counter.alloc(256));
-    HCHK(b->ent.alloc(4 * mb));
+    HCHK(b->ent.alloc(12 * mb));   // entries, keys, work order
     HCHK(hipHostMalloc((void**)&b->hq, 4 * mb * ix->d, hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&b->hL, 4 * mb, hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&b->hk, 4 * mb, hipHostMallocDefault));
@@ -1372,6 +1391,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     P.ndist = b->nd.as<uint32_t>(); P.pred_cnt = b->pred.as<uint32_t>(); P.nexp = nullptr;
     P.status = b->status.as<uint32_t>(); P.work_counter = b->counter.as<uint32_t>();
     P.entries = b->ent.as<uint32_t>();
+    P.qlist = b->ent.as<uint32_t>() + 2 * mb;
     const int occ = std::max(1, search_persistent_occupancy(P));
     const uint32_t wpc = t.waves_per_cu ? std::min<uint32_t>(t.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
     b->grid = wpc * (uint32_t)std::max(1, ctx->cus);
@@ -1471,10 +1491,12 @@ int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_t
     P.cand_ids = cand.as<uint32_t>(); P.accept = acc.as<uint8_t>(); P.cand_cnt = cnt.as<uint32_t>();
     P.ndist = nd.as<uint32_t>(); P.pred_cnt = pred.as<uint32_t>(); P.status = status.as<uint32_t>();
     P.work_counter = counter.as<uint32_t>();
-    DBuf ent;   // the sample's tree entries, computed once as the persistent form does
-    HCHK(ent.alloc(4ull * nq));
-    HCHK(launch_tree_entry_batch(P, nq, ent.as<uint32_t>(), s));
+    DBuf ent;   // the sample's tree entries and work order, computed once as the persistent form does
+    HCHK(ent.alloc(12ull * nq));
+    HCHK(launch_tree_entry_batch(P, nq, ent.as<uint32_t>(), (float*)(ent.as<uint32_t>() + nq), s));
+    HCHK(launch_entry_order(P, (const float*)(ent.as<uint32_t>() + nq), ent.as<uint32_t>() + 2ull * nq, s));
     P.entries = ent.as<uint32_t>();
+    P.qlist = ent.as<uint32_t>() + 2ull * nq;
     hipEvent_t e0, e1;
     HCHK(hipEventCreate(&e0));
     HCHK(hipEventCreate(&e1));

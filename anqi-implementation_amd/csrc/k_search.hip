@@ -18,9 +18,13 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
-// Flood nodes expanded together (their adjacency rows and all their new
-// neighbours' rows are fetched in one round).
-#define SR_FLOOD 4
+// Flood: up to FL FIFO nodes are expanded together (their adjacency rows,
+// then all their new neighbours' rows, are fetched in one round each), as
+// long as their neighbours fit the SR_TID-entry id buffer.  FL is
+// SR_FLOOD_PLAIN in the plain form and SR_FLOOD_MAX in the persistent form.
+#define SR_FLOOD_PLAIN 4
+#define SR_FLOOD_MAX 16
+#define SR_TID 256
 
 namespace {
 
@@ -35,8 +39,8 @@ struct SearchLds {
     uint32_t* hash;    // visited set
     uint32_t* res;     // collected ids (FIFO), first rcap entries
     uint32_t* spill;   // the query's output row: FIFO entries from rcap on
-    uint32_t* tid;     // 64 new ids
-    float* tdv;        // 64 distances
+    uint32_t* tid;     // SR_TID new ids
+    float* tdv;        // SR_TID distances
     uint64_t* cs;      // 64 sorted candidate keys
 };
 
@@ -69,8 +73,8 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
     }
     s.hash = (uint32_t*)take(sizeof(uint32_t) * (P.hmask + 1));
     s.res = (uint32_t*)take(sizeof(uint32_t) * P.rcap);
-    s.tid = (uint32_t*)take(sizeof(uint32_t) * SR_FLOOD * 64);
-    s.tdv = (float*)take(sizeof(float) * SR_FLOOD * 64);
+    s.tid = (uint32_t*)take(sizeof(uint32_t) * SR_TID);
+    s.tdv = (float*)take(sizeof(float) * SR_TID);
     return s;
 }
 
@@ -206,18 +210,20 @@ __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, c
     return true;
 }
 
-// Flood step over the nb (<= SR_FLOOD) FIFO entries at head: the visited-set
-// inserts run node by node in FIFO order, neighbours in adjacency order, so
-// every neighbour is attributed to the same first visitor as in a one-node-
-// at-a-time flood; collected nodes are appended in that same order.  Their
-// distances are computed in one batch.
-template <int MC>
+// Flood step over up to nb (<= FL) FIFO entries at head: the
+// visited-set inserts run node by node in FIFO order, neighbours in
+// adjacency order, so every neighbour is attributed to the same first
+// visitor as in a one-node-at-a-time flood; collected nodes are appended in
+// that same order.  Their distances are computed in one batch.  The step
+// stops before a node whose neighbours might not fit the id buffer; nb is
+// set to the number of nodes expanded (at least one: deg <= 64 <= SR_TID).
+template <int MC, int FL>
 __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t head,
-                                           uint32_t nb, uint32_t& rcnt, uint32_t& nd, int& ovf) {
+                                           uint32_t& nb, uint32_t& rcnt, uint32_t& nd, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
-    uint32_t dg[SR_FLOOD], nbr[SR_FLOOD];
+    uint32_t dg[FL], nbr[FL];
 #pragma unroll
-    for (int t = 0; t < SR_FLOOD; t++) {
+    for (int t = 0; t < FL; t++) {
         dg[t] = 0;
         nbr[t] = U32_NONE;
         if ((uint32_t)t < nb) {
@@ -227,8 +233,11 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
         }
     }
     int k = 0;
+    uint32_t done = nb;
 #pragma unroll
-    for (int t = 0; t < SR_FLOOD; t++) {
+    for (int t = 0; t < FL; t++) {
+        if ((uint32_t)t < done && k + (int)dg[t] > SR_TID) done = (uint32_t)t;
+        if ((uint32_t)t >= done) continue;
         int fresh = 0;
         if ((uint32_t)lane < dg[t]) {
             int r = hset_insert(S.hash, P.hmask, nbr[t]);
@@ -239,6 +248,7 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
         if (fresh) S.tid[k + lanes_below(fm)] = nbr[t];
         k += popc64(fm);
     }
+    nb = done;
     if (__any(ovf)) { ovf = 1; return false; }
     __syncthreads();
     batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
@@ -258,8 +268,8 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
 }
 
 // NCH == 0: beam sorted in LDS (plain form); NCH > 0: register beam of
-// 64 NCH positions (persistent form, L <= 64 NCH).
-template <int MC, int NCH>
+// 64 NCH positions (persistent form, L <= 64 NCH).  FL: flood step width.
+template <int MC, int NCH, int FL>
 __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     const int lane = __lane_id(), l = lane & 15;
     S.spill = P.cand_ids + (size_t)qi * P.out_cap;
@@ -368,8 +378,8 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     // (2) flood over collected nodes, FIFO order
     uint32_t head = 0;
     while (!ovf && head < rcnt) {
-        const uint32_t nb = min((uint32_t)SR_FLOOD, rcnt - head);
-        if (!flood_step<MC>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
+        uint32_t nb = min((uint32_t)FL, rcnt - head);
+        if (!flood_step<MC, FL>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
         head += nb;
         nx += nb;
     }
@@ -461,7 +471,7 @@ __global__ void __launch_bounds__(64) k_range_search(SearchParams P) {
     SearchLds S = carve(lds, P);
     if (blockIdx.x >= P.nq) return;
     uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
-    search_one<MC, 0>(P, S, qi);
+    search_one<MC, 0, SR_FLOOD_PLAIN>(P, S, qi);
 }
 
 // Persistent form: a fixed grid pulls query indices from a device counter.
@@ -479,7 +489,7 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
         __syncthreads();
         if (t >= nq) break;
         uint32_t qi = P.qlist ? P.qlist[t] : t;
-        search_one<MC, NCH>(P, S, qi);
+        search_one<MC, NCH, SR_FLOOD_MAX>(P, S, qi);
         __syncthreads();
     }
 }
@@ -489,7 +499,10 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
 // many queries overlap; per level the node's children, offset and
 // hyperplane are fetched together.  Same descent as search_one's
 // (rptrees.h:108-123, ties go left).  Queries beyond nq (or *nq_dev) exit.
-__global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_t* __restrict__ out) {
+// key (optional): the lifted distance from the query to its entry, the
+// scheduling key of k_entry_order.
+__global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_t* __restrict__ out,
+                                                         float* __restrict__ key) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const uint32_t nq = P.nq_dev ? *P.nq_dev : P.nq;
     const uint32_t qi = blockIdx.x * 4u + (uint32_t)g;
@@ -519,12 +532,55 @@ __global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_
         const float mg = ip - off;
         node = (mg < -1e-6f) ? left : (mg > 1e-6f ? right : left);
     }
-    if (l == 0) out[qi] = P.t_entry[node];
+    const uint32_t ep = P.t_entry[node];
+    if (l == 0) out[qi] = ep;
+    if (key) {
+        const float dep = (P.d % 16 == 0 && P.lmain && P.ltail)
+                              ? qrow_l2(QL, P.lmain + (size_t)ep * P.d, P.d, l) + P.ltail[ep]
+                              : qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
+        if (l == 0) key[qi] = dep;
+    }
 }
 
-hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint32_t* out, hipStream_t s) {
+// Work-queue order of a batch: queries by ascending key (the lifted distance
+// to the entry), so the queries likely to collect many candidates -- and so
+// to run the longest floods -- start first and the batch does not wait on a
+// late long query.  A counting sort over ORD_BUCKETS buckets of width
+// R2 / 64 (the last bucket takes every key from 4 R2 on); order within a
+// bucket is arbitrary.  One workgroup.  Every query's results are its own,
+// so the order changes only the timing.
+#define ORD_BUCKETS 256
+__global__ void __launch_bounds__(1024) k_entry_order(const float* __restrict__ key, const uint32_t* nq_dev,
+                                                      uint32_t nq_host, float inv_w, uint32_t* __restrict__ qlist) {
+    __shared__ uint32_t cnt[ORD_BUCKETS];
+    const uint32_t nq = nq_dev ? *nq_dev : nq_host;
+    const uint32_t t = threadIdx.x;
+    auto bucket = [&](uint32_t q) -> uint32_t {
+        const float b = fminf(key[q] * inv_w, (float)(ORD_BUCKETS - 1));   // NaN -> last bucket
+        return b >= 0.0f ? (uint32_t)b : 0u;
+    };
+    for (uint32_t i = t; i < ORD_BUCKETS; i += blockDim.x) cnt[i] = 0u;
+    __syncthreads();
+    for (uint32_t q = t; q < nq; q += blockDim.x) atomicAdd(&cnt[bucket(q)], 1u);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < ORD_BUCKETS; i++) { const uint32_t c = cnt[i]; cnt[i] = run; run += c; }
+    }
+    __syncthreads();
+    for (uint32_t q = t; q < nq; q += blockDim.x) qlist[atomicAdd(&cnt[bucket(q)], 1u)] = q;
+}
+
+hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint32_t* out, float* key,
+                                   hipStream_t s) {
     if (max_nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tree_entry_batch, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out);
+    hipLaunchKernelGGL(k_tree_entry_batch, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out, key);
+    return hipGetLastError();
+}
+
+hipError_t launch_entry_order(const SearchParams& P, const float* key, uint32_t* qlist, hipStream_t s) {
+    const float inv_w = 64.0f / fmaxf(P.R2f, 1e-30f);
+    hipLaunchKernelGGL(k_entry_order, dim3(1), dim3(1024), 0, s, key, P.nq_dev, P.nq, inv_w, qlist);
     return hipGetLastError();
 }
 
@@ -573,7 +629,7 @@ size_t search_lds_bytes(const SearchParams& P) {
     auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
     const size_t beam = P.rbeam ? r(8 * (size_t)P.Lmax) : 4 * r(4 * (size_t)P.Lmax) + r(8 * 64);
     return beam + r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) +
-           r(4 * SR_FLOOD * 64) + r(4 * SR_FLOOD * 64);
+           r(4 * SR_TID) + r(4 * SR_TID);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {

@@ -52,8 +52,12 @@ hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float*
 size_t search_lds_bytes(const SearchParams& P);
 int search_persistent_occupancy(const SearchParams& P);
 hipError_t launch_range_search(const SearchParams& P, bool persistent, uint32_t grid, hipStream_t s);
-// Search-tree entries of queries [0, nq) (nq from P.nq_dev when set) into out.
-hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint32_t* out, hipStream_t s);
+// Search-tree entries of queries [0, nq) (nq from P.nq_dev when set) into out;
+// key (optional): each query's lifted distance to its entry.
+hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint32_t* out, float* key,
+                                   hipStream_t s);
+// Work-queue order (qlist) of queries [0, nq) by ascending key (one workgroup).
+hipError_t launch_entry_order(const SearchParams& P, const float* key, uint32_t* qlist, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // Batch result compaction (k_batch.hip): off = exclusive scan of pred_cnt over
 // *nq_dev queries (one block), then each query's accepted ids in candidate

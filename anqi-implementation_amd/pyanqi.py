@@ -75,6 +75,7 @@ SIGS = {
     "anqi_d2h": (I32, [P, P, P, ctypes.c_size_t]),
     "anqi_norms": (I32, [P, P, U64, U32, P]),
     "anqi_lift_split": (I32, [P, P, U32, U32, P, P]),
+    "anqi_search_entries": (I32, [P, ctypes.POINTER(Index), P, U32, P]),
     "anqi_pair_d2": (I32, [P, P, P, U32, P, P, U64, P]),
     "anqi_exact_knn": (I32, [P, P, U32, U32, U32, I32, P, P]),
     "anqi_rknn_membership": (I32, [P, P, U32, P, U32, U32, P, P, ctypes.POINTER(U64), P, P]),
@@ -406,6 +407,12 @@ class Context:
                     cnt=self.empty((nq,), T.int32), nd=self.empty((nq,), T.int32),
                     pred=self.empty((nq,), T.int32), nexp=self.empty((nq,), T.int32),
                     status=self.empty((nq,), T.int32))
+
+    def search_entries(self, ix, queries):
+        """Search-tree entry point of each query (anqi_search_entries)."""
+        out = self.empty((queries.shape[0],), self.torch.int32)
+        self.call("anqi_search_entries", ctypes.byref(ix), dp(queries), queries.shape[0], dp(out))
+        return out
 
     def rknn_search(self, ix, queries, L, thr, out_cap=2048, mode=0, Lq=None, Lmax=0, kidx=None, bufs=None,
                     entries=None):

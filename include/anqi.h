@@ -255,6 +255,10 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
                              uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
                              uint32_t* status, int mode);
 
+/* Search-tree entry point of each query (device nq): the descent the search
+ * runs (rptrees.h:108-123, ties go left), for anqi_rknn_search_entries. */
+int anqi_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t* entries);
+
 /* ---- captured query batches (configuration 5: mixed k and L) -------------
  * No counterpart in the reference (its query driver loops over queries on
  * CPU threads, src/rknn_query.cpp:2692-2824): this is the batch form of the

@@ -107,6 +107,34 @@ def test_register_beam_equals_lds_beam(env, L):
         assert np.array_equal(a["acc"][i, :c], b["acc"][i, :c])
 
 
+@pytest.mark.parametrize("nq", [1, 5, 1500, 4097])
+def test_work_order_covers_every_query(env, nq):
+    """The persistent search takes queries in the order of their lifted
+    distance to the entry (a counting sort on the device).  Every query is
+    searched exactly once and its results do not depend on the order: the
+    ordered run (output buffers pre-filled with sentinels) equals the run
+    with caller-given entries, which keeps the batch order."""
+    ctx, torch = env["ctx"], env["torch"]
+    ix = env["ix"].handle
+    q = ctx.to_dev(env["q"][:nq])
+    thr = env["tables"][1].contiguous()
+    ent = ctx.search_entries(ix, q)
+    b = ctx.search_buffers(nq, 4096)
+    for k, v in b.items():
+        v.fill_(-1)
+    ctx.rknn_search(ix, q, 96, thr, mode=1, bufs=b)
+    a = ctx.rknn_search(ix, q, 96, thr, mode=1, entries=ent)
+    a = {k: v.cpu().numpy() for k, v in a.items()}
+    b = {k: v.cpu().numpy() for k, v in b.items()}
+    assert (b["status"] == 0).all()
+    for f in ("cnt", "nd", "nexp", "pred", "status"):
+        assert np.array_equal(a[f], b[f]), f
+    for i in range(nq):
+        c = int(a["cnt"][i])
+        assert np.array_equal(a["cand"][i, :c], b["cand"][i, :c])
+        assert np.array_equal(a["acc"][i, :c], b["acc"][i, :c])
+
+
 def test_chosen_build_equals_plain(env):
     """The chosen build (bf16 MFMA leaf sweep, batched gathers) yields the
     plain build's index bit for bit: AKNN lists, cutoffs, LRQ codes, lifted
