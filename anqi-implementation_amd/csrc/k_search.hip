@@ -24,7 +24,7 @@
 // SR_FLOOD_PLAIN in the plain form and SR_FLOOD_MAX in the persistent form.
 #define SR_FLOOD_PLAIN 4
 #define SR_FLOOD_MAX 16
-#define SR_TID 256
+#define SR_TID 128
 
 namespace {
 
