@@ -731,12 +731,59 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         if (visit(id, id != U32_NONE && id < P.n, dv)) process(true);
     }
     // beam over the neighbour heaps and their reverse lists
-    while (!status) {
-        uint32_t c;
-        if constexpr (QCH > 0) {
-            c = rk_take(Q, true);
+    if constexpr (QCH > 0) {
+        // Register-list form.  An expansion's adjacency (heap slots, the
+        // reverse-list bounds and its first 64 entries) is requested at once,
+        // and that of the entry next in line once c is marked (pc) is
+        // fetched while c is expanded; it is used when no new candidate of
+        // c overtakes it.  The visits keep the reference order (heap slots,
+        // then the reverse list), and every first visit of an expansion is
+        // processed before the stop test, as in the loop below.
+        uint32_t pc = U32_NONE, pa0 = U32_NONE, pa1 = U32_NONE, pr0 = 0, pr1 = 0;
+        const uint32_t cap = P.cap;
+        auto heap_slot = [&](uint32_t node, uint32_t s) -> uint32_t {
+            return s < cap ? P.hidx[(size_t)node * cap + s] : U32_NONE;
+        };
+        while (!status) {
+            const uint32_t c = rk_take(Q, true);
             if (c == U32_NONE) break;
-        } else {
+            uint32_t a0, a1, r0, r1;
+            if (c == pc) {
+                a0 = pa0; a1 = pa1; r0 = pr0; r1 = pr1;
+            } else {
+                a0 = heap_slot(c, (uint32_t)lane);
+                a1 = heap_slot(c, 64u + (uint32_t)lane);
+                r0 = P.roff[c];
+                r1 = P.roff[c + 1];
+            }
+            const uint32_t rv0 = (r0 + (uint32_t)lane < r1) ? P.rids[r0 + (uint32_t)lane] : U32_NONE;
+            pc = rk_take(Q, false);
+            if (pc != U32_NONE) {
+                pa0 = heap_slot(pc, (uint32_t)lane);
+                pa1 = heap_slot(pc, 64u + (uint32_t)lane);
+                pr0 = P.roff[pc];
+                pr1 = P.roff[pc + 1];
+            }
+            // chunks t < nh: heap slots 64 t.. ; then the reverse list from r0
+            const uint32_t nh = (cap + 63) / 64, nt = nh + (r1 - r0 + 63) / 64;
+            for (uint32_t t = 0; t < nt && !status; t++) {
+                uint32_t id;
+                if (t == 0) id = a0;
+                else if (t == 1 && nh > 1) id = a1;
+                else if (t < nh) id = heap_slot(c, 64u * t + (uint32_t)lane);
+                else if (t == nh) id = rv0;
+                else {
+                    const uint32_t sr = r0 + 64u * (t - nh) + (uint32_t)lane;
+                    id = sr < r1 ? P.rids[sr] : U32_NONE;
+                }
+                if (visit(id, id != U32_NONE && id < P.n, 0.0f) && nf + 64 > tcap) process(false);
+            }
+            if (!status) process(false);
+            if (pcount >= P.pool_cap) break;
+        }
+    } else while (!status) {
+        uint32_t c;
+        {
             uint32_t cur = U32_NONE;
             for (uint32_t b = 0; b < qsize; b += 64) {
                 uint32_t t = b + lane;
