@@ -383,10 +383,19 @@ int main(int argc, char** argv) {
     std::vector<float> rk(n);
     for (uint32_t i = 0; i < n; i++) rk[i] = aknn_keys[(size_t)i * 100 + (GRAPH_NK - 1)];
     const uint32_t dl = d + 2;
-    std::vector<float> lifted((size_t)n * dl);
     double mstar2 = 0, scale = 0;
-    ck(anqi_lift_host(base.data(), n, d, rk.data(), lifted.data(), &mstar2, &scale), "lift");
-    float* d_lifted = upload(lifted);
+    float* d_lifted = nullptr;
+    if (anqi_cli::gpu_form_chosen()) {
+        // device lift (integer soft binary64), the same bits as the host lift
+        float* d_rk = upload(rk);
+        d_lifted = dalloc<float>((size_t)n * dl);
+        ck(anqi_lift(g_ctx, d_base, n, d, d_rk, d_lifted, &mstar2, &scale), "lift");
+        anqi_dfree(g_ctx, d_rk);
+    } else {
+        std::vector<float> lifted((size_t)n * dl);
+        ck(anqi_lift_host(base.data(), n, d, rk.data(), lifted.data(), &mstar2, &scale), "lift");
+        d_lifted = upload(lifted);
+    }
     const double lift_verifier_prepare_s = since(tv0);
     std::printf("[rknn] BUILD lift M*^2=%.6g scale=%.4g (graph radius=aknn_k100; verifier=rankm_resid_u8_lrq_floor:M8:"
                 "b0:objradius_reconstruction:basis%s:cb%s:postfilter)\n",

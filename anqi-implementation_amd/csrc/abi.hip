@@ -395,6 +395,40 @@ int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_ho
     return ANQI_OK;
 }
 
+int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
+              double* mstar2, double* scale) {
+    if (n == 0 || d == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "lift: empty input");
+    hipStream_t s = ctx->stream;
+    DBuf rows, mx;
+    HCHK(rows.alloc(8 * n));
+    HCHK(mx.alloc(256));
+    // squared norms per row on the device, summed in row order here (the
+    // reference's reduction order)
+    HCHK(launch_lift_norm2(base, n, d, rows.as<uint64_t>(), s));
+    std::vector<double> sn(n);
+    HCHK(hipMemcpyAsync(sn.data(), rows.p, 8 * n, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    double sum_n2 = 0.0;
+    for (uint64_t i = 0; i < n; i++) sum_n2 += sn[i];
+    const double s_scale = 1.0 / std::sqrt(sum_n2 / (double)n + 1e-12), sc2 = s_scale * s_scale;
+    uint64_t sb, cb;
+    std::memcpy(&sb, &s_scale, 8);
+    std::memcpy(&cb, &sc2, 8);
+    HCHK(hipMemsetAsync(mx.p, 0, 8, s));
+    HCHK(launch_lift_rows(base, n, d, rk_horizon, sb, cb, lifted, mx.as<uint64_t>(), s));
+    double maxn2 = 0.0;
+    HCHK(hipMemcpyAsync(&maxn2, mx.p, 8, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    const double M2 = maxn2 * 1.0001;
+    uint64_t mb;
+    std::memcpy(&mb, &M2, 8);
+    HCHK(launch_lift_pad(lifted, n, d, mb, s));
+    HCHK(hipStreamSynchronize(s));
+    *mstar2 = M2;
+    *scale = s_scale;
+    return ANQI_OK;
+}
+
 // ---- graph pieces -----------------------------------------------------------------------------
 int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* ids,
                     uint32_t width, uint32_t cap, uint32_t* hidx, float* hval) {

@@ -46,6 +46,13 @@ struct SearchParams {
     const float* lmain;     // n x d: the lifted rows' first d coordinates
     const float* ltail;     // n: reduced term of each row's lift coordinates against a query's (1, 0)
 };
+// Lift of the base rows (k_lift.hip): pass 1 per-row squared norms (binary64
+// bits), pass 2 rows h_0..h_d and the maximum lifted norm (atomic maximum on
+// the bits of *maxbits, which starts at 0), pass 3 the padding coordinate.
+hipError_t launch_lift_norm2(const float* x, uint64_t n, uint32_t d, uint64_t* s_out, hipStream_t s);
+hipError_t launch_lift_rows(const float* x, uint64_t n, uint32_t d, const float* rk, uint64_t scale, uint64_t sc2,
+                            float* lifted, uint64_t* maxbits, hipStream_t s);
+hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, hipStream_t s);
 // Split lifted layout of an index: lmain (n x d), ltail (n).
 hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail,
                              hipStream_t s);

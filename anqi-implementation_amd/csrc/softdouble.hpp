@@ -174,6 +174,32 @@ SD_HD float sd_to_float(uint64_t b) {
     return f;
 }
 
+// sqrt, correctly rounded (b >= 0 finite; a negative b gives the default NaN).
+// The root of m 2^e (e even, m normalised to 53 or 54 bits) is the integer
+// root of M = m 2^58 (56 bits, M in [2^110, 2^112)) times 2^(e/2 - 29); the
+// 56-bit floor root plus a sticky bit for a non-zero remainder rounds to the
+// same 53 bits as the exact root (a square root is never exactly halfway).
+SD_HD uint64_t sd_sqrt(uint64_t b) {
+    if ((b & 0x7fffffffffffffffull) == 0) return b;                 // +-0
+    if (b >> 63) return 0x7ff8000000000000ull;
+    if (((b >> 52) & 0x7ff) == 0x7ff) return b;                      // inf, NaN
+    sd_num A = sd_unpack(b);
+    uint64_t m = A.m;
+    int e = A.e;
+    const int lz = __builtin_clzll(m) - 11;                          // msb to bit 52
+    m <<= lz;
+    e -= lz;
+    if (e & 1) { m <<= 1; e -= 1; }
+    sd_u128 op = (sd_u128)m << 58, res = 0, one = ((sd_u128)1) << 126;
+    while (one > op) one >>= 2;
+    while (one != 0) {
+        if (op >= res + one) { op -= res + one; res = (res >> 1) + one; }
+        else res >>= 1;
+        one >>= 2;
+    }
+    return sd_pack(0, (res << 1) | (op != 0 ? 1 : 0), e / 2 - 30);
+}
+
 SD_HD int sd_lt_zero_or_zero(uint64_t b) {     // value <= 0 (for max(0, x))
     return (b >> 63) || (b & 0x7fffffffffffffffull) == 0;
 }

@@ -58,9 +58,13 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     _sync(ctx)
     t3 = time.perf_counter()
     t["encode_s"] = t3 - t2
-    rk = cut[:, K - 1].contiguous().cpu().numpy()
-    lifted_np, mstar2, scale = pyanqi.lift_host(base_np, rk)
-    lifted = ctx.to_dev(lifted_np)
+    if precision >= 1:
+        # device lift (soft binary64), bit-identical to the host lift
+        lifted, mstar2, scale = ctx.lift(base, cut[:, K - 1].contiguous())
+    else:
+        rk = cut[:, K - 1].contiguous().cpu().numpy()
+        lifted_np, mstar2, scale = pyanqi.lift_host(base_np, rk)
+        lifted = ctx.to_dev(lifted_np)
     _sync(ctx)
     t4 = time.perf_counter()
     t["lift_s"] = t4 - t3

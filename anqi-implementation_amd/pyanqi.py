@@ -84,6 +84,7 @@ SIGS = {
     "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
+    "anqi_lift": (I32, [P, P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, I32, F32, ctypes.POINTER(F64), P, P]),
     "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
     "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32), I32]),
@@ -381,6 +382,14 @@ class Context:
         self.call("anqi_graph_build", dp(lifted), n, dl, dp(warm_ids), warm_ids.shape[1], ctypes.byref(p),
                   dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
         return links, deg, t, st
+
+    def lift(self, base, rk_horizon):
+        """Lift of device base rows on the device (anqi_lift): (lifted, mstar2, scale)."""
+        n, d = base.shape
+        lifted = self.empty((n, d + 2), self.torch.float32)
+        m2, sc = F64(), F64()
+        self.call("anqi_lift", dp(base), n, d, dp(rk_horizon), dp(lifted), ctypes.byref(m2), ctypes.byref(sc))
+        return lifted, m2.value, sc.value
 
     def lift_split(self, lifted):
         """Split lifted layout (anqi_lift_split): (lmain n x d, ltail n)."""

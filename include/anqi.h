@@ -138,6 +138,12 @@ int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk,
 /* host base (n*d), rk_horizon (n); host lifted (n*(d+2)). */
 int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
                    float* lifted, double* mstar2, double* scale);
+/* The same lift on the device (rknn_query.cpp:2203-2236): device base (n*d),
+ * rk_horizon (n), lifted (n*(d+2)).  Per-row binary64 work runs in integer
+ * soft binary64 (no FP64 instructions); the row-order sum of squared norms is
+ * taken on the host.  Same bits as anqi_lift_host. */
+int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
+              double* mstar2, double* scale);
 
 /* ---- graph construction pieces (include/index.h:1071-1295) ---------------- */
 int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,

@@ -191,6 +191,43 @@ def test_range_search_and_postfilter_on_reference_graph(ctx, S, mode, split):
     ctx.tree_free(tree)
 
 
+def _oracle_lift(base, rk):
+    n, d = base.shape
+    lifted = np.zeros((n, d + 2), np.float32)
+    m2, sc = ctypes.c_double(), ctypes.c_double()
+    oracle().or_lift(ptr(base), n, d, ptr(rk), ptr(lifted), ctypes.byref(m2), ctypes.byref(sc))
+    return lifted, m2.value, sc.value
+
+
+@pytest.mark.parametrize("case", ["sample", "mixture", "wide", "odd_dim"])
+def test_device_lift_matches_oracle(ctx, S, case):
+    """anqi_lift (per-row soft binary64 on the device, row-order sum on the
+    host) gives the oracle's lifted rows, M*^2 and scale bit for bit: the
+    reference's own sample (its lifted file), integer mixture rows, rows
+    spanning many binades (with zero rows), and an odd dimension."""
+    rng = np.random.default_rng(11)
+    if case == "sample":
+        base, rk = S.base, np.ascontiguousarray(S.rec_keys[:, 99])
+    elif case == "mixture":
+        base = workload.mixture(20000, 96, 50, 15.0, seed=3)
+        rk = rng.uniform(50.0, 5000.0, 20000).astype(np.float32)
+    elif case == "wide":
+        base = (rng.standard_normal((5000, 64)) * np.exp(rng.uniform(-12, 12, (5000, 1)))).astype(np.float32)
+        base[::97] = 0.0
+        rk = (np.exp(rng.uniform(-20, 20, 5000))).astype(np.float32)
+    else:
+        base = rng.uniform(0, 255, (3001, 37)).astype(np.float32)
+        rk = rng.uniform(0.0, 1e5, 3001).astype(np.float32)
+    base = np.ascontiguousarray(base, np.float32)
+    exp, m2, sc = _oracle_lift(base, rk)
+    if case == "sample":
+        assert np.array_equal(bits(exp), bits(S.lifted)) and m2 == S.M2 and sc == S.scale
+    say("device lift", case, base.shape)
+    got, gm2, gsc = ctx.lift(ctx.to_dev(base), ctx.to_dev(rk))
+    assert gm2 == m2 and gsc == sc
+    assert np.array_equal(bits(host(got)), bits(exp))
+
+
 def test_lift_split_layout(ctx, S):
     """lmain = the lifted rows' first d coordinates; ltail = the FP32 sum of
     the two rounded squares (1 - h0)^2 and h1^2 (the lift coordinates' term
