@@ -29,6 +29,10 @@ struct anqi_ctx {
     anqi_search_tuning tune{0, 0, 0};   // persistent search work areas / grid (0 = defaults)
     uint32_t* ent = nullptr;       // persistent search's batch: entries, keys, work order (3 x ent_cap)
     size_t ent_cap = 0;
+    // optional HIP-event timing of the range-search kernel launch itself
+    bool time_kernels = false;
+    hipEvent_t kev0 = nullptr, kev1 = nullptr;
+    double last_kernel_ms = 0.0;
 };
 
 static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
@@ -144,8 +148,22 @@ void anqi_ctx_destroy(anqi_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->counter) (void)hipFree(ctx->counter);
     if (ctx->ent) (void)hipFree(ctx->ent);
+    if (ctx->kev0) (void)hipEventDestroy(ctx->kev0);
+    if (ctx->kev1) (void)hipEventDestroy(ctx->kev1);
     delete ctx;
 }
+
+int anqi_ctx_time_kernels(anqi_ctx* ctx, int enable) {
+    if (enable && !ctx->kev0) {
+        HCHK(hipEventCreate(&ctx->kev0));
+        HCHK(hipEventCreate(&ctx->kev1));
+    }
+    ctx->time_kernels = enable != 0;
+    ctx->last_kernel_ms = 0.0;
+    return ANQI_OK;
+}
+
+double anqi_ctx_last_kernel_ms(const anqi_ctx* ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
 
 const char* anqi_last_error(const anqi_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
@@ -1273,13 +1291,21 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
         int occ = std::max(1, search_persistent_occupancy(P));
         uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
         uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
+        if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev0, s));
         HCHK(launch_range_search(P, true, grid, s));
     } else {
+        if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev0, s));
         HCHK(launch_range_search(P, false, 0, s));
     }
+    if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev1, s));
     std::vector<uint32_t> hst(nq);
     HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
+    if (ctx->time_kernels) {
+        float kms = 0.0f;
+        HCHK(hipEventElapsedTime(&kms, ctx->kev0, ctx->kev1));
+        ctx->last_kernel_ms = kms;
+    }
     std::vector<uint32_t> redo;
     for (uint32_t i = 0; i < nq; i++)
         if (hst[i] == 1 || hst[i] == 2) redo.push_back(i);

@@ -68,6 +68,8 @@ SIGS = {
     "anqi_ctx_set_stream": (I32, [P, P]),
     "anqi_ctx_stream": (P, [P]),
     "anqi_ctx_reset_stream": (I32, [P]),
+    "anqi_ctx_time_kernels": (I32, [P, I32]),
+    "anqi_ctx_last_kernel_ms": (F64, [P]),
     "anqi_ctx_set_search_tuning": (I32, [P, P]),
     "anqi_dmalloc": (I32, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "anqi_dfree": (I32, [P, P]),
@@ -183,6 +185,13 @@ class Context:
 
     def sync(self):
         self.call("anqi_ctx_sync")
+
+    def time_kernels(self, enable=True):
+        """HIP-event timing of each range-search kernel launch (anqi_ctx_time_kernels)."""
+        self.call("anqi_ctx_time_kernels", 1 if enable else 0)
+
+    def last_kernel_ms(self):
+        return float(self.L.anqi_ctx_last_kernel_ms(self.ctx))
 
     # ---- helpers returning torch tensors -------------------------------------
     def empty(self, shape, dtype):

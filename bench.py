@@ -227,20 +227,22 @@ def main():
     log("autotune", json.dumps(tuning))
 
     def timed(mode):
+        """Whole-call throughput (host clock around synchronised calls: the
+        batch's entry descent and ordering, the search, the status copy) and
+        the search kernel's own duration (HIP events the library records on
+        its stream around that launch)."""
         for _ in range(a.warmup):
             ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=mode, bufs=bufs)
         torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ctx.time_kernels(True)
         kms = 0.0
         t0 = time.perf_counter()
         for _ in range(a.steps):
-            ev0.record()
             ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=mode, bufs=bufs)
-            ev1.record()
-            ev1.synchronize()
-            kms += ev0.elapsed_time(ev1)
+            kms += ctx.last_kernel_ms()
         torch.cuda.synchronize()
         secs = time.perf_counter() - t0
+        ctx.time_kernels(False)
         return a.nq * a.steps / secs, secs, kms / a.steps
 
     qps_c, secs_c, launch_c = timed(1)

@@ -43,6 +43,12 @@ int anqi_ctx_sync(anqi_ctx* ctx);
  * default (null) stream.  anqi_ctx_reset_stream restores the context's own. */
 int anqi_ctx_set_stream(anqi_ctx* ctx, void* stream);
 int anqi_ctx_reset_stream(anqi_ctx* ctx);
+/* Timing of the range-search kernel launch itself (HIP events recorded on
+ * the context's stream around it, excluding the batch's entry descent,
+ * ordering and result copies): enable, then read the last call's
+ * milliseconds.  No counterpart in the reference (measurement support). */
+int anqi_ctx_time_kernels(anqi_ctx* ctx, int enable);
+double anqi_ctx_last_kernel_ms(const anqi_ctx* ctx);
 void* anqi_ctx_stream(anqi_ctx* ctx);
 /* Work areas and grid of the persistent search (mode 1 of anqi_rknn_search
  * and the batcher); anqi_search_autotune measures candidates.  Zero fields
