@@ -604,8 +604,10 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     HCHK(hipMemsetAsync(status.p, 0, 4ull * P.n, s));
     P.status = status.as<uint32_t>();
     uint64_t need = (uint64_t)P.pool_cap + P.cap + maxrev;   // visits / pool entries bound
-    // fast path: LDS work area
+    // fast path: LDS work area.  The register-list AKNN gather keeps its
+    // visited set in 16-bit entries (half the LDS, so more rows per CU).
     P.hmask = std::min<uint32_t>(graph ? 4096 : 8192, next_pow2(need * 3 / 2 + 64)) - 1;
+    P.h16 = (!graph && gather_register_lists(P, graph) && P.n <= H16_MAX_ID && P.hmask + 1 == H16_SLOTS) ? 1u : 0u;
     P.pmax = graph ? std::min<uint32_t>(2048, next_pow2(need + 64)) : 0;
     P.rows = nullptr;
     P.nrows = P.n;
@@ -617,6 +619,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     if (bad.empty()) return ANQI_OK;
     // slow path: global work area sized for the bound
     GatherParams Q = P;
+    Q.h16 = 0;
     Q.hmask = next_pow2(need * 2 + 64) - 1;
     Q.pmax = graph ? next_pow2(need + 64) : 0;
     size_t words = (gather_lds_bytes(Q, graph, false) + 3) / 4;

@@ -441,6 +441,52 @@ __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_
     }
     return -1;
 }
+// ---- compact LDS id set: 16-bit entries, ids < 2^20 -----------------------------
+// H16_SLOTS entries in H16_BUCKETS buckets of four (8 bytes).  h = mix20(id)
+// is a bijection of the 20-bit ids; its top 11 bits pick the home bucket and
+// an entry stores (probe distance << 9) | (low 9 bits of h): with the bucket
+// it sits in that gives back h, hence the id, so the set is exact.  Probe
+// distances 0..126; 0xFFFF (distance 127) marks an empty entry.  Probing as
+// in hset_insert: a key lives in the first bucket of its sequence that had
+// room, so a bucket with an empty entry ends a lookup.  A bucket is updated
+// with one 64-bit CAS; a lane whose CAS loses re-reads the bucket.
+#define H16_BUCKETS 2048u
+#define H16_SLOTS (4u * H16_BUCKETS)
+#define H16_EMPTY 0xFFFFu
+#define H16_MAX_ID (1u << 20)
+__host__ __device__ __forceinline__ uint32_t mix20(uint32_t id) {
+    uint32_t h = (id * 0x9E3779B1u) & 0xFFFFFu;
+    h ^= h >> 10;
+    return (h * 0x2C1B3C6Du) & 0xFFFFFu;
+}
+// Returns 1 if id was inserted (first visit), 0 if present, -1 if its probe
+// sequence is full.
+__device__ __forceinline__ int hset16_insert(uint16_t* tab, uint32_t id) {
+    const uint32_t h = mix20(id);
+    const uint32_t home = h >> 9, rem = h & 0x1FFu;
+    for (uint32_t dist = 0; dist < 127u; dist++) {
+        const uint32_t code = (dist << 9) | rem;
+        unsigned long long* bk = (unsigned long long*)(tab + 4u * ((home + dist) & (H16_BUCKETS - 1u)));
+        unsigned long long v = *bk;
+        for (;;) {
+            int fe = -1;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t x = (uint32_t)(v >> (16 * e)) & 0xFFFFu;
+                if (x == code) return 0;
+                if (x == H16_EMPTY && fe < 0) fe = e;
+            }
+            if (fe < 0) break;                                   // full: next distance
+            const unsigned long long nv =
+                (v & ~(0xFFFFull << (16 * fe))) | ((unsigned long long)code << (16 * fe));
+            const unsigned long long old = atomicCAS(bk, v, nv);
+            if (old == v) return 1;
+            v = old;
+        }
+    }
+    return -1;
+}
+
 // First key of an empty table (the search's entry point).
 __device__ __forceinline__ void hset_first(uint32_t* tab, uint32_t mask, uint32_t id) {
     tab[4u * hbucket(id, (mask + 1u) >> 2)] = id + 1u;

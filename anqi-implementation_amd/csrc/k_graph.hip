@@ -489,9 +489,13 @@ __host__ __device__ __forceinline__ bool gather_rk(const GatherParams& P, bool g
     return P.batched && P.explore <= 640 && (graph || P.K <= 128);
 }
 
+__host__ __device__ __forceinline__ size_t gather_hash_bytes(const GatherParams& P) {
+    return P.h16 ? 2 * (size_t)H16_SLOTS : 4 * (size_t)(P.hmask + 1);
+}
+
 __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P, bool graph, bool rk) {
     // hash + queue; in graph mode also the selected vectors of the prune step
-    size_t a = rnd16(4 * (size_t)(P.hmask + 1)) + (rk ? rnd16(8 * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
+    size_t a = rnd16(gather_hash_bytes(P)) + (rk ? rnd16(8 * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
     size_t b = graph ? rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) : 0;
     return a > b ? a : b;
 }
@@ -500,6 +504,8 @@ __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P,
 // processed in several lists.
 #define GX_CAP 256
 __host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P) { return P.batched ? GX_CAP : 64; }
+
+bool gather_register_lists(const GatherParams& P, bool graph) { return gather_rk(P, graph); }
 
 size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
     size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(8 * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
@@ -512,7 +518,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
     GatherLds s;
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base + o; o += rnd16(bytes); return p; };
-    s.hash = (uint32_t*)take(4 * (size_t)(P.hmask + 1));
+    s.hash = (uint32_t*)take(gather_hash_bytes(P));
     if (rk) {
         s.qd0 = (float*)take(8 * (size_t)P.explore);
         s.qd1 = nullptr; s.qp0 = nullptr; s.qp1 = nullptr;
@@ -617,7 +623,12 @@ template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
 __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, uint32_t i) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const int d = (int)P.d;
-    for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    if (P.h16) {
+        for (uint32_t j = lane; j < H16_SLOTS / 8u; j += 64)
+            ((uint4*)S.hash)[j] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    } else {
+        for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    }
     __syncthreads();
     QRow QI;
     qrow_load(QI, P.x + (size_t)i * d, d, l);
@@ -642,7 +653,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     auto visit = [&](uint32_t id, int valid, float seed_d) __attribute__((always_inline)) -> int {
         int fresh = 0;
         if (valid) {
-            int r = hset_insert(S.hash, P.hmask, id);
+            int r = P.h16 ? hset16_insert((uint16_t*)S.hash, id) : hset_insert(S.hash, P.hmask, id);
             if (r < 0) status = 1;
             fresh = (r == 1);
         }

@@ -146,8 +146,10 @@ struct GatherParams {
     uint32_t* gscratch;     // optional global work area (per block)
     size_t gscratch_words;  // words per block
     uint32_t batched;       // 1: one expansion's first visits processed together
+    uint32_t h16;           // 1: 16-bit LDS visited set of H16_SLOTS entries (n <= 2^20; hmask unused)
 };
 // rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
+bool gather_register_lists(const GatherParams& P, bool graph_mode);
 size_t gather_lds_bytes(const GatherParams& P, bool graph_mode, bool rk);
 hipError_t launch_gather(const GatherParams& P, bool graph_mode, bool global_ws, hipStream_t s);
 
