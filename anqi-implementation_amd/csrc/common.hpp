@@ -442,18 +442,19 @@ __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_
     return -1;
 }
 // ---- compact LDS id set: 16-bit entries, ids < 2^20 -----------------------------
-// H16_SLOTS entries in H16_BUCKETS buckets of four (8 bytes).  h = mix20(id)
-// is a bijection of the 20-bit ids; its top 11 bits pick the home bucket and
-// an entry stores (probe distance << 9) | (low 9 bits of h): with the bucket
-// it sits in that gives back h, hence the id, so the set is exact.  Probe
-// distances 0..126; 0xFFFF (distance 127) marks an empty entry.  Probing as
-// in hset_insert: a key lives in the first bucket of its sequence that had
-// room, so a bucket with an empty entry ends a lookup.  A bucket is updated
-// with one 64-bit CAS; a lane whose CAS loses re-reads the bucket.
-#define H16_BUCKETS 2048u
-#define H16_SLOTS (4u * H16_BUCKETS)
+// 4 << lb entries in 1 << lb buckets of four (8 bytes), lb = 8..11.  h =
+// mix20(id) is a bijection of the 20-bit ids; its top lb bits pick the home
+// bucket and an entry stores (probe distance << (20 - lb)) | (low 20 - lb bits
+// of h): with the bucket it sits in that gives back h, hence the id, so the
+// set is exact.  Probe distances 0 .. 2^(lb - 4) - 2; the all-ones entry
+// 0xFFFF (the next distance) marks an empty slot.  Probing as in hset_insert:
+// a key lives in the first bucket of its sequence that had room, so a bucket
+// with an empty entry ends a lookup.  A bucket is updated with one 64-bit
+// CAS; a lane whose CAS loses re-reads the bucket.
 #define H16_EMPTY 0xFFFFu
 #define H16_MAX_ID (1u << 20)
+#define H16_LB_GATHER 11u
+#define H16_SLOTS (4u << H16_LB_GATHER)
 __host__ __device__ __forceinline__ uint32_t mix20(uint32_t id) {
     uint32_t h = (id * 0x9E3779B1u) & 0xFFFFFu;
     h ^= h >> 10;
@@ -461,12 +462,13 @@ __host__ __device__ __forceinline__ uint32_t mix20(uint32_t id) {
 }
 // Returns 1 if id was inserted (first visit), 0 if present, -1 if its probe
 // sequence is full.
-__device__ __forceinline__ int hset16_insert(uint16_t* tab, uint32_t id) {
-    const uint32_t h = mix20(id);
-    const uint32_t home = h >> 9, rem = h & 0x1FFu;
-    for (uint32_t dist = 0; dist < 127u; dist++) {
-        const uint32_t code = (dist << 9) | rem;
-        unsigned long long* bk = (unsigned long long*)(tab + 4u * ((home + dist) & (H16_BUCKETS - 1u)));
+__device__ __forceinline__ int hset16_insert(uint16_t* tab, uint32_t lb, uint32_t id) {
+    const uint32_t rb = 20u - lb, h = mix20(id);
+    const uint32_t home = h >> rb, rem = h & ((1u << rb) - 1u);
+    const uint32_t maxd = (1u << (16u - rb)) - 1u, bmask = (1u << lb) - 1u;
+    for (uint32_t dist = 0; dist < maxd; dist++) {
+        const uint32_t code = (dist << rb) | rem;
+        unsigned long long* bk = (unsigned long long*)(tab + 4u * ((home + dist) & bmask));
         unsigned long long v = *bk;
         for (;;) {
             int fe = -1;
@@ -486,7 +488,6 @@ __device__ __forceinline__ int hset16_insert(uint16_t* tab, uint32_t id) {
     }
     return -1;
 }
-
 // First key of an empty table (the search's entry point).
 __device__ __forceinline__ void hset_first(uint32_t* tab, uint32_t mask, uint32_t id) {
     tab[4u * hbucket(id, (mask + 1u) >> 2)] = id + 1u;

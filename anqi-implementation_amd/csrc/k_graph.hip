@@ -501,15 +501,19 @@ __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P,
 }
 
 // Fresh-neighbour list of one expansion (batched form); a longer expansion is
-// processed in several lists.
-#define GX_CAP 256
-__host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P) { return P.batched ? GX_CAP : 64; }
+// processed in several lists.  The AKNN gather's is shorter: its LDS area then
+// fits seven rows per CU.
+#define GX_CAP_GRAPH 256
+#define GX_CAP_AKNN 128
+__host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P, bool graph) {
+    return P.batched ? (graph ? GX_CAP_GRAPH : GX_CAP_AKNN) : 64;
+}
 
 bool gather_register_lists(const GatherParams& P, bool graph) { return gather_rk(P, graph); }
 
 size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
     size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(8 * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
-    return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P)) + (rk ? 0 : rnd16(8 * 64));
+    return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P, graph)) + (rk ? 0 : rnd16(8 * 64));
 }
 
 // rk: register lists; the queue's exchange area is qd0 (8 explore bytes), the
@@ -545,8 +549,8 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
         s.pd2 = (float*)take(4 * (size_t)P.K);
         s.pi2 = (uint32_t*)take(4 * (size_t)P.K);
     }
-    s.tid = (uint32_t*)take(4 * (size_t)gather_tcap(P));
-    s.tdv = (float*)take(4 * (size_t)gather_tcap(P));
+    s.tid = (uint32_t*)take(4 * (size_t)gather_tcap(P, graph));
+    s.tdv = (float*)take(4 * (size_t)gather_tcap(P, graph));
     s.cs = rk ? nullptr : (uint64_t*)take(8 * 64);
     return s;
 }
@@ -653,7 +657,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     auto visit = [&](uint32_t id, int valid, float seed_d) __attribute__((always_inline)) -> int {
         int fresh = 0;
         if (valid) {
-            int r = P.h16 ? hset16_insert((uint16_t*)S.hash, id) : hset_insert(S.hash, P.hmask, id);
+            int r = P.h16 ? hset16_insert((uint16_t*)S.hash, H16_LB_GATHER, id) : hset_insert(S.hash, P.hmask, id);
             if (r < 0) status = 1;
             fresh = (r == 1);
         }
@@ -732,7 +736,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         }
         return 1;
     };
-    const uint32_t tcap = gather_tcap(P);
+    const uint32_t tcap = gather_tcap(P, GRAPH);
 
     // seeds: the row's own slots in storage order, with stored values
     for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
