@@ -368,6 +368,84 @@ __device__ __forceinline__ uint32_t rk_merge(RKeys<NCH>& B, uint64_t key, uint32
     return kc;
 }
 
+// rk_merge with a 32-bit exchange area (cap words, half the LDS): the keys
+// move in two passes, high words then low words.  The positions are those of
+// rk_merge, so the list is the same.  Between the passes a lane's register
+// holds the new high word and the old low word it still has to move.
+template <int NCH>
+__device__ __forceinline__ uint32_t rk_merge_x32(RKeys<NCH>& B, uint64_t key, uint32_t& size, uint32_t cap,
+                                                 uint32_t* kb) {
+    const int lane = __lane_id();
+    if (size == cap && !(key < rk_get(B, cap - 1))) key = ~0ull;
+    const uint64_t km = ballot(key != ~0ull);
+    if (km == 0) return 0;
+    uint32_t sc[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; j++) sc[j] = 0;
+    uint32_t mypos = U32_NONE;
+    for (uint64_t mm = km; mm; mm &= mm - 1) {
+        const int src = __ffsll((long long)mm) - 1;
+        const uint64_t cv = readlane64(key, src);
+        uint32_t pos = (uint32_t)popc64(ballot(key < cv));
+#pragma unroll
+        for (int j = 0; j < NCH; j++) {
+            pos += (uint32_t)popc64(ballot(B.k[j] < cv));
+            sc[j] += (cv < B.k[j]) ? 1u : 0u;
+        }
+        if (lane == src) mypos = pos;
+    }
+    const uint32_t kc = (uint32_t)popc64(km);
+    const uint32_t nsize = min(cap, size + kc);
+    // the old entries (ov bit j: present) with their expanded flags (bit 31
+    // of the id word)
+    uint32_t ov = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        if (B.k[j] != ~0ull) {
+            ov |= 1u << j;
+            if ((B.x >> j) & 1u) B.k[j] |= RK_FLAG;
+        }
+    }
+    // pass 1: high words
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t np = 64u * j + (uint32_t)lane + sc[j];
+        if (((ov >> j) & 1u) && np < cap) kb[np] = (uint32_t)(B.k[j] >> 32);
+    }
+    if (mypos < cap) kb[mypos] = (uint32_t)(key >> 32);
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t pos = 64u * j + (uint32_t)lane;
+        const uint32_t hi = pos < nsize ? kb[pos] : ~0u;
+        B.k[j] = ((uint64_t)hi << 32) | (uint32_t)B.k[j];   // new high word, old low word
+    }
+    wave_lds_sync();
+    // pass 2: low words
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t np = 64u * j + (uint32_t)lane + sc[j];
+        if (((ov >> j) & 1u) && np < cap) kb[np] = (uint32_t)B.k[j];
+    }
+    if (mypos < cap) kb[mypos] = (uint32_t)key;
+    wave_lds_sync();
+    B.x = 0;
+#pragma unroll
+    for (int j = 0; j < NCH; j++) {
+        const uint32_t pos = 64u * j + (uint32_t)lane;
+        uint64_t v = ~0ull;
+        if (pos < nsize) {
+            v = (B.k[j] & 0xFFFFFFFF00000000ull) | kb[pos];
+            if (v & RK_FLAG) B.x |= 1u << j;
+            v &= ~RK_FLAG;
+        }
+        B.k[j] = v;
+    }
+    size = nsize;
+    wave_lds_sync();
+    return kc;
+}
+
 // Id of the first unexpanded entry (U32_NONE if none); mark: flag it expanded.
 template <int NCH>
 __device__ __forceinline__ uint32_t rk_take(RKeys<NCH>& B, bool mark) {

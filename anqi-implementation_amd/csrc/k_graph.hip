@@ -493,9 +493,14 @@ __host__ __device__ __forceinline__ size_t gather_hash_bytes(const GatherParams&
     return P.h16 ? 2 * (size_t)H16_SLOTS : 4 * (size_t)(P.hmask + 1);
 }
 
+// Bytes per entry of a register list's LDS exchange area: the AKNN gather's
+// lists move their keys as two 32-bit halves (rk_merge_x32), which halves the
+// area and fits one more row per CU.
+__host__ __device__ __forceinline__ size_t gather_xbytes(bool graph) { return graph ? 8 : 4; }
+
 __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P, bool graph, bool rk) {
     // hash + queue; in graph mode also the selected vectors of the prune step
-    size_t a = rnd16(gather_hash_bytes(P)) + (rk ? rnd16(8 * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
+    size_t a = rnd16(gather_hash_bytes(P)) + (rk ? rnd16(gather_xbytes(graph) * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
     size_t b = graph ? rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) : 0;
     return a > b ? a : b;
 }
@@ -512,19 +517,19 @@ __host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P, 
 bool gather_register_lists(const GatherParams& P, bool graph) { return gather_rk(P, graph); }
 
 size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
-    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(8 * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
+    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(gather_xbytes(graph) * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
     return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P, graph)) + (rk ? 0 : rnd16(8 * 64));
 }
 
-// rk: register lists; the queue's exchange area is qd0 (8 explore bytes), the
-// top-K's is pd (8 K bytes); the other list arrays are absent.
+// rk: register lists; the queue's exchange area is qd0 (explore entries), the
+// top-K's is pd (K entries), gather_xbytes each; the other list arrays are absent.
 __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams& P, bool graph, bool rk) {
     GatherLds s;
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base + o; o += rnd16(bytes); return p; };
     s.hash = (uint32_t*)take(gather_hash_bytes(P));
     if (rk) {
-        s.qd0 = (float*)take(8 * (size_t)P.explore);
+        s.qd0 = (float*)take(gather_xbytes(graph) * (size_t)P.explore);
         s.qd1 = nullptr; s.qp0 = nullptr; s.qp1 = nullptr;
     } else {
         s.qd0 = (float*)take(4 * (size_t)P.explore);
@@ -541,7 +546,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
         s.pd2 = nullptr;
         s.pi2 = nullptr;
     } else if (rk) {
-        s.pd = (float*)take(8 * (size_t)P.K);
+        s.pd = (float*)take(gather_xbytes(graph) * (size_t)P.K);
         s.pi = nullptr; s.pd2 = nullptr; s.pi2 = nullptr;
     } else {
         s.pd = (float*)take(4 * (size_t)P.K);
@@ -696,7 +701,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
                 if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
                 psize += (uint32_t)k;
             } else if constexpr (PCH > 0) {
-                rk_merge(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint64_t*)S.pd);
+                rk_merge_x32(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint32_t*)S.pd);
             } else {
                 uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
                 float* ad = pb ? S.pd2 : S.pd;
@@ -715,7 +720,8 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
             pcount += (uint32_t)k;
             // queue
             if constexpr (QCH > 0) {
-                rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
+                if constexpr (GRAPH) rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
+                else rk_merge_x32(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint32_t*)S.qd0);
             } else {
                 uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
                 float* ad = S.qd(qb);
