@@ -606,9 +606,12 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     uint64_t need = (uint64_t)P.pool_cap + P.cap + maxrev;   // visits / pool entries bound
     // fast path: LDS work area.  The register-list AKNN gather keeps its
     // visited set in 16-bit entries (half the LDS, so more rows per CU).
+    // The register-list graph gather's pool holds 1024 entries (four rows per
+    // CU instead of three); rows that outgrow it are redone on the slow path.
+    const bool rk = gather_register_lists(P, graph);
     P.hmask = std::min<uint32_t>(graph ? 4096 : 8192, next_pow2(need * 3 / 2 + 64)) - 1;
-    P.h16 = (!graph && gather_register_lists(P, graph) && P.n <= H16_MAX_ID && P.hmask + 1 == H16_SLOTS) ? 1u : 0u;
-    P.pmax = graph ? std::min<uint32_t>(2048, next_pow2(need + 64)) : 0;
+    P.h16 = (!graph && rk && P.n <= H16_MAX_ID && P.hmask + 1 == H16_SLOTS) ? 1u : 0u;
+    P.pmax = graph ? std::min<uint32_t>(rk ? 1024u : 2048u, next_pow2(need + 64)) : 0;
     P.rows = nullptr;
     P.nrows = P.n;
     HCHK(launch_gather(P, graph, false, s));

@@ -493,31 +493,33 @@ __host__ __device__ __forceinline__ size_t gather_hash_bytes(const GatherParams&
     return P.h16 ? 2 * (size_t)H16_SLOTS : 4 * (size_t)(P.hmask + 1);
 }
 
-// Bytes per entry of a register list's LDS exchange area: the AKNN gather's
-// lists move their keys as two 32-bit halves (rk_merge_x32), which halves the
-// area and fits one more row per CU.
-__host__ __device__ __forceinline__ size_t gather_xbytes(bool graph) { return graph ? 8 : 4; }
+// Bytes per entry of a register list's LDS exchange area.  With the 16-bit
+// visited set the AKNN gather's lists move their keys as two 32-bit halves
+// (rk_merge_x32), which halves the area and fits an eighth row per CU.
+__host__ __device__ __forceinline__ size_t gather_xbytes(const GatherParams& P, bool graph) {
+    return (!graph && P.h16) ? 4 : 8;
+}
 
 __host__ __device__ __forceinline__ size_t gather_region0(const GatherParams& P, bool graph, bool rk) {
     // hash + queue; in graph mode also the selected vectors of the prune step
-    size_t a = rnd16(gather_hash_bytes(P)) + (rk ? rnd16(gather_xbytes(graph) * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
+    size_t a = rnd16(gather_hash_bytes(P)) + (rk ? rnd16(gather_xbytes(P, graph) * (size_t)P.explore) : 4 * rnd16(4 * (size_t)P.explore));
     size_t b = graph ? rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) : 0;
     return a > b ? a : b;
 }
 
 // Fresh-neighbour list of one expansion (batched form); a longer expansion is
-// processed in several lists.  The AKNN gather's is shorter: its LDS area then
-// fits seven rows per CU.
+// processed in several lists.  The AKNN gather's is shorter with the 16-bit
+// visited set (its LDS area then fits more rows per CU).
 #define GX_CAP_GRAPH 256
 #define GX_CAP_AKNN 128
 __host__ __device__ __forceinline__ uint32_t gather_tcap(const GatherParams& P, bool graph) {
-    return P.batched ? (graph ? GX_CAP_GRAPH : GX_CAP_AKNN) : 64;
+    return P.batched ? ((graph || !P.h16) ? GX_CAP_GRAPH : GX_CAP_AKNN) : 64;
 }
 
 bool gather_register_lists(const GatherParams& P, bool graph) { return gather_rk(P, graph); }
 
 size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
-    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(gather_xbytes(graph) * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
+    size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(gather_xbytes(P, graph) * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
     return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P, graph)) + (rk ? 0 : rnd16(8 * 64));
 }
 
@@ -529,7 +531,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
     auto take = [&](size_t bytes) { char* p = base + o; o += rnd16(bytes); return p; };
     s.hash = (uint32_t*)take(gather_hash_bytes(P));
     if (rk) {
-        s.qd0 = (float*)take(gather_xbytes(graph) * (size_t)P.explore);
+        s.qd0 = (float*)take(gather_xbytes(P, graph) * (size_t)P.explore);
         s.qd1 = nullptr; s.qp0 = nullptr; s.qp1 = nullptr;
     } else {
         s.qd0 = (float*)take(4 * (size_t)P.explore);
@@ -546,7 +548,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
         s.pd2 = nullptr;
         s.pi2 = nullptr;
     } else if (rk) {
-        s.pd = (float*)take(gather_xbytes(graph) * (size_t)P.K);
+        s.pd = (float*)take(gather_xbytes(P, graph) * (size_t)P.K);
         s.pi = nullptr; s.pd2 = nullptr; s.pi2 = nullptr;
     } else {
         s.pd = (float*)take(4 * (size_t)P.K);
@@ -628,11 +630,12 @@ __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uin
 // One row of the gather.  QCH > 0: the beam queue is a register list of
 // QCH chunks (explore <= 64 QCH); PCH > 0: so is the AKNN top-K (K <= 64 PCH).
 // Otherwise both live sorted in LDS (double buffered, binary-search merges).
-template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
+// H16: the 16-bit visited set, with the AKNN lists' 32-bit exchange (P.h16).
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, bool H16>
 __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, uint32_t i) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const int d = (int)P.d;
-    if (P.h16) {
+    if constexpr (H16) {
         for (uint32_t j = lane; j < H16_SLOTS / 8u; j += 64)
             ((uint4*)S.hash)[j] = make_uint4(~0u, ~0u, ~0u, ~0u);
     } else {
@@ -662,7 +665,9 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     auto visit = [&](uint32_t id, int valid, float seed_d) __attribute__((always_inline)) -> int {
         int fresh = 0;
         if (valid) {
-            int r = P.h16 ? hset16_insert((uint16_t*)S.hash, H16_LB_GATHER, id) : hset_insert(S.hash, P.hmask, id);
+            int r;
+            if constexpr (H16) r = hset16_insert((uint16_t*)S.hash, H16_LB_GATHER, id);
+            else r = hset_insert(S.hash, P.hmask, id);
             if (r < 0) status = 1;
             fresh = (r == 1);
         }
@@ -701,7 +706,8 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
                 if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
                 psize += (uint32_t)k;
             } else if constexpr (PCH > 0) {
-                rk_merge_x32(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint32_t*)S.pd);
+                if constexpr (H16) rk_merge_x32(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint32_t*)S.pd);
+                else rk_merge(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint64_t*)S.pd);
             } else {
                 uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
                 float* ad = pb ? S.pd2 : S.pd;
@@ -721,7 +727,8 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
             // queue
             if constexpr (QCH > 0) {
                 if constexpr (GRAPH) rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
-                else rk_merge_x32(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint32_t*)S.qd0);
+                else if constexpr (H16) rk_merge_x32(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint32_t*)S.qd0);
+                else rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
             } else {
                 uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
                 float* ad = S.qd(qb);
@@ -879,7 +886,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
 }
 
-template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, bool H16>
 __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     extern __shared__ __align__(16) char lds_raw[];
     uint32_t bi = blockIdx.x;
@@ -887,14 +894,14 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     uint32_t i = P.rows ? P.rows[bi] : bi;
     char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
     GatherLds S = gather_carve(base, P, GRAPH, QCH > 0);
-    gather_row<GRAPH, GWS, MC, QCH, PCH>(P, S, i);
+    gather_row<GRAPH, GWS, MC, QCH, PCH, H16>(P, S, i);
 }
 
-template <bool GRAPH, bool GWS, int MC, int QCH, int PCH>
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, bool H16 = false>
 static hipError_t launch_gather_t(const GatherParams& P, size_t lds, hipStream_t s) {
-    const void* f = (const void*)k_gather<GRAPH, GWS, MC, QCH, PCH>;
+    const void* f = (const void*)k_gather<GRAPH, GWS, MC, QCH, PCH, H16>;
     if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC, QCH, PCH>), dim3(P.nrows), dim3(64), lds, s, P);
+    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC, QCH, PCH, H16>), dim3(P.nrows), dim3(64), lds, s, P);
     return hipGetLastError();
 }
 
@@ -903,6 +910,10 @@ static hipError_t launch_gather_lists(const GatherParams& P, size_t lds, hipStre
     if constexpr (!GWS) {
         if (gather_rk(P, GRAPH)) {
             constexpr int PC = GRAPH ? 0 : 2;
+            if (!GRAPH && P.h16) {
+                if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC, true>(P, lds, s);
+                return launch_gather_t<GRAPH, GWS, MC, 10, PC, true>(P, lds, s);
+            }
             if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC>(P, lds, s);
             return launch_gather_t<GRAPH, GWS, MC, 10, PC>(P, lds, s);
         }
@@ -932,7 +943,8 @@ size_t commit_lds_bytes(const PruneParams& P) {
            rnd16(4 * 64);
 }
 
-template <bool GWS>
+// MC = (d >> 4) for the batched distance pass (0: one row per group at a time).
+template <bool GWS, int MC>
 __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
     extern __shared__ __align__(16) char lds_raw[];
     uint32_t bi = blockIdx.x;
@@ -956,10 +968,15 @@ __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
     QRow QI;
     qrow_load(QI, P.x + (size_t)i * d, d, l);
     float ni = P.nrm[i];
-    for (uint32_t s = g; s < m; s += 4) {
-        uint32_t j = pi[s];
-        float ip = qrow_ip(QI, P.x + (size_t)j * d, d, l);
-        if (l == 0) pd[s] = d2_combine(ip, ni, P.nrm[j]);
+    if constexpr (MC > 0) {
+        // four rows per group in flight (same FMA order and tree as qrow_ip)
+        rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, pi, (int)m, l, g, pd);
+    } else {
+        for (uint32_t s = g; s < m; s += 4) {
+            uint32_t j = pi[s];
+            float ip = qrow_ip(QI, P.x + (size_t)j * d, d, l);
+            if (l == 0) pd[s] = d2_combine(ip, ni, P.nrm[j]);
+        }
     }
     __syncthreads();
     lds_sort_pairs(pd, pi, m, cap2);
@@ -982,14 +999,23 @@ __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
     if (lane == 0) { P.deg[i] = dg; P.status[i] = 0; }
 }
 
-hipError_t launch_commit(const PruneParams& P, bool gws, hipStream_t s) {
-    if (P.nrows == 0) return hipSuccess;
+template <int MC>
+static hipError_t launch_commit_mc(const PruneParams& P, bool gws, hipStream_t s) {
     size_t lds = gws ? 0 : commit_lds_bytes(P);
     if (gws) {
-        hipLaunchKernelGGL((k_commit<true>), dim3(P.nrows), dim3(64), 0, s, P);
+        hipLaunchKernelGGL((k_commit<true, MC>), dim3(P.nrows), dim3(64), 0, s, P);
     } else {
-        hipFuncSetAttribute((const void*)k_commit<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((k_commit<false>), dim3(P.nrows), dim3(64), lds, s, P);
+        hipFuncSetAttribute((const void*)k_commit<false, MC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((k_commit<false, MC>), dim3(P.nrows), dim3(64), lds, s, P);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_commit(const PruneParams& P, bool gws, hipStream_t s) {
+    if (P.nrows == 0) return hipSuccess;
+    switch (P.d >> 4) {
+        case 8: return launch_commit_mc<8>(P, gws, s);
+        case 6: return launch_commit_mc<6>(P, gws, s);
+        default: return launch_commit_mc<0>(P, gws, s);
+    }
 }
