@@ -548,23 +548,14 @@ __global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_
 // late long query.  A counting sort over ORD_BUCKETS buckets of width
 // R2 / 64 (the last bucket takes every key from 4 R2 on); order within a
 // bucket is arbitrary.  One workgroup.  Every query's results are its own,
-// so the order changes only the timing.  With per-query beam widths (Lq) the
-// beam length matters as much: 16 L classes, widest first, each split into
-// 16 entry-distance buckets of width R2 / 4.
+// so the order changes only the timing.
 #define ORD_BUCKETS 256
 __global__ void __launch_bounds__(1024) k_entry_order(const float* __restrict__ key, const uint32_t* nq_dev,
-                                                      uint32_t nq_host, float inv_w, const uint32_t* __restrict__ Lq,
-                                                      uint32_t Lmax, uint32_t* __restrict__ qlist) {
+                                                      uint32_t nq_host, float inv_w, uint32_t* __restrict__ qlist) {
     __shared__ uint32_t cnt[ORD_BUCKETS];
     const uint32_t nq = nq_dev ? *nq_dev : nq_host;
     const uint32_t t = threadIdx.x;
     auto bucket = [&](uint32_t q) -> uint32_t {
-        if (Lq) {
-            const uint32_t L = min(max(Lq[q], 1u), Lmax);
-            const uint32_t lb = min(15u, (L - 1u) * 16u / Lmax);
-            const float b = fminf(key[q] * inv_w * 0.0625f, 15.0f);
-            return (15u - lb) * 16u + (b >= 0.0f ? (uint32_t)b : 0u);
-        }
         const float b = fminf(key[q] * inv_w, (float)(ORD_BUCKETS - 1));   // NaN -> last bucket
         return b >= 0.0f ? (uint32_t)b : 0u;
     };
@@ -589,8 +580,7 @@ hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint3
 
 hipError_t launch_entry_order(const SearchParams& P, const float* key, uint32_t* qlist, hipStream_t s) {
     const float inv_w = 64.0f / fmaxf(P.R2f, 1e-30f);
-    hipLaunchKernelGGL(k_entry_order, dim3(1), dim3(1024), 0, s, key, P.nq_dev, P.nq, inv_w, P.Lq,
-                       max(P.Lmax, 1u), qlist);
+    hipLaunchKernelGGL(k_entry_order, dim3(1), dim3(1024), 0, s, key, P.nq_dev, P.nq, inv_w, qlist);
     return hipGetLastError();
 }
 
