@@ -358,6 +358,47 @@ def test_gather_topk_matches_oracle(ctx, S, form):
         assert np.array_equal(bits(keys[i, :c]), bits(rk[:c]))
 
 
+def test_gather_topk_16bit_visited_set_matches_oracle(ctx):
+    """The chosen AKNN gather with the build's parameters (queue 600, pool 4800,
+    K 100) keeps its visited set in 16-bit entries (ids < 2^20): on 20,000
+    mixture rows the set fills to ~60 % per row, so keys sit displaced from
+    their home buckets; every checked row equals the oracle's gather."""
+    O = oracle()
+    x = workload.mixture(20000, 96, 60, 15.0, seed=9)
+    n, d = x.shape
+    nrm = np.zeros(n, np.float32)
+    O.or_norms(ptr(x), n, d, ptr(nrm))
+    # seed lists: 40 random other rows per row with their squared distances
+    # (both sides read the same arrays; slots past 40 are empty)
+    cap = 100
+    rng = np.random.default_rng(4)
+    nb = rng.integers(0, n, (n, 40)).astype(np.uint32)
+    rows = np.arange(n, dtype=np.uint32)[:, None]
+    nb = np.where(nb == rows, (nb + 1) % n, nb).astype(np.uint32)
+    hi = np.full((n, cap), U32MAX, np.uint32)
+    hv = np.full((n, cap), np.finfo(np.float32).max, np.float32)
+    hi[:, :40] = nb
+    hv[:, :40] = ((x[:, None, :] - x[nb]) ** 2).sum(-1).astype(np.float32)
+    roff = np.zeros(n + 1, np.uint32)
+    rids = np.zeros(n * cap, np.uint32)
+    O.or_reverse_csr(ptr(hi), n, cap, ptr(roff), ptr(rids))
+    say("gather topk, 16-bit visited set")
+    ids, keys, cnt = ctx.gather_topk(ctx.to_dev(x), ctx.to_dev(nrm), ctx.to_dev(hi.view(np.int32)), ctx.to_dev(hv),
+                                     ctx.to_dev(roff.view(np.int32)), ctx.to_dev(rids.view(np.int32)), 600, 4800, 100,
+                                     form=1)
+    ids, keys, cnt = host(ids).view(np.uint32), host(keys), host(cnt)
+    pd = np.zeros(16384, np.float32)
+    pi = np.zeros(16384, np.uint32)
+    for i in range(0, n, 97):
+        m = O.or_gather(ptr(x), ptr(nrm), n, d, ptr(hi), ptr(hv), cap, ptr(roff), ptr(rids), i, 600, 4800,
+                        ptr(pd), ptr(pi), 16384)
+        ri = np.zeros(100, np.uint32)
+        rk = np.zeros(100, np.float32)
+        c = O.or_pool_topk(i, ptr(pd), ptr(pi), m, 100, ptr(ri), ptr(rk))
+        assert cnt[i] == c and ids[i, :c].tolist() == ri[:c].tolist(), i
+        assert np.array_equal(bits(keys[i, :c]), bits(rk[:c]))
+
+
 def test_search_tree_matches_reference_draws(ctx, S):
     """The device-built search tree equals the oracle's restatement of the
     reference's build (one MT19937 stream, depth-first) node for node: the
