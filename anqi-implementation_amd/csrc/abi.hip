@@ -610,7 +610,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     // CU instead of three); rows that outgrow it are redone on the slow path.
     const bool rk = gather_register_lists(P, graph);
     P.hmask = std::min<uint32_t>(graph ? 4096 : 8192, next_pow2(need * 3 / 2 + 64)) - 1;
-    P.h16 = (!graph && rk && P.n <= H16_MAX_ID && P.hmask + 1 == H16_SLOTS) ? 1u : 0u;
+    P.h16 = (!graph && rk && P.hmask + 1 == H16_SLOTS) ? (P.n <= H16_MAX_ID ? 1u : P.n <= H24_MAX_ID ? 2u : 0u) : 0u;
     P.pmax = graph ? std::min<uint32_t>(rk ? 1024u : 2048u, next_pow2(need + 64)) : 0;
     P.rows = nullptr;
     P.nrows = P.n;

@@ -531,6 +531,8 @@ __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_
 // CAS; a lane whose CAS loses re-reads the bucket.
 #define H16_EMPTY 0xFFFFu
 #define H16_MAX_ID (1u << 20)
+#define H24_MAX_ID (1u << 24)
+#define H24_OVF_SLOTS 64u
 #define H16_LB_GATHER 11u
 #define H16_SLOTS (4u << H16_LB_GATHER)
 __host__ __device__ __forceinline__ uint32_t mix20(uint32_t id) {
@@ -563,6 +565,53 @@ __device__ __forceinline__ int hset16_insert(uint16_t* tab, uint32_t lb, uint32_
             if (old == v) return 1;
             v = old;
         }
+    }
+    return -1;
+}
+
+// The same 16-bit table for ids < 2^24, with a 32-bit overflow table.  h =
+// mix24(id) (a bijection of the 24-bit ids); the top 11 bits pick the home
+// bucket of 2048 and an entry stores (probe distance << 13) | (low 13 bits of
+// h), probe distances 0..6 (0xFFFF is distance 7: empty).  A key whose seven
+// buckets were full when it came is kept in ovf, H24_OVF_SLOTS 32-bit slots
+// (id + 1, 0 = empty) probed linearly.  A bucket with an empty entry ends a
+// lookup as before; seven full buckets send it on to ovf, which holds every
+// such key.
+__host__ __device__ __forceinline__ uint32_t mix24(uint32_t id) {
+    uint32_t h = (id * 0x9E3779B1u) & 0xFFFFFFu;
+    h ^= h >> 11;
+    return (h * 0x2C1B3C6Du) & 0xFFFFFFu;
+}
+__device__ __forceinline__ int hset24_insert(uint16_t* tab, uint32_t* ovf, uint32_t id) {
+    const uint32_t h = mix24(id);
+    const uint32_t home = h >> 13, rem = h & 0x1FFFu;
+    for (uint32_t dist = 0; dist < 7u; dist++) {
+        const uint32_t code = (dist << 13) | rem;
+        unsigned long long* bk = (unsigned long long*)(tab + 4u * ((home + dist) & 2047u));
+        unsigned long long v = *bk;
+        for (;;) {
+            int fe = -1;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t x = (uint32_t)(v >> (16 * e)) & 0xFFFFu;
+                if (x == code) return 0;
+                if (x == H16_EMPTY && fe < 0) fe = e;
+            }
+            if (fe < 0) break;
+            const unsigned long long nv =
+                (v & ~(0xFFFFull << (16 * fe))) | ((unsigned long long)code << (16 * fe));
+            const unsigned long long old = atomicCAS(bk, v, nv);
+            if (old == v) return 1;
+            v = old;
+        }
+    }
+    const uint32_t key = id + 1u;
+    uint32_t sl = hmix(id) & (H24_OVF_SLOTS - 1u);
+    for (uint32_t probe = 0; probe < H24_OVF_SLOTS; probe++) {
+        const uint32_t old = atomicCAS(&ovf[sl], 0u, key);
+        if (old == 0u) return 1;
+        if (old == key) return 0;
+        sl = (sl + 1u) & (H24_OVF_SLOTS - 1u);
     }
     return -1;
 }

@@ -489,7 +489,10 @@ __host__ __device__ __forceinline__ bool gather_rk(const GatherParams& P, bool g
     return P.batched && P.explore <= 640 && (graph || P.K <= 128);
 }
 
+// Visited set: 32-bit entries (hmask + 1 slots); h16 1: 16-bit entries
+// (ids < 2^20); h16 2: 16-bit entries plus the 32-bit overflow table (ids < 2^24).
 __host__ __device__ __forceinline__ size_t gather_hash_bytes(const GatherParams& P) {
+    if (P.h16 == 2) return 2 * (size_t)H16_SLOTS + 4 * (size_t)H24_OVF_SLOTS;
     return P.h16 ? 2 * (size_t)H16_SLOTS : 4 * (size_t)(P.hmask + 1);
 }
 
@@ -630,14 +633,17 @@ __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uin
 // One row of the gather.  QCH > 0: the beam queue is a register list of
 // QCH chunks (explore <= 64 QCH); PCH > 0: so is the AKNN top-K (K <= 64 PCH).
 // Otherwise both live sorted in LDS (double buffered, binary-search merges).
-// H16: the 16-bit visited set, with the AKNN lists' 32-bit exchange (P.h16).
-template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, bool H16>
+// HM: the visited set (P.h16: 0 32-bit entries; 1, 2 16-bit entries, with the
+// AKNN lists' 32-bit exchange).
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, int HM>
 __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, uint32_t i) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const int d = (int)P.d;
-    if constexpr (H16) {
+    if constexpr (HM != 0) {
         for (uint32_t j = lane; j < H16_SLOTS / 8u; j += 64)
             ((uint4*)S.hash)[j] = make_uint4(~0u, ~0u, ~0u, ~0u);
+        if constexpr (HM == 2)
+            for (uint32_t j = lane; j < H24_OVF_SLOTS; j += 64) S.hash[H16_SLOTS / 2u + j] = 0u;
     } else {
         for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
     }
@@ -666,7 +672,8 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         int fresh = 0;
         if (valid) {
             int r;
-            if constexpr (H16) r = hset16_insert((uint16_t*)S.hash, H16_LB_GATHER, id);
+            if constexpr (HM == 1) r = hset16_insert((uint16_t*)S.hash, H16_LB_GATHER, id);
+            else if constexpr (HM == 2) r = hset24_insert((uint16_t*)S.hash, S.hash + H16_SLOTS / 2u, id);
             else r = hset_insert(S.hash, P.hmask, id);
             if (r < 0) status = 1;
             fresh = (r == 1);
@@ -706,7 +713,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
                 if (lane < k) { S.pd[psize + lane] = dv; S.pi[psize + lane] = cid; }
                 psize += (uint32_t)k;
             } else if constexpr (PCH > 0) {
-                if constexpr (H16) rk_merge_x32(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint32_t*)S.pd);
+                if constexpr (HM != 0) rk_merge_x32(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint32_t*)S.pd);
                 else rk_merge(PL, (lane < k && cid != i) ? pkey(dv, cid) : ~0ull, psize, P.K, (uint64_t*)S.pd);
             } else {
                 uint64_t key = (lane < k && cid != i) ? pkey(dv, cid) : ~0ull;
@@ -727,7 +734,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
             // queue
             if constexpr (QCH > 0) {
                 if constexpr (GRAPH) rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
-                else if constexpr (H16) rk_merge_x32(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint32_t*)S.qd0);
+                else if constexpr (HM != 0) rk_merge_x32(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint32_t*)S.qd0);
                 else rk_merge(Q, lane < k ? pkey(dv, cid) : ~0ull, qsize, P.explore, (uint64_t*)S.qd0);
             } else {
                 uint64_t key = lane < k ? pkey(dv, cid) : ~0ull;
@@ -886,7 +893,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
 }
 
-template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, bool H16>
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, int HM>
 __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     extern __shared__ __align__(16) char lds_raw[];
     uint32_t bi = blockIdx.x;
@@ -894,14 +901,14 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
     uint32_t i = P.rows ? P.rows[bi] : bi;
     char* base = GWS ? (char*)(P.gscratch + (size_t)bi * P.gscratch_words) : lds_raw;
     GatherLds S = gather_carve(base, P, GRAPH, QCH > 0);
-    gather_row<GRAPH, GWS, MC, QCH, PCH, H16>(P, S, i);
+    gather_row<GRAPH, GWS, MC, QCH, PCH, HM>(P, S, i);
 }
 
-template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, bool H16 = false>
+template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, int HM = 0>
 static hipError_t launch_gather_t(const GatherParams& P, size_t lds, hipStream_t s) {
-    const void* f = (const void*)k_gather<GRAPH, GWS, MC, QCH, PCH, H16>;
+    const void* f = (const void*)k_gather<GRAPH, GWS, MC, QCH, PCH, HM>;
     if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC, QCH, PCH, H16>), dim3(P.nrows), dim3(64), lds, s, P);
+    hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC, QCH, PCH, HM>), dim3(P.nrows), dim3(64), lds, s, P);
     return hipGetLastError();
 }
 
@@ -910,9 +917,13 @@ static hipError_t launch_gather_lists(const GatherParams& P, size_t lds, hipStre
     if constexpr (!GWS) {
         if (gather_rk(P, GRAPH)) {
             constexpr int PC = GRAPH ? 0 : 2;
-            if (!GRAPH && P.h16) {
-                if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC, true>(P, lds, s);
-                return launch_gather_t<GRAPH, GWS, MC, 10, PC, true>(P, lds, s);
+            if (!GRAPH && P.h16 == 1) {
+                if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC, 1>(P, lds, s);
+                return launch_gather_t<GRAPH, GWS, MC, 10, PC, 1>(P, lds, s);
+            }
+            if (!GRAPH && P.h16 == 2) {
+                if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC, 2>(P, lds, s);
+                return launch_gather_t<GRAPH, GWS, MC, 10, PC, 2>(P, lds, s);
             }
             if (P.explore <= 128) return launch_gather_t<GRAPH, GWS, MC, 2, PC>(P, lds, s);
             return launch_gather_t<GRAPH, GWS, MC, 10, PC>(P, lds, s);

@@ -146,7 +146,8 @@ struct GatherParams {
     uint32_t* gscratch;     // optional global work area (per block)
     size_t gscratch_words;  // words per block
     uint32_t batched;       // 1: one expansion's first visits processed together
-    uint32_t h16;           // 1: 16-bit LDS visited set of H16_SLOTS entries (n <= 2^20; hmask unused)
+    uint32_t h16;           // 16-bit LDS visited set of H16_SLOTS entries (hmask unused): 1 n <= 2^20,
+                            // 2 n <= 2^24 (with a 32-bit overflow table); 0: 32-bit entries
 };
 // rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
 bool gather_register_lists(const GatherParams& P, bool graph_mode);

@@ -399,6 +399,47 @@ def test_gather_topk_16bit_visited_set_matches_oracle(ctx):
         assert np.array_equal(bits(keys[i, :c]), bits(rk[:c]))
 
 
+def test_gather_topk_24bit_visited_set_matches_oracle(ctx):
+    """With more than 2^20 rows the chosen AKNN gather keeps its visited set in
+    16-bit entries of 24-bit ids plus a 32-bit overflow table (keys whose seven
+    probe buckets are full): 1.1M rows, the build's queue/pool sizes, sampled
+    rows against the oracle's gather."""
+    O = oracle()
+    n, d = 1_100_000, 32
+    x = workload.mixture(n, d, 400, 15.0, seed=13)
+    nrm = np.zeros(n, np.float32)
+    O.or_norms(ptr(x), n, d, ptr(nrm))
+    cap = 100
+    rng = np.random.default_rng(6)
+    nb = rng.integers(0, n, (n, 40)).astype(np.uint32)
+    rows = np.arange(n, dtype=np.uint32)[:, None]
+    nb = np.where(nb == rows, (nb + 1) % n, nb).astype(np.uint32)
+    hi = np.full((n, cap), U32MAX, np.uint32)
+    hv = np.full((n, cap), np.finfo(np.float32).max, np.float32)
+    hi[:, :40] = nb
+    for c0 in range(0, n, 50000):
+        c1 = min(n, c0 + 50000)
+        hv[c0:c1, :40] = ((x[c0:c1, None, :] - x[nb[c0:c1]]) ** 2).sum(-1).astype(np.float32)
+    roff = np.zeros(n + 1, np.uint32)
+    rids = np.zeros(n * cap, np.uint32)
+    O.or_reverse_csr(ptr(hi), n, cap, ptr(roff), ptr(rids))
+    say("gather topk, 24-bit ids")
+    ids, keys, cnt = ctx.gather_topk(ctx.to_dev(x), ctx.to_dev(nrm), ctx.to_dev(hi.view(np.int32)), ctx.to_dev(hv),
+                                     ctx.to_dev(roff.view(np.int32)), ctx.to_dev(rids.view(np.int32)), 600, 4800, 100,
+                                     form=1)
+    ids, keys, cnt = host(ids).view(np.uint32), host(keys), host(cnt)
+    pd = np.zeros(16384, np.float32)
+    pi = np.zeros(16384, np.uint32)
+    for i in list(range(0, n, 9973)) + [n - 1]:
+        m = O.or_gather(ptr(x), ptr(nrm), n, d, ptr(hi), ptr(hv), cap, ptr(roff), ptr(rids), i, 600, 4800,
+                        ptr(pd), ptr(pi), 16384)
+        ri = np.zeros(100, np.uint32)
+        rk = np.zeros(100, np.float32)
+        c = O.or_pool_topk(i, ptr(pd), ptr(pi), m, 100, ptr(ri), ptr(rk))
+        assert cnt[i] == c and ids[i, :c].tolist() == ri[:c].tolist(), i
+        assert np.array_equal(bits(keys[i, :c]), bits(rk[:c]))
+
+
 def test_search_tree_matches_reference_draws(ctx, S):
     """The device-built search tree equals the oracle's restatement of the
     reference's build (one MT19937 stream, depth-first) node for node: the
