@@ -620,6 +620,20 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     if (rc) return rc;
     if (overflow_rows) *overflow_rows = (uint32_t)bad.size();
     if (bad.empty()) return ANQI_OK;
+    if (graph && rk && P.pmax < 2048u && next_pow2(need + 64) > P.pmax) {
+        // rows that outgrew the short pool: once more in LDS with the full one
+        GatherParams R = P;
+        R.pmax = std::min<uint32_t>(2048, next_pow2(need + 64));
+        DBuf rrows;
+        HCHK(rrows.alloc(4 * bad.size()));
+        HCHK(hipMemcpyAsync(rrows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
+        R.rows = rrows.as<uint32_t>();
+        R.nrows = (uint32_t)bad.size();
+        HCHK(launch_gather(R, graph, false, s));
+        rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
+        if (rc) return rc;
+        if (bad.empty()) return ANQI_OK;
+    }
     // slow path: global work area sized for the bound
     GatherParams Q = P;
     Q.h16 = 0;
