@@ -1291,8 +1291,9 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     if (P.Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "L <= 1024");
     if (mode == 1) {
         if (!ctx->counter) HCHK(hipMalloc((void**)&ctx->counter, 256));
-        HCHK(launch_fill_u32(ctx->counter, 0u, 1, s));
+        HCHK(launch_fill_u32(ctx->counter, 0u, 2, s));   // work counter, overflow count
         P.work_counter = ctx->counter;
+        P.ovf_count = ctx->counter + 1;
         if (!P.entries) {
             // the batch's tree descents first, four queries per wavefront,
             // then the work order: nearest entries first
@@ -1318,13 +1319,26 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
         HCHK(launch_range_search(P, false, 0, s));
     }
     if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev1, s));
-    std::vector<uint32_t> hst(nq);
-    HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
+    // the persistent form counts its overflows on the device: the statuses
+    // are read back only when there is something to redo
+    uint32_t novf = 1;
+    if (P.ovf_count) HCHK(hipMemcpyAsync(&novf, P.ovf_count, 4, hipMemcpyDeviceToHost, s));
+    std::vector<uint32_t> hst;
+    if (!P.ovf_count) {
+        hst.resize(nq);
+        HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
+    }
     HCHK(hipStreamSynchronize(s));
     if (ctx->time_kernels) {
         float kms = 0.0f;
         HCHK(hipEventElapsedTime(&kms, ctx->kev0, ctx->kev1));
         ctx->last_kernel_ms = kms;
+    }
+    if (novf == 0) return ANQI_OK;
+    if (hst.empty()) {
+        hst.resize(nq);
+        HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
     }
     std::vector<uint32_t> redo;
     for (uint32_t i = 0; i < nq; i++)

@@ -384,7 +384,10 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         nx += nb;
     }
     if (ovf) {
-        if (lane == 0) { P.status[qi] = (uint32_t)ovf; P.cand_cnt[qi] = rcnt; P.ndist[qi] = nd; P.pred_cnt[qi] = 0; }
+        if (lane == 0) {
+            P.status[qi] = (uint32_t)ovf; P.cand_cnt[qi] = rcnt; P.ndist[qi] = nd; P.pred_cnt[qi] = 0;
+            if (P.ovf_count) atomicAdd(P.ovf_count, 1u);
+        }
         return;
     }
     // (3) postfilter in base space

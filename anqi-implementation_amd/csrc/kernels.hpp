@@ -40,6 +40,7 @@ struct SearchParams {
     uint32_t* nexp;         // nq: expanded nodes (optional)
     uint32_t* status;       // nq: 0 ok, 1 hash full, 2 FIFO full, 3 output truncated
     uint32_t* work_counter; // persistent form
+    uint32_t* ovf_count;    // optional: number of queries that overflowed (status 1 or 2)
     const uint32_t* nq_dev; // optional: batch size read on the device (captured batches)
     uint32_t rbeam;         // set by launch_range_search: register beam (persistent form)
     // split lifted layout (persistent form, d % 16 == 0; null: lifted rows):

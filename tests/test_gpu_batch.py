@@ -135,6 +135,31 @@ def test_work_order_covers_every_query(env, nq):
         assert np.array_equal(a["acc"][i, :c], b["acc"][i, :c])
 
 
+def test_overflowing_queries_are_redone(env):
+    """A visited set far too small for the beam makes most queries overflow in
+    the persistent launch; the library counts them on the device, reads the
+    statuses back and reruns those queries with large work areas.  The answers
+    equal the per-query form's."""
+    import pyanqi
+    ctx = env["ctx"]
+    q = ctx.to_dev(env["q"][:2000])
+    thr = env["tables"][0].contiguous()
+    a = ctx.rknn_search(env["ix"].handle, q, 64, thr, out_cap=4096, mode=0)
+    a = {k: v.cpu().numpy() for k, v in a.items()}
+    ctx.set_search_tuning(pyanqi.SearchTuning(4, 128, 64))
+    try:
+        b = ctx.rknn_search(env["ix"].handle, q, 64, thr, out_cap=4096, mode=1)
+        b = {k: v.cpu().numpy() for k, v in b.items()}
+    finally:
+        ctx.set_search_tuning(None)
+    for f in ("cnt", "nd", "pred", "status"):
+        assert np.array_equal(a[f], b[f]), f
+    for i in range(a["cnt"].shape[0]):
+        c = int(a["cnt"][i])
+        assert np.array_equal(a["cand"][i, :c], b["cand"][i, :c])
+        assert np.array_equal(a["acc"][i, :c], b["acc"][i, :c])
+
+
 def test_chosen_build_equals_plain(env):
     """The chosen build (bf16 MFMA leaf sweep, batched gathers) yields the
     plain build's index bit for bit: AKNN lists, cutoffs, LRQ codes, lifted
