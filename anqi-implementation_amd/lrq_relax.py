@@ -141,7 +141,7 @@ def fit_relaxed(actx, cut, mu, G0, cb0, iters=200, lr_G=1e-3, lr_cb=None, tau0=N
         if mode == 1:
             with torch.no_grad():     # the floor relaxation assumes sorted codewords
                 cb.copy_(torch.sort(cb, dim=1).values)
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         if (it + 1) % check_every == 0 or it + 1 == iters:
             Gc = G.detach().clone().contiguous()
             cbc = _finish_codebook(cb.detach() * sc, Gc, mu, rows)
