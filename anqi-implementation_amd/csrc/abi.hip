@@ -137,7 +137,12 @@ int anqi_ctx_create(int device, anqi_ctx** out) {
     e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e != hipSuccess) { delete ctx; return ANQI_E_DEVICE; }
     ctx->stream = ctx->own;
-    hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        ctx->cus <= 0) {
+        (void)hipStreamDestroy(ctx->own);
+        delete ctx;
+        return ANQI_E_DEVICE;
+    }
     *out = ctx;
     return ANQI_OK;
 }
@@ -1144,7 +1149,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     }
     double t2 = now_s();
     S.leaf_s = t2 - t1;
-    F.perm.alloc(16);
+    HCHK(F.perm.alloc(16));
     // gather (build_approx_knn tail)
     DBuf roff, rids, ocnt;
     HCHK(roff.alloc(4ull * (n + 1)));

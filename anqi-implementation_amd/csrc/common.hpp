@@ -625,3 +625,9 @@ __device__ __forceinline__ void hset_first(uint32_t* tab, uint32_t mask, uint32_
         hipError_t _e = (expr);                                               \
         if (_e != hipSuccess) return anqi_fail_hip(ctx, _e, #expr, __LINE__); \
     } while (0)
+
+// Raises a kernel's dynamic LDS limit before its launch (launchers return
+// the error instead of launching).
+static inline hipError_t set_lds_limit(const void* f, size_t bytes) {
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}

@@ -171,7 +171,7 @@ hipError_t launch_exact_topk(const float* x, const float* nrm, uint32_t n, uint3
     (void)mfma; (void)xbf; (void)margin_scale;
     if (rows == 0) return hipSuccess;
     size_t lds = exact_lds(d, K);
-    hipFuncSetAttribute((const void*)k_exact_topk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipError_t e = set_lds_limit((const void*)k_exact_topk, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_exact_topk, dim3((rows + EX_R - 1) / EX_R), dim3(256), lds, s, x, nrm, n, d, K, row0,
                        rows, ids, keys);
     return hipGetLastError();
@@ -225,12 +225,14 @@ hipError_t launch_rknn_fill(const float* base, const float* bnorm, uint32_t n,
     uint32_t* cursor = nullptr;
     hipError_t e = hipMallocAsync((void**)&cursor, sizeof(uint32_t) * nq, s);
     if (e != hipSuccess) return e;
-    hipMemsetAsync(cursor, 0, sizeof(uint32_t) * nq, s);
-    dim3 grid((n + RK_COLS - 1) / RK_COLS, nq);
-    hipLaunchKernelGGL((k_rknn<true>), grid, dim3(256), 0, s, base, bnorm, n, q, d, rk2, cursor, off, ids, keys);
-    e = hipGetLastError();
-    hipFreeAsync(cursor, s);
-    return e;
+    e = hipMemsetAsync(cursor, 0, sizeof(uint32_t) * nq, s);
+    if (e == hipSuccess) {
+        dim3 grid((n + RK_COLS - 1) / RK_COLS, nq);
+        hipLaunchKernelGGL((k_rknn<true>), grid, dim3(256), 0, s, base, bnorm, n, q, d, rk2, cursor, off, ids, keys);
+        e = hipGetLastError();
+    }
+    const hipError_t ef = hipFreeAsync(cursor, s);
+    return e != hipSuccess ? e : ef;
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, uint64_t m, uint16_t* __restrict__ out) {

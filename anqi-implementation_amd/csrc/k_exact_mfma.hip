@@ -308,7 +308,7 @@ static hipError_t launch_exact_mfma_t(const float* x, const uint16_t* xb, const 
                                       uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
                                       uint32_t* ids, float* keys, hipStream_t s) {
     size_t lds = exact_mfma_lds(K);
-    hipFuncSetAttribute((const void*)k_exact_mfma<MC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC>, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_exact_mfma<MC>, dim3((rows + XM_R - 1) / XM_R), dim3(256), lds, s, x, xb, nrm, marg, n,
                        d, K, row0, rows, ids, keys);
     return hipGetLastError();

@@ -221,7 +221,7 @@ hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s) {
     (void)mfma;
     if (P.nleaves == 0) return hipSuccess;
     size_t lds = leaf_lds_bytes(P);
-    hipFuncSetAttribute((const void*)k_leaf_knn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipError_t e = set_lds_limit((const void*)k_leaf_knn, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_leaf_knn, dim3(P.nleaves), dim3(256), lds, s, P);
     return hipGetLastError();
 }

@@ -85,8 +85,8 @@ hipError_t launch_warm_start(const float* x, const float* nrm, uint32_t n, uint3
     hipLaunchKernelGGL(k_warm_push, dim3((n + 127) / 128), dim3(128), 0, s, ids, dv, n, width, cap, hidx,
                        hval);
     e = hipGetLastError();
-    hipFreeAsync(dv, s);
-    return e;
+    const hipError_t ef = hipFreeAsync(dv, s);
+    return e != hipSuccess ? e : ef;
 }
 
 // ---- LocalJoin ----------------------------------------------------------------------
@@ -302,7 +302,7 @@ hipError_t launch_nnd_propose_mfma(const float* x, const float* nrm, const float
     if (n == 0) return hipSuccess;
     if (!nnd_propose_mfma_supported(d, sample)) return hipErrorInvalidValue;
     const size_t lds = nnd_propose_mfma_lds(d, sample);
-    hipFuncSetAttribute((const void*)k_nnd_propose_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipError_t e = set_lds_limit((const void*)k_nnd_propose_mfma, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_nnd_propose_mfma, dim3(n), dim3(64), lds, s, x, nrm, marg, n, d, samp, scnt, sample, hval,
                        cap, keys, vals, count, max_props);
     return hipGetLastError();
@@ -372,7 +372,7 @@ hipError_t launch_nnd_propose(const float* x, const float* nrm, uint32_t n, uint
                               unsigned long long* count, uint64_t max_props, hipStream_t s) {
     if (n == 0) return hipSuccess;
     size_t lds = sizeof(float) * (size_t)sample * d + sizeof(uint32_t) * sample + 2 * sizeof(float) * sample;
-    hipFuncSetAttribute((const void*)k_nnd_propose, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipError_t e = set_lds_limit((const void*)k_nnd_propose, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_nnd_propose, dim3(n), dim3(64), lds, s, x, nrm, n, d, samp, scnt, sample, hval, cap,
                        keys, vals, count, max_props);
     return hipGetLastError();
@@ -907,7 +907,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P) {
 template <bool GRAPH, bool GWS, int MC, int QCH, int PCH, int HM = 0>
 static hipError_t launch_gather_t(const GatherParams& P, size_t lds, hipStream_t s) {
     const void* f = (const void*)k_gather<GRAPH, GWS, MC, QCH, PCH, HM>;
-    if (lds) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (lds) {
+        if (hipError_t e = set_lds_limit(f, lds); e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_gather<GRAPH, GWS, MC, QCH, PCH, HM>), dim3(P.nrows), dim3(64), lds, s, P);
     return hipGetLastError();
 }
@@ -1016,7 +1018,7 @@ static hipError_t launch_commit_mc(const PruneParams& P, bool gws, hipStream_t s
     if (gws) {
         hipLaunchKernelGGL((k_commit<true, MC>), dim3(P.nrows), dim3(64), 0, s, P);
     } else {
-        hipFuncSetAttribute((const void*)k_commit<false, MC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (hipError_t e = set_lds_limit((const void*)k_commit<false, MC>, lds); e != hipSuccess) return e;
         hipLaunchKernelGGL((k_commit<false, MC>), dim3(P.nrows), dim3(64), lds, s, P);
     }
     return hipGetLastError();

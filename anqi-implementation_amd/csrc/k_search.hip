@@ -642,8 +642,7 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
 
 template <int MC, int NCH>
 static hipError_t launch_persistent_t(const SearchParams& P, uint32_t grid, size_t lds, hipStream_t s) {
-    hipFuncSetAttribute((const void*)k_range_search_persistent<MC, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds);
+    if (hipError_t e = set_lds_limit((const void*)k_range_search_persistent<MC, NCH>, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL((k_range_search_persistent<MC, NCH>), dim3(grid), dim3(64), lds, s, P);
     return hipSuccess;
 }
@@ -656,17 +655,18 @@ static hipError_t launch_search_mc(const SearchParams& P0, bool persistent, uint
     if (!persistent || MC == 0 || (P.d & 15u) || !P.ltail) P.lmain = nullptr;
     size_t lds = search_lds_bytes(P);
     if (persistent) {
+        hipError_t e = hipSuccess;
         switch (search_nch(P.Lmax)) {
-            case 1: launch_persistent_t<MC, 1>(P, grid, lds, s); break;
-            case 2: launch_persistent_t<MC, 2>(P, grid, lds, s); break;
-            case 4: launch_persistent_t<MC, 4>(P, grid, lds, s); break;
-            case 8: launch_persistent_t<MC, 8>(P, grid, lds, s); break;
-            default: launch_persistent_t<MC, 0>(P, grid, lds, s); break;
+            case 1: e = launch_persistent_t<MC, 1>(P, grid, lds, s); break;
+            case 2: e = launch_persistent_t<MC, 2>(P, grid, lds, s); break;
+            case 4: e = launch_persistent_t<MC, 4>(P, grid, lds, s); break;
+            case 8: e = launch_persistent_t<MC, 8>(P, grid, lds, s); break;
+            default: e = launch_persistent_t<MC, 0>(P, grid, lds, s); break;
         }
+        if (e != hipSuccess) return e;
     } else {
         if (P.nq == 0) return hipSuccess;
-        hipFuncSetAttribute((const void*)k_range_search<MC>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (hipError_t e = set_lds_limit((const void*)k_range_search<MC>, lds); e != hipSuccess) return e;
         hipLaunchKernelGGL(k_range_search<MC>, dim3(P.nq), dim3(64), lds, s, P);
     }
     return hipGetLastError();
@@ -690,7 +690,7 @@ template <int MC, int NCH>
 static int occupancy_t(size_t lds) {
     int occ = 0;
     const void* f = (const void*)k_range_search_persistent<MC, NCH>;
-    hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (set_lds_limit(f, lds) != hipSuccess) return 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 64, lds) != hipSuccess) occ = 1;
     return occ;
 }
