@@ -12,7 +12,7 @@
 //   w_kj = softmax_j(-(e_k - cb_kj)^2 / tau),  q_k = sum_j w_kj cb_kj,
 //   z_k = e_k - q_k,   loss = mean_ik z_ik^2,
 //
-// and the exact floor decode (k_lrq.hip) stays the judge of the fitted
+// and the exact floor decode (k_lrq.hip) stays the arbiter of the fitted
 // basis and codebook.  Backward, per row (g = 2 z * scale):
 //   dq/de = 2/tau Var_w(cb)              -> ge_k = g_k (1 - dq_k/de_k)
 //   dL/dcb_kj = -g_k w_kj (1 + 2/tau (cb_kj - q_k)(e_k - cb_kj))

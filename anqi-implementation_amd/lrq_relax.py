@@ -10,7 +10,7 @@ plain torch, used by the tests (FP64, CPU) as the reference gradient and for
 finite differences.  fit_relaxed trains from a starting basis/codebook and
 finishes the codebook as the reference trainer does (endpoint extension to
 the full residual range, :223-226; monotone rows), so the exact floor decode
-(anqi_lrq_encode / materialize) stays the judge of the result.
+(anqi_lrq_encode / materialize) stays the arbiter of the result.
 """
 from __future__ import annotations
 
@@ -124,7 +124,7 @@ def fit_relaxed(actx, cut, mu, G0, cb0, iters=200, lr_G=1e-3, lr_cb=None, tau0=N
         tau1 = tau1 if tau1 is not None else 16.0 * base_tau
     opt = torch.optim.Adam([{"params": [G], "lr": lr_G}, {"params": [cb], "lr": 1.0}])
     rows = cut if all_rows is None else all_rows
-    # the exact floor decode judges every checkpoint; the start is one of them
+    # the exact floor decode rules on every checkpoint; the start is one of them
     best_G, best_cb = G0.detach().clone().float(), cb0.detach().clone().float()
     best = floor_rmse(actx, cut, mu, best_G, best_cb)
     log(f"[relaxed-fit] start: floor-decoded RMSE {best:.6g}")

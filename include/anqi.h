@@ -134,7 +134,7 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
  * All pointers device except *loss (host, may be NULL).  grad_basis
  * (rankm x nk) and grad_codebook (nk x code_size) receive grad_scale times
  * the gradient of the mean loss (either may be NULL).  Hand-written backward;
- * the exact floor decode (anqi_lrq_encode/materialize) judges the result. */
+ * the exact floor decode (anqi_lrq_encode/materialize) rules on the result. */
 int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
                    const float* mu, const float* basis, const float* codebook, uint32_t code_size,
                    float tau, int mode, float grad_scale, double* loss, float* grad_basis,

@@ -5,7 +5,7 @@ AKNN 100/500/64/600), and training uses the reference trainer's
 sorted).  The HIP backward is checked against the FP64 torch reference
 gradient and FP64 central finite differences (eps = 1e-4) on 1,000 of those
 rows; the relaxed fit starts from the Lloyd fit and its exact floor decode
-is the judge: RMSE over all 1,000,000 rows, and coefficients, codes and
+is the arbiter: RMSE over all 1,000,000 rows, and coefficients, codes and
 decoded thresholds (hence every accept decision) equal the reference's on
 the fitted files."""
 import json

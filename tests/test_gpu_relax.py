@@ -2,7 +2,7 @@
 the relaxed radius-reconstruction objective against the FP64 torch reference
 gradient and FP64 central finite differences (eps = 1e-4) on a 1,000-row
 subset of an AKNN cutoff matrix the method itself generates; then a relaxed
-fit whose exact floor decode is the judge: coefficients, codes and decoded
+fit whose exact floor decode is the arbiter: coefficients, codes and decoded
 thresholds equal the reference verifier's on the fitted files."""
 import numpy as np
 import pytest
