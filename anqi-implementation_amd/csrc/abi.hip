@@ -1601,7 +1601,7 @@ int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_t
     HCHK(hipEventCreate(&e1));
     double best = 1e300;
     anqi_search_tuning bt{0, 0, 0};
-    const uint32_t slot_opts[5] = {1536, 2048, 3072, 4096, 8192};
+    const uint32_t slot_opts[6] = {1536, 2048, 3072, 3584, 4096, 8192};
     const uint32_t rcap_opts[2] = {128, 512};   // later FIFO entries spill to the output rows
     std::vector<uint32_t> st(nq);
     for (uint32_t so : slot_opts) {

@@ -293,7 +293,7 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
                      uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total);
 void anqi_batcher_destroy(anqi_batcher* b);
 /* Times the persistent search on a sample batch (device queries, Lq, kidx)
- * for visited-set sizes {1536, 2048, 3072, 4096, 8192}, LDS FIFO sizes {128, 512} and
+ * for visited-set sizes {1536, 2048, 3072, 3584, 4096, 8192}, LDS FIFO sizes {128, 512} and
  * 1/2, 3/4, 4/4 of the resident waves per CU; queries that overflow are
  * charged as reruns.  Writes the fastest tuning and its time. */
 int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_tables,
