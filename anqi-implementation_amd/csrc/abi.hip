@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -27,7 +28,7 @@ struct anqi_ctx {
     int cus = 0;
     uint32_t* counter = nullptr;   // device work counter of the persistent search
     anqi_search_tuning tune{0, 0, 0};   // persistent search work areas / grid (0 = defaults)
-    uint32_t* ent = nullptr;       // persistent search's batch: entries, keys, work order (3 x ent_cap)
+    uint32_t* ent = nullptr;       // persistent search's batch: entries, keys, work order, redo list (4 x ent_cap)
     size_t ent_cap = 0;
     // optional HIP-event timing of the range-search kernel launch itself
     bool time_kernels = false;
@@ -1247,12 +1248,154 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     return ANQI_OK;
 }
 
+}  // extern "C"
+
 // ---- search ------------------------------------------------------------------------------------
 static float round_down_float(double v) {
     float f = (float)v;
     if ((double)f > v) f = std::nextafter(f, -INFINITY);
     return f;
 }
+
+// Work areas of one search call: the persistent form's counters and batch
+// arrays, the redo list, and the global tier's rows.
+struct SearchWork {
+    uint32_t* counter = nullptr;   // [0] work counter, [1] overflow count, [2] list length
+    uint32_t* ent = nullptr;       // 3 x nq: tree entries, entry keys, work order
+    uint32_t* list = nullptr;      // nq: queries to redo
+    uint32_t* gvis = nullptr;      // slots x global_stride(n) bitmap words
+    uint32_t* gfifo = nullptr;     // slots x n FIFO entries
+    uint32_t slots = 0;
+};
+// Supplies the global tier's rows for `need` queries (fewer slots: groups).
+using GlobalArea = std::function<hipError_t(uint32_t need, SearchWork& w)>;
+// Called after each global-tier group with the group's query list (device)
+// and size; block b of the group ran query list[b], FIFO row b.
+using GlobalSink = std::function<int(const uint32_t* dlist, uint32_t count, const SearchWork& w)>;
+
+static uint32_t global_stride(uint32_t n) { return ((n + 31u) / 32u + 3u) & ~3u; }
+
+// Indices of the queries whose status is in `mask` into W.list; returns the count.
+static int status_list(anqi_ctx* ctx, const uint32_t* status, uint32_t nq, uint32_t mask, SearchWork& W,
+                       uint32_t* count, hipStream_t s) {
+    HCHK(launch_status_list(status, nq, mask, W.list, W.counter + 2, s));
+    HCHK(hipMemcpyAsync(count, W.counter + 2, 4, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    return ANQI_OK;
+}
+
+// Reruns the queries that outgrew their work areas (status 1: visited set,
+// 2: candidate FIFO): first with 16K-entry LDS areas, then the rest -- and,
+// with gmask bit 3, the queries whose candidates exceed out_cap -- in the
+// global tier, whose bitmap and FIFO cannot fill.  P: the first pass's
+// parameters (its entries array, if any, indexed by query).
+static int search_redo(anqi_ctx* ctx, const SearchParams& P, uint32_t nq, SearchWork& W, uint32_t gmask,
+                       const GlobalArea& area, const GlobalSink* sink, hipStream_t s) {
+    uint32_t cnt = 0;
+    int rc = status_list(ctx, P.status, nq, (1u << 1) | (1u << 2), W, &cnt, s);
+    if (rc) return rc;
+    SearchParams Q = P;
+    Q.nq_dev = nullptr;
+    Q.ovf_count = nullptr;
+    Q.work_counter = nullptr;
+    if (cnt) {
+        Q.hmask = 16383;
+        Q.rcap = 16384;
+        Q.qlist = W.list;
+        Q.nq = cnt;
+        HCHK(launch_range_search(Q, false, 0, s));
+    }
+    rc = status_list(ctx, P.status, nq, gmask, W, &cnt, s);
+    if (rc || cnt == 0) return rc;
+    if (W.slots == 0) HCHK(area(cnt, W));
+    Q.gvis = W.gvis;
+    Q.gfifo = W.gfifo;
+    Q.gstride = global_stride(P.n);
+    Q.rcap = P.n;
+    Q.hmask = 3;
+    for (uint32_t g = 0; g < cnt; g += W.slots) {
+        Q.qlist = W.list + g;
+        Q.nq = std::min(W.slots, cnt - g);
+        HCHK(launch_range_search_global(Q, s));
+        if (sink && (rc = (*sink)(Q.qlist, Q.nq, W))) return rc;
+    }
+    HCHK(hipStreamSynchronize(s));
+    return ANQI_OK;
+}
+
+// The search behind anqi_rknn_search(_entries): first pass (mode 0: one
+// block per query; mode 1: tree descents, work order, persistent grid), then
+// search_redo for the queries it could not finish.
+static int search_run(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t L,
+                      const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx,
+                      const uint32_t* entries, uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept,
+                      uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp, uint32_t* status,
+                      int mode, SearchWork& W, const GlobalArea& area) {
+    hipStream_t s = ctx->stream;
+    SearchParams P{};
+    P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
+    P.lmain = ix->lmain; P.ltail = ix->ltail;
+    P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
+    P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
+    P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
+    P.entries = entries;
+    double sc = ix->scale;
+    std::memcpy(&P.scale_bits, &sc, 8);
+    P.R2f = round_down_float(ix->mstar2 + 1.0);
+    P.queries = queries; P.nq = nq; P.L = L; P.Lq = Lq; P.Lmax = Lq ? std::max(Lmax, 1u) : std::max(L, 1u);
+    P.thr = thr; P.kidx = kidx;
+    // LDS work area sized to the beam: visited-set slots ~ 4 L + 1536 (at least
+    // twice the typical visited count), results FIFO 1024.  Queries that
+    // overflow it are rerun by search_redo.
+    P.hmask = std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * std::max(L, Lmax) + 1536))) - 1;
+    P.rcap = 1024; P.out_cap = out_cap;
+    if (mode == 1 && ctx->tune.visited_slots >= 2 * P.Lmax) P.hmask = ((ctx->tune.visited_slots + 3u) & ~3u) - 1;
+    if (mode == 1 && ctx->tune.result_cap) P.rcap = ctx->tune.result_cap;
+    P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
+    P.nexp = nexp;
+    P.status = status;
+    if (P.Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "L <= 1024");
+    HCHK(launch_fill_u32(W.counter, 0u, 2, s));   // work counter, overflow count
+    P.work_counter = W.counter;
+    P.ovf_count = W.counter + 1;
+    if (mode == 1) {
+        if (!P.entries) {
+            // the batch's tree descents first, four queries per wavefront,
+            // then the work order: nearest entries first
+            float* key = (float*)(W.ent + nq);
+            uint32_t* order = W.ent + 2ull * nq;
+            HCHK(launch_tree_entry_batch(P, nq, W.ent, key, s));
+            HCHK(launch_entry_order(P, key, order, s));
+            P.entries = W.ent;
+            P.qlist = order;
+        }
+        int occ = std::max(1, search_persistent_occupancy(P));
+        uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
+        uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
+        if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev0, s));
+        HCHK(launch_range_search(P, true, grid, s));
+    } else {
+        if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev0, s));
+        HCHK(launch_range_search(P, false, 0, s));
+    }
+    if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev1, s));
+    // overflows are counted on the device: statuses are scanned only when
+    // there is something to redo
+    uint32_t novf = 0;
+    HCHK(hipMemcpyAsync(&novf, P.ovf_count, 4, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    if (ctx->time_kernels) {
+        float kms = 0.0f;
+        HCHK(hipEventElapsedTime(&kms, ctx->kev0, ctx->kev1));
+        ctx->last_kernel_ms = kms;
+    }
+    if (novf == 0) return ANQI_OK;
+    P.qlist = nullptr;
+    return search_redo(ctx, P, nq, W, (1u << 1) | (1u << 2), area, nullptr, s);
+}
+
+extern "C" {
+
 
 int anqi_rknn_search(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t L,
                      const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx, uint32_t out_cap,
@@ -1270,101 +1413,33 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
     if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
     if (nq == 0) return ANQI_OK;
+    if (out_cap == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "out_cap must be positive");
     hipStream_t s = ctx->stream;
-    SearchParams P{};
-    P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
-    P.lmain = ix->lmain; P.ltail = ix->ltail;
-    P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
-    P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
-    P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
-    P.entries = entries;
-    double sc = ix->scale;
-    std::memcpy(&P.scale_bits, &sc, 8);
-    P.R2f = round_down_float(ix->mstar2 + 1.0);
-    P.queries = queries; P.nq = nq; P.L = L; P.Lq = Lq; P.Lmax = Lq ? std::max(Lmax, 1u) : std::max(L, 1u);
-    P.thr = thr; P.kidx = kidx;
-    // LDS work area sized to the beam: visited-set slots ~ 4 L + 1536 (at least
-    // twice the typical visited count), results FIFO 1024.  Queries that
-    // overflow it are rerun below with 16K-entry areas.
-    P.hmask = std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * std::max(L, Lmax) + 1536))) - 1;
-    P.rcap = 1024; P.out_cap = out_cap;
-    if (mode == 1 && ctx->tune.visited_slots >= 2 * P.Lmax) P.hmask = ((ctx->tune.visited_slots + 3u) & ~3u) - 1;
-    if (mode == 1 && ctx->tune.result_cap) P.rcap = ctx->tune.result_cap;
-    P.cand_ids = cand_ids; P.accept = accept; P.cand_cnt = cand_cnt; P.ndist = ndist; P.pred_cnt = pred_cnt;
-    P.nexp = nexp;
-    P.status = status;
-    if (P.Lmax > 1024) return anqi_set_err(ctx, ANQI_E_USAGE, "L <= 1024");
-    if (mode == 1) {
-        if (!ctx->counter) HCHK(hipMalloc((void**)&ctx->counter, 256));
-        HCHK(launch_fill_u32(ctx->counter, 0u, 2, s));   // work counter, overflow count
-        P.work_counter = ctx->counter;
-        P.ovf_count = ctx->counter + 1;
-        if (!P.entries) {
-            // the batch's tree descents first, four queries per wavefront,
-            // then the work order: nearest entries first
-            if (ctx->ent_cap < nq) {
-                if (ctx->ent) { HCHK(hipStreamSynchronize(s)); HCHK(hipFree(ctx->ent)); ctx->ent = nullptr; }
-                HCHK(hipMalloc((void**)&ctx->ent, 12ull * nq));
-                ctx->ent_cap = nq;
-            }
-            float* key = (float*)(ctx->ent + nq);
-            uint32_t* order = ctx->ent + 2ull * nq;
-            HCHK(launch_tree_entry_batch(P, nq, ctx->ent, key, s));
-            HCHK(launch_entry_order(P, key, order, s));
-            P.entries = ctx->ent;
-            P.qlist = order;
-        }
-        int occ = std::max(1, search_persistent_occupancy(P));
-        uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
-        uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
-        if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev0, s));
-        HCHK(launch_range_search(P, true, grid, s));
-    } else {
-        if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev0, s));
-        HCHK(launch_range_search(P, false, 0, s));
+    if (!ctx->counter) HCHK(hipMalloc((void**)&ctx->counter, 256));
+    if (ctx->ent_cap < nq) {
+        if (ctx->ent) { HCHK(hipStreamSynchronize(s)); HCHK(hipFree(ctx->ent)); ctx->ent = nullptr; }
+        HCHK(hipMalloc((void**)&ctx->ent, 16ull * nq));
+        ctx->ent_cap = nq;
     }
-    if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev1, s));
-    // the persistent form counts its overflows on the device: the statuses
-    // are read back only when there is something to redo
-    uint32_t novf = 1;
-    if (P.ovf_count) HCHK(hipMemcpyAsync(&novf, P.ovf_count, 4, hipMemcpyDeviceToHost, s));
-    std::vector<uint32_t> hst;
-    if (!P.ovf_count) {
-        hst.resize(nq);
-        HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
-    }
-    HCHK(hipStreamSynchronize(s));
-    if (ctx->time_kernels) {
-        float kms = 0.0f;
-        HCHK(hipEventElapsedTime(&kms, ctx->kev0, ctx->kev1));
-        ctx->last_kernel_ms = kms;
-    }
-    if (novf == 0) return ANQI_OK;
-    if (hst.empty()) {
-        hst.resize(nq);
-        HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
-        HCHK(hipStreamSynchronize(s));
-    }
-    std::vector<uint32_t> redo;
-    for (uint32_t i = 0; i < nq; i++)
-        if (hst[i] == 1 || hst[i] == 2) redo.push_back(i);
-    if (!redo.empty()) {
-        SearchParams Q = P;
-        Q.hmask = 16383;
-        Q.rcap = 16384;
-        DBuf ql;
-        HCHK(ql.alloc(4 * redo.size()));
-        HCHK(hipMemcpyAsync(ql.p, redo.data(), 4 * redo.size(), hipMemcpyHostToDevice, s));
-        Q.qlist = ql.as<uint32_t>();
-        Q.nq = (uint32_t)redo.size();
-        HCHK(launch_range_search(Q, false, 0, s));
-        HCHK(hipMemcpyAsync(hst.data(), status, 4ull * nq, hipMemcpyDeviceToHost, s));
-        HCHK(hipStreamSynchronize(s));
-        for (uint32_t i : redo)
-            if (hst[i] == 1 || hst[i] == 2)
-                return anqi_set_err(ctx, ANQI_E_DEVICE, "query %u exceeds the search work area", i);
-    }
-    return ANQI_OK;
+    SearchWork W;
+    W.counter = ctx->counter;
+    W.ent = ctx->ent;
+    W.list = ctx->ent + 3ull * nq;
+    DBuf gv, gf;
+    GlobalArea area = [&](uint32_t need, SearchWork& w) -> hipError_t {
+        // global tier rows for up to 1 GiB of bitmaps and FIFOs per group
+        const size_t per = 4ull * global_stride(ix->n) + 4ull * ix->n;
+        const uint32_t slots = (uint32_t)std::max<size_t>(1, std::min<size_t>(need, (1ull << 30) / per));
+        hipError_t e;
+        if ((e = gv.alloc(4ull * global_stride(ix->n) * slots)) != hipSuccess) return e;
+        if ((e = gf.alloc(4ull * ix->n * slots)) != hipSuccess) return e;
+        w.gvis = gv.as<uint32_t>();
+        w.gfifo = gf.as<uint32_t>();
+        w.slots = slots;
+        return hipSuccess;
+    };
+    return search_run(ctx, ix, queries, nq, L, Lq, Lmax, thr, kidx, entries, out_cap, cand_ids, accept, cand_cnt,
+                      ndist, pred_cnt, nexp, status, mode, W, area);
 }
 
 }  // extern "C"
@@ -1393,7 +1468,7 @@ struct anqi_batcher {
     hipStream_t s = nullptr;
     SearchParams P{};
     anqi_index ix{};
-    uint32_t max_batch = 0, out_cap = 0, Lmax = 0, grid = 0;
+    uint32_t max_batch = 0, out_cap = 0, Lmax = 0, grid = 0, ntables = 0;
     DBuf q, Lq, kidx, nq, cand, acc, cnt, nd, pred, status, off, ids, counter, ent;
     float* hq = nullptr;
     uint32_t *hL = nullptr, *hk = nullptr, *hnq = nullptr, *hstatus = nullptr;
@@ -1413,7 +1488,8 @@ struct anqi_batcher {
 
 static hipError_t batch_enqueue(anqi_batcher* b) {
     hipError_t e;
-    if ((e = launch_fill_u32(b->counter.as<uint32_t>(), 0u, 1, b->s)) != hipSuccess) return e;
+    // work counter and overflow count
+    if ((e = launch_fill_u32(b->counter.as<uint32_t>(), 0u, 2, b->s)) != hipSuccess) return e;
     SearchParams T = b->P;
     T.entries = nullptr;
     T.qlist = nullptr;
@@ -1452,7 +1528,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     std::unique_ptr<anqi_batcher> b(new anqi_batcher());
     b->ctx = ctx;
     b->ix = *ix;
-    b->max_batch = max_batch; b->out_cap = out_cap; b->Lmax = Lmax;
+    b->max_batch = max_batch; b->out_cap = out_cap; b->Lmax = Lmax; b->ntables = ntables;
     HCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
     const size_t mb = max_batch;
     HCHK(b->q.alloc(4 * mb * ix->d));
@@ -1468,7 +1544,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     HCHK(b->off.alloc(4 * (mb + 1)));
     HCHK(b->ids.alloc(4 * mb * out_cap));
     HCHK(b->counter.alloc(256));
-    HCHK(b->ent.alloc(12 * mb));   // entries, keys, work order
+    HCHK(b->ent.alloc(16 * mb));   // entries, keys, work order, redo list
     HCHK(hipHostMalloc((void**)&b->hq, 4 * mb * ix->d, hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&b->hL, 4 * mb, hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&b->hk, 4 * mb, hipHostMallocDefault));
@@ -1484,11 +1560,15 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     const uint32_t slots = t.visited_slots ? ((t.visited_slots + 3u) & ~3u)
                                            : std::min<uint32_t>(8192, std::max<uint32_t>(2048, next_pow2(4ull * Lmax + 1536)));
     P.hmask = slots - 1;
-    P.rcap = t.result_cap ? t.result_cap : 1024;
+    // an LDS FIFO longer than the output row would let a query finish with
+    // candidates past out_cap (status 3); capped, such a query overflows
+    // (status 2) and is redone
+    P.rcap = std::min(t.result_cap ? t.result_cap : 1024u, out_cap);
     P.out_cap = out_cap;
     P.cand_ids = b->cand.as<uint32_t>(); P.accept = b->acc.as<uint8_t>(); P.cand_cnt = b->cnt.as<uint32_t>();
     P.ndist = b->nd.as<uint32_t>(); P.pred_cnt = b->pred.as<uint32_t>(); P.nexp = nullptr;
     P.status = b->status.as<uint32_t>(); P.work_counter = b->counter.as<uint32_t>();
+    P.ovf_count = b->counter.as<uint32_t>() + 1;
     P.entries = b->ent.as<uint32_t>();
     P.qlist = b->ent.as<uint32_t>() + 2 * mb;
     const int occ = std::max(1, search_persistent_occupancy(P));
@@ -1518,8 +1598,12 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
     anqi_ctx* ctx = b->ctx;
     if (nq > b->max_batch) return anqi_set_err(ctx, ANQI_E_USAGE, "batch of %u exceeds capacity %u", nq, b->max_batch);
     const uint32_t d = b->ix.d;
-    for (uint32_t i = 0; i < nq; i++)
+    for (uint32_t i = 0; i < nq; i++) {
         if (h_L[i] == 0 || h_L[i] > b->Lmax) return anqi_set_err(ctx, ANQI_E_USAGE, "query %u: L outside 1..Lmax", i);
+        if (h_kidx[i] >= b->ntables)
+            return anqi_set_err(ctx, ANQI_E_USAGE, "query %u: table index %u outside 0..%u", i, h_kidx[i],
+                                b->ntables - 1);
+    }
     std::memcpy(b->hq, h_queries, 4ull * nq * d);
     std::memcpy(b->hL, h_L, 4ull * nq);
     std::memcpy(b->hk, h_kidx, 4ull * nq);
@@ -1530,41 +1614,93 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
     HCHK(hipMemcpyAsync(b->kidx.p, b->hk, 4ull * nq, hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(b->nq.p, b->hnq, 4, hipMemcpyHostToDevice, s));
     HCHK(hipGraphLaunch(b->exec, s));
-    HCHK(hipMemcpyAsync(b->hstatus, b->status.p, 4ull * nq, hipMemcpyDeviceToHost, s));
+    HCHK(hipMemcpyAsync(b->hstatus, b->counter.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s));
     HCHK(hipMemcpyAsync(h_off, b->off.p, 4ull * (nq + 1), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
-    std::vector<uint32_t> redo;
-    for (uint32_t i = 0; i < nq; i++)
-        if (b->hstatus[i] != 0) redo.push_back(i);
-    if (!redo.empty()) {
-        // queries that outgrew the tuned work areas: rerun with 16K-entry areas
-        // (uncaptured), then rebuild the answer lists
+    // queries redone in the global tier: their answers, read from the FIFO rows
+    std::vector<uint32_t> gq;                  // query indices
+    std::vector<std::vector<uint32_t>> gans;   // accepted ids, candidate order
+    if (b->hstatus[0] != 0) {
         SearchParams Q = b->P;
-        Q.nq_dev = nullptr;
-        Q.hmask = 16383;
-        Q.rcap = 16384;
-        DBuf ql;
-        HCHK(ql.alloc(4 * redo.size()));
-        HCHK(hipMemcpyAsync(ql.p, redo.data(), 4 * redo.size(), hipMemcpyHostToDevice, s));
-        Q.qlist = ql.as<uint32_t>();
-        Q.nq = (uint32_t)redo.size();
-        HCHK(launch_range_search(Q, false, 0, s));
+        Q.nq = nq;
+        Q.qlist = nullptr;
+        SearchWork W;
+        W.counter = b->counter.as<uint32_t>();
+        W.ent = b->ent.as<uint32_t>();
+        W.list = b->ent.as<uint32_t>() + 3ull * b->max_batch;
+        DBuf gv, gf;
+        const uint32_t n = b->ix.n;
+        GlobalArea area = [&](uint32_t need, SearchWork& w) -> hipError_t {
+            const size_t per = 4ull * global_stride(n) + 4ull * n;
+            const uint32_t slots = (uint32_t)std::max<size_t>(1, std::min<size_t>(need, (1ull << 30) / per));
+            hipError_t e;
+            if ((e = gv.alloc(4ull * global_stride(n) * slots)) != hipSuccess) return e;
+            if ((e = gf.alloc(4ull * n * slots)) != hipSuccess) return e;
+            w.gvis = gv.as<uint32_t>();
+            w.gfifo = gf.as<uint32_t>();
+            w.slots = slots;
+            return hipSuccess;
+        };
+        std::vector<uint32_t> hcnt(nq), row;
+        GlobalSink sink = [&](const uint32_t* dlist, uint32_t count, const SearchWork& w) -> int {
+            const size_t at = gq.size();
+            gq.resize(at + count);
+            HCHK(hipMemcpyAsync(gq.data() + at, dlist, 4ull * count, hipMemcpyDeviceToHost, s));
+            HCHK(hipMemcpyAsync(hcnt.data(), b->cnt.p, 4ull * nq, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+            for (uint32_t j = 0; j < count; j++) {
+                const uint32_t c = hcnt[gq[at + j]];
+                row.resize(c);
+                if (c) {
+                    HCHK(hipMemcpyAsync(row.data(), w.gfifo + (size_t)j * n, 4ull * c, hipMemcpyDeviceToHost, s));
+                    HCHK(hipStreamSynchronize(s));
+                }
+                std::vector<uint32_t> acc;
+                for (uint32_t v : row)
+                    if (v >> 31) acc.push_back(v & 0x7fffffffu);
+                gans.push_back(std::move(acc));
+            }
+            return ANQI_OK;
+        };
+        // tiers: 16K-entry LDS areas, then the global tier for what is left
+        // and for candidate lists longer than out_cap
+        int rc = search_redo(ctx, Q, nq, W, (1u << 1) | (1u << 2) | (1u << 3), area, &sink, s);
+        if (rc) return rc;
         HCHK(launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(), s));
         HCHK(launch_batch_compact(b->cand.as<uint32_t>(), b->acc.as<uint8_t>(), b->cnt.as<uint32_t>(),
                                   b->off.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->out_cap,
                                   b->ids.as<uint32_t>(), s));
-        HCHK(hipMemcpyAsync(b->hstatus, b->status.p, 4ull * nq, hipMemcpyDeviceToHost, s));
         HCHK(hipMemcpyAsync(h_off, b->off.p, 4ull * (nq + 1), hipMemcpyDeviceToHost, s));
         HCHK(hipStreamSynchronize(s));
-        for (uint32_t i : redo)
-            if (b->hstatus[i] != 0)
-                return anqi_set_err(ctx, ANQI_E_DEVICE, "query %u exceeds the search work area", i);
     }
-    *total = h_off[nq];
-    if (*total > ids_cap) return anqi_set_err(ctx, ANQI_E_USAGE, "answers need %llu ids", (unsigned long long)*total);
-    if (*total) {
-        HCHK(hipMemcpyAsync(h_ids, b->ids.p, 4ull * *total, hipMemcpyDeviceToHost, s));
+    if (gq.empty()) {
+        *total = h_off[nq];
+        if (*total > ids_cap) return anqi_set_err(ctx, ANQI_E_USAGE, "answers need %llu ids", (unsigned long long)*total);
+        if (*total) {
+            HCHK(hipMemcpyAsync(h_ids, b->ids.p, 4ull * *total, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+        }
+        return ANQI_OK;
+    }
+    // splice the global tier's answers into the device CSR
+    std::vector<uint32_t> dev_off(h_off, h_off + nq + 1), dev_ids(dev_off[nq]);
+    if (!dev_ids.empty()) {
+        HCHK(hipMemcpyAsync(dev_ids.data(), b->ids.p, 4ull * dev_ids.size(), hipMemcpyDeviceToHost, s));
         HCHK(hipStreamSynchronize(s));
+    }
+    std::vector<int32_t> slot(nq, -1);
+    for (size_t j = 0; j < gq.size(); j++) slot[gq[j]] = (int32_t)j;
+    uint64_t t = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        h_off[i] = (uint32_t)t;
+        t += slot[i] >= 0 ? gans[slot[i]].size() : (uint64_t)(dev_off[i + 1] - dev_off[i]);
+    }
+    h_off[nq] = (uint32_t)t;
+    *total = t;
+    if (t > ids_cap) return anqi_set_err(ctx, ANQI_E_USAGE, "answers need %llu ids", (unsigned long long)t);
+    for (uint32_t i = 0; i < nq; i++) {
+        if (slot[i] >= 0) std::copy(gans[slot[i]].begin(), gans[slot[i]].end(), h_ids + h_off[i]);
+        else std::copy(dev_ids.begin() + dev_off[i], dev_ids.begin() + dev_off[i + 1], h_ids + h_off[i]);
     }
     return ANQI_OK;
 }

@@ -36,7 +36,8 @@ struct SearchLds {
     // buffer selection by value, so the struct stays in registers
     __device__ float* bd(int b) const { return b ? bd1 : bd0; }
     __device__ uint32_t* bp(int b) const { return b ? bp1 : bp0; }
-    uint32_t* hash;    // visited set
+    uint32_t* hash;    // visited set (LDS tiers)
+    uint32_t* gvis;    // visited bitmap of n bits (global tier)
     uint32_t* res;     // collected ids (FIFO), first rcap entries
     uint32_t* spill;   // the query's output row: FIFO entries from rcap on
     uint32_t* tid;     // SR_TID new ids
@@ -56,6 +57,9 @@ __device__ __forceinline__ uint32_t fifo_get(const SearchParams& P, const Search
     return j < P.rcap ? S.res[j] : S.spill[j];
 }
 
+// GV: global tier -- the visited set and the FIFO live in global memory
+// (set by the kernel), LDS holds the beam and the id/distance buffers.
+template <bool GV>
 __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
     SearchLds s;
     size_t o = 0;
@@ -71,11 +75,29 @@ __device__ __forceinline__ SearchLds carve(char* base, const SearchParams& P) {
         s.bp1 = (uint32_t*)take(sizeof(uint32_t) * P.Lmax);
         s.cs = (uint64_t*)take(sizeof(uint64_t) * 64);
     }
-    s.hash = (uint32_t*)take(sizeof(uint32_t) * (P.hmask + 1));
-    s.res = (uint32_t*)take(sizeof(uint32_t) * P.rcap);
+    s.gvis = nullptr;
+    if constexpr (GV) {
+        s.hash = nullptr;
+        s.res = nullptr;
+    } else {
+        s.hash = (uint32_t*)take(sizeof(uint32_t) * (P.hmask + 1));
+        s.res = (uint32_t*)take(sizeof(uint32_t) * P.rcap);
+    }
     s.tid = (uint32_t*)take(sizeof(uint32_t) * SR_TID);
     s.tdv = (float*)take(sizeof(float) * SR_TID);
     return s;
+}
+
+// Visited-set insert: 1 if id was not yet visited, 0 if it was, -1 if the
+// LDS set has no room for it (LDS tiers only; the bitmap never fills).
+template <bool GV>
+__device__ __forceinline__ int vis_insert(const SearchParams& P, const SearchLds& S, uint32_t id) {
+    if constexpr (GV) {
+        const uint32_t bit = 1u << (id & 31u);
+        return (atomicOr(S.gvis + (id >> 5), bit) & bit) ? 0 : 1;
+    } else {
+        return hset_insert(S.hash, P.hmask, id);
+    }
 }
 
 // Expands node c: visits its unvisited neighbours, computes D, appends
@@ -102,7 +124,7 @@ __device__ __forceinline__ void batch_l2(const SearchParams& P, const QRow& QL, 
     }
 }
 
-template <bool BEAM, int MC>
+template <bool BEAM, int MC, bool GV>
 __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, const QRow& QL,
                                        uint32_t c, uint32_t& rcnt, uint32_t& nd,
                                        int& buf, uint32_t& bsize, uint32_t L, int& ovf) {
@@ -114,7 +136,7 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     int fresh = 0;
     if ((uint32_t)lane < dg) {
         nb = nbr;
-        int r = hset_insert(S.hash, P.hmask, nb);
+        int r = vis_insert<GV>(P, S, nb);
         if (r < 0) ovf = 1;
         fresh = (r == 1);
     }
@@ -217,7 +239,7 @@ __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, c
 // that same order.  Their distances are computed in one batch.  The step
 // stops before a node whose neighbours might not fit the id buffer; nb is
 // set to the number of nodes expanded (at least one: deg <= 64 <= SR_TID).
-template <int MC, int FL>
+template <int MC, int FL, bool GV>
 __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t head,
                                            uint32_t& nb, uint32_t& rcnt, uint32_t& nd, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
@@ -240,7 +262,7 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
         if ((uint32_t)t >= done) continue;
         int fresh = 0;
         if ((uint32_t)lane < dg[t]) {
-            int r = hset_insert(S.hash, P.hmask, nbr[t]);
+            int r = vis_insert<GV>(P, S, nbr[t]);
             if (r < 0) ovf = 1;
             fresh = (r == 1);
         }
@@ -269,7 +291,8 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
 
 // NCH == 0: beam sorted in LDS (plain form); NCH > 0: register beam of
 // 64 NCH positions (persistent form, L <= 64 NCH).  FL: flood step width.
-template <int MC, int NCH, int FL>
+// GV: global tier (NCH == 0 only).
+template <int MC, int NCH, int FL, bool GV = false>
 __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     const int lane = __lane_id(), l = lane & 15;
     S.spill = P.cand_ids + (size_t)qi * P.out_cap;
@@ -280,7 +303,15 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     // float, then the lift coordinates (1, 0); each lane forms the elements
     // its register row holds (element 16 t + l, and the tail)
     const float* q = P.queries + (size_t)qi * P.d;
-    for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    if constexpr (GV) {
+        // clear this block's bitmap; the fence orders the stores before the
+        // visited-set atomics
+        uint4* w = (uint4*)S.gvis;
+        for (uint32_t j = lane; j < (P.gstride >> 2); j += 64) w[j] = make_uint4(0u, 0u, 0u, 0u);
+        __threadfence();
+    } else {
+        for (uint32_t j = lane; j <= P.hmask; j += 64) S.hash[j] = 0u;
+    }
     QRow QL;
     {
         // lane holds elements lane + 64 k (k < 4, dl <= 256), moved into the
@@ -322,7 +353,8 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     uint32_t nd = 1, rcnt = 0, bsize = 1, nx = 0;
     int buf = 0, ovf = 0;
     if (lane == 0) {
-        hset_first(S.hash, P.hmask, ep);
+        if constexpr (GV) atomicOr(S.gvis + (ep >> 5), 1u << (ep & 31u));
+        else hset_first(S.hash, P.hmask, ep);
         if constexpr (NCH == 0) {
             S.bd0[0] = dep;
             S.bp0[0] = ep;
@@ -373,13 +405,13 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         if (lane == 0) S.bp(buf)[cur] = c | 0x80000000u;
         __syncthreads();
         nx++;
-        if (!expand<true, MC>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+        if (!expand<true, MC, GV>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
     }
     // (2) flood over collected nodes, FIFO order
     uint32_t head = 0;
     while (!ovf && head < rcnt) {
         uint32_t nb = min((uint32_t)FL, rcnt - head);
-        if (!flood_step<MC, FL>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
+        if (!flood_step<MC, FL, GV>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
         head += nb;
         nx += nb;
     }
@@ -407,6 +439,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                 float key = fmaf(-2.0f, ip, P.bnorm[o] + qn);
                 ok = key <= thr[o];
                 if (l == 0 && j < P.out_cap) { out[j] = o; acc[j] = (uint8_t)ok; }
+                if (GV && l == 0) S.res[j] = o | (ok ? 0x80000000u : 0u);
             }
             npred += (uint32_t)popc64(ballot(ok && l == 0));
         }
@@ -452,6 +485,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                     float key = fmaf(-2.0f, ip, bn[j] + qn);
                     ok = key <= th[j];
                     if (l == 0 && s < P.out_cap) { out[s] = oid[j]; acc[s] = (uint8_t)ok; }
+                    if (GV && l == 0) S.res[s] = oid[j] | (ok ? 0x80000000u : 0u);
                 }
                 npred += (uint32_t)popc64(ballot(ok && l == 0));
             }
@@ -471,10 +505,40 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
 template <int MC>
 __global__ void __launch_bounds__(64) k_range_search(SearchParams P) {
     extern __shared__ __align__(16) char lds[];
-    SearchLds S = carve(lds, P);
+    SearchLds S = carve<false>(lds, P);
     if (blockIdx.x >= P.nq) return;
     uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
     search_one<MC, 0, SR_FLOOD_PLAIN>(P, S, qi);
+}
+
+// Global tier: block b runs query qlist[b] with its visited bitmap and FIFO
+// in global memory (rows b of gvis and gfifo), so neither can fill: the
+// reference's flood is unbounded (QueryVisited, include/index.h:33-52, is an
+// n-entry stamp array) and so is this one.  Used for the rare queries that
+// outgrow the LDS tiers.
+template <int MC>
+__global__ void __launch_bounds__(64) k_range_search_global(SearchParams P) {
+    extern __shared__ __align__(16) char lds[];
+    SearchLds S = carve<true>(lds, P);
+    if (blockIdx.x >= P.nq) return;
+    S.gvis = P.gvis + (size_t)blockIdx.x * P.gstride;
+    S.res = P.gfifo + (size_t)blockIdx.x * P.n;
+    const uint32_t qi = P.qlist ? P.qlist[blockIdx.x] : blockIdx.x;
+    search_one<MC, 0, SR_FLOOD_PLAIN, true>(P, S, qi);
+}
+
+// Status list: indices whose status is in `mask` (one block).
+__global__ void __launch_bounds__(1024) k_status_list(const uint32_t* __restrict__ status, uint32_t nq, uint32_t mask,
+                                                      uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    __shared__ uint32_t cnt;
+    if (threadIdx.x == 0) cnt = 0u;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
+        const uint32_t v = status[q];
+        if (v < 32u && ((mask >> v) & 1u)) list[atomicAdd(&cnt, 1u)] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *count = cnt;
 }
 
 // Persistent form: a fixed grid pulls query indices from a device counter.
@@ -482,7 +546,7 @@ __global__ void __launch_bounds__(64) k_range_search(SearchParams P) {
 template <int MC, int NCH>
 __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) {
     extern __shared__ __align__(16) char lds[];
-    SearchLds S = carve(lds, P);
+    SearchLds S = carve<false>(lds, P);
     __shared__ uint32_t next;
     const uint32_t nq = P.nq_dev ? *P.nq_dev : P.nq;
     for (;;) {
@@ -631,8 +695,8 @@ static int search_nch(uint32_t Lmax) {
 size_t search_lds_bytes(const SearchParams& P) {
     auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
     const size_t beam = P.rbeam ? r(8 * (size_t)P.Lmax) : 4 * r(4 * (size_t)P.Lmax) + r(8 * 64);
-    return beam + r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap) +
-           r(4 * SR_TID) + r(4 * SR_TID);
+    const size_t sets = P.gvis ? 0 : r(4 * (size_t)(P.hmask + 1)) + r(4 * (size_t)P.rcap);
+    return beam + sets + r(4 * SR_TID) + r(4 * SR_TID);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -669,6 +733,33 @@ static hipError_t launch_search_mc(const SearchParams& P0, bool persistent, uint
         if (hipError_t e = set_lds_limit((const void*)k_range_search<MC>, lds); e != hipSuccess) return e;
         hipLaunchKernelGGL(k_range_search<MC>, dim3(P.nq), dim3(64), lds, s, P);
     }
+    return hipGetLastError();
+}
+
+template <int MC>
+static hipError_t launch_global_mc(const SearchParams& P0, hipStream_t s) {
+    SearchParams P = P0;
+    P.rbeam = 0;
+    P.lmain = nullptr;
+    const size_t lds = search_lds_bytes(P);
+    if (hipError_t e = set_lds_limit((const void*)k_range_search_global<MC>, lds); e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_range_search_global<MC>, dim3(P.nq), dim3(64), lds, s, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_range_search_global(const SearchParams& P, hipStream_t s) {
+    if (P.nq == 0) return hipSuccess;
+    if (!P.gvis || !P.gfifo || (P.gstride & 3u) || (uint64_t)P.gstride * 32u < P.n) return hipErrorInvalidValue;
+    switch (search_chunks(P)) {
+        case 8: return launch_global_mc<8>(P, s);
+        case 6: return launch_global_mc<6>(P, s);
+        default: return launch_global_mc<0>(P, s);
+    }
+}
+
+hipError_t launch_status_list(const uint32_t* status, uint32_t nq, uint32_t mask, uint32_t* list, uint32_t* count,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_status_list, dim3(1), dim3(1024), 0, s, status, nq, mask, list, count);
     return hipGetLastError();
 }
 

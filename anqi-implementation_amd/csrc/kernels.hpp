@@ -46,7 +46,22 @@ struct SearchParams {
     // split lifted layout (persistent form, d % 16 == 0; null: lifted rows):
     const float* lmain;     // n x d: the lifted rows' first d coordinates
     const float* ltail;     // n: reduced term of each row's lift coordinates against a query's (1, 0)
+    // global-memory tier (k_range_search_global): block b runs query
+    // qlist[b] with a visited bitmap of n bits at gvis + b * gstride words and
+    // a candidate FIFO of n entries at gfifo + b * n, so no query outgrows it
+    uint32_t* gvis;
+    uint32_t* gfifo;
+    uint32_t gstride;       // bitmap words per block (a multiple of 4)
 };
+// Global tier: one block per listed query (P.qlist[0..P.nq)), P.gvis/gfifo
+// sized for P.nq blocks.  Statuses end 0 or 3 (more than out_cap candidates);
+// each block's FIFO row then holds the query's whole candidate list in
+// discovery order, bit 31 set on the accepted ones.
+hipError_t launch_range_search_global(const SearchParams& P, hipStream_t s);
+// Indices q < nq whose status[q] is in the set `mask` (bit v: status v) into
+// list, their count into *count (one block; list order unspecified).
+hipError_t launch_status_list(const uint32_t* status, uint32_t nq, uint32_t mask, uint32_t* list, uint32_t* count,
+                              hipStream_t s);
 // Lift of the base rows (k_lift.hip): pass 1 per-row squared norms (binary64
 // bits), pass 2 rows h_0..h_d and the maximum lifted norm (atomic maximum on
 // the bits of *maxbits, which starts at 0), pass 3 the padding coordinate.
@@ -304,7 +319,7 @@ struct RelaxParams {
 };
 hipError_t launch_relax(const RelaxParams& P, hipStream_t s);
 
-// ---- device-wide primitives (k_prims.hip, hipCUB) ----------------------------------
+// ---- device-wide primitives (k_prims.hip, rocPRIM) ---------------------------------
 hipError_t prim_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t m, void* tmp,
                                    size_t& tmp_bytes, hipStream_t s);
 hipError_t prim_sort_pairs_u64_u32(const uint64_t* kin, uint64_t* kout, const uint32_t* vin,

@@ -160,6 +160,45 @@ def test_overflowing_queries_are_redone(env):
         assert np.array_equal(a["acc"][i, :c], b["acc"][i, :c])
 
 
+def test_batcher_redoes_overflowing_queries(env):
+    """A batcher whose tuned visited set is far too small for its beams: most
+    queries overflow inside the captured graph, are counted on the device and
+    rerun with the retry tier's areas; the answers equal the per-call
+    search's."""
+    import pyanqi
+    ctx = env["ctx"]
+    rng = np.random.default_rng(8)
+    nq = 3000
+    q = env["q"][:nq]
+    L = rng.integers(64, 257, nq).astype(np.uint32)
+    kidx = rng.integers(0, 3, nq).astype(np.uint32)
+    bat = ctx.batcher(env["ix"].handle, env["tables"], max_batch=nq, Lmax=256,
+                      tuning=pyanqi.SearchTuning(4, 512, 64))
+    off, ids = bat.run(q, L, kidx)
+    eoff, eids = expected(env, q, L, kidx)
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+    bat.close()
+
+
+def test_batcher_rejects_bad_table_index(env):
+    """A query whose table index is not below the batcher's table count is
+    refused at the boundary (usage error) before anything is launched; the
+    batcher keeps working afterwards."""
+    import pyanqi
+    ctx = env["ctx"]
+    q = env["q"][:8]
+    L = np.full(8, 64, np.uint32)
+    bat = ctx.batcher(env["ix"].handle, env["tables"], max_batch=8, Lmax=64)
+    bad = np.array([0, 1, 2, 3, 0, 0, 0, 0], np.uint32)
+    with pytest.raises(pyanqi.AnqiError) as e:
+        bat.run(q, L, bad)
+    assert e.value.code == 2 and "table index" in str(e.value)
+    off, ids = bat.run(q, L, np.zeros(8, np.uint32))
+    eoff, eids = expected(env, q, L, np.zeros(8, np.uint32))
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+    bat.close()
+
+
 def test_chosen_build_equals_plain(env):
     """The chosen build (bf16 MFMA leaf sweep, batched gathers) yields the
     plain build's index bit for bit: AKNN lists, cutoffs, LRQ codes, lifted
