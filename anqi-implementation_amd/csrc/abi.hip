@@ -13,12 +13,43 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/anqi.h"
 #include "common.hpp"
 #include "kernels.hpp"
+
+// ---- work area of the entry point that allocates nothing ---------------------------------------
+// anqi_rknn_search_noalloc takes its counters, batch arrays (entries, keys,
+// work order, redo list) and global-tier rows from this array instead of
+// hipMalloc.  Each device holds its own copy until the process ends.  Host
+// code never uses the symbol itself (on the host it names a stand-in, not
+// device memory): anqi_ctx_create, with the device current, records the
+// device address hipGetSymbolAddress returns, and every kernel argument and
+// fill derives from that address.  Calls that overlap on one device share it
+// and race.
+#define NOALLOC_BYTES (64ull << 20)             // 64 MiB per device
+#define NOALLOC_MAX_NQ 65536u                   // queries per call
+#define NOALLOC_ENT_OFF 256ull                  // counters first
+#define NOALLOC_GLOBAL_OFF (NOALLOC_ENT_OFF + 16ull * NOALLOC_MAX_NQ)   // then 4 words per query
+#define NOALLOC_MAX_DEV 64
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+static void* g_scratch_ptr[NOALLOC_MAX_DEV];
+static hipError_t g_scratch_err[NOALLOC_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+// Records the current device's scratch address (once per device).
+static void scratch_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEV) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
 
 struct anqi_ctx {
     int device = 0;
@@ -137,6 +168,7 @@ int anqi_ctx_create(int device, anqi_ctx** out) {
     if (e != hipSuccess) { delete ctx; return ANQI_E_DEVICE; }
     e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e != hipSuccess) { delete ctx; return ANQI_E_DEVICE; }
+    scratch_setup(device);
     ctx->stream = ctx->own;
     if (hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         ctx->cus <= 0) {
@@ -1440,6 +1472,43 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
     };
     return search_run(ctx, ix, queries, nq, L, Lq, Lmax, thr, kidx, entries, out_cap, cand_ids, accept, cand_cnt,
                       ndist, pred_cnt, nexp, status, mode, W, area);
+}
+
+int anqi_rknn_search_noalloc(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t L,
+                             const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx,
+                             uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt,
+                             uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp, uint32_t* status) {
+    if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
+    if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
+    if (nq == 0) return ANQI_OK;
+    if (out_cap == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "out_cap must be positive");
+    if (nq > NOALLOC_MAX_NQ)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "no-alloc search: %u queries exceed its limit of %u per call", nq,
+                            NOALLOC_MAX_NQ);
+    const size_t gbytes = NOALLOC_BYTES - NOALLOC_GLOBAL_OFF;
+    const size_t per = 4ull * global_stride(ix->n) + 4ull * ix->n;   // one global-tier row pair
+    if (per > gbytes)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "no-alloc search: an index of %u rows exceeds its limit of %llu rows",
+                            ix->n, (unsigned long long)(gbytes * 8 / 33));
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEV || !g_scratch_ptr[ctx->device])
+        return anqi_set_err(ctx, ANQI_E_DEVICE, "no-alloc search: no scratch address for device %d (HIP error %d)",
+                            ctx->device,
+                            ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEV ? (int)g_scratch_err[ctx->device] : -1);
+    uint8_t* base = (uint8_t*)g_scratch_ptr[ctx->device];
+    SearchWork W;
+    W.counter = (uint32_t*)base;
+    W.ent = (uint32_t*)(base + NOALLOC_ENT_OFF);
+    W.list = W.ent + 3ull * nq;
+    const uint32_t n = ix->n;
+    GlobalArea area = [&](uint32_t need, SearchWork& w) -> hipError_t {
+        const uint32_t slots = (uint32_t)std::max<size_t>(1, std::min<size_t>(need, gbytes / per));
+        w.gvis = (uint32_t*)(base + NOALLOC_GLOBAL_OFF);
+        w.gfifo = w.gvis + (size_t)slots * global_stride(n);
+        w.slots = slots;
+        return hipSuccess;
+    };
+    return search_run(ctx, ix, queries, nq, L, Lq, Lmax, thr, kidx, nullptr, out_cap, cand_ids, accept, cand_cnt,
+                      ndist, pred_cnt, nexp, status, 1, W, area);
 }
 
 }  // extern "C"

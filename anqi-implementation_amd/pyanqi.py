@@ -102,6 +102,8 @@ SIGS = {
     "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, P, I32]),
     "anqi_rknn_search_entries": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, P, U32, P, P, P, P, P, P,
                                        P, I32]),
+    "anqi_rknn_search_noalloc": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P,
+                                       P]),
     "anqi_batcher_create": (I32, [P, ctypes.POINTER(Index), P, U32, U32, U32, U32, ctypes.POINTER(SearchTuning),
                                   ctypes.POINTER(P)]),
     "anqi_batcher_run": (I32, [P, P, P, P, U32, P, P, U64, ctypes.POINTER(U64)]),
@@ -439,6 +441,15 @@ class Context:
         self.call("anqi_rknn_search_entries", ctypes.byref(ix), dp(queries), nq, L, dp(Lq), Lmax, dp(thr), dp(kidx),
                   dp(entries), b["cand"].shape[1], dp(b["cand"]), dp(b["acc"]), dp(b["cnt"]), dp(b["nd"]),
                   dp(b["pred"]), dp(b["nexp"]), dp(b["status"]), mode)
+        return b
+
+
+    def rknn_search_noalloc(self, ix, queries, L, thr, bufs, Lq=None, Lmax=0, kidx=None):
+        """anqi_rknn_search_noalloc into caller buffers (search_buffers)."""
+        b = bufs
+        self.call("anqi_rknn_search_noalloc", ctypes.byref(ix), dp(queries), queries.shape[0], L, dp(Lq), Lmax,
+                  dp(thr), dp(kidx), b["cand"].shape[1], dp(b["cand"]), dp(b["acc"]), dp(b["cnt"]), dp(b["nd"]),
+                  dp(b["pred"]), dp(b["nexp"]), dp(b["status"]))
         return b
 
 

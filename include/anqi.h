@@ -267,6 +267,23 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
                              uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
                              uint32_t* status, int mode);
 
+/* Persistent form of anqi_rknn_search (mode 1) that allocates nothing: its
+ * counters, per-query batch arrays, redo list and global-tier rows live in a
+ * 64 MiB work area the library holds on each device for the life of the
+ * process, so callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation) can
+ * search.  No counterpart in the reference (its query loop allocates per
+ * thread, src/rknn_query.cpp:2692-2824).  Same arguments, outputs and
+ * statuses as anqi_rknn_search with mode 1 and the context's search tuning.
+ * Limits: nq <= 65536 per call and n <= about 16.0M rows (one global-tier
+ * bitmap + FIFO row pair); larger inputs return ANQI_E_USAGE before anything
+ * runs.  Calls that overlap on one device share the work area and race. */
+int anqi_rknn_search_noalloc(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq,
+                             uint32_t L, const uint32_t* Lq, uint32_t Lmax, const float* thr,
+                             const uint32_t* kidx, uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept,
+                             uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
+                             uint32_t* status);
+
 /* Search-tree entry point of each query (device nq): the descent the search
  * runs (rptrees.h:108-123, ties go left), for anqi_rknn_search_entries. */
 int anqi_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t* entries);
