@@ -65,6 +65,7 @@ struct anqi_ctx {
     bool time_kernels = false;
     hipEvent_t kev0 = nullptr, kev1 = nullptr;
     double last_kernel_ms = 0.0;
+    uint32_t redo_retry = 0, redo_global = 0;   // last search: queries rerun per tier
 };
 
 static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
@@ -202,6 +203,13 @@ int anqi_ctx_time_kernels(anqi_ctx* ctx, int enable) {
 }
 
 double anqi_ctx_last_kernel_ms(const anqi_ctx* ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
+
+int anqi_ctx_redo_counts(const anqi_ctx* ctx, uint32_t* retry, uint32_t* global) {
+    if (!ctx) return ANQI_E_USAGE;
+    if (retry) *retry = ctx->redo_retry;
+    if (global) *global = ctx->redo_global;
+    return ANQI_OK;
+}
 
 const char* anqi_last_error(const anqi_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
@@ -1322,10 +1330,12 @@ static int status_list(anqi_ctx* ctx, const uint32_t* status, uint32_t nq, uint3
 // global tier, whose bitmap and FIFO cannot fill.  P: the first pass's
 // parameters (its entries array, if any, indexed by query).
 static int search_redo(anqi_ctx* ctx, const SearchParams& P, uint32_t nq, SearchWork& W, uint32_t gmask,
-                       const GlobalArea& area, const GlobalSink* sink, hipStream_t s) {
+                       const GlobalArea& area, const GlobalSink* sink, hipStream_t s, uint32_t* n_retry,
+                       uint32_t* n_global) {
     uint32_t cnt = 0;
     int rc = status_list(ctx, P.status, nq, (1u << 1) | (1u << 2), W, &cnt, s);
     if (rc) return rc;
+    *n_retry = cnt;
     SearchParams Q = P;
     Q.nq_dev = nullptr;
     Q.ovf_count = nullptr;
@@ -1338,6 +1348,7 @@ static int search_redo(anqi_ctx* ctx, const SearchParams& P, uint32_t nq, Search
         HCHK(launch_range_search(Q, false, 0, s));
     }
     rc = status_list(ctx, P.status, nq, gmask, W, &cnt, s);
+    *n_global = cnt;
     if (rc || cnt == 0) return rc;
     if (W.slots == 0) HCHK(area(cnt, W));
     Q.gvis = W.gvis;
@@ -1411,6 +1422,7 @@ static int search_run(anqi_ctx* ctx, const anqi_index* ix, const float* queries,
         HCHK(launch_range_search(P, false, 0, s));
     }
     if (ctx->time_kernels) HCHK(hipEventRecord(ctx->kev1, s));
+    ctx->redo_retry = ctx->redo_global = 0;
     // overflows are counted on the device: statuses are scanned only when
     // there is something to redo
     uint32_t novf = 0;
@@ -1423,7 +1435,8 @@ static int search_run(anqi_ctx* ctx, const anqi_index* ix, const float* queries,
     }
     if (novf == 0) return ANQI_OK;
     P.qlist = nullptr;
-    return search_redo(ctx, P, nq, W, (1u << 1) | (1u << 2), area, nullptr, s);
+    return search_redo(ctx, P, nq, W, (1u << 1) | (1u << 2), area, nullptr, s, &ctx->redo_retry,
+                       &ctx->redo_global);
 }
 
 extern "C" {
@@ -1538,6 +1551,7 @@ struct anqi_batcher {
     SearchParams P{};
     anqi_index ix{};
     uint32_t max_batch = 0, out_cap = 0, Lmax = 0, grid = 0, ntables = 0;
+    uint32_t redo_retry = 0, redo_global = 0;   // last run: queries rerun per tier
     DBuf q, Lq, kidx, nq, cand, acc, cnt, nd, pred, status, off, ids, counter, ent;
     float* hq = nullptr;
     uint32_t *hL = nullptr, *hk = nullptr, *hnq = nullptr, *hstatus = nullptr;
@@ -1662,6 +1676,13 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
 
 void anqi_batcher_destroy(anqi_batcher* b) { delete b; }
 
+int anqi_batcher_redo_counts(const anqi_batcher* b, uint32_t* retry, uint32_t* global) {
+    if (!b) return ANQI_E_USAGE;
+    if (retry) *retry = b->redo_retry;
+    if (global) *global = b->redo_global;
+    return ANQI_OK;
+}
+
 int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_L, const uint32_t* h_kidx,
                      uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total) {
     anqi_ctx* ctx = b->ctx;
@@ -1682,6 +1703,7 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
     HCHK(hipMemcpyAsync(b->Lq.p, b->hL, 4ull * nq, hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(b->kidx.p, b->hk, 4ull * nq, hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(b->nq.p, b->hnq, 4, hipMemcpyHostToDevice, s));
+    b->redo_retry = b->redo_global = 0;
     HCHK(hipGraphLaunch(b->exec, s));
     HCHK(hipMemcpyAsync(b->hstatus, b->counter.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s));
     HCHK(hipMemcpyAsync(h_off, b->off.p, 4ull * (nq + 1), hipMemcpyDeviceToHost, s));
@@ -1733,7 +1755,8 @@ int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_
         };
         // tiers: 16K-entry LDS areas, then the global tier for what is left
         // and for candidate lists longer than out_cap
-        int rc = search_redo(ctx, Q, nq, W, (1u << 1) | (1u << 2) | (1u << 3), area, &sink, s);
+        int rc = search_redo(ctx, Q, nq, W, (1u << 1) | (1u << 2) | (1u << 3), area, &sink, s, &b->redo_retry,
+                             &b->redo_global);
         if (rc) return rc;
         HCHK(launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(), s));
         HCHK(launch_batch_compact(b->cand.as<uint32_t>(), b->acc.as<uint8_t>(), b->cnt.as<uint32_t>(),

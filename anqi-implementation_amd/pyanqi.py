@@ -71,6 +71,8 @@ SIGS = {
     "anqi_ctx_time_kernels": (I32, [P, I32]),
     "anqi_ctx_last_kernel_ms": (F64, [P]),
     "anqi_ctx_set_search_tuning": (I32, [P, P]),
+    "anqi_ctx_redo_counts": (I32, [P, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
+    "anqi_batcher_redo_counts": (I32, [P, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
     "anqi_dmalloc": (I32, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
     "anqi_dfree": (I32, [P, P]),
     "anqi_h2d": (I32, [P, P, P, ctypes.c_size_t]),
@@ -334,6 +336,12 @@ class Context:
                   dp(kidx), Lmax, ctypes.byref(t), ctypes.byref(ms))
         return t, ms.value
 
+    def redo_counts(self):
+        """(retry tier, global tier) queries of the last search call."""
+        r, g = U32(), U32()
+        self.call("anqi_ctx_redo_counts", ctypes.byref(r), ctypes.byref(g))
+        return r.value, g.value
+
     def set_search_tuning(self, tuning=None):
         self.call("anqi_ctx_set_search_tuning", ctypes.byref(tuning) if tuning is not None else None)
 
@@ -499,6 +507,12 @@ class Batcher:
             self.ctx.check(rc)
             break
         return off, self.ids[:int(tot.value)].copy()
+
+    def redo_counts(self):
+        """(retry tier, global tier) queries of the last run."""
+        r, g = U32(), U32()
+        self.ctx.check(self.ctx.L.anqi_batcher_redo_counts(self.h, ctypes.byref(r), ctypes.byref(g)))
+        return r.value, g.value
 
     def close(self):
         if self.h:

@@ -17,6 +17,7 @@ Workloads (sizes as used by the tests and the benchmark):
 from __future__ import annotations
 
 import hashlib
+import os
 import struct
 
 import numpy as np
@@ -74,11 +75,15 @@ def mixture_centres(centres: int, d: int, seed: int = 0) -> np.ndarray:
 
 
 def mixture(n: int, d: int, centres: int, sigma: float, seed: int,
-            centre_seed: int = 0, chunk: int = 1 << 16, rounded: bool = True) -> np.ndarray:
-    """n rows of clip(round(c_u + sigma * N(0, I)), 0, 255), u uniform."""
+            centre_seed: int = 0, chunk: int = 1 << 16, rounded: bool = True, threads: int = 0) -> np.ndarray:
+    """n rows of clip(round(c_u + sigma * N(0, I)), 0, 255), u uniform.
+    Chunks are independent counter ranges, so they are generated on a thread
+    pool (threads 0: min(16, cores)); the bytes do not depend on it."""
+    from concurrent.futures import ThreadPoolExecutor
     c = mixture_centres(centres, d, centre_seed)
     out = np.empty((n, d), dtype=np.float32)
-    for s in range(0, n, chunk):
+
+    def fill(s):
         e = min(n, s + chunk)
         pick = (u64(seed, 3, s, e - s) % np.uint64(centres)).astype(np.int64)
         z = normal(seed, 4, s * d, (e - s) * d).reshape(e - s, d)
@@ -86,6 +91,10 @@ def mixture(n: int, d: int, centres: int, sigma: float, seed: int,
         if rounded:
             v = np.rint(v)
         out[s:e] = (np.clip(v, 0.0, 255.0) + 0.0).astype(np.float32)
+
+    workers = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(fill, range(0, n, chunk)))
     return out
 
 

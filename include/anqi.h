@@ -49,6 +49,13 @@ int anqi_ctx_reset_stream(anqi_ctx* ctx);
  * milliseconds.  No counterpart in the reference (measurement support). */
 int anqi_ctx_time_kernels(anqi_ctx* ctx, int enable);
 double anqi_ctx_last_kernel_ms(const anqi_ctx* ctx);
+/* Queries of the last anqi_rknn_search* call on the context that were rerun
+ * because they outgrew their LDS work areas: with the 16K-entry retry areas
+ * (*retry) and then in the global-memory tier (*global; with the per-query
+ * visited bitmap and FIFO in HBM, as unbounded as the reference's flood,
+ * include/index.h:33-52).  Either pointer may be NULL.  No counterpart in
+ * the reference (measurement support). */
+int anqi_ctx_redo_counts(const anqi_ctx* ctx, uint32_t* retry, uint32_t* global);
 void* anqi_ctx_stream(anqi_ctx* ctx);
 /* Work areas and grid of the persistent search (mode 1 of anqi_rknn_search
  * and the batcher); anqi_search_autotune measures candidates.  Zero fields
@@ -309,6 +316,9 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
 int anqi_batcher_run(anqi_batcher* b, const float* h_queries, const uint32_t* h_L, const uint32_t* h_kidx,
                      uint32_t nq, uint32_t* h_off, uint32_t* h_ids, uint64_t ids_cap, uint64_t* total);
 void anqi_batcher_destroy(anqi_batcher* b);
+/* The same counts for the batcher's last run (the global tier also takes the
+ * queries whose candidates exceed out_cap; their answers are complete). */
+int anqi_batcher_redo_counts(const anqi_batcher* b, uint32_t* retry, uint32_t* global);
 /* Times the persistent search on a sample batch (device queries, Lq, kidx)
  * for visited-set sizes {1536, 2048, 3072, 3584, 4096, 8192}, LDS FIFO sizes {128, 512} and
  * 1/2, 3/4, 4/4 of the resident waves per CU; queries that overflow are
