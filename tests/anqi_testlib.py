@@ -234,6 +234,17 @@ def or_tree(tree):
     return t
 
 
+def trainer_cutoffs(n=3000, seed=77):
+    """Synthetic AKNN cutoff matrix for the trainer fixture (tests/golden/
+    trainer3k): rows ascending over the 100 ranks, an offset, a per-row scale
+    and squared-uniform increments, all from the counter-based generator."""
+    import workload
+    u = workload.uniform(seed, 9, 0, n * 100).reshape(n, 100)
+    scale = 20.0 + 200.0 * workload.uniform(seed, 10, 0, n)[:, None]
+    off = 100.0 * workload.uniform(seed, 11, 0, n)[:, None]
+    return (off + np.cumsum(scale * u * u, axis=1)).astype(np.float32)
+
+
 class Sample:
     """The 256 x 128 reference fixtures."""
 

@@ -13,11 +13,15 @@ nothing else).  Outputs are data only (inputs and expected outputs):
             verifier artifact, rknn_query stdout for k=10/50/100, per-query
             range-search candidate lists at L=16,32,64,128, materialized
             thresholds and postfilter decisions, distance bit vectors.
+  trainer3k/ the reference trainer's radbasis and codebook for the synthetic
+            3,000 x 100 cutoff matrix of anqi_testlib.trainer_cutoffs, with a
+            2,000-row training sample (exercises the seeded row sampling and
+            quantile interpolation that the 256-row sample cannot).
   mix20k/   first 20,000 rows of the million-row mixture + 200 queries:
             reference recall numbers only (AKNN recall vs exact, rknn_query
             recall/precision per L at k=10) -- the recall-parity targets.
 
-Usage: python3 tests/golden/make_golden.py [--only sample|mix20k|mix1m] [--work DIR]
+Usage: python3 tests/golden/make_golden.py [--only sample|trainer|mix20k|mix1m] [--work DIR]
 """
 from __future__ import annotations
 
@@ -246,6 +250,28 @@ def make_sample(out_dir):
     print("[golden] sample fixtures written to", out_dir)
 
 
+def make_trainer(out_dir, n=3000, train_sample=2000):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from anqi_testlib import trainer_cutoffs
+    os.makedirs(out_dir, exist_ok=True)
+    keys = trainer_cutoffs(n)
+    with tempfile.TemporaryDirectory() as work:
+        pfx = os.path.join(work, "tr")
+        workload.write_fbin(pfx + "_base.bin", np.zeros((n, 1), np.float32))   # the trainer reads n from it
+        rec = np.zeros((n, 100), dtype=np.dtype([("id", "<u4"), ("key", "<f4")]))
+        rec["id"] = (np.arange(n)[:, None] + 1 + np.arange(100)[None, :]) % n
+        rec["key"] = keys
+        rec.tofile(pfx + "_baseknn_aknn.rec")
+        run([sys.executable, os.path.join(REF_SRC, "tools", "train_final_m8_u8.py"), "--pfx", pfx,
+             "--train-sample", str(train_sample), "--codebook-workers", "1"])
+        for suf in ("_radbasis_M8.bin", "_learned_rq_codebook_M8_u8.bin"):
+            shutil.copy(pfx + suf, os.path.join(out_dir, "trainer3k" + suf))
+    with open(os.path.join(out_dir, "trainer3k.json"), "w") as f:
+        json.dump({"n": n, "train_sample": train_sample, "cutoffs": "anqi_testlib.trainer_cutoffs(3000, 77)",
+                   "cutoffs_sha256": workload.sha256(keys)}, f, indent=1)
+    print("[golden] trainer fixture written to", out_dir)
+
+
 def make_mix(out_dir, n=20000, nq=200):
     os.makedirs(out_dir, exist_ok=True)
     base = workload.million_base(n)
@@ -306,13 +332,15 @@ def make_mix1m(out_dir, work_root=None):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", choices=["sample", "mix20k", "mix1m"])
+    ap.add_argument("--only", choices=["sample", "trainer", "mix20k", "mix1m"])
     ap.add_argument("--work", default=None, help="scratch directory for the 1M run")
     a = ap.parse_args()
     if not os.path.exists(os.path.join(REF_DIR, "libref_harness.so")):
         raise SystemExit("build the reference first: make -C oracle ref")
     if a.only in (None, "sample"):
         make_sample(os.path.join(HERE, "sample"))
+    if a.only in (None, "trainer"):
+        make_trainer(os.path.join(HERE, "trainer3k"))
     if a.only in (None, "mix20k"):
         make_mix(os.path.join(HERE, "mix20k"))
     if a.only == "mix1m":

@@ -2,7 +2,9 @@
 1,000,000 x 128 mixture (10,000 queries, k = 10) against the reference's own
 pipeline on the same inputs (tests/golden/mix1m, reference tools compiled from
 the reference sources): recall and precision within 0.5 points at every
-swept L, and the lift meta within rounding of the reference's."""
+swept L, mean #dist within 1 % and mean candidates within 1 of the
+reference's printed values, and the lift meta within rounding of the
+reference's."""
 import json
 import os
 
@@ -43,5 +45,11 @@ def test_recall_precision_per_L_match_reference():
             f"({p['precision']:.4f}), cand {m['cand']:.1f} ({p['cand']}), dist {m['dist']:.0f} ({p['dist']})")
         assert abs(m["recall"] - p["recall"]) <= 0.005, p["L"]
         assert abs(m["precision"] - p["precision"]) <= 0.005, p["L"]
+        # the reference prints mean candidates and mean #dist (graph + verify
+        # distances) rounded to integers; the two graphs come from different
+        # (nondeterministic) AKNN sources, so the work per query is held to
+        # 1 % of the reference's, the candidates to its rounding
+        assert abs(m["dist"] - p["dist"]) <= 0.01 * p["dist"], p["L"]
+        assert abs(m["cand"] - p["cand"]) <= 1.0, p["L"]
     ctx.tree_free(ix.tree)
     ctx.close()
