@@ -66,6 +66,8 @@ struct anqi_ctx {
     hipEvent_t kev0 = nullptr, kev1 = nullptr;
     double last_kernel_ms = 0.0;
     uint32_t redo_retry = 0, redo_global = 0;   // last search: queries rerun per tier
+    int lrq_form = 0;                           // 0 certified float-float, 1 soft binary64
+    unsigned long long* lrq_fb = nullptr;       // device: encode / decode fallback counts
 };
 
 static int anqi_set_err(anqi_ctx* ctx, int code, const char* fmt, ...) {
@@ -186,6 +188,7 @@ void anqi_ctx_destroy(anqi_ctx* ctx) {
     if (ctx->own) { (void)hipStreamSynchronize(ctx->own); (void)hipStreamDestroy(ctx->own); }
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->counter) (void)hipFree(ctx->counter);
+    if (ctx->lrq_fb) (void)hipFree(ctx->lrq_fb);
     if (ctx->ent) (void)hipFree(ctx->ent);
     if (ctx->kev0) (void)hipEventDestroy(ctx->kev0);
     if (ctx->kev1) (void)hipEventDestroy(ctx->kev1);
@@ -345,10 +348,28 @@ int anqi_rknn_membership(anqi_ctx* ctx, const float* base, uint32_t n, const flo
 }
 
 // ---- verifier ------------------------------------------------------------------------------
+// Fallback counter slot (0 encode, 1 decode) of the certified forms, zeroed.
+static int lrq_counter(anqi_ctx* ctx, int slot, unsigned long long** out) {
+    if (!ctx->lrq_fb) {
+        HCHK(hipMalloc((void**)&ctx->lrq_fb, 2 * sizeof(unsigned long long)));
+        HCHK(launch_fill_u32((uint32_t*)ctx->lrq_fb, 0u, 4, ctx->stream));
+    }
+    HCHK(launch_fill_u32((uint32_t*)(ctx->lrq_fb + slot), 0u, 2, ctx->stream));
+    *out = ctx->lrq_fb + slot;
+    return ANQI_OK;
+}
+
 int anqi_lrq_encode(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm,
                     const float* mu, const float* basis, const float* codebook, uint32_t code_size,
                     float* coeff, uint8_t* codes, uint8_t* codes_kmajor) {
     if (code_size != 256) return anqi_set_err(ctx, ANQI_E_INPUT, "only u8 codebooks (256 levels) are supported");
+    if (ctx->lrq_form == 0 && lrq_fast_supported(nk, rankm, code_size)) {
+        unsigned long long* fb = nullptr;
+        if (int rc = lrq_counter(ctx, 0, &fb)) return rc;
+        HCHK(launch_lrq_encode_fast(cutoffs, n, nk, rankm, mu, basis, codebook, coeff, codes, codes_kmajor, fb,
+                                    ctx->stream));
+        return ANQI_OK;
+    }
     HCHK(launch_lrq_encode(cutoffs, n, nk, rankm, mu, basis, codebook, code_size, coeff, codes, codes_kmajor,
                            ctx->stream));
     return ANQI_OK;
@@ -359,16 +380,50 @@ int anqi_transpose_codes(anqi_ctx* ctx, const uint8_t* codes, uint64_t n, uint32
     return ANQI_OK;
 }
 
+int anqi_lrq_materialize_multi(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
+                               uint32_t nk, uint32_t rankm, const float* mu, const float* basis,
+                               const float* codebook, uint32_t code_size, const uint32_t* ks, uint32_t nks,
+                               double slack, float* thr) {
+    for (uint32_t t = 0; t < nks; t++)
+        if (ks[t] < 1 || ks[t] > nk) return anqi_set_err(ctx, ANQI_E_USAGE, "LRQ verifier is not ready for k=%u", ks[t]);
+    if (nks == 0 || n == 0) return ANQI_OK;
+    if (ctx->lrq_form == 0 && slack == 1.0 && lrq_fast_supported(nk, rankm, code_size)) {
+        unsigned long long* fb = nullptr;
+        if (int rc = lrq_counter(ctx, 1, &fb)) return rc;
+        HCHK(launch_lrq_decode(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, ks, nks, thr, fb,
+                               ctx->stream));
+        return ANQI_OK;
+    }
+    uint64_t sb;
+    std::memcpy(&sb, &slack, 8);
+    for (uint32_t t = 0; t < nks; t++)
+        HCHK(launch_lrq_materialize(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, ks[t], sb,
+                                    thr + (uint64_t)t * n, ctx->stream));
+    return ANQI_OK;
+}
+
 int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
                          uint32_t nk, uint32_t rankm, const float* mu, const float* basis,
                          const float* codebook, uint32_t code_size, uint32_t k_one_based, double slack,
                          float* thr) {
-    if (k_one_based < 1 || k_one_based > nk)
-        return anqi_set_err(ctx, ANQI_E_USAGE, "LRQ verifier is not ready for k=%u", k_one_based);
-    uint64_t sb;
-    std::memcpy(&sb, &slack, 8);
-    HCHK(launch_lrq_materialize(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, k_one_based,
-                                sb, thr, ctx->stream));
+    return anqi_lrq_materialize_multi(ctx, coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size,
+                                      &k_one_based, 1, slack, thr);
+}
+
+int anqi_ctx_set_lrq_form(anqi_ctx* ctx, int form) {
+    if (form != 0 && form != 1) return anqi_set_err(ctx, ANQI_E_USAGE, "lrq form: 0 (certified) or 1 (soft)");
+    ctx->lrq_form = form;
+    return ANQI_OK;
+}
+
+int anqi_ctx_lrq_fallbacks(anqi_ctx* ctx, uint64_t* encode, uint64_t* decode) {
+    unsigned long long h[2] = {0, 0};
+    if (ctx->lrq_fb) {
+        HCHK(hipMemcpyAsync(h, ctx->lrq_fb, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        HCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (encode) *encode = h[0];
+    if (decode) *decode = h[1];
     return ANQI_OK;
 }
 

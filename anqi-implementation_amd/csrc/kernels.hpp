@@ -103,6 +103,19 @@ hipError_t launch_lrq_materialize(const float* coeff, const uint8_t* codes_kmajo
                                   hipStream_t s);
 hipError_t launch_transpose_codes(const uint8_t* codes, uint64_t n, uint32_t nk,
                                   uint8_t* codes_kmajor, hipStream_t s);
+// FP32 float-float forms with certified rounding and soft-binary64 fallback
+// for the rare uncertified element (same bits as the soft forms above);
+// fallbacks (optional, device) counts the recomputed elements.
+#define LRQ_MAX_TABLES 8
+bool lrq_fast_supported(uint32_t nk, uint32_t rankm, uint32_t code_size);
+hipError_t launch_lrq_encode_fast(const float* cut, uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
+                                  const float* basis, const float* cb, float* coeff, uint8_t* codes,
+                                  uint8_t* codes_kmajor, unsigned long long* fallbacks, hipStream_t s);
+// Thresholds of nks ranks (host list, one-based) into thr (nks x n), slack 1.
+hipError_t launch_lrq_decode(const float* coeff, const uint8_t* codes_kmajor, uint64_t n, uint32_t nk,
+                             uint32_t rankm, const float* mu, const float* basis, const float* cb,
+                             uint32_t code_size, const uint32_t* k_one_based, uint32_t nks, float* thr,
+                             unsigned long long* fallbacks, hipStream_t s);
 
 // ---- distances (k_dist.hip) ---------------------------------------------------
 hipError_t launch_norms(const float* x, uint64_t n, uint32_t d, float* out, hipStream_t s);

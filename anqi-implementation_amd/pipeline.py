@@ -95,6 +95,11 @@ def thresholds(ctx, ix, k, slack=1.0):
     return ctx.lrq_materialize(ix.coeff, ix.codes_km, ix.mu, ix.G, ix.cb, k, slack)
 
 
+def threshold_tables(ctx, ix, ks, slack=1.0):
+    """One table per k in ks (len(ks) x n), decoded in one pass over the rows."""
+    return ctx.lrq_materialize_multi(ix.coeff, ix.codes_km, ix.mu, ix.G, ix.cb, list(ks), slack)
+
+
 def ground_truth(ctx, base, queries, k):
     """Exact RkNN CSR (tools/dataset_gt.cpp): exact top-k thresholds, then
     membership key <= r_k."""

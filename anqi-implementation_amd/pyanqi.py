@@ -86,6 +86,9 @@ SIGS = {
     "anqi_aknn_build": (I32, [P, P, U32, U32, ctypes.POINTER(AknnParams), P, P, ctypes.POINTER(AknnStats)]),
     "anqi_lrq_encode": (I32, [P, P, U64, U32, U32, P, P, P, U32, P, P, P]),
     "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
+    "anqi_lrq_materialize_multi": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, P, U32, F64, P]),
+    "anqi_ctx_set_lrq_form": (I32, [P, I32]),
+    "anqi_ctx_lrq_fallbacks": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lift": (I32, [P, P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
@@ -247,6 +250,25 @@ class Context:
         self.call("anqi_lrq_encode", dp(cut), n, nk, rankm, dp(mu), dp(basis), dp(cb), cb.shape[1], dp(coeff),
                   dp(codes), dp(codes_km))
         return coeff, codes, codes_km
+
+    def lrq_materialize_multi(self, coeff, codes_km, mu, basis, cb, ks, slack=1.0, out=None):
+        """Thresholds for every k in ks, one pass (anqi_lrq_materialize_multi): (len(ks), n)."""
+        n, rankm = coeff.shape
+        kh = np.ascontiguousarray(ks, np.uint32)
+        out = out if out is not None else self.empty((len(kh), n), self.torch.float32)
+        self.call("anqi_lrq_materialize_multi", dp(coeff), dp(codes_km), n, mu.shape[0], rankm, dp(mu), dp(basis),
+                  dp(cb), cb.shape[1], hp(kh), len(kh), slack, dp(out))
+        return out
+
+    def set_lrq_form(self, form):
+        """0: certified float-float (default); 1: soft binary64 throughout."""
+        self.call("anqi_ctx_set_lrq_form", form)
+
+    def lrq_fallbacks(self):
+        """(encode, decode) elements the last calls recomputed in soft binary64."""
+        e, d = U64(), U64()
+        self.call("anqi_ctx_lrq_fallbacks", ctypes.byref(e), ctypes.byref(d))
+        return e.value, d.value
 
     def transpose_codes(self, codes):
         n, nk = codes.shape

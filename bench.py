@@ -138,6 +138,81 @@ def load_traffic(kernel, workload_tag):
         return json.load(f)
 
 
+def stage_lrq(ctx, a):
+    """Verifier kernels on the 1M x 128 mixture's AKNN cutoffs: encode
+    (coefficients + codes) and the threshold decode for k = 10 and for
+    k = 10, 50, 100 in one pass, HIP-event timed on torch's stream, with
+    algorithmic bytes (encode: 400 B cutoffs read, 32 B coefficients and
+    2 x 100 B codes written per row; decode: 32 B coefficients + 1 code byte
+    read and 4 B written per row and rank, the coefficients once)."""
+    import torch
+    import lrq_fit
+    import workload
+    if a.n <= 1_000_000:
+        base = workload.million_base(a.n)
+        x = ctx.to_dev(base)
+        ids, keys, _ = ctx.aknn_build(x, K=100, leaf=500, trees=64, ef=600, precision=1)
+        cut = torch.clamp(keys, min=0.0).contiguous()
+        del ids, keys, x
+        source = "AKNN cutoffs of the 1M x 128 mixture (first n rows)"
+    else:
+        # larger n: radius-like synthetic rows (ascending, 2e4..1e5), seed 5
+        g = torch.Generator(device="cuda:0").manual_seed(5)
+        cut = (torch.rand((a.n, 1), device="cuda:0", generator=g) * 4e4 + 2e4 +
+               torch.cumsum(torch.rand((a.n, 100), device="cuda:0", generator=g) * 400, dim=1)).float().contiguous()
+        source = "synthetic radius-like rows (torch generator seed 5)"
+    mu, G, cb, _ = lrq_fit.fit(cut[:1_000_000].contiguous(), log=lambda *z: None)
+    mu, G, cb = ctx.to_dev(mu), ctx.to_dev(G), ctx.to_dev(cb)
+    n, nk = cut.shape
+    reps = a.steps
+
+    def timed(fn):
+        """Mean GPU time of one call: events around each call (host launch
+        gaps between calls are not counted; the call's counter reset is)."""
+        fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for e0, e1 in ev:
+            e0.record()
+            fn()
+            e1.record()
+        torch.cuda.synchronize()
+        return sum(e0.elapsed_time(e1) for e0, e1 in ev) / reps
+
+    out = {"stage": "lrq", "n": n, "nk": nk, "reps": reps, "cutoffs": source}
+    res = {}
+
+    def enc():
+        res["enc"] = ctx.lrq_encode(cut, mu, G, cb)
+
+    ms = timed(enc)
+    coeff, codes, codes_km = res["enc"]
+    byts = n * (4 * nk + 32 + 2 * nk)
+    out["encode"] = {"ms": ms, "bytes": byts, "GBps": byts / ms / 1e6, "frac": byts / ms / 1e6 / HBM_PEAK_GBS,
+                     "soft_fallbacks": ctx.lrq_fallbacks()[0]}
+    for ks in ((10,), (10, 50, 100)):
+        thr = ctx.empty((len(ks), n), torch.float32)
+        ms = timed(lambda: ctx.lrq_materialize_multi(coeff, codes_km, mu, G, cb, list(ks), out=thr))
+        byts = n * (32 + 5 * len(ks))
+        out["decode_k" + "_".join(map(str, ks))] = {
+            "ms": ms, "bytes": byts, "GBps": byts / ms / 1e6, "frac": byts / ms / 1e6 / HBM_PEAK_GBS,
+            "soft_fallbacks": ctx.lrq_fallbacks()[1]}
+    # the all-soft binary64 form on the same inputs, for the record
+    ctx.set_lrq_form(1)
+    try:
+        ms = timed(enc)
+        same = all(torch.equal(u, v) for u, v in zip(res["enc"], (coeff, codes, codes_km)))
+        out["encode_soft_form"] = {"ms": ms, "same_output": bool(same)}
+        thr1 = ctx.empty((1, n), torch.float32)
+        ms = timed(lambda: ctx.lrq_materialize_multi(coeff, codes_km, mu, G, cb, [10], out=thr1))
+        out["decode_k10_soft_form"] = {"ms": ms}
+    finally:
+        ctx.set_lrq_form(0)
+    thr_c = ctx.lrq_materialize_multi(coeff, codes_km, mu, G, cb, [10])
+    out["decode_k10_soft_form"]["same_output"] = bool(torch.equal(thr_c, thr1))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +226,8 @@ def main():
     ap.add_argument("--no-plain-build", action="store_true", help="skip the plain-form build timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--detail", default="")
+    ap.add_argument("--stage", choices=["lrq"], default="",
+                    help="time one stage's kernels instead of the headline measurement (prints its own JSON)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 1 GiB device-to-device copy (counter calibration)")
     a = ap.parse_args()
@@ -167,6 +244,9 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     ctx = pyanqi.Context(0)
+    if a.stage:
+        print(json.dumps({"lrq": stage_lrq}[a.stage](ctx, a)), flush=True)
+        return
     t = time.perf_counter()
     base = workload.million_base(a.n)
     queries = workload.million_query(a.nq)

@@ -130,6 +130,23 @@ int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes
                          uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
                          const float* basis, const float* codebook, uint32_t code_size,
                          uint32_t k_one_based, double slack, float* thr);
+/* The same for nks ranks ks[0..nks) (host, one-based) in one pass over the
+ * rows: thr is device nks x n, table t for rank ks[t] (e.g. the batcher's
+ * k = 10, 50, 100 tables).  materialize_k once per rank in the reference
+ * (src/rknn_query.cpp:722-753). */
+int anqi_lrq_materialize_multi(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor,
+                               uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
+                               const float* basis, const float* codebook, uint32_t code_size,
+                               const uint32_t* ks, uint32_t nks, double slack, float* thr);
+/* Arithmetic of the verifier's encode and decode on this context: 0
+ * (default) FP32 float-float sums whose rounding to float is certified
+ * against the binary64 evaluation's error bound, soft binary64 only for the
+ * elements it cannot certify; 1 soft binary64 for every element.  Both give
+ * the reference's bits (src/rknn_query.cpp:679-753); form 1 exists to check
+ * that.  anqi_ctx_lrq_fallbacks reports how many elements the last encode
+ * and the last decode recomputed in soft binary64 (form 0). */
+int anqi_ctx_set_lrq_form(anqi_ctx* ctx, int form);
+int anqi_ctx_lrq_fallbacks(anqi_ctx* ctx, uint64_t* encode, uint64_t* decode);
 
 /* ---- relaxed verifier fit (tools/train_final_m8_u8.py:81-156 objective) --
  * Radius reconstruction with the codeword choice relaxed to a softmax over

@@ -24,7 +24,7 @@ def env():
     ctx = pyanqi.Context(0)
     base = workload.mixture(30000, 96, 120, 15.0, seed=21)
     ix = pipeline.build(ctx, base, log=say)
-    tables = torch.stack([pipeline.thresholds(ctx, ix, k) for k in (10, 50, 100)]).contiguous()
+    tables = pipeline.threshold_tables(ctx, ix, (10, 50, 100))
     q = workload.mixture(20000, 96, 120, 15.0, seed=22)
     yield dict(ctx=ctx, ix=ix, tables=tables, q=q, torch=torch)
     ctx.tree_free(ix.tree)

@@ -45,7 +45,7 @@ def env():
     base = workload.mixture(N, D, CENTRES, SIGMA, seed=0, rounded=False)
     say(f"base {N} x {D} generated in {time.perf_counter() - t:.1f}s")
     ix = pipeline.build(ctx, base, precision=1, log=say)
-    tables = torch.stack([pipeline.thresholds(ctx, ix, k) for k in KS]).contiguous()
+    tables = pipeline.threshold_tables(ctx, ix, KS)
     t = time.perf_counter()
     q = workload.mixture(2_000_000, D, CENTRES, SIGMA, seed=1, rounded=False)
     say(f"2,000,000 requests generated in {time.perf_counter() - t:.1f}s")

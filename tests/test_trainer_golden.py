@@ -6,7 +6,7 @@ the synthetic 3,000 x 100 cutoff matrix with a 2,000-row training sample
 The mean and the codebook must be bit-identical; each basis row must equal
 the reference's row or its negation bit for bit (an eigenvector's sign is
 arbitrary, and a flipped row flips its coefficients too, so the decoded
-radii are the same).  Runs on the CPU (torch FP64)."""
+radii are the same).  Runs on the CPU (torch FP64, one thread)."""
 import json
 import os
 
@@ -17,9 +17,17 @@ from anqi_testlib import GOLDEN, read_basis, read_codebook, read_rec, trainer_cu
 
 
 def fit_cpu(cut, **kw):
+    """The fit on the CPU with one torch thread: its FP64 column sums and
+    products then add in numpy's order (row after row), which is what makes
+    the low bits equal; more threads split those sums and change them."""
     import torch
     import lrq_fit
-    return lrq_fit.fit(torch.from_numpy(np.ascontiguousarray(cut, np.float32)), log=lambda *a: None, **kw)
+    prev = torch.get_num_threads()
+    torch.set_num_threads(1)
+    try:
+        return lrq_fit.fit(torch.from_numpy(np.ascontiguousarray(cut, np.float32)), log=lambda *a: None, **kw)
+    finally:
+        torch.set_num_threads(prev)
 
 
 def check(mu, G, cb, prefix):
