@@ -180,6 +180,7 @@ struct PointStats {
     double sum_recall = 0, sum_prec = 0, sum_f1 = 0, sum_candidate_recall = 0;
     double sum_cand = 0, sum_pred = 0, sum_ndist = 0, sum_graph_ndist = 0, sum_verify_ndist = 0;
     size_t nonempty = 0, tasks = 0, batches = 0;
+    double search_secs = 0;
     double secs = 0;
 };
 
@@ -344,9 +345,9 @@ int main(int argc, char** argv) {
         d_aknn = dalloc<uint32_t>((size_t)n * 100);
         d_keys = dalloc<float>((size_t)n * 100);
         ck(anqi_aknn_build(g_ctx, d_base, n, d, &ap, d_aknn, d_keys, &st), "AKNN source");
+        // .rec keys: max(0, d2) (tools/build_aknn_source.cpp:119-137)
+        ck(anqi_aknn_cutoffs(g_ctx, d_keys, n, 100, d_keys, nullptr), "AKNN cutoffs");
         download(aknn_keys, d_keys, (size_t)n * 100);
-        for (auto& k : aknn_keys) k = std::max(0.0f, k);
-        ck(anqi_h2d(g_ctx, d_keys, aknn_keys.data(), aknn_keys.size() * 4), "upload");
         std::printf("[rknn] built original-space AKNN on the GPU (n=%u K=100 in %.2fs)\n", n, st.total_s);
     }
     std::printf("[rknn] original-space AKNN warm start ready: rows=%u source=%s\n", n,
@@ -490,9 +491,15 @@ int main(int argc, char** argv) {
         d_acc = dalloc<uint8_t>((size_t)nq * cap);
     };
     alloc_out(out_cap);
-    std::vector<uint32_t> cand, cnt, nd;
-    std::vector<uint8_t> acc;
-    // one batch: search + post-filter on the device, results on the host
+    // candidates on the host as CSR: coff (nq + 1), cids / cacc (coff[nq])
+    std::vector<uint32_t> cnt, nd, coff, cids;
+    std::vector<uint8_t> cacc;
+    uint32_t* d_coff = dalloc<uint32_t>((size_t)nq + 1);
+    uint32_t* d_cids = nullptr;
+    uint8_t* d_cacc = nullptr;
+    uint64_t ccap = 0;
+    // one batch: search + post-filter on the device; the candidates (ids and
+    // accept bytes) compacted on the device and copied to the host
     auto search = [&](size_t L) {
         for (;;) {
             ck(anqi_rknn_search(g_ctx, &ix, d_query, nq, (uint32_t)L, nullptr, (uint32_t)L, d_thr, nullptr, out_cap,
@@ -505,8 +512,20 @@ int main(int argc, char** argv) {
             alloc_out(mx + 1024);
         }
         download(nd, d_nd, nq);
-        download(cand, d_cand, (size_t)nq * out_cap);
-        download(acc, d_acc, (size_t)nq * out_cap);
+        uint64_t total = 0;
+        for (uint32_t c : cnt) total += c;
+        if (total > ccap) {
+            if (d_cids) anqi_dfree(g_ctx, d_cids);
+            if (d_cacc) anqi_dfree(g_ctx, d_cacc);
+            ccap = total + total / 4 + 1024;
+            d_cids = dalloc<uint32_t>(ccap);
+            d_cacc = dalloc<uint8_t>(ccap);
+        }
+        ck(anqi_compact_candidates(g_ctx, nq, out_cap, d_cand, d_acc, d_cnt, d_coff, d_cids, d_cacc, ccap, &total),
+           "candidate compaction");
+        download(coff, d_coff, (size_t)nq + 1);
+        download(cids, d_cids, total);
+        download(cacc, d_cacc, total);
     };
     std::vector<std::vector<uint32_t>> truth(nq);
     for (uint32_t i = 0; i < nq; i++) {
@@ -514,11 +533,16 @@ int main(int argc, char** argv) {
         std::sort(truth[i].begin(), truth[i].end());
         truth[i].erase(std::unique(truth[i].begin(), truth[i].end()), truth[i].end());
     }
+    // QPS as the reference times it (src/rknn_query.cpp:2705-2806): each
+    // batch's search and its ground-truth evaluation inside the timer;
+    // search_secs keeps the search and candidate copy alone
     auto run_point = [&](size_t L, size_t batches) {
         PointStats s;
         const auto t0 = std::chrono::steady_clock::now();
         for (size_t b = 0; b < batches; b++) {
+            const auto ts = std::chrono::steady_clock::now();
             search(L);
+            s.search_secs += since(ts);
             for (uint32_t i = 0; i < nq; i++) {
                 const auto& T = truth[i];
                 if (T.empty()) continue;
@@ -527,9 +551,9 @@ int main(int argc, char** argv) {
                 std::vector<uint32_t> pred;
                 size_t chit = 0;
                 for (uint32_t t = 0; t < c; t++) {
-                    const uint32_t id = cand[(size_t)i * out_cap + t];
+                    const uint32_t id = cids[coff[i] + t];
                     if (std::binary_search(T.begin(), T.end(), id)) chit++;
-                    if (acc[(size_t)i * out_cap + t]) pred.push_back(id);
+                    if (cacc[coff[i] + t]) pred.push_back(id);
                 }
                 std::sort(pred.begin(), pred.end());
                 pred.erase(std::unique(pred.begin(), pred.end()), pred.end());
@@ -566,6 +590,9 @@ int main(int argc, char** argv) {
         std::printf("[rknn] DIST_BREAKDOWN L_descent=%zu #graph_dist=%.0f #verify_dist=%.0f #total_dist=%.0f\n", L,
                     s.sum_graph_ndist / dn, s.sum_verify_ndist / dn, s.sum_ndist / dn);
         std::printf("[rknn] CANDIDATE_RECALL L_descent=%zu recall=%.6f\n", L, s.sum_candidate_recall / dn);
+        if (s.search_secs > 0.0)
+            std::printf("[rknn] SEARCH_QPS L_descent=%zu qps=%.0f (device search, post-filter and candidate copy; "
+                        "ground-truth evaluation excluded)\n", L, (double)s.tasks / s.search_secs);
         std::fflush(stdout);
     };
     for (size_t L : Lds) {
@@ -604,7 +631,7 @@ int main(int argc, char** argv) {
             agg.sum_candidate_recall += m.sum_candidate_recall; agg.sum_cand += m.sum_cand;
             agg.sum_pred += m.sum_pred; agg.sum_ndist += m.sum_ndist; agg.sum_graph_ndist += m.sum_graph_ndist;
             agg.sum_verify_ndist += m.sum_verify_ndist; agg.nonempty += m.nonempty; agg.tasks += m.tasks;
-            agg.batches += m.batches; agg.secs += m.secs;
+            agg.batches += m.batches; agg.secs += m.secs; agg.search_secs += m.search_secs;
             std::printf("[rknn] HOT_REPEAT L_descent=%zu repeat=%zu/%zu batches=%zu tasks=%zu seconds=%.6f QPS=%.3f\n",
                         L, r + 1, bench_repeats, m.batches, m.tasks, m.secs, q);
         }

@@ -514,6 +514,12 @@ int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_ho
     return ANQI_OK;
 }
 
+int anqi_aknn_cutoffs(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon) {
+    if (K == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "aknn_cutoffs: K must be positive");
+    HCHK(launch_aknn_cutoffs(keys, n, K, cut, horizon, ctx->stream));
+    return ANQI_OK;
+}
+
 int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
               double* mstar2, double* scale) {
     if (n == 0 || d == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "lift: empty input");
@@ -1644,6 +1650,22 @@ static hipError_t batch_enqueue(anqi_batcher* b) {
 }
 
 extern "C" {
+
+int anqi_compact_candidates(anqi_ctx* ctx, uint32_t nq, uint32_t out_cap, const uint32_t* cand_ids,
+                            const uint8_t* accept, const uint32_t* cand_cnt, uint32_t* off, uint32_t* ids,
+                            uint8_t* acc_out, uint64_t ids_cap, uint64_t* total) {
+    hipStream_t s = ctx->stream;
+    HCHK(launch_cand_scan(cand_cnt, nq, out_cap, off, s));
+    uint32_t t = 0;
+    HCHK(hipMemcpyAsync(&t, off + nq, 4, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    *total = t;
+    if (!ids) return ANQI_OK;
+    if (t > ids_cap) return anqi_set_err(ctx, ANQI_E_USAGE, "candidates need %u entries, capacity %llu", t,
+                                         (unsigned long long)ids_cap);
+    HCHK(launch_cand_copy(cand_ids, accept, off, nq, out_cap, ids, acc_out, s));
+    return ANQI_OK;
+}
 
 int anqi_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* queries, uint32_t nq, uint32_t* entries) {
     if (!ix || !ix->tree || ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS)

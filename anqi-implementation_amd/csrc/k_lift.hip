@@ -87,3 +87,27 @@ hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, h
     hipLaunchKernelGGL(k_lift_pad, dim3(lift_grid(n)), dim3(256), 0, s, lifted, n, d, m2);
     return hipGetLastError();
 }
+
+// AKNN keys -> verifier cutoffs and lift horizon (tools/build_aknn_source.cpp
+// writes key = max(0, d2) for L2 into the .rec, :119-137; the lift's graph
+// radius is the row's K-th cutoff, src/rknn_query.cpp:2203-2236): cut =
+// max(0, keys) elementwise (NaN stays NaN, as std::max(0.0f, x) would keep
+// 0 only for x < 0), horizon[i] = cut[i][K-1].
+__global__ void __launch_bounds__(256) k_aknn_cutoffs(const float* __restrict__ keys, uint64_t m, uint32_t K,
+                                                      float* __restrict__ cut, float* __restrict__ horizon) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const float v = keys[t];
+    const float c = v < 0.0f ? 0.0f : v;
+    cut[t] = c;
+    if (horizon && t % K == K - 1) horizon[t / K] = c;
+}
+
+hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon,
+                               hipStream_t s) {
+    const uint64_t m = n * K;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_aknn_cutoffs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, keys, m, K, cut,
+                       horizon);
+    return hipGetLastError();
+}

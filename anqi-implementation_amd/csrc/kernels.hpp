@@ -69,6 +69,9 @@ hipError_t launch_lift_norm2(const float* x, uint64_t n, uint32_t d, uint64_t* s
 hipError_t launch_lift_rows(const float* x, uint64_t n, uint32_t d, const float* rk, uint64_t scale, uint64_t sc2,
                             float* lifted, uint64_t* maxbits, hipStream_t s);
 hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, hipStream_t s);
+// cut = max(0, keys) (n x K), horizon (optional, n) = cut[:, K-1].
+hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon,
+                               hipStream_t s);
 // Split lifted layout of an index: lmain (n x d), ltail (n).
 hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail,
                              hipStream_t s);
@@ -90,6 +93,11 @@ hipError_t launch_batch_scan(const uint32_t* pred_cnt, const uint32_t* nq_dev, u
 hipError_t launch_batch_compact(const uint32_t* cand_ids, const uint8_t* accept, const uint32_t* cand_cnt,
                                 const uint32_t* off, const uint32_t* nq_dev, uint32_t max_nq, uint32_t out_cap,
                                 uint32_t* ids, hipStream_t s);
+// Candidate CSR (k_batch.hip): off = exclusive scan of min(cnt, out_cap),
+// then each query's candidate ids and accept bytes.
+hipError_t launch_cand_scan(const uint32_t* cnt, uint32_t nq, uint32_t out_cap, uint32_t* off, hipStream_t s);
+hipError_t launch_cand_copy(const uint32_t* cand, const uint8_t* acc, const uint32_t* off, uint32_t nq,
+                            uint32_t out_cap, uint32_t* ids, uint8_t* acc_out, hipStream_t s);
 
 // ---- verifier (k_lrq.hip) ---------------------------------------------------
 hipError_t launch_lrq_encode(const float* cut, uint64_t n, uint32_t nk, uint32_t rankm,

@@ -10,8 +10,9 @@ Restates tools/train_final_m8_u8.py (constants :16-25) with torch tensors:
   main                 :190-265 (first 120,000 rows seed the basis; the
                                 training sample is default_rng(10).choice,
                                 sorted; 3 alternating rounds)
-All of the fit runs in FP64 on the device.  Outputs are the radbasis and
-codebook arrays (float32) in the reference file layouts.
+The basis runs in FP64; the codebooks in FP64 or FP32 (fit's codebook_dtype).
+Outputs are the radbasis and codebook arrays (float32) in the reference file
+layouts.
 """
 from __future__ import annotations
 
@@ -73,9 +74,10 @@ def train_codebooks(residual, iterations=LLOYD_ITERS):
     torch = _torch()
     vals = torch.sort(residual.T.contiguous(), dim=1).values          # (K, N)
     k, n = vals.shape
-    q = (torch.arange(CODE_SIZE, dtype=torch.float64, device=vals.device) + 0.5) / CODE_SIZE
+    dt = vals.dtype
+    q = (torch.arange(CODE_SIZE, dtype=dt, device=vals.device) + 0.5) / CODE_SIZE
     centres = _quantile_linear(vals, q)
-    prefix = torch.cat([torch.zeros(k, 1, dtype=torch.float64, device=vals.device), torch.cumsum(vals, dim=1)], 1)
+    prefix = torch.cat([torch.zeros(k, 1, dtype=dt, device=vals.device), torch.cumsum(vals, dim=1)], 1)
     active = torch.ones(k, dtype=torch.bool, device=vals.device)
     for _ in range(iterations):
         bounds = (centres[:, :-1] + centres[:, 1:]) * 0.5
@@ -84,7 +86,7 @@ def train_codebooks(residual, iterations=LLOYD_ITERS):
         ends = torch.cat([cuts, torch.full((k, 1), n, dtype=cuts.dtype, device=vals.device)], 1)
         counts = ends - starts
         sums = torch.gather(prefix, 1, ends) - torch.gather(prefix, 1, starts)
-        upd = torch.where(counts > 0, sums / torch.clamp(counts, min=1).to(torch.float64), centres)
+        upd = torch.where(counts > 0, sums / torch.clamp(counts, min=1).to(dt), centres)
         upd = torch.cummax(upd, dim=1).values
         done = (upd - centres).abs().amax(dim=1) < 1e-12
         centres = torch.where(active[:, None], upd, centres)
@@ -112,9 +114,16 @@ def decode_nearest(residual, codebooks):
 
 
 def fit(thresholds, train_sample=TRAIN_ROWS, seed=SEED, recursive_iters=RECURSIVE_ITERS,
-        lloyd_iters=LLOYD_ITERS, chunk=65536, log=print):
-    """thresholds: (n, 100) float32 device tensor (the AKNN cutoff keys).
-    Returns (mean[100] f32, basis[8,100] f32, codebook[100,256] f32) as numpy."""
+        lloyd_iters=LLOYD_ITERS, chunk=65536, log=print, codebook_dtype=None):
+    """thresholds: (n, 100) float32 tensor (the AKNN cutoff keys).
+    Returns (mean[100] f32, basis[8,100] f32, codebook[100,256] f32) as numpy.
+    The basis (mean, covariance, eigenvectors) and the rank-M projection run
+    in FP64, as the reference's numpy does.  codebook_dtype sets the
+    arithmetic of the residual codebooks (quantile start, Lloyd rounds,
+    nearest decode): FP64 (the default, the reference's arithmetic, bit-equal
+    to its files on one CPU thread) or FP32 (what the GPU pipeline uses, so
+    that FP64 on the device stays in the least-squares/eigen step; the exact
+    floor decode of the encode judges whichever codebook results)."""
     torch = _torch()
     t0 = time.perf_counter()
     n = thresholds.shape[0]
@@ -126,16 +135,18 @@ def fit(thresholds, train_sample=TRAIN_ROWS, seed=SEED, recursive_iters=RECURSIV
     rng = np.random.default_rng(seed)
     ids = np.sort(rng.choice(n, size=sample_n, replace=False))
     rows = thresholds[torch.from_numpy(ids).to(thresholds.device)].to(torch.float64)
+    cdt = codebook_dtype or torch.float64
     codebooks = None
     for it in range(recursive_iters):
         recon = reconstruct(rows, mean, basis)
         resid = rows - recon
-        codebooks = train_codebooks(resid, lloyd_iters)
-        dec = decode_nearest(resid, codebooks)
+        codebooks = train_codebooks(resid.to(cdt), lloyd_iters)
+        dec = decode_nearest(resid.to(cdt), codebooks).to(torch.float64)
         rmse = float(torch.sqrt(torch.mean((recon + dec - rows) ** 2)))
         log(f"[verifier-train] round={it + 1}/{recursive_iters} sample_rmse={rmse:.6g} explained={explained:.6f}")
         if it + 1 < recursive_iters:
             mean, basis, explained = fit_basis(rows - dec)
+    codebooks = codebooks.to(torch.float64)
     lo = torch.full((K_MAX,), float("inf"), dtype=torch.float64, device=thresholds.device)
     hi = torch.full((K_MAX,), float("-inf"), dtype=torch.float64, device=thresholds.device)
     for s in range(0, n, chunk):

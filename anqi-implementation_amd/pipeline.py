@@ -39,7 +39,7 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     t0 = time.perf_counter()
     ids, keys, st = ctx.aknn_build(base, K=K, leaf=leaf, trees=trees, ef=ef, ef_alpha=ef_alpha, seed=seed,
                                    precision=precision)
-    cut = T.clamp(keys, min=0.0)                    # .rec key = max(0, d2)
+    cut, horizon = ctx.aknn_cutoffs(keys)          # .rec key = max(0, d2); lift horizon = key K
     _sync(ctx)
     t1 = time.perf_counter()
     t["aknn_source_s"] = t1 - t0
@@ -48,7 +48,7 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
                             leaf_pairs=int(st.leaf_pairs), leaves=int(st.leaves), max_leaf=int(st.max_leaf),
                             short_rows=int(st.short_rows), overflow_rows=int(st.overflow_rows),
                             leaf_survivors=int(st.leaf_survivors))
-    mu, G, cb, fit_s = lrq_fit.fit(cut, log=lambda *a: None)
+    mu, G, cb, fit_s = lrq_fit.fit(cut, log=lambda *a: None, codebook_dtype=T.float32)
     _sync(ctx)
     t2 = time.perf_counter()
     t["verifier_train_s"] = t2 - t1
@@ -60,9 +60,9 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     t["encode_s"] = t3 - t2
     if precision >= 1:
         # device lift (soft binary64), bit-identical to the host lift
-        lifted, mstar2, scale = ctx.lift(base, cut[:, K - 1].contiguous())
+        lifted, mstar2, scale = ctx.lift(base, horizon)
     else:
-        rk = cut[:, K - 1].contiguous().cpu().numpy()
+        rk = horizon.cpu().numpy()
         lifted_np, mstar2, scale = pyanqi.lift_host(base_np, rk)
         lifted = ctx.to_dev(lifted_np)
     _sync(ctx)

@@ -90,6 +90,7 @@ SIGS = {
     "anqi_ctx_set_lrq_form": (I32, [P, I32]),
     "anqi_ctx_lrq_fallbacks": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
+    "anqi_aknn_cutoffs": (I32, [P, P, U64, U32, P, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lift": (I32, [P, P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, I32, F32, ctypes.POINTER(F64), P, P]),
@@ -109,6 +110,7 @@ SIGS = {
                                        P, I32]),
     "anqi_rknn_search_noalloc": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P,
                                        P]),
+    "anqi_compact_candidates": (I32, [P, U32, U32, P, P, P, P, P, P, U64, ctypes.POINTER(U64)]),
     "anqi_batcher_create": (I32, [P, ctypes.POINTER(Index), P, U32, U32, U32, U32, ctypes.POINTER(SearchTuning),
                                   ctypes.POINTER(P)]),
     "anqi_batcher_run": (I32, [P, P, P, P, U32, P, P, U64, ctypes.POINTER(U64)]),
@@ -240,6 +242,14 @@ class Context:
         st = AknnStats()
         self.call("anqi_aknn_build", dp(x), n, d, ctypes.byref(p), dp(ids), dp(keys), ctypes.byref(st))
         return ids, keys, st
+
+    def aknn_cutoffs(self, keys):
+        """(cut = max(0, keys), horizon = cut[:, K-1]) on the device (anqi_aknn_cutoffs)."""
+        n, K = keys.shape
+        cut = self.empty((n, K), self.torch.float32)
+        hz = self.empty((n,), self.torch.float32)
+        self.call("anqi_aknn_cutoffs", dp(keys), n, K, dp(cut), dp(hz))
+        return cut, hz
 
     def lrq_encode(self, cut, mu, basis, cb):
         n, nk = cut.shape
@@ -473,6 +483,19 @@ class Context:
                   dp(b["pred"]), dp(b["nexp"]), dp(b["status"]), mode)
         return b
 
+
+    def compact_candidates(self, bufs):
+        """(off[nq+1], ids, accept) of a search's buffers as device CSR (anqi_compact_candidates)."""
+        nq, cap = bufs["cand"].shape
+        off = self.empty((nq + 1,), self.torch.int32)
+        tot = U64()
+        self.call("anqi_compact_candidates", nq, cap, dp(bufs["cand"]), dp(bufs["acc"]), dp(bufs["cnt"]), dp(off),
+                  None, None, 0, ctypes.byref(tot))
+        ids = self.empty((max(1, tot.value),), self.torch.int32)
+        acc = self.empty((max(1, tot.value),), self.torch.uint8)
+        self.call("anqi_compact_candidates", nq, cap, dp(bufs["cand"]), dp(bufs["acc"]), dp(bufs["cnt"]), dp(off),
+                  dp(ids), dp(acc), tot.value, ctypes.byref(tot))
+        return off, ids[:tot.value], acc[:tot.value]
 
     def rknn_search_noalloc(self, ix, queries, L, thr, bufs, Lq=None, Lmax=0, kidx=None):
         """anqi_rknn_search_noalloc into caller buffers (search_buffers)."""

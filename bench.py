@@ -152,7 +152,7 @@ def stage_lrq(ctx, a):
         base = workload.million_base(a.n)
         x = ctx.to_dev(base)
         ids, keys, _ = ctx.aknn_build(x, K=100, leaf=500, trees=64, ef=600, precision=1)
-        cut = torch.clamp(keys, min=0.0).contiguous()
+        cut, _ = ctx.aknn_cutoffs(keys)
         del ids, keys, x
         source = "AKNN cutoffs of the 1M x 128 mixture (first n rows)"
     else:
@@ -161,7 +161,7 @@ def stage_lrq(ctx, a):
         cut = (torch.rand((a.n, 1), device="cuda:0", generator=g) * 4e4 + 2e4 +
                torch.cumsum(torch.rand((a.n, 100), device="cuda:0", generator=g) * 400, dim=1)).float().contiguous()
         source = "synthetic radius-like rows (torch generator seed 5)"
-    mu, G, cb, _ = lrq_fit.fit(cut[:1_000_000].contiguous(), log=lambda *z: None)
+    mu, G, cb, _ = lrq_fit.fit(cut[:1_000_000].contiguous(), log=lambda *z: None, codebook_dtype=torch.float32)
     mu, G, cb = ctx.to_dev(mu), ctx.to_dev(G), ctx.to_dev(cb)
     n, nk = cut.shape
     reps = a.steps

@@ -164,6 +164,12 @@ int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk,
                    float tau, int mode, float grad_scale, double* loss, float* grad_basis,
                    float* grad_codebook);
 
+/* AKNN keys (anqi_aknn_build, raw d2) -> the cutoffs the .rec holds,
+ * cut = max(0, keys) (tools/build_aknn_source.cpp:119-137), and the lift's
+ * horizon[i] = cut[i][K-1] (src/rknn_query.cpp:2203-2236).  device keys and
+ * cut (n*K, may alias), horizon (n, may be NULL). */
+int anqi_aknn_cutoffs(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon);
+
 /* ---- lift (src/rknn_query.cpp:2203-2236), host FP64 ----------------------- */
 /* host base (n*d), rk_horizon (n); host lifted (n*(d+2)). */
 int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
@@ -307,6 +313,17 @@ int anqi_rknn_search_noalloc(anqi_ctx* ctx, const anqi_index* ix, const float* q
                              const uint32_t* kidx, uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept,
                              uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
                              uint32_t* status);
+
+/* The candidates of a search's dense outputs as CSR: off (device, nq+1) =
+ * exclusive scan of min(cand_cnt, out_cap); ids / acc_out (device,
+ * *total entries) = each query's candidate ids and accept bytes in
+ * discovery order.  *total is returned first; with ids == NULL only off and
+ * *total are computed (size the buffers, then call again).  The reference
+ * keeps per-query vectors (src/rknn_query.cpp:2705-2776); this is the form a
+ * host reads them in. */
+int anqi_compact_candidates(anqi_ctx* ctx, uint32_t nq, uint32_t out_cap, const uint32_t* cand_ids,
+                            const uint8_t* accept, const uint32_t* cand_cnt, uint32_t* off, uint32_t* ids,
+                            uint8_t* acc_out, uint64_t ids_cap, uint64_t* total);
 
 /* Search-tree entry point of each query (device nq): the descent the search
  * runs (rptrees.h:108-123, ties go left), for anqi_rknn_search_entries. */
