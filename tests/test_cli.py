@@ -64,6 +64,28 @@ def test_rknn_query_exit_codes(tmp):
             assert rr.returncode == code, (args, rr.returncode)
 
 
+def gt_cases(tmp):
+    b = str(tmp / "small_base.bin")
+    return [([], 2), (["--base", b], 2), (["--base", b, "--out_prefix", str(tmp / "g"), "--mode", "x"], 2),
+            (["--base", b, "--out_prefix", str(tmp / "g"), "--bogus", "1"], 2),
+            (["--base", b, "--out_prefix", str(tmp / "g"), "--mode", "thresholds", "--K", "0"], 2),
+            (["--base", b, "--out_prefix", str(tmp / "g"), "--mode", "thresholds", "--K", "50"], 2),
+            (["--base", b, "--out_prefix", str(tmp / "g"), "--mode", "thresholds", "--K", "10", "--rk_k", "11"], 2),
+            (["--base", b, "--out_prefix", str(tmp / "g"), "--metric", "cos", "--query", b], 2),
+            (["--base", str(tmp / "missing.bin"), "--out_prefix", str(tmp / "g"), "--mode", "thresholds"], 3),
+            (["--base", b, "--out_prefix", str(tmp / "g"), "--query", str(tmp / "missing.bin"),
+              "--mode", "ground-truth", "--K", "10", "--threshold-table", str(tmp / "missing_gt.bin")], 3)]
+
+
+def test_dataset_gt_exit_codes(tmp):
+    for args, code in gt_cases(tmp):
+        r = run("dataset_gt", args)
+        assert r.returncode == code, (args, r.returncode, r.stderr)
+        if ref_available() and os.path.exists(os.path.join(REFBIN, "dataset_gt")):
+            rr = run("dataset_gt", args, ref=True)
+            assert rr.returncode == code, (args, rr.returncode)
+
+
 def test_rknn_query_rejects_unpinned_configuration():
     """A configuration the GPU build does not implement is refused (exit 2)
     instead of silently computing the pinned one."""

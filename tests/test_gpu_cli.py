@@ -77,3 +77,26 @@ def test_build_aknn_source_sample(tmp_path):
     assert rec >= ref_rec - 0.005
     meta = open(out + "_aknn_build_time.meta").read().split("\n")
     assert meta[0] == "ANQI_AKNN_BUILD_V1" and meta[1].startswith("seconds ")
+
+
+def test_dataset_gt_sample_files_equal_reference(tmp_path):
+    """dataset_gt on the sample: the exact top-100 table and the RkNN CSR at
+    k = 10, 50, 100 are byte-identical to the reference tool's files
+    (tests/golden/sample, made with --tile 64); the ground-truth mode reads
+    the threshold table the thresholds mode wrote."""
+    pfx = os.path.join(GOLDEN, "sample", "sample")
+    out = str(tmp_path / "g")
+    tool = os.path.join(BIN, "dataset_gt")
+    r = subprocess.run([tool, "--base", pfx + "_base.bin", "--K", "100", "--rk_k", "100", "--mode", "thresholds",
+                        "--out_prefix", out], capture_output=True, text=True, timeout=300)
+    say(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    assert open(out + "_baseknn_gt.bin", "rb").read() == open(pfx + "_baseknn_gt.bin", "rb").read()
+    for k in (10, 50, 100):
+        o = str(tmp_path / f"q{k}")
+        r = subprocess.run([tool, "--base", pfx + "_base.bin", "--query", pfx + "_query.bin", "--K", "100",
+                            "--rk_k", str(k), "--mode", "ground-truth", "--threshold-table",
+                            out + "_baseknn_gt.bin", "--out_prefix", o], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert open(o + "_rknn_gt.bin", "rb").read() == open(pfx + f"_rknn_gt_k{k}.bin", "rb").read(), k
+
