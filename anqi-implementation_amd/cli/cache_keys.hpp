@@ -1,0 +1,141 @@
+// cache_keys.hpp -- the reference's cache keys and cache file names for the
+// configuration rknn_query implements, so that a run reuses (and hands over)
+// the graph, lift and verifier caches exactly as src/rknn_query.cpp does:
+//
+//   graph key      graph_cache_key (rknn_query.cpp:301-341) plus the
+//                  post-filter graph-cutoff suffix (:1843-1854), sanitized
+//   radius key     radius_cache_key, rank-M residual branch (:1217-1242)
+//   file names     :1855-1861 (lift, meta, state), :1870-1873 (marker and
+//                  state key), :1999 (verifier), include/index.h:1704-1770
+//                  (generate_config: the graph index itself)
+//
+// Keys are FNV-1a 64 over bytes; file identities are stat() fields, so a
+// rewritten input invalidates every cache built from it.
+#pragma once
+#include <sys/stat.h>
+
+#include <string>
+
+#include "formats.hpp"
+
+namespace anqi_cli {
+
+// path:dev<>:ino<>:sz<>:mt<>:ct<> (rknn_query.cpp:226-234)
+inline std::string file_stat_identity(const std::string& p) {
+    struct stat st {};
+    if (::stat(p.c_str(), &st) != 0) return p + ":missing";
+    return p + ":dev" + std::to_string((uint64_t)st.st_dev) + ":ino" + std::to_string((uint64_t)st.st_ino) +
+           ":sz" + std::to_string((uint64_t)st.st_size) + ":mt" + std::to_string((int64_t)st.st_mtime) +
+           ":ct" + std::to_string((int64_t)st.st_ctime);
+}
+
+// 16 hex digits of the FNV-1a 64 of a string (rknn_query.cpp:236-243)
+inline std::string hash16(const std::string& s) { return hex16(fnv1a(s.data(), s.size())); }
+
+// every byte outside [0-9A-Za-z_-] becomes '_' (rknn_query.cpp:271-278)
+inline std::string sanitize_tag(std::string s) {
+    for (char& c : s)
+        if (!((c >= '0' && c <= '9') || (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || c == '-' || c == '_'))
+            c = '_';
+    return s;
+}
+
+// Present at all (even empty): the reference tests these with getenv alone.
+inline bool env_set(const char* k) { return std::getenv(k) != nullptr; }
+
+struct CacheInputs {
+    std::string pfx;           // dataset prefix
+    std::string graph_dir;     // ANQI_GRAPH_CACHE_DIR with a trailing '/'
+    std::string verifier_dir;  // ANQI_VERIFIER_CACHE_DIR with a trailing '/'
+    std::string rec;           // ANQI_PREBUILT_TOPK_REC; empty: AKNN source built on the GPU
+    std::string basis, codebook;
+    size_t M, leaf, trees, ef;  // command-line graph degree and AKNN parameters
+    size_t lift_nnk;            // ANQI_LIFTNNK (50)
+    uint64_t n;
+    uint32_t dim;               // lifted dimension d + 2
+};
+
+struct CacheNames {
+    std::string threshold_source, radius_key, graph_key, state_key;
+    std::string approxlift, meta, state, marker, graph_index, verifier;
+};
+
+// Identity of the AKNN cutoffs (threshold_source_identity, rknn_query.cpp:
+// 1059-1063).  Without a prebuilt .rec the reference reads the cutoffs from
+// PFX_baseknn_aknn.bin; this tool builds them on the GPU from the base and
+// the AKNN parameters instead, so its identity names those, and no cache of
+// a reference run is taken for one of these runs or the other way round.
+inline std::string threshold_source_identity(const CacheInputs& in) {
+    if (in.rec.empty())
+        return file_stat_identity(in.pfx + "_base.bin") + ":gpu_aknn:leaf" + std::to_string(in.leaf) + ":nt" +
+               std::to_string(in.trees) + ":ef" + std::to_string(in.ef) + ":k100";
+    return file_stat_identity(in.rec) + ":interleaved_k" + env_or("ANQI_PREBUILT_TOPK_K", "100");
+}
+
+// The edge-policy part of the graph key (rknn_query.cpp:314-324) and of the
+// index name (include/index.h:1745-1767) for the plain policies accepted here.
+inline std::string edge_suffix(const char* sep) {
+    if (!env_set("ANQI_EDGE_POLICY")) return "";
+    const std::string s(sep);
+    return "_edge" + s + env_or("ANQI_EDGE_POLICY", "none") + "_eks" + s + env_or("ANQI_EDGE_KS", "10,50,100") +
+           "_enk" + s + env_or("ANQI_EDGE_NK", "100") + "_erb" + s + env_or("ANQI_EDGE_RESID_BUDGET", "16") + "_erl" +
+           s + env_or("ANQI_EDGE_RESID_LAMBDA", "1.0") + "_ebo" + s + env_or("ANQI_EDGE_BALL_OCCLUSION", "all");
+}
+
+inline CacheNames cache_names(const CacheInputs& in) {
+    CacheNames c;
+    c.threshold_source = threshold_source_identity(in);
+    // radius_cache_key, rank-M residual modes with a learned codebook
+    const char* bias = std::getenv("ANQI_RESID_BIAS");
+    const char* basis_id = std::getenv("ANQI_RQ_BASIS_ID");
+    const char* cb_id = std::getenv("ANQI_RQ_CODEBOOK_ID");
+    const std::string src_id = env_set("ANQI_RADIUS_SOURCE_ID")
+                                   ? std::string(env_or("ANQI_RADIUS_SOURCE_ID", "missing_radius_source_id"))
+                                   : c.threshold_source;
+    c.radius_key = std::string("rankm_resid_u8_lrq_floor:M") + env_or("ANQI_RANKM", "8") + ":b" + (bias ? bias : "0") +
+                   ":obj" + env_or("ANQI_RQ_OBJECTIVE", "radius_reconstruction") + ":basis" +
+                   ((basis_id && *basis_id) ? std::string(basis_id) : fingerprint(in.basis)) + ":cb" +
+                   ((cb_id && *cb_id) ? std::string(cb_id) : fingerprint(in.codebook)) + ":src" + hash16(src_id) +
+                   ":rqpred_v3";
+    // graph_cache_key for the any-k lifted geometry, graph rank 0
+    std::string g = "M" + std::to_string(in.M) + "_leaf" + std::to_string(in.leaf) + "_nt" + std::to_string(in.trees) +
+                    "_ef" + std::to_string(in.ef) + "_geomanyk_lift_rankm0_specific0_alpha" + env_or("ANQI_ALPHA", "1.2") +
+                    "_liftnnk" + env_or("ANQI_LIFTNNK", "50") + "_graphnnd" + env_or("ANQI_GRAPHNND", "6") +
+                    edge_suffix("") + "_vam0";
+    g = sanitize_tag(g);
+    // the post-filter verifier's graph is keyed by its cutoff source instead
+    // of the radius key
+    const std::string radius_id = env_set("ANQI_GRAPH_RADIUS_ID")
+                                      ? std::string(env_or("ANQI_GRAPH_RADIUS_ID", "aknn_k100_v1"))
+                                      : file_stat_identity(in.pfx + "_base.bin") + "|" + c.threshold_source + ":k100:v2";
+    c.graph_key = sanitize_tag(g + "_graph_cutoff_k100_" + hash16(radius_id));
+    c.state_key = "graph=" + c.graph_key;
+    const std::string tag = hash16(c.graph_key);
+    c.approxlift = in.graph_dir + "_approxlift_graph_" + tag + ".bin";
+    c.meta = in.graph_dir + "_anqi_meta_graph_" + tag + ".txt";
+    c.state = in.graph_dir + "_anqi_graph_state_" + tag + ".txt";
+    c.marker = in.graph_dir + "_anqi_" + c.graph_key + ".built";
+    c.verifier = in.verifier_dir + "_anqi_verifier_" + hash16(c.radius_key) + ".bin";
+    // generate_config of the lifted index (ef_alpha 8 and the prune alpha are
+    // floats there, printed by std::to_string); the graph key reaches it
+    // through ANQI_GRAPH_CACHE_KEY, which the reference sets before building
+    const char* alpha = std::getenv("ANQI_ALPHA");
+    float prune_alpha = alpha ? (float)std::atof(alpha) : 1.2f;
+    if (!(prune_alpha > 0.0f)) prune_alpha = 1.2f;   // the index's own default (index.h:143)
+    const std::string cfg = "fd_metric_l2_M_" + std::to_string(in.M) + "_nnk_" + std::to_string(in.lift_nnk) +
+                            "_ntrees_64_n_" + std::to_string(in.n) + "_dim_" + std::to_string(in.dim) + "_efa_" +
+                            std::to_string(8.0f) + "_pa_" + std::to_string(prune_alpha) + "_er_100_graphnnd_" +
+                            env_or("ANQI_GRAPHNND", "6") + "_gk_" + hash16(c.graph_key) + edge_suffix("_");
+    c.graph_index = in.graph_dir + sanitize_tag(cfg);
+    return c;
+}
+
+// First line of a text file, "" when it cannot be read.
+inline std::string first_line(const std::string& p) {
+    std::ifstream f(p);
+    std::string s;
+    std::getline(f, s);
+    return s;
+}
+
+}  // namespace anqi_cli

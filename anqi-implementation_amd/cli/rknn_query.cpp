@@ -13,6 +13,13 @@
 // unset or set to the pinned values; any other value is rejected (exit 2)
 // rather than silently computing something else.
 //
+// Caches follow the reference's protocol (rknn_query.cpp:1841-1894, 2244-2297):
+// the lifted vectors, lift meta, state file and graph index (generate_config
+// name) under ANQI_GRAPH_CACHE_DIR, the verifier artifact under
+// ANQI_VERIFIER_CACHE_DIR, all named by the same FNV keys, so a second run (at
+// another k) loads the index instead of building it, and caches move between
+// this tool and the reference's.
+//
 // Everything from the AKNN cutoffs on runs on the GPU through libanqi_hip.so:
 // LRQ encode + threshold decode, lift, graph build, search tree, range search
 // and post-filter.  Recall/precision/F1 and distance counts are computed on
@@ -20,11 +27,13 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <iomanip>
 #include <map>
 #include <set>
 #include <string>
 #include <vector>
 
+#include "cache_keys.hpp"
 #include "formats.hpp"
 
 using namespace anqi_cli;
@@ -102,22 +111,29 @@ const std::vector<Pin>& pins() {
         {"ANQI_GRAPHNND", {"6"}},
         {"ANQI_PREBUILT_TOPK_K", {"100"}},
         {"ANQI_SEARCH_SCORE", {"center"}},
-        {"ANQI_SPECIFIC", {"0"}},
         {"ANQI_BETA", {"0.5"}},
         {"ANQI_RCEPS", {"0", "0.0"}},
-        {"ANQI_VAMANA", {"0"}},
         {"ANQI_EDGE_POLICY", {"rng", "default"}},
         {"ANQI_CANDIDATE_ENVELOPE", {"aknn", "aknn_k100", "default"}},
     };
     return p;
 }
-const char* const kUnsupported[] = {
-    "ANQI_REQUIRE_GRAPH_CACHE", "ANQI_REUSE_ORIG_KNN", "ANQI_EDGE_RESID_FILE", "ANQI_GRAPH_RADIUS_SOURCE_PFX",
-    "ANQI_PREBUILT_WARM_START_COMMAND", "ANQI_ANALYTIC_RANKM_MODEL", "ANQI_RADIUS_KNOTS", "ANQI_RADIUS_RELAX",
-    "ANQI_QUERY_SINGLE_ENTRY", "ANQI_VAMANA_QUERY_ENTRY",
-};
+// Switched on by their presence alone, whatever the value (the reference
+// tests them with getenv alone), or by any non-empty value.
+const char* const kPresence[] = {"ANQI_VAMANA", "ANQI_REUSE_ORIG_KNN", "ANQI_EDGE_RESID_FILE", "ANQI_BASE_U8BIN",
+                                 "ANQI_QUERY_SINGLE_ENTRY", "ANQI_VAMANA_QUERY_ENTRY"};
+const char* const kNonEmpty[] = {"ANQI_GRAPH_RADIUS_SOURCE_PFX", "ANQI_PREBUILT_WARM_START_COMMAND"};
 
 void check_configuration() {
+    if (std::getenv("ANQI_SPECIFIC")) {   // rknn_query.cpp:1806-1811
+        std::fprintf(stderr, "[rknn] independent postfilter verifier requires a shared graph; unset ANQI_SPECIFIC\n");
+        std::exit(2);
+    }
+    for (const char* name : kPresence)
+        if (std::getenv(name)) {
+            std::fprintf(stderr, "[rknn] %s is not supported by the GPU build (unset it)\n", name);
+            std::exit(2);
+        }
     for (const Pin& p : pins()) {
         const char* v = std::getenv(p.name);
         if (!v || !*v) continue;
@@ -129,12 +145,17 @@ void check_configuration() {
             std::exit(2);
         }
     }
-    for (const char* name : kUnsupported) {
+    for (const char* name : kNonEmpty) {
         const char* v = std::getenv(name);
-        if (v && *v && std::string(v) != "0") {
+        if (v && *v) {
             std::fprintf(stderr, "[rknn] %s is not supported by the GPU build\n", name);
             std::exit(2);
         }
+    }
+    const char* relax = std::getenv("ANQI_RADIUS_RELAX");   // off when it reads as 0 (rknn_query.cpp:841-845)
+    if (relax && *relax && std::atof(relax) != 0.0) {
+        std::fprintf(stderr, "[rknn] ANQI_RADIUS_RELAX is not supported by the GPU build\n");
+        std::exit(2);
     }
 }
 
@@ -286,14 +307,31 @@ int main(int argc, char** argv) {
     const std::string DSDIR = PFX.substr(0, PFX.rfind('/') + 1);
     const std::string graph_dir = cache_dir("ANQI_GRAPH_CACHE_DIR", DSDIR);
     const std::string verifier_dir = cache_dir("ANQI_VERIFIER_CACHE_DIR", DSDIR);
-    std::printf("[rknn] graph geometry=anyk_lift graph_rankm=0 verification_rankm=%d "
-                "radius_mode=rankm_resid_u8_lrq_floor placement=postfilter\n", RANKM);
-    std::printf("[rknn] component caches: graph=%s verifier=%s\n", graph_dir.c_str(), verifier_dir.c_str());
-
-    // rank-M basis and codebook (rknn_query.cpp:1896-1934, 407-454)
+    const char* rec_env = std::getenv("ANQI_PREBUILT_TOPK_REC");
+    const std::string rec_p = (rec_env && *rec_env) ? std::string(rec_env) : std::string();
     const std::string basis_p = PFX + "_radbasis_M8.bin";
     const char* cbenv = std::getenv("ANQI_RQ_CODEBOOK");
     const std::string cb_p = (cbenv && *cbenv) ? std::string(cbenv) : PFX + "_learned_rq_codebook_M8_u8.bin";
+    const uint32_t dl = d + 2;
+    const auto tb0 = std::chrono::steady_clock::now();
+
+    // cache keys, state and the reuse decision (rknn_query.cpp:1841-1894)
+    const CacheNames cn =
+        cache_names({PFX, graph_dir, verifier_dir, rec_p, basis_p, cb_p, Marg, leafA, ntA, efA, 50, n, dl});
+    std::printf("[rknn] graph geometry=anyk_lift graph_rankm=0 verification_rankm=%d "
+                "radius_mode=rankm_resid_u8_lrq_floor placement=postfilter\n", RANKM);
+    std::printf("[rknn] graph cache key=%s\n", cn.graph_key.c_str());
+    const bool can_load = exists(cn.state) && first_line(cn.state) == cn.state_key && exists(cn.marker) &&
+                          exists(cn.approxlift) && exists(cn.meta) && exists(cn.graph_index);
+    std::printf("[rknn] component caches: graph=%s verifier=%s\n", graph_dir.c_str(), verifier_dir.c_str());
+    if (env_flag("ANQI_REQUIRE_GRAPH_CACHE") && !can_load) {
+        std::fprintf(stderr, "[rknn] required graph cache is incomplete or incompatible: dir=%s key=%s\n",
+                     graph_dir.c_str(), cn.graph_key.c_str());
+        return 3;
+    }
+    const std::string radius_desc = cn.radius_key + ":postfilter";
+
+    // rank-M basis and codebook (rknn_query.cpp:1896-1934, 407-454)
     std::vector<float> mu, G, cb;
     if (!read_basis(basis_p, NK, RANKM, mu, G)) {
         std::fprintf(stderr, "[rknn] invalid or missing rank-M basis %s\n", basis_p.c_str());
@@ -306,129 +344,250 @@ int main(int argc, char** argv) {
         return 3;
     }
     const uint32_t code_size = (uint32_t)(cb.size() / NK);
+    // CompactLrqVerifier::persistent_bytes (rknn_query.cpp:631-636): payload,
+    // mean, basis, codebook and the 68 header bytes
+    const double verifier_bytes =
+        (double)n * (RANKM * 4 + NK) + 4.0 * ((double)NK + (double)RANKM * NK + (double)NK * code_size) + 68.0;
 
-    ck(anqi_ctx_create(0, &g_ctx), "context");
-    const auto tb0 = std::chrono::steady_clock::now();
-
-    // AKNN source: the prebuilt .rec (rknn_query.cpp:890-1010) or a GPU build
-    std::printf("[rknn] original-space AKNN: leaf=%zu n_trees=%zu ef=%zu metric=l2\n", leafA, ntA, efA);
-    std::vector<uint32_t> aknn_ids;
-    std::vector<float> aknn_keys;
-    const char* rec_env = std::getenv("ANQI_PREBUILT_TOPK_REC");
-    std::string rec_p = (rec_env && *rec_env) ? std::string(rec_env) : std::string();
-    float* d_base = upload(base);
-    uint32_t* d_aknn = nullptr;
-    float* d_keys = nullptr;
-    if (!rec_p.empty()) {
-        if (file_size(rec_p) != (uint64_t)n * 100 * 8) {
-            std::fprintf(stderr, "[rknn] bad prebuilt top-k table %s: bytes=%lld expected=%llu\n", rec_p.c_str(),
-                         (long long)(exists(rec_p) ? (long long)file_size(rec_p) : -1LL),
-                         (unsigned long long)((uint64_t)n * 100 * 8));
+    // a verifier artifact with this radius key is reused whatever the graph
+    // cache holds (rknn_query.cpp:1999-2012)
+    std::vector<float> coeff;
+    std::vector<uint8_t> codes;
+    bool verifier_ready = false;
+    double verifier_load_ms = 0.0, encode_save_ms = 0.0;
+    if (exists(cn.verifier)) {
+        const auto tl = std::chrono::steady_clock::now();
+        if (!read_verifier(cn.verifier, n, NK, RANKM, coeff, codes)) {
+            std::fprintf(stderr, "[rknn] invalid LRQ verifier artifact %s\n", cn.verifier.c_str());
             return 3;
         }
-        if (!read_rec(rec_p, n, 100, aknn_ids, aknn_keys)) {
-            std::fprintf(stderr, "[rknn] cannot read prebuilt top-k %s\n", rec_p.c_str());
-            return 3;
-        }
-        for (size_t t = 0; t < aknn_ids.size(); t++)
-            if (aknn_ids[t] >= n) {
-                std::fprintf(stderr, "[rknn] invalid prebuilt top-k id at row=%zu\n", t / 100);
-                return 3;
-            }
-        std::printf("[rknn] loaded contiguous prebuilt original KNN %s (n=%u source_k=100 width=100)\n", rec_p.c_str(),
-                    n);
-        d_aknn = upload(aknn_ids);
-        d_keys = upload(aknn_keys);
-    } else {
-        anqi_aknn_params ap{100, (uint32_t)leafA, (uint32_t)ntA, (uint32_t)efA, 8, 100, 10, 0};
-        anqi_aknn_stats st{};
-        d_aknn = dalloc<uint32_t>((size_t)n * 100);
-        d_keys = dalloc<float>((size_t)n * 100);
-        ck(anqi_aknn_build(g_ctx, d_base, n, d, &ap, d_aknn, d_keys, &st), "AKNN source");
-        // .rec keys: max(0, d2) (tools/build_aknn_source.cpp:119-137)
-        ck(anqi_aknn_cutoffs(g_ctx, d_keys, n, 100, d_keys, nullptr), "AKNN cutoffs");
-        download(aknn_keys, d_keys, (size_t)n * 100);
-        std::printf("[rknn] built original-space AKNN on the GPU (n=%u K=100 in %.2fs)\n", n, st.total_s);
+        verifier_load_ms = since(tl) * 1e3;
+        verifier_ready = true;
+        std::printf("[rknn] loaded independent compact verifier mode=rank-M+LRQ bits=8 %s (persistent=%.3fGB "
+                    "load_ms=%.3f)\n", cn.verifier.c_str(), verifier_bytes / 1e9, verifier_load_ms);
     }
-    std::printf("[rknn] original-space AKNN warm start ready: rows=%u source=%s\n", n,
-                rec_p.empty() ? "gpu" : "prebuilt-contiguous");
-    const double source_prepare_s = since(tb0);
 
-    // verifier: rank-8 coefficients + u8 floor codes (rknn_query.cpp:603-753)
-    const auto tv0 = std::chrono::steady_clock::now();
+    // HIP runtime start-up is left out of the build time
+    const auto tc0 = std::chrono::steady_clock::now();
+    ck(anqi_ctx_create(0, &g_ctx), "context");
+    const double ctx_s = since(tc0);
+
+    float* d_base = upload(base);
     float* d_mu = upload(mu);
     float* d_G = upload(G);
     float* d_cb = upload(cb);
     float* d_coeff = dalloc<float>((size_t)n * RANKM);
-    uint8_t* d_codes = dalloc<uint8_t>((size_t)n * NK);
     uint8_t* d_codes_km = dalloc<uint8_t>((size_t)n * NK);
-    ck(anqi_lrq_encode(g_ctx, d_keys, n, NK, RANKM, d_mu, d_G, d_cb, code_size, d_coeff, d_codes, d_codes_km),
-       "LRQ encode");
-    std::vector<float> coeff;
-    std::vector<uint8_t> codes;
-    download(coeff, d_coeff, (size_t)n * RANKM);
-    download(codes, d_codes, (size_t)n * NK);
-    const std::string vkey = hex16(fnv1a(codes.data(), codes.size(), fnv1a(coeff.data(), coeff.size() * 4)));
-    const std::string vpath = verifier_dir + "_anqi_verifier_gpu_" + vkey + ".bin";
-    const auto ts0 = std::chrono::steady_clock::now();
-    if (!write_verifier(vpath, coeff, codes, n, NK, RANKM)) {
-        std::fprintf(stderr, "[rknn] failed writing verifier %s\n", vpath.c_str());
-        return 3;
+    if (verifier_ready) {
+        ck(anqi_h2d(g_ctx, d_coeff, coeff.data(), coeff.size() * 4), "upload");
+        uint8_t* d_codes = upload(codes);
+        ck(anqi_transpose_codes(g_ctx, d_codes, n, NK, d_codes_km), "code transpose");
+        ck(anqi_ctx_sync(g_ctx), "sync");
+        anqi_dfree(g_ctx, d_codes);
     }
-    const double encode_save_ms = since(ts0) * 1e3 + 0.0;
-    const double verifier_bytes = (double)n * (RANKM * 4 + NK);
-    std::printf("[rknn] built independent compact verifier mode=rank-M+LRQ bits=8 %s (persistent=%.3fGB "
-                "encode_save_ms=%.3f)\n", vpath.c_str(), verifier_bytes / 1e9, encode_save_ms);
 
-    // lift (rknn_query.cpp:2203-2236) with the AKNN 100th key as horizon
-    std::vector<float> rk(n);
-    for (uint32_t i = 0; i < n; i++) rk[i] = aknn_keys[(size_t)i * 100 + (GRAPH_NK - 1)];
-    const uint32_t dl = d + 2;
+    // AKNN source: the prebuilt .rec (rknn_query.cpp:890-1010, 1101-1133) or a
+    // GPU build; needed to build the graph or an absent verifier
+    std::vector<uint32_t> aknn_ids;
+    std::vector<float> aknn_keys;
+    uint32_t* d_aknn = nullptr;
+    float* d_keys = nullptr;
+    double source_prepare_s = 0.0;
+    auto load_source = [&]() -> int {
+        if (d_keys) return 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        std::printf("[rknn] original-space AKNN: leaf=%zu n_trees=%zu ef=%zu metric=l2\n", leafA, ntA, efA);
+        if (!rec_p.empty()) {
+            if (file_size(rec_p) != (uint64_t)n * 100 * 8) {
+                std::fprintf(stderr, "[rknn] bad prebuilt top-k table %s: bytes=%lld expected=%llu\n", rec_p.c_str(),
+                             (long long)(exists(rec_p) ? (long long)file_size(rec_p) : -1LL),
+                             (unsigned long long)((uint64_t)n * 100 * 8));
+                return 3;
+            }
+            if (!read_rec(rec_p, n, 100, aknn_ids, aknn_keys)) {
+                std::fprintf(stderr, "[rknn] cannot read prebuilt top-k %s\n", rec_p.c_str());
+                return 3;
+            }
+            for (size_t t = 0; t < aknn_ids.size(); t++)
+                if (aknn_ids[t] >= n) {
+                    std::fprintf(stderr, "[rknn] invalid prebuilt top-k id at row=%zu\n", t / 100);
+                    return 3;
+                }
+            std::printf("[rknn] loaded contiguous prebuilt original KNN %s (n=%u source_k=100 width=100)\n",
+                        rec_p.c_str(), n);
+            d_aknn = upload(aknn_ids);
+            d_keys = upload(aknn_keys);
+        } else {
+            anqi_aknn_params ap{100, (uint32_t)leafA, (uint32_t)ntA, (uint32_t)efA, 8, 100, 10, 0};
+            anqi_aknn_stats st{};
+            d_aknn = dalloc<uint32_t>((size_t)n * 100);
+            d_keys = dalloc<float>((size_t)n * 100);
+            ck(anqi_aknn_build(g_ctx, d_base, n, d, &ap, d_aknn, d_keys, &st), "AKNN source");
+            // .rec keys: max(0, d2) (tools/build_aknn_source.cpp:119-137)
+            ck(anqi_aknn_cutoffs(g_ctx, d_keys, n, 100, d_keys, nullptr), "AKNN cutoffs");
+            download(aknn_keys, d_keys, (size_t)n * 100);
+            std::printf("[rknn] built original-space AKNN on the GPU (n=%u K=100 in %.2fs)\n", n, st.total_s);
+        }
+        std::printf("[rknn] original-space AKNN warm start ready: rows=%u source=%s\n", n,
+                    rec_p.empty() ? "gpu" : "prebuilt-contiguous");
+        if (!rec_p.empty()) {
+            std::printf("[rknn] loaded AKNN cutoffs from interleaved top-k %s (n=%u K=%zu)\n", rec_p.c_str(), n, NK);
+            std::printf("[rknn] loaded AKNN cutoff column from interleaved top-k %s (n=%u k=%zu)\n", rec_p.c_str(), n,
+                        GRAPH_NK);
+        }
+        source_prepare_s = since(t0);
+        return 0;
+    };
+    // verifier: rank-8 coefficients + u8 floor codes (rknn_query.cpp:603-753),
+    // saved under the radius key (ensure_compact_lrq, :2028-2049)
+    auto encode_verifier = [&]() -> int {
+        if (verifier_ready) return 0;
+        const auto te = std::chrono::steady_clock::now();
+        uint8_t* d_codes = dalloc<uint8_t>((size_t)n * NK);
+        ck(anqi_lrq_encode(g_ctx, d_keys, n, NK, RANKM, d_mu, d_G, d_cb, code_size, d_coeff, d_codes, d_codes_km),
+           "LRQ encode");
+        download(coeff, d_coeff, (size_t)n * RANKM);
+        download(codes, d_codes, (size_t)n * NK);
+        anqi_dfree(g_ctx, d_codes);
+        if (!write_verifier(cn.verifier, coeff, codes, n, NK, RANKM)) {
+            std::fprintf(stderr, "[rknn] failed writing verifier %s\n", cn.verifier.c_str());
+            return 3;
+        }
+        encode_save_ms = since(te) * 1e3;
+        verifier_ready = true;
+        std::printf("[rknn] built independent compact verifier mode=rank-M+LRQ bits=8 %s (persistent=%.3fGB "
+                    "encode_save_ms=%.3f)\n", cn.verifier.c_str(), verifier_bytes / 1e9, encode_save_ms);
+        return 0;
+    };
+
     double mstar2 = 0, scale = 0;
     float* d_lifted = nullptr;
-    if (anqi_cli::gpu_form_chosen()) {
-        // device lift (integer soft binary64), the same bits as the host lift
-        float* d_rk = upload(rk);
-        d_lifted = dalloc<float>((size_t)n * dl);
-        ck(anqi_lift(g_ctx, d_base, n, d, d_rk, d_lifted, &mstar2, &scale), "lift");
-        anqi_dfree(g_ctx, d_rk);
-    } else {
-        std::vector<float> lifted((size_t)n * dl);
-        ck(anqi_lift_host(base.data(), n, d, rk.data(), lifted.data(), &mstar2, &scale), "lift");
-        d_lifted = upload(lifted);
-    }
-    const double lift_verifier_prepare_s = since(tv0);
-    std::printf("[rknn] BUILD lift M*^2=%.6g scale=%.4g (graph radius=aknn_k100; verifier=rankm_resid_u8_lrq_floor:M8:"
-                "b0:objradius_reconstruction:basis%s:cb%s:postfilter)\n",
-                mstar2, scale, fingerprint(basis_p).c_str(), fingerprint(cb_p).c_str());
-    std::printf("[rknn] graph degree M=%zu\n", Marg);
-
-    // graph + search tree (index.h:1071-1295)
-    const auto tg0 = std::chrono::steady_clock::now();
-    anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10, anqi_cli::gpu_form_chosen() ? 1 : 0};
-    anqi_graph_stats gst{};
-    anqi_tree tree{};
     uint32_t* d_links = dalloc<uint32_t>((size_t)n * Marg);
     uint32_t* d_deg = dalloc<uint32_t>(n);
-    ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_aknn, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
+    anqi_tree tree{};
+    double lift_verifier_prepare_s = 0.0, graph_index_s = 0.0;
+    const std::string index_name = cn.graph_index.substr(graph_dir.size());
+    bool graph_loaded = false;
+    if (can_load) {
+        // cached lift and graph (rknn_query.cpp:2052-2077; index.h:1826-1860)
+        {
+            std::ifstream f(cn.meta);
+            if (!(f >> mstar2 >> scale)) {
+                std::fprintf(stderr, "[rknn] bad meta cache %s\n", cn.meta.c_str());
+                return 3;
+            }
+        }
+        std::vector<float> lifted;
+        uint32_t ln = 0, ld = 0;
+        if (!read_fbin(cn.approxlift, lifted, ln, ld) || ln != n || ld != dl) {
+            std::fprintf(stderr, "[rknn] bad lifted vector cache %s\n", cn.approxlift.c_str());
+            return 3;
+        }
+        d_lifted = upload(lifted);
+        if (!verifier_ready)
+            if (int rc = load_source()) return rc;
+        if (int rc = encode_verifier()) return rc;
+        std::printf("[rknn] loaded cached index (M=%zu): M*^2=%.6g scale=%.4g radius=%s\n", Marg, mstar2, scale,
+                    radius_desc.c_str());
+        std::printf("[rknn] graph degree M=%zu\n", Marg);
+        std::printf("Index config: %s\n", index_name.c_str());
+        std::vector<uint32_t> hdeg, hlinks, left, right, entry;
+        std::vector<float> offset, hyper;
+        if (read_graph(cn.graph_index, n, dl, Marg, hdeg, hlinks, left, right, offset, hyper, entry)) {
+            std::printf("Loading index from: %s\n", cn.graph_index.c_str());
+            for (uint64_t i = 0; i < n; i++)   // device rows: slots past deg hold UINT32_MAX
+                for (uint64_t j = hdeg[i]; j < Marg; j++) hlinks[i * Marg + j] = UINT32_MAX;
+            ck(anqi_h2d(g_ctx, d_links, hlinks.data(), hlinks.size() * 4), "upload");
+            ck(anqi_h2d(g_ctx, d_deg, hdeg.data(), hdeg.size() * 4), "upload");
+            ck(anqi_tree_upload(g_ctx, (uint32_t)left.size(), (uint32_t)left.size() - 1, dl, left.data(),
+                                right.data(), offset.data(), hyper.data(), entry.data(), &tree),
+               "tree upload");
+            graph_loaded = true;
+        } else {
+            std::fprintf(stderr, "incompatible or truncated cached index: %s\n", cn.graph_index.c_str());
+        }
+    } else {
+        if (int rc = load_source()) return rc;
+        const auto tv0 = std::chrono::steady_clock::now();
+        if (int rc = encode_verifier()) return rc;
+        std::printf("[rknn] RADIUS_MODE=%s conceptual_radius_store=%.3fGB\n", radius_desc.c_str(),
+                    verifier_bytes / 1e9);
+        // lift (rknn_query.cpp:2203-2236) with the AKNN 100th key as horizon
+        std::vector<float> rk(n);
+        for (uint32_t i = 0; i < n; i++) rk[i] = aknn_keys[(size_t)i * 100 + (GRAPH_NK - 1)];
+        std::vector<float> lifted;
+        if (anqi_cli::gpu_form_chosen()) {
+            // device lift (integer soft binary64), the same bits as the host lift
+            float* d_rk = upload(rk);
+            d_lifted = dalloc<float>((size_t)n * dl);
+            ck(anqi_lift(g_ctx, d_base, n, d, d_rk, d_lifted, &mstar2, &scale), "lift");
+            anqi_dfree(g_ctx, d_rk);
+            download(lifted, d_lifted, (size_t)n * dl);
+        } else {
+            lifted.resize((size_t)n * dl);
+            ck(anqi_lift_host(base.data(), n, d, rk.data(), lifted.data(), &mstar2, &scale), "lift");
+            d_lifted = upload(lifted);
+        }
+        // lift, meta and state files (rknn_query.cpp:2231-2245)
+        if (!write_fbin(cn.approxlift, lifted.data(), n, dl))
+            std::fprintf(stderr, "[rknn] cannot write lifted vector cache %s\n", cn.approxlift.c_str());
+        { std::ofstream f(cn.meta); f << std::setprecision(17) << mstar2 << " " << scale << "\n"; }
+        { std::ofstream f(cn.state); f << cn.state_key << "\n"; }
+        lift_verifier_prepare_s = since(tv0);
+        std::printf("[rknn] BUILD lift M*^2=%.6g scale=%.4g (graph radius=aknn_k100; verifier=%s)\n", mstar2, scale,
+                    radius_desc.c_str());
+        std::printf("[rknn] graph degree M=%zu\n", Marg);
+        std::printf("Index config: %s\n", index_name.c_str());
+    }
+    if (!graph_loaded) {
+        // graph + search tree (index.h:1071-1295), saved with its marker (:1293;
+        // rknn_query.cpp:2297)
+        if (int rc = load_source()) return rc;
+        const auto tg0 = std::chrono::steady_clock::now();
+        anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10, anqi_cli::gpu_form_chosen() ? 1 : 0};
+        anqi_graph_stats gst{};
+        ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_aknn, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
+        std::vector<uint32_t> hdeg, hlinks, left(tree.nodes), right(tree.nodes);
+        std::vector<float> offset(tree.nodes), hyper((size_t)tree.nodes * dl);
+        download(hdeg, d_deg, n);
+        download(hlinks, d_links, (size_t)n * Marg);
+        ck(anqi_d2h(g_ctx, left.data(), tree.left, left.size() * 4), "download");
+        ck(anqi_d2h(g_ctx, right.data(), tree.right, right.size() * 4), "download");
+        ck(anqi_d2h(g_ctx, offset.data(), tree.offset, offset.size() * 4), "download");
+        ck(anqi_d2h(g_ctx, hyper.data(), tree.hyper, hyper.size() * 4), "download");
+        if (write_graph(cn.graph_index, n, dl, Marg, hdeg, hlinks, left, right, offset, hyper, tree.leaf_off,
+                        tree.leaf_ids, tree.root, 100, (float)gst.total_s)) {
+            std::printf("Index saved to: %s\n", cn.graph_index.c_str());
+            std::ofstream(cn.marker) << "1\n";
+        } else {
+            std::fprintf(stderr, "[rknn] cannot write graph cache %s\n", cn.graph_index.c_str());
+        }
+        graph_index_s = since(tg0);
+    }
     float* d_bnorm = dalloc<float>(n);
     ck(anqi_norms(g_ctx, d_base, n, d, d_bnorm), "norms");
     ck(anqi_ctx_sync(g_ctx), "sync");
-    const double graph_index_s = since(tg0);
-    const double build_s = since(tb0);
-    const double aknn_build_s = external_seconds("ANQI_AKNN_BUILD_SECONDS");
-    const double verifier_train_s = external_seconds("ANQI_VERIFIER_TRAIN_SECONDS");
-    const double end_to_end = aknn_build_s + verifier_train_s + build_s;
-    const double idx_gb = ((double)n * Marg * 4 + (double)n * dl * 4) / 1e9;
-    std::printf("[rknn] BUILD_TIME=%.1fs INDEX_SIZE=%.2fGB (aknn_source=%.1fs + verifier_training=%.1fs + index=%.1fs, "
-                "graph M=%zu + graph dim=%u + verification=%.3fGB + envelope=%.3fGB)\n",
-                end_to_end, idx_gb, aknn_build_s, verifier_train_s, build_s, Marg, dl, 0.0, 0.0);
-    if (env_flag("ANQI_BUILD_BREAKDOWN"))
-        std::printf("[rknn] BUILD_BREAKDOWN aknn_source_s=%.6f verifier_train_s=%.6f source_prepare_s=%.6f "
-                    "lift_verifier_prepare_s=%.6f graph_index_s=%.6f index_total_s=%.6f end_to_end_s=%.6f "
-                    "verifier_encode_save_ms=%.3f\n",
-                    aknn_build_s, verifier_train_s, source_prepare_s, lift_verifier_prepare_s, graph_index_s, build_s,
-                    end_to_end, encode_save_ms);
+    const double build_s = since(tb0) - ctx_s;
+    const double idx_gb = ((double)n * Marg * 4 + (double)n * dl * 4 + verifier_bytes) / 1e9;
+    if (can_load && graph_loaded) {
+        std::printf("[rknn] INDEX_LOAD_TIME=%.1fs INDEX_SIZE=%.2fGB (graph M=%zu + graph dim=%u + verification=%.3fGB "
+                    "+ envelope=%.3fGB)\n", build_s, idx_gb, Marg, dl, verifier_bytes / 1e9, 0.0);
+        if (env_flag("ANQI_BUILD_BREAKDOWN"))
+            std::printf("[rknn] LOAD_BREAKDOWN index_load_s=%.6f verifier_load_ms=%.3f\n", build_s, verifier_load_ms);
+    } else {
+        const double aknn_build_s = external_seconds("ANQI_AKNN_BUILD_SECONDS");
+        const double verifier_train_s = external_seconds("ANQI_VERIFIER_TRAIN_SECONDS");
+        const double end_to_end = aknn_build_s + verifier_train_s + build_s;
+        std::printf("[rknn] BUILD_TIME=%.1fs INDEX_SIZE=%.2fGB (aknn_source=%.1fs + verifier_training=%.1fs + "
+                    "index=%.1fs, graph M=%zu + graph dim=%u + verification=%.3fGB + envelope=%.3fGB)\n",
+                    end_to_end, idx_gb, aknn_build_s, verifier_train_s, build_s, Marg, dl, verifier_bytes / 1e9, 0.0);
+        if (env_flag("ANQI_BUILD_BREAKDOWN"))
+            std::printf("[rknn] BUILD_BREAKDOWN aknn_source_s=%.6f verifier_train_s=%.6f source_prepare_s=%.6f "
+                        "lift_verifier_prepare_s=%.6f graph_index_s=%.6f index_total_s=%.6f end_to_end_s=%.6f "
+                        "verifier_encode_save_ms=%.3f\n",
+                        aknn_build_s, verifier_train_s, source_prepare_s, lift_verifier_prepare_s, graph_index_s,
+                        build_s, end_to_end, encode_save_ms);
+    }
     std::fflush(stdout);
 
     // thresholds for Kq (materialize_k, rknn_query.cpp:720-753, slack 1)

@@ -4,6 +4,8 @@ before any GPU work, so they run here; where the reference binaries are built
 (oracle/_ref) the same invocation is run through them and the exit codes
 compared."""
 import os
+import re
+import shutil
 import subprocess
 
 import numpy as np
@@ -91,7 +93,63 @@ def test_rknn_query_rejects_unpinned_configuration():
     instead of silently computing the pinned one."""
     pfx = os.path.join(GOLDEN, "sample", "sample")
     for env in ({"ANQI_RADIUS_MODE": "approx"}, {"ANQI_VERIFIER_PLACEMENT": "integrated"},
-                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_REUSE_ORIG_KNN": "1"}):
+                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_REUSE_ORIG_KNN": "1"},
+                {"ANQI_VAMANA": "0"}, {"ANQI_RADIUS_RELAX": "0.5"}):
         r = run("rknn_query", ["16", pfx, "48", "500", "64", "600", "10"], env)
         assert r.returncode == 2, (env, r.returncode, r.stderr)
         assert "not supported" in r.stderr
+    # ANQI_SPECIFIC is on when set at all; the reference refuses it with a
+    # post-filter verifier (src/rknn_query.cpp:1806-1811)
+    r = run("rknn_query", ["16", pfx, "48", "500", "64", "600", "10"], {"ANQI_SPECIFIC": "0"})
+    assert r.returncode == 2 and "unset ANQI_SPECIFIC" in r.stderr
+
+
+# scripts/run_final.sh's exports
+PIN_ENV = {
+    "ANQI_GRAPH_GEOMETRY": "anyk_lift", "ANQI_RADIUS_MODE": "rankm_resid_u8_lrq_floor",
+    "ANQI_VERIFIER_PLACEMENT": "postfilter", "ANQI_RANKM": "8", "ANQI_NK": "100", "ANQI_GRAPH_NK": "100",
+    "ANQI_RANKM_SLACK": "0", "ANQI_RECHECK": "0", "ANQI_RQ_OBJECTIVE": "radius_reconstruction",
+    "ANQI_ALPHA": "1.2", "ANQI_LIFTNNK": "50", "ANQI_GRAPHNND": "6", "ANQI_BUILD_BREAKDOWN": "1",
+    "ANQI_PREBUILT_TOPK_K": "100", "ANQI_BENCH_CALIBRATED_TIMING": "0", "OMP_NUM_THREADS": "2",
+}
+SAMPLE_FILES = ("_base.bin", "_query.bin", "_lifted.meta", "_baseknn_aknn.rec", "_radbasis_M8.bin",
+                "_learned_rq_codebook_M8_u8.bin", "_rknn_gt_k10.bin", "_rknn_gt_k50.bin")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFBIN, "rknn_query")), reason="reference tools not built here")
+def test_cache_names_equal_reference(tmp_path):
+    """The reference's rknn_query builds its caches for a copy of the sample;
+    rknn_query here, with ANQI_REQUIRE_GRAPH_CACHE=1 on the same files, prints
+    the same graph cache key, finds every cache file under the names it
+    computes (state file and its key, .built marker, lifted vectors, lift meta,
+    generate_config graph index) and loads the verifier artifact the reference
+    wrote, so it passes the cache check and goes on to the GPU.  A rewritten
+    .rec (new mtime/ctime in its stat identity) changes the key in both tools
+    the same way, and both then refuse the required cache (exit 3)."""
+    pfx = str(tmp_path / "sample")
+    for suf in SAMPLE_FILES:
+        shutil.copy(os.path.join(GOLDEN, "sample", "sample" + suf), pfx + suf)
+    env = dict(PIN_ENV, ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec",
+               ANQI_GRAPH_CACHE_DIR=str(tmp_path / "graph"), ANQI_VERIFIER_CACHE_DIR=str(tmp_path / "verifier"))
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    args = ["16", pfx, "48", "500", "64", "600", "10"]
+    rr = run("rknn_query", args, env, ref=True)
+    assert rr.returncode == 0, rr.stderr
+    key = re.search(r"graph cache key=(\S+)", rr.stdout).group(1)
+    ver = re.search(r"built independent compact verifier mode=\S+ bits=8 (\S+) ", rr.stdout).group(1)
+    assert sum(f.startswith("fd_metric_l2_M_48_") for f in os.listdir(env["ANQI_GRAPH_CACHE_DIR"])) == 1
+    req = dict(env, ANQI_REQUIRE_GRAPH_CACHE="1")
+    r = run("rknn_query", args, req)
+    assert f"[rknn] graph cache key={key}\n" in r.stdout
+    assert f"loaded independent compact verifier mode=rank-M+LRQ bits=8 {ver} " in r.stdout
+    assert "required graph cache" not in r.stderr
+    assert r.returncode == 0 or (r.returncode == 3 and "GPU failure in context" in r.stderr), r.stderr
+    st = os.stat(pfx + "_baseknn_aknn.rec")
+    os.utime(pfx + "_baseknn_aknn.rec", (st.st_atime + 60, st.st_mtime + 60))
+    keys = []
+    for ref in (True, False):
+        r = run("rknn_query", args, req, ref=ref)
+        assert r.returncode == 3 and "required graph cache is incomplete or incompatible" in r.stderr, (ref, r.stderr)
+        keys.append(re.search(r"graph cache key=(\S+)", r.stdout).group(1))
+    assert keys[0] == keys[1] != key

@@ -6,6 +6,7 @@ build_aknn_source writes a valid .rec whose recall against the exact top-100
 is not below the reference source's."""
 import os
 import re
+import shutil
 import subprocess
 
 import numpy as np
@@ -100,3 +101,85 @@ def test_dataset_gt_sample_files_equal_reference(tmp_path):
         assert r.returncode == 0, r.stderr
         assert open(o + "_rknn_gt.bin", "rb").read() == open(pfx + f"_rknn_gt_k{k}.bin", "rb").read(), k
 
+
+
+def answers(text):
+    """The per-L answer lines (recall, precision, F1, candidates, distances),
+    the QPS field cut off."""
+    out = []
+    for line in text.splitlines():
+        if line.startswith("[rknn] L_descent="):
+            out.append(line.split("| QPS")[0])
+        elif line.startswith(("[rknn] DIST_BREAKDOWN", "[rknn] CANDIDATE_RECALL")):
+            out.append(line)
+    return out
+
+
+def cli_env(tmp_path, name):
+    pfx = os.path.join(GOLDEN, "sample", "sample")
+    env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
+    env.update(PIN_ENV)
+    env["ANQI_PREBUILT_TOPK_REC"] = pfx + "_baseknn_aknn.rec"
+    env["ANQI_GRAPH_CACHE_DIR"] = str(tmp_path / name / "graph")
+    env["ANQI_VERIFIER_CACHE_DIR"] = str(tmp_path / name / "verifier")
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    return pfx, env
+
+
+def rknn(pfx, env, k):
+    say(f"rknn_query sample k={k} caches {env['ANQI_GRAPH_CACHE_DIR']}")
+    r = subprocess.run([os.path.join(BIN, "rknn_query"), "16,32,64,128", pfx, "48", "500", "64", "600", str(k)],
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+BUILD_LINES = ("original-space AKNN", "built independent compact verifier", "BUILD lift", "BUILD_TIME", "Index saved")
+
+
+def test_rknn_query_second_run_loads_its_caches(tmp_path):
+    """A run at k = 10 builds and writes the caches; a second run at k = 50
+    loads the lift, graph and verifier (no build line) and answers exactly as
+    a fresh build at k = 50 does (src/rknn_query.cpp:1871-1894, 2052-2077)."""
+    pfx, env = cli_env(tmp_path, "shared")
+    first = rknn(pfx, env, 10)
+    assert all(s in first for s in BUILD_LINES)
+    second = rknn(pfx, env, 50)
+    say(second[-1500:])
+    assert "[rknn] loaded cached index (M=48)" in second and "INDEX_LOAD_TIME" in second
+    assert "loaded independent compact verifier" in second and "Loading index from:" in second
+    assert not any(s in second for s in BUILD_LINES)
+    _, fresh_env = cli_env(tmp_path, "fresh")
+    fresh = rknn(pfx, fresh_env, 50)
+    assert "BUILD_TIME" in fresh
+    assert answers(second) == answers(fresh) and len(answers(fresh)) == 12
+
+
+def test_rknn_query_loads_reference_caches(tmp_path):
+    """The reference's own cache files for the sample (graph index, lifted
+    vectors, lift meta and verifier artifact of the reference run that wrote
+    tests/golden/sample) put under the names this tool computes: runs at
+    k = 10, 50 and 100 load them and print the reference's lift line and, per
+    L_descent, its recall, precision, F1, candidate and distance counts
+    exactly."""
+    pfx, env = cli_env(tmp_path, "ref")
+    rknn(pfx, env, 10)
+    g, v = env["ANQI_GRAPH_CACHE_DIR"], env["ANQI_VERIFIER_CACHE_DIR"]
+    names = os.listdir(g)
+    swap = {"fd_metric_l2_M_48_": "_graph.bin", "_approxlift_graph_": "_approxlift.bin",
+            "_anqi_meta_graph_": "_graph_meta.txt"}
+    for head, suf in swap.items():
+        mine = [f for f in names if f.startswith(head)]
+        assert len(mine) == 1, (head, names)
+        shutil.copy(pfx + suf, os.path.join(g, mine[0]))
+    ver = os.listdir(v)
+    assert len(ver) == 1
+    shutil.copy(pfx + "_verifier.anqivf2", os.path.join(v, ver[0]))
+    for k in (10, 50, 100):
+        out = rknn(pfx, env, k)
+        ref = open(pfx + f"_rknn_stdout_k{k}.txt").read()
+        assert not any(s in out for s in BUILD_LINES), k
+        lift = re.search(r"M\*\^2=\S+ scale=\S+", ref.split("BUILD lift" if k == 10 else "loaded cached index")[1])
+        assert f"[rknn] loaded cached index (M=48): {lift.group(0)}" in out, k
+        assert answers(out) == answers(ref), (k, answers(out), answers(ref))
