@@ -625,14 +625,24 @@ int main(int argc, char** argv) {
         std::printf("[rknn] flattened_repeats=1 tasks=%zu\n", (size_t)nq * bench_repeats);
 
     anqi_index ix{d_base, d_bnorm, d_lifted, d_deg, d_links, n, d, dl, (uint32_t)Marg, &tree, mstar2, scale,
-                  nullptr, nullptr};
+                  nullptr, nullptr, nullptr, nullptr};
     if (anqi_cli::gpu_form_chosen() && d % 16 == 0) {
-        // persistent search: aligned main rows + precomputed lift terms
+        // persistent search: aligned main rows, precomputed lift terms and
+        // sentinel adjacency rows
         float* d_lmain = dalloc<float>((size_t)n * d);
         float* d_ltail = dalloc<float>(n);
+        uint32_t* d_plinks = dalloc<uint32_t>((size_t)n * Marg);
         ck(anqi_lift_split(g_ctx, d_lifted, n, d, d_lmain, d_ltail), "lift split");
+        ck(anqi_pack_links(g_ctx, d_links, d_deg, n, (uint32_t)Marg, d_plinks), "sentinel adjacency");
         ix.lmain = d_lmain;
         ix.ltail = d_ltail;
+        ix.plinks = d_plinks;
+        // byte rows when every base value is a float 0..255 (certified)
+        uint8_t* d_brows = dalloc<uint8_t>((size_t)n * d);
+        int exact = 0;
+        ck(anqi_byte_rows(g_ctx, d_base, d_lmain, n, d, scale, d_brows, &exact), "byte rows");
+        if (exact) ix.brows = d_brows;
+        else anqi_dfree(g_ctx, d_brows);
     }
     float* d_query = upload(query);
     uint32_t out_cap = 4096;

@@ -434,6 +434,30 @@ int anqi_lift_split(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t d, 
     return ANQI_OK;
 }
 
+int anqi_byte_rows(anqi_ctx* ctx, const float* base, const float* lmain, uint32_t n, uint32_t d, double scale,
+                   uint8_t* brows, int* exact) {
+    if (!base || !lmain || !brows || !exact || d == 0 || (d & 15u) || d > 16 * QMAX_CHUNKS)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "byte_rows: null array, or d not a multiple of 16 up to %d",
+                            16 * QMAX_CHUNKS);
+    DBuf bad;
+    HCHK(bad.alloc(256));
+    HCHK(launch_fill_u32(bad.as<uint32_t>(), 0u, 1, ctx->stream));
+    HCHK(launch_byte_rows(base, lmain, n, d, scale, brows, bad.as<uint32_t>(), ctx->stream));
+    uint32_t nbad = 0;
+    HCHK(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    *exact = nbad == 0;
+    return ANQI_OK;
+}
+
+int anqi_pack_links(anqi_ctx* ctx, const uint32_t* links, const uint32_t* deg, uint32_t n, uint32_t M,
+                    uint32_t* plinks) {
+    if (!links || !deg || !plinks || M == 0 || M > 64)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "pack_links: null array or M outside 1..64");
+    HCHK(launch_pack_links(links, deg, n, M, plinks, ctx->stream));
+    return ANQI_OK;
+}
+
 // ---- lift (host FP64; src/rknn_query.cpp:2203-2236) ------------------------------------------
 int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
                    const float* basis, const float* codebook, uint32_t code_size, float tau, int mode,
@@ -1438,7 +1462,7 @@ static int search_run(anqi_ctx* ctx, const anqi_index* ix, const float* queries,
     hipStream_t s = ctx->stream;
     SearchParams P{};
     P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
-    P.lmain = ix->lmain; P.ltail = ix->ltail;
+    P.lmain = ix->lmain; P.ltail = ix->ltail; P.plinks = ix->plinks; P.brows = ix->brows;
     P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
     P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
     P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;
@@ -1597,7 +1621,7 @@ int anqi_rknn_search_noalloc(anqi_ctx* ctx, const anqi_index* ix, const float* q
 static void fill_search_params(SearchParams& P, const anqi_index* ix) {
     P = SearchParams{};
     P.base = ix->base; P.bnorm = ix->bnorm; P.lifted = ix->lifted; P.deg = ix->deg; P.links = ix->links;
-    P.lmain = ix->lmain; P.ltail = ix->ltail;
+    P.lmain = ix->lmain; P.ltail = ix->ltail; P.plinks = ix->plinks; P.brows = ix->brows;
     P.n = ix->n; P.d = ix->d; P.dl = ix->dl; P.M = ix->M;
     P.t_left = ix->tree->left; P.t_right = ix->tree->right; P.t_off = ix->tree->offset;
     P.t_hyper = ix->tree->hyper; P.t_entry = ix->tree->entry; P.t_root = ix->tree->root;

@@ -153,65 +153,88 @@ __device__ __forceinline__ float qrow_l2(const QRow& q, const float* __restrict_
 
 // Batched row distances: for rows ids[s], s < k, group g (16 lanes) takes
 // s = g, g+4, ..., g + 4 (RB - 1) of each 4 RB-row pass and issues all RB
-// rows' loads before reducing any, so the fetches overlap.  Same per-lane
-// FMA order and reduction tree as qrow_l2 / qrow_ip, hence the same bits.
+// rows' loads before reducing any, so the fetches overlap; the last pass
+// halves RB while the rows left fit, so a short batch issues no loads or
+// arithmetic for absent rows.  Same per-lane FMA order and reduction tree as
+// qrow_l2 / qrow_ip, hence the same bits.
 // MC = number of full 16-float chunks of a row of X (dim = 16 MC + r, r < 16).
 //   MODE 0: out[s] = ||q - x||^2
 //   MODE 1: out[s] = d2_combine(<q, x>, nq, nrm[id])   (nndindex::d2)
 //   MODE 2: out[s] = ||q - x||^2 over the dim (= 16 MC) coordinates of X's rows
 //           + nrm[id] (the rest of the sum, already reduced: split lifted rows)
+template <int MC, int MODE, int RB>
+__device__ __forceinline__ void rows_pass(const QRow& q, const float* __restrict__ X, uint32_t dim,
+                                          const float* __restrict__ nrm, float nq, const uint32_t* ids, int k,
+                                          int l, int g, float* out, int s0) {
+    const int r = (int)dim - 16 * MC;
+    float x[RB][MC];
+    float xt[RB], nb[RB];
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+        const int s = s0 + g + 4 * j;
+        if (s < k) {
+            const uint32_t id = ids[s];
+            const float* row = X + (size_t)id * dim;
+#pragma unroll
+            for (int t = 0; t < MC; t++) x[j][t] = row[16 * t + l];
+            xt[j] = (l < r) ? row[16 * MC + l] : 0.0f;
+            nb[j] = MODE != 0 ? nrm[id] : 0.0f;
+        } else {
+#pragma unroll
+            for (int t = 0; t < MC; t++) x[j][t] = 0.0f;
+            xt[j] = 0.0f;
+            nb[j] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+        float acc = 0.0f;
+        float res;
+        if (MODE != 1) {
+#pragma unroll
+            for (int t = 0; t < MC; t++) {
+                float a = q.v[t] - x[j][t];
+                acc = fmaf(a, a, acc);
+            }
+            res = tree16(acc);
+            if (MODE == 0 && r) {
+                float a = q.tail - xt[j];
+                res += tree16(a * a);
+            }
+            if (MODE == 2) res += nb[j];
+        } else {
+#pragma unroll
+            for (int t = 0; t < MC; t++) acc = fmaf(q.v[t], x[j][t], acc);
+            res = tree16(acc);
+            if (r) res += tree16(q.tail * xt[j]);
+            res = fmaf(-2.0f, res, nq + nb[j]);
+        }
+        const int s = s0 + g + 4 * j;
+        if (l == 0 && s < k) out[s] = res;
+    }
+}
+
+// the last pass: 0 < k - s0 <= 4 RB rows
+template <int MC, int MODE, int RB>
+__device__ __forceinline__ void rows_tail(const QRow& q, const float* __restrict__ X, uint32_t dim,
+                                          const float* __restrict__ nrm, float nq, const uint32_t* ids, int k,
+                                          int l, int g, float* out, int s0) {
+    if constexpr (RB > 1) {
+        if (k - s0 <= 2 * RB) {
+            rows_tail<MC, MODE, RB / 2>(q, X, dim, nrm, nq, ids, k, l, g, out, s0);
+            return;
+        }
+    }
+    rows_pass<MC, MODE, RB>(q, X, dim, nrm, nq, ids, k, l, g, out, s0);
+}
+
 template <int MC, int MODE, int RB = 4>
 __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restrict__ X, uint32_t dim,
                                            const float* __restrict__ nrm, float nq,
                                            const uint32_t* ids, int k, int l, int g, float* out) {
-    const int r = (int)dim - 16 * MC;
-    for (int s0 = 0; s0 < k; s0 += 4 * RB) {
-        float x[RB][MC];
-        float xt[RB], nb[RB];
-#pragma unroll
-        for (int j = 0; j < RB; j++) {
-            const int s = s0 + g + 4 * j;
-            if (s < k) {
-                const uint32_t id = ids[s];
-                const float* row = X + (size_t)id * dim;
-#pragma unroll
-                for (int t = 0; t < MC; t++) x[j][t] = row[16 * t + l];
-                xt[j] = (l < r) ? row[16 * MC + l] : 0.0f;
-                nb[j] = MODE != 0 ? nrm[id] : 0.0f;
-            } else {
-#pragma unroll
-                for (int t = 0; t < MC; t++) x[j][t] = 0.0f;
-                xt[j] = 0.0f;
-                nb[j] = 0.0f;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < RB; j++) {
-            float acc = 0.0f;
-            float res;
-            if (MODE == 0 || MODE == 2) {
-#pragma unroll
-                for (int t = 0; t < MC; t++) {
-                    float a = q.v[t] - x[j][t];
-                    acc = fmaf(a, a, acc);
-                }
-                res = tree16(acc);
-                if (MODE == 0 && r) {
-                    float a = q.tail - xt[j];
-                    res += tree16(a * a);
-                }
-                if (MODE == 2) res += nb[j];
-            } else {
-#pragma unroll
-                for (int t = 0; t < MC; t++) acc = fmaf(q.v[t], x[j][t], acc);
-                res = tree16(acc);
-                if (r) res += tree16(q.tail * xt[j]);
-                res = fmaf(-2.0f, res, nq + nb[j]);
-            }
-            const int s = s0 + g + 4 * j;
-            if (l == 0 && s < k) out[s] = res;
-        }
-    }
+    int s0 = 0;
+    for (; k - s0 > 4 * RB; s0 += 4 * RB) rows_pass<MC, MODE, RB>(q, X, dim, nrm, nq, ids, k, l, g, out, s0);
+    if (s0 < k) rows_tail<MC, MODE, RB>(q, X, dim, nrm, nq, ids, k, l, g, out, s0);
 }
 
 // nndindex::d2 (include/index.h:172-176): na + nb - 2 ip.

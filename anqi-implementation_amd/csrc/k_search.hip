@@ -100,20 +100,126 @@ __device__ __forceinline__ int vis_insert(const SearchParams& P, const SearchLds
     }
 }
 
+// Node c's adjacency row, one neighbour per lane (U32_NONE on lanes >= M).
+// Lanes below dg may hold neighbours: dg is the degree, or M with the
+// sentinel rows (P.plinks: U32_NONE past the degree, so no degree read).  A
+// lane holds a neighbour iff lane < dg and its id is not U32_NONE.  PK: the
+// sentinel rows (P.plinks, set exactly when the kernel is instantiated with PK).
+template <bool PK>
+__device__ __forceinline__ void adj_load(const SearchParams& P, uint32_t c, uint32_t& dg, uint32_t& nbr) {
+    const uint32_t lane = (uint32_t)__lane_id();
+    const uint32_t* row = (PK ? P.plinks : P.links) + (size_t)c * P.M;
+    nbr = lane < P.M ? row[lane] : U32_NONE;
+    dg = PK ? P.M : P.deg[c];
+}
+
 // Expands node c: visits its unvisited neighbours, computes D, appends
 // collected ones to the FIFO.  In beam mode also merges them into the beam.
 // Returns false on a capacity overflow.
+// ---- byte rows -----------------------------------------------------------
+// When every base value is one of the floats 0..255, a row is kept as d bytes
+// (P.brows, d = 16 MC), ordered for the 16-lane groups: lane l's MC bytes
+// l MC .. l MC + MC - 1 are elements l, 16 + l, ..., 16 (MC - 1) + l.  The
+// base value is float(b); the lifted main coordinate is
+// float(double(b) * scale), the lift's own binary64 product and rounding
+// (rknn_query.cpp:2203-2236), so both are the bits of the float rows.
+// anqi_byte_rows certifies that on every element before an index uses it.
+template <int MC>
+__device__ __forceinline__ void brow_fetch(const uint8_t* __restrict__ B, uint32_t id, int l, uint32_t (&w)[2]) {
+    const uint8_t* p = B + (size_t)id * (16 * MC) + (size_t)l * MC;
+    if constexpr (MC == 8) {
+        const uint2 v = *reinterpret_cast<const uint2*>(p);
+        w[0] = v.x;
+        w[1] = v.y;
+    } else if constexpr (MC % 2 == 0) {
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(p);
+        w[0] = 0u;
+        w[1] = 0u;
+#pragma unroll
+        for (int i = 0; i < MC / 2; i++) w[i >> 1] |= (uint32_t)h[i] << (16 * (i & 1));
+    } else {
+        w[0] = 0u;
+        w[1] = 0u;
+#pragma unroll
+        for (int i = 0; i < MC; i++) w[i >> 2] |= (uint32_t)p[i] << (8 * (i & 3));
+    }
+}
+__device__ __forceinline__ uint32_t brow_byte(const uint32_t (&w)[2], int t) {
+    return (w[t >> 2] >> (8 * (t & 3))) & 255u;
+}
+// the lifted coordinate of byte value b (scale: the lift's binary64 scale)
+__device__ __forceinline__ float brow_lifted(uint32_t b, double scale) { return (float)((double)b * scale); }
+
+// rows_batch MODE 2 / 3 (split lifted rows) over byte rows: the same FMA
+// order and reduction tree, the coordinates rebuilt from the bytes.
+template <int MC, int RB>
+__device__ __forceinline__ void bytes_pass(const QRow& q, const uint8_t* __restrict__ B,
+                                           const float* __restrict__ ltail, double scale, const uint32_t* ids, int k,
+                                           int l, int g, float* out, int s0) {
+    uint32_t w[RB][2];
+    float nb[RB];
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+        const int s = s0 + g + 4 * j;
+        w[j][0] = 0u;
+        w[j][1] = 0u;
+        nb[j] = 0.0f;
+        if (s < k) {
+            const uint32_t id = ids[s];
+            brow_fetch<MC>(B, id, l, w[j]);
+            nb[j] = ltail[id];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < MC; t++) {
+            const float a = q.v[t] - brow_lifted(brow_byte(w[j], t), scale);
+            acc = fmaf(a, a, acc);
+        }
+        const float res = tree16(acc) + nb[j];
+        const int s = s0 + g + 4 * j;
+        if (l == 0 && s < k) out[s] = res;
+    }
+}
+
+template <int MC, int RB>
+__device__ __forceinline__ void bytes_tail(const QRow& q, const uint8_t* __restrict__ B,
+                                           const float* __restrict__ ltail, double scale, const uint32_t* ids, int k,
+                                           int l, int g, float* out, int s0) {
+    if constexpr (RB > 1) {
+        if (k - s0 <= 2 * RB) {
+            bytes_tail<MC, RB / 2>(q, B, ltail, scale, ids, k, l, g, out, s0);
+            return;
+        }
+    }
+    bytes_pass<MC, RB>(q, B, ltail, scale, ids, k, l, g, out, s0);
+}
+
+template <int MC, int RB = 4>
+__device__ __forceinline__ void rows_batch_bytes(const QRow& q, const uint8_t* __restrict__ B,
+                                                 const float* __restrict__ ltail, double scale,
+                                                 const uint32_t* ids, int k, int l, int g, float* out) {
+    int s0 = 0;
+    for (; k - s0 > 4 * RB; s0 += 4 * RB) bytes_pass<MC, RB>(q, B, ltail, scale, ids, k, l, g, out, s0);
+    if (s0 < k) bytes_tail<MC, RB>(q, B, ltail, scale, ids, k, l, g, out, s0);
+}
+
 // D for the k new neighbours tid[0..k).  Group g (16 lanes) takes rows g,
 // g+4, g+8, g+12 of each 16-row pass and loads all four before reducing any,
 // so their fetches overlap.  MC = number of full 16-float chunks of a lifted
 // row (d' = 16 MC + r, r < 16); MC == 0 selects the generic loop.
 // Split layout (P.lmain, d = 16 MC): the d leading coordinates come from the
-// aligned rows of lmain and the two lift coordinates' reduced term from
-// ltail; D = tree16(main) + tail, the same two terms the whole-row form adds.
-template <int MC>
+// aligned rows of lmain (or the byte rows) and the two lift coordinates'
+// reduced term from ltail; D = tree16(main) + tail, the same two terms the
+// whole-row form adds.
+template <int MC, bool BR>
 __device__ __forceinline__ void batch_l2(const SearchParams& P, const QRow& QL, const uint32_t* tid,
                                          float* tdv, int k, int l, int g) {
-    if constexpr (MC > 0) {
+    if constexpr (MC > 0 && BR) {
+        rows_batch_bytes<MC>(QL, P.brows, P.ltail, __longlong_as_double((long long)P.scale_bits), tid, k, l, g, tdv);
+    } else if constexpr (MC > 0) {
         if (P.lmain) rows_batch<MC, 2>(QL, P.lmain, P.d, P.ltail, 0.0f, tid, k, l, g, tdv);
         else rows_batch<MC, 0>(QL, P.lifted, P.dl, nullptr, 0.0f, tid, k, l, g, tdv);
     } else {
@@ -124,17 +230,17 @@ __device__ __forceinline__ void batch_l2(const SearchParams& P, const QRow& QL, 
     }
 }
 
-template <bool BEAM, int MC, bool GV>
+template <bool BEAM, int MC, bool GV, bool PK, bool BR>
 __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, const QRow& QL,
                                        uint32_t c, uint32_t& rcnt, uint32_t& nd,
                                        int& buf, uint32_t& bsize, uint32_t L, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
-    // degree and neighbour row fetched together (slots past deg are ignored)
-    const uint32_t dg = P.deg[c];
-    const uint32_t nbr = ((uint32_t)lane < P.M) ? P.links[(size_t)c * P.M + lane] : U32_NONE;
+    // degree and neighbour row fetched together
+    uint32_t dg, nbr;
+    adj_load<PK>(P, c, dg, nbr);
     uint32_t nb = U32_NONE;
     int fresh = 0;
-    if ((uint32_t)lane < dg) {
+    if ((uint32_t)lane < dg && nbr != U32_NONE) {
         nb = nbr;
         int r = vis_insert<GV>(P, S, nb);
         if (r < 0) ovf = 1;
@@ -145,7 +251,7 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     int k = popc64(fm);
     if (fresh) S.tid[lanes_below(fm)] = nb;
     __syncthreads();
-    batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
+    batch_l2<MC, BR>(P, QL, S.tid, S.tdv, k, l, g);
     __syncthreads();
     nd += (uint32_t)k;
     float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
@@ -198,15 +304,15 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
 }
 
 // Register-resident beam (persistent form): an RKeys list (common.hpp).
-// dg, nbr: node c's degree and adjacency row (lane-held), fetched by the caller.
-template <int NCH, int MC>
+// dg, nbr: node c's adjacency row (adj_load), fetched by the caller.
+template <int NCH, int MC, bool PK, bool BR>
 __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t dg,
                                           uint32_t nbr, uint32_t& rcnt, uint32_t& nd, RKeys<NCH>& B,
                                           uint32_t& bsize, uint32_t L, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     uint32_t nb = U32_NONE;
     int fresh = 0;
-    if ((uint32_t)lane < dg) {
+    if ((uint32_t)lane < dg && nbr != U32_NONE) {
         nb = nbr;
         int r = hset_insert(S.hash, P.hmask, nb);
         if (r < 0) ovf = 1;
@@ -217,7 +323,7 @@ __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, c
     const int k = popc64(fm);
     if (fresh) S.tid[lanes_below(fm)] = nb;
     __syncthreads();
-    batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
+    batch_l2<MC, BR>(P, QL, S.tid, S.tdv, k, l, g);
     __syncthreads();
     nd += (uint32_t)k;
     const float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
@@ -239,7 +345,7 @@ __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, c
 // that same order.  Their distances are computed in one batch.  The step
 // stops before a node whose neighbours might not fit the id buffer; nb is
 // set to the number of nodes expanded (at least one: deg <= 64 <= SR_TID).
-template <int MC, int FL, bool GV>
+template <int MC, int FL, bool GV, bool PK, bool BR>
 __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, const QRow& QL, uint32_t head,
                                            uint32_t& nb, uint32_t& rcnt, uint32_t& nd, int& ovf) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
@@ -248,20 +354,18 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
     for (int t = 0; t < FL; t++) {
         dg[t] = 0;
         nbr[t] = U32_NONE;
-        if ((uint32_t)t < nb) {
-            const uint32_t c = fifo_get(P, S, head + t);
-            dg[t] = P.deg[c];
-            nbr[t] = ((uint32_t)lane < P.M) ? P.links[(size_t)c * P.M + lane] : U32_NONE;
-        }
+        if ((uint32_t)t < nb) adj_load<PK>(P, fifo_get(P, S, head + t), dg[t], nbr[t]);
     }
     int k = 0;
     uint32_t done = nb;
 #pragma unroll
     for (int t = 0; t < FL; t++) {
-        if ((uint32_t)t < done && k + (int)dg[t] > SR_TID) done = (uint32_t)t;
+        const int has = (uint32_t)lane < dg[t] && nbr[t] != U32_NONE;
+        const int deg = PK ? popc64(ballot(has)) : (int)dg[t];
+        if ((uint32_t)t < done && k + deg > SR_TID) done = (uint32_t)t;
         if ((uint32_t)t >= done) continue;
         int fresh = 0;
-        if ((uint32_t)lane < dg[t]) {
+        if (has) {
             int r = vis_insert<GV>(P, S, nbr[t]);
             if (r < 0) ovf = 1;
             fresh = (r == 1);
@@ -273,7 +377,7 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
     nb = done;
     if (__any(ovf)) { ovf = 1; return false; }
     __syncthreads();
-    batch_l2<MC>(P, QL, S.tid, S.tdv, k, l, g);
+    batch_l2<MC, BR>(P, QL, S.tid, S.tdv, k, l, g);
     __syncthreads();
     nd += (uint32_t)k;
     for (int b = 0; b < k; b += 64) {
@@ -292,7 +396,7 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
 // NCH == 0: beam sorted in LDS (plain form); NCH > 0: register beam of
 // 64 NCH positions (persistent form, L <= 64 NCH).  FL: flood step width.
 // GV: global tier (NCH == 0 only).
-template <int MC, int NCH, int FL, bool GV = false>
+template <int MC, int NCH, int FL, bool GV = false, bool PK = false, bool BR = false>
 __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     const int lane = __lane_id(), l = lane & 15;
     S.spill = P.cand_ids + (size_t)qi * P.out_cap;
@@ -381,16 +485,12 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                 dg = pdg;
                 nbr = pnb;
             } else {
-                dg = P.deg[c];
-                nbr = ((uint32_t)lane < P.M) ? P.links[(size_t)c * P.M + lane] : U32_NONE;
+                adj_load<PK>(P, c, dg, nbr);
             }
             pc = take_next(false);
-            if (pc != U32_NONE) {
-                pdg = P.deg[pc];
-                pnb = ((uint32_t)lane < P.M) ? P.links[(size_t)pc * P.M + lane] : U32_NONE;
-            }
+            if (pc != U32_NONE) adj_load<PK>(P, pc, pdg, pnb);
             nx++;
-            if (!expand_rb<NCH, MC>(P, S, QL, dg, nbr, rcnt, nd, B, bsize, L, ovf)) break;
+            if (!expand_rb<NCH, MC, PK, BR>(P, S, QL, dg, nbr, rcnt, nd, B, bsize, L, ovf)) break;
         }
     } else for (;;) {
         uint32_t cur = U32_NONE;
@@ -405,13 +505,13 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
         if (lane == 0) S.bp(buf)[cur] = c | 0x80000000u;
         __syncthreads();
         nx++;
-        if (!expand<true, MC, GV>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
+        if (!expand<true, MC, GV, PK, BR>(P, S, QL, c, rcnt, nd, buf, bsize, L, ovf)) break;
     }
     // (2) flood over collected nodes, FIFO order
     uint32_t head = 0;
     while (!ovf && head < rcnt) {
         uint32_t nb = min((uint32_t)FL, rcnt - head);
-        if (!flood_step<MC, FL, GV>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
+        if (!flood_step<MC, FL, GV, PK, BR>(P, S, QL, head, nb, rcnt, nd, ovf)) break;
         head += nb;
         nx += nb;
     }
@@ -458,10 +558,18 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                 if (s < rcnt) {
                     const uint32_t o = fifo_get(P, S, s);
                     oid[j] = o;
-                    const float* row = P.base + (size_t)o * P.d;
+                    if constexpr (BR) {
+                        uint32_t w[2];
+                        brow_fetch<M1>(P.brows, o, l, w);
 #pragma unroll
-                    for (int t = 0; t < M1; t++) x[j][t] = row[16 * t + l];
-                    xt[j] = (l < rb) ? row[16 * M1 + l] : 0.0f;
+                        for (int t = 0; t < M1; t++) x[j][t] = (float)brow_byte(w, t);
+                        xt[j] = 0.0f;   // d = 16 MC
+                    } else {
+                        const float* row = P.base + (size_t)o * P.d;
+#pragma unroll
+                        for (int t = 0; t < M1; t++) x[j][t] = row[16 * t + l];
+                        xt[j] = (l < rb) ? row[16 * M1 + l] : 0.0f;
+                    }
                     bn[j] = P.bnorm[o];
                     th[j] = thr[o];
                 } else {
@@ -543,7 +651,7 @@ __global__ void __launch_bounds__(1024) k_status_list(const uint32_t* __restrict
 
 // Persistent form: a fixed grid pulls query indices from a device counter.
 // Every wave exits once the counter passes nq.
-template <int MC, int NCH>
+template <int MC, int NCH, bool PK, bool BR>
 __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) {
     extern __shared__ __align__(16) char lds[];
     SearchLds S = carve<false>(lds, P);
@@ -556,7 +664,7 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
         __syncthreads();
         if (t >= nq) break;
         uint32_t qi = P.qlist ? P.qlist[t] : t;
-        search_one<MC, NCH, SR_FLOOD_MAX>(P, S, qi);
+        search_one<MC, NCH, SR_FLOOD_MAX, false, PK, BR>(P, S, qi);
         __syncthreads();
     }
 }
@@ -679,11 +787,65 @@ hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float*
     return hipGetLastError();
 }
 
+// Byte rows (see brow_fetch): one thread per element.  *bad counts the
+// elements that are not one of the floats 0..255 (-0 included) or whose
+// lifted coordinate lmain[i][e] is not brow_lifted(b, scale) bit for bit.
+__global__ void k_byte_rows(const float* __restrict__ base, const float* __restrict__ lmain, uint32_t n, uint32_t d,
+                            double scale, uint8_t* __restrict__ brows, uint32_t* __restrict__ bad) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)n * d) return;
+    const uint32_t i = (uint32_t)(t / d), e = (uint32_t)(t % d);
+    const uint32_t mc = d >> 4, l = e & 15u, c = e >> 4;
+    const float x = base[t];
+    const uint32_t b = (x >= 0.0f && x <= 255.0f) ? (uint32_t)x : 0u;
+    const bool ok = __float_as_uint(x) == __float_as_uint((float)b) &&
+                    __float_as_uint(lmain[t]) == __float_as_uint(brow_lifted(b, scale));
+    brows[(size_t)i * d + l * mc + c] = (uint8_t)b;
+    if (!ok) atomicAdd(bad, 1u);
+}
+
+hipError_t launch_byte_rows(const float* base, const float* lmain, uint32_t n, uint32_t d, double scale,
+                            uint8_t* brows, uint32_t* bad, hipStream_t s) {
+    const uint64_t el = (uint64_t)n * d;
+    if (el == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_byte_rows, dim3((unsigned)((el + 255) / 256)), dim3(256), 0, s, base, lmain, n, d, scale,
+                       brows, bad);
+    return hipGetLastError();
+}
+
+// Sentinel adjacency: one thread per slot.
+__global__ void k_pack_links(const uint32_t* __restrict__ links, const uint32_t* __restrict__ deg, uint32_t n,
+                             uint32_t M, uint32_t* __restrict__ plinks) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)n * M) return;
+    const uint32_t i = (uint32_t)(t / M), j = (uint32_t)(t % M);
+    plinks[t] = j < deg[i] ? links[t] : U32_NONE;
+}
+
+hipError_t launch_pack_links(const uint32_t* links, const uint32_t* deg, uint32_t n, uint32_t M, uint32_t* plinks,
+                             hipStream_t s) {
+    const uint64_t slots = (uint64_t)n * M;
+    if (slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_links, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, links, deg, n, M,
+                       plinks);
+    return hipGetLastError();
+}
+
 // Chunk count shared by the lifted rows (d + 2) and the base rows (d).
 static int search_chunks(const SearchParams& P) {
     const int m = (int)(P.dl >> 4);
     if ((int)(P.d >> 4) != m) return 0;
     return (m == 8 || m == 6) ? m : 0;
+}
+
+// Split rows (and the packed adjacency, which needs them): persistent form
+// with whole 16-float chunks only.
+static void split_layout(SearchParams& P, bool persistent, int mc) {
+    if (!persistent || mc == 0 || (P.d & 15u) || !P.ltail) P.lmain = nullptr;
+    if (!P.lmain) {
+        P.plinks = nullptr;
+        P.brows = nullptr;
+    }
 }
 
 // Register-beam chunks for beams up to Lmax (0: Lmax > 512, LDS beam).
@@ -704,19 +866,41 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     if (i < n) p[i] = v;
 }
 
+template <int MC, int NCH, bool PK, bool BR>
+static hipError_t launch_persistent_pk(const SearchParams& P, uint32_t grid, size_t lds, hipStream_t s) {
+    const void* f = (const void*)k_range_search_persistent<MC, NCH, PK, BR>;
+    if (hipError_t e = set_lds_limit(f, lds); e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_range_search_persistent<MC, NCH, PK, BR>), dim3(grid), dim3(64), lds, s, P);
+    return hipSuccess;
+}
+
+// The persistent kernel for P's layout: packed adjacency and byte rows only
+// with the split layout (MC > 0).
+template <int MC, int NCH>
+static const void* persistent_kernel(const SearchParams& P) {
+    if constexpr (MC > 0) {
+        if (P.plinks && P.brows) return (const void*)k_range_search_persistent<MC, NCH, true, true>;
+        if (P.plinks) return (const void*)k_range_search_persistent<MC, NCH, true, false>;
+        if (P.brows) return (const void*)k_range_search_persistent<MC, NCH, false, true>;
+    }
+    return (const void*)k_range_search_persistent<MC, NCH, false, false>;
+}
+
 template <int MC, int NCH>
 static hipError_t launch_persistent_t(const SearchParams& P, uint32_t grid, size_t lds, hipStream_t s) {
-    if (hipError_t e = set_lds_limit((const void*)k_range_search_persistent<MC, NCH>, lds); e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_range_search_persistent<MC, NCH>), dim3(grid), dim3(64), lds, s, P);
-    return hipSuccess;
+    if constexpr (MC > 0) {
+        if (P.plinks && P.brows) return launch_persistent_pk<MC, NCH, true, true>(P, grid, lds, s);
+        if (P.plinks) return launch_persistent_pk<MC, NCH, true, false>(P, grid, lds, s);
+        if (P.brows) return launch_persistent_pk<MC, NCH, false, true>(P, grid, lds, s);
+    }
+    return launch_persistent_pk<MC, NCH, false, false>(P, grid, lds, s);
 }
 
 template <int MC>
 static hipError_t launch_search_mc(const SearchParams& P0, bool persistent, uint32_t grid, hipStream_t s) {
     SearchParams P = P0;
     P.rbeam = persistent && search_nch(P.Lmax) > 0;
-    // split rows: persistent form with whole 16-float chunks only
-    if (!persistent || MC == 0 || (P.d & 15u) || !P.ltail) P.lmain = nullptr;
+    split_layout(P, persistent, MC);
     size_t lds = search_lds_bytes(P);
     if (persistent) {
         hipError_t e = hipSuccess;
@@ -741,6 +925,8 @@ static hipError_t launch_global_mc(const SearchParams& P0, hipStream_t s) {
     SearchParams P = P0;
     P.rbeam = 0;
     P.lmain = nullptr;
+    P.plinks = nullptr;
+    P.brows = nullptr;
     const size_t lds = search_lds_bytes(P);
     if (hipError_t e = set_lds_limit((const void*)k_range_search_global<MC>, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_range_search_global<MC>, dim3(P.nq), dim3(64), lds, s, P);
@@ -778,9 +964,9 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
 }
 
 template <int MC, int NCH>
-static int occupancy_t(size_t lds) {
+static int occupancy_t(const SearchParams& P, size_t lds) {
     int occ = 0;
-    const void* f = (const void*)k_range_search_persistent<MC, NCH>;
+    const void* f = persistent_kernel<MC, NCH>(P);
     if (set_lds_limit(f, lds) != hipSuccess) return 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 64, lds) != hipSuccess) occ = 1;
     return occ;
@@ -789,17 +975,18 @@ static int occupancy_t(size_t lds) {
 template <int MC>
 static int occupancy_mc(const SearchParams& P, size_t lds) {
     switch (search_nch(P.Lmax)) {
-        case 1: return occupancy_t<MC, 1>(lds);
-        case 2: return occupancy_t<MC, 2>(lds);
-        case 4: return occupancy_t<MC, 4>(lds);
-        case 8: return occupancy_t<MC, 8>(lds);
-        default: return occupancy_t<MC, 0>(lds);
+        case 1: return occupancy_t<MC, 1>(P, lds);
+        case 2: return occupancy_t<MC, 2>(P, lds);
+        case 4: return occupancy_t<MC, 4>(P, lds);
+        case 8: return occupancy_t<MC, 8>(P, lds);
+        default: return occupancy_t<MC, 0>(P, lds);
     }
 }
 
 int search_persistent_occupancy(const SearchParams& P0) {
     SearchParams P = P0;
     P.rbeam = search_nch(P.Lmax) > 0;
+    split_layout(P, true, search_chunks(P));
     const size_t lds = search_lds_bytes(P);
     switch (search_chunks(P)) {
         case 8: return occupancy_mc<8>(P, lds);

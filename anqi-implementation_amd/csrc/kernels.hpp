@@ -46,6 +46,12 @@ struct SearchParams {
     // split lifted layout (persistent form, d % 16 == 0; null: lifted rows):
     const float* lmain;     // n x d: the lifted rows' first d coordinates
     const float* ltail;     // n: reduced term of each row's lift coordinates against a query's (1, 0)
+    // sentinel adjacency (with lmain only; null: links + deg): n x M ids,
+    // U32_NONE past the degree, so an expansion reads no degree
+    const uint32_t* plinks;
+    // byte rows (with lmain only; null: float rows): n x d bytes, lane-ordered
+    // (k_search.hip brow_fetch), certified by launch_byte_rows
+    const uint8_t* brows;
     // global-memory tier (k_range_search_global): block b runs query
     // qlist[b] with a visited bitmap of n bits at gvis + b * gstride words and
     // a candidate FIFO of n entries at gfifo + b * n, so no query outgrows it
@@ -72,6 +78,15 @@ hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, h
 // cut = max(0, keys) (n x K), horizon (optional, n) = cut[:, K-1].
 hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon,
                                hipStream_t s);
+// Byte rows of a base whose values are all floats 0..255: brows as
+// brow_fetch reads them; *bad (zeroed by the caller) counts elements whose
+// base value or lifted coordinate (lmain) the bytes do not reproduce.
+hipError_t launch_byte_rows(const float* base, const float* lmain, uint32_t n, uint32_t d, double scale,
+                            uint8_t* brows, uint32_t* bad, hipStream_t s);
+// Sentinel adjacency for the split layout: plinks[i M + j] = links[i M + j]
+// for j < deg[i], else U32_NONE.
+hipError_t launch_pack_links(const uint32_t* links, const uint32_t* deg, uint32_t n, uint32_t M, uint32_t* plinks,
+                             hipStream_t s);
 // Split lifted layout of an index: lmain (n x d), ltail (n).
 hipError_t launch_lift_split(const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail,
                              hipStream_t s);

@@ -57,7 +57,7 @@ class SearchTuning(ctypes.Structure):
 class Index(ctypes.Structure):
     _fields_ = [("base", P), ("bnorm", P), ("lifted", P), ("deg", P), ("links", P), ("n", U32), ("d", U32),
                 ("dl", U32), ("M", U32), ("tree", ctypes.POINTER(Tree)), ("mstar2", F64), ("scale", F64),
-                ("lmain", P), ("ltail", P)]
+                ("lmain", P), ("ltail", P), ("plinks", P), ("brows", P)]
 
 
 SIGS = {
@@ -79,6 +79,8 @@ SIGS = {
     "anqi_d2h": (I32, [P, P, P, ctypes.c_size_t]),
     "anqi_norms": (I32, [P, P, U64, U32, P]),
     "anqi_lift_split": (I32, [P, P, U32, U32, P, P]),
+    "anqi_pack_links": (I32, [P, P, P, U32, U32, P]),
+    "anqi_byte_rows": (I32, [P, P, P, U32, U32, F64, P, ctypes.POINTER(I32)]),
     "anqi_search_entries": (I32, [P, ctypes.POINTER(Index), P, U32, P]),
     "anqi_pair_d2": (I32, [P, P, P, U32, P, P, U64, P]),
     "anqi_exact_knn": (I32, [P, P, U32, U32, U32, I32, P, P]),
@@ -451,14 +453,39 @@ class Context:
         self.call("anqi_lift_split", dp(lifted), n, d, dp(lmain), dp(ltail))
         return lmain, ltail
 
-    def make_index(self, base, bnorm, lifted, deg, links, tree, mstar2, scale, split=True):
-        """split: give the persistent search the split lifted layout (d % 16 == 0)."""
-        lmain = ltail = None
+    def pack_links(self, links, deg):
+        """Sentinel adjacency (anqi_pack_links): n x M ids, UINT32_MAX past the degree."""
+        n, M = links.shape
+        plinks = self.empty((n, M), self.torch.int32)
+        self.call("anqi_pack_links", dp(links), dp(deg), n, M, dp(plinks))
+        return plinks
+
+    def byte_rows(self, base, lmain, scale):
+        """Byte rows (anqi_byte_rows): the n x d uint8 array, or None when the
+        base is not all floats 0..255 (the bytes would not be exact)."""
+        n, d = base.shape
+        brows = self.empty((n, d), self.torch.uint8)
+        exact = I32()
+        self.call("anqi_byte_rows", dp(base), dp(lmain), n, d, scale, dp(brows), ctypes.byref(exact))
+        return brows if exact.value else None
+
+    def make_index(self, base, bnorm, lifted, deg, links, tree, mstar2, scale, split=True, packed=True,
+                   byte_rows=True):
+        """split: give the persistent search the split lifted layout (d % 16 == 0);
+        packed: and the sentinel adjacency rows (no degree read);
+        byte_rows: and byte rows, when the base values are all floats 0..255."""
+        lmain = ltail = plinks = brows = None
         if split and base.shape[1] % 16 == 0:
             lmain, ltail = self.lift_split(lifted)
+            if packed:
+                plinks = self.pack_links(links, deg)
+            if byte_rows:
+                brows = self.byte_rows(base, lmain, scale)
         ix = Index(dp(base), dp(bnorm), dp(lifted), dp(deg), dp(links), base.shape[0], base.shape[1],
-                   lifted.shape[1], links.shape[1], ctypes.pointer(tree), mstar2, scale, dp(lmain), dp(ltail))
-        ix._keep = (base, bnorm, lifted, deg, links, tree, lmain, ltail)
+                   lifted.shape[1], links.shape[1], ctypes.pointer(tree), mstar2, scale, dp(lmain), dp(ltail),
+                   dp(plinks), dp(brows))
+        ix._keep = (base, bnorm, lifted, deg, links, tree, lmain, ltail, plinks, brows)
+        ix.byte_rows = brows is not None
         return ix
 
     def search_buffers(self, nq, out_cap):

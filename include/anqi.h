@@ -267,6 +267,12 @@ typedef struct {
      * persistent search reads the lifted rows); used when d % 16 == 0 */
     const float* lmain;     /* device n*d */
     const float* ltail;     /* device n */
+    /* optional, with lmain/ltail: sentinel adjacency from anqi_pack_links
+     * (NULL: the search reads links and deg) */
+    const uint32_t* plinks; /* device n*M */
+    /* optional, with lmain/ltail: byte rows from anqi_byte_rows when it
+     * reports them exact (NULL: the search reads float rows) */
+    const uint8_t* brows;   /* device n*d */
 } anqi_index;
 /* Split lifted layout for the persistent search: lmain = the first d
  * coordinates of each lifted row (rows of d floats, so a row covers d/32
@@ -276,6 +282,25 @@ typedef struct {
  * whole-row D.  No counterpart in the reference (a layout of
  * rknn_query.cpp:2203-2236's lifted rows).  d % 16 == 0. */
 int anqi_lift_split(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t d, float* lmain, float* ltail);
+/* Byte rows for the persistent search over the split layout, for a base
+ * whose values are all the floats 0..255 (SIFT-like data): brows holds each
+ * row's d values as bytes, ordered for the search's 16-lane groups.  The
+ * search rebuilds the base value float(b) and the lifted coordinate
+ * float(double(b) * scale) -- the lift's own binary64 product and rounding
+ * (rknn_query.cpp:2203-2236) -- from them, reading d bytes per distance
+ * instead of 4d.  *exact = 1 iff every element's base value and lmain
+ * coordinate are reproduced bit for bit; otherwise brows must not be used.
+ * No counterpart in the reference (a layout of its base and lifted rows).
+ * d % 16 == 0; scale: the lift's scale (anqi_lift). */
+int anqi_byte_rows(anqi_ctx* ctx, const float* base, const float* lmain, uint32_t n, uint32_t d, double scale,
+                   uint8_t* brows, int* exact);
+/* Sentinel adjacency for the persistent search over the split layout:
+ * plinks[i M + j] = links[i M + j] for j < deg[i], else UINT32_MAX, whatever
+ * links holds past the degree, so an expansion reads its row and no degree.
+ * No counterpart in the reference (a layout of include/index.h:65-69's
+ * links for index.h:1535-1652's expansions).  M <= 64. */
+int anqi_pack_links(anqi_ctx* ctx, const uint32_t* links, const uint32_t* deg, uint32_t n, uint32_t M,
+                    uint32_t* plinks);
 /* queries (device nq*d, base space).  thr: device thresholds for k (n floats)
  * or, with kidx, tables (ntables*n) picked per query.  Lq/kidx optional
  * device per-query arrays.  Outputs (device): cand_ids/accept nq*out_cap,
