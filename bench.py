@@ -59,6 +59,24 @@ def query_bytes(ix, bufs):
     return 4 * dl * nd + 4 * (G + 1) * nx + (4 * d + 4) * nc + 4 * d * nq
 
 
+def layout_bytes(ix, bufs):
+    """Bytes the index layout in use reads for the same batch: with byte rows
+    d + 4 per graph distance (row bytes + lift term) and d + 8 per verified
+    candidate (row bytes + norm + threshold); with sentinel adjacency rows 4 G
+    per expanded node (no degree); otherwise the float-row figures."""
+    h = ix.handle
+    d, G = ix.base.shape[1], ix.links.shape[1]
+    nd = bufs["nd"].double().sum().item()
+    nx = bufs["nexp"].double().sum().item()
+    nc = bufs["cnt"].double().sum().item()
+    nq = bufs["cnt"].shape[0]
+    br = bool(getattr(h, "byte_rows", False))
+    per_d = (d + 4) if br else 4 * (d + 1)
+    per_x = 4 * G if h.plinks else 4 * (G + 1)
+    per_c = (d + 8) if br else (4 * d + 8)
+    return per_d * nd + per_x * nx + per_c * nc + 4 * d * nq
+
+
 def cpu_baseline(ctx, ix, queries_np, thr, L, seconds_target=12.0):
     """The reference's own query loop (range_search + postfilter, compiled
     from the reference sources) on this host's cores, over the same graph."""
@@ -299,6 +317,7 @@ def main():
     point = [m for m in sweep if m["L"] == chosen_L][0]
     ctx.rknn_search(ix.handle, qd, chosen_L, thr, mode=1, bufs=bufs)
     B = query_bytes(ix, bufs)
+    B_layout = layout_bytes(ix, bufs)
     # chosen form: persistent launch with autotuned work areas and grid
     tune, tune_ms = ctx.search_autotune(ix.handle, thr, qd, None, None, chosen_L)
     ctx.set_search_tuning(tune)
@@ -381,6 +400,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["bytes_per_launch"] if tr else None,
                      "kernel": kern, "launch_ms": launch_ms, "bytes_per_launch": B,
+                     # the plain form reads float rows, links and degrees
+                     "layout": ({"byte_rows": bool(getattr(ix.handle, "byte_rows", False)),
+                                 "sentinel_adjacency": bool(ix.handle.plinks), "bytes_per_launch": B_layout}
+                                if mode == 1 else {"byte_rows": False, "sentinel_adjacency": False,
+                                                   "bytes_per_launch": B}),
                      **({"traffic_source": tr["source"]} if tr else {})},
         "cpu_baseline": cpu,
     }
