@@ -120,10 +120,13 @@ __device__ __forceinline__ void adj_load(const SearchParams& P, uint32_t c, uint
 // When every base value is one of the floats 0..255, a row is kept as d bytes
 // (P.brows, d = 16 MC), ordered for the 16-lane groups: lane l's MC bytes
 // l MC .. l MC + MC - 1 are elements l, 16 + l, ..., 16 (MC - 1) + l.  The
-// base value is float(b); the lifted main coordinate is
-// float(double(b) * scale), the lift's own binary64 product and rounding
-// (rknn_query.cpp:2203-2236), so both are the bits of the float rows.
-// anqi_byte_rows certifies that on every element before an index uses it.
+// base value is float(b).  The lifted main coordinate is the lift's
+// float(double(b) * scale) (rknn_query.cpp:2203-2236); it is rebuilt in FP32
+// as fma(b, s_hi, b * s_lo) with s_hi + s_lo = scale to about 2^-48, which
+// is within 2^-47 of b * scale and so rounds the same way unless b * scale
+// lies that close to a rounding boundary.  anqi_byte_rows checks the rebuilt
+// value against the lifted rows on every element before an index uses the
+// bytes, so a base with such a value keeps its float rows.
 template <int MC>
 __device__ __forceinline__ void brow_fetch(const uint8_t* __restrict__ B, uint32_t id, int l, uint32_t (&w)[2]) {
     const uint8_t* p = B + (size_t)id * (16 * MC) + (size_t)l * MC;
@@ -147,14 +150,25 @@ __device__ __forceinline__ void brow_fetch(const uint8_t* __restrict__ B, uint32
 __device__ __forceinline__ uint32_t brow_byte(const uint32_t (&w)[2], int t) {
     return (w[t >> 2] >> (8 * (t & 3))) & 255u;
 }
-// the lifted coordinate of byte value b (scale: the lift's binary64 scale)
-__device__ __forceinline__ float brow_lifted(uint32_t b, double scale) { return (float)((double)b * scale); }
+// scale split into s_hi = float(scale) and s_lo = float(scale - s_hi)
+struct BScale {
+    float hi, lo;
+};
+__device__ __forceinline__ BScale bscale(double scale) {
+    const float hi = (float)scale;
+    return BScale{hi, (float)(scale - (double)hi)};
+}
+// the lifted coordinate of byte value b
+__device__ __forceinline__ float brow_lifted(uint32_t b, BScale s) {
+    const float bf = (float)b;
+    return fmaf(bf, s.hi, bf * s.lo);
+}
 
 // rows_batch MODE 2 / 3 (split lifted rows) over byte rows: the same FMA
 // order and reduction tree, the coordinates rebuilt from the bytes.
 template <int MC, int RB>
 __device__ __forceinline__ void bytes_pass(const QRow& q, const uint8_t* __restrict__ B,
-                                           const float* __restrict__ ltail, double scale, const uint32_t* ids, int k,
+                                           const float* __restrict__ ltail, BScale scale, const uint32_t* ids, int k,
                                            int l, int g, float* out, int s0) {
     uint32_t w[RB][2];
     float nb[RB];
@@ -186,7 +200,7 @@ __device__ __forceinline__ void bytes_pass(const QRow& q, const uint8_t* __restr
 
 template <int MC, int RB>
 __device__ __forceinline__ void bytes_tail(const QRow& q, const uint8_t* __restrict__ B,
-                                           const float* __restrict__ ltail, double scale, const uint32_t* ids, int k,
+                                           const float* __restrict__ ltail, BScale scale, const uint32_t* ids, int k,
                                            int l, int g, float* out, int s0) {
     if constexpr (RB > 1) {
         if (k - s0 <= 2 * RB) {
@@ -199,7 +213,7 @@ __device__ __forceinline__ void bytes_tail(const QRow& q, const uint8_t* __restr
 
 template <int MC, int RB = 4>
 __device__ __forceinline__ void rows_batch_bytes(const QRow& q, const uint8_t* __restrict__ B,
-                                                 const float* __restrict__ ltail, double scale,
+                                                 const float* __restrict__ ltail, BScale scale,
                                                  const uint32_t* ids, int k, int l, int g, float* out) {
     int s0 = 0;
     for (; k - s0 > 4 * RB; s0 += 4 * RB) bytes_pass<MC, RB>(q, B, ltail, scale, ids, k, l, g, out, s0);
@@ -218,7 +232,8 @@ template <int MC, bool BR>
 __device__ __forceinline__ void batch_l2(const SearchParams& P, const QRow& QL, const uint32_t* tid,
                                          float* tdv, int k, int l, int g) {
     if constexpr (MC > 0 && BR) {
-        rows_batch_bytes<MC>(QL, P.brows, P.ltail, __longlong_as_double((long long)P.scale_bits), tid, k, l, g, tdv);
+        rows_batch_bytes<MC>(QL, P.brows, P.ltail, bscale(__longlong_as_double((long long)P.scale_bits)), tid, k, l,
+                             g, tdv);
     } else if constexpr (MC > 0) {
         if (P.lmain) rows_batch<MC, 2>(QL, P.lmain, P.d, P.ltail, 0.0f, tid, k, l, g, tdv);
         else rows_batch<MC, 0>(QL, P.lifted, P.dl, nullptr, 0.0f, tid, k, l, g, tdv);
@@ -799,7 +814,7 @@ __global__ void k_byte_rows(const float* __restrict__ base, const float* __restr
     const float x = base[t];
     const uint32_t b = (x >= 0.0f && x <= 255.0f) ? (uint32_t)x : 0u;
     const bool ok = __float_as_uint(x) == __float_as_uint((float)b) &&
-                    __float_as_uint(lmain[t]) == __float_as_uint(brow_lifted(b, scale));
+                    __float_as_uint(lmain[t]) == __float_as_uint(brow_lifted(b, bscale(scale)));
     brows[(size_t)i * d + l * mc + c] = (uint8_t)b;
     if (!ok) atomicAdd(bad, 1u);
 }

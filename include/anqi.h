@@ -285,11 +285,11 @@ int anqi_lift_split(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t d, 
 /* Byte rows for the persistent search over the split layout, for a base
  * whose values are all the floats 0..255 (SIFT-like data): brows holds each
  * row's d values as bytes, ordered for the search's 16-lane groups.  The
- * search rebuilds the base value float(b) and the lifted coordinate
- * float(double(b) * scale) -- the lift's own binary64 product and rounding
- * (rknn_query.cpp:2203-2236) -- from them, reading d bytes per distance
- * instead of 4d.  *exact = 1 iff every element's base value and lmain
- * coordinate are reproduced bit for bit; otherwise brows must not be used.
+ * search rebuilds the base value float(b) and, in FP32, the lifted
+ * coordinate -- the lift's float(double(b) * scale), rknn_query.cpp:
+ * 2203-2236 -- from them, reading d bytes per distance instead of 4d.
+ * *exact = 1 iff every element's base value and lmain coordinate are
+ * reproduced bit for bit; otherwise brows must not be used.
  * No counterpart in the reference (a layout of its base and lifted rows).
  * d % 16 == 0; scale: the lift's scale (anqi_lift). */
 int anqi_byte_rows(anqi_ctx* ctx, const float* base, const float* lmain, uint32_t n, uint32_t d, double scale,
