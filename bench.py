@@ -27,6 +27,8 @@ import argparse
 import ctypes
 import json
 import os
+import re
+import subprocess
 import sys
 import tempfile
 import time
@@ -77,6 +79,110 @@ def layout_bytes(ix, bufs):
     return per_d * nd + per_x * nx + per_c * nc + 4 * d * nq
 
 
+# scripts/run_final.sh's pinned configuration for the reference's rknn_query
+RUN_FINAL_ENV = {
+    "ANQI_GRAPH_GEOMETRY": "anyk_lift", "ANQI_RADIUS_MODE": "rankm_resid_u8_lrq_floor",
+    "ANQI_VERIFIER_PLACEMENT": "postfilter", "ANQI_RANKM": "8", "ANQI_NK": "100", "ANQI_GRAPH_NK": "100",
+    "ANQI_RANKM_SLACK": "0", "ANQI_RECHECK": "0", "ANQI_RQ_OBJECTIVE": "radius_reconstruction",
+    "ANQI_ALPHA": "1.2", "ANQI_LIFTNNK": "50", "ANQI_GRAPHNND": "6", "ANQI_PREBUILT_TOPK_K": "100",
+    "ANQI_BUILD_BREAKDOWN": "1", "ANQI_BENCH_CALIBRATED_TIMING": "0",
+}
+
+
+def host_cpu():
+    """Host processor as lscpu reports it, and the cores this process may use."""
+    info = {"threads_available": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+                info["smt"] = "on" if int(v) > 1 else "off"
+            elif k in ("Socket(s)", "Core(s) per socket", "CPU(s)"):
+                info[k.lower().replace("(s)", "s").replace(" ", "_")] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError) as e:
+        info["lscpu_error"] = repr(e)
+    return info
+
+
+def cpu_build_baseline(ctx, base_np, queries_np, n_sub, threads):
+    """The reference's own build, compiled from its sources (oracle/_ref), on
+    the first n_sub rows of the bench base at the reference's thread count
+    (nndindex uses min(64, hardware threads)): build_aknn_source (the AKNN
+    source), then rknn_query's index build (verifier encode, lift, graph and
+    its cache files), each timed by the tool's own clock (the "[aknn] DONE
+    elapsed_s" and "BUILD_BREAKDOWN index_total_s" lines,
+    tools/build_aknn_source.cpp:157-161, src/rknn_query.cpp:2339, 2368-2375).
+    The verifier files and the ground truth that rknn_query needs as inputs
+    are made here and are not timed.  Beside it, this build's chosen form on
+    the same rows."""
+    import torch
+    import formats
+    import lrq_fit
+    import pipeline
+    refbin = os.path.join(HERE, "oracle", "_ref")
+    aknn_bin, query_bin = os.path.join(refbin, "build_aknn_source"), os.path.join(refbin, "rknn_query")
+    if not (os.access(aknn_bin, os.X_OK) and os.access(query_bin, os.X_OK)):
+        return None
+    sub = np.ascontiguousarray(base_np[:n_sub])
+    q = np.ascontiguousarray(queries_np[:256])
+    env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
+    env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        pfx = os.path.join(td, "sub")
+        formats.write_fbin(pfx + "_base.bin", sub)
+        formats.write_fbin(pfx + "_query.bin", q)
+        with open(pfx + "_lifted.meta", "w") as f:
+            f.write("0 128 100 l2 1.0\n")
+        r = subprocess.run([aknn_bin, pfx + "_base.bin", pfx, "100", "500", "64", "600", str(threads), "10", "l2"],
+                           env=env, capture_output=True, text=True, timeout=900)
+        m = re.search(r"elapsed_s=([0-9.eE+-]+)", r.stdout)
+        if r.returncode != 0 or not m:
+            log("reference build_aknn_source failed", r.returncode, r.stderr[-300:])
+            return None
+        aknn_s = float(m.group(1))
+        _, keys = formats.read_rec(pfx + "_baseknn_aknn.rec", n_sub)
+        mu, G, cb, _ = lrq_fit.fit(ctx.to_dev(np.maximum(keys, 0.0)), log=lambda *a: None,
+                                   codebook_dtype=torch.float32)
+        lrq_fit.write_basis(pfx + "_radbasis_M8.bin", mu, G)
+        lrq_fit.write_codebook(pfx + "_learned_rq_codebook_M8_u8.bin", cb)
+        bd = ctx.to_dev(sub)
+        _, ek = ctx.exact_knn(bd, 10, precision=1)
+        off, gids, gkeys = ctx.rknn_membership(bd, ctx.to_dev(q), ek[:, 9].contiguous())
+        tot = int(off[-1])
+        formats.write_csr(pfx + "_rknn_gt_k10.bin", off, gids.cpu().numpy().view(np.uint32)[:tot],
+                          gkeys.cpu().numpy()[:tot])
+        qenv = dict(env, **RUN_FINAL_ENV)
+        qenv.update({"ANQI_PREBUILT_TOPK_REC": pfx + "_baseknn_aknn.rec",
+                     "ANQI_GRAPH_CACHE_DIR": td + "/graph", "ANQI_VERIFIER_CACHE_DIR": td + "/verifier"})
+        os.makedirs(td + "/graph")
+        os.makedirs(td + "/verifier")
+        r = subprocess.run([query_bin, "64", pfx, "48", "500", "64", "600", "10"], env=qenv, capture_output=True,
+                           text=True, timeout=900)
+        m = re.search(r"BUILD_BREAKDOWN .*index_total_s=([0-9.eE+-]+)", r.stdout)
+        if r.returncode != 0 or not m:
+            log("reference rknn_query failed", r.returncode, r.stderr[-300:])
+            return None
+        index_s = float(m.group(1))
+    gix = pipeline.build(ctx, sub, precision=1, log=lambda *a: None)
+    gt = gix.times
+    ctx.tree_free(gix.tree)
+    del gix
+    gpu_no_train = gt["build_s"] - gt["verifier_train_s"]
+    return {"value": aknn_s + index_s, "unit": "s", "cores": threads, "kind": "reference",
+            "sample": f"first {n_sub} rows of the bench base; build_aknn_source 100 500 64 600 {threads} 10 l2, "
+                      f"then rknn_query's index build under scripts/run_final.sh's pins (verifier training "
+                      f"is a separate tool in the reference and is not included)",
+            "aknn_source_s": aknn_s, "index_s": index_s,
+            "gpu_chosen_build_s_same_rows": gt["build_s"],
+            "gpu_chosen_build_s_same_rows_without_verifier_training": gpu_no_train,
+            "ratio_reference_over_gpu": (aknn_s + index_s) / gpu_no_train}
+
+
 def cpu_baseline(ctx, ix, queries_np, thr, L, seconds_target=12.0):
     """The reference's own query loop (range_search + postfilter, compiled
     from the reference sources) on this host's cores, over the same graph."""
@@ -92,7 +198,7 @@ def cpu_baseline(ctx, ix, queries_np, thr, L, seconds_target=12.0):
                                  ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                  ctypes.POINTER(ctypes.c_uint64)]
     import formats
-    cores = os.cpu_count() or 1
+    cores = len(os.sched_getaffinity(0))
     with tempfile.TemporaryDirectory(dir="/tmp") as td:
         lp = os.path.join(td, "lifted.bin")
         gp = os.path.join(td, "graph.bin")
@@ -243,6 +349,8 @@ def main():
     ap.add_argument("--form", choices=["chosen", "plain"], default="chosen", help="form reported as value")
     ap.add_argument("--no-plain-build", action="store_true", help="skip the plain-form build timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-build-n", type=int, default=100_000,
+                    help="rows of the bench base the reference's own build is timed on")
     ap.add_argument("--detail", default="")
     ap.add_argument("--stage", choices=["lrq"], default="",
                     help="time one stage's kernels instead of the headline measurement (prints its own JSON)")
@@ -375,6 +483,18 @@ def main():
                     "queries_with_tree_tie": int((det["ties"] > 0).sum())}
         except Exception as e:   # reported, never fatal for the GPU measurement
             log("cpu baseline failed:", repr(e))
+        try:
+            threads = min(64, len(os.sched_getaffinity(0)))
+            t = time.perf_counter()
+            bb = cpu_build_baseline(ctx, base, queries, min(a.cpu_build_n, a.n), threads)
+            log(f"reference build baseline in {time.perf_counter() - t:.1f}s:", json.dumps(bb))
+            if bb is not None:
+                cpu = cpu if cpu is not None else {}
+                cpu["build"] = bb
+        except Exception as e:
+            log("cpu build baseline failed:", repr(e))
+        if cpu is not None:
+            cpu["host"] = host_cpu()
     build_c = ix.times["build_s"]
     other = {"chosen_query_qps": qps_c, "plain_query_qps": qps_p, "query_qps_ratio": qps_c / qps_p,
              "chosen_build_s": build_c, "chosen_search_tuning": tuning}
