@@ -728,12 +728,23 @@ static int failed_rows(anqi_ctx* ctx, const uint32_t* status, uint32_t n, std::v
     return ANQI_OK;
 }
 
-static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev, uint32_t* overflow_rows) {
+// work (optional, host): [0] distances computed, [1] adjacency ids read.
+static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev, uint32_t* overflow_rows,
+                      uint64_t* work = nullptr) {
     hipStream_t s = ctx->stream;
-    DBuf status;
+    DBuf status, wk;
     HCHK(status.alloc(4ull * P.n));
     HCHK(hipMemsetAsync(status.p, 0, 4ull * P.n, s));
     P.status = status.as<uint32_t>();
+    HCHK(wk.alloc(16));
+    HCHK(hipMemsetAsync(wk.p, 0, 16, s));
+    P.work = wk.as<unsigned long long>();
+    auto read_work = [&]() -> int {
+        if (!work) return ANQI_OK;
+        HCHK(hipMemcpyAsync(work, wk.p, 16, hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        return ANQI_OK;
+    };
     uint64_t need = (uint64_t)P.pool_cap + P.cap + maxrev;   // visits / pool entries bound
     // fast path: LDS work area.  The register-list AKNN gather keeps its
     // visited set in 16-bit entries (half the LDS, so more rows per CU).
@@ -750,7 +761,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     int rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
     if (rc) return rc;
     if (overflow_rows) *overflow_rows = (uint32_t)bad.size();
-    if (bad.empty()) return ANQI_OK;
+    if (bad.empty()) return read_work();
     if (graph && rk && P.pmax < 2048u && next_pow2(need + 64) > P.pmax) {
         // rows that outgrew the short pool: once more in LDS with the full one
         GatherParams R = P;
@@ -763,7 +774,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
         HCHK(launch_gather(R, graph, false, s));
         rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
         if (rc) return rc;
-        if (bad.empty()) return ANQI_OK;
+        if (bad.empty()) return read_work();
     }
     // slow path: global work area sized for the bound
     GatherParams Q = P;
@@ -787,7 +798,7 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
     if (rc) return rc;
     if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "gather: %zu rows exceed the work area", bad.size());
-    return ANQI_OK;
+    return read_work();
 }
 
 int anqi_gather_topk(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx,
@@ -1298,8 +1309,11 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     G.cap = K; G.roff = roff.as<uint32_t>(); G.rids = rids.as<uint32_t>(); G.explore = ef; G.pool_cap = pool_cap;
     G.K = K; G.out_ids = ids; G.out_keys = keys; G.out_cnt = ocnt.as<uint32_t>();
     G.batched = p->precision >= 1 ? 1u : 0u;
-    rc = gather_run(ctx, G, false, maxrev, &S.overflow_rows);
+    uint64_t work[2] = {0, 0};
+    rc = gather_run(ctx, G, false, maxrev, &S.overflow_rows, work);
     if (rc) return rc;
+    S.gather_d2 = work[0];
+    S.gather_adj = work[1];
     std::vector<uint32_t> oc(n);
     HCHK(hipMemcpyAsync(oc.data(), ocnt.p, 4ull * n, hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
@@ -1356,8 +1370,11 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     G.pool_cap = std::max<uint32_t>(p->M, (p->ef_alpha ? p->ef_alpha : 8) * G.explore);
     G.M = p->M; G.alpha = p->alpha; G.out_ids = O.as<uint32_t>(); G.out_cnt = ocnt.as<uint32_t>();
     G.batched = p->form ? 1u : 0u;
-    rc = gather_run(ctx, G, true, maxrev, &S.overflow_rows);
+    uint64_t work[2] = {0, 0};
+    rc = gather_run(ctx, G, true, maxrev, &S.overflow_rows, work);
     if (rc) return rc;
+    S.gather_d2 = work[0];
+    S.gather_adj = work[1];
     HCHK(hipStreamSynchronize(s));
     double t4 = now_s();
     S.gather_s = t4 - t3;

@@ -558,6 +558,9 @@ __device__ __forceinline__ int hset_insert(uint32_t* tab, uint32_t mask, uint32_
 #define H24_OVF_SLOTS 64u
 #define H16_LB_GATHER 11u
 #define H16_SLOTS (4u << H16_LB_GATHER)
+// the AKNN gather's 20-bit table: 4,096 entries (8 KB), so more rows fit a CU
+#define H16_LB_AKNN 10u
+#define H16_SLOTS_AKNN (4u << H16_LB_AKNN)
 __host__ __device__ __forceinline__ uint32_t mix20(uint32_t id) {
     uint32_t h = (id * 0x9E3779B1u) & 0xFFFFFu;
     h ^= h >> 10;

@@ -490,10 +490,12 @@ __host__ __device__ __forceinline__ bool gather_rk(const GatherParams& P, bool g
 }
 
 // Visited set: 32-bit entries (hmask + 1 slots); h16 1: 16-bit entries
-// (ids < 2^20); h16 2: 16-bit entries plus the 32-bit overflow table (ids < 2^24).
+// (ids < 2^20, H16_SLOTS_AKNN of them); h16 2: 16-bit entries plus the
+// 32-bit overflow table (ids < 2^24).  A row whose first visits do not fit
+// ends with status 1 and is redone on the global-memory path.
 __host__ __device__ __forceinline__ size_t gather_hash_bytes(const GatherParams& P) {
     if (P.h16 == 2) return 2 * (size_t)H16_SLOTS + 4 * (size_t)H24_OVF_SLOTS;
-    return P.h16 ? 2 * (size_t)H16_SLOTS : 4 * (size_t)(P.hmask + 1);
+    return P.h16 ? 2 * (size_t)H16_SLOTS_AKNN : 4 * (size_t)(P.hmask + 1);
 }
 
 // Bytes per entry of a register list's LDS exchange area.  With the 16-bit
@@ -640,7 +642,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
     const int d = (int)P.d;
     if constexpr (HM != 0) {
-        for (uint32_t j = lane; j < H16_SLOTS / 8u; j += 64)
+        for (uint32_t j = lane; j < (HM == 1 ? H16_SLOTS_AKNN : H16_SLOTS) / 8u; j += 64)
             ((uint4*)S.hash)[j] = make_uint4(~0u, ~0u, ~0u, ~0u);
         if constexpr (HM == 2)
             for (uint32_t j = lane; j < H24_OVF_SLOTS; j += 64) S.hash[H16_SLOTS / 2u + j] = 0u;
@@ -652,6 +654,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     qrow_load(QI, P.x + (size_t)i * d, d, l);
     const float ni = P.nrm[i];
     uint32_t qsize = 0, psize = 0, pcount = 0;
+    uint64_t nd2 = 0, nadj = 0;   // work counts (distances computed, adjacency ids read)
     int qb = 0, pb = 0, status = 0;
     RKeys<(QCH > 0 ? QCH : 1)> Q;
     RKeys<(PCH > 0 ? PCH : 1)> PL;
@@ -672,7 +675,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         int fresh = 0;
         if (valid) {
             int r;
-            if constexpr (HM == 1) r = hset16_insert((uint16_t*)S.hash, H16_LB_GATHER, id);
+            if constexpr (HM == 1) r = hset16_insert((uint16_t*)S.hash, H16_LB_AKNN, id);
             else if constexpr (HM == 2) r = hset24_insert((uint16_t*)S.hash, S.hash + H16_SLOTS / 2u, id);
             else r = hset_insert(S.hash, P.hmask, id);
             if (r < 0) status = 1;
@@ -690,9 +693,14 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         if (kall == 0) return 1;
         __syncthreads();
         if (!seeded) {
+            nd2 += kall;
             if constexpr (MC > 0) {
-                // register-list form: eight rows per group in flight
-                if constexpr (QCH > 0) rows_batch<MC, 1, 16>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
+                // register-list form: sixteen rows per group in flight for
+                // the graph (about eight first visits per expansion), four
+                // for the AKNN source (one or two), whose registers then
+                // leave room for more rows per CU
+                if constexpr (QCH > 0 && GRAPH) rows_batch<MC, 1, 16>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
+                else if constexpr (QCH > 0) rows_batch<MC, 1, 4>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
                 else rows_batch<MC, 1>(QI, P.x, (uint32_t)d, P.nrm, ni, S.tid, (int)kall, l, g, S.tdv);
             } else {
                 for (uint32_t s = g; s < kall; s += 4) {
@@ -801,6 +809,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
             }
             // chunks t < nh: heap slots 64 t.. ; then the reverse list from r0
             const uint32_t nh = (cap + 63) / 64, nt = nh + (r1 - r0 + 63) / 64;
+            nadj += cap + (r1 - r0);
             for (uint32_t t = 0; t < nt && !status; t++) {
                 uint32_t id;
                 if (t == 0) id = a0;
@@ -837,6 +846,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
             if (visit(id, id != U32_NONE && id < P.n, 0.0f) && (!P.batched || nf + 64 > tcap)) process(false);
         }
         uint32_t r0 = P.roff[c], r1 = P.roff[c + 1];
+        nadj += P.cap + (r1 - r0);
         for (uint32_t s0 = r0; s0 < r1 && !status; s0 += 64) {
             uint32_t s = s0 + lane;
             uint32_t id = s < r1 ? P.rids[s] : U32_NONE;
@@ -850,6 +860,10 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     if (status) {
         if (lane == 0) { P.status[i] = (uint32_t)status; P.out_cnt[i] = 0; }
         return;
+    }
+    if (P.work && lane == 0) {
+        atomicAdd(P.work, (unsigned long long)nd2);
+        atomicAdd(P.work + 1, (unsigned long long)(nadj + P.cap));   // the seeds: the row's own slots
     }
     if (!GRAPH) {
         if constexpr (PCH > 0) {

@@ -200,6 +200,7 @@ struct GatherParams {
     uint32_t batched;       // 1: one expansion's first visits processed together
     uint32_t h16;           // 16-bit LDS visited set of H16_SLOTS entries (hmask unused): 1 n <= 2^20,
                             // 2 n <= 2^24 (with a 32-bit overflow table); 0: 32-bit entries
+    unsigned long long* work;  // optional: [0] += distances computed, [1] += adjacency ids read (finished rows)
 };
 // rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
 bool gather_register_lists(const GatherParams& P, bool graph_mode);

@@ -47,7 +47,8 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     t["aknn_detail"] = dict(forest_s=st.forest_s, leaf_s=st.leaf_s, gather_s=st.gather_s,
                             leaf_pairs=int(st.leaf_pairs), leaves=int(st.leaves), max_leaf=int(st.max_leaf),
                             short_rows=int(st.short_rows), overflow_rows=int(st.overflow_rows),
-                            leaf_survivors=int(st.leaf_survivors))
+                            leaf_survivors=int(st.leaf_survivors), gather_d2=int(st.gather_d2),
+                            gather_adj=int(st.gather_adj))
     mu, G, cb, fit_s = lrq_fit.fit(cut, log=lambda *a: None, codebook_dtype=T.float32)
     _sync(ctx)
     t2 = time.perf_counter()
@@ -78,7 +79,8 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     t["graph_detail"] = dict(tree_s=gst.tree_s, warm_s=gst.warm_s, nnd_s=gst.nnd_s, gather_s=gst.gather_s,
                              commit_s=gst.commit_s, nnd_passes=gst.nnd_passes,
                              nnd_upd=[int(gst.nnd_upd[i]) for i in range(gst.nnd_passes)],
-                             overflow_rows=int(gst.overflow_rows))
+                             overflow_rows=int(gst.overflow_rows), gather_d2=int(gst.gather_d2),
+                             gather_adj=int(gst.gather_adj))
     t["build_s"] = t5 - t0
     log(f"graph {t5 - t4:.2f}s build total {t5 - t0:.2f}s")
     ix.base, ix.base_np, ix.lifted, ix.links, ix.deg, ix.tree = base, base_np, lifted, links, deg, tree

@@ -30,7 +30,7 @@ class AknnParams(ctypes.Structure):
 class AknnStats(ctypes.Structure):
     _fields_ = [("forest_s", F64), ("leaf_s", F64), ("gather_s", F64), ("total_s", F64),
                 ("leaf_pairs", U64), ("leaves", U32), ("max_leaf", U32), ("short_rows", U32),
-                ("overflow_rows", U32), ("leaf_survivors", U64)]
+                ("overflow_rows", U32), ("leaf_survivors", U64), ("gather_d2", U64), ("gather_adj", U64)]
 
 
 class Tree(ctypes.Structure):
@@ -47,7 +47,7 @@ class GraphParams(ctypes.Structure):
 class GraphStats(ctypes.Structure):
     _fields_ = [("tree_s", F64), ("warm_s", F64), ("nnd_s", F64), ("gather_s", F64), ("commit_s", F64),
                 ("total_s", F64), ("nnd_passes", I32), ("nnd_upd", ctypes.c_int64 * 16),
-                ("overflow_rows", U32)]
+                ("overflow_rows", U32), ("gather_d2", U64), ("gather_adj", U64)]
 
 
 class SearchTuning(ctypes.Structure):

@@ -262,6 +262,33 @@ def load_traffic(kernel, workload_tag):
         return json.load(f)
 
 
+def stage_build(ctx, a):
+    """The chosen-form index build of the bench base (precision 1: bf16 MFMA
+    leaf sweep, batched gathers), after a warm-up build of 20,000 rows, with
+    each stage's host time and the gathers' work counted on the device: the
+    algorithmic bytes of a gather are 4 d per distance computed plus 4 per
+    adjacency id read (d = 128 for the AKNN gather, d' = 130 for the graph
+    gather)."""
+    import pipeline
+    import workload
+    base = workload.million_base(a.n)
+    w = pipeline.build(ctx, base[:20000], precision=1, log=lambda *x: None)
+    ctx.tree_free(w.tree)
+    del w
+    ix = pipeline.build(ctx, base, precision=1, log=log)
+    t = ix.times
+    d = base.shape[1]
+    ad, gd = t["aknn_detail"], t["graph_detail"]
+    out = {"n": a.n, "times": {k: v for k, v in t.items() if not isinstance(v, dict)}, "aknn": ad, "graph": gd}
+    for name, det, dim in (("aknn_gather", ad, d), ("graph_gather", gd, d + 2)):
+        b = 4.0 * dim * det["gather_d2"] + 4.0 * det["gather_adj"]
+        out[name] = {"distances": det["gather_d2"], "adjacency_ids": det["gather_adj"], "algorithmic_bytes": b,
+                     "per_row_distances": det["gather_d2"] / a.n, "per_row_adjacency_ids": det["gather_adj"] / a.n,
+                     "stage_s": det["gather_s"], "stage_GBps": b / det["gather_s"] / 1e9}
+    ctx.tree_free(ix.tree)
+    return out
+
+
 def stage_lrq(ctx, a):
     """Verifier kernels on the 1M x 128 mixture's AKNN cutoffs: encode
     (coefficients + codes) and the threshold decode for k = 10 and for
@@ -352,7 +379,7 @@ def main():
     ap.add_argument("--cpu-build-n", type=int, default=100_000,
                     help="rows of the bench base the reference's own build is timed on")
     ap.add_argument("--detail", default="")
-    ap.add_argument("--stage", choices=["lrq"], default="",
+    ap.add_argument("--stage", choices=["lrq", "build"], default="",
                     help="time one stage's kernels instead of the headline measurement (prints its own JSON)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 1 GiB device-to-device copy (counter calibration)")
@@ -371,7 +398,7 @@ def main():
         raise SystemExit("bench.py needs a HIP device")
     ctx = pyanqi.Context(0)
     if a.stage:
-        print(json.dumps({"lrq": stage_lrq}[a.stage](ctx, a)), flush=True)
+        print(json.dumps({"lrq": stage_lrq, "build": stage_build}[a.stage](ctx, a)), flush=True)
         return
     t = time.perf_counter()
     base = workload.million_base(a.n)

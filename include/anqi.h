@@ -109,6 +109,7 @@ typedef struct {
     uint64_t leaf_pairs;
     uint32_t leaves, max_leaf, short_rows, overflow_rows;
     uint64_t leaf_survivors;  /* precision 1/2: ordered pairs the MFMA filter let through */
+    uint64_t gather_d2, gather_adj;  /* gather work: distances computed, adjacency ids read */
 } anqi_aknn_stats;
 /* device x (n*d); device ids/keys (n*K) rows sorted by (dist, id), self
  * excluded; keys are the raw d2 (the tool clamps at 0 when writing .rec). */
@@ -246,6 +247,7 @@ typedef struct {
     int nnd_passes;
     int64_t nnd_upd[16];
     uint32_t overflow_rows;
+    uint64_t gather_d2, gather_adj;  /* gather work: distances computed, adjacency ids read */
 } anqi_graph_stats;
 /* device lifted (n*dl), warm_ids (n*width); device links (n*M), deg (n);
  * link slots past deg[i] hold UINT32_MAX. */
