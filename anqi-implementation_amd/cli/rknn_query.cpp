@@ -251,10 +251,9 @@ int main(int argc, char** argv) {
     }
     std::printf("[rknn] metric=%s M*^2=%.6g d=%zu k=%zu L_descent_points=%zu\n", metric.c_str(), Mstar2_meta, d0, kk,
                 Lds.size());
-    if (metric != "l2") {
-        std::fprintf(stderr, "[rknn] metric=%s is not supported by the GPU build (l2 only)\n", metric.c_str());
-        return 2;
-    }
+    // "l2" or "ip": the lift coordinate, R2, the post-filter key and the
+    // threshold clamp depend on it (rknn_query.cpp:751, 2227, 2555-2561, 2728)
+    const int anqi_metric = metric == "ip" ? ANQI_METRIC_IP : ANQI_METRIC_L2;
     check_configuration();
     const size_t NK = 100, GRAPH_NK = 100;
     const int RANKM = 8;
@@ -396,7 +395,8 @@ int main(int argc, char** argv) {
     auto load_source = [&]() -> int {
         if (d_keys) return 0;
         const auto t0 = std::chrono::steady_clock::now();
-        std::printf("[rknn] original-space AKNN: leaf=%zu n_trees=%zu ef=%zu metric=l2\n", leafA, ntA, efA);
+        std::printf("[rknn] original-space AKNN: leaf=%zu n_trees=%zu ef=%zu metric=%s\n", leafA, ntA, efA,
+                    metric.c_str());
         if (!rec_p.empty()) {
             if (file_size(rec_p) != (uint64_t)n * 100 * 8) {
                 std::fprintf(stderr, "[rknn] bad prebuilt top-k table %s: bytes=%lld expected=%llu\n", rec_p.c_str(),
@@ -418,6 +418,11 @@ int main(int argc, char** argv) {
             d_aknn = upload(aknn_ids);
             d_keys = upload(aknn_keys);
         } else {
+            if (anqi_metric != ANQI_METRIC_L2) {
+                std::fprintf(stderr, "[rknn] metric=%s needs the prebuilt AKNN source (ANQI_PREBUILT_TOPK_REC)\n",
+                             metric.c_str());
+                return 2;
+            }
             anqi_aknn_params ap{100, (uint32_t)leafA, (uint32_t)ntA, (uint32_t)efA, 8, 100, 10, 0};
             anqi_aknn_stats st{};
             d_aknn = dalloc<uint32_t>((size_t)n * 100);
@@ -520,12 +525,13 @@ int main(int argc, char** argv) {
             // device lift (integer soft binary64), the same bits as the host lift
             float* d_rk = upload(rk);
             d_lifted = dalloc<float>((size_t)n * dl);
-            ck(anqi_lift(g_ctx, d_base, n, d, d_rk, d_lifted, &mstar2, &scale), "lift");
+            ck(anqi_lift_metric(g_ctx, d_base, n, d, d_rk, anqi_metric, d_lifted, &mstar2, &scale), "lift");
             anqi_dfree(g_ctx, d_rk);
             download(lifted, d_lifted, (size_t)n * dl);
         } else {
             lifted.resize((size_t)n * dl);
-            ck(anqi_lift_host(base.data(), n, d, rk.data(), lifted.data(), &mstar2, &scale), "lift");
+            ck(anqi_lift_host_metric(base.data(), n, d, rk.data(), anqi_metric, lifted.data(), &mstar2, &scale),
+               "lift");
             d_lifted = upload(lifted);
         }
         // lift, meta and state files (rknn_query.cpp:2231-2245)
@@ -593,8 +599,9 @@ int main(int argc, char** argv) {
     // thresholds for Kq (materialize_k, rknn_query.cpp:720-753, slack 1)
     const auto tm0 = std::chrono::steady_clock::now();
     float* d_thr = dalloc<float>(n);
-    ck(anqi_lrq_materialize(g_ctx, d_coeff, d_codes_km, n, NK, RANKM, d_mu, d_G, d_cb, code_size, (uint32_t)Kq, 1.0,
-                            d_thr),
+    const uint32_t kq32 = (uint32_t)Kq;
+    ck(anqi_lrq_materialize_metric(g_ctx, d_coeff, d_codes_km, n, NK, RANKM, d_mu, d_G, d_cb, code_size, &kq32, 1, 1.0,
+                                   anqi_metric, d_thr),
        "threshold decode");
     ck(anqi_ctx_sync(g_ctx), "sync");
     std::printf("[rknn] independent verifier ready: mode=rank-M+LRQ k=%zu slack=0 persistent=%.3fGB "
@@ -625,7 +632,7 @@ int main(int argc, char** argv) {
         std::printf("[rknn] flattened_repeats=1 tasks=%zu\n", (size_t)nq * bench_repeats);
 
     anqi_index ix{d_base, d_bnorm, d_lifted, d_deg, d_links, n, d, dl, (uint32_t)Marg, &tree, mstar2, scale,
-                  nullptr, nullptr, nullptr, nullptr};
+                  nullptr, nullptr, nullptr, nullptr, anqi_metric};
     if (anqi_cli::gpu_form_chosen() && d % 16 == 0) {
         // persistent search: aligned main rows, precomputed lift terms and
         // sentinel adjacency rows

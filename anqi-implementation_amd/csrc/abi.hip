@@ -380,26 +380,37 @@ int anqi_transpose_codes(anqi_ctx* ctx, const uint8_t* codes, uint64_t n, uint32
     return ANQI_OK;
 }
 
-int anqi_lrq_materialize_multi(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
-                               uint32_t nk, uint32_t rankm, const float* mu, const float* basis,
-                               const float* codebook, uint32_t code_size, const uint32_t* ks, uint32_t nks,
-                               double slack, float* thr) {
+int anqi_lrq_materialize_metric(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
+                                uint32_t nk, uint32_t rankm, const float* mu, const float* basis,
+                                const float* codebook, uint32_t code_size, const uint32_t* ks, uint32_t nks,
+                                double slack, int metric, float* thr) {
+    if (metric != ANQI_METRIC_L2 && metric != ANQI_METRIC_IP)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "metric: ANQI_METRIC_L2 or ANQI_METRIC_IP");
     for (uint32_t t = 0; t < nks; t++)
         if (ks[t] < 1 || ks[t] > nk) return anqi_set_err(ctx, ANQI_E_USAGE, "LRQ verifier is not ready for k=%u", ks[t]);
     if (nks == 0 || n == 0) return ANQI_OK;
+    const int clamp = metric == ANQI_METRIC_L2;
     if (ctx->lrq_form == 0 && slack == 1.0 && lrq_fast_supported(nk, rankm, code_size)) {
         unsigned long long* fb = nullptr;
         if (int rc = lrq_counter(ctx, 1, &fb)) return rc;
-        HCHK(launch_lrq_decode(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, ks, nks, thr, fb,
-                               ctx->stream));
+        HCHK(launch_lrq_decode(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, ks, nks, clamp,
+                               thr, fb, ctx->stream));
         return ANQI_OK;
     }
     uint64_t sb;
     std::memcpy(&sb, &slack, 8);
     for (uint32_t t = 0; t < nks; t++)
         HCHK(launch_lrq_materialize(coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, ks[t], sb,
-                                    thr + (uint64_t)t * n, ctx->stream));
+                                    clamp, thr + (uint64_t)t * n, ctx->stream));
     return ANQI_OK;
+}
+
+int anqi_lrq_materialize_multi(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
+                               uint32_t nk, uint32_t rankm, const float* mu, const float* basis,
+                               const float* codebook, uint32_t code_size, const uint32_t* ks, uint32_t nks,
+                               double slack, float* thr) {
+    return anqi_lrq_materialize_metric(ctx, coeff, codes_kmajor, n, nk, rankm, mu, basis, codebook, code_size, ks,
+                                       nks, slack, ANQI_METRIC_L2, thr);
 }
 
 int anqi_lrq_materialize(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
@@ -497,8 +508,10 @@ int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk,
     return ANQI_OK;
 }
 
-int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
-                   double* mstar2, double* scale) {
+int anqi_lift_host_metric(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, int metric,
+                          float* lifted, double* mstar2, double* scale) {
+    if (metric != ANQI_METRIC_L2 && metric != ANQI_METRIC_IP) return ANQI_E_USAGE;
+    const bool ip = metric == ANQI_METRIC_IP;
     // squared norms summed in row order (the reference's OpenMP reduction
     // with one thread); GCC contracts the multiply-adds into fma.
     double sum_n2 = 0.0;
@@ -520,7 +533,7 @@ int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_ho
             h[j] = (float)v;
             nr2 = std::fma(v, v, nr2);
         }
-        h[d] = (float)(0.5 * std::fma((double)rk_horizon[i], sc2, -nr2));
+        h[d] = ip ? (float)((double)rk_horizon[i] * sc2) : (float)(0.5 * std::fma((double)rk_horizon[i], sc2, -nr2));
         double hn2 = std::fma((double)h[d], (double)h[d], nr2);
         if (hn2 > maxn2) maxn2 = hn2;
     }
@@ -538,15 +551,22 @@ int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_ho
     return ANQI_OK;
 }
 
+int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
+                   double* mstar2, double* scale) {
+    return anqi_lift_host_metric(base, n, d, rk_horizon, ANQI_METRIC_L2, lifted, mstar2, scale);
+}
+
 int anqi_aknn_cutoffs(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon) {
     if (K == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "aknn_cutoffs: K must be positive");
     HCHK(launch_aknn_cutoffs(keys, n, K, cut, horizon, ctx->stream));
     return ANQI_OK;
 }
 
-int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
-              double* mstar2, double* scale) {
+int anqi_lift_metric(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
+                     int metric, float* lifted, double* mstar2, double* scale) {
     if (n == 0 || d == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "lift: empty input");
+    if (metric != ANQI_METRIC_L2 && metric != ANQI_METRIC_IP)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "lift: metric ANQI_METRIC_L2 or ANQI_METRIC_IP");
     hipStream_t s = ctx->stream;
     DBuf rows, mx;
     HCHK(rows.alloc(8 * n));
@@ -564,7 +584,7 @@ int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const fl
     std::memcpy(&sb, &s_scale, 8);
     std::memcpy(&cb, &sc2, 8);
     HCHK(hipMemsetAsync(mx.p, 0, 8, s));
-    HCHK(launch_lift_rows(base, n, d, rk_horizon, sb, cb, lifted, mx.as<uint64_t>(), s));
+    HCHK(launch_lift_rows(base, n, d, rk_horizon, sb, cb, metric == ANQI_METRIC_IP, lifted, mx.as<uint64_t>(), s));
     double maxn2 = 0.0;
     HCHK(hipMemcpyAsync(&maxn2, mx.p, 8, hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
@@ -576,6 +596,11 @@ int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const fl
     *mstar2 = M2;
     *scale = s_scale;
     return ANQI_OK;
+}
+
+int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
+              double* mstar2, double* scale) {
+    return anqi_lift_metric(ctx, base, n, d, rk_horizon, ANQI_METRIC_L2, lifted, mstar2, scale);
 }
 
 // ---- graph pieces -----------------------------------------------------------------------------
@@ -1487,6 +1512,7 @@ static int search_run(anqi_ctx* ctx, const anqi_index* ix, const float* queries,
     double sc = ix->scale;
     std::memcpy(&P.scale_bits, &sc, 8);
     P.R2f = round_down_float(ix->mstar2 + 1.0);
+    P.ip = ix->metric == ANQI_METRIC_IP;
     P.queries = queries; P.nq = nq; P.L = L; P.Lq = Lq; P.Lmax = Lq ? std::max(Lmax, 1u) : std::max(L, 1u);
     P.thr = thr; P.kidx = kidx;
     // LDS work area sized to the beam: visited-set slots ~ 4 L + 1536 (at least
@@ -1514,6 +1540,14 @@ static int search_run(anqi_ctx* ctx, const anqi_index* ix, const float* queries,
             P.entries = W.ent;
             P.qlist = order;
         }
+    }
+    if (P.ip) {
+        // per-query R2f into the entry keys' slot, free once the work order is set
+        float* r2 = (float*)(W.ent + nq);
+        HCHK(launch_query_r2(P, nq, ix->mstar2, r2, s));
+        P.qr2 = r2;
+    }
+    if (mode == 1) {
         int occ = std::max(1, search_persistent_occupancy(P));
         uint32_t wpc = ctx->tune.waves_per_cu ? std::min<uint32_t>(ctx->tune.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
         uint32_t grid = wpc * (uint32_t)std::max(1, ctx->cus);
@@ -1557,7 +1591,8 @@ int anqi_rknn_search_entries(anqi_ctx* ctx, const anqi_index* ix, const float* q
                              const uint32_t* entries, uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept,
                              uint32_t* cand_cnt, uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp,
                              uint32_t* status, int mode) {
-    if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
+    if (!ix || !ix->tree || ix->M > 64 || (ix->metric != ANQI_METRIC_L2 && ix->metric != ANQI_METRIC_IP))
+        return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
     if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
     if (nq == 0) return ANQI_OK;
     if (out_cap == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "out_cap must be positive");
@@ -1593,7 +1628,8 @@ int anqi_rknn_search_noalloc(anqi_ctx* ctx, const anqi_index* ix, const float* q
                              const uint32_t* Lq, uint32_t Lmax, const float* thr, const uint32_t* kidx,
                              uint32_t out_cap, uint32_t* cand_ids, uint8_t* accept, uint32_t* cand_cnt,
                              uint32_t* ndist, uint32_t* pred_cnt, uint32_t* nexp, uint32_t* status) {
-    if (!ix || !ix->tree || ix->M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
+    if (!ix || !ix->tree || ix->M > 64 || (ix->metric != ANQI_METRIC_L2 && ix->metric != ANQI_METRIC_IP))
+        return anqi_set_err(ctx, ANQI_E_USAGE, "invalid index");
     if (ix->dl != ix->d + 2 || ix->d > 16 * QMAX_CHUNKS) return anqi_set_err(ctx, ANQI_E_USAGE, "bad dimensions");
     if (nq == 0) return ANQI_OK;
     if (out_cap == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "out_cap must be positive");
@@ -1645,6 +1681,7 @@ static void fill_search_params(SearchParams& P, const anqi_index* ix) {
     double sc = ix->scale;
     std::memcpy(&P.scale_bits, &sc, 8);
     P.R2f = round_down_float(ix->mstar2 + 1.0);
+    P.ip = ix->metric == ANQI_METRIC_IP;
 }
 
 struct anqi_batcher {
@@ -1681,6 +1718,8 @@ static hipError_t batch_enqueue(anqi_batcher* b) {
     float* key = (float*)(b->ent.as<uint32_t>() + b->max_batch);
     if ((e = launch_tree_entry_batch(T, b->max_batch, b->ent.as<uint32_t>(), key, b->s)) != hipSuccess) return e;
     if ((e = launch_entry_order(T, key, b->ent.as<uint32_t>() + 2ull * b->max_batch, b->s)) != hipSuccess) return e;
+    // inner product: the queries' R2f replace the consumed entry keys (b->P.qr2)
+    if (b->P.ip && (e = launch_query_r2(T, b->max_batch, b->ix.mstar2, key, b->s)) != hipSuccess) return e;
     if ((e = launch_range_search(b->P, true, b->grid, b->s)) != hipSuccess) return e;
     if ((e = launch_batch_scan(b->pred.as<uint32_t>(), b->nq.as<uint32_t>(), b->max_batch, b->off.as<uint32_t>(),
                                b->s)) != hipSuccess)
@@ -1772,6 +1811,7 @@ int anqi_batcher_create(anqi_ctx* ctx, const anqi_index* ix, const float* thr_ta
     P.ovf_count = b->counter.as<uint32_t>() + 1;
     P.entries = b->ent.as<uint32_t>();
     P.qlist = b->ent.as<uint32_t>() + 2 * mb;
+    if (P.ip) P.qr2 = (const float*)(b->ent.as<uint32_t>() + mb);   // batch_enqueue's k_query_r2
     const int occ = std::max(1, search_persistent_occupancy(P));
     const uint32_t wpc = t.waves_per_cu ? std::min<uint32_t>(t.waves_per_cu, (uint32_t)occ) : (uint32_t)occ;
     b->grid = wpc * (uint32_t)std::max(1, ctx->cus);
@@ -1942,6 +1982,10 @@ int anqi_search_autotune(anqi_ctx* ctx, const anqi_index* ix, const float* thr_t
     HCHK(launch_entry_order(P, (const float*)(ent.as<uint32_t>() + nq), ent.as<uint32_t>() + 2ull * nq, s));
     P.entries = ent.as<uint32_t>();
     P.qlist = ent.as<uint32_t>() + 2ull * nq;
+    if (P.ip) {
+        HCHK(launch_query_r2(P, nq, ix->mstar2, (float*)(ent.as<uint32_t>() + nq), s));
+        P.qr2 = (const float*)(ent.as<uint32_t>() + nq);
+    }
     hipEvent_t e0, e1;
     HCHK(hipEventCreate(&e0));
     HCHK(hipEventCreate(&e1));

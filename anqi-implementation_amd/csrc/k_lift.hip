@@ -26,10 +26,11 @@ __global__ void __launch_bounds__(256) k_lift_norm2(const float* __restrict__ x,
 }
 
 // Pass 2: v_j = o_j * scale, h_j = float(v_j), nr2 = fma(v_j, v_j, nr2);
-// h_d = float(0.5 * fma(rk_i, scale^2, -nr2)); max of fma(h_d, h_d, nr2).
+// h_d = float(0.5 * fma(rk_i, scale^2, -nr2)) (L2) or float(rk_i * scale^2)
+// (inner product, ip != 0); max of fma(h_d, h_d, nr2).
 __global__ void __launch_bounds__(256) k_lift_rows(const float* __restrict__ x, uint64_t n, uint32_t d,
                                                    const float* __restrict__ rk, uint64_t scale, uint64_t sc2,
-                                                   float* __restrict__ lifted,
+                                                   int ip, float* __restrict__ lifted,
                                                    unsigned long long* __restrict__ maxbits) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -41,8 +42,13 @@ __global__ void __launch_bounds__(256) k_lift_rows(const float* __restrict__ x, 
         h[j] = sd_to_float(v);
         nr2 = sd_fma(v, v, nr2);
     }
-    const uint64_t t = sd_fma(sd_from_float(rk[i]), sc2, nr2 ^ (1ull << 63));
-    const float hd = sd_to_float(sd_mul(t, 0x3fe0000000000000ull));   // 0.5 * t
+    float hd;
+    if (ip) {
+        hd = sd_to_float(sd_mul(sd_from_float(rk[i]), sc2));
+    } else {
+        const uint64_t t = sd_fma(sd_from_float(rk[i]), sc2, nr2 ^ (1ull << 63));
+        hd = sd_to_float(sd_mul(t, 0x3fe0000000000000ull));   // 0.5 * t
+    }
     h[d] = hd;
     const uint64_t hdd = sd_from_float(hd);
     const uint64_t hn2 = sd_fma(hdd, hdd, nr2);
@@ -75,9 +81,9 @@ hipError_t launch_lift_norm2(const float* x, uint64_t n, uint32_t d, uint64_t* s
 }
 
 hipError_t launch_lift_rows(const float* x, uint64_t n, uint32_t d, const float* rk, uint64_t scale, uint64_t sc2,
-                            float* lifted, uint64_t* maxbits, hipStream_t s) {
+                            int ip, float* lifted, uint64_t* maxbits, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lift_rows, dim3(lift_grid(n)), dim3(256), 0, s, x, n, d, rk, scale, sc2, lifted,
+    hipLaunchKernelGGL(k_lift_rows, dim3(lift_grid(n)), dim3(256), 0, s, x, n, d, rk, scale, sc2, ip, lifted,
                        (unsigned long long*)maxbits);
     return hipGetLastError();
 }

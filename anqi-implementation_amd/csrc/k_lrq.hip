@@ -65,12 +65,13 @@ __global__ void k_lrq_codes(const float* __restrict__ cut, uint64_t n, uint32_t 
     if (codes_km) codes_km[(uint64_t)k * n + i] = code;
 }
 
-// t_k(i) = float(max(0, slack * (mu_k + sum_m c_m G_mk + cb_k[code])))
+// t_k(i) = float(max(0, slack * (mu_k + sum_m c_m G_mk + cb_k[code]))); for
+// the inner-product metric (clamp 0) without the max (rknn_query.cpp:751)
 __global__ void k_lrq_materialize(const float* __restrict__ coeff, const uint8_t* __restrict__ codes_km,
                                   uint64_t n, uint32_t nk, uint32_t rankm,
                                   const float* __restrict__ mu, const float* __restrict__ basis,
                                   const float* __restrict__ cb, uint32_t code_size, uint32_t k,
-                                  uint64_t slack_bits, float* __restrict__ thr) {
+                                  uint64_t slack_bits, int clamp, float* __restrict__ thr) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint64_t v = sd_from_float(mu[k]);
@@ -88,7 +89,7 @@ __global__ void k_lrq_materialize(const float* __restrict__ coeff, const uint8_t
     uint8_t code = codes_km[(uint64_t)k * n + i];
     v = sd_add(v, sd_from_float(cb[(size_t)k * code_size + code]));
     uint64_t scaled = sd_mul(slack_bits, v);
-    thr[i] = sd_lt_zero_or_zero(scaled) ? 0.0f : sd_to_float(scaled);
+    thr[i] = clamp && sd_lt_zero_or_zero(scaled) ? 0.0f : sd_to_float(scaled);
 }
 
 // ---- float-float forms with certified rounding ---------------------------
@@ -145,7 +146,8 @@ __global__ void __launch_bounds__(LRQ_ENC_ROWS) k_lrq_encode_rows(
 }
 
 // Thresholds for up to LRQ_MAX_TABLES ranks in one pass over the rows (slack
-// 1): t_k(i) = float(max(0, mu_k + sum_m c_m G_mk + cb_k[code_ik]))
+// 1): t_k(i) = float(max(0, mu_k + sum_m c_m G_mk + cb_k[code_ik])), without
+// the max for the inner-product metric (clamp 0)
 // (rknn_query.cpp:722-753).  A block of 256 threads stages the ranks'
 // codewords in LDS, then each thread takes LRQ_DEC_ROWS rows 256 apart: their
 // coefficients (32 B) and code bytes are all requested before any is used,
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__(256) k_lrq_decode(const float* __restrict__ co
                                                     const uint8_t* __restrict__ codes_km, uint64_t n, uint32_t nk,
                                                     const float* __restrict__ mu,
                                                     const float* __restrict__ basis, const float* __restrict__ cb,
-                                                    LrqRanks ranks, float* __restrict__ thr,
+                                                    LrqRanks ranks, int clamp, float* __restrict__ thr,
                                                     unsigned long long* fallbacks) {
     __shared__ float lcb[LRQ_MAX_TABLES * 256];
     const uint32_t tid = threadIdx.x;
@@ -196,7 +198,8 @@ __global__ void __launch_bounds__(256) k_lrq_decode(const float* __restrict__ co
 #pragma unroll
         for (int j = 0; j < LRQ_DEC_ROWS; j++) {
             const uint64_t i = r0 + 256u * j;
-            if (i < n) thr[(uint64_t)t * n + i] = lrq_thr(c[j], basis, nk, k, mu[k], lcb[t * 256 + code[t][j]], &nfb);
+            if (i < n)
+                thr[(uint64_t)t * n + i] = lrq_thr(c[j], basis, nk, k, mu[k], lcb[t * 256 + code[t][j]], &nfb, clamp != 0);
         }
     }
     if (fallbacks && nfb) atomicAdd(fallbacks, (unsigned long long)nfb);
@@ -228,12 +231,12 @@ hipError_t launch_lrq_encode(const float* cut, uint64_t n, uint32_t nk, uint32_t
 hipError_t launch_lrq_materialize(const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
                                   uint32_t nk, uint32_t rankm, const float* mu,
                                   const float* basis, const float* cb, uint32_t code_size,
-                                  uint32_t k_one_based, uint64_t slack_bits, float* thr,
+                                  uint32_t k_one_based, uint64_t slack_bits, int clamp, float* thr,
                                   hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_lrq_materialize, dim3(blocks_for(n, 256)), dim3(256), 0, s, coeff,
                        codes_kmajor, n, nk, rankm, mu, basis, cb, code_size, k_one_based - 1,
-                       slack_bits, thr);
+                       slack_bits, clamp, thr);
     return hipGetLastError();
 }
 
@@ -254,8 +257,8 @@ hipError_t launch_lrq_encode_fast(const float* cut, uint64_t n, uint32_t nk, uin
 
 hipError_t launch_lrq_decode(const float* coeff, const uint8_t* codes_kmajor, uint64_t n, uint32_t nk,
                              uint32_t rankm, const float* mu, const float* basis, const float* cb,
-                             uint32_t code_size, const uint32_t* k_one_based, uint32_t nks, float* thr,
-                             unsigned long long* fallbacks, hipStream_t s) {
+                             uint32_t code_size, const uint32_t* k_one_based, uint32_t nks, int clamp,
+                             float* thr, unsigned long long* fallbacks, hipStream_t s) {
     if (n == 0 || nks == 0) return hipSuccess;
     for (uint32_t t0 = 0; t0 < nks; t0 += LRQ_MAX_TABLES) {
         LrqRanks r{};
@@ -263,7 +266,7 @@ hipError_t launch_lrq_decode(const float* coeff, const uint8_t* codes_kmajor, ui
         for (uint32_t t = 0; t < r.count; t++) r.k[t] = k_one_based[t0 + t] - 1;
         (void)code_size;   // 256 (lrq_fast_supported)
         hipLaunchKernelGGL(k_lrq_decode, dim3(blocks_for(n, 256 * LRQ_DEC_ROWS)), dim3(256), 0, s, coeff,
-                           codes_kmajor, n, nk, mu, basis, cb, r, thr + (uint64_t)t0 * n, fallbacks);
+                           codes_kmajor, n, nk, mu, basis, cb, r, clamp, thr + (uint64_t)t0 * n, fallbacks);
     }
     return hipGetLastError();
 }

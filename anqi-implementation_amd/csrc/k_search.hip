@@ -6,15 +6,20 @@
 // keyed on lifted L2^2 from the search-tree entry, then an unbounded flood over
 // every node with D <= R2; collected candidates are the nodes with D <= R2 in
 // discovery order.  The comparison double(float D) <= R2 is done as
-// D <= R2f where R2f is the largest float <= R2 (computed on the host).
+// D <= R2f where R2f is the largest float <= R2: R2 = M*^2 + 1 for L2
+// (computed on the host), and per query R2 = |s q|^2 + 1 + M*^2 for the
+// inner product (rknn_query.cpp:2728-2729, k_query_r2 below).
 // The postfilter is src/rknn_query.cpp:2762-2766: key = on + qn - 2 ip in base
-// space (each term by the FP32 contract), accept iff key <= t_k[o].
+// space (each term by the FP32 contract), or -ip for the inner product
+// (:2555-2561), accept iff key <= t_k[o].
 //
 // Batch semantics: merging one expansion's (up to 64) neighbours into the
 // sorted beam at once equals inserting them one by one (total (dist, id)
 // order, ids unique thanks to the visited set).  The flood is a closure, so
 // its visiting order does not change the collected set; it is processed in
 // FIFO order anyway, so candidate lists match the reference's order.
+#include <cstring>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -43,6 +48,7 @@ struct SearchLds {
     uint32_t* tid;     // SR_TID new ids
     float* tdv;        // SR_TID distances
     uint64_t* cs;      // 64 sorted candidate keys
+    float r2;          // the query's R2f (collect and flood iff D <= r2)
 };
 
 // Collected-candidate FIFO: entries below rcap live in LDS, later ones in
@@ -271,7 +277,7 @@ __device__ __forceinline__ bool expand(const SearchParams& P, SearchLds& S, cons
     nd += (uint32_t)k;
     float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
     uint32_t myid = (lane < k) ? S.tid[lane] : U32_NONE;
-    int hit = (lane < k) && (dv <= P.R2f);
+    int hit = (lane < k) && (dv <= S.r2);
     uint64_t hm = ballot(hit);
     int nh = popc64(hm);
     if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
@@ -343,7 +349,7 @@ __device__ __forceinline__ bool expand_rb(const SearchParams& P, SearchLds& S, c
     nd += (uint32_t)k;
     const float dv = (lane < k) ? S.tdv[lane] : F32_BIG;
     const uint32_t myid = (lane < k) ? S.tid[lane] : U32_NONE;
-    const int hit = (lane < k) && (dv <= P.R2f);
+    const int hit = (lane < k) && (dv <= S.r2);
     const uint64_t hm = ballot(hit);
     const int nh = popc64(hm);
     if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
@@ -397,7 +403,7 @@ __device__ __forceinline__ bool flood_step(const SearchParams& P, SearchLds& S, 
     nd += (uint32_t)k;
     for (int b = 0; b < k; b += 64) {
         const int s = b + lane;
-        const int hit = s < k && S.tdv[s] <= P.R2f;
+        const int hit = s < k && S.tdv[s] <= S.r2;
         const uint64_t hm = ballot(hit);
         const int nh = popc64(hm);
         if (rcnt + (uint32_t)nh > fifo_cap(P)) { ovf = 2; return false; }
@@ -418,6 +424,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     // per-query beam width, clamped to 1..Lmax (the LDS beam areas' size)
     const uint32_t L = max(1u, min(P.Lq ? P.Lq[qi] : P.L, P.Lmax));
     const float* thr = P.thr + (P.kidx ? (size_t)P.kidx[qi] * P.n : 0);
+    S.r2 = P.qr2 ? P.qr2[qi] : P.R2f;
     // query transform (rknn_query.cpp:2719-2730), binary64 product rounded to
     // float, then the lift coordinates (1, 0); each lane forms the elements
     // its register row holds (element 16 t + l, and the tail)
@@ -478,9 +485,9 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
             S.bd0[0] = dep;
             S.bp0[0] = ep;
         }
-        if (dep <= P.R2f) fifo_put(P, S, 0, ep);
+        if (dep <= S.r2) fifo_put(P, S, 0, ep);
     }
-    if (dep <= P.R2f) rcnt = 1;
+    if (dep <= S.r2) rcnt = 1;
     __syncthreads();
     // (1) beam descent
     if constexpr (NCH > 0) {
@@ -551,7 +558,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
             if (j < rcnt) {
                 uint32_t o = fifo_get(P, S, j);
                 float ip = qrow_ip(QB, P.base + (size_t)o * P.d, P.d, l);
-                float key = fmaf(-2.0f, ip, P.bnorm[o] + qn);
+                float key = P.ip ? -ip : fmaf(-2.0f, ip, P.bnorm[o] + qn);
                 ok = key <= thr[o];
                 if (l == 0 && j < P.out_cap) { out[j] = o; acc[j] = (uint8_t)ok; }
                 if (GV && l == 0) S.res[j] = o | (ok ? 0x80000000u : 0u);
@@ -585,7 +592,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                         for (int t = 0; t < M1; t++) x[j][t] = row[16 * t + l];
                         xt[j] = (l < rb) ? row[16 * M1 + l] : 0.0f;
                     }
-                    bn[j] = P.bnorm[o];
+                    bn[j] = P.ip ? 0.0f : P.bnorm[o];
                     th[j] = thr[o];
                 } else {
 #pragma unroll
@@ -605,7 +612,7 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
                 const uint32_t s = b + (uint32_t)(g + 4 * j);
                 int ok = 0;
                 if (s < rcnt) {
-                    float key = fmaf(-2.0f, ip, bn[j] + qn);
+                    float key = P.ip ? -ip : fmaf(-2.0f, ip, bn[j] + qn);
                     ok = key <= th[j];
                     if (l == 0 && s < P.out_cap) { out[s] = oid[j]; acc[s] = (uint8_t)ok; }
                     if (GV && l == 0) S.res[s] = oid[j] | (ok ? 0x80000000u : 0u);
@@ -765,6 +772,37 @@ hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint3
                                    hipStream_t s) {
     if (max_nq == 0) return hipSuccess;
     hipLaunchKernelGGL(k_tree_entry_batch, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out, key);
+    return hipGetLastError();
+}
+
+// Inner-product metric: each query's R2 = |s q|^2 + 1 + M*^2 (rknn_query.cpp:
+// 2719-2729: v_j = double(q_j) * s, qn2s = fma(v_j, v_j, qn2s) in element
+// order, then (qn2s + 1) + M*^2), as R2f = the largest float <= R2 -- the
+// range search's double(float D) <= R2 test in FP32.  One thread per query,
+// in soft binary64 (the sum is sequential); queries beyond nq (or *nq_dev)
+// exit.
+__global__ void __launch_bounds__(256) k_query_r2(SearchParams P, uint64_t m2_bits, float* __restrict__ r2) {
+    const uint32_t nq = P.nq_dev ? *P.nq_dev : P.nq;
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    const float* q = P.queries + (size_t)qi * P.d;
+    uint64_t qn2s = 0;
+    for (uint32_t j = 0; j < P.d; j++) {
+        const uint64_t v = sd_mul(sd_from_float(q[j]), P.scale_bits);
+        qn2s = sd_fma(v, v, qn2s);
+    }
+    const uint64_t R2 = sd_add(sd_add(qn2s, 0x3ff0000000000000ull), m2_bits);   // R2 > 0
+    float f = sd_to_float(R2);
+    // both positive: the bit patterns order the values
+    if (sd_from_float(f) > R2) f = __uint_as_float(__float_as_uint(f) - 1u);
+    r2[qi] = f;
+}
+
+hipError_t launch_query_r2(const SearchParams& P, uint32_t max_nq, double mstar2, float* r2, hipStream_t s) {
+    if (max_nq == 0) return hipSuccess;
+    uint64_t mb;
+    std::memcpy(&mb, &mstar2, 8);
+    hipLaunchKernelGGL(k_query_r2, dim3((max_nq + 255) / 256), dim3(256), 0, s, P, mb, r2);
     return hipGetLastError();
 }
 

@@ -22,6 +22,8 @@ struct SearchParams {
     const uint32_t* entries;   // optional per-query entry points (tree bypassed)
     uint64_t scale_bits;    // binary64 lift scale
     float R2f;              // largest float <= M*^2 + 1
+    const float* qr2;       // inner product: per-query R2f (k_query_r2); null: R2f for every query
+    int ip;                 // inner-product metric: post-filter key -<o, q>
     const float* queries;   // nq x d (base space)
     uint32_t nq;
     const uint32_t* qlist;  // optional query index list
@@ -73,7 +75,7 @@ hipError_t launch_status_list(const uint32_t* status, uint32_t nq, uint32_t mask
 // the bits of *maxbits, which starts at 0), pass 3 the padding coordinate.
 hipError_t launch_lift_norm2(const float* x, uint64_t n, uint32_t d, uint64_t* s_out, hipStream_t s);
 hipError_t launch_lift_rows(const float* x, uint64_t n, uint32_t d, const float* rk, uint64_t scale, uint64_t sc2,
-                            float* lifted, uint64_t* maxbits, hipStream_t s);
+                            int ip, float* lifted, uint64_t* maxbits, hipStream_t s);
 hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, hipStream_t s);
 // cut = max(0, keys) (n x K), horizon (optional, n) = cut[:, K-1].
 hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon,
@@ -99,6 +101,9 @@ hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint3
                                    hipStream_t s);
 // Work-queue order (qlist) of queries [0, nq) by ascending key (one workgroup).
 hipError_t launch_entry_order(const SearchParams& P, const float* key, uint32_t* qlist, hipStream_t s);
+// Inner-product metric: per-query R2f (largest float <= |s q|^2 + 1 + M*^2)
+// into r2[0..nq) (nq from *P.nq_dev when set; max_nq threads launched).
+hipError_t launch_query_r2(const SearchParams& P, uint32_t max_nq, double mstar2, float* r2, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // Batch result compaction (k_batch.hip): off = exclusive scan of pred_cnt over
 // *nq_dev queries (one block), then each query's accepted ids in candidate
@@ -122,7 +127,7 @@ hipError_t launch_lrq_encode(const float* cut, uint64_t n, uint32_t nk, uint32_t
 hipError_t launch_lrq_materialize(const float* coeff, const uint8_t* codes_kmajor, uint64_t n,
                                   uint32_t nk, uint32_t rankm, const float* mu,
                                   const float* basis, const float* cb, uint32_t code_size,
-                                  uint32_t k_one_based, uint64_t slack_bits, float* thr,
+                                  uint32_t k_one_based, uint64_t slack_bits, int clamp, float* thr,
                                   hipStream_t s);
 hipError_t launch_transpose_codes(const uint8_t* codes, uint64_t n, uint32_t nk,
                                   uint8_t* codes_kmajor, hipStream_t s);
@@ -137,8 +142,8 @@ hipError_t launch_lrq_encode_fast(const float* cut, uint64_t n, uint32_t nk, uin
 // Thresholds of nks ranks (host list, one-based) into thr (nks x n), slack 1.
 hipError_t launch_lrq_decode(const float* coeff, const uint8_t* codes_kmajor, uint64_t n, uint32_t nk,
                              uint32_t rankm, const float* mu, const float* basis, const float* cb,
-                             uint32_t code_size, const uint32_t* k_one_based, uint32_t nks, float* thr,
-                             unsigned long long* fallbacks, hipStream_t s);
+                             uint32_t code_size, const uint32_t* k_one_based, uint32_t nks, int clamp,
+                             float* thr, unsigned long long* fallbacks, hipStream_t s);
 
 // ---- distances (k_dist.hip) ---------------------------------------------------
 hipError_t launch_norms(const float* x, uint64_t n, uint32_t d, float* out, hipStream_t s);

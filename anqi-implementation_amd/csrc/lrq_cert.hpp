@@ -55,13 +55,15 @@ LRQ_SOFT uint32_t lrq_code_soft(float rkk, Coef8 c, const float* basis, uint32_t
     return a == 0u ? 0u : a - 1u;
 }
 
-// Soft binary64 threshold of rank k, slack 1: float(max(0, mu_k + sum_m c_m G_mk + cw)).
-LRQ_SOFT float lrq_thr_soft(Coef8 c, const float* basis, uint32_t nk, uint32_t k, float muk, float cw) {
+// Soft binary64 threshold of rank k, slack 1: float(max(0, mu_k + sum_m c_m G_mk + cw)),
+// or, for the inner-product metric (clamp false), float(mu_k + sum_m c_m G_mk + cw).
+LRQ_SOFT float lrq_thr_soft(Coef8 c, const float* basis, uint32_t nk, uint32_t k, float muk, float cw,
+                            bool clamp = true) {
     uint64_t s = sd_from_float(muk);
     for (uint32_t m = 0; m < LRQ_RANKM; m++)
         s = sd_fma(sd_from_float(c.v[m]), sd_from_float(basis[(size_t)m * nk + k]), s);
     s = sd_add(s, sd_from_float(cw));
-    return sd_lt_zero_or_zero(s) ? 0.0f : sd_to_float(s);
+    return clamp && sd_lt_zero_or_zero(s) ? 0.0f : sd_to_float(s);
 }
 
 // Coefficients of row rk.  Binary64 error of the reference's sum: one
@@ -158,16 +160,17 @@ FF_HD uint32_t lrq_code(float rkk, Coef8 c, const float* basis, uint32_t nk, uin
 }
 
 // Threshold of rank k, slack 1.  Binary64 error: rank + 1 roundings, each at
-// most 2^-53 (|mu_k| + sum_m |c_m G_mk| + |cw|).
+// most 2^-53 (|mu_k| + sum_m |c_m G_mk| + |cw|).  clamp: the L2 metric's
+// max(0, .); the inner-product metric keeps the sign (rknn_query.cpp:751).
 FF_HD float lrq_thr(Coef8 c, const float* basis, uint32_t nk, uint32_t k, float muk, float cw,
-                    uint32_t* fallbacks) {
+                    uint32_t* fallbacks, bool clamp = true) {
     FFShort y = ffs_start(muk);
 #pragma unroll
     for (uint32_t m = 0; m < LRQ_RANKM; m++) ffs_add_prod(y, c.v[m], basis[(size_t)m * nk + k]);
     ffs_add(y, cw);
     const FFSum x = ffs_finish(y, LRQ_RANKM + 1, 2 * LRQ_RANKM + 1);
     const float dref = x.mag * ((float)(LRQ_RANKM + 1) * 0x1.03p-53f);
-    if (ff_round_certain(x, dref)) return x.hi > 0.0f ? x.hi : 0.0f;   // a certified negative gives +0
+    if (ff_round_certain(x, dref)) return !clamp || x.hi > 0.0f ? x.hi : 0.0f;   // clamped: a negative gives +0
     ++*fallbacks;
-    return lrq_thr_soft(c, basis, nk, k, muk, cw);
+    return lrq_thr_soft(c, basis, nk, k, muk, cw, clamp);
 }

@@ -57,7 +57,11 @@ class SearchTuning(ctypes.Structure):
 class Index(ctypes.Structure):
     _fields_ = [("base", P), ("bnorm", P), ("lifted", P), ("deg", P), ("links", P), ("n", U32), ("d", U32),
                 ("dl", U32), ("M", U32), ("tree", ctypes.POINTER(Tree)), ("mstar2", F64), ("scale", F64),
-                ("lmain", P), ("ltail", P), ("plinks", P), ("brows", P)]
+                ("lmain", P), ("ltail", P), ("plinks", P), ("brows", P), ("metric", I32)]
+
+
+METRIC_L2, METRIC_IP = 0, 1
+METRICS = {"l2": METRIC_L2, "ip": METRIC_IP}
 
 
 SIGS = {
@@ -89,12 +93,15 @@ SIGS = {
     "anqi_lrq_encode": (I32, [P, P, U64, U32, U32, P, P, P, U32, P, P, P]),
     "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
     "anqi_lrq_materialize_multi": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, P, U32, F64, P]),
+    "anqi_lrq_materialize_metric": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, P, U32, F64, I32, P]),
     "anqi_ctx_set_lrq_form": (I32, [P, I32]),
     "anqi_ctx_lrq_fallbacks": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "anqi_aknn_cutoffs": (I32, [P, P, U64, U32, P, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lift": (I32, [P, P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
+    "anqi_lift_host_metric": (I32, [P, U64, U32, P, I32, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
+    "anqi_lift_metric": (I32, [P, P, U64, U32, P, I32, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lrq_relax": (I32, [P, P, U64, U32, U32, P, P, P, U32, F32, I32, F32, ctypes.POINTER(F64), P, P]),
     "anqi_warm_start": (I32, [P, P, P, U32, U32, P, U32, U32, P, P]),
     "anqi_nndescent": (I32, [P, P, P, U32, U32, U32, P, P, I32, I32, P, ctypes.POINTER(I32), I32]),
@@ -263,13 +270,13 @@ class Context:
                   dp(codes), dp(codes_km))
         return coeff, codes, codes_km
 
-    def lrq_materialize_multi(self, coeff, codes_km, mu, basis, cb, ks, slack=1.0, out=None):
-        """Thresholds for every k in ks, one pass (anqi_lrq_materialize_multi): (len(ks), n)."""
+    def lrq_materialize_multi(self, coeff, codes_km, mu, basis, cb, ks, slack=1.0, out=None, metric=METRIC_L2):
+        """Thresholds for every k in ks, one pass (anqi_lrq_materialize_metric): (len(ks), n)."""
         n, rankm = coeff.shape
         kh = np.ascontiguousarray(ks, np.uint32)
         out = out if out is not None else self.empty((len(kh), n), self.torch.float32)
-        self.call("anqi_lrq_materialize_multi", dp(coeff), dp(codes_km), n, mu.shape[0], rankm, dp(mu), dp(basis),
-                  dp(cb), cb.shape[1], hp(kh), len(kh), slack, dp(out))
+        self.call("anqi_lrq_materialize_metric", dp(coeff), dp(codes_km), n, mu.shape[0], rankm, dp(mu), dp(basis),
+                  dp(cb), cb.shape[1], hp(kh), len(kh), slack, metric, dp(out))
         return out
 
     def set_lrq_form(self, form):
@@ -288,12 +295,17 @@ class Context:
         self.call("anqi_transpose_codes", dp(codes), n, nk, dp(out))
         return out
 
-    def lrq_materialize(self, coeff, codes_km, mu, basis, cb, k, slack=1.0, out=None):
+    def lrq_materialize(self, coeff, codes_km, mu, basis, cb, k, slack=1.0, out=None, metric=METRIC_L2):
         n, rankm = coeff.shape
         nk = mu.shape[0]
         thr = out if out is not None else self.empty((n,), self.torch.float32)
-        self.call("anqi_lrq_materialize", dp(coeff), dp(codes_km), n, nk, rankm, dp(mu), dp(basis), dp(cb),
-                  cb.shape[1], k, slack, dp(thr))
+        if metric == METRIC_L2:
+            self.call("anqi_lrq_materialize", dp(coeff), dp(codes_km), n, nk, rankm, dp(mu), dp(basis), dp(cb),
+                      cb.shape[1], k, slack, dp(thr))
+        else:
+            kh = np.array([k], np.uint32)
+            self.call("anqi_lrq_materialize_metric", dp(coeff), dp(codes_km), n, nk, rankm, dp(mu), dp(basis),
+                      dp(cb), cb.shape[1], hp(kh), 1, slack, metric, dp(thr))
         return thr
 
     def warm_start(self, x, nrm, ids, cap):
@@ -436,12 +448,13 @@ class Context:
                   dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
         return links, deg, t, st
 
-    def lift(self, base, rk_horizon):
-        """Lift of device base rows on the device (anqi_lift): (lifted, mstar2, scale)."""
+    def lift(self, base, rk_horizon, metric=METRIC_L2):
+        """Lift of device base rows on the device (anqi_lift_metric): (lifted, mstar2, scale)."""
         n, d = base.shape
         lifted = self.empty((n, d + 2), self.torch.float32)
         m2, sc = F64(), F64()
-        self.call("anqi_lift", dp(base), n, d, dp(rk_horizon), dp(lifted), ctypes.byref(m2), ctypes.byref(sc))
+        self.call("anqi_lift_metric", dp(base), n, d, dp(rk_horizon), metric, dp(lifted), ctypes.byref(m2),
+                  ctypes.byref(sc))
         return lifted, m2.value, sc.value
 
     def lift_split(self, lifted):
@@ -470,10 +483,11 @@ class Context:
         return brows if exact.value else None
 
     def make_index(self, base, bnorm, lifted, deg, links, tree, mstar2, scale, split=True, packed=True,
-                   byte_rows=True):
+                   byte_rows=True, metric=METRIC_L2):
         """split: give the persistent search the split lifted layout (d % 16 == 0);
         packed: and the sentinel adjacency rows (no degree read);
-        byte_rows: and byte rows, when the base values are all floats 0..255."""
+        byte_rows: and byte rows, when the base values are all floats 0..255;
+        metric: METRIC_L2 or METRIC_IP (the lift must be of the same metric)."""
         lmain = ltail = plinks = brows = None
         if split and base.shape[1] % 16 == 0:
             lmain, ltail = self.lift_split(lifted)
@@ -483,7 +497,7 @@ class Context:
                 brows = self.byte_rows(base, lmain, scale)
         ix = Index(dp(base), dp(bnorm), dp(lifted), dp(deg), dp(links), base.shape[0], base.shape[1],
                    lifted.shape[1], links.shape[1], ctypes.pointer(tree), mstar2, scale, dp(lmain), dp(ltail),
-                   dp(plinks), dp(brows))
+                   dp(plinks), dp(brows), metric)
         ix._keep = (base, bnorm, lifted, deg, links, tree, lmain, ltail, plinks, brows)
         ix.byte_rows = brows is not None
         return ix
@@ -533,13 +547,13 @@ class Context:
         return b
 
 
-def lift_host(base: np.ndarray, rk_horizon: np.ndarray):
+def lift_host(base: np.ndarray, rk_horizon: np.ndarray, metric=METRIC_L2):
     base = np.ascontiguousarray(base, np.float32)
     rk = np.ascontiguousarray(rk_horizon, np.float32)
     n, d = base.shape
     out = np.zeros((n, d + 2), np.float32)
     m2, sc = F64(), F64()
-    rc = lib().anqi_lift_host(hp(base), n, d, hp(rk), hp(out), ctypes.byref(m2), ctypes.byref(sc))
+    rc = lib().anqi_lift_host_metric(hp(base), n, d, hp(rk), metric, hp(out), ctypes.byref(m2), ctypes.byref(sc))
     if rc != 0:
         raise AnqiError(rc, "lift failed")
     return out, m2.value, sc.value

@@ -32,6 +32,11 @@ extern "C" {
 #define ANQI_E_IO 6        /* write failure (exit 6) */
 #define ANQI_E_DEVICE 7    /* HIP / device failure */
 
+/* Metrics (the reference's "l2" and "ip", include/distance.h:178-186;
+ * src/rknn_query.cpp:1611-1617) */
+#define ANQI_METRIC_L2 0
+#define ANQI_METRIC_IP 1
+
 typedef struct anqi_ctx anqi_ctx;
 
 /* ---- context ------------------------------------------------------------- */
@@ -139,6 +144,13 @@ int anqi_lrq_materialize_multi(anqi_ctx* ctx, const float* coeff, const uint8_t*
                                uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
                                const float* basis, const float* codebook, uint32_t code_size,
                                const uint32_t* ks, uint32_t nks, double slack, float* thr);
+/* The same with the metric: ANQI_METRIC_IP keeps the sign,
+ * thr[i] = float(slack * (mu_k + sum_m c_m G_mk + cb_k[code])), as
+ * materialize_k does for "ip" (src/rknn_query.cpp:722-753, :751). */
+int anqi_lrq_materialize_metric(anqi_ctx* ctx, const float* coeff, const uint8_t* codes_kmajor,
+                                uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
+                                const float* basis, const float* codebook, uint32_t code_size,
+                                const uint32_t* ks, uint32_t nks, double slack, int metric, float* thr);
 /* Arithmetic of the verifier's encode and decode on this context: 0
  * (default) FP32 float-float sums whose rounding to float is certified
  * against the binary64 evaluation's error bound, soft binary64 only for the
@@ -181,6 +193,13 @@ int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_ho
  * taken on the host.  Same bits as anqi_lift_host. */
 int anqi_lift(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon, float* lifted,
               double* mstar2, double* scale);
+/* Both lifts with the metric: ANQI_METRIC_IP sets the lift coordinate to
+ * float(r * s^2) instead of float(0.5 (r s^2 - |s o|^2)) (src/rknn_query.cpp:
+ * 2227); everything else is shared.  The first two are the L2 forms above. */
+int anqi_lift_host_metric(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, int metric,
+                          float* lifted, double* mstar2, double* scale);
+int anqi_lift_metric(anqi_ctx* ctx, const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
+                     int metric, float* lifted, double* mstar2, double* scale);
 
 /* ---- graph construction pieces (include/index.h:1071-1295) ---------------- */
 int anqi_warm_start(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d,
@@ -275,6 +294,11 @@ typedef struct {
     /* optional, with lmain/ltail: byte rows from anqi_byte_rows when it
      * reports them exact (NULL: the search reads float rows) */
     const uint8_t* brows;   /* device n*d */
+    /* ANQI_METRIC_L2 (0) or ANQI_METRIC_IP: the inner-product search takes
+     * each query's R2 = |s q|^2 + 1 + M*^2 and the post-filter key -<o, q>
+     * (src/rknn_query.cpp:2555-2561, 2719-2730); the lifted rows must come
+     * from the lift of the same metric */
+    int metric;
 } anqi_index;
 /* Split lifted layout for the persistent search: lmain = the first d
  * coordinates of each lifted row (rows of d floats, so a row covers d/32

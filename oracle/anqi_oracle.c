@@ -29,6 +29,23 @@
  *   or_commit                     harness calls into the reference's
  *                                 HeapList/nndindex methods on identical inputs
  *                                 (1 thread), bit-exact.
+ *   inner-product metric ("ip"):
+ *   or_ipdist                     float bits vs the reference's DistCal "ip"
+ *                                 distance (harness), random vectors, d 1..300
+ *                                 (the auto-vectorised loop of GCC 11.4 at the
+ *                                 reference's flags; another compiler's
+ *                                 vector-width choice moves the fused tail).
+ *   or_lift_metric(ip)            the reference rknn_query's _approxlift file
+ *                                 and graph meta for the "ip" sample
+ *                                 (tests/golden/sample_ip), bit-exact.
+ *   or_query_lift, or_recheck_key_ip, or_lrq_materialize_metric(ip)
+ *                                 harness restatements of the reference's
+ *                                 expressions (same flags); the reference
+ *                                 rknn_query's stdout (recall, precision,
+ *                                 candidates, #dist) on the "ip" sample.
+ *   or_exact_topk_metric / or_rknn_membership_metric (ip)
+ *                                 the reference dataset_gt's "ip" files for
+ *                                 the sample, bit-exact.
  *   or_nndescent_snapshot         not a reference schedule (the reference
  *                                 LocalJoin is lock-order dependent); pinned
  *                                 by invariants: every applied push obeys the
@@ -147,6 +164,12 @@ void or_sort_pairs(float* d, uint32_t* id, uint32_t m) {
  * over ascending j gives the K smallest (key, id) pairs. */
 int or_exact_topk(const float* base, uint32_t n, uint32_t d, uint32_t K,
                   uint32_t* ids, float* keys, int threads) {
+    return or_exact_topk_metric(base, n, d, K, 0, ids, keys, threads);
+}
+
+/* dataset_gt.cpp:206-245 with the metric: key = -ip for "ip" (:225). */
+int or_exact_topk_metric(const float* base, uint32_t n, uint32_t d, uint32_t K, int ip_metric,
+                         uint32_t* ids, float* keys, int threads) {
     if (K >= n) return -1;
     float* bn = (float*)malloc(sizeof(float) * n);
     or_norms(base, n, d, bn);
@@ -167,7 +190,7 @@ int or_exact_topk(const float* base, uint32_t n, uint32_t d, uint32_t K,
             for (uint32_t j = 0; j < n; j++) {
                 if (j == i) continue;
                 float ip = or_ip(xi, base + (size_t)j * d, d);
-                float key = fmaf(-2.0f, ip, bn[i] + bn[j]);
+                float key = ip_metric ? -ip : fmaf(-2.0f, ip, bn[i] + bn[j]);
                 if (cnt == K && !pair_less(key, j, kd[K - 1], ki[K - 1])) continue;
                 uint32_t p = cnt < K ? cnt : K - 1;
                 while (p > 0 && pair_less(key, j, kd[p - 1], ki[p - 1])) {
@@ -190,6 +213,13 @@ int or_exact_topk(const float* base, uint32_t n, uint32_t d, uint32_t K,
 uint64_t or_rknn_membership(const float* base, uint32_t n, const float* query,
                             uint32_t nq, uint32_t d, const float* rk2,
                             uint32_t* off, uint32_t* ids, float* keys, int threads) {
+    return or_rknn_membership_metric(base, n, query, nq, d, 0, rk2, off, ids, keys, threads);
+}
+
+/* dataset_gt.cpp:283-313 with the metric: key = -ip for "ip" (:292). */
+uint64_t or_rknn_membership_metric(const float* base, uint32_t n, const float* query,
+                                   uint32_t nq, uint32_t d, int ip_metric, const float* rk2,
+                                   uint32_t* off, uint32_t* ids, float* keys, int threads) {
     float* bn = (float*)malloc(sizeof(float) * n);
     or_norms(base, n, d, bn);
 #ifdef _OPENMP
@@ -209,7 +239,7 @@ uint64_t or_rknn_membership(const float* base, uint32_t n, const float* query,
             uint32_t c = 0;
             for (uint32_t j = 0; j < n; j++) {
                 float ip = or_ip(q, base + (size_t)j * d, d);
-                float key = fmaf(-2.0f, ip, qn + bn[j]);
+                float key = ip_metric ? -ip : fmaf(-2.0f, ip, qn + bn[j]);
                 if (key <= rk2[j]) { hd[c] = key; hi[c] = j; c++; }
             }
             cnt[qq] = c;
@@ -517,6 +547,45 @@ void or_lift_query(const float* q, uint32_t d, double scale, float* qb) {
     qb[d + 1] = 0.0f;
 }
 
+/* rknn_query.cpp:2719-2730 (GRAPH_RANKM == 0), both metrics: the lifted
+ * query and the range-search bound R2 (v_j = double(q_j) * scale,
+ * qn2s = fma(v_j, v_j, qn2s) in order; R2 = (qn2s + 1) + M*^2 for "ip",
+ * M*^2 + 1 for "l2"). */
+double or_query_lift(const float* q, uint32_t d, double scale, double mstar2, int ip_metric, float* qb) {
+    double qn2s = 0.0;
+    for (uint32_t j = 0; j < d; j++) {
+        double v = (double)q[j] * scale;
+        qb[j] = (float)v;
+        qn2s = fma(v, v, qn2s);
+    }
+    qb[d] = 1.0f;
+    qb[d + 1] = 0.0f;
+    return ip_metric ? (qn2s + 1.0) + mstar2 : mstar2 + 1.0;
+}
+
+/* rknn_query.cpp:2555-2561, "ip": key = -<o, q>. */
+float or_recheck_key_ip(const float* o, const float* q, uint32_t d) { return -or_ip(o, q, d); }
+
+/* include/distance.h:170-175: nndindex's "ip" distance 1 - sum_i x_i y_i, a
+ * scalar loop accumulated in element order from 0 in FP32.  GCC 11.4 at -O3
+ * (-march=x86-64-v4, the reference Makefile's flags otherwise) vectorises the
+ * products -- 16 at a time, then one block of 8 if at least 8 remain -- and
+ * adds them to the sum one by one in element order (the reduction is kept in
+ * order); the last d mod 8 elements run as scalar fused multiply-adds
+ * (-ffp-contract=fast).  So: p += fl(x_i y_i) for i < nv, p = fma(x_i, y_i, p)
+ * after, nv = 16 floor(d / 16) + (8 if d mod 16 >= 8). */
+float or_ipdist(const float* x, const float* y, uint32_t d) {
+    uint32_t nv = 16u * (d / 16u);
+    if (d - nv >= 8u) nv += 8u;
+    float p = 0.0f;
+    for (uint32_t i = 0; i < nv; i++) {
+        float t = x[i] * y[i];
+        p = p + t;
+    }
+    for (uint32_t i = nv; i < d; i++) p = fmaf(x[i], y[i], p);
+    return 1.0f - p;
+}
+
 /* rknn_query.cpp:2555-2561 */
 float or_recheck_key(const float* o, const float* q, uint32_t d) {
     float ip = or_ip(o, q, d);
@@ -568,6 +637,16 @@ void or_lrq_materialize(const float* coeff, const uint8_t* codes, uint64_t n,
                         const float* basis, const float* codebook,
                         uint32_t code_size, uint32_t k_one_based, double slack,
                         float* thr) {
+    or_lrq_materialize_metric(coeff, codes, n, nk, rankm, mu, basis, codebook, code_size, k_one_based, slack, 0,
+                              thr);
+}
+
+/* rknn_query.cpp:722-753 (bits 8); "ip" keeps the sign (:751). */
+void or_lrq_materialize_metric(const float* coeff, const uint8_t* codes, uint64_t n,
+                               uint32_t nk, uint32_t rankm, const float* mu,
+                               const float* basis, const float* codebook,
+                               uint32_t code_size, uint32_t k_one_based, double slack,
+                               int ip_metric, float* thr) {
     uint32_t k = k_one_based - 1;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)n; ii++) {
@@ -577,13 +656,20 @@ void or_lrq_materialize(const float* coeff, const uint8_t* codes, uint64_t n,
             value = fma((double)ci[m], (double)basis[(size_t)m * nk + k], value);
         value += (double)codebook[(size_t)k * code_size + codes[(size_t)ii * nk + k]];
         double scaled = slack * value;
-        thr[ii] = (float)(0.0 < scaled ? scaled : 0.0);
+        thr[ii] = ip_metric ? (float)scaled : (float)(0.0 < scaled ? scaled : 0.0);
     }
 }
 
 /* rknn_query.cpp:2203-2236 (GRAPH_RANKM == 0, L2, postfilter horizon). */
 void or_lift(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
              float* lifted, double* mstar2, double* scale) {
+    or_lift_metric(base, n, d, rk_horizon, 0, lifted, mstar2, scale);
+}
+
+/* rknn_query.cpp:2203-2236 (GRAPH_RANKM == 0, postfilter horizon), both
+ * metrics: the lift coordinate is float(r s^2) for "ip" (:2227). */
+void or_lift_metric(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, int ip_metric,
+                    float* lifted, double* mstar2, double* scale) {
     double sum_n2 = 0.0;
     for (uint64_t i = 0; i < n; i++) {
         const float* o = base + i * d;
@@ -603,7 +689,8 @@ void or_lift(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
             h[j] = (float)v;
             nr2 = fma(v, v, nr2);
         }
-        h[d] = (float)(0.5 * fma((double)rk_horizon[i], sc2, -nr2));
+        h[d] = ip_metric ? (float)((double)rk_horizon[i] * sc2)
+                         : (float)(0.5 * fma((double)rk_horizon[i], sc2, -nr2));
         double hn2 = fma((double)h[d], (double)h[d], nr2);
         if (hn2 > maxn2) maxn2 = hn2;
     }

@@ -37,6 +37,12 @@ int or_exact_topk(const float* base, uint32_t n, uint32_t d, uint32_t K,
 uint64_t or_rknn_membership(const float* base, uint32_t n, const float* query,
                             uint32_t nq, uint32_t d, const float* rk2,
                             uint32_t* off, uint32_t* ids, float* keys, int threads);
+/* The same two with the metric: key = -<x, y> for "ip" (dataset_gt.cpp:225, 292). */
+int or_exact_topk_metric(const float* base, uint32_t n, uint32_t d, uint32_t K, int ip_metric,
+                         uint32_t* ids, float* keys, int threads);
+uint64_t or_rknn_membership_metric(const float* base, uint32_t n, const float* query,
+                                   uint32_t nq, uint32_t d, int ip_metric, const float* rk2,
+                                   uint32_t* off, uint32_t* ids, float* keys, int threads);
 
 /* ---- bounded max-heap rows (include/heap.h) ------------------------------ */
 /* HeapList::push_nolock / check_push(update_reverse=false), heap.h:83-186.
@@ -103,6 +109,12 @@ void or_lift_query(const float* q, uint32_t d, double scale, float* qb);
 
 /* Postfilter key, src/rknn_query.cpp:2555-2561. */
 float or_recheck_key(const float* o, const float* q, uint32_t d);
+/* Inner-product metric ("ip"): postfilter key -<o, q> (:2555-2561); the
+ * lifted query and R2 for both metrics (:2719-2730; ip_metric 0: L2); the
+ * nndindex "ip" distance 1 - <x, y> (include/distance.h:170-175). */
+float or_recheck_key_ip(const float* o, const float* q, uint32_t d);
+double or_query_lift(const float* q, uint32_t d, double scale, double mstar2, int ip_metric, float* qb);
+float or_ipdist(const float* x, const float* y, uint32_t d);
 
 /* ---- LRQ verifier (src/rknn_query.cpp:603-839), FP64 on the host --------- */
 uint32_t or_lrq_encode_value(double v, const float* centers, uint32_t code_size); /* :482-513 floor */
@@ -114,6 +126,11 @@ void or_lrq_materialize(const float* coeff, const uint8_t* codes, uint64_t n,
                         const float* basis, const float* codebook,
                         uint32_t code_size, uint32_t k_one_based, double slack,
                         float* thr);                                              /* :722-753 */
+void or_lrq_materialize_metric(const float* coeff, const uint8_t* codes, uint64_t n,
+                               uint32_t nk, uint32_t rankm, const float* mu,
+                               const float* basis, const float* codebook,
+                               uint32_t code_size, uint32_t k_one_based, double slack,
+                               int ip_metric, float* thr);                        /* :722-753, :751 */
 
 /* ---- lift (src/rknn_query.cpp:2203-2236), FP64 on the host --------------- */
 /* Writes n*(d+2) lifted floats; returns M*^2 and scale through pointers.
@@ -121,6 +138,8 @@ void or_lrq_materialize(const float* coeff, const uint8_t* codes, uint64_t n,
  * OpenMP reduction; identical with one thread). */
 void or_lift(const float* base, uint64_t n, uint32_t d, const float* rk_horizon,
              float* lifted, double* mstar2, double* scale);
+void or_lift_metric(const float* base, uint64_t n, uint32_t d, const float* rk_horizon, int ip_metric,
+                    float* lifted, double* mstar2, double* scale);   /* :2227 for "ip" */
 
 /* ---- graph build pieces (include/index.h) -------------------------------- */
 /* Warm start, index.h:1204-1218: push width AKNN ids with lifted d2 into rows

@@ -44,6 +44,27 @@ float ref_recheck_key(const float* o, const float* q, uint32_t d) {
     return on + qn - 2.0f * ip;
 }
 
+// The "ip" metric: nndindex's distance (DistCal bound to "ip",
+// include/distance.h:170-186), the postfilter key -<o, q> of
+// rknn_query.cpp:2555-2561, and the query transform with its R2 bound of
+// rknn_query.cpp:2719-2730 (same expressions, compiled with the reference's
+// flags; the reference computes them inline in main).
+float ref_ipdist(const float* x, const float* y, uint32_t d) {
+    DistCal dc(d, "ip");
+    return dc.calculate(x, y);
+}
+float ref_recheck_key_ip(const float* o, const float* q, uint32_t d) {
+    float ip = DistCal::InnerProductSIMD16ExtAVX512_(o, q, d);
+    return -ip;
+}
+double ref_query_lift(const float* q, uint32_t d, double scale, double Mstar2, int is_ip, float* qbar_t) {
+    double qn2s = 0;
+    for (size_t j = 0; j < d; j++) { double v = (double)q[j] * scale; qbar_t[j] = (float)v; qn2s += v * v; }
+    qbar_t[d] = 1.0f;
+    qbar_t[d + 1] = 0.0f;
+    return is_ip ? (qn2s + 1.0 + Mstar2) : (Mstar2 + 1.0);
+}
+
 // HeapList::push_nolock over a stream; returns accepted count.
 int ref_heap_stream(uint32_t cap, const uint32_t* nodes, const float* vals, uint32_t m,
                     uint32_t* out_idx, float* out_val) {
@@ -240,6 +261,21 @@ void ref_lrq_build(uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
     v.build(src);
     std::memcpy(coeff, v.coeff.data(), v.coeff.size() * sizeof(float));
     std::memcpy(codes, v.packed_codes.data(), v.packed_codes.size());
+}
+
+void ref_lrq_materialize_metric(uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,
+                                const float* basis, const float* codebook, uint32_t code_size,
+                                const float* coeff, const uint8_t* codes, uint32_t k,
+                                double slack, int is_ip, float* thr) {
+    CompactLrqVerifier v;
+    std::vector<float> m(mu, mu + nk), b(basis, basis + (size_t)rankm * nk),
+        cb(codebook, codebook + (size_t)nk * code_size);
+    v.configure(n, nk, (int)rankm, 8, "floor", m, b, cb, code_size);
+    v.coeff.assign(coeff, coeff + n * rankm);
+    v.packed_codes.assign(codes, codes + n * nk);
+    std::vector<float> t;
+    v.materialize_k(k, slack, is_ip != 0, t);
+    std::memcpy(thr, t.data(), n * sizeof(float));
 }
 
 void ref_lrq_materialize(uint64_t n, uint32_t nk, uint32_t rankm, const float* mu,

@@ -55,6 +55,13 @@ ORACLE_SIGS = {
     "or_lrq_encode": (None, [P, U64, U32, U32, P, P, P, U32, P, P]),
     "or_lrq_materialize": (None, [P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "or_lift": (None, [P, U64, U32, P, P, P, P]),
+    "or_lift_metric": (None, [P, U64, U32, P, I32, P, P, P]),
+    "or_query_lift": (F64, [P, U32, F64, F64, I32, P]),
+    "or_recheck_key_ip": (F32, [P, P, U32]),
+    "or_ipdist": (F32, [P, P, U32]),
+    "or_lrq_materialize_metric": (None, [P, P, U64, U32, U32, P, P, P, U32, U32, F64, I32, P]),
+    "or_exact_topk_metric": (I32, [P, U32, U32, U32, I32, P, P, I32]),
+    "or_rknn_membership_metric": (U64, [P, U32, P, U32, U32, I32, P, P, P, P, I32]),
     "or_warm_start": (None, [P, P, U32, U32, P, U32, U32, P, P]),
     "or_nndescent_seq": (I32, [P, P, U32, U32, U32, P, P, I32, I32, P]),
     "or_nndescent_snapshot": (I32, [P, P, U32, U32, U32, P, P, I32, I32, P]),
@@ -74,6 +81,9 @@ REF_SIGS = {
     "ref_l2": (F32, [P, P, U32]),
     "ref_norm": (F32, [P, U32]),
     "ref_recheck_key": (F32, [P, P, U32]),
+    "ref_ipdist": (F32, [P, P, U32]),
+    "ref_recheck_key_ip": (F32, [P, P, U32]),
+    "ref_query_lift": (F64, [P, U32, F64, F64, I32, P]),
     "ref_heap_stream": (I32, [U32, P, P, U32, P, P]),
     "ref_queue_stream": (U32, [U32, P, P, P, U32, P, P, P, P]),
     "ref_index_open": (P, [ctypes.c_char_p, U64, U64, U64, U64, U64, F32, F32, ctypes.c_char_p]),
@@ -91,6 +101,7 @@ REF_SIGS = {
     "ref_commit": (None, [P, P, P, U32, P, P]),
     "ref_lrq_build": (None, [U64, U32, U32, P, P, P, U32, P, P, P]),
     "ref_lrq_materialize": (None, [U64, U32, U32, P, P, P, U32, P, P, U32, F64, P]),
+    "ref_lrq_materialize_metric": (None, [U64, U32, U32, P, P, P, U32, P, P, U32, F64, I32, P]),
     "ref_lrq_encode_value": (U32, [F64, P, U32]),
     "ref_tree_build": (P, [P, U64, U64, U64, U64, U64]),
     "ref_tree_nodes": (U64, [P, P]),
@@ -246,10 +257,12 @@ def trainer_cutoffs(n=3000, seed=77):
 
 
 class Sample:
-    """The 256 x 128 reference fixtures."""
+    """The 256 x 128 reference fixtures (name "sample"), or the 1,000 x 32
+    inner-product ones (name "sample_ip")."""
 
-    def __init__(self):
-        g = os.path.join(GOLDEN, "sample", "sample")
+    def __init__(self, name="sample"):
+        g = os.path.join(GOLDEN, name, name)
+        self.metric = 1 if name == "sample_ip" else 0
         self.base = read_fbin(g + "_base.bin")
         self.query = read_fbin(g + "_query.bin")
         self.n, self.d = self.base.shape

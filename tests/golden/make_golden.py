@@ -21,7 +21,14 @@ nothing else).  Outputs are data only (inputs and expected outputs):
             reference recall numbers only (AKNN recall vs exact, rknn_query
             recall/precision per L at k=10) -- the recall-parity targets.
 
-Usage: python3 tests/golden/make_golden.py [--only sample|trainer|mix20k|mix1m] [--work DIR]
+  sample_ip/ a 1,000 x 32 N(0,1) sample under the inner-product metric: the reference's
+            build_aknn_source / dataset_gt / rknn_query with metric ip, its
+            lift, graph, verifier artifact and stdout, harness dumps (R2 per
+            query, candidate lists, thresholds, post-filter decisions); the
+            basis and codebook come from lrq_fit (pinned to the reference
+            trainer by the sample and trainer3k fixtures).
+
+Usage: python3 tests/golden/make_golden.py [--only sample|sample_ip|trainer|mix20k|mix1m] [--work DIR]
 """
 from __future__ import annotations
 
@@ -78,26 +85,43 @@ def run(cmd, env=None, log=None):
     return r.stdout
 
 
-def pipeline(pfx, base, query, aknn_args, threads, L_list, ks, work):
+def fit_with_lrq_fit(pfx, n):
+    """Basis and codebook files from this repository's trainer (lrq_fit, CPU,
+    one torch thread), which tests/test_trainer_golden.py holds bit-exact to
+    the reference trainer's files; used where the reference trainer was not
+    run for the fixture."""
+    import torch
+    import lrq_fit
+    ids, keys = read_rec(pfx + "_baseknn_aknn.rec", n)
+    torch.set_num_threads(1)
+    mean, basis, cb, _ = lrq_fit.fit(torch.from_numpy(np.ascontiguousarray(keys, np.float32)), log=lambda *a: None)
+    lrq_fit.write_basis(pfx + "_radbasis_M8.bin", mean, basis)
+    lrq_fit.write_codebook(pfx + "_learned_rq_codebook_M8_u8.bin", cb)
+
+
+def pipeline(pfx, base, query, aknn_args, threads, L_list, ks, work, metric="l2", trainer="reference"):
     workload.write_fbin(pfx + "_base.bin", base)
     workload.write_fbin(pfx + "_query.bin", query)
     with open(pfx + "_lifted.meta", "w") as f:
-        f.write(f"0 {base.shape[1]} 100 l2 1.0\n")
+        f.write(f"0 {base.shape[1]} 100 {metric} 1.0\n")
     omp = {"OMP_NUM_THREADS": str(threads)}
     run([os.path.join(REF_DIR, "build_aknn_source"), pfx + "_base.bin", pfx] + aknn_args, env=omp)
     thr = os.path.join(work, "thr")
     run([os.path.join(REF_DIR, "dataset_gt"), "--base", pfx + "_base.bin", "--K", "100",
          "--rk_k", "100", "--threads", str(threads), "--tile", "64", "--mode", "thresholds",
-         "--out_prefix", thr], env=omp)
+         "--metric", metric, "--out_prefix", thr], env=omp)
     for k in ks:
         t = os.path.join(work, f"gt{k}")
         run([os.path.join(REF_DIR, "dataset_gt"), "--base", pfx + "_base.bin", "--query",
              pfx + "_query.bin", "--K", "100", "--rk_k", str(k), "--threads", str(threads),
-             "--tile", "64", "--mode", "ground-truth", "--threshold-table",
+             "--tile", "64", "--mode", "ground-truth", "--metric", metric, "--threshold-table",
              thr + "_baseknn_gt.bin", "--out_prefix", t], env=omp)
         shutil.copy(t + "_rknn_gt.bin", f"{pfx}_rknn_gt_k{k}.bin")
-    run([sys.executable, os.path.join(REF_SRC, "tools", "train_final_m8_u8.py"), "--pfx", pfx,
-         "--threshold-source", pfx + "_baseknn_aknn.rec", "--codebook-workers", "1"], env=omp)
+    if trainer == "reference":
+        run([sys.executable, os.path.join(REF_SRC, "tools", "train_final_m8_u8.py"), "--pfx", pfx,
+             "--threshold-source", pfx + "_baseknn_aknn.rec", "--codebook-workers", "1"], env=omp)
+    else:
+        fit_with_lrq_fit(pfx, base.shape[0])
     runroot = os.path.join(work, "run")
     outs = {}
     for k in ks:
@@ -250,6 +274,109 @@ def make_sample(out_dir):
     print("[golden] sample fixtures written to", out_dir)
 
 
+def make_sample_ip(out_dir):
+    """A sample under the inner-product metric ("ip"), 1,000 x 32 N(0,1) (seed
+    2) with 64 queries (seed 3): the reference's
+    build_aknn_source and dataset_gt with metric ip, basis and codebook from
+    lrq_fit on that .rec (the trainer is metric-blind and already pinned),
+    then the reference rknn_query for k = 10, 50, 100 and harness dumps on its
+    graph: per-query R2 and candidate lists, thresholds (materialize_k for
+    ip), post-filter keys -<o, q> and decisions."""
+    os.makedirs(out_dir, exist_ok=True)
+    # 1,000 rows (n >> K = 100, so the flood does not cover the graph), d = 32
+    base, query = workload.gaussian(1000, 32, 2), workload.gaussian(64, 32, 3)
+    n, d = base.shape
+    L_list = [16, 32, 64, 128]
+    with tempfile.TemporaryDirectory() as work:
+        pfx = os.path.join(work, "sample_ip")
+        thr, runroot, outs = pipeline(pfx, base, query, ["100", "64", "8", "64", "1", "10", "ip"],
+                                      1, L_list, [10, 50, 100], work, metric="ip", trainer="lrq_fit")
+        for suf in ["_base.bin", "_query.bin", "_baseknn_aknn.rec", "_radbasis_M8.bin",
+                    "_learned_rq_codebook_M8_u8.bin", "_rknn_gt_k10.bin", "_rknn_gt_k50.bin",
+                    "_rknn_gt_k100.bin", "_lifted.meta"]:
+            shutil.copy(pfx + suf, os.path.join(out_dir, "sample_ip" + suf))
+        shutil.copy(thr + "_baseknn_gt.bin", os.path.join(out_dir, "sample_ip_baseknn_gt.bin"))
+        gdir = os.path.join(runroot, "graph")
+        graph = glob.glob(os.path.join(gdir, "fd_metric_l2_M_48*"))
+        lift = glob.glob(os.path.join(gdir, "_approxlift_graph_*.bin"))
+        meta = glob.glob(os.path.join(gdir, "_anqi_meta_graph_*.txt"))
+        ver = glob.glob(os.path.join(runroot, "verifier", "_anqi_verifier_*.bin"))
+        assert len(graph) == 1 and len(lift) == 1 and len(meta) == 1 and len(ver) == 1, (graph, lift, meta, ver)
+        shutil.copy(graph[0], os.path.join(out_dir, "sample_ip_graph.bin"))
+        shutil.copy(lift[0], os.path.join(out_dir, "sample_ip_approxlift.bin"))
+        shutil.copy(meta[0], os.path.join(out_dir, "sample_ip_graph_meta.txt"))
+        shutil.copy(ver[0], os.path.join(out_dir, "sample_ip_verifier.anqivf2"))
+        for k, o in outs.items():
+            with open(os.path.join(out_dir, f"sample_ip_rknn_stdout_k{k}.txt"), "w") as f:
+                f.write("\n".join(l for l in o.splitlines() if l.startswith("[rknn]")) + "\n")
+        H = ctypes.CDLL(os.path.join(REF_DIR, "libref_harness.so"))
+        vp, u32, u64, f32, f64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+        H.ref_index_open.restype = vp
+        H.ref_index_open.argtypes = [ctypes.c_char_p, u64, u64, u64, u64, u64, f32, f32, ctypes.c_char_p]
+        H.ref_index_load_graph.argtypes = [vp, ctypes.c_char_p]
+        H.ref_range_search.restype = u32
+        H.ref_range_search.argtypes = [vp, vp, f64, u64, vp, u32, vp]
+        H.ref_tree_entry.restype = u32
+        H.ref_tree_entry.argtypes = [vp, vp]
+        H.ref_query_lift.restype = f64
+        H.ref_query_lift.argtypes = [vp, u32, f64, f64, ctypes.c_int, vp]
+        H.ref_recheck_key_ip.restype = f32
+        H.ref_recheck_key_ip.argtypes = [vp, vp, u32]
+        H.ref_lrq_materialize_metric.argtypes = [u64, u32, u32, vp, vp, vp, u32, vp, vp, u32, f64, ctypes.c_int, vp]
+        folder = os.path.join(work, "hx") + "/"
+        os.makedirs(folder, exist_ok=True)
+        h = H.ref_index_open(lift[0].encode(), n, d + 2, 48, 50, 100, 8.0, 1.2, folder.encode())
+        assert H.ref_index_load_graph(h, graph[0].encode()) == 0
+        M2, scale = map(float, open(meta[0]).read().split())
+        qb = np.zeros((len(query), d + 2), dtype=np.float32)
+        R2 = np.array([H.ref_query_lift(np.ascontiguousarray(query[i]).ctypes.data, d, scale, M2, 1,
+                                        qb[i].ctypes.data) for i in range(len(query))], dtype=np.float64)
+        entries = np.array([H.ref_tree_entry(h, qb[i].ctypes.data) for i in range(len(query))], dtype=np.uint32)
+        cands = {}
+        for L in L_list:
+            offs, ids, nd = [0], [], []
+            for i in range(len(query)):
+                buf = np.zeros(n, dtype=np.uint32)
+                ndist = ctypes.c_uint64(0)
+                c = H.ref_range_search(h, qb[i].ctypes.data, float(R2[i]), L, buf.ctypes.data, n, ctypes.byref(ndist))
+                ids.extend(buf[:c].tolist())
+                offs.append(len(ids))
+                nd.append(ndist.value)
+            cands[L] = (np.array(offs, dtype=np.uint32), np.array(ids, dtype=np.uint32), np.array(nd, dtype=np.uint64))
+        raw = open(ver[0], "rb").read()
+        hdr = struct.unpack("<8sQIIIII", raw[:36])
+        assert hdr[0] == b"ANQIVF2\0" and hdr[1] == n
+        coeff = np.frombuffer(raw[36:36 + n * 8 * 4], dtype=np.float32)
+        codes = np.frombuffer(raw[36 + n * 32: 36 + n * 32 + n * 100], dtype=np.uint8)
+        rb = open(pfx + "_radbasis_M8.bin", "rb").read()
+        mu = np.frombuffer(rb[8:8 + 400], dtype=np.float32)
+        basis = np.frombuffer(rb[408:408 + 3200], dtype=np.float32)
+        cb = np.frombuffer(open(pfx + "_learned_rq_codebook_M8_u8.bin", "rb").read()[24:], dtype=np.float32)
+        thrs, dec = {}, {}
+        o, ids, _ = cands[128]
+        keys = np.array([H.ref_recheck_key_ip(np.ascontiguousarray(base[ids[j]]).ctypes.data,
+                                              np.ascontiguousarray(query[qi]).ctypes.data, d)
+                         for qi in range(len(query)) for j in range(o[qi], o[qi + 1])], dtype=np.float32)
+        for k in (10, 50, 100):
+            t = np.zeros(n, dtype=np.float32)
+            H.ref_lrq_materialize_metric(n, 100, 8, mu.ctypes.data, basis.ctypes.data, cb.ctypes.data, 256,
+                                         coeff.ctypes.data, codes.ctypes.data, k, 1.0, 1, t.ctypes.data)
+            thrs[k] = t
+            dec[k] = (keys <= t[ids]).astype(np.uint8)
+        np.savez_compressed(os.path.join(out_dir, "sample_ip_harness.npz"),
+                            entries=entries, R2=R2, M2=np.float64(M2), scale=np.float64(scale), qb=qb,
+                            **{f"cand_off_L{L}": cands[L][0] for L in L_list},
+                            **{f"cand_ids_L{L}": cands[L][1] for L in L_list},
+                            **{f"ndist_L{L}": cands[L][2] for L in L_list},
+                            **{f"thr_k{k}": thrs[k] for k in thrs},
+                            **{f"dec_k{k}": dec[k] for k in dec},
+                            postfilter_keys_L128=keys)
+        points = {k: parse_points(o) for k, o in outs.items()}
+    with open(os.path.join(out_dir, "sample_ip_points.json"), "w") as f:
+        json.dump({str(k): v for k, v in points.items()}, f, indent=1)
+    print("[golden] sample_ip fixtures written to", out_dir)
+
+
 def make_trainer(out_dir, n=3000, train_sample=2000):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from anqi_testlib import trainer_cutoffs
@@ -332,13 +459,15 @@ def make_mix1m(out_dir, work_root=None):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", choices=["sample", "trainer", "mix20k", "mix1m"])
+    ap.add_argument("--only", choices=["sample", "sample_ip", "trainer", "mix20k", "mix1m"])
     ap.add_argument("--work", default=None, help="scratch directory for the 1M run")
     a = ap.parse_args()
     if not os.path.exists(os.path.join(REF_DIR, "libref_harness.so")):
         raise SystemExit("build the reference first: make -C oracle ref")
     if a.only in (None, "sample"):
         make_sample(os.path.join(HERE, "sample"))
+    if a.only in (None, "sample_ip"):
+        make_sample_ip(os.path.join(HERE, "sample_ip"))
     if a.only in (None, "trainer"):
         make_trainer(os.path.join(HERE, "trainer3k"))
     if a.only in (None, "mix20k"):
