@@ -52,10 +52,6 @@ int main(int argc, char** argv) {
         std::cerr << "[aknn] metric must be l2 or ip\n";
         return 2;
     }
-    if (metric == "ip") {
-        std::cerr << "[aknn] metric=ip is not supported by the GPU build (l2 only)\n";
-        return 2;
-    }
     const auto started = std::chrono::steady_clock::now();
     uint32_t n = 0, d = 0;
     {
@@ -81,8 +77,9 @@ int main(int argc, char** argv) {
         std::cerr << "[aknn] device allocation failed: " << anqi_last_error(ctx) << "\n";
         return 3;
     }
+    const bool ip = metric == "ip";
     anqi_aknn_params p{(uint32_t)k, (uint32_t)leaf, (uint32_t)trees, (uint32_t)ef, 8, 100, seed,
-                       anqi_cli::gpu_form_chosen() ? 1 : 0};
+                       anqi_cli::gpu_form_chosen() ? 1 : 0, ip ? ANQI_METRIC_IP : ANQI_METRIC_L2};
     anqi_aknn_stats st{};
     int rc = anqi_aknn_build(ctx, (const float*)dx, n, d, &p, (uint32_t*)dids, (float*)dkeys, &st);
     if (rc != ANQI_OK) {
@@ -122,7 +119,8 @@ int main(int argc, char** argv) {
         std::vector<Rec> rowrec(k);
         for (size_t row = 0; row < n; ++row) {
             for (size_t rank = 0; rank < k; ++rank)
-                rowrec[rank] = Rec{ids[row * k + rank], std::max(0.0f, dist[row * k + rank])};
+                rowrec[rank] = Rec{ids[row * k + rank],
+                                   ip ? dist[row * k + rank] - 1.0f : std::max(0.0f, dist[row * k + rank])};
             out.write((const char*)rowrec.data(), (std::streamsize)(k * sizeof(Rec)));
         }
         out.close();

@@ -122,10 +122,8 @@ int main(int argc, char** argv) {
         std::cerr << "[dgt] metric must be l2 or ip\n";
         return 2;
     }
-    if (metric == "ip") {
-        std::cerr << "[dgt] metric=ip is not supported by the GPU build (l2 only)\n";
-        return 2;
-    }
+    // key = -<x, y> for "ip" (dataset_gt.cpp:225, 292)
+    const int anqi_metric = metric == "ip" ? ANQI_METRIC_IP : ANQI_METRIC_L2;
     if (nthreads > (size_t)INT_MAX) {
         std::cerr << "[dgt] thread count is too large\n";
         return 2;
@@ -150,7 +148,7 @@ int main(int argc, char** argv) {
         const auto t1 = std::chrono::steady_clock::now();
         uint32_t* d_ids = dalloc<uint32_t>((size_t)n * K);
         float* d_keys = dalloc<float>((size_t)n * K);
-        ck(anqi_exact_knn(g_ctx, d_base, n, D, (uint32_t)K, 1, d_ids, d_keys), "exact top-K");
+        ck(anqi_exact_knn_metric(g_ctx, d_base, n, D, (uint32_t)K, 1, anqi_metric, d_ids, d_keys), "exact top-K");
         std::vector<uint32_t> ids((size_t)n * K);
         std::vector<float> keys((size_t)n * K);
         ck(anqi_d2h(g_ctx, ids.data(), d_ids, ids.size() * 4), "download");
@@ -234,7 +232,7 @@ int main(int argc, char** argv) {
         ck(anqi_h2d(g_ctx, d_rk, rk2.data(), (size_t)n * 4), "upload");
         std::vector<uint32_t> off((size_t)nq + 1);
         uint64_t total = 0;
-        ck(anqi_rknn_membership(g_ctx, d_base, n, d_q, nq, D, d_rk, off.data(), &total, nullptr, nullptr),
+        ck(anqi_rknn_membership_metric(g_ctx, d_base, n, d_q, nq, D, anqi_metric, d_rk, off.data(), &total, nullptr, nullptr),
            "membership count");
         if (total > UINT32_MAX) {
             std::cerr << "[dgt] reverse-kNN output exceeds uint32 CSR capacity\n";
@@ -242,7 +240,7 @@ int main(int argc, char** argv) {
         }
         uint32_t* d_ids = dalloc<uint32_t>(total ? total : 1);
         float* d_keys = dalloc<float>(total ? total : 1);
-        ck(anqi_rknn_membership(g_ctx, d_base, n, d_q, nq, D, d_rk, off.data(), &total, d_ids, d_keys),
+        ck(anqi_rknn_membership_metric(g_ctx, d_base, n, d_q, nq, D, anqi_metric, d_rk, off.data(), &total, d_ids, d_keys),
            "membership");
         std::vector<uint32_t> ids(total);
         std::vector<float> keys(total);

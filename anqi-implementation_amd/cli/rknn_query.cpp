@@ -418,18 +418,13 @@ int main(int argc, char** argv) {
             d_aknn = upload(aknn_ids);
             d_keys = upload(aknn_keys);
         } else {
-            if (anqi_metric != ANQI_METRIC_L2) {
-                std::fprintf(stderr, "[rknn] metric=%s needs the prebuilt AKNN source (ANQI_PREBUILT_TOPK_REC)\n",
-                             metric.c_str());
-                return 2;
-            }
-            anqi_aknn_params ap{100, (uint32_t)leafA, (uint32_t)ntA, (uint32_t)efA, 8, 100, 10, 0};
+            anqi_aknn_params ap{100, (uint32_t)leafA, (uint32_t)ntA, (uint32_t)efA, 8, 100, 10, 0, anqi_metric};
             anqi_aknn_stats st{};
             d_aknn = dalloc<uint32_t>((size_t)n * 100);
             d_keys = dalloc<float>((size_t)n * 100);
             ck(anqi_aknn_build(g_ctx, d_base, n, d, &ap, d_aknn, d_keys, &st), "AKNN source");
-            // .rec keys: max(0, d2) (tools/build_aknn_source.cpp:119-137)
-            ck(anqi_aknn_cutoffs(g_ctx, d_keys, n, 100, d_keys, nullptr), "AKNN cutoffs");
+            // .rec keys: max(0, d2), or the "ip" distance minus 1 (tools/build_aknn_source.cpp:119-137)
+            ck(anqi_aknn_cutoffs_metric(g_ctx, d_keys, n, 100, anqi_metric, d_keys, nullptr), "AKNN cutoffs");
             download(aknn_keys, d_keys, (size_t)n * 100);
             std::printf("[rknn] built original-space AKNN on the GPU (n=%u K=100 in %.2fs)\n", n, st.total_s);
         }

@@ -272,6 +272,14 @@ int anqi_pair_d2(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t d, co
 // ---- exact -----------------------------------------------------------------------------
 int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t K, int precision,
                    uint32_t* ids, float* keys) {
+    return anqi_exact_knn_metric(ctx, x, n, d, K, precision, ANQI_METRIC_L2, ids, keys);
+}
+
+int anqi_exact_knn_metric(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t K, int precision,
+                          int metric, uint32_t* ids, float* keys) {
+    if (metric != ANQI_METRIC_L2 && metric != ANQI_METRIC_IP)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "exact_knn: metric ANQI_METRIC_L2 or ANQI_METRIC_IP");
+    const int ipm = metric == ANQI_METRIC_IP;
     if (K == 0 || K >= n || K > 128) return anqi_set_err(ctx, ANQI_E_USAGE, "exact_knn: need 0 < K < n, K <= 128");
     if (d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
     hipStream_t s = ctx->stream;
@@ -295,9 +303,10 @@ int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32
         float Y = 0.0f, E = 0.0f;
         for (uint32_t i = 0; i < n; i++) { Y = std::max(Y, hn[i]); E = std::max(E, he[i]); }
         HCHK(launch_exact_margin(nx.as<float>(), ex.as<float>(), n, d, Y, E, marg.as<float>(), s));
-        HCHK(launch_exact_mfma(x, xb.as<uint16_t>(), nrm.as<float>(), marg.as<float>(), n, d, K, 0, n, ids, keys, s));
+        HCHK(launch_exact_mfma(x, xb.as<uint16_t>(), nrm.as<float>(), marg.as<float>(), n, d, K, 0, n, ipm, ids,
+                               keys, s));
     } else {
-        HCHK(launch_exact_topk(x, nrm.as<float>(), n, d, K, 0, n, 0, nullptr, 0.0f, ids, keys, s));
+        HCHK(launch_exact_topk(x, nrm.as<float>(), n, d, K, 0, n, ipm, ids, keys, s));
     }
     HCHK(hipStreamSynchronize(s));
     return ANQI_OK;
@@ -306,13 +315,22 @@ int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32
 int anqi_rknn_membership(anqi_ctx* ctx, const float* base, uint32_t n, const float* q, uint32_t nq,
                          uint32_t d, const float* rk2, uint32_t* off, uint64_t* total, uint32_t* ids,
                          float* keys) {
+    return anqi_rknn_membership_metric(ctx, base, n, q, nq, d, ANQI_METRIC_L2, rk2, off, total, ids, keys);
+}
+
+int anqi_rknn_membership_metric(anqi_ctx* ctx, const float* base, uint32_t n, const float* q, uint32_t nq,
+                                uint32_t d, int metric, const float* rk2, uint32_t* off, uint64_t* total,
+                                uint32_t* ids, float* keys) {
+    if (metric != ANQI_METRIC_L2 && metric != ANQI_METRIC_IP)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "rknn_membership: metric ANQI_METRIC_L2 or ANQI_METRIC_IP");
+    const int ipm = metric == ANQI_METRIC_IP;
     DBuf nrm, cnt;
     HCHK(nrm.alloc(sizeof(float) * (size_t)n));
     HCHK(cnt.alloc(sizeof(uint32_t) * (size_t)nq));
     HCHK(launch_norms(base, n, d, nrm.as<float>(), ctx->stream));
     if (!ids) {
         HCHK(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * nq, ctx->stream));
-        HCHK(launch_rknn_count(base, nrm.as<float>(), n, q, nq, d, rk2, cnt.as<uint32_t>(), ctx->stream));
+        HCHK(launch_rknn_count(base, nrm.as<float>(), n, q, nq, d, ipm, rk2, cnt.as<uint32_t>(), ctx->stream));
         std::vector<uint32_t> c(nq);
         HCHK(hipMemcpyAsync(c.data(), cnt.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ctx->stream));
         HCHK(hipStreamSynchronize(ctx->stream));
@@ -325,7 +343,8 @@ int anqi_rknn_membership(anqi_ctx* ctx, const float* base, uint32_t n, const flo
     DBuf doff;
     HCHK(doff.alloc(sizeof(uint32_t) * (size_t)(nq + 1)));
     HCHK(hipMemcpyAsync(doff.p, off, sizeof(uint32_t) * (nq + 1), hipMemcpyHostToDevice, ctx->stream));
-    HCHK(launch_rknn_fill(base, nrm.as<float>(), n, q, nq, d, rk2, doff.as<uint32_t>(), ids, keys, ctx->stream));
+    HCHK(launch_rknn_fill(base, nrm.as<float>(), n, q, nq, d, ipm, rk2, doff.as<uint32_t>(), ids, keys,
+                          ctx->stream));
     // order each row by (key, id) on the host (rows are short)
     uint64_t tot = off[nq];
     std::vector<uint32_t> hid(tot);
@@ -557,8 +576,15 @@ int anqi_lift_host(const float* base, uint64_t n, uint32_t d, const float* rk_ho
 }
 
 int anqi_aknn_cutoffs(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon) {
+    return anqi_aknn_cutoffs_metric(ctx, keys, n, K, ANQI_METRIC_L2, cut, horizon);
+}
+
+int anqi_aknn_cutoffs_metric(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, int metric, float* cut,
+                             float* horizon) {
     if (K == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "aknn_cutoffs: K must be positive");
-    HCHK(launch_aknn_cutoffs(keys, n, K, cut, horizon, ctx->stream));
+    if (metric != ANQI_METRIC_L2 && metric != ANQI_METRIC_IP)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "aknn_cutoffs: metric ANQI_METRIC_L2 or ANQI_METRIC_IP");
+    HCHK(launch_aknn_cutoffs(keys, n, K, metric == ANQI_METRIC_IP, cut, horizon, ctx->stream));
     return ANQI_OK;
 }
 
@@ -922,7 +948,7 @@ struct Forest {
 };
 
 static int build_forest(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t T, uint32_t leaf,
-                        uint32_t max_depth, uint64_t seed, bool want_nodes, Forest& F) {
+                        uint32_t max_depth, uint64_t seed, bool want_nodes, Forest& F, bool angular = false) {
     hipStream_t s = ctx->stream;
     uint64_t total = (uint64_t)T * n;
     DBuf perm2, seg_of, side, flags, lscan, tmp;
@@ -998,7 +1024,7 @@ static int build_forest(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, u
         L.x = x; L.n = n; L.d = d; L.perm = F.perm.as<uint32_t>(); L.perm_out = perm2.as<uint32_t>();
         L.seg_of = seg_of.as<uint32_t>(); L.seg_start = dstart.as<uint32_t>(); L.seg_size = dsize.as<uint32_t>();
         L.seg_pick = dpick.as<uint32_t>(); L.seg_hyper = hyp->as<float>(); L.seg_off = doff.as<float>();
-        L.side = side.as<uint8_t>(); L.nseg = ns; L.total = total;
+        L.side = side.as<uint8_t>(); L.nseg = ns; L.total = total; L.angular = angular ? 1 : 0;
         HCHK(launch_split_planes(L, s));
         HCHK(launch_split_sides(L, seed ^ ((uint64_t)depth * 0x100000001ull), s));
         HCHK(hipMemsetAsync(dleft.p, 0, 4ull * ns, s));
@@ -1202,6 +1228,11 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
                     uint32_t* ids, float* keys, anqi_aknn_stats* st) {
     if (!p || p->K == 0 || p->K >= n || p->K > 256) return anqi_set_err(ctx, ANQI_E_INPUT, "invalid base file or K");
     if (d > 16 * QMAX_CHUNKS + 15) return anqi_set_err(ctx, ANQI_E_USAGE, "dimension %u unsupported", d);
+    if (p->metric != ANQI_METRIC_L2 && p->metric != ANQI_METRIC_IP)
+        return anqi_set_err(ctx, ANQI_E_USAGE, "aknn: metric ANQI_METRIC_L2 or ANQI_METRIC_IP");
+    // "ip": angular splits and 1 - <x, y> distances (index.h:149, 164-176);
+    // the leaf pairs run in the FP32 kernel
+    const int ipm = p->metric == ANQI_METRIC_IP;
     hipStream_t s = ctx->stream;
     anqi_aknn_stats S{};
     double t0 = now_s();
@@ -1211,7 +1242,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     HCHK(launch_norms(x, n, d, nrm.as<float>(), s));
     Forest F;
     int rc = build_forest(ctx, x, n, d, p->trees ? p->trees : 16, std::max<uint32_t>(p->leaf ? p->leaf : K, 10),
-                          p->max_depth ? p->max_depth : 100, p->seed, false, F);
+                          p->max_depth ? p->max_depth : 100, p->seed, false, F, ipm != 0);
     if (rc) return rc;
     HCHK(hipStreamSynchronize(s));
     double t1 = now_s();
@@ -1228,7 +1259,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     for (auto& lv : F.leaves)
         for (auto& pr : lv) max_leaf_all = std::max(max_leaf_all, pr.second);
     if (p->precision < 0 || p->precision > 2) return anqi_set_err(ctx, ANQI_E_USAGE, "precision %d", p->precision);
-    const bool use_mfma = p->precision >= 1 && leaf_mfma_supported(d, max_leaf_all, K);
+    const bool use_mfma = !ipm && p->precision >= 1 && leaf_mfma_supported(d, max_leaf_all, K);
     const bool use_fp8 = use_mfma && p->precision == 2;
     DBuf xb, lmarg, nsurv, cbuf, ccnt;
     float fp8_scale = 1.0f;
@@ -1293,6 +1324,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
         LP.K = K; LP.ids = lid.as<uint32_t>(); LP.keys = lkey.as<float>(); LP.cnt = lcnt.as<uint32_t>();
         LP.max_leaf = mx;
         LP.nsurv = use_mfma ? nsurv.as<unsigned long long>() : nullptr;
+        LP.ipm = ipm;
         if (use_mfma)
             HCHK(launch_leaf_mfma(LP, use_fp8, xb.p, fp8_scale, lmarg.as<float>(), cbuf.as<uint32_t>(),
                                   ccnt.as<uint32_t>(), max_leaf_all, s));
@@ -1301,8 +1333,8 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
         HCHK(hipStreamSynchronize(s));   // keeps ds/dz alive until the launch ends
         if ((t & 7) == 7 || t + 1 == T) progress("leaf all-pairs: tree %u/%u", t + 1, T);
     }
-    HCHK(launch_random_fill(x, nrm.as<float>(), n, d, K, p->seed ^ 0x5EEDull, lid.as<uint32_t>(), lkey.as<float>(),
-                            lcnt.as<uint32_t>(), s));
+    HCHK(launch_random_fill(x, nrm.as<float>(), n, d, K, p->seed ^ 0x5EEDull, ipm, lid.as<uint32_t>(),
+                            lkey.as<float>(), lcnt.as<uint32_t>(), s));
     HCHK(hipStreamSynchronize(s));
     if (use_mfma) {
         HCHK(hipMemcpy(&S.leaf_survivors, nsurv.p, 8, hipMemcpyDeviceToHost));
@@ -1334,6 +1366,7 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
     G.cap = K; G.roff = roff.as<uint32_t>(); G.rids = rids.as<uint32_t>(); G.explore = ef; G.pool_cap = pool_cap;
     G.K = K; G.out_ids = ids; G.out_keys = keys; G.out_cnt = ocnt.as<uint32_t>();
     G.batched = p->precision >= 1 ? 1u : 0u;
+    G.ipm = ipm;
     uint64_t work[2] = {0, 0};
     rc = gather_run(ctx, G, false, maxrev, &S.overflow_rows, work);
     if (rc) return rc;

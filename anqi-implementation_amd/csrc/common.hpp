@@ -162,6 +162,7 @@ __device__ __forceinline__ float qrow_l2(const QRow& q, const float* __restrict_
 //   MODE 1: out[s] = d2_combine(<q, x>, nq, nrm[id])   (nndindex::d2)
 //   MODE 2: out[s] = ||q - x||^2 over the dim (= 16 MC) coordinates of X's rows
 //           + nrm[id] (the rest of the sum, already reduced: split lifted rows)
+//   MODE 3: out[s] = -<q, x>   (the inner-product key, dataset_gt.cpp:225)
 template <int MC, int MODE, int RB>
 __device__ __forceinline__ void rows_pass(const QRow& q, const float* __restrict__ X, uint32_t dim,
                                           const float* __restrict__ nrm, float nq, const uint32_t* ids, int k,
@@ -178,7 +179,7 @@ __device__ __forceinline__ void rows_pass(const QRow& q, const float* __restrict
 #pragma unroll
             for (int t = 0; t < MC; t++) x[j][t] = row[16 * t + l];
             xt[j] = (l < r) ? row[16 * MC + l] : 0.0f;
-            nb[j] = MODE != 0 ? nrm[id] : 0.0f;
+            nb[j] = (MODE == 1 || MODE == 2) ? nrm[id] : 0.0f;
         } else {
 #pragma unroll
             for (int t = 0; t < MC; t++) x[j][t] = 0.0f;
@@ -190,7 +191,7 @@ __device__ __forceinline__ void rows_pass(const QRow& q, const float* __restrict
     for (int j = 0; j < RB; j++) {
         float acc = 0.0f;
         float res;
-        if (MODE != 1) {
+        if (MODE != 1 && MODE != 3) {
 #pragma unroll
             for (int t = 0; t < MC; t++) {
                 float a = q.v[t] - x[j][t];
@@ -207,7 +208,7 @@ __device__ __forceinline__ void rows_pass(const QRow& q, const float* __restrict
             for (int t = 0; t < MC; t++) acc = fmaf(q.v[t], x[j][t], acc);
             res = tree16(acc);
             if (r) res += tree16(q.tail * xt[j]);
-            res = fmaf(-2.0f, res, nq + nb[j]);
+            res = MODE == 3 ? -res : fmaf(-2.0f, res, nq + nb[j]);
         }
         const int s = s0 + g + 4 * j;
         if (l == 0 && s < k) out[s] = res;
@@ -235,6 +236,32 @@ __device__ __forceinline__ void rows_batch(const QRow& q, const float* __restric
     int s0 = 0;
     for (; k - s0 > 4 * RB; s0 += 4 * RB) rows_pass<MC, MODE, RB>(q, X, dim, nrm, nq, ids, k, l, g, out, s0);
     if (s0 < k) rows_tail<MC, MODE, RB>(q, X, dim, nrm, nq, ids, k, l, g, out, s0);
+}
+
+// nndindex's "ip" distance 1 - <x, y> (include/distance.h:170-175), a scalar
+// loop in the reference that GCC 11 at the reference's flags vectorises: the
+// products of the first nv = 16 floor(d / 16) + (8 if d mod 16 >= 8) elements
+// are rounded and added to the sum one at a time in element order, the last
+// d mod 8 run as fused multiply-adds.  One lane per pair (the sum is
+// sequential); rows 16-byte aligned when d % 4 == 0.
+__device__ __forceinline__ float ipdist_seq(const float* __restrict__ x, const float* __restrict__ y, uint32_t d) {
+    uint32_t nv = 16u * (d / 16u);
+    if (d - nv >= 8u) nv += 8u;
+    float p = 0.0f;
+    if ((d & 3u) == 0u) {
+        for (uint32_t i = 0; i < nv; i += 4) {
+            const float4 a = *reinterpret_cast<const float4*>(x + i);
+            const float4 b = *reinterpret_cast<const float4*>(y + i);
+            p = p + a.x * b.x;
+            p = p + a.y * b.y;
+            p = p + a.z * b.z;
+            p = p + a.w * b.w;
+        }
+    } else {
+        for (uint32_t i = 0; i < nv; i++) p = p + x[i] * y[i];
+    }
+    for (uint32_t i = nv; i < d; i++) p = fmaf(x[i], y[i], p);
+    return 1.0f - p;
 }
 
 // nndindex::d2 (include/index.h:172-176): na + nb - 2 ip.

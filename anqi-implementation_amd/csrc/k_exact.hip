@@ -25,9 +25,10 @@
 #define EX_C 64        // columns per tile
 #define EX_RPG 4       // rows per 16-lane group
 
+// ipm: the inner-product key -<x_i, x_j> (dataset_gt.cpp:225) instead of d2.
 __global__ void __launch_bounds__(256) k_exact_topk(const float* __restrict__ x, const float* __restrict__ nrm,
                                                     uint32_t n, uint32_t d, uint32_t K, uint32_t row0,
-                                                    uint32_t rows, uint32_t* __restrict__ out_ids,
+                                                    uint32_t rows, int ipm, uint32_t* __restrict__ out_ids,
                                                     float* __restrict__ out_keys) {
     extern __shared__ __align__(16) char lds[];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l = lane & 15;
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(256) k_exact_topk(const float* __restrict__ x,
 #pragma unroll
             for (int q = 0; q < EX_RPG; q++) {
                 float ip = qrow_ip(QR[q], y, (int)d, l);
-                float key = d2_combine(ip, nr[q], nc);
+                float key = ipm ? -ip : d2_combine(ip, nr[q], nc);
                 if (l == 0 && rid[q] < rend && cid != rid[q]) {
                     uint64_t pk = pkey(key, cid);
                     if (pk < th[q]) {
@@ -165,23 +166,22 @@ static size_t exact_lds(uint32_t d, uint32_t K) {
 }
 
 hipError_t launch_exact_topk(const float* x, const float* nrm, uint32_t n, uint32_t d,
-                             uint32_t K, uint32_t row0, uint32_t rows, int mfma,
-                             const uint16_t* xbf, float margin_scale, uint32_t* ids,
+                             uint32_t K, uint32_t row0, uint32_t rows, int ipm, uint32_t* ids,
                              float* keys, hipStream_t s) {
-    (void)mfma; (void)xbf; (void)margin_scale;
     if (rows == 0) return hipSuccess;
     size_t lds = exact_lds(d, K);
     if (hipError_t e = set_lds_limit((const void*)k_exact_topk, lds); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_exact_topk, dim3((rows + EX_R - 1) / EX_R), dim3(256), lds, s, x, nrm, n, d, K, row0,
-                       rows, ids, keys);
+                       rows, ipm, ids, keys);
     return hipGetLastError();
 }
 
 // ---- reverse-kNN membership ------------------------------------------------------------
 #define RK_COLS 4096
+// ipm: the inner-product key -<q, x_j> (dataset_gt.cpp:292) instead of d2.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_rknn(const float* __restrict__ base, const float* __restrict__ bnorm,
-                                              uint32_t n, const float* __restrict__ q, uint32_t d,
+                                              uint32_t n, const float* __restrict__ q, uint32_t d, int ipm,
                                               const float* __restrict__ rk2, uint32_t* __restrict__ counts,
                                               const uint32_t* __restrict__ off, uint32_t* __restrict__ ids,
                                               float* __restrict__ keys) {
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) k_rknn(const float* __restrict__ base, co
     uint32_t local = 0;
     for (uint32_t j = c0 + grp; j < c1; j += 16) {
         float ip = qrow_ip(Q, base + (size_t)j * d, (int)d, l);
-        float key = fmaf(-2.0f, ip, qn + bnorm[j]);
+        float key = ipm ? -ip : fmaf(-2.0f, ip, qn + bnorm[j]);
         if (l == 0 && key <= rk2[j]) {
             if (FILL) {
                 uint32_t p = atomicAdd(&counts[qi], 1u);
@@ -209,17 +209,17 @@ __global__ void __launch_bounds__(256) k_rknn(const float* __restrict__ base, co
 }
 
 hipError_t launch_rknn_count(const float* base, const float* bnorm, uint32_t n,
-                             const float* q, uint32_t nq, uint32_t d, const float* rk2,
+                             const float* q, uint32_t nq, uint32_t d, int ipm, const float* rk2,
                              uint32_t* counts, hipStream_t s) {
     if (nq == 0 || n == 0) return hipSuccess;
     dim3 grid((n + RK_COLS - 1) / RK_COLS, nq);
-    hipLaunchKernelGGL((k_rknn<false>), grid, dim3(256), 0, s, base, bnorm, n, q, d, rk2, counts,
+    hipLaunchKernelGGL((k_rknn<false>), grid, dim3(256), 0, s, base, bnorm, n, q, d, ipm, rk2, counts,
                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (float*)nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_rknn_fill(const float* base, const float* bnorm, uint32_t n,
-                            const float* q, uint32_t nq, uint32_t d, const float* rk2,
+                            const float* q, uint32_t nq, uint32_t d, int ipm, const float* rk2,
                             const uint32_t* off, uint32_t* ids, float* keys, hipStream_t s) {
     if (nq == 0 || n == 0) return hipSuccess;
     uint32_t* cursor = nullptr;
@@ -228,7 +228,8 @@ hipError_t launch_rknn_fill(const float* base, const float* bnorm, uint32_t n,
     e = hipMemsetAsync(cursor, 0, sizeof(uint32_t) * nq, s);
     if (e == hipSuccess) {
         dim3 grid((n + RK_COLS - 1) / RK_COLS, nq);
-        hipLaunchKernelGGL((k_rknn<true>), grid, dim3(256), 0, s, base, bnorm, n, q, d, rk2, cursor, off, ids, keys);
+        hipLaunchKernelGGL((k_rknn<true>), grid, dim3(256), 0, s, base, bnorm, n, q, d, ipm, rk2, cursor, off, ids,
+                           keys);
         e = hipGetLastError();
     }
     const hipError_t ef = hipFreeAsync(cursor, s);

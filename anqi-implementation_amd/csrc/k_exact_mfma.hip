@@ -125,7 +125,9 @@ __device__ __forceinline__ void xm_merge(XmLds& S, int w, int r, uint32_t K, uin
     XM_WSYNC();
 }
 
-template <int MC>
+// IPM: the inner-product key -<x_i, x_j> (dataset_gt.cpp:225); approx = -ip_bf16,
+// and M_i (twice a bound on the inner product's error) still covers it.
+template <int MC, bool IPM>
 __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x, const uint16_t* __restrict__ xb,
                                                     const float* __restrict__ nrm, const float* __restrict__ marg,
                                                     uint32_t n, uint32_t d, uint32_t K, uint32_t row0,
@@ -211,7 +213,7 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
             const float bc = S.bnc[cl];
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                float approx = fmaf(-2.0f, acc[v], bnr[v] + bc);
+                float approx = IPM ? -acc[v] : fmaf(-2.0f, acc[v], bnr[v] + bc);
                 if (cid < n && rid[v] < rend && cid != rid[v] && approx <= thrm[v]) {
                     int rl = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                     uint32_t p = atomicAdd(&S.scnt[rl], 1u);
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
                 const int k = (int)min(64u, cnt - b0);
                 if (lane < k) xid[lane] = c0 + S.surv[(size_t)r * XM_C + b0 + lane];
                 XM_WSYNC();
-                rows_batch<MC, 1>(QR, x, d, nrm, nr, xid, k, l, g, xd);
+                rows_batch<MC, IPM ? 3 : 1>(QR, x, d, nrm, nr, xid, k, l, g, xd);
                 XM_WSYNC();
                 const uint64_t th = S.thr[r];
                 uint64_t key = ~0ull;
@@ -303,27 +305,35 @@ bool exact_mfma_supported(uint32_t d, uint32_t K) {
     return d <= 16 * XM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && K <= 128;
 }
 
-template <int MC>
+template <int MC, bool IPM>
 static hipError_t launch_exact_mfma_t(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
                                       uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
                                       uint32_t* ids, float* keys, hipStream_t s) {
     size_t lds = exact_mfma_lds(K);
-    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC>, lds); e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_exact_mfma<MC>, dim3((rows + XM_R - 1) / XM_R), dim3(256), lds, s, x, xb, nrm, marg, n,
-                       d, K, row0, rows, ids, keys);
+    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC, IPM>, lds); e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_exact_mfma<MC, IPM>), dim3((rows + XM_R - 1) / XM_R), dim3(256), lds, s, x, xb, nrm, marg,
+                       n, d, K, row0, rows, ids, keys);
     return hipGetLastError();
 }
 
+template <bool IPM>
+static hipError_t launch_exact_mfma_m(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
+                                      uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
+                                      uint32_t* ids, float* keys, hipStream_t s) {
+    switch (d >> 4) {
+        case 8: return launch_exact_mfma_t<8, IPM>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        case 7: return launch_exact_mfma_t<7, IPM>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        default: return launch_exact_mfma_t<6, IPM>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+    }
+}
+
 hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nrm, const float* marg, uint32_t n,
-                             uint32_t d, uint32_t K, uint32_t row0, uint32_t rows, uint32_t* ids, float* keys,
-                             hipStream_t s) {
+                             uint32_t d, uint32_t K, uint32_t row0, uint32_t rows, int ipm, uint32_t* ids,
+                             float* keys, hipStream_t s) {
     if (rows == 0) return hipSuccess;
     if (!exact_mfma_supported(d, K)) return hipErrorInvalidValue;
-    switch (d >> 4) {
-        case 8: return launch_exact_mfma_t<8>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-        case 7: return launch_exact_mfma_t<7>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-        default: return launch_exact_mfma_t<6>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-    }
+    return ipm ? launch_exact_mfma_m<true>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
+               : launch_exact_mfma_m<false>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
 }
 
 hipError_t launch_exact_norms2(const float* x, const float* nrm, uint32_t n, uint32_t d, float* nx, float* ex,

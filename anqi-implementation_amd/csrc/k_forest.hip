@@ -20,6 +20,39 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+// The "ip" metric's split (rptrees.h:41-52): h = o0 / |o0| - o1 / |o1|,
+// normalised, offset 0 (norms below EPS taken as 1).  Lane l of the group
+// holds elements l, 16 + l, ... of h in registers; its squared norm is
+// reduced with the inner-product tree.
+__device__ void angular_plane(const ForestLevel& F, const float* a, const float* b, float* h, uint32_t sgi,
+                              int l) {
+    const int d = (int)F.d, m = d >> 4, r = d & 15;
+    float n0 = sqrtf(g16_ip(a, a, d, l));
+    float n1 = sqrtf(g16_ip(b, b, d, l));
+    if (fabsf(n0) < 1e-6f) n0 = 1.0f;
+    if (fabsf(n1) < 1e-6f) n1 = 1.0f;
+    float hv[QMAX_CHUNKS + 1];
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < QMAX_CHUNKS; t++) {
+        hv[t] = 0.0f;
+        if (t < m) {
+            hv[t] = a[16 * t + l] / n0 - b[16 * t + l] / n1;
+            acc = fmaf(hv[t], hv[t], acc);
+        }
+    }
+    float hn2 = tree16(acc);
+    hv[QMAX_CHUNKS] = (l < r) ? a[16 * m + l] / n0 - b[16 * m + l] / n1 : 0.0f;
+    if (r) hn2 += tree16(hv[QMAX_CHUNKS] * hv[QMAX_CHUNKS]);
+    float hn = sqrtf(hn2);
+    if (fabsf(hn) < 1e-6f) hn = 1.0f;
+#pragma unroll
+    for (int t = 0; t < QMAX_CHUNKS; t++)
+        if (t < m) h[16 * t + l] = hv[t] / hn;
+    if (l < r) h[16 * m + l] = hv[QMAX_CHUNKS] / hn;
+    if (l == 0) F.seg_off[sgi] = 0.0f;
+}
+
 // One 16-lane group per segment: hyperplane and offset.
 __global__ void __launch_bounds__(256) k_split_planes(ForestLevel F) {
     uint32_t sgi = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
@@ -31,6 +64,10 @@ __global__ void __launch_bounds__(256) k_split_planes(ForestLevel F) {
     const float* a = F.x + (size_t)p0 * F.d;
     const float* b = F.x + (size_t)p1 * F.d;
     float* h = F.seg_hyper + (size_t)sgi * F.d;
+    if (F.angular) {
+        angular_plane(F, a, b, h, sgi, l);
+        return;
+    }
     for (uint32_t k = l; k < F.d; k += 16) h[k] = a[k] - b[k];
     // offset = <h, midpoint>, midpoint = (a + b) / 2 (exact halving)
     int m = (int)F.d >> 4, r = (int)F.d & 15;
@@ -147,7 +184,23 @@ __global__ void __launch_bounds__(256) k_leaf_knn(LeafParams P) {
         qrow_load(QR, P.x + (size_t)r * d, d, l);
         float nr = P.nrm[r];
         uint32_t nc = 0;
-        for (uint32_t b0 = 0; b0 < size; b0 += 4) {
+        if (P.ipm) {
+            // "ip": one member per lane, 1 - <x_r, x_c> summed in element order
+            for (uint32_t b0 = 0; b0 < size; b0 += 64) {
+                const uint32_t b = b0 + (uint32_t)lane;
+                int ok = 0;
+                float dv = 0.0f;
+                uint32_t c = 0;
+                if (b < size && b != a) {
+                    c = mem[b];
+                    dv = ipdist_seq(P.x + (size_t)r * d, P.x + (size_t)c * d, (uint32_t)d);
+                    ok = pkey(dv, c) < thr;
+                }
+                uint64_t m = ballot(ok);
+                if (ok) { uint32_t p = nc + lanes_below(m); cd[p] = dv; ci[p] = c; }
+                nc += (uint32_t)popc64(m);
+            }
+        } else for (uint32_t b0 = 0; b0 < size; b0 += 4) {
             uint32_t b = b0 + (uint32_t)g;
             int ok = 0;
             float dv = 0.0f;
@@ -229,7 +282,7 @@ hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s) {
 // init_knn's fill for short rows (index.h:1452-1460): K - cnt random draws,
 // each inserted unless already present.  Rare; one thread per row.
 __global__ void k_random_fill(const float* __restrict__ x, const float* __restrict__ nrm, uint32_t n,
-                              uint32_t d, uint32_t K, uint64_t seed, uint32_t* __restrict__ ids,
+                              uint32_t d, uint32_t K, uint64_t seed, int ipm, uint32_t* __restrict__ ids,
                               float* __restrict__ keys, uint32_t* __restrict__ cnt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -243,9 +296,13 @@ __global__ void k_random_fill(const float* __restrict__ x, const float* __restri
         bool dup = false;
         for (uint32_t t = 0; t < c; t++) dup |= (rid[t] == r);
         if (dup) continue;
-        // sequential FP32 contract (16 lanes emulated in registers)
         const float* a = x + (size_t)i * d;
         const float* b = x + (size_t)r * d;
+        float dv;
+        if (ipm) {
+            dv = ipdist_seq(a, b, d);
+        } else {
+        // sequential FP32 contract (16 lanes emulated in registers)
         float acc[16];
         for (int q = 0; q < 16; q++) acc[q] = 0.0f;
         uint32_t m = d >> 4, rr = d & 15;
@@ -262,7 +319,8 @@ __global__ void k_random_fill(const float* __restrict__ x, const float* __restri
             for (int q = 0; q < 4; q++) t6[q] = t3[q + 4] + t3[q];
             ip += (t6[0] + t6[2]) + (t6[1] + t6[3]);
         }
-        float dv = fmaf(-2.0f, ip, nrm[i] + nrm[r]);
+        dv = fmaf(-2.0f, ip, nrm[i] + nrm[r]);
+        }
         // sorted insertion
         uint32_t p = c;
         while (p > 0 && (rk[p - 1] > dv || (rk[p - 1] == dv && rid[p - 1] > r))) {
@@ -275,10 +333,10 @@ __global__ void k_random_fill(const float* __restrict__ x, const float* __restri
 }
 
 hipError_t launch_random_fill(const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t K,
-                              uint64_t seed, uint32_t* ids, float* keys, uint32_t* cnt, hipStream_t s) {
+                              uint64_t seed, int ipm, uint32_t* ids, float* keys, uint32_t* cnt, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_random_fill, dim3((n + 127) / 128), dim3(128), 0, s, x, nrm, n, d, K, seed, ids, keys,
-                       cnt);
+    hipLaunchKernelGGL(k_random_fill, dim3((n + 127) / 128), dim3(128), 0, s, x, nrm, n, d, K, seed, ipm, ids,
+                       keys, cnt);
     return hipGetLastError();
 }
 

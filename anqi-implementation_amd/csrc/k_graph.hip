@@ -694,7 +694,11 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         __syncthreads();
         if (!seeded) {
             nd2 += kall;
-            if constexpr (MC > 0) {
+            if (!GRAPH && P.ipm) {
+                // "ip" (AKNN source only): one first visit per lane
+                for (uint32_t t = (uint32_t)lane; t < kall; t += 64)
+                    S.tdv[t] = ipdist_seq(P.x + (size_t)i * d, P.x + (size_t)S.tid[t] * d, (uint32_t)d);
+            } else if constexpr (MC > 0) {
                 // register-list form: sixteen rows per group in flight for
                 // the graph (about eight first visits per expansion), four
                 // for the AKNN source (one or two), whose registers then

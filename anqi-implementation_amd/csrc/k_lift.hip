@@ -99,21 +99,23 @@ hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, h
 // radius is the row's K-th cutoff, src/rknn_query.cpp:2203-2236): cut =
 // max(0, keys) elementwise (NaN stays NaN, as std::max(0.0f, x) would keep
 // 0 only for x < 0), horizon[i] = cut[i][K-1].
+// For the "ip" metric (ipm) the .rec key is the distance minus 1
+// (build_aknn_source.cpp:125-126), not clamped.
 __global__ void __launch_bounds__(256) k_aknn_cutoffs(const float* __restrict__ keys, uint64_t m, uint32_t K,
-                                                      float* __restrict__ cut, float* __restrict__ horizon) {
+                                                      int ipm, float* __restrict__ cut, float* __restrict__ horizon) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
     const float v = keys[t];
-    const float c = v < 0.0f ? 0.0f : v;
+    const float c = ipm ? v - 1.0f : (v < 0.0f ? 0.0f : v);
     cut[t] = c;
     if (horizon && t % K == K - 1) horizon[t / K] = c;
 }
 
-hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon,
+hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, int ipm, float* cut, float* horizon,
                                hipStream_t s) {
     const uint64_t m = n * K;
     if (m == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_aknn_cutoffs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, keys, m, K, cut,
+    hipLaunchKernelGGL(k_aknn_cutoffs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, keys, m, K, ipm, cut,
                        horizon);
     return hipGetLastError();
 }

@@ -78,7 +78,7 @@ hipError_t launch_lift_rows(const float* x, uint64_t n, uint32_t d, const float*
                             int ip, float* lifted, uint64_t* maxbits, hipStream_t s);
 hipError_t launch_lift_pad(float* lifted, uint64_t n, uint32_t d, uint64_t m2, hipStream_t s);
 // cut = max(0, keys) (n x K), horizon (optional, n) = cut[:, K-1].
-hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon,
+hipError_t launch_aknn_cutoffs(const float* keys, uint64_t n, uint32_t K, int ipm, float* cut, float* horizon,
                                hipStream_t s);
 // Byte rows of a base whose values are all floats 0..255: brows as
 // brow_fetch reads them; *bad (zeroed by the caller) counts elements whose
@@ -206,6 +206,7 @@ struct GatherParams {
     uint32_t h16;           // 16-bit LDS visited set of H16_SLOTS entries (hmask unused): 1 n <= 2^20,
                             // 2 n <= 2^24 (with a 32-bit overflow table); 0: 32-bit entries
     unsigned long long* work;  // optional: [0] += distances computed, [1] += adjacency ids read (finished rows)
+    int ipm;                // AKNN "ip" metric: distance 1 - <x_i, x_c> (ipdist_seq), one lane per pair
 };
 // rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
 bool gather_register_lists(const GatherParams& P, bool graph_mode);
@@ -249,6 +250,7 @@ struct ForestLevel {
     uint8_t* side;          // trees x n
     uint32_t nseg;
     uint64_t total;         // trees * n
+    int angular;            // the "ip" metric's split (rptrees.h:41-52): unit-difference normal, offset 0
 };
 hipError_t launch_split_planes(const ForestLevel& F, hipStream_t s);
 hipError_t launch_split_sides(const ForestLevel& F, uint64_t seed, hipStream_t s);
@@ -267,6 +269,7 @@ struct LeafParams {
     uint32_t* cnt;              // n
     uint32_t max_leaf;
     unsigned long long* nsurv;  // MFMA sweeps: pairs passing the filter (may be null)
+    int ipm;                    // "ip" metric: distance 1 - <x, y> (ipdist_seq), FP32 kernel only
 };
 size_t leaf_lds_bytes(const LeafParams& P);
 hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s);
@@ -280,27 +283,27 @@ hipError_t launch_leaf_mfma(const LeafParams& P, bool fp8, const void* xq, float
 hipError_t launch_to_fp8(const float* x, const float* nrm, uint32_t n, uint32_t d, float scale, uint8_t* x8,
                          float* nx, float* ex, hipStream_t s);
 hipError_t launch_random_fill(const float* x, const float* nrm, uint32_t n, uint32_t d, uint32_t K,
-                              uint64_t seed, uint32_t* ids, float* keys, uint32_t* cnt,
+                              uint64_t seed, int ipm, uint32_t* ids, float* keys, uint32_t* cnt,
                               hipStream_t s);
 
 // ---- exact kNN / RkNN membership (k_exact.hip) ------------------------------------
+// ipm != 0: inner-product keys -<x, y> (dataset_gt.cpp:225, 292), else d2.
 hipError_t launch_exact_topk(const float* x, const float* nrm, uint32_t n, uint32_t d,
-                             uint32_t K, uint32_t row0, uint32_t rows, int mfma,
-                             const uint16_t* xbf, float margin_scale, uint32_t* ids,
+                             uint32_t K, uint32_t row0, uint32_t rows, int ipm, uint32_t* ids,
                              float* keys, hipStream_t s);
 hipError_t launch_rknn_count(const float* base, const float* bnorm, uint32_t n,
-                             const float* q, uint32_t nq, uint32_t d, const float* rk2,
+                             const float* q, uint32_t nq, uint32_t d, int ipm, const float* rk2,
                              uint32_t* counts, hipStream_t s);
 hipError_t launch_rknn_fill(const float* base, const float* bnorm, uint32_t n,
-                            const float* q, uint32_t nq, uint32_t d, const float* rk2,
+                            const float* q, uint32_t nq, uint32_t d, int ipm, const float* rk2,
                             const uint32_t* off, uint32_t* ids, float* keys, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, uint64_t m, uint16_t* out, hipStream_t s);
 // MFMA-filtered exact top-K (k_exact_mfma.hip); same output as launch_exact_topk.
 bool exact_mfma_supported(uint32_t d, uint32_t K);
 size_t exact_mfma_lds(uint32_t K);
 hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nrm, const float* marg, uint32_t n,
-                             uint32_t d, uint32_t K, uint32_t row0, uint32_t rows, uint32_t* ids, float* keys,
-                             hipStream_t s);
+                             uint32_t d, uint32_t K, uint32_t row0, uint32_t rows, int ipm, uint32_t* ids,
+                             float* keys, hipStream_t s);
 hipError_t launch_exact_norms2(const float* x, const float* nrm, uint32_t n, uint32_t d, float* nx, float* ex,
                                hipStream_t s);
 hipError_t launch_exact_margin(const float* nx, const float* ex, uint32_t n, uint32_t d, float Y, float E,

@@ -27,10 +27,13 @@ class Index:
 
 
 def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, precision=0,
-          M=48, nn_k=50, graph_leaf=100, explore=100, iters=6, sample=20, alpha=1.2, log=print):
+          M=48, nn_k=50, graph_leaf=100, explore=100, iters=6, sample=20, alpha=1.2, log=print,
+          metric=pyanqi.METRIC_L2):
     """Builds the whole index from host base rows; returns Index with device
     tensors and the stage timings (seconds, host clock around synchronised
-    stages: the reference's end_to_end_s decomposition)."""
+    stages: the reference's end_to_end_s decomposition).  metric: L2 or the
+    inner product ("ip": AKNN on 1 - <x, y>, .rec keys minus 1, the ip lift,
+    per-query R2 and the -<o, q> post-filter)."""
     T = ctx.torch
     ix = Index()
     t = {}
@@ -38,8 +41,8 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     _sync(ctx)
     t0 = time.perf_counter()
     ids, keys, st = ctx.aknn_build(base, K=K, leaf=leaf, trees=trees, ef=ef, ef_alpha=ef_alpha, seed=seed,
-                                   precision=precision)
-    cut, horizon = ctx.aknn_cutoffs(keys)          # .rec key = max(0, d2); lift horizon = key K
+                                   precision=precision, metric=metric)
+    cut, horizon = ctx.aknn_cutoffs(keys, metric)  # .rec key = max(0, d2) (ip: d - 1); lift horizon = key K
     _sync(ctx)
     t1 = time.perf_counter()
     t["aknn_source_s"] = t1 - t0
@@ -61,10 +64,10 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     t["encode_s"] = t3 - t2
     if precision >= 1:
         # device lift (soft binary64), bit-identical to the host lift
-        lifted, mstar2, scale = ctx.lift(base, horizon)
+        lifted, mstar2, scale = ctx.lift(base, horizon, metric)
     else:
         rk = horizon.cpu().numpy()
-        lifted_np, mstar2, scale = pyanqi.lift_host(base_np, rk)
+        lifted_np, mstar2, scale = pyanqi.lift_host(base_np, rk, metric)
         lifted = ctx.to_dev(lifted_np)
     _sync(ctx)
     t4 = time.perf_counter()
@@ -88,27 +91,30 @@ def build(ctx, base_np, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, 
     ix.mu, ix.G, ix.cb, ix.coeff, ix.codes, ix.codes_km = mu_d, G_d, cb_d, coeff, codes, codes_km
     ix.mstar2, ix.scale = mstar2, scale
     ix.bnorm = ctx.norms(base)
-    ix.handle = ctx.make_index(base, ix.bnorm, lifted, deg, links, tree, mstar2, scale)
+    ix.handle = ctx.make_index(base, ix.bnorm, lifted, deg, links, tree, mstar2, scale, metric=metric)
+    ix.metric = metric
     ix.times = t
     return ix
 
 
 def thresholds(ctx, ix, k, slack=1.0):
-    return ctx.lrq_materialize(ix.coeff, ix.codes_km, ix.mu, ix.G, ix.cb, k, slack)
+    return ctx.lrq_materialize(ix.coeff, ix.codes_km, ix.mu, ix.G, ix.cb, k, slack,
+                               metric=getattr(ix, "metric", pyanqi.METRIC_L2))
 
 
 def threshold_tables(ctx, ix, ks, slack=1.0):
     """One table per k in ks (len(ks) x n), decoded in one pass over the rows."""
-    return ctx.lrq_materialize_multi(ix.coeff, ix.codes_km, ix.mu, ix.G, ix.cb, list(ks), slack)
+    return ctx.lrq_materialize_multi(ix.coeff, ix.codes_km, ix.mu, ix.G, ix.cb, list(ks), slack,
+                                     metric=getattr(ix, "metric", pyanqi.METRIC_L2))
 
 
-def ground_truth(ctx, base, queries, k):
+def ground_truth(ctx, base, queries, k, metric=pyanqi.METRIC_L2):
     """Exact RkNN CSR (tools/dataset_gt.cpp): exact top-k thresholds, then
-    membership key <= r_k."""
+    membership key <= r_k (key -<x, y> for the inner product)."""
     n = base.shape[0]
-    ids, keys = ctx.exact_knn(base, k, precision=1)   # bf16 MFMA sweep, FP32-exact output
+    ids, keys = ctx.exact_knn(base, k, precision=1, metric=metric)   # bf16 MFMA sweep, FP32-exact output
     rk2 = keys[:, k - 1].contiguous()
-    off, gids, gkeys = ctx.rknn_membership(base, queries, rk2)
+    off, gids, gkeys = ctx.rknn_membership(base, queries, rk2, metric=metric)
     return off, gids.cpu().numpy().view(np.uint32), rk2
 
 

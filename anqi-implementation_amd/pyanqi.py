@@ -24,7 +24,7 @@ F64 = ctypes.c_double
 
 class AknnParams(ctypes.Structure):
     _fields_ = [("K", U32), ("leaf", U32), ("trees", U32), ("ef", U32), ("ef_alpha", U32),
-                ("max_depth", U32), ("seed", U64), ("precision", I32)]
+                ("max_depth", U32), ("seed", U64), ("precision", I32), ("metric", I32)]
 
 
 class AknnStats(ctypes.Structure):
@@ -89,6 +89,8 @@ SIGS = {
     "anqi_pair_d2": (I32, [P, P, P, U32, P, P, U64, P]),
     "anqi_exact_knn": (I32, [P, P, U32, U32, U32, I32, P, P]),
     "anqi_rknn_membership": (I32, [P, P, U32, P, U32, U32, P, P, ctypes.POINTER(U64), P, P]),
+    "anqi_exact_knn_metric": (I32, [P, P, U32, U32, U32, I32, I32, P, P]),
+    "anqi_rknn_membership_metric": (I32, [P, P, U32, P, U32, U32, I32, P, P, ctypes.POINTER(U64), P, P]),
     "anqi_aknn_build": (I32, [P, P, U32, U32, ctypes.POINTER(AknnParams), P, P, ctypes.POINTER(AknnStats)]),
     "anqi_lrq_encode": (I32, [P, P, U64, U32, U32, P, P, P, U32, P, P, P]),
     "anqi_transpose_codes": (I32, [P, P, U64, U32, P]),
@@ -98,6 +100,7 @@ SIGS = {
     "anqi_ctx_lrq_fallbacks": (I32, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
     "anqi_lrq_materialize": (I32, [P, P, P, U64, U32, U32, P, P, P, U32, U32, F64, P]),
     "anqi_aknn_cutoffs": (I32, [P, P, U64, U32, P, P]),
+    "anqi_aknn_cutoffs_metric": (I32, [P, P, U64, U32, I32, P, P]),
     "anqi_lift_host": (I32, [P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lift": (I32, [P, P, U64, U32, P, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
     "anqi_lift_host_metric": (I32, [P, U64, U32, P, I32, P, ctypes.POINTER(F64), ctypes.POINTER(F64)]),
@@ -224,40 +227,43 @@ class Context:
         self.call("anqi_norms", dp(x), n, d, dp(out))
         return out
 
-    def exact_knn(self, x, K, precision=0):
+    def exact_knn(self, x, K, precision=0, metric=METRIC_L2):
         n, d = x.shape
         ids = self.empty((n, K), self.torch.int32)
         keys = self.empty((n, K), self.torch.float32)
-        self.call("anqi_exact_knn", dp(x), n, d, K, precision, dp(ids), dp(keys))
+        self.call("anqi_exact_knn_metric", dp(x), n, d, K, precision, metric, dp(ids), dp(keys))
         return ids, keys
 
-    def rknn_membership(self, base, q, rk2):
+    def rknn_membership(self, base, q, rk2, metric=METRIC_L2):
         n, d = base.shape
         nq = q.shape[0]
         off = np.zeros(nq + 1, np.uint32)
         tot = U64(0)
-        self.call("anqi_rknn_membership", dp(base), n, dp(q), nq, d, dp(rk2), hp(off), ctypes.byref(tot), None, None)
+        self.call("anqi_rknn_membership_metric", dp(base), n, dp(q), nq, d, metric, dp(rk2), hp(off),
+                  ctypes.byref(tot), None, None)
         ids = self.empty((max(1, tot.value),), self.torch.int32)
         keys = self.empty((max(1, tot.value),), self.torch.float32)
-        self.call("anqi_rknn_membership", dp(base), n, dp(q), nq, d, dp(rk2), hp(off), ctypes.byref(tot), dp(ids),
-                  dp(keys))
+        self.call("anqi_rknn_membership_metric", dp(base), n, dp(q), nq, d, metric, dp(rk2), hp(off),
+                  ctypes.byref(tot), dp(ids), dp(keys))
         return off, ids[:tot.value], keys[:tot.value]
 
-    def aknn_build(self, x, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, precision=0, max_depth=100):
+    def aknn_build(self, x, K=100, leaf=500, trees=64, ef=600, ef_alpha=8, seed=10, precision=0, max_depth=100,
+                   metric=METRIC_L2):
         n, d = x.shape
         ids = self.empty((n, K), self.torch.int32)
         keys = self.empty((n, K), self.torch.float32)
-        p = AknnParams(K, leaf, trees, ef, ef_alpha, max_depth, seed, precision)
+        p = AknnParams(K, leaf, trees, ef, ef_alpha, max_depth, seed, precision, metric)
         st = AknnStats()
         self.call("anqi_aknn_build", dp(x), n, d, ctypes.byref(p), dp(ids), dp(keys), ctypes.byref(st))
         return ids, keys, st
 
-    def aknn_cutoffs(self, keys):
-        """(cut = max(0, keys), horizon = cut[:, K-1]) on the device (anqi_aknn_cutoffs)."""
+    def aknn_cutoffs(self, keys, metric=METRIC_L2):
+        """(cut = max(0, keys) -- keys - 1 for "ip" --, horizon = cut[:, K-1]) on
+        the device (anqi_aknn_cutoffs_metric)."""
         n, K = keys.shape
         cut = self.empty((n, K), self.torch.float32)
         hz = self.empty((n,), self.torch.float32)
-        self.call("anqi_aknn_cutoffs", dp(keys), n, K, dp(cut), dp(hz))
+        self.call("anqi_aknn_cutoffs_metric", dp(keys), n, K, metric, dp(cut), dp(hz))
         return cut, hz
 
     def lrq_encode(self, cut, mu, basis, cb):

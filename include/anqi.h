@@ -96,6 +96,14 @@ int anqi_exact_knn(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32
 int anqi_rknn_membership(anqi_ctx* ctx, const float* base, uint32_t n, const float* q,
                          uint32_t nq, uint32_t d, const float* rk2, uint32_t* off,
                          uint64_t* total, uint32_t* ids, float* keys);
+/* The same two with the metric: ANQI_METRIC_IP orders and tests by the key
+ * -<x, y> (dataset_gt.cpp:225, 292, "--metric ip"); the first two are the
+ * L2 forms. */
+int anqi_exact_knn_metric(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint32_t K,
+                          int precision, int metric, uint32_t* ids, float* keys);
+int anqi_rknn_membership_metric(anqi_ctx* ctx, const float* base, uint32_t n, const float* q,
+                                uint32_t nq, uint32_t d, int metric, const float* rk2, uint32_t* off,
+                                uint64_t* total, uint32_t* ids, float* keys);
 
 /* ---- AKNN source (build_aknn_source, include/index.h:1300-1461) ----------- */
 typedef struct {
@@ -108,6 +116,10 @@ typedef struct {
     uint64_t seed;       /* 10 */
     int precision;       /* 0 FP32 leaf tiles, 1 bf16 MFMA filtered, 2 FP8-e4m3
                             MFMA filtered (trial); every mode gives the same lists */
+    int metric;          /* ANQI_METRIC_L2 (0) or ANQI_METRIC_IP: "ip" splits on
+                            unit-difference hyperplanes (rptrees.h:41-52) and ranks
+                            by 1 - <x, y> (distance.h:170-175); its leaf pairs run
+                            in the FP32 kernel whatever the precision */
 } anqi_aknn_params;
 typedef struct {
     double forest_s, leaf_s, gather_s, total_s;
@@ -117,7 +129,8 @@ typedef struct {
     uint64_t gather_d2, gather_adj;  /* gather work: distances computed, adjacency ids read */
 } anqi_aknn_stats;
 /* device x (n*d); device ids/keys (n*K) rows sorted by (dist, id), self
- * excluded; keys are the raw d2 (the tool clamps at 0 when writing .rec). */
+ * excluded; keys are the raw distances (d2, or 1 - <x, y> for "ip"; the
+ * tool writes max(0, d2), or the distance minus 1 for "ip", into the .rec). */
 int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d,
                     const anqi_aknn_params* p, uint32_t* ids, float* keys,
                     anqi_aknn_stats* stats);
@@ -182,6 +195,10 @@ int anqi_lrq_relax(anqi_ctx* ctx, const float* cutoffs, uint64_t n, uint32_t nk,
  * horizon[i] = cut[i][K-1] (src/rknn_query.cpp:2203-2236).  device keys and
  * cut (n*K, may alias), horizon (n, may be NULL). */
 int anqi_aknn_cutoffs(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, float* cut, float* horizon);
+/* The same for the metric: ANQI_METRIC_IP gives cut = keys - 1 (the "ip"
+ * .rec key, tools/build_aknn_source.cpp:125-126). */
+int anqi_aknn_cutoffs_metric(anqi_ctx* ctx, const float* keys, uint64_t n, uint32_t K, int metric, float* cut,
+                             float* horizon);
 
 /* ---- lift (src/rknn_query.cpp:2203-2236), host FP64 ----------------------- */
 /* host base (n*d), rk_horizon (n); host lifted (n*(d+2)). */

@@ -74,6 +74,144 @@ def test_thresholds_match_reference(ctx, S, form):
         ctx.set_lrq_form(0)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_exact_knn_and_membership_match_dataset_gt(ctx, S, precision):
+    """dataset_gt --metric ip (tools/dataset_gt.cpp:206-313): the exact
+    top-100 table and the RkNN CSR at k = 10, 50, 100 equal the reference
+    tool's, ids and key bits."""
+    import pyanqi
+    ids, keys = ctx.exact_knn(ctx.to_dev(S.base), 100, precision=precision, metric=pyanqi.METRIC_IP)
+    assert np.array_equal(host(ids).view(np.uint32), S.gt_ids)
+    assert np.array_equal(bits(host(keys)), bits(S.gt_keys))
+    for k in (10, 50, 100):
+        rk = ctx.to_dev(np.ascontiguousarray(S.gt_keys[:, k - 1]))
+        off, cids, ckeys = ctx.rknn_membership(ctx.to_dev(S.base), ctx.to_dev(S.query), rk,
+                                               metric=pyanqi.METRIC_IP)
+        goff, gids, gkeys = S.csr[k]
+        assert np.array_equal(off, goff) and np.array_equal(host(cids).view(np.uint32), gids), k
+        assert np.array_equal(bits(host(ckeys)), bits(gkeys)), k
+
+
+@pytest.mark.parametrize("case", ["gaussian-2500x128-K100", "mixture-1500x96-K50", "wide-norms-2000x128-K64"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_exact_knn_ip_matches_oracle(ctx, case, precision):
+    """Inner-product exact top-K at widths the bf16 MFMA sweep takes (d 96 and
+    128): both sweeps return the oracle's ids and key bits."""
+    import pyanqi
+    import workload
+    from anqi_testlib import oracle, ptr
+    rng = np.random.default_rng(11)
+    make = {
+        "gaussian-2500x128-K100": (lambda: workload.gaussian(2500, 128, seed=5), 100),
+        "mixture-1500x96-K50": (lambda: workload.mixture(1500, 96, 50, 20.0, seed=7), 50),
+        "wide-norms-2000x128-K64": (lambda: (rng.standard_normal((2000, 128)) *
+                                             np.exp(rng.uniform(-3, 3, (2000, 1)))), 64),
+    }
+    mk, K = make[case]
+    x = np.ascontiguousarray(mk(), np.float32)
+    n, d = x.shape
+    say("exact knn ip", case, "precision", precision)
+    ids, keys = ctx.exact_knn(ctx.to_dev(x), K, precision=precision, metric=pyanqi.METRIC_IP)
+    ri = np.zeros((n, K), np.uint32)
+    rk = np.zeros((n, K), np.float32)
+    oracle().or_exact_topk_metric(ptr(x), n, d, K, 1, ptr(ri), ptr(rk), 8)
+    assert np.array_equal(host(ids).view(np.uint32), ri)
+    assert np.array_equal(bits(host(keys)), bits(rk))
+
+
+def test_dataset_gt_ip_files_equal_reference(tmp_path):
+    pfx = os.path.join(GOLDEN, "sample_ip", "sample_ip")
+    out = str(tmp_path / "g")
+    tool = os.path.join(REPO, "anqi-implementation_amd", "bin", "dataset_gt")
+    r = subprocess.run([tool, "--base", pfx + "_base.bin", "--K", "100", "--rk_k", "100", "--mode", "thresholds",
+                        "--metric", "ip", "--out_prefix", out], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert open(out + "_baseknn_gt.bin", "rb").read() == open(pfx + "_baseknn_gt.bin", "rb").read()
+    for k in (10, 50, 100):
+        o = str(tmp_path / f"q{k}")
+        r = subprocess.run([tool, "--base", pfx + "_base.bin", "--query", pfx + "_query.bin", "--K", "100",
+                            "--rk_k", str(k), "--mode", "ground-truth", "--metric", "ip", "--threshold-table",
+                            out + "_baseknn_gt.bin", "--out_prefix", o], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert open(o + "_rknn_gt.bin", "rb").read() == open(pfx + f"_rknn_gt_k{k}.bin", "rb").read(), k
+
+
+def _aknn_checks(x, ids, keys, gt_ids):
+    """Row invariants of build_aknn_source.cpp:93-117, every key equal to the
+    reference's "ip" distance of its pair (oracle, pinned to the reference's
+    compiled function), and the recall against the exact top-K."""
+    from anqi_testlib import oracle, ptr
+    O = oracle()
+    n, K = ids.shape
+    d = x.shape[1]
+    assert (ids < n).all() and (ids != np.arange(n)[:, None]).all()
+    assert all(len(set(r)) == K for r in ids.tolist())
+    assert (np.diff(keys, axis=1) >= 0).all()
+    rows = range(0, n, max(1, n // 200))
+    for i in rows:
+        for j in range(K):
+            ref = O.or_ipdist(ptr(x[i]), ptr(x[ids[i, j]]), d)
+            assert bits(keys[i, j]) == bits(ref), (i, j)
+    return np.mean([len(set(ids[i]) & set(gt_ids[i])) for i in range(n)]) / K
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_aknn_build_ip_on_reference_sample(ctx, S, precision):
+    """The AKNN source for "ip" (build_aknn_source ... ip, index.h:1300-1461
+    with angular splits and 1 - <x, y>): valid rows, distances bit-exact to
+    the reference's function, recall against dataset_gt's exact ip top-100 not
+    below the reference source's (its forest draws differ by design)."""
+    import pyanqi
+    ids, keys, st = ctx.aknn_build(ctx.to_dev(S.base), K=100, leaf=64, trees=8, ef=64, precision=precision,
+                                   metric=pyanqi.METRIC_IP)
+    ids, keys = host(ids).view(np.uint32), host(keys)
+    rec = _aknn_checks(S.base, ids, keys, S.gt_ids)
+    ref_rec = np.mean([len(set(S.rec_ids[i]) & set(S.gt_ids[i])) for i in range(S.n)]) / 100
+    say("ip aknn recall", rec, "reference source", ref_rec, "short rows", st.short_rows)
+    assert rec >= ref_rec - 0.005
+    cut, hz = ctx.aknn_cutoffs(ctx.to_dev(keys), pyanqi.METRIC_IP)
+    assert np.array_equal(bits(host(cut)), bits((keys - np.float32(1)).astype(np.float32)))
+    assert np.array_equal(bits(host(hz)), bits(host(cut)[:, 99]))
+
+
+def test_aknn_build_ip_mixture_20k(ctx):
+    """A 20,000 x 96 mixture under the build's default parameters (leaf 500,
+    64 trees, ef 600): valid rows, distance bits, and the recall of the GPU
+    source against the GPU exact ip top-100 (itself bit-exact to dataset_gt)."""
+    import pyanqi
+    import workload
+    x = np.ascontiguousarray(workload.mixture(20000, 96, 200, 20.0, seed=9), np.float32)
+    xd = ctx.to_dev(x)
+    ids, keys, st = ctx.aknn_build(xd, K=100, precision=1, metric=pyanqi.METRIC_IP)
+    gt, _ = ctx.exact_knn(xd, 100, precision=1, metric=pyanqi.METRIC_IP)
+    rec = _aknn_checks(x, host(ids).view(np.uint32), host(keys), host(gt).view(np.uint32))
+    say("ip aknn 20k recall", rec, "gather", st.gather_s, "s")
+    assert rec >= 0.9
+
+
+def test_build_aknn_source_ip_tool(tmp_path, S):
+    pfx = os.path.join(GOLDEN, "sample_ip", "sample_ip")
+    out = str(tmp_path / "a")
+    tool = os.path.join(REPO, "anqi-implementation_amd", "bin", "build_aknn_source")
+    r = subprocess.run([tool, pfx + "_base.bin", out, "100", "64", "8", "64", "1", "10", "ip"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "metric=ip" in r.stdout
+    rec = np.fromfile(out + "_baseknn_aknn.rec", dtype=np.dtype([("id", "<u4"), ("key", "<f4")])).reshape(S.n, 100)
+    ids, keys = rec["id"].copy(), rec["key"].copy()
+    # the .rec key is the distance minus 1 (build_aknn_source.cpp:125-126)
+    assert (np.diff(keys, axis=1) >= 0).all() and (ids != np.arange(S.n)[:, None]).all()
+    from anqi_testlib import oracle, ptr
+    O = oracle()
+    for i in range(0, S.n, 25):
+        for j in range(100):
+            dv = np.float32(O.or_ipdist(ptr(S.base[i]), ptr(S.base[ids[i, j]]), S.d))
+            assert bits(keys[i, j]) == bits(np.float32(dv - np.float32(1))), (i, j)
+    rec_mine = np.mean([len(set(ids[i]) & set(S.gt_ids[i])) for i in range(S.n)]) / 100
+    ref_rec = np.mean([len(set(S.rec_ids[i]) & set(S.gt_ids[i])) for i in range(S.n)]) / 100
+    assert rec_mine >= ref_rec - 0.005
+
+
 def _index(ctx, S, split):
     import pyanqi
     t = S.tree
