@@ -20,58 +20,68 @@
 // 32 (w >> 1) .. +32 against tile columns 64 (w & 1) .. +64 as two 32x32 MFMA
 // accumulators; the operand fragments follow the instruction's lane layout
 // (A: row l % 32, k 8 (l / 32) .. +8; B: column l % 32, same k; C element v:
-// row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  Surviving pairs go to a
-// per-row list; the owning wavefront re-scores them with the FP32 contract and
-// merges those beating the row's K-th pair into its sorted list.
+// row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  Surviving column ids
+// collect in a per-row LDS buffer across tiles; only when some row's buffer
+// could not take another tile (or after the last tile) do the wavefronts
+// re-score their rows' buffers with the FP32 contract and merge those beating
+// the row's K-th pair into its sorted list.  Between merges the filter keeps
+// the thresholds of the last merge, which are never below the current ones,
+// so it only lets more pairs through; batching the merges this way takes the
+// per-survivor merge latency off the sweep.
 #include "common.hpp"
 #include "kernels.hpp"
 
 typedef __bf16 xm_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float xm_f32x16 __attribute__((ext_vector_type(16)));
 
-#define XM_R 64                      // rows per block
 #define XM_C 128                     // columns per tile
 #define XM_KS 8                      // 16-wide k steps: d <= 128, zero padded
 #define XM_BS (16 * XM_KS + 8)       // bf16 per staged tile row (272 bytes)
-#define XM_VEC ((XM_C * 16 * XM_KS / 8) / 256)   // 16-byte vectors per thread per tile
+#define XM_TV (XM_C * 16 * XM_KS / 8)   // 16-byte vectors per tile (2048)
 
+// A block is RG groups of 32 rows (2 RG wavefronts, wavefront w: rows
+// 32 (w >> 1) .., tile columns 64 (w & 1) ..): RG = 2 (64 rows) or 4 (128
+// rows, when the LDS holds their lists).
 struct XmLds {
     uint16_t* btile;   // XM_C x XM_BS bf16
     float* bnc;        // XM_C column norms
-    float* ld;         // XM_R x K
-    uint32_t* li;      // XM_R x K
-    uint8_t* surv;     // XM_R x XM_C column offsets
-    uint32_t* scnt;    // XM_R
-    uint32_t* lcnt;    // XM_R
-    uint64_t* thr;     // XM_R: K-th (key, id) pair, ~0 while the list is short
-    float* thrm;       // XM_R: K-th key + M_i (filter bound), +inf while short
-    float* sd;         // 4 x K merge scratch
-    uint32_t* si;      // 4 x K
-    uint64_t* cs;      // 4 x 64 sorted new pairs
-    uint32_t* xid;     // 4 x 64 column ids under exact evaluation
-    float* xd;         // 4 x 64 exact keys
+    float* ld;         // R x K
+    uint32_t* li;      // R x K
+    uint32_t* surv;    // R x sc column ids (survivors since the last merge)
+    uint32_t* scnt;    // R
+    uint32_t* flag;    // set when some row's buffer could not take another tile
+    uint32_t* lcnt;    // R
+    uint64_t* thr;     // R: K-th (key, id) pair, ~0 while the list is short
+    float* thrm;       // R: K-th key + M_i (filter bound), +inf while short
+    float* sd;         // waves x K merge scratch
+    uint32_t* si;      // waves x K
+    uint64_t* cs;      // waves x 64 sorted new pairs
+    uint32_t* xid;     // waves x 64 column ids under exact evaluation
+    float* xd;         // waves x 64 exact keys
 };
 
 __host__ __device__ inline size_t xm_r16(size_t b) { return (b + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t xm_carve(char* base, uint32_t K, XmLds* s) {
+__host__ __device__ inline size_t xm_carve(char* base, uint32_t R, uint32_t K, uint32_t sc, XmLds* s) {
+    const uint32_t NW = R / 16;
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base ? base + o : nullptr; o += xm_r16(bytes); return p; };
     char* p;
     p = take(2ull * XM_C * XM_BS); if (s) s->btile = (uint16_t*)p;
     p = take(4ull * XM_C); if (s) s->bnc = (float*)p;
-    p = take(4ull * XM_R * K); if (s) s->ld = (float*)p;
-    p = take(4ull * XM_R * K); if (s) s->li = (uint32_t*)p;
-    p = take(1ull * XM_R * XM_C); if (s) s->surv = (uint8_t*)p;
-    p = take(4ull * XM_R); if (s) s->scnt = (uint32_t*)p;
-    p = take(4ull * XM_R); if (s) s->lcnt = (uint32_t*)p;
-    p = take(8ull * XM_R); if (s) s->thr = (uint64_t*)p;
-    p = take(4ull * XM_R); if (s) s->thrm = (float*)p;
-    p = take(4ull * 4 * K); if (s) s->sd = (float*)p;
-    p = take(4ull * 4 * K); if (s) s->si = (uint32_t*)p;
-    p = take(8ull * 4 * 64); if (s) s->cs = (uint64_t*)p;
-    p = take(4ull * 4 * 64); if (s) s->xid = (uint32_t*)p;
-    p = take(4ull * 4 * 64); if (s) s->xd = (float*)p;
+    p = take(4ull * R * K); if (s) s->ld = (float*)p;
+    p = take(4ull * R * K); if (s) s->li = (uint32_t*)p;
+    p = take(4ull * R * sc); if (s) s->surv = (uint32_t*)p;
+    p = take(4ull * R); if (s) s->scnt = (uint32_t*)p;
+    p = take(16); if (s) s->flag = (uint32_t*)p;
+    p = take(4ull * R); if (s) s->lcnt = (uint32_t*)p;
+    p = take(8ull * R); if (s) s->thr = (uint64_t*)p;
+    p = take(4ull * R); if (s) s->thrm = (float*)p;
+    p = take(4ull * NW * K); if (s) s->sd = (float*)p;
+    p = take(4ull * NW * K); if (s) s->si = (uint32_t*)p;
+    p = take(8ull * NW * 64); if (s) s->cs = (uint64_t*)p;
+    p = take(4ull * NW * 64); if (s) s->xid = (uint32_t*)p;
+    p = take(4ull * NW * 64); if (s) s->xd = (float*)p;
     return o;
 }
 
@@ -127,24 +137,28 @@ __device__ __forceinline__ void xm_merge(XmLds& S, int w, int r, uint32_t K, uin
 
 // IPM: the inner-product key -<x_i, x_j> (dataset_gt.cpp:225); approx = -ip_bf16,
 // and M_i (twice a bound on the inner product's error) still covers it.
-template <int MC, bool IPM>
-__global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x, const uint16_t* __restrict__ xb,
-                                                    const float* __restrict__ nrm, const float* __restrict__ marg,
-                                                    uint32_t n, uint32_t d, uint32_t K, uint32_t row0,
-                                                    uint32_t rows, uint32_t* __restrict__ out_ids,
-                                                    float* __restrict__ out_keys) {
+template <int MC, bool IPM, int RG>
+__global__ void __launch_bounds__(128 * RG) k_exact_mfma(const float* __restrict__ x,
+                                                         const uint16_t* __restrict__ xb,
+                                                         const float* __restrict__ nrm,
+                                                         const float* __restrict__ marg, uint32_t n, uint32_t d,
+                                                         uint32_t K, uint32_t sc, uint32_t row0, uint32_t rows,
+                                                         uint32_t* __restrict__ out_ids,
+                                                         float* __restrict__ out_keys) {
+    constexpr int R = 32 * RG, NT = 128 * RG, NV = XM_TV / NT;
     extern __shared__ __align__(16) char lds[];
     XmLds S;
-    xm_carve(lds, K, &S);
+    xm_carve(lds, R, K, sc, &S);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l = lane & 15, g = lane >> 4;
-    const uint32_t rb = row0 + blockIdx.x * XM_R;
-    const uint32_t rend = min(row0 + rows, rb + XM_R);
-    for (int r = tid; r < XM_R; r += 256) {
+    const uint32_t rb = row0 + blockIdx.x * R;
+    const uint32_t rend = min(row0 + rows, rb + R);
+    for (int r = tid; r < R; r += NT) {
         S.scnt[r] = 0;
         S.lcnt[r] = 0;
         S.thr[r] = ~0ull;
         S.thrm[r] = __builtin_huge_valf();
     }
+    if (tid == 0) *S.flag = 0u;
     // A fragments: this wave's 32 rows, bf16, zero padded past d
     const int h = w >> 1, ch = w & 1;
     const uint32_t arow = rb + 32 * h + (lane & 31);
@@ -158,7 +172,7 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
             afr[s][e] = __builtin_bit_cast(__bf16, b);
         }
     }
-    // per-lane accumulator rows: v -> row 8 (v / 4) + 4 (l / 32) + v % 4 of the half
+    // per-lane accumulator rows: v -> row 8 (v / 4) + 4 (l / 32) + v % 4 of the group
     float bnr[16];
     uint32_t rid[16];
 #pragma unroll
@@ -169,11 +183,11 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
     }
     // prefetch of tile 0
     const uint32_t vrow = d / 8;              // 16-byte vectors per base row
-    uint4 pf[XM_VEC];
+    uint4 pf[NV];
     auto fetch = [&](uint32_t c0) {
 #pragma unroll
-        for (int u = 0; u < XM_VEC; u++) {
-            uint32_t e = (uint32_t)u * 256 + (uint32_t)tid;     // vector index in the tile
+        for (int u = 0; u < NV; u++) {
+            uint32_t e = (uint32_t)u * NT + (uint32_t)tid;     // vector index in the tile
             uint32_t c = e / (16 * XM_KS / 8), kv = e % (16 * XM_KS / 8);
             uint4 z = make_uint4(0, 0, 0, 0);
             if (c0 + c < n && kv < vrow) z = *(const uint4*)(xb + (size_t)(c0 + c) * d + 8 * kv);
@@ -185,43 +199,51 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
     for (uint32_t c0 = 0; c0 < n; c0 += XM_C) {
         // stage the prefetched tile, then prefetch the next one
 #pragma unroll
-        for (int u = 0; u < XM_VEC; u++) {
-            uint32_t e = (uint32_t)u * 256 + (uint32_t)tid;
+        for (int u = 0; u < NV; u++) {
+            uint32_t e = (uint32_t)u * NT + (uint32_t)tid;
             uint32_t c = e / (16 * XM_KS / 8), kv = e % (16 * XM_KS / 8);
             *(uint4*)(S.btile + (size_t)c * XM_BS + 8 * kv) = pf[u];
         }
-        for (uint32_t c = tid; c < XM_C; c += 256) S.bnc[c] = (c0 + c < n) ? nrm[c0 + c] : 0.0f;
+        for (uint32_t c = tid; c < XM_C; c += NT) S.bnc[c] = (c0 + c < n) ? nrm[c0 + c] : 0.0f;
         __syncthreads();
         if (c0 + XM_C < n) fetch(c0 + XM_C);
-        // sweep: two 32x32 accumulators per wave
+        // sweep: two 32x32 accumulators per wave, their MFMA chains interleaved
         float thrm[16];
 #pragma unroll
         for (int v = 0; v < 16; v++) thrm[v] = S.thrm[32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)];
+        xm_f32x16 acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[t][v] = 0.0f;
+        const uint16_t* bcol0 = S.btile + (size_t)(64 * ch + (lane & 31)) * XM_BS + 8 * (lane >> 5);
+        const uint16_t* bcol1 = bcol0 + 32 * XM_BS;
+#pragma unroll
+        for (int s = 0; s < XM_KS; s++) {
+            const xm_bf16x8 b0 = *(const xm_bf16x8*)(bcol0 + 16 * s);
+            const xm_bf16x8 b1 = *(const xm_bf16x8*)(bcol1 + 16 * s);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[s], b0, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[s], b1, acc[1], 0, 0, 0);
+        }
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int cl = 64 * ch + 32 * t + (lane & 31);     // tile column of this lane
-            xm_f32x16 acc;
-#pragma unroll
-            for (int v = 0; v < 16; v++) acc[v] = 0.0f;
-            const uint16_t* bcol = S.btile + (size_t)cl * XM_BS + 8 * (lane >> 5);
-#pragma unroll
-            for (int s = 0; s < XM_KS; s++) {
-                xm_bf16x8 bfr = *(const xm_bf16x8*)(bcol + 16 * s);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[s], bfr, acc, 0, 0, 0);
-            }
             const uint32_t cid = c0 + (uint32_t)cl;
             const float bc = S.bnc[cl];
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                float approx = IPM ? -acc[v] : fmaf(-2.0f, acc[v], bnr[v] + bc);
+                float approx = IPM ? -acc[t][v] : fmaf(-2.0f, acc[t][v], bnr[v] + bc);
                 if (cid < n && rid[v] < rend && cid != rid[v] && approx <= thrm[v]) {
                     int rl = 32 * h + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    // below sc - XM_C before the tile, at most XM_C more: p < sc
                     uint32_t p = atomicAdd(&S.scnt[rl], 1u);
-                    S.surv[(size_t)rl * XM_C + p] = (uint8_t)cl;
+                    S.surv[(size_t)rl * sc + p] = cid;
+                    if (p + 1u > sc - XM_C) *S.flag = 1u;
                 }
             }
         }
         __syncthreads();
+        if (!*S.flag && c0 + XM_C < n) continue;
         // exact re-scoring and merge: wave w owns rows 16 w .. 16 w + 15
         for (int rr = 0; rr < 16; rr++) {
             const int r = w * 16 + rr;
@@ -236,7 +258,7 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
             float* xd = S.xd + w * 64;
             for (uint32_t b0 = 0; b0 < cnt; b0 += 64) {
                 const int k = (int)min(64u, cnt - b0);
-                if (lane < k) xid[lane] = c0 + S.surv[(size_t)r * XM_C + b0 + lane];
+                if (lane < k) xid[lane] = S.surv[(size_t)r * sc + b0 + lane];
                 XM_WSYNC();
                 rows_batch<MC, IPM ? 3 : 1>(QR, x, d, nrm, nr, xid, k, l, g, xd);
                 XM_WSYNC();
@@ -253,6 +275,7 @@ __global__ void __launch_bounds__(256) k_exact_mfma(const float* __restrict__ x,
             if (lane == 0) S.scnt[r] = 0;
         }
         __syncthreads();
+        if (tid == 0) *S.flag = 0u;   // ordered before the next filter by the next tile's barrier
     }
     for (int r = w * 16; r < w * 16 + 16; r++) {
         uint32_t row = rb + r;
@@ -298,32 +321,54 @@ __global__ void k_exact_margin(const float* __restrict__ nx, const float* __rest
     marg[i] = m * 1.01f + 1e-30f + 2.0f * (float)d * 1.2e-38f;
 }
 
-size_t exact_mfma_lds(uint32_t K) { return xm_carve(nullptr, K, nullptr); }
+// Survivor buffer entries per row for R-row blocks: what the LDS left by the
+// rest holds (a multiple of 64, at most 1024), or 0 when that is less than
+// one tile plus 64.  K <= 128 always fits 64-row blocks (159,504 bytes at
+// K = 128 with 192 entries).
+static uint32_t exact_mfma_sc(uint32_t R, uint32_t K) {
+    const size_t budget = 160 * 1024 - 2048;
+    const size_t rest = xm_carve(nullptr, R, K, 0, nullptr);
+    size_t sc = rest < budget ? (budget - rest) / (4 * (size_t)R) : 0;
+    sc = std::min<size_t>(1024, sc) & ~size_t(63);
+    return sc >= XM_C + 64 ? (uint32_t)sc : 0u;
+}
+
+// 128-row blocks (two wavefronts per SIMD) when their lists and a survivor
+// buffer fit (small K, e.g. K <= 12), else 64-row blocks.
+static int exact_mfma_rg(uint32_t K) { return exact_mfma_sc(128, K) ? 4 : 2; }
+
+size_t exact_mfma_lds(uint32_t K) {
+    const uint32_t R = 32u * (uint32_t)exact_mfma_rg(K);
+    return xm_carve(nullptr, R, K, exact_mfma_sc(R, K), nullptr);
+}
 
 bool exact_mfma_supported(uint32_t d, uint32_t K) {
     const uint32_t mc = d >> 4;
     return d <= 16 * XM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && K <= 128;
 }
 
-template <int MC, bool IPM>
+template <int MC, bool IPM, int RG>
 static hipError_t launch_exact_mfma_t(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
                                       uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
                                       uint32_t* ids, float* keys, hipStream_t s) {
-    size_t lds = exact_mfma_lds(K);
-    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC, IPM>, lds); e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_exact_mfma<MC, IPM>), dim3((rows + XM_R - 1) / XM_R), dim3(256), lds, s, x, xb, nrm, marg,
-                       n, d, K, row0, rows, ids, keys);
+    constexpr uint32_t R = 32 * RG;
+    const uint32_t sc = exact_mfma_sc(R, K);
+    if (sc == 0) return hipErrorInvalidValue;
+    size_t lds = xm_carve(nullptr, R, K, sc, nullptr);
+    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC, IPM, RG>, lds); e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_exact_mfma<MC, IPM, RG>), dim3((rows + R - 1) / R), dim3(128 * RG), lds, s, x, xb, nrm,
+                       marg, n, d, K, sc, row0, rows, ids, keys);
     return hipGetLastError();
 }
 
-template <bool IPM>
+template <bool IPM, int RG>
 static hipError_t launch_exact_mfma_m(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
                                       uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
                                       uint32_t* ids, float* keys, hipStream_t s) {
     switch (d >> 4) {
-        case 8: return launch_exact_mfma_t<8, IPM>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-        case 7: return launch_exact_mfma_t<7, IPM>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-        default: return launch_exact_mfma_t<6, IPM>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        case 8: return launch_exact_mfma_t<8, IPM, RG>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        case 7: return launch_exact_mfma_t<7, IPM, RG>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        default: return launch_exact_mfma_t<6, IPM, RG>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
     }
 }
 
@@ -332,8 +377,11 @@ hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nr
                              float* keys, hipStream_t s) {
     if (rows == 0) return hipSuccess;
     if (!exact_mfma_supported(d, K)) return hipErrorInvalidValue;
-    return ipm ? launch_exact_mfma_m<true>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
-               : launch_exact_mfma_m<false>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+    if (exact_mfma_rg(K) == 4)
+        return ipm ? launch_exact_mfma_m<true, 4>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
+                   : launch_exact_mfma_m<false, 4>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+    return ipm ? launch_exact_mfma_m<true, 2>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
+               : launch_exact_mfma_m<false, 2>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
 }
 
 hipError_t launch_exact_norms2(const float* x, const float* nrm, uint32_t n, uint32_t d, float* nx, float* ex,
