@@ -45,8 +45,8 @@ typedef float xm_f32x16 __attribute__((ext_vector_type(16)));
 struct XmLds {
     uint16_t* btile;   // XM_C x XM_BS bf16
     float* bnc;        // XM_C column norms
-    float* ld;         // R x K
-    uint32_t* li;      // R x K
+    float* ld;         // R x K (or, with the lists in global memory, waves x K)
+    uint32_t* li;      // R x K (or waves x K)
     uint32_t* surv;    // R x sc column ids (survivors since the last merge)
     uint32_t* scnt;    // R
     uint32_t* flag;    // set when some row's buffer could not take another tile
@@ -62,15 +62,17 @@ struct XmLds {
 
 __host__ __device__ inline size_t xm_r16(size_t b) { return (b + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t xm_carve(char* base, uint32_t R, uint32_t K, uint32_t sc, XmLds* s) {
+// gl: the rows' lists live in the output arrays (global memory) and a merge
+// stages its row's list in a per-wavefront LDS copy (ld / li: waves x K).
+__host__ __device__ inline size_t xm_carve(char* base, uint32_t R, uint32_t K, uint32_t sc, bool gl, XmLds* s) {
     const uint32_t NW = R / 16;
     size_t o = 0;
     auto take = [&](size_t bytes) { char* p = base ? base + o : nullptr; o += xm_r16(bytes); return p; };
     char* p;
     p = take(2ull * XM_C * XM_BS); if (s) s->btile = (uint16_t*)p;
     p = take(4ull * XM_C); if (s) s->bnc = (float*)p;
-    p = take(4ull * R * K); if (s) s->ld = (float*)p;
-    p = take(4ull * R * K); if (s) s->li = (uint32_t*)p;
+    p = take(4ull * (gl ? NW : R) * K); if (s) s->ld = (float*)p;
+    p = take(4ull * (gl ? NW : R) * K); if (s) s->li = (uint32_t*)p;
     p = take(4ull * R * sc); if (s) s->surv = (uint32_t*)p;
     p = take(4ull * R); if (s) s->scnt = (uint32_t*)p;
     p = take(16); if (s) s->flag = (uint32_t*)p;
@@ -93,18 +95,21 @@ __host__ __device__ inline size_t xm_carve(char* base, uint32_t R, uint32_t K, u
     } while (0)
 
 // Merges the wave's sorted new pairs (key per lane, bc of them, ascending,
-// all below the row's K-th pair) into row r's sorted list.
+// all below the row's K-th pair) into row r's sorted list: in LDS, or (gk /
+// gi non-null) in global memory, staged through the wavefront's LDS copy.
 __device__ __forceinline__ void xm_merge(XmLds& S, int w, int r, uint32_t K, uint64_t key, int bc,
-                                         float mrow) {
+                                         float mrow, float* gk, uint32_t* gi) {
     const int lane = __lane_id();
     uint64_t* csw = S.cs + w * 64;
     csw[lane] = key;
+    const uint32_t size = S.lcnt[r];
+    float* rd = S.ld + (size_t)(gk ? w : r) * K;
+    uint32_t* ri = S.li + (size_t)(gk ? w : r) * K;
+    if (gk)
+        for (uint32_t i = lane; i < size; i += 64) { rd[i] = gk[i]; ri[i] = gi[i]; }
     XM_WSYNC();
-    float* rd = S.ld + (size_t)r * K;
-    uint32_t* ri = S.li + (size_t)r * K;
     float* od = S.sd + (size_t)w * K;
     uint32_t* oi = S.si + (size_t)w * K;
-    const uint32_t size = S.lcnt[r];
     for (uint32_t i = lane; i < size; i += 64) {
         uint64_t bk = pkey(rd[i], ri[i]);
         int lo = 0, hi = bc;
@@ -124,6 +129,10 @@ __device__ __forceinline__ void xm_merge(XmLds& S, int w, int r, uint32_t K, uin
     XM_WSYNC();
     const uint32_t ns = min(K, size + (uint32_t)bc);
     for (uint32_t i = lane; i < ns; i += 64) { rd[i] = od[i]; ri[i] = oi[i]; }
+    if (gk) {
+        for (uint32_t i = lane; i < ns; i += 64) { gk[i] = od[i]; gi[i] = oi[i]; }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the next merge of the row reads them back
+    }
     XM_WSYNC();
     if (lane == 0) {
         S.lcnt[r] = ns;
@@ -137,7 +146,8 @@ __device__ __forceinline__ void xm_merge(XmLds& S, int w, int r, uint32_t K, uin
 
 // IPM: the inner-product key -<x_i, x_j> (dataset_gt.cpp:225); approx = -ip_bf16,
 // and M_i (twice a bound on the inner product's error) still covers it.
-template <int MC, bool IPM, int RG>
+// GL: the rows' lists in the output arrays (large K at 128-row blocks).
+template <int MC, bool IPM, int RG, bool GL>
 __global__ void __launch_bounds__(128 * RG) k_exact_mfma(const float* __restrict__ x,
                                                          const uint16_t* __restrict__ xb,
                                                          const float* __restrict__ nrm,
@@ -148,7 +158,7 @@ __global__ void __launch_bounds__(128 * RG) k_exact_mfma(const float* __restrict
     constexpr int R = 32 * RG, NT = 128 * RG, NV = XM_TV / NT;
     extern __shared__ __align__(16) char lds[];
     XmLds S;
-    xm_carve(lds, R, K, sc, &S);
+    xm_carve(lds, R, K, sc, GL, &S);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l = lane & 15, g = lane >> 4;
     const uint32_t rb = row0 + blockIdx.x * R;
     const uint32_t rend = min(row0 + rows, rb + R);
@@ -270,7 +280,11 @@ __global__ void __launch_bounds__(128 * RG) k_exact_mfma(const float* __restrict
                 }
                 key = wave_sort64(key);
                 const int bc = popc64(ballot(key != ~0ull));
-                if (bc) xm_merge(S, w, r, K, key, bc, mrow);
+                if (bc) {
+                    float* gk = GL ? out_keys + (size_t)(row - row0) * K : nullptr;
+                    uint32_t* gi = GL ? out_ids + (size_t)(row - row0) * K : nullptr;
+                    xm_merge(S, w, r, K, key, bc, mrow, gk, gi);
+                }
             }
             if (lane == 0) S.scnt[r] = 0;
         }
@@ -281,8 +295,15 @@ __global__ void __launch_bounds__(128 * RG) k_exact_mfma(const float* __restrict
         uint32_t row = rb + r;
         if (row >= rend) continue;
         for (uint32_t i = lane; i < K; i += 64) {
-            out_ids[(size_t)(row - row0) * K + i] = i < S.lcnt[r] ? S.li[(size_t)r * K + i] : U32_NONE;
-            out_keys[(size_t)(row - row0) * K + i] = i < S.lcnt[r] ? S.ld[(size_t)r * K + i] : F32_BIG;
+            if (GL) {
+                if (i >= S.lcnt[r]) {
+                    out_ids[(size_t)(row - row0) * K + i] = U32_NONE;
+                    out_keys[(size_t)(row - row0) * K + i] = F32_BIG;
+                }
+            } else {
+                out_ids[(size_t)(row - row0) * K + i] = i < S.lcnt[r] ? S.li[(size_t)r * K + i] : U32_NONE;
+                out_keys[(size_t)(row - row0) * K + i] = i < S.lcnt[r] ? S.ld[(size_t)r * K + i] : F32_BIG;
+            }
         }
     }
 }
@@ -321,25 +342,34 @@ __global__ void k_exact_margin(const float* __restrict__ nx, const float* __rest
     marg[i] = m * 1.01f + 1e-30f + 2.0f * (float)d * 1.2e-38f;
 }
 
-// Survivor buffer entries per row for R-row blocks: what the LDS left by the
-// rest holds (a multiple of 64, at most 1024), or 0 when that is less than
-// one tile plus 64.  K <= 128 always fits 64-row blocks (159,504 bytes at
-// K = 128 with 192 entries).
-static uint32_t exact_mfma_sc(uint32_t R, uint32_t K) {
+// Survivor buffer entries per row for R-row blocks (lists in LDS, or in
+// global memory with gl): what the LDS left by the rest holds (a multiple of
+// 64, at most 1024), or 0 when that is less than one tile plus 64.
+static uint32_t exact_mfma_sc(uint32_t R, uint32_t K, bool gl) {
     const size_t budget = 160 * 1024 - 2048;
-    const size_t rest = xm_carve(nullptr, R, K, 0, nullptr);
+    const size_t rest = xm_carve(nullptr, R, K, 0, gl, nullptr);
     size_t sc = rest < budget ? (budget - rest) / (4 * (size_t)R) : 0;
     sc = std::min<size_t>(1024, sc) & ~size_t(63);
     return sc >= XM_C + 64 ? (uint32_t)sc : 0u;
 }
 
-// 128-row blocks (two wavefronts per SIMD) when their lists and a survivor
-// buffer fit (small K, e.g. K <= 12), else 64-row blocks.
-static int exact_mfma_rg(uint32_t K) { return exact_mfma_sc(128, K) ? 4 : 2; }
+// 128-row blocks (two wavefronts per SIMD): lists in LDS when they fit with
+// a survivor buffer (small K), else in global memory (K <= 128 fits:
+// 192 buffer entries per row); 64-row blocks with LDS lists otherwise.
+struct XmShape {
+    int rg;
+    bool gl;
+    uint32_t sc;
+};
+static XmShape exact_mfma_shape(uint32_t K) {
+    if (uint32_t sc = exact_mfma_sc(128, K, false)) return {4, false, sc};
+    if (uint32_t sc = exact_mfma_sc(128, K, true)) return {4, true, sc};
+    return {2, false, exact_mfma_sc(64, K, false)};
+}
 
 size_t exact_mfma_lds(uint32_t K) {
-    const uint32_t R = 32u * (uint32_t)exact_mfma_rg(K);
-    return xm_carve(nullptr, R, K, exact_mfma_sc(R, K), nullptr);
+    const XmShape h = exact_mfma_shape(K);
+    return xm_carve(nullptr, 32u * (uint32_t)h.rg, K, h.sc, h.gl, nullptr);
 }
 
 bool exact_mfma_supported(uint32_t d, uint32_t K) {
@@ -347,29 +377,39 @@ bool exact_mfma_supported(uint32_t d, uint32_t K) {
     return d <= 16 * XM_KS && d % 8 == 0 && mc >= 6 && mc <= 8 && K <= 128;
 }
 
-template <int MC, bool IPM, int RG>
+template <int MC, bool IPM, int RG, bool GL>
 static hipError_t launch_exact_mfma_t(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
-                                      uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
+                                      uint32_t n, uint32_t d, uint32_t K, uint32_t sc, uint32_t row0, uint32_t rows,
                                       uint32_t* ids, float* keys, hipStream_t s) {
     constexpr uint32_t R = 32 * RG;
-    const uint32_t sc = exact_mfma_sc(R, K);
     if (sc == 0) return hipErrorInvalidValue;
-    size_t lds = xm_carve(nullptr, R, K, sc, nullptr);
-    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC, IPM, RG>, lds); e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_exact_mfma<MC, IPM, RG>), dim3((rows + R - 1) / R), dim3(128 * RG), lds, s, x, xb, nrm,
-                       marg, n, d, K, sc, row0, rows, ids, keys);
+    size_t lds = xm_carve(nullptr, R, K, sc, GL, nullptr);
+    if (hipError_t e = set_lds_limit((const void*)k_exact_mfma<MC, IPM, RG, GL>, lds); e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_exact_mfma<MC, IPM, RG, GL>), dim3((rows + R - 1) / R), dim3(128 * RG), lds, s, x, xb,
+                       nrm, marg, n, d, K, sc, row0, rows, ids, keys);
     return hipGetLastError();
 }
 
-template <bool IPM, int RG>
+template <bool IPM, int RG, bool GL>
 static hipError_t launch_exact_mfma_m(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
-                                      uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
+                                      uint32_t n, uint32_t d, uint32_t K, uint32_t sc, uint32_t row0, uint32_t rows,
                                       uint32_t* ids, float* keys, hipStream_t s) {
     switch (d >> 4) {
-        case 8: return launch_exact_mfma_t<8, IPM, RG>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-        case 7: return launch_exact_mfma_t<7, IPM, RG>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-        default: return launch_exact_mfma_t<6, IPM, RG>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+        case 8: return launch_exact_mfma_t<8, IPM, RG, GL>(x, xb, nrm, marg, n, d, K, sc, row0, rows, ids, keys, s);
+        case 7: return launch_exact_mfma_t<7, IPM, RG, GL>(x, xb, nrm, marg, n, d, K, sc, row0, rows, ids, keys, s);
+        default: return launch_exact_mfma_t<6, IPM, RG, GL>(x, xb, nrm, marg, n, d, K, sc, row0, rows, ids, keys, s);
     }
+}
+
+template <bool IPM>
+static hipError_t launch_exact_mfma_i(const float* x, const uint16_t* xb, const float* nrm, const float* marg,
+                                      uint32_t n, uint32_t d, uint32_t K, uint32_t row0, uint32_t rows,
+                                      uint32_t* ids, float* keys, hipStream_t s) {
+    const XmShape h = exact_mfma_shape(K);
+    if (h.rg == 4 && h.gl)
+        return launch_exact_mfma_m<IPM, 4, true>(x, xb, nrm, marg, n, d, K, h.sc, row0, rows, ids, keys, s);
+    if (h.rg == 4) return launch_exact_mfma_m<IPM, 4, false>(x, xb, nrm, marg, n, d, K, h.sc, row0, rows, ids, keys, s);
+    return launch_exact_mfma_m<IPM, 2, false>(x, xb, nrm, marg, n, d, K, h.sc, row0, rows, ids, keys, s);
 }
 
 hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nrm, const float* marg, uint32_t n,
@@ -377,11 +417,8 @@ hipError_t launch_exact_mfma(const float* x, const uint16_t* xb, const float* nr
                              float* keys, hipStream_t s) {
     if (rows == 0) return hipSuccess;
     if (!exact_mfma_supported(d, K)) return hipErrorInvalidValue;
-    if (exact_mfma_rg(K) == 4)
-        return ipm ? launch_exact_mfma_m<true, 4>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
-                   : launch_exact_mfma_m<false, 4>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
-    return ipm ? launch_exact_mfma_m<true, 2>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
-               : launch_exact_mfma_m<false, 2>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
+    return ipm ? launch_exact_mfma_i<true>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s)
+               : launch_exact_mfma_i<false>(x, xb, nrm, marg, n, d, K, row0, rows, ids, keys, s);
 }
 
 hipError_t launch_exact_norms2(const float* x, const float* nrm, uint32_t n, uint32_t d, float* nx, float* ex,
