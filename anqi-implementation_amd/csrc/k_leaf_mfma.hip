@@ -131,6 +131,160 @@ __global__ void __launch_bounds__(256) k_leaf_sweep(LeafParams P, const void* __
     }
 }
 
+// The same pass with the leaf staged in LDS (when it fits): the members'
+// rows (bf16, or e4m3 bytes), ids, norms and filter bounds are copied once per
+// leaf, so every A and B fragment of the sweep is an LDS read instead of a
+// gather through the permutation.  Rows are padded to 16 LM_KS + 8 elements
+// (the pad spreads the fragment reads over the banks); elements past d are
+// zero.  Same survivors as k_leaf_sweep (the merge pass does not depend on
+// their order).
+#define LM_RS_EL (16 * LM_KS + 8)
+template <bool F8>
+__host__ __device__ constexpr uint32_t lm_row_bytes() { return LM_RS_EL * (F8 ? 1u : 2u); }
+template <bool F8>
+__host__ __device__ inline size_t lm_stage_lds(uint32_t max_leaf) {
+    return (size_t)max_leaf * lm_row_bytes<F8>() + 12ull * max_leaf;
+}
+
+template <bool F8>
+__device__ __forceinline__ void lm_frag_lds(LmFrag<F8>& f, const char* rows, uint32_t a, uint32_t k0, bool ok) {
+    if constexpr (F8) {
+        f.v = ok ? *(const long*)(rows + (size_t)a * lm_row_bytes<true>() + k0) : 0l;
+    } else {
+        uint4 z = make_uint4(0, 0, 0, 0);
+        if (ok) z = *(const uint4*)(rows + (size_t)a * lm_row_bytes<false>() + 2 * k0);
+        f.v = __builtin_bit_cast(lm_bf16x8, z);
+    }
+}
+
+// 512 threads: two wavefronts per SIMD (one block fills a CU's LDS), so one
+// wavefront's MFMA chain overlaps another's filter arithmetic.
+#define LM_STAGE_THREADS 512
+template <int MC, bool F8>
+__global__ void __launch_bounds__(LM_STAGE_THREADS) k_leaf_sweep_lds(LeafParams P, const void* __restrict__ xq,
+                                                                     float s2,
+                                                        const float* __restrict__ marg, uint32_t* __restrict__ cbuf,
+                                                        uint32_t* __restrict__ ccnt, uint32_t capc) {
+    extern __shared__ __align__(16) char lds[];
+    const uint32_t leaf = blockIdx.x;
+    if (leaf >= P.nleaves) return;
+    constexpr uint32_t ES = F8 ? 1u : 2u, RB = lm_row_bytes<F8>();
+    constexpr uint32_t D = 16 * MC;                    // P.d
+    const uint32_t m = P.leaf_size[leaf], K = P.K, ML = P.max_leaf;
+    char* rows = lds;
+    float* mnrm = (float*)(lds + (size_t)ML * RB);
+    float* mthr = mnrm + ML;
+    uint32_t* mid = (uint32_t*)(mthr + ML);
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t* mem = P.perm + P.leaf_start[leaf];
+    // stage the members: rows (16-byte vectors), then the zero tail past d
+    constexpr uint32_t VPR = D * ES / 16;
+    // eight 16-byte loads in flight per thread before their LDS stores
+    constexpr uint32_t U = 8;
+    for (uint32_t e0 = 0; e0 < m * VPR; e0 += U * LM_STAGE_THREADS) {
+        uint4 z[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t e = e0 + u * LM_STAGE_THREADS + (uint32_t)tid;
+            z[u] = make_uint4(0, 0, 0, 0);
+            if (e < m * VPR) z[u] = *(const uint4*)((const char*)xq + (size_t)mem[e / VPR] * D * ES + 16u * (e % VPR));
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t e = e0 + u * LM_STAGE_THREADS + (uint32_t)tid;
+            if (e < m * VPR) *(uint4*)(rows + (size_t)(e / VPR) * RB + 16u * (e % VPR)) = z[u];
+        }
+    }
+    if constexpr (D < 16 * LM_KS) {
+        constexpr uint32_t TV = (16 * LM_KS - D) * ES / 16;     // 16-byte vectors of the zero tail
+        for (uint32_t e = (uint32_t)tid; e < m * TV; e += LM_STAGE_THREADS) {
+            const uint32_t a = e / TV, kv = e % TV;
+            *(uint4*)(rows + (size_t)a * RB + D * ES + 16u * kv) = make_uint4(0, 0, 0, 0);
+        }
+    }
+    for (uint32_t a = (uint32_t)tid; a < m; a += LM_STAGE_THREADS) {
+        const uint32_t id = mem[a];
+        mid[a] = id;
+        mnrm[a] = P.nrm[id];
+        float t = __builtin_huge_valf();
+        if (P.cnt[id] >= K) t = P.keys[(size_t)id * K + K - 1] + marg[id];
+        mthr[a] = t;
+    }
+    __syncthreads();
+    const uint32_t nrb = (m + 31) / 32;
+    uint32_t nsv = 0;   // pairs this lane let through the filter
+    for (uint32_t rb = (uint32_t)w; rb < nrb; rb += LM_STAGE_THREADS / 64) {
+        const uint32_t am = rb * 32 + (lane & 31);
+        LmFrag<F8> afr[LM_KS];
+#pragma unroll
+        for (int s = 0; s < LM_KS; s++) lm_frag_lds<F8>(afr[s], rows, am, 16 * s + 8 * (lane >> 5), am < m);
+        uint32_t rid[16];
+        float bnr[16], thrm[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const uint32_t a = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            rid[v] = a < m ? mid[a] : U32_NONE;
+            bnr[v] = a < m ? mnrm[a] : 0.0f;
+            thrm[v] = a < m ? mthr[a] : -__builtin_huge_valf();
+        }
+        // survivor slots without atomics: accumulator element v of lane l
+        // belongs to row 8 (v / 4) + 4 (l / 32) + v % 4, the same row for the
+        // 32 lanes of a half-wavefront, so a ballot gives each passing lane
+        // its slot and cnt[v] (uniform over the half) the row's count
+        uint32_t cnt[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) cnt[v] = 0;
+        const uint64_t half = (lane < 32) ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
+        const uint64_t below = (1ull << lane) - 1ull;
+        auto filter = [&](uint32_t ct, const lm_f32x16& acc) __attribute__((always_inline)) {
+            const uint32_t cm = ct * 32 + (lane & 31);
+            const bool cok = cm < m;
+            const float bc = cok ? mnrm[cm] : 0.0f;
+            const uint32_t crow = cok ? mid[cm] : U32_NONE;
+            const bool diag = ct == rb;   // the only tile holding self pairs
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const uint32_t a = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const float approx = fmaf(-2.0f, F8 ? acc[v] * s2 : acc[v], bnr[v] + bc);
+                const bool pass = cok && (!diag || cm != a) && approx <= thrm[v];   // thrm = -inf past m
+                const uint64_t pm = ballot(pass) & half;
+                if (pass) {
+                    cbuf[(size_t)rid[v] * capc + cnt[v] + (uint32_t)popc64(pm & below)] = crow;
+                    nsv++;
+                }
+                cnt[v] += (uint32_t)popc64(pm);
+            }
+        };
+        // column tiles two at a time, their MFMA chains interleaved
+        for (uint32_t ct = 0; ct < nrb; ct += 2) {
+            const uint32_t cm0 = ct * 32 + (lane & 31), cm1 = cm0 + 32;
+            lm_f32x16 acc0, acc1;
+#pragma unroll
+            for (int v = 0; v < 16; v++) { acc0[v] = 0.0f; acc1[v] = 0.0f; }
+#pragma unroll
+            for (int s = 0; s < LM_KS; s++) {
+                LmFrag<F8> b0, b1;
+                lm_frag_lds<F8>(b0, rows, cm0, 16 * s + 8 * (lane >> 5), cm0 < m);
+                lm_frag_lds<F8>(b1, rows, cm1, 16 * s + 8 * (lane >> 5), cm1 < m);
+                acc0 = lm_mma(afr[s], b0, acc0);
+                acc1 = lm_mma(afr[s], b1, acc1);
+            }
+            filter(ct, acc0);
+            if (ct + 1 < nrb) filter(ct + 1, acc1);
+        }
+        // row 8 (v / 4) + 4 h + v % 4 of the block: lane 32 h writes cnt[v]
+        if ((lane & 31) == 0)
+#pragma unroll
+            for (int v = 0; v < 16; v++)
+                if (rid[v] != U32_NONE) ccnt[rid[v]] = cnt[v];
+    }
+    if (P.nsurv) {
+        for (int o = 32; o > 0; o >>= 1) nsv += __shfl_xor(nsv, o);
+        if (lane == 0 && nsv) atomicAdd(P.nsurv, (unsigned long long)nsv);
+    }
+}
+
 // Drops the lanes' candidate keys that equal a key of the list.
 template <int NCH>
 __device__ __forceinline__ uint64_t rk_drop_listed(const RKeys<NCH>& B, uint64_t key) {
@@ -203,6 +357,12 @@ bool leaf_mfma_supported(uint32_t d, uint32_t max_leaf, uint32_t K) {
 template <int MC, bool F8>
 static void leaf_sweep_t(const LeafParams& P, const void* xq, float s2, const float* marg, uint32_t* cbuf,
                          uint32_t* ccnt, uint32_t capc, hipStream_t s) {
+    const size_t lds = lm_stage_lds<F8>(P.max_leaf);
+    if (lds <= 160 * 1024 && set_lds_limit((const void*)k_leaf_sweep_lds<MC, F8>, lds) == hipSuccess) {
+        hipLaunchKernelGGL((k_leaf_sweep_lds<MC, F8>), dim3(P.nleaves), dim3(LM_STAGE_THREADS), lds, s, P, xq, s2,
+                           marg, cbuf, ccnt, capc);
+        return;
+    }
     hipLaunchKernelGGL((k_leaf_sweep<MC, F8>), dim3(P.nleaves), dim3(256), 0, s, P, xq, s2, marg, cbuf, ccnt, capc);
 }
 

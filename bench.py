@@ -289,6 +289,27 @@ def stage_build(ctx, a):
     return out
 
 
+def stage_exact(ctx, a):
+    """The exact self-excluded top-K of the bench base (tools/dataset_gt.cpp:
+    206-245: the bf16 MFMA sweep with FP32 re-scoring, precision 1) for
+    K = --exact-k, after a 20,000-row warm-up; host time around the
+    synchronised call and the algorithmic work 2 d n (n - 1) flop."""
+    import torch
+    import workload
+    base = workload.million_base(a.n)
+    xd = ctx.to_dev(base)
+    ctx.exact_knn(xd[:20000].contiguous(), a.exact_k, precision=1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ctx.exact_knn(xd, a.exact_k, precision=1)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t
+    n, d = base.shape
+    flop = 2.0 * d * n * (n - 1)
+    return {"n": n, "d": d, "K": a.exact_k, "seconds": secs, "flop": flop, "TFLOPs": flop / secs / 1e12,
+            "frac_bf16_peak": flop / secs / 2.5e15}
+
+
 def stage_lrq(ctx, a):
     """Verifier kernels on the 1M x 128 mixture's AKNN cutoffs: encode
     (coefficients + codes) and the threshold decode for k = 10 and for
@@ -379,7 +400,8 @@ def main():
     ap.add_argument("--cpu-build-n", type=int, default=100_000,
                     help="rows of the bench base the reference's own build is timed on")
     ap.add_argument("--detail", default="")
-    ap.add_argument("--stage", choices=["lrq", "build"], default="",
+    ap.add_argument("--exact-k", type=int, default=100, help="K of --stage exact")
+    ap.add_argument("--stage", choices=["lrq", "build", "exact"], default="",
                     help="time one stage's kernels instead of the headline measurement (prints its own JSON)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 1 GiB device-to-device copy (counter calibration)")
@@ -398,7 +420,8 @@ def main():
         raise SystemExit("bench.py needs a HIP device")
     ctx = pyanqi.Context(0)
     if a.stage:
-        print(json.dumps({"lrq": stage_lrq, "build": stage_build}[a.stage](ctx, a)), flush=True)
+        print(json.dumps({"lrq": stage_lrq, "build": stage_build, "exact": stage_exact}[a.stage](ctx, a)),
+              flush=True)
         return
     t = time.perf_counter()
     base = workload.million_base(a.n)
