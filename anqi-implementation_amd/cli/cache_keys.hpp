@@ -60,6 +60,13 @@ struct CacheNames {
     std::string approxlift, meta, state, marker, graph_index, verifier;
 };
 
+// The base vectors' file: ANQI_BASE_U8BIN when set, else the fbin
+// (base_vector_source, rknn_query.cpp:109-112).
+inline std::string base_vector_source(const std::string& fbin_path) {
+    const char* u8 = std::getenv("ANQI_BASE_U8BIN");
+    return (u8 && *u8) ? std::string(u8) : fbin_path;
+}
+
 // Identity of the AKNN cutoffs (threshold_source_identity, rknn_query.cpp:
 // 1059-1063).  Without a prebuilt .rec the reference reads the cutoffs from
 // PFX_baseknn_aknn.bin; this tool builds them on the GPU from the base and
@@ -67,7 +74,7 @@ struct CacheNames {
 // a reference run is taken for one of these runs or the other way round.
 inline std::string threshold_source_identity(const CacheInputs& in) {
     if (in.rec.empty())
-        return file_stat_identity(in.pfx + "_base.bin") + ":gpu_aknn:leaf" + std::to_string(in.leaf) + ":nt" +
+        return file_stat_identity(base_vector_source(in.pfx + "_base.bin")) + ":gpu_aknn:leaf" + std::to_string(in.leaf) + ":nt" +
                std::to_string(in.trees) + ":ef" + std::to_string(in.ef) + ":k100";
     return file_stat_identity(in.rec) + ":interleaved_k" + env_or("ANQI_PREBUILT_TOPK_K", "100");
 }
@@ -107,7 +114,8 @@ inline CacheNames cache_names(const CacheInputs& in) {
     // of the radius key
     const std::string radius_id = env_set("ANQI_GRAPH_RADIUS_ID")
                                       ? std::string(env_or("ANQI_GRAPH_RADIUS_ID", "aknn_k100_v1"))
-                                      : file_stat_identity(in.pfx + "_base.bin") + "|" + c.threshold_source + ":k100:v2";
+                                      : file_stat_identity(base_vector_source(in.pfx + "_base.bin")) + "|" +
+                                            c.threshold_source + ":k100:v2";
     c.graph_key = sanitize_tag(g + "_graph_cutoff_k100_" + hash16(radius_id));
     c.state_key = "graph=" + c.graph_key;
     const std::string tag = hash16(c.graph_key);

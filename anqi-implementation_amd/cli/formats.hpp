@@ -53,6 +53,23 @@ inline bool read_fbin(const std::string& p, std::vector<float>& x, uint32_t& n, 
     return (bool)f.read((char*)x.data(), values * 4);
 }
 
+// uint8 base file [u32 n][u32 d][n*d u8] read as floats (load_u8bin_as_float,
+// src/rknn_query.cpp:55-107): 0 ok, 1 cannot open or read the header, 2 bad
+// size; the values are float(u8).
+inline int read_u8bin_as_float(const std::string& p, std::vector<float>& x, uint32_t& n, uint32_t& d) {
+    std::ifstream f(p, std::ios::binary);
+    if (!f) return 1;
+    if (!f.read((char*)&n, 4) || !f.read((char*)&d, 4)) return 1;
+    const uint64_t values = (uint64_t)n * d;
+    if (n == 0 || d == 0 || file_size(p) != 8 + values) return 2;
+    std::vector<uint8_t> b(values);
+    if (!f.read((char*)b.data(), (std::streamsize)values)) return 1;
+    x.resize(values);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)values; i++) x[(size_t)i] = (float)b[(size_t)i];
+    return 0;
+}
+
 inline bool write_fbin(const std::string& p, const float* x, uint32_t n, uint32_t d) {
     std::ofstream f(p, std::ios::binary | std::ios::trunc);
     f.write((const char*)&n, 4);

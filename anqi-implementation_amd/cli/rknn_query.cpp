@@ -24,8 +24,12 @@
 // LRQ encode + threshold decode, lift, graph build, search tree, range search
 // and post-filter.  Recall/precision/F1 and distance counts are computed on
 // the host exactly as the reference does (rknn_query.cpp:2776-2798, 2891-2913).
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstring>
 #include <cmath>
 #include <iomanip>
 #include <map>
@@ -120,9 +124,83 @@ const std::vector<Pin>& pins() {
 }
 // Switched on by their presence alone, whatever the value (the reference
 // tests them with getenv alone), or by any non-empty value.
-const char* const kPresence[] = {"ANQI_VAMANA", "ANQI_REUSE_ORIG_KNN", "ANQI_EDGE_RESID_FILE", "ANQI_BASE_U8BIN",
+const char* const kPresence[] = {"ANQI_VAMANA", "ANQI_REUSE_ORIG_KNN", "ANQI_EDGE_RESID_FILE",
                                  "ANQI_QUERY_SINGLE_ENTRY", "ANQI_VAMANA_QUERY_ENTRY"};
-const char* const kNonEmpty[] = {"ANQI_GRAPH_RADIUS_SOURCE_PFX", "ANQI_PREBUILT_WARM_START_COMMAND"};
+const char* const kNonEmpty[] = {"ANQI_GRAPH_RADIUS_SOURCE_PFX"};
+
+// Base vectors: the fbin, or ANQI_BASE_U8BIN's uint8 file read as floats
+// (load_base_vectors, rknn_query.cpp:55-123).  Exits 3 on a bad file.
+void load_base(const std::string& PFX, std::vector<float>& base, uint32_t& n, uint32_t& d) {
+    const char* u8 = std::getenv("ANQI_BASE_U8BIN");
+    if (u8 && *u8) {
+        const int rc = read_u8bin_as_float(u8, base, n, d);
+        if (rc == 1) {
+            std::fprintf(stderr, "[rknn] cannot read uint8 base header %s\n", u8);
+            std::exit(3);
+        }
+        if (rc == 2) {
+            std::fprintf(stderr, "[rknn] bad uint8 base %s: n=%u d=%u bytes=%lld expected=%llu\n", u8, n, d,
+                         (long long)file_size(u8), (unsigned long long)(8 + (uint64_t)n * d));
+            std::exit(3);
+        }
+        std::printf("[rknn] loaded uint8 base directly as float: %s n=%u d=%u %.3fGB\n", u8, n, d,
+                    (double)n * d * 4 / 1e9);
+        return;
+    }
+    if (!read_fbin(PFX + "_base.bin", base, n, d)) {
+        std::fprintf(stderr, "[rknn] cannot read base vectors %s_base.bin\n", PFX.c_str());
+        std::exit(3);
+    }
+}
+
+// Warm-start ids from a stream of n rows of `stride` records -- {u32 id, f32
+// distance}, or u32 ids alone (ids_only) -- keeping the first `width` ids of
+// each row (nndindex::build_index's streamed warm start, include/index.h:
+// 1127-1203): from a command's stdout, or from a file.  Returns false when
+// the stream ends early, has extra data or the command fails.
+bool read_warm_stream(FILE* in, bool ids_only, uint32_t n, uint32_t stride, uint32_t width,
+                      std::vector<uint32_t>& ids) {
+    ids.assign((size_t)n * width, 0u);
+    const size_t rec = ids_only ? 4 : 8;
+    std::vector<char> row((size_t)stride * rec);
+    for (uint32_t i = 0; i < n; i++) {
+        if (std::fread(row.data(), rec, stride, in) != stride) return false;
+        for (uint32_t k = 0; k < width; k++) std::memcpy(&ids[(size_t)i * width + k], row.data() + k * rec, 4);
+    }
+    return std::fgetc(in) == EOF;
+}
+
+// ANQI_COMPACT_PREBUILT_TOPK_IDS_INPLACE: rewrites the .rec in place as its
+// rows' first `width` ids (u32) and truncates it
+// (compact_prebuilt_topk_ids_in_place, rknn_query.cpp:988-1057).
+bool compact_rec_in_place(const std::string& path, uint32_t n, uint32_t source_k, uint32_t width) {
+    std::FILE* f = std::fopen(path.c_str(), "r+b");
+    if (!f) return false;
+    std::vector<Rec> r(source_k);
+    std::vector<uint32_t> ids(width);
+    for (uint32_t i = 0; i < n; i++) {   // row i's ids land at or before its records: read before written
+        if (std::fseek(f, (long)((uint64_t)i * source_k * sizeof(Rec)), SEEK_SET) ||
+            std::fread(r.data(), sizeof(Rec), source_k, f) != source_k) {
+            std::fclose(f);
+            return false;
+        }
+        for (uint32_t k = 0; k < width; k++) {
+            if (r[k].id >= n) {
+                std::fprintf(stderr, "[rknn] invalid warm-start id at row=%u k=%u\n", i, k);
+                std::exit(3);
+            }
+            ids[k] = r[k].id;
+        }
+        if (std::fseek(f, (long)((uint64_t)i * width * 4), SEEK_SET) ||
+            std::fwrite(ids.data(), 4, width, f) != width) {
+            std::fclose(f);
+            return false;
+        }
+    }
+    const bool ok = std::fflush(f) == 0 && ::ftruncate(fileno(f), (off_t)((uint64_t)n * width * 4)) == 0 &&
+                    ::fsync(fileno(f)) == 0;
+    return std::fclose(f) == 0 && ok;
+}
 
 void check_configuration() {
     if (std::getenv("ANQI_SPECIFIC")) {   // rknn_query.cpp:1806-1811
@@ -281,10 +359,7 @@ int main(int argc, char** argv) {
 
     std::vector<float> base, query;
     uint32_t n = 0, d = 0, nq = 0, dq = 0;
-    if (!read_fbin(PFX + "_base.bin", base, n, d)) {
-        std::fprintf(stderr, "[rknn] cannot read base vectors %s_base.bin\n", PFX.c_str());
-        return 3;
-    }
+    load_base(PFX, base, n, d);
     if (!read_fbin(PFX + "_query.bin", query, nq, dq)) {
         std::fprintf(stderr, "[rknn] cannot read query vectors %s_query.bin\n", PFX.c_str());
         return 3;
@@ -308,6 +383,19 @@ int main(int argc, char** argv) {
     const std::string verifier_dir = cache_dir("ANQI_VERIFIER_CACHE_DIR", DSDIR);
     const char* rec_env = std::getenv("ANQI_PREBUILT_TOPK_REC");
     const std::string rec_p = (rec_env && *rec_env) ? std::string(rec_env) : std::string();
+    // warm-start id sources for a prebuilt .rec (rknn_query.cpp:2101-2135,
+    // 2170-2190): a command streaming the rows, or the .rec compacted in place
+    // to ids after the preparation and consumed by the warm start; or the .rec
+    // removed once the cutoffs are read
+    const char* stream_env = std::getenv("ANQI_PREBUILT_WARM_START_COMMAND");
+    const std::string warm_cmd = (!rec_p.empty() && stream_env && *stream_env) ? std::string(stream_env) : "";
+    const bool compact_ids = !rec_p.empty() && env_flag("ANQI_COMPACT_PREBUILT_TOPK_IDS_INPLACE");
+    const bool release_rec = !rec_p.empty() && !compact_ids && env_flag("ANQI_RELEASE_PREBUILT_TOPK_AFTER_PREP");
+    if (compact_ids && !warm_cmd.empty()) {
+        std::fprintf(stderr, "[rknn] compact local warm-start IDs cannot be combined with "
+                             "ANQI_PREBUILT_WARM_START_COMMAND\n");
+        return 2;
+    }
     const std::string basis_p = PFX + "_radbasis_M8.bin";
     const char* cbenv = std::getenv("ANQI_RQ_CODEBOOK");
     const std::string cb_p = (cbenv && *cbenv) ? std::string(cbenv) : PFX + "_learned_rq_codebook_M8_u8.bin";
@@ -413,8 +501,15 @@ int main(int argc, char** argv) {
                     std::fprintf(stderr, "[rknn] invalid prebuilt top-k id at row=%zu\n", t / 100);
                     return 3;
                 }
-            std::printf("[rknn] loaded contiguous prebuilt original KNN %s (n=%u source_k=100 width=100)\n",
-                        rec_p.c_str(), n);
+            if (compact_ids)
+                std::printf("[rknn] deferred original KNN IDs to compact local graph warm-start (source_k=100 "
+                            "width=100)\n");
+            else if (!warm_cmd.empty())
+                std::printf("[rknn] deferred original KNN IDs to streamed graph warm-start (source_k=100 "
+                            "width=100)\n");
+            else
+                std::printf("[rknn] loaded contiguous prebuilt original KNN %s (n=%u source_k=100 width=100)\n",
+                            rec_p.c_str(), n);
             d_aknn = upload(aknn_ids);
             d_keys = upload(aknn_keys);
         } else {
@@ -429,7 +524,8 @@ int main(int argc, char** argv) {
             std::printf("[rknn] built original-space AKNN on the GPU (n=%u K=100 in %.2fs)\n", n, st.total_s);
         }
         std::printf("[rknn] original-space AKNN warm start ready: rows=%u source=%s\n", n,
-                    rec_p.empty() ? "gpu" : "prebuilt-contiguous");
+                    rec_p.empty() ? "gpu" : (compact_ids || !warm_cmd.empty() ? "prebuilt-stream"
+                                                                              : "prebuilt-contiguous"));
         if (!rec_p.empty()) {
             std::printf("[rknn] loaded AKNN cutoffs from interleaved top-k %s (n=%u K=%zu)\n", rec_p.c_str(), n, NK);
             std::printf("[rknn] loaded AKNN cutoff column from interleaved top-k %s (n=%u k=%zu)\n", rec_p.c_str(), n,
@@ -457,6 +553,67 @@ int main(int argc, char** argv) {
         verifier_ready = true;
         std::printf("[rknn] built independent compact verifier mode=rank-M+LRQ bits=8 %s (persistent=%.3fGB "
                     "encode_save_ms=%.3f)\n", cn.verifier.c_str(), verifier_bytes / 1e9, encode_save_ms);
+        return 0;
+    };
+
+    // memory released while the graph is loaded or built and read back after
+    // (ANQI_RELEASE_VERIFIER_DURING_GRAPH_INDEX / ANQI_RELEASE_BASE_DURING_GRAPH_INDEX,
+    // rknn_query.cpp:2257-2280, 2303-2320): the verifier payload (host and
+    // device) reloaded from its artifact, the base vectors from their file
+    bool released = false, verifier_released = false, base_released = false;
+    auto release_for_index = [&]() {
+        if (released) return;
+        released = true;
+        if (env_flag("ANQI_RELEASE_VERIFIER_DURING_GRAPH_INDEX") && verifier_ready) {
+            const double bytes = (double)coeff.capacity() * 4 + (double)codes.capacity();
+            std::vector<float>().swap(coeff);
+            std::vector<uint8_t>().swap(codes);
+            anqi_dfree(g_ctx, d_coeff);
+            anqi_dfree(g_ctx, d_codes_km);
+            d_coeff = nullptr;
+            d_codes_km = nullptr;
+            verifier_ready = false;
+            verifier_released = true;
+            std::printf("[rknn] released compact verifier payload during graph index construction: %.3fGB\n",
+                        bytes / 1e9);
+        }
+        if (env_flag("ANQI_RELEASE_BASE_DURING_GRAPH_INDEX")) {
+            std::vector<float>().swap(base);
+            anqi_dfree(g_ctx, d_base);
+            d_base = nullptr;
+            base_released = true;
+            std::printf("[rknn] released original base before graph index construction: %.3fGB\n",
+                        (double)n * d * 4 / 1e9);
+        }
+    };
+    auto reload_after_index = [&]() -> int {
+        if (verifier_released) {
+            if (!read_verifier(cn.verifier, n, NK, RANKM, coeff, codes)) {
+                std::fprintf(stderr, "[rknn] invalid LRQ verifier artifact %s\n", cn.verifier.c_str());
+                return 3;
+            }
+            d_coeff = upload(coeff);
+            d_codes_km = dalloc<uint8_t>((size_t)n * NK);
+            uint8_t* d_codes = upload(codes);
+            ck(anqi_transpose_codes(g_ctx, d_codes, n, NK, d_codes_km), "code transpose");
+            ck(anqi_ctx_sync(g_ctx), "sync");
+            anqi_dfree(g_ctx, d_codes);
+            verifier_ready = true;
+            std::printf("[rknn] reloaded compact verifier after graph index construction: %.3fGB\n",
+                        verifier_bytes / 1e9);
+        }
+        if (base_released) {
+            uint32_t rn = 0, rd = 0;
+            load_base(PFX, base, rn, rd);
+            if (rn != n || rd != d) {
+                std::fprintf(stderr, "[rknn] reloaded base shape mismatch: got n=%u d=%u expected n=%u d=%u\n", rn,
+                             rd, n, d);
+                return 3;
+            }
+            d_base = upload(base);
+            std::printf("[rknn] reloaded original base after graph index construction: %.3fGB\n",
+                        (double)n * d * 4 / 1e9);
+        }
         return 0;
     };
 
@@ -491,6 +648,7 @@ int main(int argc, char** argv) {
                     radius_desc.c_str());
         std::printf("[rknn] graph degree M=%zu\n", Marg);
         std::printf("Index config: %s\n", index_name.c_str());
+        release_for_index();
         std::vector<uint32_t> hdeg, hlinks, left, right, entry;
         std::vector<float> offset, hyper;
         if (read_graph(cn.graph_index, n, dl, Marg, hdeg, hlinks, left, right, offset, hyper, entry)) {
@@ -544,10 +702,51 @@ int main(int argc, char** argv) {
         // graph + search tree (index.h:1071-1295), saved with its marker (:1293;
         // rknn_query.cpp:2297)
         if (int rc = load_source()) return rc;
+        release_for_index();
+        // the warm start's ids: streamed, read back from the compacted .rec, or
+        // the source's own (index.h:1127-1218)
+        uint32_t* d_warm = d_aknn;
+        if (!warm_cmd.empty() || compact_ids) {
+            if (compact_ids) {
+                if (!compact_rec_in_place(rec_p, n, 100, 100)) {
+                    std::fprintf(stderr, "[rknn] cannot finalize compact warm-start IDs %s\n", rec_p.c_str());
+                    return 3;
+                }
+                std::printf("[rknn] compacted regenerable prebuilt top-k to IDs in place: %s source_k=100 width=100 "
+                            "bytes=%.3fGB\n", rec_p.c_str(), (double)n * 100 * 4 / 1e9);
+            }
+            std::FILE* in = compact_ids ? std::fopen(rec_p.c_str(), "rb") : ::popen(warm_cmd.c_str(), "r");
+            if (!in) {
+                std::fprintf(stderr, "[rknn] cannot start streamed warm-start %s\n",
+                             compact_ids ? rec_p.c_str() : "command");
+                return 3;
+            }
+            std::vector<uint32_t> wids;
+            const bool ok = read_warm_stream(in, compact_ids, n, 100, 100, wids);
+            const int st = compact_ids ? std::fclose(in) : ::pclose(in);
+            if (!ok || st != 0) {
+                std::fprintf(stderr, "[rknn] streamed warm-start ended early, produced extra data, or failed\n");
+                return 3;
+            }
+            if (compact_ids && ::unlink(rec_p.c_str()) != 0) {
+                std::fprintf(stderr, "[rknn] cannot release consumed streamed warm-start file %s\n", rec_p.c_str());
+                return 3;
+            }
+            d_warm = upload(wids);
+        } else if (release_rec) {
+            if (::unlink(rec_p.c_str()) != 0) {
+                std::fprintf(stderr, "[rknn] cannot release regenerable prebuilt top-k %s: %s\n", rec_p.c_str(),
+                             std::strerror(errno));
+                return 3;
+            }
+            std::printf("[rknn] released regenerable prebuilt top-k after verifier/lift preparation: %s\n",
+                        rec_p.c_str());
+        }
         const auto tg0 = std::chrono::steady_clock::now();
         anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10, anqi_cli::gpu_form_chosen() ? 1 : 0};
         anqi_graph_stats gst{};
-        ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_aknn, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
+        ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_warm, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
+        if (d_warm != d_aknn) anqi_dfree(g_ctx, d_warm);
         std::vector<uint32_t> hdeg, hlinks, left(tree.nodes), right(tree.nodes);
         std::vector<float> offset(tree.nodes), hyper((size_t)tree.nodes * dl);
         download(hdeg, d_deg, n);
@@ -565,6 +764,7 @@ int main(int argc, char** argv) {
         }
         graph_index_s = since(tg0);
     }
+    if (int rc = reload_after_index()) return rc;
     float* d_bnorm = dalloc<float>(n);
     ck(anqi_norms(g_ctx, d_base, n, d, d_bnorm), "norms");
     ck(anqi_ctx_sync(g_ctx), "sync");

@@ -153,3 +153,53 @@ def test_cache_names_equal_reference(tmp_path):
         assert r.returncode == 3 and "required graph cache is incomplete or incompatible" in r.stderr, (ref, r.stderr)
         keys.append(re.search(r"graph cache key=(\S+)", r.stdout).group(1))
     assert keys[0] == keys[1] != key
+
+
+def _u8_sample(tmp_path):
+    """A copy of the sample whose base values are integers 0..255, as the
+    fbin and as a uint8 file (ANQI_BASE_U8BIN, src/rknn_query.cpp:55-123)."""
+    pfx = str(tmp_path / "sample")
+    for suf in SAMPLE_FILES:
+        shutil.copy(os.path.join(GOLDEN, "sample", "sample" + suf), pfx + suf)
+    base = np.fromfile(pfx + "_base.bin", dtype=np.float32, offset=8).reshape(256, 128)
+    q8 = np.clip(np.rint(base * 40 + 128), 0, 255).astype(np.uint8)
+    with open(pfx + "_base.bin", "wb") as f:
+        f.write(np.array([256, 128], np.uint32).tobytes() + q8.astype(np.float32).tobytes())
+    with open(pfx + "_base.u8bin", "wb") as f:
+        f.write(np.array([256, 128], np.uint32).tobytes() + q8.tobytes())
+    return pfx
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFBIN, "rknn_query")), reason="reference tools not built here")
+def test_u8_base_cache_key_and_bad_files_equal_reference(tmp_path):
+    """With ANQI_BASE_U8BIN the reference keys the graph cache by the uint8
+    file's identity and prints its load line; this tool prints the same key
+    and line, finds the reference's caches, and gives the same exit codes for
+    a truncated uint8 file (3) and for compact warm-start ids combined with a
+    warm-start command (2)."""
+    pfx = _u8_sample(tmp_path)
+    env = dict(PIN_ENV, ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec", ANQI_BASE_U8BIN=pfx + "_base.u8bin",
+               ANQI_GRAPH_CACHE_DIR=str(tmp_path / "graph"), ANQI_VERIFIER_CACHE_DIR=str(tmp_path / "verifier"))
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    args = ["16", pfx, "48", "500", "64", "600", "10"]
+    rr = run("rknn_query", args, env, ref=True)
+    assert rr.returncode == 0, rr.stderr
+    key = re.search(r"graph cache key=(\S+)", rr.stdout).group(1)
+    line = [l for l in rr.stdout.splitlines() if l.startswith("[rknn] loaded uint8 base directly as float")]
+    r = run("rknn_query", args, dict(env, ANQI_REQUIRE_GRAPH_CACHE="1"))
+    assert f"[rknn] graph cache key={key}\n" in r.stdout and line and line[0] in r.stdout.splitlines()
+    assert "required graph cache" not in r.stderr
+    assert r.returncode == 0 or (r.returncode == 3 and "GPU failure in context" in r.stderr), r.stderr
+    # the key differs from the fbin run's (the uint8 file's identity is in it)
+    r2 = run("rknn_query", args, dict(env, ANQI_BASE_U8BIN="", ANQI_REQUIRE_GRAPH_CACHE="1"))
+    assert re.search(r"graph cache key=(\S+)", r2.stdout).group(1) != key
+    with open(pfx + "_bad.u8bin", "wb") as f:
+        f.write(np.array([256, 128], np.uint32).tobytes() + bytes(1000))
+    for ref in (True, False):
+        r = run("rknn_query", args, dict(env, ANQI_BASE_U8BIN=pfx + "_bad.u8bin"), ref=ref)
+        assert r.returncode == 3 and "bad uint8 base" in r.stderr, (ref, r.returncode, r.stderr)
+        r = run("rknn_query", args, dict(env, ANQI_COMPACT_PREBUILT_TOPK_IDS_INPLACE="1",
+                                         ANQI_PREBUILT_WARM_START_COMMAND="cat " + pfx + "_baseknn_aknn.rec",
+                                         ANQI_GRAPH_CACHE_DIR=str(tmp_path / "g2") + "/"), ref=ref)
+        assert r.returncode == 2 and "cannot be combined" in r.stderr, (ref, r.returncode, r.stderr)

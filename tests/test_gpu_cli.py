@@ -183,3 +183,73 @@ def test_rknn_query_loads_reference_caches(tmp_path):
         lift = re.search(r"M\*\^2=\S+ scale=\S+", ref.split("BUILD lift" if k == 10 else "loaded cached index")[1])
         assert f"[rknn] loaded cached index (M=48): {lift.group(0)}" in out, k
         assert answers(out) == answers(ref), (k, answers(out), answers(ref))
+
+
+def test_ingestion_options_give_the_same_answers(tmp_path):
+    """The reference's scale options change how the inputs reach the build,
+    not the answers: the base read from a uint8 file (ANQI_BASE_U8BIN), the
+    warm-start ids streamed from a command (ANQI_PREBUILT_WARM_START_COMMAND)
+    or from the .rec compacted in place to ids and then removed
+    (ANQI_COMPACT_PREBUILT_TOPK_IDS_INPLACE), the .rec removed after the
+    preparation (ANQI_RELEASE_PREBUILT_TOPK_AFTER_PREP), the verifier and the
+    base released while the graph is built and read back after
+    (ANQI_RELEASE_*_DURING_GRAPH_INDEX; src/rknn_query.cpp:55-123, 988-1057,
+    2101-2190, 2257-2320).  Every run prints the answer lines of the plain
+    run on a base of integer values 0..255."""
+    src = os.path.join(GOLDEN, "sample", "sample")
+    pfx = str(tmp_path / "s")
+    for suf in ("_query.bin", "_lifted.meta", "_baseknn_aknn.rec", "_radbasis_M8.bin",
+                "_learned_rq_codebook_M8_u8.bin", "_rknn_gt_k10.bin"):
+        shutil.copy(src + suf, pfx + suf)
+    base = np.fromfile(src + "_base.bin", dtype=np.float32, offset=8).reshape(256, 128)
+    q8 = np.clip(np.rint(base * 40 + 128), 0, 255).astype(np.uint8)
+    with open(pfx + "_base.bin", "wb") as f:
+        f.write(np.array([256, 128], np.uint32).tobytes() + q8.astype(np.float32).tobytes())
+    with open(pfx + "_base.u8bin", "wb") as f:
+        f.write(np.array([256, 128], np.uint32).tobytes() + q8.tobytes())
+
+    def go(name, extra, rec=None):
+        env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
+        env.update(PIN_ENV)
+        env["ANQI_PREBUILT_TOPK_REC"] = rec or pfx + "_baseknn_aknn.rec"
+        env["ANQI_GRAPH_CACHE_DIR"] = str(tmp_path / name / "graph")
+        env["ANQI_VERIFIER_CACHE_DIR"] = str(tmp_path / name / "verifier")
+        os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+        os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+        env.update(extra)
+        say("rknn_query", name)
+        r = subprocess.run([os.path.join(BIN, "rknn_query"), "16,32,64,128", pfx, "48", "500", "64", "600", "10"],
+                           env=env, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (name, r.stderr[-2000:])
+        return r.stdout
+
+    plain = go("plain", {})
+    want = answers(plain)
+    assert len(want) == 12
+    out = go("u8", {"ANQI_BASE_U8BIN": pfx + "_base.u8bin"})
+    assert "[rknn] loaded uint8 base directly as float" in out and answers(out) == want
+    out = go("stream", {"ANQI_PREBUILT_WARM_START_COMMAND": "cat " + pfx + "_baseknn_aknn.rec"})
+    assert "deferred original KNN IDs to streamed graph warm-start" in out and answers(out) == want
+    for name, flag, line in (("compact", "ANQI_COMPACT_PREBUILT_TOPK_IDS_INPLACE", "compacted regenerable prebuilt"),
+                             ("release", "ANQI_RELEASE_PREBUILT_TOPK_AFTER_PREP", "released regenerable prebuilt")):
+        rec = str(tmp_path / (name + ".rec"))
+        shutil.copy(pfx + "_baseknn_aknn.rec", rec)
+        out = go(name, {flag: "1"}, rec=rec)
+        assert line in out and answers(out) == want and not os.path.exists(rec), name
+    out = go("release_mem", {"ANQI_RELEASE_VERIFIER_DURING_GRAPH_INDEX": "1",
+                             "ANQI_RELEASE_BASE_DURING_GRAPH_INDEX": "1"})
+    assert "reloaded compact verifier after graph index construction" in out
+    assert "reloaded original base after graph index construction" in out and answers(out) == want
+    # a stream that ends early is refused
+    env_bad = {"ANQI_PREBUILT_WARM_START_COMMAND": "head -c 1000 " + pfx + "_baseknn_aknn.rec"}
+    env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
+    env.update(PIN_ENV)
+    env.update(env_bad)
+    env["ANQI_PREBUILT_TOPK_REC"] = pfx + "_baseknn_aknn.rec"
+    env["ANQI_GRAPH_CACHE_DIR"] = str(tmp_path / "bad" / "graph")
+    env["ANQI_VERIFIER_CACHE_DIR"] = str(tmp_path / "bad" / "verifier")
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    r = subprocess.run([os.path.join(BIN, "rknn_query"), "16", pfx, "48", "500", "64", "600", "10"], env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 3 and "streamed warm-start ended early" in r.stderr
