@@ -968,3 +968,302 @@ void or_commit(const float* x, const float* nrm, uint32_t n, uint32_t d,
     }
     free(roff); free(rids); free(cur);
 }
+
+/* ---- Vamana refinement and the single graph entry -------------------------- */
+
+/* greedy_collect (index.h:831-855) and greedy_collect_from_entry (:857-880):
+ * a beam of Lb entries from `entry` over the fixed-degree graph; every first
+ * visit except row i itself joins the pool, in visit order. */
+uint32_t or_vamana_collect(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                           const uint32_t* deg, const uint32_t* links, uint32_t M,
+                           uint32_t i, uint32_t entry, uint32_t Lb,
+                           float* pool_d, uint32_t* pool_id, uint32_t pool_max) {
+    uint8_t* vis = (uint8_t*)calloc(n, 1);
+    float* qd = (float*)malloc(sizeof(float) * (Lb + 1));
+    uint32_t* qp = (uint32_t*)malloc(sizeof(uint32_t) * (Lb + 1));
+    or_queue q;
+    or_queue_init(&q, qd, qp, Lb);
+    uint32_t m = 0;
+    const float* xi = x + (size_t)i * d;
+#define PUSHV(id_, d_) do { if ((id_) != i) { if (m < pool_max) { pool_d[m] = (d_); pool_id[m] = (id_); } m++; } } while (0)
+    const uint32_t ep = entry < n ? entry : 0;
+    vis[ep] = 1;
+    {
+        const float dv = or_d2(xi, x + (size_t)ep * d, nrm[i], nrm[ep], d);
+        or_queue_insert(&q, ep, dv);
+        PUSHV(ep, dv);
+    }
+    while (q.cur < q.size) {
+        const uint32_t cur = or_queue_pop_closest(&q);
+        for (uint32_t j = 0; j < deg[cur]; j++) {
+            const uint32_t nb = links[(size_t)cur * M + j];
+            if (nb >= n || vis[nb]) continue;
+            vis[nb] = 1;
+            const float dv = or_d2(xi, x + (size_t)nb * d, nrm[i], nrm[nb], d);
+            or_queue_insert(&q, nb, dv);
+            PUSHV(nb, dv);
+        }
+    }
+#undef PUSHV
+    free(vis); free(qd); free(qp);
+    return m;
+}
+
+/* append_aknn_edges (index.h:766-769): the heap slots with their distances
+ * recomputed; append_current_edges (:757-764). */
+static uint32_t vam_append_heap(const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx,
+                                uint32_t cap, uint32_t i, float* pd, uint32_t* pi, uint32_t m) {
+    for (uint32_t s = 0; s < cap; s++) {
+        const uint32_t j = hidx[(size_t)i * cap + s];
+        if (j == UINT32_MAX || j >= n || j == i) continue;
+        pd[m] = or_d2(x + (size_t)i * d, x + (size_t)j * d, nrm[i], nrm[j], d);
+        pi[m++] = j;
+    }
+    return m;
+}
+static uint32_t vam_append_links(const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* deg,
+                                 const uint32_t* links, uint32_t M, uint32_t i, float* pd, uint32_t* pi,
+                                 uint32_t m) {
+    for (uint32_t j = 0; j < deg[i]; j++) {
+        const uint32_t nb = links[(size_t)i * M + j];
+        if (nb == UINT32_MAX || nb >= n || nb == i) continue;
+        pd[m] = or_d2(x + (size_t)i * d, x + (size_t)nb * d, nrm[i], nrm[nb], d);
+        pi[m++] = nb;
+    }
+    return m;
+}
+
+/* One pass of the global style (index.h:1267-1279): per row, collect from the
+ * row's tree entry, add the heap slots, prune to M; then commit_edges. */
+void or_vamana_global_pass(const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* entries,
+                           const uint32_t* hidx, uint32_t cap, uint32_t Lb, uint32_t M, float alpha,
+                           uint32_t* links, uint32_t* deg) {
+    uint32_t* O = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * M);
+    uint32_t* ocnt = (uint32_t*)malloc(sizeof(uint32_t) * n);
+#pragma omp parallel
+    {
+        float* pd = (float*)malloc(sizeof(float) * ((size_t)n + cap));
+        uint32_t* pi = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + cap));
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            const uint32_t i = (uint32_t)ii;
+            uint32_t m = or_vamana_collect(x, nrm, n, d, deg, links, M, i, entries[i], Lb, pd, pi, n);
+            m = vam_append_heap(x, nrm, n, d, hidx, cap, i, pd, pi, m);
+            ocnt[i] = or_prune(x, nrm, d, i, n, pd, pi, m, M, alpha, O + (size_t)i * M);
+        }
+        free(pd); free(pi);
+    }
+    or_commit(x, nrm, n, d, O, ocnt, M, alpha, links, deg);
+    free(O); free(ocnt);
+}
+
+/* MT19937-64 (Nishimura 2000; std::mt19937_64's parameters). */
+void or_mt64_seed(or_mt64* m, uint64_t seed) {
+    m->x[0] = seed;
+    for (uint32_t i = 1; i < 312; i++) m->x[i] = 6364136223846793005ull * (m->x[i - 1] ^ (m->x[i - 1] >> 62)) + i;
+    m->i = 312;
+}
+
+uint64_t or_mt64_next(or_mt64* m) {
+    if (m->i >= 312) {
+        for (uint32_t k = 0; k < 312; k++) {
+            const uint64_t y = (m->x[k] & 0xFFFFFFFF80000000ull) | (m->x[(k + 1) % 312] & 0x7FFFFFFFull);
+            m->x[k] = m->x[(k + 156) % 312] ^ (y >> 1) ^ ((y & 1u) ? 0xB5026F5AA96619E9ull : 0ull);
+        }
+        m->i = 0;
+    }
+    uint64_t y = m->x[m->i++];
+    y ^= (y >> 29) & 0x5555555555555555ull;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ull;
+    y ^= (y << 37) & 0xFFF7EEE000000000ull;
+    y ^= y >> 43;
+    return y;
+}
+
+/* Lemire's nearly-divisionless draw in [0, range) from a 64-bit engine: the
+ * 128-bit product's high word, rejecting while its low word is below
+ * 2^64 mod range (libstdc++'s uniform_int_distribution for a full 64-bit
+ * engine). */
+static uint64_t mt64_below(or_mt64* m, uint64_t range) {
+    unsigned __int128 p = (unsigned __int128)or_mt64_next(m) * range;
+    uint64_t low = (uint64_t)p;
+    if (low < range) {
+        const uint64_t t = (0ull - range) % range;
+        while (low < t) {
+            p = (unsigned __int128)or_mt64_next(m) * range;
+            low = (uint64_t)p;
+        }
+    }
+    return (uint64_t)(p >> 64);
+}
+
+/* std::shuffle as libstdc++ (GCC 11 bits/stl_algo.h) runs it with a 64-bit
+ * engine over n < 2^32 elements: one draw in [0, b(b+1)) gives the swap
+ * positions of two successive elements (x / (b+1), x % (b+1)), after one
+ * single draw in [0, 2) when n is even. */
+void or_shuffle_u32(uint32_t* a, uint32_t n, or_mt64* g) {
+    if (n == 0) return;
+    uint32_t i = 1;
+    uint32_t t;
+    if ((n % 2u) == 0) {
+        const uint64_t p = mt64_below(g, 2);
+        t = a[i]; a[i] = a[p]; a[p] = t;
+        i++;
+    }
+    while (i != n) {
+        const uint64_t b = (uint64_t)i + 1;
+        const uint64_t xr = mt64_below(g, b * (b + 1));
+        const uint64_t p1 = xr / (b + 1), p2 = xr % (b + 1);
+        t = a[i]; a[i] = a[p1]; a[p1] = t;
+        i++;
+        t = a[i]; a[i] = a[p2]; a[p2] = t;
+        i++;
+    }
+}
+
+/* choose_vamana_entry's "medoid" (index.h:190-222): the sampled row nearest
+ * (in binary64, fused squares) to the sample's binary64 mean. */
+uint32_t or_vamana_medoid(const float* x, uint32_t n, uint32_t d, uint64_t sample) {
+    if (sample < 1) sample = 1;
+    if (sample > n) sample = n;
+    uint64_t step = n / sample;
+    if (step < 1) step = 1;
+    double* mean = (double*)calloc(d, sizeof(double));
+    for (uint64_t t = 0; t < sample; t++) {
+        const float* r = x + ((t * step) % n) * (size_t)d;
+        for (uint32_t j = 0; j < d; j++) mean[j] += (double)r[j];
+    }
+    for (uint32_t j = 0; j < d; j++) mean[j] /= (double)sample;
+    uint32_t best = 0;
+    double bestd = INFINITY;
+    for (uint64_t t = 0; t < sample; t++) {
+        const uint64_t id = (t * step) % n;
+        const float* r = x + id * (size_t)d;
+        double dd = 0.0;
+        for (uint32_t j = 0; j < d; j++) {
+            const double z = (double)r[j] - mean[j];
+            dd = fma(z, z, dd);
+        }
+        if (dd < bestd) { bestd = dd; best = (uint32_t)id; }
+    }
+    free(mean);
+    return best;
+}
+
+/* vamana_batch_refine (index.h:932-1063): per pass (prune alpha early_alpha
+ * before the last pass when use_early), the order reshuffled by
+ * mt19937_64(seed + 0x9e3779b97f4a7c15 (pass + 1)); batches of 1, 2, 4, ...
+ * rows (base^k - 1 .. base^(k+1) - 1 while base^k <= max_batch), then
+ * max_batch rows.  A batch's rows collect from the one entry, add their heap
+ * slots and current edges and are pruned against the graph as it stood
+ * before the batch, then written; every row they point to is pruned again
+ * over its new edges plus the batch rows that point to it. */
+void or_vamana_batch(const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx, uint32_t cap,
+                     uint32_t Lb, int passes, double batch_frac, uint64_t max_batch_cap, double batch_base,
+                     float alpha, float early_alpha, int use_early, uint32_t entry, uint64_t seed, uint32_t M,
+                     uint32_t* links, uint32_t* deg) {
+    if (n == 0 || passes <= 0) return;
+    if (batch_frac <= 0.0) batch_frac = 0.02;
+    double mb = batch_frac * (double)n;
+    if (mb < 1.0) mb = 1.0;
+    uint64_t max_batch = (uint64_t)mb;
+    if (max_batch > max_batch_cap) max_batch = max_batch_cap;
+    if (max_batch == 0) max_batch = n;
+    if (batch_base <= 1.0) batch_base = 2.0;
+    uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    uint32_t* O = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * M);
+    uint32_t* ocnt = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* rcnt = (uint32_t*)calloc(n, sizeof(uint32_t));
+    uint32_t* roff = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    uint32_t* rsrc = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * M);
+    uint32_t* tl = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (int pass = 0; pass < passes; pass++) {
+        const float pa = (use_early && pass + 1 < passes) ? early_alpha : alpha;
+        or_mt64 g;
+        or_mt64_seed(&g, seed + 0x9e3779b97f4a7c15ull * (uint64_t)(pass + 1));
+        or_shuffle_u32(order, n, &g);
+        uint64_t count = 0, inc = 0;
+        while (count < n) {
+            uint64_t fl = count, ce;
+            const double p0 = pow(batch_base, (double)inc);
+            if (p0 <= (double)max_batch) {
+                fl = (uint64_t)p0 - 1;
+                const double p1 = pow(batch_base, (double)(inc + 1));
+                ce = (uint64_t)p1 - 1;
+                if (ce > n) ce = n;
+                count = ce;
+            } else {
+                ce = count + max_batch;
+                if (ce > n) ce = n;
+                count = ce;
+            }
+            inc++;
+            if (ce <= fl || fl >= n) continue;
+            const uint32_t bsz = (uint32_t)(ce - fl);
+#pragma omp parallel
+            {
+                float* pd = (float*)malloc(sizeof(float) * ((size_t)n + cap + M));
+                uint32_t* pi = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + cap + M));
+#pragma omp for schedule(dynamic, 16)
+                for (int64_t b = 0; b < (int64_t)bsz; b++) {
+                    const uint32_t node = order[fl + b];
+                    uint32_t m = or_vamana_collect(x, nrm, n, d, deg, links, M, node, entry, Lb, pd, pi, n);
+                    m = vam_append_heap(x, nrm, n, d, hidx, cap, node, pd, pi, m);
+                    m = vam_append_links(x, nrm, n, d, deg, links, M, node, pd, pi, m);
+                    ocnt[node] = or_prune(x, nrm, d, node, n, pd, pi, m, M, pa, O + (size_t)node * M);
+                }
+                free(pd); free(pi);
+            }
+            /* write_ids, then the rows they point to */
+            uint32_t nt = 0;
+            for (uint32_t b = 0; b < bsz; b++) {
+                const uint32_t node = order[fl + b];
+                for (uint32_t t = 0; t < ocnt[node]; t++) {
+                    const uint32_t dst = O[(size_t)node * M + t];
+                    links[(size_t)node * M + t] = dst;
+                    if (dst < n && rcnt[dst]++ == 0) tl[nt++] = dst;
+                }
+                for (uint32_t t = ocnt[node]; t < M; t++) links[(size_t)node * M + t] = UINT32_MAX;
+                deg[node] = ocnt[node];
+            }
+            roff[0] = 0;
+            for (uint32_t t = 0; t < nt; t++) roff[t + 1] = roff[t] + rcnt[tl[t]];
+            for (uint32_t t = 0; t < nt; t++) rcnt[tl[t]] = roff[t];   /* cursors */
+            for (uint32_t b = 0; b < bsz; b++) {
+                const uint32_t node = order[fl + b];
+                for (uint32_t t = 0; t < ocnt[node]; t++) {
+                    const uint32_t dst = O[(size_t)node * M + t];
+                    if (dst < n) rsrc[rcnt[dst]++] = node;
+                }
+            }
+#pragma omp parallel
+            {
+                float* pd = (float*)malloc(sizeof(float) * ((size_t)n + M));
+                uint32_t* pi = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + M));
+                uint32_t* out = (uint32_t*)malloc(sizeof(uint32_t) * M);
+#pragma omp for schedule(dynamic, 16)
+                for (int64_t t = 0; t < (int64_t)nt; t++) {
+                    const uint32_t dst = tl[t];
+                    uint32_t m = vam_append_links(x, nrm, n, d, deg, links, M, dst, pd, pi, 0);
+                    for (uint32_t p = roff[t]; p < roff[t + 1]; p++) {
+                        const uint32_t src = rsrc[p];
+                        if (src == dst || src >= n) continue;
+                        pd[m] = or_d2(x + (size_t)dst * d, x + (size_t)src * d, nrm[dst], nrm[src], d);
+                        pi[m++] = src;
+                    }
+                    const uint32_t dg = or_prune(x, nrm, d, dst, n, pd, pi, m, M, pa, out);
+                    /* rows are pruned from copies, so the writes may follow in any order */
+#pragma omp critical
+                    {
+                        for (uint32_t s = 0; s < M; s++) links[(size_t)dst * M + s] = s < dg ? out[s] : UINT32_MAX;
+                        deg[dst] = dg;
+                    }
+                }
+                free(pd); free(pi); free(out);
+            }
+            for (uint32_t t = 0; t < nt; t++) rcnt[tl[t]] = 0;
+        }
+    }
+    free(order); free(O); free(ocnt); free(rcnt); free(roff); free(rsrc); free(tl);
+}

@@ -182,6 +182,31 @@ void or_commit(const float* x, const float* nrm, uint32_t n, uint32_t d,
                const uint32_t* O, const uint32_t* ocnt, uint32_t M, float alpha,
                uint32_t* links, uint32_t* deg);
 
+/* ---- Vamana refinement (include/index.h:831-1065, 1262-1282) -------------- */
+/* greedy_collect / greedy_collect_from_entry (index.h:831-880): pool in visit
+ * order (row i itself excluded); returns its size (truncated at pool_max). */
+uint32_t or_vamana_collect(const float* x, const float* nrm, uint32_t n, uint32_t d,
+                           const uint32_t* deg, const uint32_t* links, uint32_t M,
+                           uint32_t i, uint32_t entry, uint32_t Lb,
+                           float* pool_d, uint32_t* pool_id, uint32_t pool_max);
+/* One pass of ANQI_VAMANA_STYLE=global (index.h:1267-1279); entries[i] = the
+ * tree entry of lifted row i; links/deg updated in place. */
+void or_vamana_global_pass(const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* entries,
+                           const uint32_t* hidx, uint32_t cap, uint32_t Lb, uint32_t M, float alpha,
+                           uint32_t* links, uint32_t* deg);
+/* ANQI_VAMANA_STYLE=batch (vamana_batch_refine, index.h:932-1063). */
+void or_vamana_batch(const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* hidx, uint32_t cap,
+                     uint32_t Lb, int passes, double batch_frac, uint64_t max_batch_cap, double batch_base,
+                     float alpha, float early_alpha, int use_early, uint32_t entry, uint64_t seed, uint32_t M,
+                     uint32_t* links, uint32_t* deg);
+/* choose_vamana_entry "medoid"/"centroid" (index.h:190-222). */
+uint32_t or_vamana_medoid(const float* x, uint32_t n, uint32_t d, uint64_t sample);
+/* std::mt19937_64 and libstdc++'s std::shuffle over it (GCC 11). */
+typedef struct { uint64_t x[312]; uint32_t i; } or_mt64;
+void or_mt64_seed(or_mt64* m, uint64_t seed);
+uint64_t or_mt64_next(or_mt64* m);
+void or_shuffle_u32(uint32_t* a, uint32_t n, or_mt64* g);
+
 /* ---- small helpers shared by tests -------------------------------------- */
 void or_sort_pairs(float* d, uint32_t* id, uint32_t m);   /* (dist, id) ascending */
 

@@ -9,6 +9,7 @@
 #undef main
 
 #include <omp.h>
+#include <thread>
 
 using nndgraph::HeapList;
 using nndgraph::Neighbor;
@@ -246,6 +247,53 @@ void ref_commit(void* h, const uint32_t* O, const uint32_t* ocnt, uint32_t M,
     for (size_t i = 0; i < ix->n; i++) OO[i].assign(O + i * M, O + i * M + ocnt[i]);
     ix->commit_edges(OO);
     ref_graph_arrays(h, deg, links);
+}
+
+// build_index (index.h:1071-1295) on one thread under the current environment
+// (ANQI_REUSE_ORIG_KNN, ANQI_VAMANA*, ...), warm-started from the given ids:
+// the search tree, graph and Vamana passes as the reference builds them.
+// Exports the graph and graph_entry_.  The build runs on a fresh thread: the
+// tree's generator is a thread_local engine seeded by its first call
+// (include/utils.h:169-173), so only a new thread draws the tree of the
+// index's own seed.
+void ref_build_graph(void* h, const uint32_t* ids, uint32_t width, uint32_t* deg, uint32_t* links,
+                     uint32_t* graph_entry) {
+    nndindex* ix = static_cast<RefIndex*>(h)->idx;
+    ix->n_threads = 1;
+    std::vector<std::vector<uint32_t>> nbr(ix->n);
+    for (size_t i = 0; i < ix->n; i++) nbr[i].assign(ids + i * width, ids + (i + 1) * width);
+    std::thread t([&]() { ix->build_index(true, false, &nbr); });
+    t.join();
+    ref_graph_arrays(h, deg, links);
+    *graph_entry = ix->graph_entry_;
+}
+
+// greedy_collect (index.h:831-855; entry = UINT32_MAX) or
+// greedy_collect_from_entry (:857-880) for row i over the index's current
+// graph and tree.  Returns the pool size (pool truncated at pmax).
+uint32_t ref_vamana_collect(void* h, uint32_t i, uint32_t entry, uint64_t Lb, float* pd, uint32_t* pi,
+                            uint32_t pmax) {
+    nndindex* ix = static_cast<RefIndex*>(h)->idx;
+    std::vector<std::pair<float, uint32_t>> pool;
+    if (entry == UINT32_MAX) ix->greedy_collect(i, pool, Lb);
+    else ix->greedy_collect_from_entry(i, pool, Lb, entry);
+    for (size_t t = 0; t < pool.size() && t < pmax; t++) { pd[t] = pool[t].first; pi[t] = pool[t].second; }
+    return (uint32_t)pool.size();
+}
+
+// std::shuffle of 0..n-1 with std::mt19937_64(seed), as vamana_batch_refine
+// draws its insertion order (index.h:956-958).
+void ref_shuffle(uint32_t n, uint64_t seed, uint32_t* out) {
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::mt19937_64 rng(seed);
+    std::shuffle(order.begin(), order.end(), rng);
+    std::copy(order.begin(), order.end(), out);
+}
+
+// choose_vamana_entry (index.h:190-222) under the current environment.
+uint32_t ref_vamana_entry(void* h) {
+    return static_cast<RefIndex*>(h)->idx->choose_vamana_entry();
 }
 
 // ---- CompactLrqVerifier (rknn_query.cpp:603-839), policy "floor" ---------

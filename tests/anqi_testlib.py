@@ -73,6 +73,14 @@ ORACLE_SIGS = {
     "or_sort_pairs": (None, [P, P, U32]),
     "or_mt_stream": (None, [U64, P, U64]),
     "or_rp_tree_build": (U32, [P, U32, U32, U32, U32, U64, U32, P, P, P, P, P, P, P]),
+    "or_vamana_collect": (U32, [P, P, U32, U32, P, P, U32, U32, U32, U32, P, P, U32]),
+    "or_vamana_global_pass": (None, [P, P, U32, U32, P, P, U32, U32, U32, F32, P, P]),
+    "or_vamana_batch": (None, [P, P, U32, U32, P, U32, U32, I32, F64, U64, F64, F32, F32, I32, U32, U64, U32,
+                               P, P]),
+    "or_vamana_medoid": (U32, [P, U32, U32, U64]),
+    "or_mt64_seed": (None, [P, U64]),
+    "or_mt64_next": (U64, [P]),
+    "or_shuffle_u32": (None, [P, U32, P]),
 }
 
 REF_SIGS = {
@@ -99,6 +107,10 @@ REF_SIGS = {
     "ref_gather": (U32, [P, P, P, U32, U32, U64, P, P, U32]),
     "ref_prune": (U32, [P, U32, P, P, U32, P]),
     "ref_commit": (None, [P, P, P, U32, P, P]),
+    "ref_build_graph": (None, [P, P, U32, P, P, P]),
+    "ref_shuffle": (None, [U32, U64, P]),
+    "ref_vamana_collect": (U32, [P, U32, U32, U64, P, P, U32]),
+    "ref_vamana_entry": (U32, [P]),
     "ref_lrq_build": (None, [U64, U32, U32, P, P, P, U32, P, P, P]),
     "ref_lrq_materialize": (None, [U64, U32, U32, P, P, P, U32, P, P, U32, F64, P]),
     "ref_lrq_materialize_metric": (None, [U64, U32, U32, P, P, P, U32, P, P, U32, F64, I32, P]),
