@@ -796,17 +796,20 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
         HCHK(hipStreamSynchronize(s));
         return ANQI_OK;
     };
-    uint64_t need = (uint64_t)P.pool_cap + P.cap + maxrev;   // visits / pool entries bound
+    // visits / pool entries bound; a Vamana collect has no pool cap, so its
+    // slow path is sized for every row it can meet (up to 2^18 entries)
+    uint64_t need = P.vmode ? std::min<uint64_t>((uint64_t)P.n + P.cap + P.vM, 1u << 18)
+                            : (uint64_t)P.pool_cap + P.cap + maxrev;
     // fast path: LDS work area.  The register-list AKNN gather keeps its
     // visited set in 16-bit entries (half the LDS, so more rows per CU).
     // The register-list graph gather's pool holds 1024 entries (four rows per
     // CU instead of three); rows that outgrow it are redone on the slow path.
+    // A Vamana collect (about a thousand visits at L = 128) starts at 2048.
     const bool rk = gather_register_lists(P, graph);
     P.hmask = std::min<uint32_t>(graph ? 4096 : 8192, next_pow2(need * 3 / 2 + 64)) - 1;
     P.h16 = (!graph && rk && P.hmask + 1 == H16_SLOTS) ? (P.n <= H16_MAX_ID ? 1u : P.n <= H24_MAX_ID ? 2u : 0u) : 0u;
-    P.pmax = graph ? std::min<uint32_t>(rk ? 1024u : 2048u, next_pow2(need + 64)) : 0;
-    P.rows = nullptr;
-    P.nrows = P.n;
+    P.pmax = graph ? std::min<uint32_t>((rk && !P.vmode) ? 1024u : 2048u, next_pow2(need + 64)) : 0;
+    if (!P.rows) P.nrows = P.n;   // else: the given row list
     HCHK(launch_gather(P, graph, false, s));
     std::vector<uint32_t> bad;
     int rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
@@ -890,6 +893,38 @@ int anqi_gather_prune(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t 
     return gather_run(ctx, P, true, maxrev, nullptr);
 }
 
+// The prune of commit_edges over P.rows (or every row) in LDS, then the rows
+// whose pool outgrew it with a global work area.  P.status: n words.
+static int commit_run(anqi_ctx* ctx, PruneParams P, uint32_t maxin) {
+    hipStream_t s = ctx->stream;
+    const uint32_t n = P.n, M = P.M;
+    HCHK(hipMemsetAsync(P.status, 0, 4ull * n, s));
+    if (!P.rows) P.nrows = n;
+    P.pmax = std::min<uint32_t>(2048, next_pow2((uint64_t)M + maxin + 64));
+    HCHK(launch_commit(P, false, s));
+    std::vector<uint32_t> bad;
+    int rc = failed_rows(ctx, P.status, n, bad);
+    if (rc) return rc;
+    if (bad.empty()) return ANQI_OK;
+    PruneParams Q = P;
+    Q.pmax = next_pow2((uint64_t)M + maxin + 64);
+    size_t words = (commit_lds_bytes(Q) + 3) / 4;
+    DBuf rows, ws;
+    HCHK(rows.alloc(4 * bad.size()));
+    HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
+    HCHK(hipMemsetAsync(P.status, 0, 4ull * n, s));
+    HCHK(ws.alloc(4 * words * bad.size()));
+    Q.rows = rows.as<uint32_t>();
+    Q.nrows = (uint32_t)bad.size();
+    Q.gscratch = ws.as<uint32_t>();
+    Q.gscratch_words = words;
+    HCHK(launch_commit(Q, true, s));
+    rc = failed_rows(ctx, P.status, n, bad);
+    if (rc) return rc;
+    if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "commit: %zu rows exceed the work area", bad.size());
+    return ANQI_OK;
+}
+
 int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* O,
                 const uint32_t* ocnt, uint32_t M, float alpha, uint32_t* links, uint32_t* deg) {
     if (M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "graph degree <= 64");
@@ -901,35 +936,12 @@ int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uin
     uint32_t maxin = 0;
     int rc = reverse_csr(ctx, O, n, M, ocnt, n, roff.as<uint32_t>(), rids.as<uint32_t>(), &maxin);
     if (rc) return rc;
-    HCHK(hipMemsetAsync(status.p, 0, 4ull * n, s));
     PruneParams P{};
     P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.O = O; P.ocnt = ocnt; P.roff = roff.as<uint32_t>();
     P.rids = rids.as<uint32_t>(); P.M = M; P.alpha = alpha; P.links = links; P.deg = deg;
-    P.status = status.as<uint32_t>(); P.nrows = n;
-    P.pmax = std::min<uint32_t>(2048, next_pow2((uint64_t)M + maxin + 64));
+    P.status = status.as<uint32_t>();
     HCHK(launch_fill_u32(links, U32_NONE, (uint64_t)n * M, s));   // slots past deg[i]
-    HCHK(launch_commit(P, false, s));
-    std::vector<uint32_t> bad;
-    rc = failed_rows(ctx, status.as<uint32_t>(), n, bad);
-    if (rc) return rc;
-    if (bad.empty()) return ANQI_OK;
-    PruneParams Q = P;
-    Q.pmax = next_pow2((uint64_t)M + maxin + 64);
-    size_t words = (commit_lds_bytes(Q) + 3) / 4;
-    DBuf rows, ws;
-    HCHK(rows.alloc(4 * bad.size()));
-    HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
-    HCHK(hipMemsetAsync(status.p, 0, 4ull * n, s));
-    HCHK(ws.alloc(4 * words * bad.size()));
-    Q.rows = rows.as<uint32_t>();
-    Q.nrows = (uint32_t)bad.size();
-    Q.gscratch = ws.as<uint32_t>();
-    Q.gscratch_words = words;
-    HCHK(launch_commit(Q, true, s));
-    rc = failed_rows(ctx, status.as<uint32_t>(), n, bad);
-    if (rc) return rc;
-    if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "commit: %zu rows exceed the work area", bad.size());
-    return ANQI_OK;
+    return commit_run(ctx, P, maxin);
 }
 
 }  // extern "C"
@@ -1384,10 +1396,192 @@ int anqi_aknn_build(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, const
 }
 
 // ---- graph -------------------------------------------------------------------------------------
+// ---- Vamana refinement ----------------------------------------------------------------
+// std::mt19937_64 (Nishimura 2000) and std::shuffle as libstdc++ (GCC 11) runs
+// it with a 64-bit engine over fewer than 2^32 elements: after one draw in
+// [0, 2) when the length is even, each draw in [0, b (b + 1)) places two
+// successive elements (x / (b + 1), x % (b + 1)); a draw in [0, r) is
+// Lemire's nearly-divisionless method on the 128-bit product.
+struct Mt64 {
+    uint64_t x[312];
+    uint32_t i = 312;
+    explicit Mt64(uint64_t seed) {
+        x[0] = seed;
+        for (uint32_t k = 1; k < 312; k++) x[k] = 6364136223846793005ull * (x[k - 1] ^ (x[k - 1] >> 62)) + k;
+    }
+    uint64_t next() {
+        if (i >= 312) {
+            for (uint32_t k = 0; k < 312; k++) {
+                const uint64_t y = (x[k] & 0xFFFFFFFF80000000ull) | (x[(k + 1) % 312] & 0x7FFFFFFFull);
+                x[k] = x[(k + 156) % 312] ^ (y >> 1) ^ ((y & 1u) ? 0xB5026F5AA96619E9ull : 0ull);
+            }
+            i = 0;
+        }
+        uint64_t y = x[i++];
+        y ^= (y >> 29) & 0x5555555555555555ull;
+        y ^= (y << 17) & 0x71D67FFFEDA60000ull;
+        y ^= (y << 37) & 0xFFF7EEE000000000ull;
+        return y ^ (y >> 43);
+    }
+    uint64_t below(uint64_t r) {
+        unsigned __int128 p = (unsigned __int128)next() * r;
+        uint64_t low = (uint64_t)p;
+        if (low < r) {
+            const uint64_t t = (0ull - r) % r;
+            while (low < t) {
+                p = (unsigned __int128)next() * r;
+                low = (uint64_t)p;
+            }
+        }
+        return (uint64_t)(p >> 64);
+    }
+};
+
+static void shuffle_u32(std::vector<uint32_t>& a, Mt64& g) {
+    const uint64_t n = a.size();
+    if (n == 0) return;
+    uint64_t i = 1;
+    if (n % 2 == 0) std::swap(a[i++], a[g.below(2)]);
+    while (i != n) {
+        const uint64_t b = i + 1, r = g.below(b * (b + 1));
+        std::swap(a[i++], a[r / (b + 1)]);
+        std::swap(a[i++], a[r % (b + 1)]);
+    }
+}
+
+// The gather of a Vamana collect: graph mode, beam L from the entries, pool
+// uncapped, pruned with alpha into O/ocnt (rows indexed by row id).
+static GatherParams vamana_gather_params(const float* lifted, const float* nrm, uint32_t n, uint32_t dl,
+                                         const uint32_t* hidx, const float* hval, uint32_t cap,
+                                         const anqi_graph_params* p, const uint32_t* links, const uint32_t* deg,
+                                         float alpha, uint32_t* O, uint32_t* ocnt) {
+    GatherParams G{};
+    G.x = lifted; G.nrm = nrm; G.n = n; G.d = dl; G.hidx = hidx; G.hval = hval; G.cap = cap;
+    G.explore = p->vamana_L ? p->vamana_L : 128;
+    G.pool_cap = UINT32_MAX;
+    G.M = p->M; G.alpha = alpha; G.out_ids = O; G.out_cnt = ocnt;
+    G.batched = 1;
+    G.vlinks = links; G.vdeg = deg; G.vM = p->M;
+    return G;
+}
+
+// ANQI_VAMANA after the commit (include/index.h:1262-1282): global passes
+// (collect from each row's tree entry, heap slots, prune, commit_edges) or
+// the batch insertion of vamana_batch_refine (:932-1063).
+static int vamana_refine(anqi_ctx* ctx, const float* lifted, const float* nrm, uint32_t n, uint32_t dl,
+                         const uint32_t* hidx, const float* hval, uint32_t cap, const anqi_graph_params* p,
+                         const anqi_tree* tree, uint32_t* links, uint32_t* deg) {
+    hipStream_t s = ctx->stream;
+    const uint32_t M = p->M;
+    const int passes = p->vamana_passes;
+    if (n == 0 || passes <= 0) return ANQI_OK;
+    DBuf O, ocnt;
+    HCHK(O.alloc(4ull * n * M));
+    HCHK(ocnt.alloc(4ull * n));
+    if (p->vamana == 1) {
+        DBuf ent;
+        HCHK(ent.alloc(4ull * n));
+        HCHK(launch_tree_entry_rows(lifted, n, dl, tree->left, tree->right, tree->offset, tree->hyper, tree->entry,
+                                    tree->root, ent.as<uint32_t>(), s));
+        for (int pass = 0; pass < passes; pass++) {
+            GatherParams G = vamana_gather_params(lifted, nrm, n, dl, hidx, hval, cap, p, links, deg, p->alpha,
+                                                  O.as<uint32_t>(), ocnt.as<uint32_t>());
+            G.vmode = 1;
+            G.ventry = ent.as<uint32_t>();
+            int rc = gather_run(ctx, G, true, 0, nullptr);
+            if (rc) return rc;
+            rc = anqi_commit(ctx, lifted, nrm, n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), M, p->alpha, links, deg);
+            if (rc) return rc;
+            progress("vamana global pass %d", pass);
+        }
+        return ANQI_OK;
+    }
+    // batch style
+    double frac = p->vamana_batch_frac > 0.0 ? p->vamana_batch_frac : 0.02;
+    uint64_t max_batch = std::min<uint64_t>((uint64_t)std::max(1.0, frac * (double)n), p->vamana_max_batch);
+    if (max_batch == 0) max_batch = n;
+    const double base = p->vamana_batch_base > 1.0 ? p->vamana_batch_base : 2.0;
+    const uint32_t entry = p->vamana_entry < n ? p->vamana_entry : 0;
+    const uint64_t maxb = n;   // a batch never exceeds the rows (base^k (base - 1) may exceed max_batch)
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    DBuf dorder, Ob, obcnt, roff, rids, tl, tc, status;
+    HCHK(dorder.alloc(4ull * n));
+    HCHK(Ob.alloc(4ull * maxb * M));
+    HCHK(obcnt.alloc(4ull * maxb));
+    HCHK(roff.alloc(4ull * (n + 1)));
+    HCHK(rids.alloc(4ull * maxb * M));
+    HCHK(tl.alloc(4ull * n));
+    HCHK(tc.alloc(4));
+    HCHK(status.alloc(4ull * n));
+    for (int pass = 0; pass < passes; pass++) {
+        const float alpha = (p->vamana_early_alpha_one && pass + 1 < passes) ? p->vamana_early_alpha : p->alpha;
+        Mt64 g(p->seed + 0x9e3779b97f4a7c15ull * (uint64_t)(pass + 1));
+        shuffle_u32(order, g);
+        HCHK(hipMemcpyAsync(dorder.p, order.data(), 4ull * n, hipMemcpyHostToDevice, s));
+        uint64_t count = 0, inc = 0;
+        while (count < n) {
+            uint64_t fl = count, ce;
+            const double p0 = std::pow(base, (double)inc);
+            if (p0 <= (double)max_batch) {
+                fl = (uint64_t)p0 - 1;
+                ce = std::min<uint64_t>((uint64_t)std::pow(base, (double)(inc + 1)) - 1, n);
+            } else {
+                ce = std::min<uint64_t>(count + max_batch, n);
+            }
+            count = ce;
+            inc++;
+            if (ce <= fl || fl >= n) continue;
+            const uint32_t nb = (uint32_t)(ce - fl);
+            const uint32_t* brows = dorder.as<uint32_t>() + fl;
+            GatherParams G = vamana_gather_params(lifted, nrm, n, dl, hidx, hval, cap, p, links, deg, alpha,
+                                                  O.as<uint32_t>(), ocnt.as<uint32_t>());
+            G.vmode = 2;
+            G.ventry0 = entry;
+            G.rows = brows;
+            G.nrows = nb;
+            int rc = gather_run(ctx, G, true, 0, nullptr);
+            if (rc) return rc;
+            HCHK(launch_vamana_write(brows, nb, O.as<uint32_t>(), ocnt.as<uint32_t>(), M, links, deg,
+                                     Ob.as<uint32_t>(), obcnt.as<uint32_t>(), s));
+            uint32_t maxin = 0;
+            rc = reverse_csr(ctx, Ob.as<uint32_t>(), nb, M, obcnt.as<uint32_t>(), n, roff.as<uint32_t>(),
+                             rids.as<uint32_t>(), &maxin);
+            if (rc) return rc;
+            HCHK(launch_nonempty_rows(roff.as<uint32_t>(), n, tl.as<uint32_t>(), tc.as<uint32_t>(), s));
+            uint32_t nt = 0;
+            HCHK(hipMemcpyAsync(&nt, tc.p, 4, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+            if (nt == 0) continue;
+            // the rows the batch points to: their new edges plus the batch rows
+            // pointing to them, pruned in place
+            PruneParams P{};
+            P.x = lifted; P.nrm = nrm; P.n = n; P.d = dl; P.O = links; P.ocnt = deg;
+            P.roff = roff.as<uint32_t>(); P.rids = rids.as<uint32_t>(); P.rmap = brows;
+            P.M = M; P.alpha = alpha; P.links = links; P.deg = deg; P.status = status.as<uint32_t>();
+            P.rows = tl.as<uint32_t>(); P.nrows = nt;
+            rc = commit_run(ctx, P, maxin);
+            if (rc) return rc;
+        }
+        progress("vamana batch pass %d", pass);
+    }
+    return ANQI_OK;
+}
+
+int anqi_vamana_medoid(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint64_t sample, uint32_t* entry) {
+    if (!ctx || !x || !entry || n == 0 || d == 0) return anqi_set_err(ctx, ANQI_E_USAGE, "vamana medoid: need rows");
+    DBuf mean, best;
+    HCHK(mean.alloc(8ull * d));
+    HCHK(best.alloc(16));
+    HCHK(launch_vamana_medoid(x, n, d, sample, mean.as<double>(), best.as<unsigned long long>(), entry, ctx->stream));
+    return ANQI_OK;
+}
+
 int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl, const uint32_t* warm_ids,
                      uint32_t width, const anqi_graph_params* p, uint32_t* links, uint32_t* deg, anqi_tree* tree,
                      anqi_graph_stats* st) {
-    if (!p || p->M == 0 || p->M > 64 || p->nn_k == 0 || p->nn_k > 256)
+    if (!p || p->M == 0 || p->M > 64 || p->nn_k == 0 || p->nn_k > 256 || p->vamana < 0 || p->vamana > 2 ||
+        (p->vamana && (p->vamana_L == 0 || p->vamana_L > 1024)))
         return anqi_set_err(ctx, ANQI_E_USAGE, "invalid graph parameters");
     hipStream_t s = ctx->stream;
     anqi_graph_stats S{};
@@ -1443,6 +1637,14 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     HCHK(hipStreamSynchronize(s));
     double t5 = now_s();
     S.commit_s = t5 - t4;
+    if (p->vamana) {
+        rc = vamana_refine(ctx, lifted, nrm.as<float>(), n, dl, hidx.as<uint32_t>(), hval.as<float>(), cap, p, tree,
+                           links, deg);
+        if (rc) return rc;
+        HCHK(hipStreamSynchronize(s));
+        S.vamana_s = now_s() - t5;
+        t5 = now_s();
+    }
     S.total_s = t5 - t0;
     if (st) *st = S;
     return ANQI_OK;

@@ -770,15 +770,48 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     };
     const uint32_t tcap = gather_tcap(P, GRAPH);
 
-    // seeds: the row's own slots in storage order, with stored values
-    for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
-        uint32_t s = s0 + lane;
-        uint32_t id = s < P.cap ? P.hidx[(size_t)i * P.cap + s] : U32_NONE;
-        float dv = s < P.cap ? P.hval[(size_t)i * P.cap + s] : F32_BIG;
-        if (visit(id, id != U32_NONE && id < P.n, dv)) process(true);
+    if (GRAPH && P.vmode) {
+        // Vamana collect (index.h:831-880): the beam starts at the entry,
+        // whose distance is computed like every later visit's
+        const uint32_t ep = P.ventry ? P.ventry[i] : P.ventry0;
+        if (visit(lane == 0 ? ep : U32_NONE, lane == 0 && ep < P.n, 0.0f)) process(false);
+    } else {
+        // seeds: the row's own slots in storage order, with stored values
+        for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
+            uint32_t s = s0 + lane;
+            uint32_t id = s < P.cap ? P.hidx[(size_t)i * P.cap + s] : U32_NONE;
+            float dv = s < P.cap ? P.hval[(size_t)i * P.cap + s] : F32_BIG;
+            if (visit(id, id != U32_NONE && id < P.n, dv)) process(true);
+        }
     }
-    // beam over the neighbour heaps and their reverse lists
-    if constexpr (QCH > 0) {
+    if (GRAPH && P.vmode) {
+        // the beam expands rows of the fixed-degree graph (one neighbour per
+        // lane, M <= 64) until no entry is left unexpanded
+        while (!status) {
+            uint32_t c;
+            if constexpr (QCH > 0) {
+                c = rk_take(Q, true);
+                if (c == U32_NONE) break;
+            } else {
+                uint32_t cur = U32_NONE;
+                for (uint32_t b = 0; b < qsize; b += 64) {
+                    uint32_t t = b + lane;
+                    uint64_t m = ballot(t < qsize && !(S.qp(qb)[t] >> 31));
+                    if (m) { cur = b + (uint32_t)(__ffsll((long long)m) - 1); break; }
+                }
+                if (cur == U32_NONE) break;
+                c = S.qp(qb)[cur];
+                __syncthreads();
+                if (lane == 0) S.qp(qb)[cur] = c | 0x80000000u;
+                __syncthreads();
+            }
+            const uint32_t dg = P.vdeg[c];
+            const uint32_t id = (uint32_t)lane < dg ? P.vlinks[(size_t)c * P.vM + lane] : U32_NONE;
+            nadj += dg;
+            if (visit(id, id != U32_NONE && id < P.n, 0.0f)) process(false);
+        }
+    } else if constexpr (QCH > 0) {
+        // beam over the neighbour heaps and their reverse lists
         // Register-list form.  An expansion's adjacency (heap slots, the
         // reverse-list bounds and its first 64 entries) is requested at once,
         // and that of the entry next in line once c is marked (pc) is
@@ -860,6 +893,23 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         // expansion's first visits together (in lists of up to tcap)
         if (!status) process(false);
         if (pcount >= P.pool_cap) break;
+    }
+    if (GRAPH && P.vmode && !status) {
+        // append_aknn_edges (index.h:766-769): the row's heap slots, whose
+        // stored distances are the d2 the heap took them with; vmode 2 also
+        // append_current_edges (:757-764).  Through the visited set: an id
+        // already in the pool carries the same distance there.
+        for (uint32_t s0 = 0; s0 < P.cap && !status; s0 += 64) {
+            const uint32_t s = s0 + lane;
+            const uint32_t id = s < P.cap ? P.hidx[(size_t)i * P.cap + s] : U32_NONE;
+            const float dv = s < P.cap ? P.hval[(size_t)i * P.cap + s] : F32_BIG;
+            if (visit(id, id != U32_NONE && id < P.n && id != i, dv)) process(true);
+        }
+        if (P.vmode == 2 && !status) {
+            const uint32_t dg = P.vdeg[i];
+            const uint32_t id = (uint32_t)lane < dg ? P.vlinks[(size_t)i * P.vM + lane] : U32_NONE;
+            if (visit(id, id != U32_NONE && id < P.n && id != i, 0.0f)) process(false);
+        }
     }
     if (status) {
         if (lane == 0) { P.status[i] = (uint32_t)status; P.out_cnt[i] = 0; }
@@ -992,9 +1042,12 @@ __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
     uint32_t m = no + (r1 - r0);
     uint32_t cap2 = 64;
     while (cap2 < m) cap2 <<= 1;
-    if (cap2 > P.pmax) { if (lane == 0) { P.status[i] = 2; P.deg[i] = 0; } return; }
-    for (uint32_t t = lane; t < m; t += 64)
-        pi[t] = t < no ? P.O[(size_t)i * P.M + t] : P.rids[r0 + (t - no)];
+    // on overflow the row keeps its degree: O may be the links themselves
+    if (cap2 > P.pmax) { if (lane == 0) P.status[i] = 2; return; }
+    for (uint32_t t = lane; t < m; t += 64) {
+        const uint32_t r = t < no ? 0u : P.rids[r0 + (t - no)];
+        pi[t] = t < no ? P.O[(size_t)i * P.M + t] : (P.rmap ? P.rmap[r] : r);
+    }
     __syncthreads();
     QRow QI;
     qrow_load(QI, P.x + (size_t)i * d, d, l);
@@ -1027,6 +1080,7 @@ __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
     GatherParams G{};
     G.x = P.x; G.nrm = P.nrm; G.n = P.n; G.d = P.d; G.M = P.M; G.alpha = P.alpha;
     uint32_t dg = rng_select(G, pd, pi, w, selv, sel, P.links + (size_t)i * P.M);
+    for (uint32_t t = dg + (uint32_t)lane; t < P.M; t += 64) P.links[(size_t)i * P.M + t] = U32_NONE;
     if (lane == 0) { P.deg[i] = dg; P.status[i] = 0; }
 }
 
@@ -1049,4 +1103,140 @@ hipError_t launch_commit(const PruneParams& P, bool gws, hipStream_t s) {
         case 6: return launch_commit_mc<6>(P, gws, s);
         default: return launch_commit_mc<0>(P, gws, s);
     }
+}
+
+// ---- Vamana refinement helpers (include/index.h:190-222, 831-1065) ---------------------
+
+// The search-tree entry of each row (rptrees.h:108-123, ties go left), the
+// row itself descending: four rows per wavefront, one per 16-lane group.
+__global__ void __launch_bounds__(64) k_tree_entry_rows(const float* __restrict__ x, uint32_t n, uint32_t dl,
+                                                        const uint32_t* __restrict__ left,
+                                                        const uint32_t* __restrict__ right,
+                                                        const float* __restrict__ off,
+                                                        const float* __restrict__ hyper,
+                                                        const uint32_t* __restrict__ entry, uint32_t root,
+                                                        uint32_t* __restrict__ out) {
+    const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
+    const uint32_t i = blockIdx.x * 4u + (uint32_t)g;
+    if (i >= n) return;                         // whole 16-lane groups
+    QRow QL;
+    qrow_load(QL, x + (size_t)i * dl, (int)dl, l);
+    uint32_t node = root;
+    for (;;) {
+        const uint32_t lc = left[node], rc = right[node];
+        const float ip = qrow_ip(QL, hyper + (size_t)node * dl, (int)dl, l);
+        if (lc == U32_NONE) break;
+        const float mg = ip - off[node];
+        node = (mg < -1e-6f) ? lc : (mg > 1e-6f ? rc : lc);
+    }
+    if (l == 0) out[i] = entry[node];
+}
+
+hipError_t launch_tree_entry_rows(const float* x, uint32_t n, uint32_t dl, const uint32_t* left,
+                                  const uint32_t* right, const float* off, const float* hyper,
+                                  const uint32_t* entry, uint32_t root, uint32_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tree_entry_rows, dim3((n + 3) / 4), dim3(64), 0, s, x, n, dl, left, right, off, hyper,
+                       entry, root, out);
+    return hipGetLastError();
+}
+
+// write_ids (index.h:897-903) for a batch, one wavefront per row.
+__global__ void __launch_bounds__(64) k_vamana_write(const uint32_t* __restrict__ rows, uint32_t nb,
+                                                     const uint32_t* __restrict__ O,
+                                                     const uint32_t* __restrict__ ocnt, uint32_t M,
+                                                     uint32_t* __restrict__ links, uint32_t* __restrict__ deg,
+                                                     uint32_t* __restrict__ Ob, uint32_t* __restrict__ obcnt) {
+    const uint32_t b = blockIdx.x, lane = (uint32_t)__lane_id();
+    if (b >= nb) return;
+    const uint32_t node = rows[b], c = ocnt[node];
+    for (uint32_t t = lane; t < M; t += 64) {
+        const uint32_t v = t < c ? O[(size_t)node * M + t] : U32_NONE;
+        links[(size_t)node * M + t] = v;
+        Ob[(size_t)b * M + t] = v;
+    }
+    if (lane == 0) { deg[node] = c; obcnt[b] = c; }
+}
+
+hipError_t launch_vamana_write(const uint32_t* rows, uint32_t nb, const uint32_t* O, const uint32_t* ocnt,
+                               uint32_t M, uint32_t* links, uint32_t* deg, uint32_t* Ob, uint32_t* obcnt,
+                               hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vamana_write, dim3(nb), dim3(64), 0, s, rows, nb, O, ocnt, M, links, deg, Ob, obcnt);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_nonempty_rows(const uint32_t* __restrict__ roff, uint32_t n,
+                                                       uint32_t* __restrict__ list, uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int has = i < n && roff[i + 1] > roff[i];
+    const uint64_t m = ballot(has);
+    uint32_t base = 0;
+    if (__lane_id() == 0 && m) base = atomicAdd(cnt, (uint32_t)popc64(m));
+    base = __shfl(base, 0);
+    if (has) list[base + lanes_below(m)] = i;
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+hipError_t launch_nonempty_rows(const uint32_t* roff, uint32_t n, uint32_t* list, uint32_t* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, cnt);
+    if (n) hipLaunchKernelGGL(k_nonempty_rows, dim3((n + 255) / 256), dim3(256), 0, s, roff, n, list, cnt);
+    return hipGetLastError();
+}
+
+// choose_vamana_entry's medoid: sampled rows (t step) % n.  The mean of each
+// coordinate is summed in sample order in binary64, one thread per
+// coordinate; each sampled row's squared distance to it is a fused binary64
+// sum in coordinate order (the reference's loop as GCC contracts it).
+__global__ void __launch_bounds__(64) k_vamana_mean(const float* __restrict__ x, uint32_t n, uint32_t dl,
+                                                    uint64_t sample, uint64_t step, double* __restrict__ mean) {
+    const uint32_t j = blockIdx.x * 64u + (uint32_t)__lane_id();
+    if (j >= dl) return;
+    double acc = 0.0;
+    for (uint64_t t = 0; t < sample; t++) acc += (double)x[((t * step) % n) * dl + j];
+    mean[j] = acc / (double)sample;
+}
+
+__device__ __forceinline__ double vamana_dd(const float* r, const double* mean, uint32_t dl) {
+    double dd = 0.0;
+    for (uint32_t j = 0; j < dl; j++) {
+        const double z = (double)r[j] - mean[j];
+        dd = fma(z, z, dd);
+    }
+    return dd;
+}
+
+// pass 0: best[0] = least distance (bit patterns of non-negative doubles
+// order as integers); pass 1: best[1] = least t attaining it.
+__global__ void __launch_bounds__(256) k_vamana_medoid(const float* __restrict__ x, uint32_t n, uint32_t dl,
+                                                       uint64_t sample, uint64_t step,
+                                                       const double* __restrict__ mean,
+                                                       unsigned long long* __restrict__ best, int pass) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= sample) return;
+    const double dd = vamana_dd(x + ((t * step) % n) * dl, mean, dl);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(dd);
+    if (pass == 0) atomicMin(best, b);
+    else if (b == best[0]) atomicMin(best + 1, (unsigned long long)t);
+}
+
+__global__ void k_vamana_medoid_init(unsigned long long* best) { best[0] = ~0ull; best[1] = ~0ull; }
+
+hipError_t launch_vamana_medoid(const float* x, uint32_t n, uint32_t dl, uint64_t sample, double* mean,
+                                unsigned long long* best, uint32_t* out, hipStream_t s) {
+    if (n == 0) return hipErrorInvalidValue;
+    if (sample < 1) sample = 1;
+    if (sample > n) sample = n;
+    const uint64_t step = n / sample >= 1 ? n / sample : 1;
+    hipLaunchKernelGGL(k_vamana_mean, dim3((dl + 63) / 64), dim3(64), 0, s, x, n, dl, sample, step, mean);
+    hipLaunchKernelGGL(k_vamana_medoid_init, dim3(1), dim3(1), 0, s, best);
+    const dim3 grid((uint32_t)((sample + 255) / 256));
+    hipLaunchKernelGGL(k_vamana_medoid, grid, dim3(256), 0, s, x, n, dl, sample, step, mean, best, 0);
+    hipLaunchKernelGGL(k_vamana_medoid, grid, dim3(256), 0, s, x, n, dl, sample, step, mean, best, 1);
+    unsigned long long bt = 0;
+    if (hipError_t e = hipMemcpyAsync(&bt, best + 1, 8, hipMemcpyDeviceToHost, s); e != hipSuccess) return e;
+    if (hipError_t e = hipStreamSynchronize(s); e != hipSuccess) return e;
+    *out = (uint32_t)((bt * step) % n);
+    return hipGetLastError();
 }

@@ -207,6 +207,16 @@ struct GatherParams {
                             // 2 n <= 2^24 (with a 32-bit overflow table); 0: 32-bit entries
     unsigned long long* work;  // optional: [0] += distances computed, [1] += adjacency ids read (finished rows)
     int ipm;                // AKNN "ip" metric: distance 1 - <x_i, x_c> (ipdist_seq), one lane per pair
+    // Vamana collect (graph mode): vmode 1 -- the beam starts at ventry[i]
+    // (or ventry0) and expands the fixed-degree graph vlinks/vdeg (row width
+    // vM) instead of the heaps; the row's heap slots join the pool after the
+    // beam; vmode 2 -- its current edges as well
+    int vmode;
+    const uint32_t* vlinks;
+    const uint32_t* vdeg;
+    uint32_t vM;
+    const uint32_t* ventry;
+    uint32_t ventry0;
 };
 // rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
 bool gather_register_lists(const GatherParams& P, bool graph_mode);
@@ -231,9 +241,29 @@ struct PruneParams {
     uint32_t nrows;
     uint32_t* gscratch;
     size_t gscratch_words;
+    const uint32_t* rmap;   // optional: rids hold indices into rmap (the ids are rmap[r])
 };
 size_t commit_lds_bytes(const PruneParams& P);
 hipError_t launch_commit(const PruneParams& P, bool global_ws, hipStream_t s);
+
+// ---- Vamana refinement (k_graph.hip) ---------------------------------------------
+// Search-tree entry of each row of x (n x dl; the rows themselves descend).
+hipError_t launch_tree_entry_rows(const float* x, uint32_t n, uint32_t dl, const uint32_t* left,
+                                  const uint32_t* right, const float* off, const float* hyper,
+                                  const uint32_t* entry, uint32_t root, uint32_t* out, hipStream_t s);
+// write_ids for a batch: links/deg rows of rows[0..nb) from O/ocnt (n x M,
+// indexed by row id), slots past the degree UINT32_MAX; the batch's rows also
+// compacted into Ob/obcnt (nb x M) for the reverse lists.
+hipError_t launch_vamana_write(const uint32_t* rows, uint32_t nb, const uint32_t* O, const uint32_t* ocnt,
+                               uint32_t M, uint32_t* links, uint32_t* deg, uint32_t* Ob, uint32_t* obcnt,
+                               hipStream_t s);
+// rows i with roff[i+1] > roff[i] into list (order unspecified), count in *cnt (zeroed here).
+hipError_t launch_nonempty_rows(const uint32_t* roff, uint32_t n, uint32_t* list, uint32_t* cnt, hipStream_t s);
+// choose_vamana_entry's sampled medoid: mean (binary64, sequential per
+// coordinate) of rows (t step) % n, t < sample, then the first sampled row
+// of least fused binary64 squared distance.  work: dl doubles + 2 words.
+hipError_t launch_vamana_medoid(const float* x, uint32_t n, uint32_t dl, uint64_t sample, double* mean,
+                                unsigned long long* best, uint32_t* out, hipStream_t s);
 
 // ---- RP forest + leaf brute force (k_forest.hip) --------------------------------
 struct ForestLevel {

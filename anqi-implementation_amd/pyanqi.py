@@ -41,13 +41,19 @@ class Tree(ctypes.Structure):
 
 class GraphParams(ctypes.Structure):
     _fields_ = [("M", U32), ("nn_k", U32), ("leaf", U32), ("explore", U32), ("ef_alpha", U32),
-                ("iters", I32), ("sample", I32), ("alpha", F32), ("seed", U64), ("form", I32)]
+                ("iters", I32), ("sample", I32), ("alpha", F32), ("seed", U64), ("form", I32),
+                ("vamana", I32), ("vamana_L", U32), ("vamana_passes", I32), ("vamana_batch_frac", F64),
+                ("vamana_max_batch", U64), ("vamana_batch_base", F64), ("vamana_early_alpha", F32),
+                ("vamana_early_alpha_one", I32), ("vamana_entry", U32)]
+
+
+VAMANA_STYLES = {"global": 1, "batch": 2}
 
 
 class GraphStats(ctypes.Structure):
     _fields_ = [("tree_s", F64), ("warm_s", F64), ("nnd_s", F64), ("gather_s", F64), ("commit_s", F64),
                 ("total_s", F64), ("nnd_passes", I32), ("nnd_upd", ctypes.c_int64 * 16),
-                ("overflow_rows", U32), ("gather_d2", U64), ("gather_adj", U64)]
+                ("overflow_rows", U32), ("gather_d2", U64), ("gather_adj", U64), ("vamana_s", F64)]
 
 
 class SearchTuning(ctypes.Structure):
@@ -115,6 +121,7 @@ SIGS = {
     "anqi_rp_tree": (I32, [P, P, U32, U32, U32, U32, U64, ctypes.POINTER(Tree)]),
     "anqi_tree_upload": (I32, [P, U32, U32, U32, P, P, P, P, P, ctypes.POINTER(Tree)]),
     "anqi_tree_free": (I32, [P, ctypes.POINTER(Tree)]),
+    "anqi_vamana_medoid": (I32, [P, P, U32, U32, U64, P]),
     "anqi_graph_build": (I32, [P, P, U32, U32, P, U32, ctypes.POINTER(GraphParams), P, P, ctypes.POINTER(Tree),
                                ctypes.POINTER(GraphStats)]),
     "anqi_rknn_search": (I32, [P, ctypes.POINTER(Index), P, U32, U32, P, U32, P, P, U32, P, P, P, P, P, P, P, I32]),
@@ -443,16 +450,29 @@ class Context:
         self.call("anqi_tree_free", ctypes.byref(t))
 
     def graph_build(self, lifted, warm_ids, M=48, nn_k=50, leaf=100, explore=100, ef_alpha=8, iters=6, sample=20,
-                    alpha=1.2, seed=10, form=0):
+                    alpha=1.2, seed=10, form=0, vamana=None, vamana_L=128, vamana_passes=2, vamana_batch_frac=0.02,
+                    vamana_max_batch=1000000, vamana_batch_base=2.0, vamana_early_alpha=1.0,
+                    vamana_early_alpha_one=1, vamana_entry=0):
+        """anqi_graph_build; vamana: None, "global" or "batch" (ANQI_VAMANA_STYLE);
+        iters=0 skips the lifted LocalJoin (ANQI_REUSE_ORIG_KNN)."""
         n, dl = lifted.shape
         links = self.empty((n, M), self.torch.int32)
         deg = self.empty((n,), self.torch.int32)
         t = Tree()
         st = GraphStats()
-        p = GraphParams(M, nn_k, leaf, explore, ef_alpha, iters, sample, alpha, seed, form)
+        p = GraphParams(M, nn_k, leaf, explore, ef_alpha, iters, sample, alpha, seed, form,
+                        VAMANA_STYLES[vamana] if vamana else 0, vamana_L, vamana_passes, vamana_batch_frac,
+                        vamana_max_batch, vamana_batch_base, vamana_early_alpha, vamana_early_alpha_one, vamana_entry)
         self.call("anqi_graph_build", dp(lifted), n, dl, dp(warm_ids), warm_ids.shape[1], ctypes.byref(p),
                   dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
         return links, deg, t, st
+
+    def vamana_medoid(self, x, sample=None):
+        """choose_vamana_entry "medoid" over device rows (anqi_vamana_medoid)."""
+        n, d = x.shape
+        e = U32()
+        self.call("anqi_vamana_medoid", dp(x), n, d, min(n, 100000) if sample is None else sample, ctypes.byref(e))
+        return e.value
 
     def lift(self, base, rk_horizon, metric=METRIC_L2):
         """Lift of device base rows on the device (anqi_lift_metric): (lifted, mstar2, scale)."""

@@ -277,6 +277,23 @@ typedef struct {
     uint64_t seed;     /* 10 */
     int form;          /* 0 plain (per-chunk gather), 1 chosen (per-expansion
                           batched gather); both give the same graph */
+    /* Vamana refinement after the commit (include/index.h:1262-1282; zero:
+     * off).  1: ANQI_VAMANA_STYLE=global -- each pass collects every row's
+     * beam from its own tree entry over the current graph, adds its heap
+     * slots, prunes and commits (:1267-1279); 2: "batch" --
+     * vamana_batch_refine (:932-1063): rows in a shuffled order, in growing
+     * batches from one entry, each batch's rows and the rows they point to
+     * re-pruned before the next batch. */
+    int vamana;
+    uint32_t vamana_L;            /* ANQI_VAMANA_L, 128; beam width <= 1024 */
+    int vamana_passes;            /* ANQI_VAMANA_PASS, 2 */
+    double vamana_batch_frac;     /* ANQI_VAMANA_BATCH_FRAC, 0.02 */
+    uint64_t vamana_max_batch;    /* ANQI_VAMANA_MAX_BATCH, 1000000 */
+    double vamana_batch_base;     /* ANQI_VAMANA_BATCH_BASE, 2 */
+    float vamana_early_alpha;     /* ANQI_VAMANA_EARLY_ALPHA, 1.0: the prune alpha before the last pass */
+    int vamana_early_alpha_one;   /* ANQI_VAMANA_EARLY_ALPHA_ONE, 1 */
+    uint32_t vamana_entry;        /* the batch style's entry (choose_vamana_entry: 0, an id
+                                     mod n, or anqi_vamana_medoid's row) */
 } anqi_graph_params;
 typedef struct {
     double tree_s, warm_s, nnd_s, gather_s, commit_s, total_s;
@@ -284,7 +301,13 @@ typedef struct {
     int64_t nnd_upd[16];
     uint32_t overflow_rows;
     uint64_t gather_d2, gather_adj;  /* gather work: distances computed, adjacency ids read */
+    double vamana_s;                 /* Vamana passes */
 } anqi_graph_stats;
+/* choose_vamana_entry "medoid" (include/index.h:190-222): the sampled row
+ * (rows (t * step) % n, step = max(1, n / sample), t < sample, sample
+ * clamped to [1, n]) nearest, in binary64, to the sample's binary64 mean.
+ * x: device rows (the lifted rows for the graph). */
+int anqi_vamana_medoid(anqi_ctx* ctx, const float* x, uint32_t n, uint32_t d, uint64_t sample, uint32_t* entry);
 /* device lifted (n*dl), warm_ids (n*width); device links (n*M), deg (n);
  * link slots past deg[i] hold UINT32_MAX. */
 int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl,
