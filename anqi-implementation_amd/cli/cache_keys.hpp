@@ -89,6 +89,20 @@ inline std::string edge_suffix(const char* sep) {
            s + env_or("ANQI_EDGE_RESID_LAMBDA", "1.0") + "_ebo" + s + env_or("ANQI_EDGE_BALL_OCCLUSION", "all");
 }
 
+// The Vamana part of the graph key (rknn_query.cpp:325-340): the style and
+// parameters as the environment spells them, "_vam0" when it is off.
+inline std::string vamana_suffix() {
+    if (!env_set("ANQI_VAMANA")) return "_vam0";
+    const std::string style = env_or("ANQI_VAMANA_STYLE", "global");
+    std::string k = "_vam1_style" + style + "_L" + env_or("ANQI_VAMANA_L", "128") + "_P" +
+                    env_or("ANQI_VAMANA_PASS", "2") + "_bf" + env_or("ANQI_VAMANA_BATCH_FRAC", "0.02") + "_mb" +
+                    env_or("ANQI_VAMANA_MAX_BATCH", "1000000") + "_bb" + env_or("ANQI_VAMANA_BATCH_BASE", "2") +
+                    "_ea" + env_or("ANQI_VAMANA_EARLY_ALPHA", "1.0") + "_eao" +
+                    env_or("ANQI_VAMANA_EARLY_ALPHA_ONE", "1") + "_entry" + env_or("ANQI_VAMANA_ENTRY", "0");
+    if (style == "batch") k += "_implcsr1";
+    return k;
+}
+
 inline CacheNames cache_names(const CacheInputs& in) {
     CacheNames c;
     c.threshold_source = threshold_source_identity(in);
@@ -108,7 +122,7 @@ inline CacheNames cache_names(const CacheInputs& in) {
     std::string g = "M" + std::to_string(in.M) + "_leaf" + std::to_string(in.leaf) + "_nt" + std::to_string(in.trees) +
                     "_ef" + std::to_string(in.ef) + "_geomanyk_lift_rankm0_specific0_alpha" + env_or("ANQI_ALPHA", "1.2") +
                     "_liftnnk" + env_or("ANQI_LIFTNNK", "50") + "_graphnnd" + env_or("ANQI_GRAPHNND", "6") +
-                    edge_suffix("") + "_vam0";
+                    (env_set("ANQI_REUSE_ORIG_KNN") ? "_reuseorig1" : "") + edge_suffix("") + vamana_suffix();
     g = sanitize_tag(g);
     // the post-filter verifier's graph is keyed by its cutoff source instead
     // of the radius key
@@ -133,7 +147,8 @@ inline CacheNames cache_names(const CacheInputs& in) {
     const std::string cfg = "fd_metric_l2_M_" + std::to_string(in.M) + "_nnk_" + std::to_string(in.lift_nnk) +
                             "_ntrees_64_n_" + std::to_string(in.n) + "_dim_" + std::to_string(in.dim) + "_efa_" +
                             std::to_string(8.0f) + "_pa_" + std::to_string(prune_alpha) + "_er_100_graphnnd_" +
-                            env_or("ANQI_GRAPHNND", "6") + "_gk_" + hash16(c.graph_key) + edge_suffix("_");
+                            env_or("ANQI_GRAPHNND", "6") + "_gk_" + hash16(c.graph_key) + edge_suffix("_") +
+                            (env_set("ANQI_REUSE_ORIG_KNN") ? "_reuseorig1" : "");
     c.graph_index = in.graph_dir + sanitize_tag(cfg);
     return c;
 }

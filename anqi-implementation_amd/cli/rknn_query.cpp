@@ -124,8 +124,17 @@ const std::vector<Pin>& pins() {
 }
 // Switched on by their presence alone, whatever the value (the reference
 // tests them with getenv alone), or by any non-empty value.
-const char* const kPresence[] = {"ANQI_VAMANA", "ANQI_REUSE_ORIG_KNN", "ANQI_EDGE_RESID_FILE",
-                                 "ANQI_QUERY_SINGLE_ENTRY", "ANQI_VAMANA_QUERY_ENTRY"};
+const char* const kPresence[] = {"ANQI_EDGE_RESID_FILE"};
+
+// A set variable is read with atoi / atof, even when empty (index.h:178-186).
+int envi(const char* k, int d) {
+    const char* v = std::getenv(k);
+    return v ? std::atoi(v) : d;
+}
+double envd(const char* k, double d) {
+    const char* v = std::getenv(k);
+    return v ? std::atof(v) : d;
+}
 const char* const kNonEmpty[] = {"ANQI_GRAPH_RADIUS_SOURCE_PFX"};
 
 // Base vectors: the fbin, or ANQI_BASE_U8BIN's uint8 file read as floats
@@ -227,6 +236,13 @@ void check_configuration() {
         const char* v = std::getenv(name);
         if (v && *v) {
             std::fprintf(stderr, "[rknn] %s is not supported by the GPU build\n", name);
+            std::exit(2);
+        }
+    }
+    if (std::getenv("ANQI_VAMANA")) {
+        const int L = envi("ANQI_VAMANA_L", 128);
+        if (L <= 0 || L > 1024) {
+            std::fprintf(stderr, "[rknn] ANQI_VAMANA_L=%d is not supported by the GPU build (1..1024)\n", L);
             std::exit(2);
         }
     }
@@ -698,6 +714,23 @@ int main(int argc, char** argv) {
         std::printf("[rknn] graph degree M=%zu\n", Marg);
         std::printf("Index config: %s\n", index_name.c_str());
     }
+    // choose_vamana_entry (index.h:190-222) over the lifted rows: the Vamana
+    // batch style's entry and, with ANQI_QUERY_SINGLE_ENTRY or
+    // ANQI_VAMANA_QUERY_ENTRY, every query's (:1606-1607)
+    const bool single_entry = std::getenv("ANQI_QUERY_SINGLE_ENTRY") || std::getenv("ANQI_VAMANA_QUERY_ENTRY");
+    uint32_t graph_entry = 0;
+    if (single_entry || std::getenv("ANQI_VAMANA")) {
+        const char* ev = std::getenv("ANQI_VAMANA_ENTRY");
+        const std::string sv = ev ? ev : "";
+        if (sv == "medoid" || sv == "centroid") {
+            const uint64_t sample = (uint64_t)(size_t)envi("ANQI_VAMANA_ENTRY_SAMPLE", (int)std::min<uint64_t>(n, 100000));
+            ck(anqi_vamana_medoid(g_ctx, d_lifted, n, dl, sample, &graph_entry), "vamana entry");
+        } else if (!sv.empty()) {
+            char* endp = nullptr;
+            const unsigned long long id = std::strtoull(ev, &endp, 10);
+            graph_entry = (endp != ev && n > 0) ? (uint32_t)(id % n) : 0;
+        }
+    }
     if (!graph_loaded) {
         // graph + search tree (index.h:1071-1295), saved with its marker (:1293;
         // rknn_query.cpp:2297)
@@ -743,7 +776,32 @@ int main(int argc, char** argv) {
                         rec_p.c_str());
         }
         const auto tg0 = std::chrono::steady_clock::now();
-        anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, 6, 20, 1.2f, 10, anqi_cli::gpu_form_chosen() ? 1 : 0};
+        // ANQI_REUSE_ORIG_KNN: no lifted LocalJoin (index.h:1202-1238); ANQI_VAMANA:
+        // the refinement passes after the commit (:1262-1282)
+        anqi_graph_params gp{(uint32_t)Marg, 50, 100, 100, 8, std::getenv("ANQI_REUSE_ORIG_KNN") ? 0 : 6, 20, 1.2f, 10,
+                             anqi_cli::gpu_form_chosen() ? 1 : 0};
+        if (std::getenv("ANQI_VAMANA")) {
+            const char* sv = std::getenv("ANQI_VAMANA_STYLE");
+            const std::string style = sv ? sv : "global";
+            gp.vamana = (style == "global" || style == "old") ? 1 : 2;
+            gp.vamana_L = (uint32_t)envi("ANQI_VAMANA_L", 128);
+            gp.vamana_passes = envi("ANQI_VAMANA_PASS", 2);
+            gp.vamana_batch_frac = envd("ANQI_VAMANA_BATCH_FRAC", 0.02);
+            gp.vamana_max_batch = (uint64_t)(size_t)envi("ANQI_VAMANA_MAX_BATCH", 1000000);
+            gp.vamana_batch_base = envd("ANQI_VAMANA_BATCH_BASE", 2.0);
+            const char* ea = std::getenv("ANQI_VAMANA_EARLY_ALPHA");
+            gp.vamana_early_alpha = ea ? (float)std::atof(ea) : 1.0f;
+            gp.vamana_early_alpha_one = envi("ANQI_VAMANA_EARLY_ALPHA_ONE", 1) != 0 ? 1 : 0;
+            gp.vamana_entry = graph_entry;
+            if (gp.vamana == 2)
+                std::fprintf(stderr, "[vamana-batch] entry=%u L=%u pass=%d batch_frac=%.4g max_batch=%llu base=%.3g "
+                                     "alpha=%.3g rev=csr\n", graph_entry, gp.vamana_L, gp.vamana_passes,
+                             gp.vamana_batch_frac > 0 ? gp.vamana_batch_frac : 0.02,
+                             (unsigned long long)std::min<uint64_t>(
+                                 (uint64_t)std::max(1.0, (gp.vamana_batch_frac > 0 ? gp.vamana_batch_frac : 0.02) *
+                                                             (double)n), gp.vamana_max_batch),
+                             gp.vamana_batch_base > 1.0 ? gp.vamana_batch_base : 2.0, 1.2);
+        }
         anqi_graph_stats gst{};
         ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_warm, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
         if (d_warm != d_aknn) anqi_dfree(g_ctx, d_warm);
@@ -871,12 +929,20 @@ int main(int argc, char** argv) {
     uint64_t ccap = 0;
     // one batch: search + post-filter on the device; the candidates (ids and
     // accept bytes) compacted on the device and copied to the host
+    uint32_t* d_entries = nullptr;   // single-entry queries: every query starts at graph_entry
+    if (single_entry) d_entries = upload(std::vector<uint32_t>(nq, graph_entry));
     auto search = [&](size_t L) {
         for (;;) {
-            ck(anqi_rknn_search(g_ctx, &ix, d_query, nq, (uint32_t)L, nullptr, (uint32_t)L, d_thr, nullptr, out_cap,
-                                d_cand, d_acc, d_cnt, d_nd, d_pred, nullptr, d_status,
-                                anqi_cli::gpu_form_chosen() ? 1 : 0),
-               "range search");
+            if (d_entries)
+                ck(anqi_rknn_search_entries(g_ctx, &ix, d_query, nq, (uint32_t)L, nullptr, (uint32_t)L, d_thr,
+                                            nullptr, d_entries, out_cap, d_cand, d_acc, d_cnt, d_nd, d_pred, nullptr,
+                                            d_status, anqi_cli::gpu_form_chosen() ? 1 : 0),
+                   "range search");
+            else
+                ck(anqi_rknn_search(g_ctx, &ix, d_query, nq, (uint32_t)L, nullptr, (uint32_t)L, d_thr, nullptr,
+                                    out_cap, d_cand, d_acc, d_cnt, d_nd, d_pred, nullptr, d_status,
+                                    anqi_cli::gpu_form_chosen() ? 1 : 0),
+                   "range search");
             download(cnt, d_cnt, nq);
             uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
             if (mx <= out_cap) break;

@@ -28,7 +28,8 @@ nothing else).  Outputs are data only (inputs and expected outputs):
             basis and codebook come from lrq_fit (pinned to the reference
             trainer by the sample and trainer3k fixtures).
 
-Usage: python3 tests/golden/make_golden.py [--only sample|sample_ip|trainer|mix20k|mix1m] [--work DIR]
+Usage: python3 tests/golden/make_golden.py [--only sample|sample_ip|sample_variants|trainer|mix20k|mix1m]
+       [--work DIR]
 """
 from __future__ import annotations
 
@@ -377,6 +378,47 @@ def make_sample_ip(out_dir):
     print("[golden] sample_ip fixtures written to", out_dir)
 
 
+# Build variants of the reference's rknn_query on the sample: without the
+# lifted LocalJoin (ANQI_REUSE_ORIG_KNN, which leaves every build step
+# deterministic whatever the thread count), with the Vamana passes of each
+# style, and with every query starting at one entry (ANQI_QUERY_SINGLE_ENTRY).
+SAMPLE_VARIANTS = {
+    "reuse": {"ANQI_REUSE_ORIG_KNN": "1"},
+    "vamana_global": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_VAMANA": "1", "ANQI_VAMANA_L": "64"},
+    "vamana_batch": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_VAMANA": "1", "ANQI_VAMANA_STYLE": "batch",
+                     "ANQI_VAMANA_L": "64", "ANQI_VAMANA_PASS": "3", "ANQI_VAMANA_ENTRY": "medoid"},
+    "single_entry": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_QUERY_SINGLE_ENTRY": "1", "ANQI_VAMANA_ENTRY": "7"},
+}
+
+
+def make_sample_variants(out_dir, sample_dir):
+    """For each variant: the reference's graph cache file and its [rknn]
+    stdout lines at k = 10 on the sample fixture."""
+    os.makedirs(out_dir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as work:
+        pfx = os.path.join(work, "sample")
+        for suf in ["_base.bin", "_query.bin", "_baseknn_aknn.rec", "_radbasis_M8.bin",
+                    "_learned_rq_codebook_M8_u8.bin", "_rknn_gt_k10.bin", "_lifted.meta"]:
+            shutil.copy(os.path.join(sample_dir, "sample" + suf), pfx + suf)
+        for name, extra in SAMPLE_VARIANTS.items():
+            env = dict(PIN_ENV, OMP_NUM_THREADS="4", ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec",
+                       ANQI_AKNN_BUILD_SECONDS="0", ANQI_VERIFIER_TRAIN_SECONDS="0", **extra)
+            env["ANQI_GRAPH_CACHE_DIR"] = os.path.join(work, name, "graph")
+            env["ANQI_VERIFIER_CACHE_DIR"] = os.path.join(work, name, "verifier")
+            os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+            os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+            out = run([os.path.join(REF_DIR, "rknn_query"), "16,32,64,128", pfx, "48", "500", "64", "600", "10"],
+                      env=env)
+            graph = glob.glob(os.path.join(env["ANQI_GRAPH_CACHE_DIR"], "fd_metric_l2_M_48*"))
+            assert len(graph) == 1, graph
+            shutil.copy(graph[0], os.path.join(out_dir, f"{name}_graph.bin"))
+            with open(os.path.join(out_dir, f"{name}_stdout.txt"), "w") as f:
+                f.write("\n".join(l for l in out.splitlines() if l.startswith("[rknn]")) + "\n")
+    with open(os.path.join(out_dir, "variants.json"), "w") as f:
+        json.dump(SAMPLE_VARIANTS, f, indent=1)
+    print("[golden] sample build variants written to", out_dir)
+
+
 def make_trainer(out_dir, n=3000, train_sample=2000):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from anqi_testlib import trainer_cutoffs
@@ -459,7 +501,7 @@ def make_mix1m(out_dir, work_root=None):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", choices=["sample", "sample_ip", "trainer", "mix20k", "mix1m"])
+    ap.add_argument("--only", choices=["sample", "sample_ip", "sample_variants", "trainer", "mix20k", "mix1m"])
     ap.add_argument("--work", default=None, help="scratch directory for the 1M run")
     a = ap.parse_args()
     if not os.path.exists(os.path.join(REF_DIR, "libref_harness.so")):
@@ -468,6 +510,8 @@ def main():
         make_sample(os.path.join(HERE, "sample"))
     if a.only in (None, "sample_ip"):
         make_sample_ip(os.path.join(HERE, "sample_ip"))
+    if a.only in (None, "sample_variants"):
+        make_sample_variants(os.path.join(HERE, "sample_variants"), os.path.join(HERE, "sample"))
     if a.only in (None, "trainer"):
         make_trainer(os.path.join(HERE, "trainer3k"))
     if a.only in (None, "mix20k"):

@@ -93,8 +93,9 @@ def test_rknn_query_rejects_unpinned_configuration():
     instead of silently computing the pinned one."""
     pfx = os.path.join(GOLDEN, "sample", "sample")
     for env in ({"ANQI_RADIUS_MODE": "approx"}, {"ANQI_VERIFIER_PLACEMENT": "integrated"},
-                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_REUSE_ORIG_KNN": "1"},
-                {"ANQI_VAMANA": "0"}, {"ANQI_RADIUS_RELAX": "0.5"}):
+                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_EDGE_RESID_FILE": "x"},
+                {"ANQI_EDGE_POLICY": "ball_rng"}, {"ANQI_VAMANA": "0", "ANQI_VAMANA_L": "2000"},
+                {"ANQI_RADIUS_RELAX": "0.5"}):
         r = run("rknn_query", ["16", pfx, "48", "500", "64", "600", "10"], env)
         assert r.returncode == 2, (env, r.returncode, r.stderr)
         assert "not supported" in r.stderr
@@ -203,3 +204,36 @@ def test_u8_base_cache_key_and_bad_files_equal_reference(tmp_path):
                                          ANQI_PREBUILT_WARM_START_COMMAND="cat " + pfx + "_baseknn_aknn.rec",
                                          ANQI_GRAPH_CACHE_DIR=str(tmp_path / "g2") + "/"), ref=ref)
         assert r.returncode == 2 and "cannot be combined" in r.stderr, (ref, r.returncode, r.stderr)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFBIN, "rknn_query")), reason="reference tools not built here")
+@pytest.mark.parametrize("extra", [
+    {"ANQI_REUSE_ORIG_KNN": "1"},
+    {"ANQI_VAMANA": "1"},
+    {"ANQI_VAMANA": "", "ANQI_VAMANA_STYLE": "batch", "ANQI_VAMANA_L": "64", "ANQI_VAMANA_ENTRY": "medoid",
+     "ANQI_REUSE_ORIG_KNN": "0"},
+    {"ANQI_VAMANA": "1", "ANQI_VAMANA_STYLE": "old", "ANQI_VAMANA_PASS": "1", "ANQI_VAMANA_BATCH_FRAC": "0.1",
+     "ANQI_VAMANA_EARLY_ALPHA": "1.05", "ANQI_QUERY_SINGLE_ENTRY": "1"},
+], ids=["reuse", "vamana", "vamana_batch", "vamana_old"])
+def test_variant_cache_names_equal_reference(tmp_path, extra):
+    """The build variants (ANQI_REUSE_ORIG_KNN, ANQI_VAMANA* in both styles)
+    key the graph cache as the reference does (src/rknn_query.cpp:301-341,
+    include/index.h:1704-1770): after the reference's run, this tool finds all
+    its caches under the same key and file names."""
+    pfx = str(tmp_path / "sample")
+    for suf in SAMPLE_FILES:
+        shutil.copy(os.path.join(GOLDEN, "sample", "sample" + suf), pfx + suf)
+    env = dict(PIN_ENV, ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec",
+               ANQI_GRAPH_CACHE_DIR=str(tmp_path / "graph"), ANQI_VERIFIER_CACHE_DIR=str(tmp_path / "verifier"),
+               **extra)
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    args = ["16", pfx, "48", "500", "64", "600", "10"]
+    rr = run("rknn_query", args, env, ref=True)
+    assert rr.returncode == 0, rr.stderr
+    key = re.search(r"graph cache key=(\S+)", rr.stdout).group(1)
+    assert ("_vam1_style" in key) == ("ANQI_VAMANA" in extra) and ("_reuseorig1" in key) == ("ANQI_REUSE_ORIG_KNN" in extra)
+    r = run("rknn_query", args, dict(env, ANQI_REQUIRE_GRAPH_CACHE="1"))
+    assert f"[rknn] graph cache key={key}\n" in r.stdout
+    assert "required graph cache" not in r.stderr
+    assert r.returncode == 0 or (r.returncode == 3 and "GPU failure in context" in r.stderr), r.stderr

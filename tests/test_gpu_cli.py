@@ -253,3 +253,36 @@ def test_ingestion_options_give_the_same_answers(tmp_path):
     r = subprocess.run([os.path.join(BIN, "rknn_query"), "16", pfx, "48", "500", "64", "600", "10"], env=env,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 3 and "streamed warm-start ended early" in r.stderr
+
+
+@pytest.mark.parametrize("variant", ["reuse", "vamana_global", "vamana_batch", "single_entry"])
+def test_build_variants_equal_reference(tmp_path, variant):
+    """Build variants of the reference (tests/golden/sample_variants, its own
+    rknn_query on the sample): ANQI_REUSE_ORIG_KNN (no lifted LocalJoin,
+    include/index.h:1202-1238), ANQI_VAMANA in both styles (:932-1063,
+    1262-1282) and ANQI_QUERY_SINGLE_ENTRY (:1606-1607).  The graph written
+    here equals the reference's link for link, and the answer lines are its."""
+    import json
+    from anqi_testlib import read_graph
+    vdir = os.path.join(GOLDEN, "sample_variants")
+    extra = json.load(open(os.path.join(vdir, "variants.json")))[variant]
+    src = os.path.join(GOLDEN, "sample", "sample")
+    env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
+    env.update(PIN_ENV)
+    env.update(extra)
+    env["ANQI_PREBUILT_TOPK_REC"] = src + "_baseknn_aknn.rec"
+    env["ANQI_GRAPH_CACHE_DIR"] = str(tmp_path / "graph")
+    env["ANQI_VERIFIER_CACHE_DIR"] = str(tmp_path / "verifier")
+    os.makedirs(env["ANQI_GRAPH_CACHE_DIR"])
+    os.makedirs(env["ANQI_VERIFIER_CACHE_DIR"])
+    say("rknn_query", variant)
+    r = subprocess.run([os.path.join(BIN, "rknn_query"), "16,32,64,128", src, "48", "500", "64", "600", "10"],
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = open(os.path.join(vdir, f"{variant}_stdout.txt")).read()
+    assert answers(r.stdout) == answers(ref) and len(answers(ref)) == 12
+    mine = [f for f in os.listdir(env["ANQI_GRAPH_CACHE_DIR"]) if f.startswith("fd_metric_l2_M_48")]
+    assert len(mine) == 1
+    d1, l1, _ = read_graph(os.path.join(env["ANQI_GRAPH_CACHE_DIR"], mine[0]), 256, 48)
+    d2, l2, _ = read_graph(os.path.join(vdir, f"{variant}_graph.bin"), 256, 48)
+    assert np.array_equal(d1, d2) and np.array_equal(l1, l2)
