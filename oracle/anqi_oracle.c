@@ -1267,3 +1267,215 @@ void or_vamana_batch(const float* x, const float* nrm, uint32_t n, uint32_t d, c
     }
     free(order); free(O); free(ocnt); free(rcnt); free(roff); free(rsrc); free(tl);
 }
+
+/* ---- edge policies of prune_pool_to_ids (include/index.h:283-829) ---------- */
+
+static float pol_d2(const float* x, const float* nrm, uint32_t d, uint32_t a, uint32_t b) {
+    return or_d2(x + (size_t)a * d, x + (size_t)b * d, nrm[a], nrm[b], d);
+}
+static int pol_has(const uint32_t* ids, uint32_t m, uint32_t id) {
+    for (uint32_t t = 0; t < m; t++) if (ids[t] == id) return 1;
+    return 0;
+}
+/* edge_radj (:291-293): scale^2 times the residual of rank k */
+static double pol_radj(const or_edge_policy* E, uint32_t id, int k) {
+    return E->scale2 * (double)E->resid[(size_t)id * E->nk + (size_t)(k - 1)];
+}
+/* edge_sort_radj (:295-303) */
+static double pol_sort_radj(const or_edge_policy* E, uint32_t id) {
+    if (!E->ball || E->nks == 0) return 0.0;
+    double best = -INFINITY;
+    for (uint32_t t = 0; t < E->nks; t++) {
+        const double r = pol_radj(E, id, E->ks[t]);
+        if (r > best) best = r;
+    }
+    return isfinite(best) ? best : 0.0;
+}
+/* ball_occludes_distance (:474-490), both sides fused as GCC builds them */
+static int pol_ball_occ(const or_edge_policy* E, float alpha, uint32_t base, uint32_t sid, float dbase, float dsc) {
+    if (!E->ball || E->nks == 0) return 0;
+    int any = 0, all = 1;
+    for (uint32_t t = 0; t < E->nks; t++) {
+        const double lhs = fma(-E->lambda, pol_radj(E, sid, E->ks[t]), (double)alpha * (double)dsc);
+        const double rhs = fma(-E->lambda, pol_radj(E, base, E->ks[t]), (double)dbase);
+        const int occ = lhs < rhs;
+        any = any || occ;
+        all = all && occ;
+        if (!E->occlude_all && any) return 1;
+        if (E->occlude_all && !all) return 0;
+    }
+    return E->occlude_all ? all : any;
+}
+/* lsg_distance2 (:832-838) */
+static double pol_lsg(const or_edge_policy* E, uint32_t a, uint32_t b, double raw) {
+    if (!E->lsg || E->lsg_alpha == 0.0) return raw;
+    const double mp = (double)E->mu[a] * (double)E->mu[b];
+    const double den = pow(mp, E->lsg_alpha);
+    if (!isfinite(den) || den <= 0.0) return raw;
+    return raw / den;
+}
+
+typedef struct { double k; uint32_t id; float d; } pol_key;
+static int pol_key_cmp(const void* a, const void* b) {
+    const pol_key* x = (const pol_key*)a;
+    const pol_key* y = (const pol_key*)b;
+    if (x->k < y->k) return -1;
+    if (x->k > y->k) return 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+/* rng_extend (:267-281) over (d, id) */
+static uint32_t pol_rng(const float* x, const float* nrm, uint32_t d, float alpha, const float* pd, const uint32_t* pi,
+                        uint32_t m, uint32_t cap, uint32_t* ids, uint32_t deg) {
+    for (uint32_t t = 0; t < m; t++) {
+        if (deg >= cap) break;
+        const uint32_t cid = pi[t];
+        if (pol_has(ids, deg, cid)) continue;
+        int occ = 0;
+        for (uint32_t s = 0; s < deg && !occ; s++) occ = alpha * pol_d2(x, nrm, d, ids[s], cid) < pd[t];
+        if (!occ) ids[deg++] = cid;
+    }
+    return deg;
+}
+/* rng_extend_ball (:505-519) over the ball order, occlusion by ball_occludes */
+static uint32_t pol_rng_ball(const float* x, const float* nrm, uint32_t d, float alpha, const or_edge_policy* E,
+                             uint32_t base, const pol_key* bp, uint32_t m, uint32_t cap, uint32_t* ids, uint32_t deg) {
+    for (uint32_t t = 0; t < m; t++) {
+        if (deg >= cap) break;
+        const uint32_t cid = bp[t].id;
+        if (pol_has(ids, deg, cid)) continue;
+        int occ = 0;
+        for (uint32_t s = 0; s < deg && !occ; s++)
+            occ = pol_ball_occ(E, alpha, base, ids[s], bp[t].d, pol_d2(x, nrm, d, ids[s], cid));
+        if (!occ) ids[deg++] = cid;
+    }
+    return deg;
+}
+/* rng_extend_lsg (:853-871) over the lsg order */
+static uint32_t pol_rng_lsg(const float* x, const float* nrm, uint32_t d, float alpha, const or_edge_policy* E,
+                            const pol_key* lp, uint32_t m, uint32_t cap, uint32_t* ids, uint32_t deg) {
+    for (uint32_t t = 0; t < m; t++) {
+        if (deg >= cap) break;
+        const uint32_t cid = lp[t].id;
+        if (pol_has(ids, deg, cid)) continue;
+        int occ = 0;
+        for (uint32_t s = 0; s < deg && !occ; s++) {
+            const double dsc = pol_lsg(E, ids[s], cid, (double)pol_d2(x, nrm, d, ids[s], cid));
+            occ = (double)alpha * dsc < lp[t].k;
+        }
+        if (!occ) ids[deg++] = cid;
+    }
+    return deg;
+}
+/* add_residual_cover_edges (:441-472) */
+static uint32_t pol_cover(const or_edge_policy* E, uint32_t n, uint32_t base, const float* pd, const uint32_t* pi,
+                          uint32_t m, uint32_t M, uint32_t* ids, uint32_t deg) {
+    if (!E->resid_cover || deg >= M) return deg;
+    const uint32_t total = E->budget < M - deg ? E->budget : M - deg;
+    if (total == 0 || E->nks == 0) return deg;
+    uint32_t per_k = (total + E->nks - 1) / E->nks;
+    if (per_k < 1) per_k = 1;
+    uint32_t added = 0;
+    pol_key* sc = (pol_key*)malloc(sizeof(pol_key) * (m ? m : 1));
+    for (uint32_t t = 0; t < E->nks; t++) {
+        if (added >= total || deg >= M) break;
+        const int k = E->ks[t];
+        uint32_t c = 0;
+        for (uint32_t u = 0; u < m; u++) {
+            const uint32_t cid = pi[u];
+            if (cid == UINT32_MAX || cid >= n || cid == base || pol_has(ids, deg, cid)) continue;
+            const double r = (double)E->resid[(size_t)cid * E->nk + (size_t)(k - 1)];
+            const double risk = r > 0.0 ? r : 0.0;
+            if (risk <= 0.0) continue;
+            sc[c].k = fma(-(E->lambda * E->scale2), risk, (double)pd[u]);
+            sc[c].id = cid;
+            c++;
+        }
+        qsort(sc, c, sizeof(pol_key), pol_key_cmp);
+        uint32_t ak = 0;
+        for (uint32_t u = 0; u < c; u++) {
+            if (deg >= M || added >= total || ak >= per_k) break;
+            if (pol_has(ids, deg, sc[u].id)) continue;
+            ids[deg++] = sc[u].id;
+            ak++;
+            added++;
+        }
+    }
+    free(sc);
+    return deg;
+}
+
+/* prune_pool_to_ids (:772-829) under an edge policy; the sort_unique step as
+ * in or_prune.  Policies whose configuration left every flag off prune like
+ * or_prune. */
+uint32_t or_prune_policy(const float* x, const float* nrm, uint32_t d, uint32_t base, uint32_t n, float* pool_d,
+                         uint32_t* pool_id, uint32_t m, uint32_t M, float alpha, const or_edge_policy* E,
+                         uint32_t* out) {
+    /* sort_unique_candidates (:225-248) */
+    uint32_t w = 0;
+    for (uint32_t t = 0; t < m; t++) {
+        uint32_t id = pool_id[t];
+        if (id == UINT32_MAX || id >= n || id == base) continue;
+        pool_id[w] = id; pool_d[w] = pool_d[t]; w++;
+    }
+    {
+        or_prop* P = (or_prop*)malloc(sizeof(or_prop) * (w ? w : 1));
+        for (uint32_t t = 0; t < w; t++) { P[t].row = pool_id[t]; P[t].id = 0; P[t].d = pool_d[t]; }
+        qsort(P, w, sizeof(or_prop), prop_cmp);
+        uint32_t u = 0;
+        for (uint32_t r = 0; r < w;) {
+            uint32_t e = r + 1;
+            while (e < w && P[e].row == P[r].row) e++;
+            pool_id[u] = P[r].row; pool_d[u] = P[r].d; u++;
+            r = e;
+        }
+        free(P);
+        w = u;
+    }
+    or_sort_pairs(pool_d, pool_id, w);
+    uint32_t deg = 0;
+    pol_key* lp = (pol_key*)malloc(sizeof(pol_key) * (w ? w : 1));
+    pol_key* bp = (pol_key*)malloc(sizeof(pol_key) * (w ? w : 1));
+    /* lsg_sort_candidates (:840-851); ball_sort_candidates (:494-503, key
+     * d - lambda * max_k radj, fused) */
+    for (uint32_t t = 0; t < w; t++) {
+        lp[t].k = pol_lsg(E, base, pool_id[t], (double)pool_d[t]);
+        lp[t].id = pool_id[t];
+        lp[t].d = pool_d[t];
+        bp[t].k = E->ball ? fma(-E->lambda, pol_sort_radj(E, pool_id[t]), (double)pool_d[t]) : (double)t;
+        bp[t].id = pool_id[t];
+        bp[t].d = pool_d[t];
+    }
+    qsort(lp, w, sizeof(pol_key), pol_key_cmp);
+    if (E->ball) qsort(bp, w, sizeof(pol_key), pol_key_cmp);
+    const uint32_t bud = E->budget < M ? E->budget : M;
+    if (E->lsg && E->ball && E->lsg_alpha != 0.0) {
+        uint32_t cap = M - bud;
+        if (cap == 0) cap = M;
+        deg = pol_rng_lsg(x, nrm, d, alpha, E, lp, w, cap, out, deg);
+        deg = pol_rng_ball(x, nrm, d, alpha, E, base, bp, w, M, out, deg);
+        deg = pol_rng_lsg(x, nrm, d, alpha, E, lp, w, M, out, deg);
+    } else if (E->lsg && E->lsg_alpha != 0.0) {
+        deg = pol_rng_lsg(x, nrm, d, alpha, E, lp, w, M, out, deg);
+    } else if (E->ball) {
+        if (E->ball_pure) {
+            deg = pol_rng_ball(x, nrm, d, alpha, E, base, bp, w, M, out, deg);
+            deg = pol_rng(x, nrm, d, alpha, pool_d, pool_id, w, M, out, deg);
+        } else {
+            uint32_t cap = M - bud;
+            if (cap == 0) cap = M;
+            deg = pol_rng(x, nrm, d, alpha, pool_d, pool_id, w, cap, out, deg);
+            deg = pol_rng_ball(x, nrm, d, alpha, E, base, bp, w, M, out, deg);
+            deg = pol_rng(x, nrm, d, alpha, pool_d, pool_id, w, M, out, deg);
+        }
+    } else {
+        const uint32_t cb = E->resid_cover ? bud : 0;
+        uint32_t cap = M - cb;
+        if (cap == 0) cap = M;
+        deg = pol_rng(x, nrm, d, alpha, pool_d, pool_id, w, cap, out, deg);
+        deg = pol_cover(E, n, base, pool_d, pool_id, w, M, out, deg);
+        deg = pol_rng(x, nrm, d, alpha, pool_d, pool_id, w, M, out, deg);
+    }
+    free(lp); free(bp);
+    return deg;
+}

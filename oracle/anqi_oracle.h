@@ -207,6 +207,25 @@ void or_mt64_seed(or_mt64* m, uint64_t seed);
 uint64_t or_mt64_next(or_mt64* m);
 void or_shuffle_u32(uint32_t* a, uint32_t n, or_mt64* g);
 
+/* ---- edge policies (include/index.h:283-829) ------------------------------- */
+/* The configured state of configure_edge_residual_cover (:305-438): which
+ * branch prune_pool_to_ids takes and its inputs.  resid: n*nk residuals;
+ * ks: 1-based ranks (<= nk); mu: n lsg weights. */
+typedef struct {
+    int resid_cover, ball, ball_pure, occlude_all, lsg;
+    double lsg_alpha;
+    const float* mu;
+    const float* resid;
+    uint32_t nk;
+    const int32_t* ks;
+    uint32_t nks;
+    uint32_t budget;
+    double lambda, scale2;
+} or_edge_policy;
+uint32_t or_prune_policy(const float* x, const float* nrm, uint32_t d, uint32_t base, uint32_t n, float* pool_d,
+                         uint32_t* pool_id, uint32_t m, uint32_t M, float alpha, const or_edge_policy* E,
+                         uint32_t* out);
+
 /* ---- small helpers shared by tests -------------------------------------- */
 void or_sort_pairs(float* d, uint32_t* id, uint32_t m);   /* (dist, id) ascending */
 

@@ -81,6 +81,7 @@ ORACLE_SIGS = {
     "or_mt64_seed": (None, [P, U64]),
     "or_mt64_next": (U64, [P]),
     "or_shuffle_u32": (None, [P, U32, P]),
+    "or_prune_policy": (U32, [P, P, U32, U32, U32, P, P, U32, U32, F32, P, P]),
 }
 
 REF_SIGS = {
@@ -354,3 +355,73 @@ def tree_datasets():
     return [("mixture-20000x128-leaf100", mix, 100, 100), ("mixture-5000-leaf7", mix[:5000], 7, 100),
             ("ties-4000x20-leaf10", ties, 10, 100), ("degenerate-500x8-leaf10", degen, 10, 100),
             ("depthcap-500x8-depth3", degen, 10, 3)]
+
+
+class OrEdgePolicy(ctypes.Structure):
+    """oracle/anqi_oracle.h or_edge_policy (the configured edge policy)."""
+    _fields_ = [("resid_cover", ctypes.c_int), ("ball", ctypes.c_int), ("ball_pure", ctypes.c_int),
+                ("occlude_all", ctypes.c_int), ("lsg", ctypes.c_int), ("lsg_alpha", ctypes.c_double),
+                ("mu", ctypes.c_void_p), ("resid", ctypes.c_void_p), ("nk", ctypes.c_uint32),
+                ("ks", ctypes.c_void_p), ("nks", ctypes.c_uint32), ("budget", ctypes.c_uint32),
+                ("lambda_", ctypes.c_double), ("scale2", ctypes.c_double)]
+
+
+def edge_policy_config(env, n, M, folder):
+    """configure_edge_residual_cover (include/index.h:305-438) restated from
+    an environment dict: (flags dict, residual array or None, mu or None);
+    None when the policy leaves every flag off.  Inputs are read as the
+    reference reads them (residuals from ANQI_EDGE_RESID_FILE or
+    folder + "_anqi_rankm_residual.bin", scale from folder + "_anqi_meta.txt")."""
+    import numpy as _np
+    pol = env.get("ANQI_EDGE_POLICY")
+    if not pol:
+        return None
+    f = dict(resid_cover=0, ball=0, ball_pure=0, occlude_all=1, lsg=0, lsg_alpha=1.0, mu=None, resid=None, nk=100,
+             ks=[], budget=16, lam=1.0, scale2=1.0)
+    if pol in ("lsg_rng", "lsg_bubble_hybrid"):
+        f["lsg_alpha"] = float(env.get("ANQI_LSG_ALPHA", "1.0") or 0)
+        f["mu"] = _np.fromfile(env["ANQI_LSG_MU_FILE"], dtype=_np.float32)
+        f["lsg"] = 1
+        if pol == "lsg_rng":
+            return f
+    needs = ("residual_cover", "ball_rng", "ball_hybrid", "ball_multik", "bubble_rng", "bubble_hybrid",
+             "bubble_multik", "bubble_auto", "bubble_auto_full", "lsg_bubble_hybrid")
+    if pol not in needs:
+        return f if f["lsg"] else None
+    nk = int(env.get("ANQI_EDGE_NK", "100") or 0) or 100
+    f["nk"] = nk
+    f["budget"] = min(max(0, int(env.get("ANQI_EDGE_RESID_BUDGET", "16") or 0)), M)
+    f["lam"] = float(env.get("ANQI_EDGE_RESID_LAMBDA", "1.0") or 0)
+    ks = [int(t) for t in env.get("ANQI_EDGE_KS", "10,50,100").split(",") if t and int(t) > 0]
+    f["ks"] = [k for k in ks if 1 <= k <= nk]
+    if not f["ks"]:
+        return f if f["lsg"] else None
+    auto = pol in ("bubble_auto", "bubble_auto_full")
+    if f["budget"] == 0 and pol != "ball_rng" and not auto:
+        return f if f["lsg"] else None
+    try:
+        mstar, scale = map(float, open(folder + "_anqi_meta.txt").read().split()[:2])
+        f["scale2"] = scale * scale
+    except (OSError, ValueError):
+        pass
+    f["resid"] = _np.fromfile(env.get("ANQI_EDGE_RESID_FILE") or folder + "_anqi_rankm_residual.bin",
+                              dtype=_np.float32)
+    assert f["resid"].size == n * nk
+    f["resid_cover"] = 1
+    if pol != "residual_cover":
+        f["resid_cover"] = 0
+        f["ball"] = 1
+        f["ball_pure"] = int(pol in ("ball_rng", "bubble_rng"))
+        f["occlude_all"] = int(env.get("ANQI_EDGE_BALL_OCCLUSION", "all") != "any")
+    return f
+
+
+def or_edge_policy(f):
+    """ctypes or_edge_policy for a flags dict (keeps its arrays alive)."""
+    import numpy as _np
+    ks = _np.array(f["ks"] or [0], dtype=_np.int32)
+    E = OrEdgePolicy(f["resid_cover"], f["ball"], f["ball_pure"], f["occlude_all"], f["lsg"], f["lsg_alpha"],
+                     ptr(f["mu"]) if f["mu"] is not None else None, ptr(f["resid"]) if f["resid"] is not None else None,
+                     f["nk"], ptr(ks), len(f["ks"]), f["budget"], f["lam"], f["scale2"])
+    E._keep = (f, ks)
+    return E
