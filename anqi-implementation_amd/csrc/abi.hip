@@ -810,13 +810,26 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
     P.h16 = (!graph && rk && P.hmask + 1 == H16_SLOTS) ? (P.n <= H16_MAX_ID ? 1u : P.n <= H24_MAX_ID ? 2u : 0u) : 0u;
     P.pmax = graph ? std::min<uint32_t>((rk && !P.vmode) ? 1024u : 2048u, next_pow2(need + 64)) : 0;
     if (!P.rows) P.nrows = P.n;   // else: the given row list
-    HCHK(launch_gather(P, graph, false, s));
     std::vector<uint32_t> bad;
-    int rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
-    if (rc) return rc;
-    if (overflow_rows) *overflow_rows = (uint32_t)bad.size();
-    if (bad.empty()) return read_work();
-    if (graph && rk && P.pmax < 2048u && next_pow2(need + 64) > P.pmax) {
+    int rc = ANQI_OK;
+    if (graph && P.pol.on) {
+        // an edge-policy prune needs more room than LDS holds: every row
+        // takes the global work area
+        bad.resize(P.nrows);
+        if (P.rows) {
+            HCHK(hipMemcpyAsync(bad.data(), P.rows, 4ull * P.nrows, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+        } else {
+            for (uint32_t r = 0; r < P.nrows; r++) bad[r] = r;
+        }
+    } else {
+        HCHK(launch_gather(P, graph, false, s));
+        rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
+        if (rc) return rc;
+        if (overflow_rows) *overflow_rows = (uint32_t)bad.size();
+        if (bad.empty()) return read_work();
+    }
+    if (graph && rk && !P.pol.on && P.pmax < 2048u && next_pow2(need + 64) > P.pmax) {
         // rows that outgrew the short pool: once more in LDS with the full one
         GatherParams R = P;
         R.pmax = std::min<uint32_t>(2048, next_pow2(need + 64));
@@ -830,26 +843,35 @@ static int gather_run(anqi_ctx* ctx, GatherParams P, bool graph, uint32_t maxrev
         if (rc) return rc;
         if (bad.empty()) return read_work();
     }
-    // slow path: global work area sized for the bound
-    GatherParams Q = P;
-    Q.h16 = 0;
-    Q.hmask = next_pow2(need * 2 + 64) - 1;
-    Q.pmax = graph ? next_pow2(need + 64) : 0;
-    size_t words = (gather_lds_bytes(Q, graph, false) + 3) / 4;
-    DBuf rows, ws;
-    HCHK(rows.alloc(4 * bad.size()));
-    HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
-    HCHK(hipMemsetAsync(status.p, 0, 4ull * P.n, s));
-    const uint32_t chunk = 4096;
-    HCHK(ws.alloc(4 * words * std::min<size_t>(chunk, bad.size())));
-    for (size_t b = 0; b < bad.size(); b += chunk) {
-        Q.rows = rows.as<uint32_t>() + b;
-        Q.nrows = (uint32_t)std::min<size_t>(chunk, bad.size() - b);
-        Q.gscratch = ws.as<uint32_t>();
-        Q.gscratch_words = words;
-        HCHK(launch_gather(Q, graph, true, s));
+    // slow path: global work area sized for the bound (rows in chunks)
+    auto run_global = [&](uint32_t hmask, uint32_t pmax) -> int {
+        GatherParams Q = P;
+        Q.h16 = 0;
+        Q.hmask = hmask;
+        Q.pmax = graph ? pmax : 0;
+        size_t words = (gather_lds_bytes(Q, graph, false) + 3) / 4;
+        DBuf rows, ws;
+        HCHK(rows.alloc(4 * bad.size()));
+        HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
+        HCHK(hipMemsetAsync(status.p, 0, 4ull * P.n, s));
+        const uint32_t chunk = 4096;
+        HCHK(ws.alloc(4 * words * std::min<size_t>(chunk, bad.size())));
+        for (size_t b = 0; b < bad.size(); b += chunk) {
+            Q.rows = rows.as<uint32_t>() + b;
+            Q.nrows = (uint32_t)std::min<size_t>(chunk, bad.size() - b);
+            Q.gscratch = ws.as<uint32_t>();
+            Q.gscratch_words = words;
+            HCHK(launch_gather(Q, graph, true, s));
+        }
+        return failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
+    };
+    if (graph && P.pol.on) {
+        // every row of an edge-policy prune: first with the fast path's sizes
+        rc = run_global(P.hmask, P.pmax);
+        if (rc) return rc;
+        if (bad.empty()) return read_work();
     }
-    rc = failed_rows(ctx, status.as<uint32_t>(), P.n, bad);
+    rc = run_global((uint32_t)next_pow2(need * 2 + 64) - 1, graph ? (uint32_t)next_pow2(need + 64) : 0);
     if (rc) return rc;
     if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "gather: %zu rows exceed the work area", bad.size());
     return read_work();
@@ -901,11 +923,23 @@ static int commit_run(anqi_ctx* ctx, PruneParams P, uint32_t maxin) {
     HCHK(hipMemsetAsync(P.status, 0, 4ull * n, s));
     if (!P.rows) P.nrows = n;
     P.pmax = std::min<uint32_t>(2048, next_pow2((uint64_t)M + maxin + 64));
-    HCHK(launch_commit(P, false, s));
     std::vector<uint32_t> bad;
-    int rc = failed_rows(ctx, P.status, n, bad);
-    if (rc) return rc;
-    if (bad.empty()) return ANQI_OK;
+    int rc = ANQI_OK;
+    if (P.pol.on) {
+        // an edge-policy prune takes the global work area for every row
+        bad.resize(P.nrows);
+        if (P.rows) {
+            HCHK(hipMemcpyAsync(bad.data(), P.rows, 4ull * P.nrows, hipMemcpyDeviceToHost, s));
+            HCHK(hipStreamSynchronize(s));
+        } else {
+            for (uint32_t r = 0; r < P.nrows; r++) bad[r] = r;
+        }
+    } else {
+        HCHK(launch_commit(P, false, s));
+        rc = failed_rows(ctx, P.status, n, bad);
+        if (rc) return rc;
+        if (bad.empty()) return ANQI_OK;
+    }
     PruneParams Q = P;
     Q.pmax = next_pow2((uint64_t)M + maxin + 64);
     size_t words = (commit_lds_bytes(Q) + 3) / 4;
@@ -913,20 +947,34 @@ static int commit_run(anqi_ctx* ctx, PruneParams P, uint32_t maxin) {
     HCHK(rows.alloc(4 * bad.size()));
     HCHK(hipMemcpyAsync(rows.p, bad.data(), 4 * bad.size(), hipMemcpyHostToDevice, s));
     HCHK(hipMemsetAsync(P.status, 0, 4ull * n, s));
-    HCHK(ws.alloc(4 * words * bad.size()));
-    Q.rows = rows.as<uint32_t>();
-    Q.nrows = (uint32_t)bad.size();
-    Q.gscratch = ws.as<uint32_t>();
-    Q.gscratch_words = words;
-    HCHK(launch_commit(Q, true, s));
+    const size_t chunk = 4096;
+    HCHK(ws.alloc(4 * words * std::min(chunk, bad.size())));
+    for (size_t b = 0; b < bad.size(); b += chunk) {
+        Q.rows = rows.as<uint32_t>() + b;
+        Q.nrows = (uint32_t)std::min(chunk, bad.size() - b);
+        Q.gscratch = ws.as<uint32_t>();
+        Q.gscratch_words = words;
+        HCHK(launch_commit(Q, true, s));
+    }
     rc = failed_rows(ctx, P.status, n, bad);
     if (rc) return rc;
     if (!bad.empty()) return anqi_set_err(ctx, ANQI_E_DEVICE, "commit: %zu rows exceed the work area", bad.size());
     return ANQI_OK;
 }
 
+// commit_edges (include/index.h:907-930) under an edge policy.
+static int commit_policy(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* O,
+                         const uint32_t* ocnt, uint32_t M, float alpha, uint32_t* links, uint32_t* deg,
+                         const EdgePolicyDev& pol);
+
 int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* O,
                 const uint32_t* ocnt, uint32_t M, float alpha, uint32_t* links, uint32_t* deg) {
+    return commit_policy(ctx, x, nrm, n, d, O, ocnt, M, alpha, links, deg, EdgePolicyDev{});
+}
+
+static int commit_policy(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uint32_t d, const uint32_t* O,
+                         const uint32_t* ocnt, uint32_t M, float alpha, uint32_t* links, uint32_t* deg,
+                         const EdgePolicyDev& pol) {
     if (M > 64) return anqi_set_err(ctx, ANQI_E_USAGE, "graph degree <= 64");
     hipStream_t s = ctx->stream;
     DBuf roff, rids, status;
@@ -940,8 +988,33 @@ int anqi_commit(anqi_ctx* ctx, const float* x, const float* nrm, uint32_t n, uin
     P.x = x; P.nrm = nrm; P.n = n; P.d = d; P.O = O; P.ocnt = ocnt; P.roff = roff.as<uint32_t>();
     P.rids = rids.as<uint32_t>(); P.M = M; P.alpha = alpha; P.links = links; P.deg = deg;
     P.status = status.as<uint32_t>();
+    P.pol = pol;
     HCHK(launch_fill_u32(links, U32_NONE, (uint64_t)n * M, s));   // slots past deg[i]
     return commit_run(ctx, P, maxin);
+}
+
+// The device form of an edge policy; false (with the error set) when it is
+// not one this build runs.
+static bool edge_policy_dev(anqi_ctx* ctx, const anqi_edge_policy* e, uint32_t M, EdgePolicyDev& out) {
+    out = EdgePolicyDev{};
+    if (!e || !(e->resid_cover || e->ball || e->lsg)) return true;
+    if (e->nks > 16 || ((e->resid_cover || e->ball) && (!e->resid || e->nk == 0)) || (e->lsg && !e->mu)) {
+        anqi_set_err(ctx, ANQI_E_USAGE, "edge policy: need the residual table (nk > 0), at most 16 ranks, lsg weights");
+        return false;
+    }
+    for (uint32_t t = 0; t < e->nks; t++)
+        if (e->ks[t] < 1 || (uint32_t)e->ks[t] > e->nk) {
+            anqi_set_err(ctx, ANQI_E_USAGE, "edge policy: rank %d outside 1..%u", e->ks[t], e->nk);
+            return false;
+        }
+    out.on = 1;
+    out.resid_cover = e->resid_cover; out.ball = e->ball; out.ball_pure = e->ball_pure;
+    out.occlude_all = e->occlude_all; out.lsg = e->lsg;
+    out.lsg_alpha = e->lsg_alpha; out.lambda = e->lambda; out.scale2 = e->scale2;
+    out.mu = e->mu; out.resid = e->resid; out.nk = e->nk; out.nks = e->nks;
+    out.budget = std::min(e->budget, M);
+    for (uint32_t t = 0; t < e->nks; t++) out.ks[t] = e->ks[t];
+    return true;
 }
 
 }  // extern "C"
@@ -1470,7 +1543,7 @@ static GatherParams vamana_gather_params(const float* lifted, const float* nrm, 
 // the batch insertion of vamana_batch_refine (:932-1063).
 static int vamana_refine(anqi_ctx* ctx, const float* lifted, const float* nrm, uint32_t n, uint32_t dl,
                          const uint32_t* hidx, const float* hval, uint32_t cap, const anqi_graph_params* p,
-                         const anqi_tree* tree, uint32_t* links, uint32_t* deg) {
+                         const anqi_tree* tree, const EdgePolicyDev& pol, uint32_t* links, uint32_t* deg) {
     hipStream_t s = ctx->stream;
     const uint32_t M = p->M;
     const int passes = p->vamana_passes;
@@ -1488,9 +1561,11 @@ static int vamana_refine(anqi_ctx* ctx, const float* lifted, const float* nrm, u
                                                   O.as<uint32_t>(), ocnt.as<uint32_t>());
             G.vmode = 1;
             G.ventry = ent.as<uint32_t>();
+            G.pol = pol;
             int rc = gather_run(ctx, G, true, 0, nullptr);
             if (rc) return rc;
-            rc = anqi_commit(ctx, lifted, nrm, n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), M, p->alpha, links, deg);
+            rc = commit_policy(ctx, lifted, nrm, n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), M, p->alpha, links,
+                               deg, pol);
             if (rc) return rc;
             progress("vamana global pass %d", pass);
         }
@@ -1540,6 +1615,7 @@ static int vamana_refine(anqi_ctx* ctx, const float* lifted, const float* nrm, u
             G.ventry0 = entry;
             G.rows = brows;
             G.nrows = nb;
+            G.pol = pol;
             int rc = gather_run(ctx, G, true, 0, nullptr);
             if (rc) return rc;
             HCHK(launch_vamana_write(brows, nb, O.as<uint32_t>(), ocnt.as<uint32_t>(), M, links, deg,
@@ -1560,6 +1636,7 @@ static int vamana_refine(anqi_ctx* ctx, const float* lifted, const float* nrm, u
             P.roff = roff.as<uint32_t>(); P.rids = rids.as<uint32_t>(); P.rmap = brows;
             P.M = M; P.alpha = alpha; P.links = links; P.deg = deg; P.status = status.as<uint32_t>();
             P.rows = tl.as<uint32_t>(); P.nrows = nt;
+            P.pol = pol;
             rc = commit_run(ctx, P, maxin);
             if (rc) return rc;
         }
@@ -1622,6 +1699,9 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     G.pool_cap = std::max<uint32_t>(p->M, (p->ef_alpha ? p->ef_alpha : 8) * G.explore);
     G.M = p->M; G.alpha = p->alpha; G.out_ids = O.as<uint32_t>(); G.out_cnt = ocnt.as<uint32_t>();
     G.batched = p->form ? 1u : 0u;
+    EdgePolicyDev pol;
+    if (!edge_policy_dev(ctx, p->edge, p->M, pol)) return ANQI_E_USAGE;
+    G.pol = pol;
     uint64_t work[2] = {0, 0};
     rc = gather_run(ctx, G, true, maxrev, &S.overflow_rows, work);
     if (rc) return rc;
@@ -1631,15 +1711,15 @@ int anqi_graph_build(anqi_ctx* ctx, const float* lifted, uint32_t n, uint32_t dl
     double t4 = now_s();
     S.gather_s = t4 - t3;
     progress("graph: gather+prune %.2fs (overflow rows %u); commit", S.gather_s, S.overflow_rows);
-    rc = anqi_commit(ctx, lifted, nrm.as<float>(), n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), p->M, p->alpha,
-                     links, deg);
+    rc = commit_policy(ctx, lifted, nrm.as<float>(), n, dl, O.as<uint32_t>(), ocnt.as<uint32_t>(), p->M, p->alpha,
+                       links, deg, pol);
     if (rc) return rc;
     HCHK(hipStreamSynchronize(s));
     double t5 = now_s();
     S.commit_s = t5 - t4;
     if (p->vamana) {
         rc = vamana_refine(ctx, lifted, nrm.as<float>(), n, dl, hidx.as<uint32_t>(), hval.as<float>(), cap, p, tree,
-                           links, deg);
+                           pol, links, deg);
         if (rc) return rc;
         HCHK(hipStreamSynchronize(s));
         S.vamana_s = now_s() - t5;

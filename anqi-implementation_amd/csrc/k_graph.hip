@@ -22,6 +22,7 @@
 // heap rule.  The result is deterministic.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "edge_policy.hpp"
 
 // ---- heap rule (one thread) ---------------------------------------------------
 __device__ __forceinline__ int heap_push(uint32_t* idx, float* val, uint32_t cap, uint32_t node,
@@ -479,6 +480,7 @@ struct GatherLds {
     uint64_t* cs;
     float* selv;      // prune: selected vectors (aliases the hash region)
     uint32_t* sel;
+    char* pscratch;   // edge-policy prune scratch (after every other region)
 };
 
 __host__ __device__ __forceinline__ size_t rnd16(size_t b) { return (b + 15) & ~size_t(15); }
@@ -525,7 +527,8 @@ bool gather_register_lists(const GatherParams& P, bool graph) { return gather_rk
 
 size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
     size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(gather_xbytes(P, graph) * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
-    return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P, graph)) + (rk ? 0 : rnd16(8 * 64));
+    return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P, graph)) + (rk ? 0 : rnd16(8 * 64)) +
+           ((graph && P.pol.on) ? rnd16(policy_scratch_bytes(std::max<uint32_t>(P.pmax, 64))) : 0);
 }
 
 // rk: register lists; the queue's exchange area is qd0 (explore entries), the
@@ -564,6 +567,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
     s.tid = (uint32_t*)take(4 * (size_t)gather_tcap(P, graph));
     s.tdv = (float*)take(4 * (size_t)gather_tcap(P, graph));
     s.cs = rk ? nullptr : (uint64_t*)take(8 * 64);
+    s.pscratch = (graph && P.pol.on) ? take(policy_scratch_bytes(max(P.pmax, 64u))) : nullptr;
     return s;
 }
 
@@ -957,7 +961,15 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
     while (cap2 < w) cap2 <<= 1;
     if (cap2 > P.pmax) { if (lane == 0) { P.status[i] = 2; P.out_cnt[i] = 0; } return; }
     lds_sort_pairs(S.pd, S.pi, w, cap2);
-    uint32_t deg = rng_select(P, S.pd, S.pi, w, S.selv, S.sel, P.out_ids + (size_t)i * P.M);
+    uint32_t deg;
+    if (P.pol.on) {
+        // the work area ends with the policy's scratch (global work area only)
+        __shared__ float pol_nrm[64];
+        deg = policy_select(P.x, P.nrm, P.n, d, P.M, P.alpha, P.pol, i, S.pd, S.pi, w, S.pscratch, cap2, S.selv, S.sel,
+                            pol_nrm, P.out_ids + (size_t)i * P.M);
+    } else {
+        deg = rng_select(P, S.pd, S.pi, w, S.selv, S.sel, P.out_ids + (size_t)i * P.M);
+    }
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
 }
 
@@ -1021,7 +1033,7 @@ hipError_t launch_gather(const GatherParams& P, bool graph, bool gws, hipStream_
 // ---- commit: cand = O_i u R_i, prune to M (include/index.h:907-930) -----------------
 size_t commit_lds_bytes(const PruneParams& P) {
     return 2 * rnd16(4 * (size_t)P.pmax) + rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) +
-           rnd16(4 * 64);
+           rnd16(4 * 64) + (P.pol.on ? rnd16(policy_scratch_bytes(std::max<uint32_t>(P.pmax, 64))) : 0);
 }
 
 // MC = (d >> 4) for the batched distance pass (0: one row per group at a time).
@@ -1077,9 +1089,18 @@ __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
         w += (uint32_t)popc64(km);
         __syncthreads();
     }
-    GatherParams G{};
-    G.x = P.x; G.nrm = P.nrm; G.n = P.n; G.d = P.d; G.M = P.M; G.alpha = P.alpha;
-    uint32_t dg = rng_select(G, pd, pi, w, selv, sel, P.links + (size_t)i * P.M);
+    uint32_t dg;
+    if (P.pol.on) {
+        __shared__ float pol_nrm[64];
+        char* scratch = base + 2 * rnd16(4 * (size_t)P.pmax) + rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) +
+                        rnd16(4 * 64);   // the last region of commit_lds_bytes
+        dg = policy_select(P.x, P.nrm, P.n, d, P.M, P.alpha, P.pol, i, pd, pi, w, scratch, cap2, selv, sel, pol_nrm,
+                           P.links + (size_t)i * P.M);
+    } else {
+        GatherParams G{};
+        G.x = P.x; G.nrm = P.nrm; G.n = P.n; G.d = P.d; G.M = P.M; G.alpha = P.alpha;
+        dg = rng_select(G, pd, pi, w, selv, sel, P.links + (size_t)i * P.M);
+    }
     for (uint32_t t = dg + (uint32_t)lane; t < P.M; t += 64) P.links[(size_t)i * P.M + t] = U32_NONE;
     if (lane == 0) { P.deg[i] = dg; P.status[i] = 0; }
 }

@@ -178,6 +178,18 @@ hipError_t launch_scatter_rev(const uint32_t* ids, uint32_t rows, uint32_t width
                               const uint32_t* rowcnt, uint32_t n, uint32_t* cursor,
                               uint32_t* rids, hipStream_t s);
 
+// An edge policy of the graph prunes (include/index.h:283-829), configured
+// (configure_edge_residual_cover, :305-438); on == 0: the plain prune.
+struct EdgePolicyDev {
+    int on;                     // 0: the plain alpha-RNG prune
+    int resid_cover, ball, ball_pure, occlude_all, lsg;
+    double lsg_alpha, lambda, scale2;
+    const float* mu;            // n lsg weights
+    const float* resid;         // n x nk residuals
+    uint32_t nk, nks, budget;
+    int32_t ks[16];             // 1-based ranks
+};
+
 struct GatherParams {
     const float* x;
     const float* nrm;
@@ -217,6 +229,7 @@ struct GatherParams {
     uint32_t vM;
     const uint32_t* ventry;
     uint32_t ventry0;
+    EdgePolicyDev pol;      // graph mode: the prune's edge policy (work area: global only)
 };
 // rk: register-list form of the batched gather (see k_graph.hip:gather_rk).
 bool gather_register_lists(const GatherParams& P, bool graph_mode);
@@ -242,6 +255,7 @@ struct PruneParams {
     uint32_t* gscratch;
     size_t gscratch_words;
     const uint32_t* rmap;   // optional: rids hold indices into rmap (the ids are rmap[r])
+    EdgePolicyDev pol;      // the prune's edge policy (work area: global only)
 };
 size_t commit_lds_bytes(const PruneParams& P);
 hipError_t launch_commit(const PruneParams& P, bool global_ws, hipStream_t s);

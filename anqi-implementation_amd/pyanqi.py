@@ -39,12 +39,19 @@ class Tree(ctypes.Structure):
                 ("leaf_ids", ctypes.POINTER(U32))]
 
 
+class EdgePolicy(ctypes.Structure):
+    """include/anqi.h anqi_edge_policy (device pointers for mu and resid)."""
+    _fields_ = [("resid_cover", I32), ("ball", I32), ("ball_pure", I32), ("occlude_all", I32), ("lsg", I32),
+                ("lsg_alpha", F64), ("mu", P), ("resid", P), ("nk", U32), ("ks", I32 * 16), ("nks", U32),
+                ("budget", U32), ("lambda_", F64), ("scale2", F64)]
+
+
 class GraphParams(ctypes.Structure):
     _fields_ = [("M", U32), ("nn_k", U32), ("leaf", U32), ("explore", U32), ("ef_alpha", U32),
                 ("iters", I32), ("sample", I32), ("alpha", F32), ("seed", U64), ("form", I32),
                 ("vamana", I32), ("vamana_L", U32), ("vamana_passes", I32), ("vamana_batch_frac", F64),
                 ("vamana_max_batch", U64), ("vamana_batch_base", F64), ("vamana_early_alpha", F32),
-                ("vamana_early_alpha_one", I32), ("vamana_entry", U32)]
+                ("vamana_early_alpha_one", I32), ("vamana_entry", U32), ("edge", ctypes.POINTER(EdgePolicy))]
 
 
 VAMANA_STYLES = {"global": 1, "batch": 2}
@@ -452,7 +459,7 @@ class Context:
     def graph_build(self, lifted, warm_ids, M=48, nn_k=50, leaf=100, explore=100, ef_alpha=8, iters=6, sample=20,
                     alpha=1.2, seed=10, form=0, vamana=None, vamana_L=128, vamana_passes=2, vamana_batch_frac=0.02,
                     vamana_max_batch=1000000, vamana_batch_base=2.0, vamana_early_alpha=1.0,
-                    vamana_early_alpha_one=1, vamana_entry=0):
+                    vamana_early_alpha_one=1, vamana_entry=0, edge=None):
         """anqi_graph_build; vamana: None, "global" or "batch" (ANQI_VAMANA_STYLE);
         iters=0 skips the lifted LocalJoin (ANQI_REUSE_ORIG_KNN)."""
         n, dl = lifted.shape
@@ -462,7 +469,8 @@ class Context:
         st = GraphStats()
         p = GraphParams(M, nn_k, leaf, explore, ef_alpha, iters, sample, alpha, seed, form,
                         VAMANA_STYLES[vamana] if vamana else 0, vamana_L, vamana_passes, vamana_batch_frac,
-                        vamana_max_batch, vamana_batch_base, vamana_early_alpha, vamana_early_alpha_one, vamana_entry)
+                        vamana_max_batch, vamana_batch_base, vamana_early_alpha, vamana_early_alpha_one, vamana_entry,
+                        ctypes.pointer(edge) if edge is not None else None)
         self.call("anqi_graph_build", dp(lifted), n, dl, dp(warm_ids), warm_ids.shape[1], ctypes.byref(p),
                   dp(links), dp(deg), ctypes.byref(t), ctypes.byref(st))
         return links, deg, t, st

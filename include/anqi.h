@@ -265,6 +265,28 @@ int anqi_tree_upload(anqi_ctx* ctx, uint32_t nodes, uint32_t root, uint32_t d,
                      const float* hyper, const uint32_t* entry, anqi_tree* out);
 int anqi_tree_free(anqi_ctx* ctx, anqi_tree* t);
 
+/* An edge policy of the graph's prunes (include/index.h:283-829) as
+ * configure_edge_residual_cover (:305-438) leaves it: every prune of the build
+ * (the gathered pools, commit_edges and the Vamana passes) follows
+ * prune_pool_to_ids's branch for these flags.  All flags zero: the plain
+ * alpha-RNG prune.  bubble_auto / bubble_auto_full have no form here. */
+typedef struct {
+    int resid_cover;    /* "residual_cover" */
+    int ball;           /* "ball_*", "bubble_*" (but bubble_auto*), "lsg_bubble_hybrid" */
+    int ball_pure;      /* "ball_rng", "bubble_rng" */
+    int occlude_all;    /* ANQI_EDGE_BALL_OCCLUSION other than "any" */
+    int lsg;            /* "lsg_rng", "lsg_bubble_hybrid" */
+    double lsg_alpha;   /* ANQI_LSG_ALPHA, 1 */
+    const float* mu;    /* device n: ANQI_LSG_MU_FILE */
+    const float* resid; /* device n*nk: the residual table */
+    uint32_t nk;        /* ANQI_EDGE_NK, 100 */
+    int32_t ks[16];     /* ANQI_EDGE_KS, 1-based ranks <= nk */
+    uint32_t nks;       /* at most 16 */
+    uint32_t budget;    /* min(ANQI_EDGE_RESID_BUDGET, M), 16 */
+    double lambda;      /* ANQI_EDGE_RESID_LAMBDA, 1 */
+    double scale2;      /* the lift scale squared (_anqi_meta.txt), 1 when absent */
+} anqi_edge_policy;
+
 typedef struct {
     uint32_t M;        /* graph degree 48 */
     uint32_t nn_k;     /* LIFTNNK 50 */
@@ -294,6 +316,7 @@ typedef struct {
     int vamana_early_alpha_one;   /* ANQI_VAMANA_EARLY_ALPHA_ONE, 1 */
     uint32_t vamana_entry;        /* the batch style's entry (choose_vamana_entry: 0, an id
                                      mod n, or anqi_vamana_medoid's row) */
+    const anqi_edge_policy* edge; /* NULL: the plain prune */
 } anqi_graph_params;
 typedef struct {
     double tree_s, warm_s, nnd_s, gather_s, commit_s, total_s;

@@ -63,6 +63,8 @@
 #endif
 
 #define OR_EPS 1e-6f   /* include/distance.h:14 */
+static uint32_t prune_sel(const float* x, const float* nrm, uint32_t d, uint32_t base, uint32_t n, float* pd,
+                          uint32_t* pi, uint32_t m, uint32_t M, float alpha, uint32_t* out);
 
 /* GCC _mm512_reduce_add_ps tree (avx512fintrin.h, used by distance.h):
  * T3[i]=a[i+8]+a[i]; T6[i]=T3[i+4]+T3[i]; T8=[T6[0]+T6[2], T6[1]+T6[3]];
@@ -962,11 +964,23 @@ void or_commit(const float* x, const float* nrm, uint32_t n, uint32_t d,
                 uint32_t j = rids[p];
                 pd[m] = or_d2(x + (size_t)i * d, x + (size_t)j * d, nrm[i], nrm[j], d); pi[m] = j; m++;
             }
-            deg[i] = or_prune(x, nrm, d, i, n, pd, pi, m, M, alpha, links + (size_t)i * M);
+            deg[i] = prune_sel(x, nrm, d, i, n, pd, pi, m, M, alpha, links + (size_t)i * M);
         }
         free(pd); free(pi);
     }
     free(roff); free(rids); free(cur);
+}
+
+/* ---- the prune used by commit_edges and the Vamana passes ---------------------- */
+/* or_set_edge_policy: the edge policy every later or_commit / or_vamana_*
+ * prune follows (NULL: the plain prune), as the reference's index applies its
+ * configured policy to each prune_pool_to_ids. */
+static const or_edge_policy* g_edge_policy = 0;
+void or_set_edge_policy(const or_edge_policy* E) { g_edge_policy = E; }
+static uint32_t prune_sel(const float* x, const float* nrm, uint32_t d, uint32_t base, uint32_t n, float* pd,
+                          uint32_t* pi, uint32_t m, uint32_t M, float alpha, uint32_t* out) {
+    if (g_edge_policy) return or_prune_policy(x, nrm, d, base, n, pd, pi, m, M, alpha, g_edge_policy, out);
+    return or_prune(x, nrm, d, base, n, pd, pi, m, M, alpha, out);
 }
 
 /* ---- Vamana refinement and the single graph entry -------------------------- */
@@ -1049,7 +1063,7 @@ void or_vamana_global_pass(const float* x, const float* nrm, uint32_t n, uint32_
             const uint32_t i = (uint32_t)ii;
             uint32_t m = or_vamana_collect(x, nrm, n, d, deg, links, M, i, entries[i], Lb, pd, pi, n);
             m = vam_append_heap(x, nrm, n, d, hidx, cap, i, pd, pi, m);
-            ocnt[i] = or_prune(x, nrm, d, i, n, pd, pi, m, M, alpha, O + (size_t)i * M);
+            ocnt[i] = prune_sel(x, nrm, d, i, n, pd, pi, m, M, alpha, O + (size_t)i * M);
         }
         free(pd); free(pi);
     }
@@ -1211,7 +1225,7 @@ void or_vamana_batch(const float* x, const float* nrm, uint32_t n, uint32_t d, c
                     uint32_t m = or_vamana_collect(x, nrm, n, d, deg, links, M, node, entry, Lb, pd, pi, n);
                     m = vam_append_heap(x, nrm, n, d, hidx, cap, node, pd, pi, m);
                     m = vam_append_links(x, nrm, n, d, deg, links, M, node, pd, pi, m);
-                    ocnt[node] = or_prune(x, nrm, d, node, n, pd, pi, m, M, pa, O + (size_t)node * M);
+                    ocnt[node] = prune_sel(x, nrm, d, node, n, pd, pi, m, M, pa, O + (size_t)node * M);
                 }
                 free(pd); free(pi);
             }
@@ -1252,7 +1266,7 @@ void or_vamana_batch(const float* x, const float* nrm, uint32_t n, uint32_t d, c
                         pd[m] = or_d2(x + (size_t)dst * d, x + (size_t)src * d, nrm[dst], nrm[src], d);
                         pi[m++] = src;
                     }
-                    const uint32_t dg = or_prune(x, nrm, d, dst, n, pd, pi, m, M, pa, out);
+                    const uint32_t dg = prune_sel(x, nrm, d, dst, n, pd, pi, m, M, pa, out);
                     /* rows are pruned from copies, so the writes may follow in any order */
 #pragma omp critical
                     {

@@ -225,6 +225,8 @@ typedef struct {
 uint32_t or_prune_policy(const float* x, const float* nrm, uint32_t d, uint32_t base, uint32_t n, float* pool_d,
                          uint32_t* pool_id, uint32_t m, uint32_t M, float alpha, const or_edge_policy* E,
                          uint32_t* out);
+/* The policy or_commit and or_vamana_* prune with from now on (NULL: plain). */
+void or_set_edge_policy(const or_edge_policy* E);
 
 /* ---- small helpers shared by tests -------------------------------------- */
 void or_sort_pairs(float* d, uint32_t* id, uint32_t m);   /* (dist, id) ascending */

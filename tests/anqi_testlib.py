@@ -82,6 +82,7 @@ ORACLE_SIGS = {
     "or_mt64_next": (U64, [P]),
     "or_shuffle_u32": (None, [P, U32, P]),
     "or_prune_policy": (U32, [P, P, U32, U32, U32, P, P, U32, U32, F32, P, P]),
+    "or_set_edge_policy": (None, [P]),
 }
 
 REF_SIGS = {

@@ -147,3 +147,35 @@ def test_policies_change_the_prune():
                 changed += not (da == db and np.array_equal(a[:da], b[:db]))
             assert changed > 0, name
             write_inputs(folder, n)
+
+
+GRAPH_CASES = [
+    ("residual_cover", None), ("ball_hybrid", None), ("bubble_rng_nk", None), ("lsg_bubble_hybrid", None),
+    ("lsg_rng", "batch"), ("ball_rng", "global"), ("residual_cover_ks", "batch"),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("name,vamana", GRAPH_CASES, ids=[f"{a}-{b}" for a, b in GRAPH_CASES])
+def test_policy_graph_matches_reference_build_index(name, vamana):
+    """The whole graph build under a policy (every prune: the gathered pools,
+    commit_edges and the Vamana passes) equals the reference's single-threaded
+    build_index with ANQI_REUSE_ORIG_KNN, link for link."""
+    from test_oracle_vamana import oracle_graph, reference_graph, same_graph
+    S = Sample()
+    x, ids = np.ascontiguousarray(S.lifted), np.ascontiguousarray(S.rec_ids)
+    n = x.shape[0]
+    with tempfile.TemporaryDirectory() as td:
+        folder = td + "/"
+        kv = dict(POLICIES[name])
+        write_inputs(folder, n, int(kv.get("ANQI_EDGE_NK", "100")))
+        kv["ANQI_LSG_MU_FILE"] = folder + "mu.bin"
+        E = or_edge_policy(edge_policy_config(kv, n, M, folder))
+        vk = {} if vamana is None else dict(vamana=vamana, L=64)
+        rk = {} if vamana is None else dict(ANQI_VAMANA="1", ANQI_VAMANA_STYLE=vamana, ANQI_VAMANA_L="64")
+        want = oracle_graph(x, ids, policy=E, **vk)[:2]
+        got = reference_graph(x, ids, files=(folder + "_anqi_rankm_residual.bin", folder + "_anqi_meta.txt"),
+                              **kv, **rk)[:2]
+        assert same_graph(want, got) == "", (name, vamana)
+        plain = oracle_graph(x, ids, **vk)[:2]
+        assert same_graph(want, plain) != "", (name, vamana)

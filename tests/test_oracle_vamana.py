@@ -111,10 +111,20 @@ def test_vamana_entry_choice_matches_reference():
 
 
 def oracle_graph(x, ids, vamana=None, L=64, passes=2, frac=0.02, max_batch=1000000, base=2.0, early=1.0,
-                 early_one=1, entry=0, seed=10):
+                 early_one=1, entry=0, seed=10, policy=None):
     """The reference's build_index with ANQI_REUSE_ORIG_KNN set, restated:
-    search tree, warm start, gather + prune, commit, then the Vamana passes."""
+    search tree, warm start, gather + prune, commit, then the Vamana passes;
+    every prune under the edge policy (an or_edge_policy) when given."""
     O = oracle()
+    O.or_set_edge_policy(ctypes.byref(policy) if policy is not None else None)
+    try:
+        return _oracle_graph(O, x, ids, vamana, L, passes, frac, max_batch, base, early, early_one, entry, seed,
+                             policy)
+    finally:
+        O.or_set_edge_policy(None)
+
+
+def _oracle_graph(O, x, ids, vamana, L, passes, frac, max_batch, base, early, early_one, entry, seed, policy):
     x = np.ascontiguousarray(x, np.float32)
     ids = np.ascontiguousarray(ids, np.uint32)
     n, d = x.shape
@@ -134,7 +144,11 @@ def oracle_graph(x, ids, vamana=None, L=64, passes=2, frac=0.02, max_batch=10000
         m = O.or_gather(ptr(x), ptr(nrm), n, d, ptr(hidx), ptr(hval), CAP, ptr(roff), ptr(rids), i, EXPLORE, POOL,
                         ptr(pd), ptr(pi), 8 * n)
         out = np.zeros(M, np.uint32)
-        ocnt[i] = O.or_prune(ptr(x), ptr(nrm), d, i, n, ptr(pd), ptr(pi), m, M, ALPHA, ptr(out))
+        if policy is None:
+            ocnt[i] = O.or_prune(ptr(x), ptr(nrm), d, i, n, ptr(pd), ptr(pi), m, M, ALPHA, ptr(out))
+        else:
+            ocnt[i] = O.or_prune_policy(ptr(x), ptr(nrm), d, i, n, ptr(pd), ptr(pi), m, M, ALPHA,
+                                        ctypes.byref(policy), ptr(out))
         Orows[i] = out
     links = np.zeros((n, M), np.uint32)
     deg = np.zeros(n, np.uint32)
@@ -151,10 +165,14 @@ def oracle_graph(x, ids, vamana=None, L=64, passes=2, frac=0.02, max_batch=10000
     return links, deg, hidx, hval
 
 
-def reference_graph(x, ids, **kv):
+def reference_graph(x, ids, files=(), **kv):
+    """files: paths linked into the index's folder (the edge policies'
+    residual table and meta file)."""
     R = ref()
     n = x.shape[0]
     ix = _Idx(x)
+    for f in files:
+        os.link(f, os.path.join(ix.tmp.name, os.path.basename(f)))
     try:
         deg = np.zeros(n, np.uint32)
         links = np.zeros((n, M), np.uint32)
