@@ -79,14 +79,36 @@ inline std::string threshold_source_identity(const CacheInputs& in) {
     return file_stat_identity(in.rec) + ":interleaved_k" + env_or("ANQI_PREBUILT_TOPK_K", "100");
 }
 
-// The edge-policy part of the graph key (rknn_query.cpp:314-324) and of the
-// index name (include/index.h:1745-1767) for the plain policies accepted here.
-inline std::string edge_suffix(const char* sep) {
+// The edge-policy part of the graph key (rknn_query.cpp:314-324).
+inline std::string edge_suffix() {
     if (!env_set("ANQI_EDGE_POLICY")) return "";
-    const std::string s(sep);
-    return "_edge" + s + env_or("ANQI_EDGE_POLICY", "none") + "_eks" + s + env_or("ANQI_EDGE_KS", "10,50,100") +
-           "_enk" + s + env_or("ANQI_EDGE_NK", "100") + "_erb" + s + env_or("ANQI_EDGE_RESID_BUDGET", "16") + "_erl" +
-           s + env_or("ANQI_EDGE_RESID_LAMBDA", "1.0") + "_ebo" + s + env_or("ANQI_EDGE_BALL_OCCLUSION", "all");
+    std::string k = std::string("_edge") + env_or("ANQI_EDGE_POLICY", "none") + "_eks" +
+                    env_or("ANQI_EDGE_KS", "10,50,100") + "_enk" + env_or("ANQI_EDGE_NK", "100") + "_erb" +
+                    env_or("ANQI_EDGE_RESID_BUDGET", "16") + "_erl" + env_or("ANQI_EDGE_RESID_LAMBDA", "1.0") +
+                    "_ebo" + env_or("ANQI_EDGE_BALL_OCCLUSION", "all");
+    if (env_set("ANQI_EDGE_RESID_FILE")) k += "_erf" + hash16(fingerprint(env_or("ANQI_EDGE_RESID_FILE", "")));
+    return k;
+}
+
+// The edge-policy part of the index name (generate_config, include/index.h:1745-1767).
+inline std::string edge_config_suffix() {
+    if (!env_set("ANQI_EDGE_POLICY")) return "";
+    const std::string pol = env_or("ANQI_EDGE_POLICY", "none");
+    std::string c = "_edge_" + pol;
+    if (pol == "lsg_rng" || pol == "lsg_bubble_hybrid") {
+        std::string mu_key = env_or("ANQI_LSG_CACHE_KEY", "");
+        if (mu_key.empty()) mu_key = env_or("ANQI_LSG_MU_FILE", "");
+        c += std::string("_lsa_") + env_or("ANQI_LSG_ALPHA", "1.0") + "_lsgmu_" + hash16(mu_key);
+    }
+    if (pol != "lsg_rng") {
+        c += std::string("_eks_") + env_or("ANQI_EDGE_KS", "10,50,100") + "_enk_" + env_or("ANQI_EDGE_NK", "100") +
+             "_erb_" + env_or("ANQI_EDGE_RESID_BUDGET", "16") + "_erl_" + env_or("ANQI_EDGE_RESID_LAMBDA", "1.0") +
+             "_ebo_" + env_or("ANQI_EDGE_BALL_OCCLUSION", "all");
+        if (pol == "bubble_auto" || pol == "bubble_auto_full")
+            c += std::string("_eac_") + env_or("ANQI_EDGE_AUTO_CANDIDATES", "M") + "_eas_" +
+                 env_or("ANQI_EDGE_AUTO_MAX_SWAPS", "M");
+    }
+    return c;
 }
 
 // The Vamana part of the graph key (rknn_query.cpp:325-340): the style and
@@ -122,7 +144,7 @@ inline CacheNames cache_names(const CacheInputs& in) {
     std::string g = "M" + std::to_string(in.M) + "_leaf" + std::to_string(in.leaf) + "_nt" + std::to_string(in.trees) +
                     "_ef" + std::to_string(in.ef) + "_geomanyk_lift_rankm0_specific0_alpha" + env_or("ANQI_ALPHA", "1.2") +
                     "_liftnnk" + env_or("ANQI_LIFTNNK", "50") + "_graphnnd" + env_or("ANQI_GRAPHNND", "6") +
-                    (env_set("ANQI_REUSE_ORIG_KNN") ? "_reuseorig1" : "") + edge_suffix("") + vamana_suffix();
+                    (env_set("ANQI_REUSE_ORIG_KNN") ? "_reuseorig1" : "") + edge_suffix() + vamana_suffix();
     g = sanitize_tag(g);
     // the post-filter verifier's graph is keyed by its cutoff source instead
     // of the radius key
@@ -147,7 +169,7 @@ inline CacheNames cache_names(const CacheInputs& in) {
     const std::string cfg = "fd_metric_l2_M_" + std::to_string(in.M) + "_nnk_" + std::to_string(in.lift_nnk) +
                             "_ntrees_64_n_" + std::to_string(in.n) + "_dim_" + std::to_string(in.dim) + "_efa_" +
                             std::to_string(8.0f) + "_pa_" + std::to_string(prune_alpha) + "_er_100_graphnnd_" +
-                            env_or("ANQI_GRAPHNND", "6") + "_gk_" + hash16(c.graph_key) + edge_suffix("_") +
+                            env_or("ANQI_GRAPHNND", "6") + "_gk_" + hash16(c.graph_key) + edge_config_suffix() +
                             (env_set("ANQI_REUSE_ORIG_KNN") ? "_reuseorig1" : "");
     c.graph_index = in.graph_dir + sanitize_tag(cfg);
     return c;

@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <sstream>
 #include <cmath>
 #include <iomanip>
 #include <map>
@@ -117,14 +118,11 @@ const std::vector<Pin>& pins() {
         {"ANQI_SEARCH_SCORE", {"center"}},
         {"ANQI_BETA", {"0.5"}},
         {"ANQI_RCEPS", {"0", "0.0"}},
-        {"ANQI_EDGE_POLICY", {"rng", "default"}},
         {"ANQI_CANDIDATE_ENVELOPE", {"aknn", "aknn_k100", "default"}},
     };
     return p;
 }
-// Switched on by their presence alone, whatever the value (the reference
-// tests them with getenv alone), or by any non-empty value.
-const char* const kPresence[] = {"ANQI_EDGE_RESID_FILE"};
+// Switched on by any non-empty value.
 
 // A set variable is read with atoi / atof, even when empty (index.h:178-186).
 int envi(const char* k, int d) {
@@ -134,6 +132,119 @@ int envi(const char* k, int d) {
 double envd(const char* k, double d) {
     const char* v = std::getenv(k);
     return v ? std::atof(v) : d;
+}
+
+// A file of exactly `count` floats into v; the reference's edge-policy
+// messages on failure (exit 2).
+bool read_floats_exact(const std::string& path, size_t count, std::vector<float>& v, const char* what) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        std::fprintf(stderr, "[edge-policy] %s requested but missing %s\n", what, path.c_str());
+        return false;
+    }
+    f.seekg(0, std::ios::end);
+    const std::streamoff got = f.tellg();
+    if (got < 0 || (size_t)got != count * sizeof(float)) {
+        std::fprintf(stderr, "[edge-policy] %s size mismatch %s: expected %zu bytes, got %lld\n", what, path.c_str(),
+                     count * sizeof(float), (long long)got);
+        return false;
+    }
+    f.seekg(0, std::ios::beg);
+    v.resize(count);
+    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(count * sizeof(float)));
+    if ((size_t)f.gcount() != count * sizeof(float)) {
+        std::fprintf(stderr, "[edge-policy] short %s %s\n", what, path.c_str());
+        return false;
+    }
+    return true;
+}
+
+// configure_edge_residual_cover (include/index.h:305-438): ANQI_EDGE_POLICY
+// and its inputs as the lifted index reads them (folder: the graph cache
+// directory).  The flags stay zero when the policy leaves the plain prune.
+// Host copies of the tables in resid / mu.  Returns 0, or 2 after a message.
+int configure_edge_policy(uint32_t n, uint32_t M, const std::string& folder, anqi_edge_policy& E,
+                          std::vector<float>& resid, std::vector<float>& mu) {
+    E = anqi_edge_policy{};
+    E.lsg_alpha = 1.0; E.lambda = 1.0; E.scale2 = 1.0; E.occlude_all = 1;
+    const char* pv = std::getenv("ANQI_EDGE_POLICY");
+    if (!pv || !*pv) return 0;
+    const std::string pol(pv);
+    const bool lsg_policy = pol == "lsg_rng" || pol == "lsg_bubble_hybrid";
+    if (lsg_policy) {
+        E.lsg_alpha = envd("ANQI_LSG_ALPHA", 1.0);
+        if (!std::isfinite(E.lsg_alpha) || E.lsg_alpha < 0.0) {
+            std::fprintf(stderr, "[edge-policy] invalid ANQI_LSG_ALPHA=%.17g\n", E.lsg_alpha);
+            return 2;
+        }
+        const char* mp = std::getenv("ANQI_LSG_MU_FILE");
+        if (!mp || !*mp) {
+            std::fprintf(stderr, "[edge-policy] lsg_rng requires ANQI_LSG_MU_FILE\n");
+            return 2;
+        }
+        if (!read_floats_exact(mp, n, mu, "lsg mu table")) return 2;
+        double mn = INFINITY, mx = 0.0;
+        long double sum = 0.0;
+        for (size_t i = 0; i < n; i++) {
+            const double m = mu[i];
+            if (!std::isfinite(m) || m <= 0.0) {
+                std::fprintf(stderr, "[edge-policy] invalid lsg mu[%zu]=%.9g\n", i, m);
+                return 2;
+            }
+            mn = std::min(mn, m);
+            mx = std::max(mx, m);
+            sum += m;
+        }
+        E.lsg = 1;
+        std::fprintf(stderr, "[edge-policy] %s lsg enabled alpha=%.6g mu=%s min=%.6g mean=%.6g max=%.6g\n",
+                     pol.c_str(), E.lsg_alpha, mp, mn, n ? (double)(sum / n) : 0.0, mx);
+        if (pol == "lsg_rng") return 0;
+    }
+    static const char* const kResid[] = {"residual_cover", "ball_rng", "ball_hybrid", "ball_multik",
+                                         "bubble_rng", "bubble_hybrid", "bubble_multik", "lsg_bubble_hybrid"};
+    if (std::find(std::begin(kResid), std::end(kResid), pol) == std::end(kResid)) return 0;
+    uint32_t nk = (uint32_t)std::max(0, envi("ANQI_EDGE_NK", 100));
+    if (nk == 0) nk = 100;
+    E.budget = (uint32_t)std::min<int64_t>(std::max(0, envi("ANQI_EDGE_RESID_BUDGET", 16)), (int64_t)M);
+    E.lambda = envd("ANQI_EDGE_RESID_LAMBDA", 1.0);
+    {
+        const char* kv = std::getenv("ANQI_EDGE_KS");
+        std::stringstream ss(kv ? kv : "10,50,100");
+        std::string tok;
+        while (std::getline(ss, tok, ',')) {
+            if (tok.empty()) continue;
+            const int k = std::atoi(tok.c_str());
+            if (k < 1 || (uint32_t)k > nk) continue;
+            if (E.nks == 16) {
+                std::fprintf(stderr, "[rknn] ANQI_EDGE_KS with more than 16 ranks is not supported by the GPU build\n");
+                return 2;
+            }
+            E.ks[E.nks++] = k;
+        }
+    }
+    if (E.nks == 0) return 0;
+    if (E.budget == 0 && pol != "ball_rng") return 0;
+    {
+        std::ifstream mf(folder + "_anqi_meta.txt");
+        double mstar = 0.0, scale = 1.0;
+        if (mf >> mstar >> scale) E.scale2 = scale * scale;
+    }
+    const char* rf = std::getenv("ANQI_EDGE_RESID_FILE");
+    const std::string rpath = rf ? std::string(rf) : folder + "_anqi_rankm_residual.bin";
+    if (!read_floats_exact(rpath, (size_t)n * nk, resid, "residual table")) return 2;
+    E.nk = nk;
+    E.resid_cover = 1;
+    if (pol != "residual_cover") {
+        E.resid_cover = 0;
+        E.ball = 1;
+        E.ball_pure = pol == "ball_rng" || pol == "bubble_rng";
+        const char* occ = std::getenv("ANQI_EDGE_BALL_OCCLUSION");
+        E.occlude_all = !(occ && std::string(occ) == "any");
+    }
+    std::fprintf(stderr, "[edge-policy] %s enabled ks=%s budget=%u lambda=%.4g scale2=%.6g nk=%u%s%s\n", pol.c_str(),
+                 std::getenv("ANQI_EDGE_KS") ? std::getenv("ANQI_EDGE_KS") : "10,50,100", E.budget, E.lambda,
+                 E.scale2, E.nk, E.ball ? " ball_occlusion=" : "", E.ball ? (E.occlude_all ? "all" : "any") : "");
+    return 0;
 }
 const char* const kNonEmpty[] = {"ANQI_GRAPH_RADIUS_SOURCE_PFX"};
 
@@ -216,11 +327,13 @@ void check_configuration() {
         std::fprintf(stderr, "[rknn] independent postfilter verifier requires a shared graph; unset ANQI_SPECIFIC\n");
         std::exit(2);
     }
-    for (const char* name : kPresence)
-        if (std::getenv(name)) {
-            std::fprintf(stderr, "[rknn] %s is not supported by the GPU build (unset it)\n", name);
+    {
+        const char* pol = std::getenv("ANQI_EDGE_POLICY");
+        if (pol && (std::string(pol) == "bubble_auto" || std::string(pol) == "bubble_auto_full")) {
+            std::fprintf(stderr, "[rknn] ANQI_EDGE_POLICY=%s is not supported by the GPU build\n", pol);
             std::exit(2);
         }
+    }
     for (const Pin& p : pins()) {
         const char* v = std::getenv(p.name);
         if (!v || !*v) continue;
@@ -802,8 +915,20 @@ int main(int argc, char** argv) {
                                                              (double)n), gp.vamana_max_batch),
                              gp.vamana_batch_base > 1.0 ? gp.vamana_batch_base : 2.0, 1.2);
         }
+        // the edge policy of every prune (configure_edge_residual_cover runs
+        // when the index is built, index.h:1079)
+        anqi_edge_policy edge{};
+        std::vector<float> edge_resid, edge_mu;
+        if (int rc = configure_edge_policy(n, (uint32_t)Marg, graph_dir, edge, edge_resid, edge_mu)) return rc;
+        float* d_edge_resid = edge_resid.empty() ? nullptr : upload(edge_resid);
+        float* d_edge_mu = edge_mu.empty() ? nullptr : upload(edge_mu);
+        edge.resid = d_edge_resid;
+        edge.mu = d_edge_mu;
+        if (edge.resid_cover || edge.ball || edge.lsg) gp.edge = &edge;
         anqi_graph_stats gst{};
         ck(anqi_graph_build(g_ctx, d_lifted, n, dl, d_warm, 100, &gp, d_links, d_deg, &tree, &gst), "graph build");
+        if (d_edge_resid) anqi_dfree(g_ctx, d_edge_resid);
+        if (d_edge_mu) anqi_dfree(g_ctx, d_edge_mu);
         if (d_warm != d_aknn) anqi_dfree(g_ctx, d_warm);
         std::vector<uint32_t> hdeg, hlinks, left(tree.nodes), right(tree.nodes);
         std::vector<float> offset(tree.nodes), hyper((size_t)tree.nodes * dl);

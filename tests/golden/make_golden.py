@@ -388,19 +388,38 @@ SAMPLE_VARIANTS = {
     "vamana_batch": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_VAMANA": "1", "ANQI_VAMANA_STYLE": "batch",
                      "ANQI_VAMANA_L": "64", "ANQI_VAMANA_PASS": "3", "ANQI_VAMANA_ENTRY": "medoid"},
     "single_entry": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_QUERY_SINGLE_ENTRY": "1", "ANQI_VAMANA_ENTRY": "7"},
+    # edge policies over a residual table and lsg weights written beside the
+    # fixture ("{dir}" is the fixture directory)
+    "edge_ball": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_EDGE_POLICY": "ball_hybrid",
+                  "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin"},
+    "edge_cover_vamana": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_EDGE_POLICY": "residual_cover", "ANQI_EDGE_KS": "5,50",
+                          "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin", "ANQI_VAMANA": "1",
+                          "ANQI_VAMANA_STYLE": "batch", "ANQI_VAMANA_L": "64"},
+    "edge_lsg": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_EDGE_POLICY": "lsg_bubble_hybrid",
+                 "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin", "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin"},
 }
+
+
+def write_edge_inputs(out_dir, n=256, nk=100):
+    """A signed residual table on the scale of the sample's lifted distances,
+    and lsg weights in [0.5, 2]."""
+    rng = np.random.default_rng(3)
+    (rng.standard_normal((n, nk)) * 2.0e4).astype(np.float32).tofile(os.path.join(out_dir, "edge_resid.bin"))
+    rng.uniform(0.5, 2.0, n).astype(np.float32).tofile(os.path.join(out_dir, "edge_mu.bin"))
 
 
 def make_sample_variants(out_dir, sample_dir):
     """For each variant: the reference's graph cache file and its [rknn]
     stdout lines at k = 10 on the sample fixture."""
     os.makedirs(out_dir, exist_ok=True)
+    write_edge_inputs(out_dir)
     with tempfile.TemporaryDirectory() as work:
         pfx = os.path.join(work, "sample")
         for suf in ["_base.bin", "_query.bin", "_baseknn_aknn.rec", "_radbasis_M8.bin",
                     "_learned_rq_codebook_M8_u8.bin", "_rknn_gt_k10.bin", "_lifted.meta"]:
             shutil.copy(os.path.join(sample_dir, "sample" + suf), pfx + suf)
         for name, extra in SAMPLE_VARIANTS.items():
+            extra = {k: v.replace("{dir}", out_dir) for k, v in extra.items()}
             env = dict(PIN_ENV, OMP_NUM_THREADS="4", ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec",
                        ANQI_AKNN_BUILD_SECONDS="0", ANQI_VERIFIER_TRAIN_SECONDS="0", **extra)
             env["ANQI_GRAPH_CACHE_DIR"] = os.path.join(work, name, "graph")

@@ -93,8 +93,8 @@ def test_rknn_query_rejects_unpinned_configuration():
     instead of silently computing the pinned one."""
     pfx = os.path.join(GOLDEN, "sample", "sample")
     for env in ({"ANQI_RADIUS_MODE": "approx"}, {"ANQI_VERIFIER_PLACEMENT": "integrated"},
-                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_EDGE_RESID_FILE": "x"},
-                {"ANQI_EDGE_POLICY": "ball_rng"}, {"ANQI_VAMANA": "0", "ANQI_VAMANA_L": "2000"},
+                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_EDGE_POLICY": "bubble_auto"},
+                {"ANQI_EDGE_POLICY": "bubble_auto_full"}, {"ANQI_VAMANA": "0", "ANQI_VAMANA_L": "2000"},
                 {"ANQI_RADIUS_RELAX": "0.5"}):
         r = run("rknn_query", ["16", pfx, "48", "500", "64", "600", "10"], env)
         assert r.returncode == 2, (env, r.returncode, r.stderr)
@@ -214,7 +214,12 @@ def test_u8_base_cache_key_and_bad_files_equal_reference(tmp_path):
      "ANQI_REUSE_ORIG_KNN": "0"},
     {"ANQI_VAMANA": "1", "ANQI_VAMANA_STYLE": "old", "ANQI_VAMANA_PASS": "1", "ANQI_VAMANA_BATCH_FRAC": "0.1",
      "ANQI_VAMANA_EARLY_ALPHA": "1.05", "ANQI_QUERY_SINGLE_ENTRY": "1"},
-], ids=["reuse", "vamana", "vamana_batch", "vamana_old"])
+    {"ANQI_EDGE_POLICY": "ball_hybrid", "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin", "ANQI_EDGE_KS": "5,50"},
+    {"ANQI_EDGE_POLICY": "lsg_bubble_hybrid", "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin",
+     "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin", "ANQI_LSG_ALPHA": "0.75", "ANQI_REUSE_ORIG_KNN": "1"},
+    {"ANQI_EDGE_POLICY": "lsg_rng", "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin", "ANQI_VAMANA": "1"},
+    {"ANQI_EDGE_POLICY": "rng"},
+], ids=["reuse", "vamana", "vamana_batch", "vamana_old", "edge_ball", "edge_lsg_hybrid", "edge_lsg", "edge_rng"])
 def test_variant_cache_names_equal_reference(tmp_path, extra):
     """The build variants (ANQI_REUSE_ORIG_KNN, ANQI_VAMANA* in both styles)
     key the graph cache as the reference does (src/rknn_query.cpp:301-341,
@@ -223,6 +228,8 @@ def test_variant_cache_names_equal_reference(tmp_path, extra):
     pfx = str(tmp_path / "sample")
     for suf in SAMPLE_FILES:
         shutil.copy(os.path.join(GOLDEN, "sample", "sample" + suf), pfx + suf)
+    vdir = os.path.join(GOLDEN, "sample_variants")
+    extra = {k: v.replace("{dir}", vdir) for k, v in extra.items()}
     env = dict(PIN_ENV, ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec",
                ANQI_GRAPH_CACHE_DIR=str(tmp_path / "graph"), ANQI_VERIFIER_CACHE_DIR=str(tmp_path / "verifier"),
                **extra)
@@ -233,6 +240,7 @@ def test_variant_cache_names_equal_reference(tmp_path, extra):
     assert rr.returncode == 0, rr.stderr
     key = re.search(r"graph cache key=(\S+)", rr.stdout).group(1)
     assert ("_vam1_style" in key) == ("ANQI_VAMANA" in extra) and ("_reuseorig1" in key) == ("ANQI_REUSE_ORIG_KNN" in extra)
+    assert ("_edge" in key) == ("ANQI_EDGE_POLICY" in extra)
     r = run("rknn_query", args, dict(env, ANQI_REQUIRE_GRAPH_CACHE="1"))
     assert f"[rknn] graph cache key={key}\n" in r.stdout
     assert "required graph cache" not in r.stderr

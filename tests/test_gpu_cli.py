@@ -255,17 +255,19 @@ def test_ingestion_options_give_the_same_answers(tmp_path):
     assert r.returncode == 3 and "streamed warm-start ended early" in r.stderr
 
 
-@pytest.mark.parametrize("variant", ["reuse", "vamana_global", "vamana_batch", "single_entry"])
+@pytest.mark.parametrize("variant", ["reuse", "vamana_global", "vamana_batch", "single_entry", "edge_ball",
+                                     "edge_cover_vamana", "edge_lsg"])
 def test_build_variants_equal_reference(tmp_path, variant):
     """Build variants of the reference (tests/golden/sample_variants, its own
     rknn_query on the sample): ANQI_REUSE_ORIG_KNN (no lifted LocalJoin,
     include/index.h:1202-1238), ANQI_VAMANA in both styles (:932-1063,
-    1262-1282) and ANQI_QUERY_SINGLE_ENTRY (:1606-1607).  The graph written
-    here equals the reference's link for link, and the answer lines are its."""
+    1262-1282), ANQI_QUERY_SINGLE_ENTRY (:1606-1607) and ANQI_EDGE_POLICY
+    (:283-829) over a residual table and lsg weights.  The graph written here
+    equals the reference's link for link, and the answer lines are its."""
     import json
     from anqi_testlib import read_graph
     vdir = os.path.join(GOLDEN, "sample_variants")
-    extra = json.load(open(os.path.join(vdir, "variants.json")))[variant]
+    extra = {k: v.replace("{dir}", vdir) for k, v in json.load(open(os.path.join(vdir, "variants.json")))[variant].items()}
     src = os.path.join(GOLDEN, "sample", "sample")
     env = {k: v for k, v in os.environ.items() if not k.startswith("ANQI_")}
     env.update(PIN_ENV)
