@@ -72,12 +72,13 @@ inline std::string base_vector_source(const std::string& fbin_path) {
 // PFX_baseknn_aknn.bin; this tool builds them on the GPU from the base and
 // the AKNN parameters instead, so its identity names those, and no cache of
 // a reference run is taken for one of these runs or the other way round.
-inline std::string threshold_source_identity(const CacheInputs& in) {
+inline std::string threshold_source_identity(const CacheInputs& in, const std::string& pfx) {
     if (in.rec.empty())
-        return file_stat_identity(base_vector_source(in.pfx + "_base.bin")) + ":gpu_aknn:leaf" + std::to_string(in.leaf) + ":nt" +
-               std::to_string(in.trees) + ":ef" + std::to_string(in.ef) + ":k100";
+        return file_stat_identity(base_vector_source(pfx + "_base.bin")) + ":gpu_aknn:leaf" + std::to_string(in.leaf) +
+               ":nt" + std::to_string(in.trees) + ":ef" + std::to_string(in.ef) + ":k100";
     return file_stat_identity(in.rec) + ":interleaved_k" + env_or("ANQI_PREBUILT_TOPK_K", "100");
 }
+inline std::string threshold_source_identity(const CacheInputs& in) { return threshold_source_identity(in, in.pfx); }
 
 // The edge-policy part of the graph key (rknn_query.cpp:314-324).
 inline std::string edge_suffix() {
@@ -147,11 +148,13 @@ inline CacheNames cache_names(const CacheInputs& in) {
                     (env_set("ANQI_REUSE_ORIG_KNN") ? "_reuseorig1" : "") + edge_suffix() + vamana_suffix();
     g = sanitize_tag(g);
     // the post-filter verifier's graph is keyed by its cutoff source instead
-    // of the radius key
+    // of the radius key, the dataset named by ANQI_GRAPH_RADIUS_SOURCE_PFX
+    // when it is set (rknn_query.cpp:1843-1853)
+    const std::string gsrc = env_or("ANQI_GRAPH_RADIUS_SOURCE_PFX", in.pfx.c_str());
     const std::string radius_id = env_set("ANQI_GRAPH_RADIUS_ID")
                                       ? std::string(env_or("ANQI_GRAPH_RADIUS_ID", "aknn_k100_v1"))
-                                      : file_stat_identity(base_vector_source(in.pfx + "_base.bin")) + "|" +
-                                            c.threshold_source + ":k100:v2";
+                                      : file_stat_identity(base_vector_source(gsrc + "_base.bin")) + "|" +
+                                            threshold_source_identity(in, gsrc) + ":k100:v2";
     c.graph_key = sanitize_tag(g + "_graph_cutoff_k100_" + hash16(radius_id));
     c.state_key = "graph=" + c.graph_key;
     const std::string tag = hash16(c.graph_key);

@@ -122,7 +122,6 @@ const std::vector<Pin>& pins() {
     };
     return p;
 }
-// Switched on by any non-empty value.
 
 // A set variable is read with atoi / atof, even when empty (index.h:178-186).
 int envi(const char* k, int d) {
@@ -246,7 +245,6 @@ int configure_edge_policy(uint32_t n, uint32_t M, const std::string& folder, anq
                  E.scale2, E.nk, E.ball ? " ball_occlusion=" : "", E.ball ? (E.occlude_all ? "all" : "any") : "");
     return 0;
 }
-const char* const kNonEmpty[] = {"ANQI_GRAPH_RADIUS_SOURCE_PFX"};
 
 // Base vectors: the fbin, or ANQI_BASE_U8BIN's uint8 file read as floats
 // (load_base_vectors, rknn_query.cpp:55-123).  Exits 3 on a bad file.
@@ -342,13 +340,6 @@ void check_configuration() {
                          "[rknn] %s=%s is not supported by the GPU build (supported: %s; this build implements the "
                          "configuration pinned by scripts/run_final.sh)\n",
                          p.name, v, p.ok[0].c_str());
-            std::exit(2);
-        }
-    }
-    for (const char* name : kNonEmpty) {
-        const char* v = std::getenv(name);
-        if (v && *v) {
-            std::fprintf(stderr, "[rknn] %s is not supported by the GPU build\n", name);
             std::exit(2);
         }
     }

@@ -219,7 +219,9 @@ def test_u8_base_cache_key_and_bad_files_equal_reference(tmp_path):
      "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin", "ANQI_LSG_ALPHA": "0.75", "ANQI_REUSE_ORIG_KNN": "1"},
     {"ANQI_EDGE_POLICY": "lsg_rng", "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin", "ANQI_VAMANA": "1"},
     {"ANQI_EDGE_POLICY": "rng"},
-], ids=["reuse", "vamana", "vamana_batch", "vamana_old", "edge_ball", "edge_lsg_hybrid", "edge_lsg", "edge_rng"])
+    {"ANQI_GRAPH_RADIUS_SOURCE_PFX": "{golden}/sample"},
+], ids=["reuse", "vamana", "vamana_batch", "vamana_old", "edge_ball", "edge_lsg_hybrid", "edge_lsg", "edge_rng",
+        "radius_source_pfx"])
 def test_variant_cache_names_equal_reference(tmp_path, extra):
     """The build variants (ANQI_REUSE_ORIG_KNN, ANQI_VAMANA* in both styles)
     key the graph cache as the reference does (src/rknn_query.cpp:301-341,
@@ -229,7 +231,7 @@ def test_variant_cache_names_equal_reference(tmp_path, extra):
     for suf in SAMPLE_FILES:
         shutil.copy(os.path.join(GOLDEN, "sample", "sample" + suf), pfx + suf)
     vdir = os.path.join(GOLDEN, "sample_variants")
-    extra = {k: v.replace("{dir}", vdir) for k, v in extra.items()}
+    extra = {k: v.replace("{dir}", vdir).replace("{golden}", os.path.join(GOLDEN, "sample")) for k, v in extra.items()}
     env = dict(PIN_ENV, ANQI_PREBUILT_TOPK_REC=pfx + "_baseknn_aknn.rec",
                ANQI_GRAPH_CACHE_DIR=str(tmp_path / "graph"), ANQI_VERIFIER_CACHE_DIR=str(tmp_path / "verifier"),
                **extra)
