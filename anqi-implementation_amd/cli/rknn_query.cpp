@@ -199,12 +199,15 @@ int configure_edge_policy(uint32_t n, uint32_t M, const std::string& folder, anq
                      pol.c_str(), E.lsg_alpha, mp, mn, n ? (double)(sum / n) : 0.0, mx);
         if (pol == "lsg_rng") return 0;
     }
-    static const char* const kResid[] = {"residual_cover", "ball_rng", "ball_hybrid", "ball_multik",
-                                         "bubble_rng", "bubble_hybrid", "bubble_multik", "lsg_bubble_hybrid"};
+    static const char* const kResid[] = {"residual_cover", "ball_rng",      "ball_hybrid",   "ball_multik",
+                                         "bubble_rng",     "bubble_hybrid", "bubble_multik", "bubble_auto",
+                                         "bubble_auto_full", "lsg_bubble_hybrid"};
     if (std::find(std::begin(kResid), std::end(kResid), pol) == std::end(kResid)) return 0;
     uint32_t nk = (uint32_t)std::max(0, envi("ANQI_EDGE_NK", 100));
     if (nk == 0) nk = 100;
     E.budget = (uint32_t)std::min<int64_t>(std::max(0, envi("ANQI_EDGE_RESID_BUDGET", 16)), (int64_t)M);
+    E.auto_candidates = (uint32_t)std::max(1, envi("ANQI_EDGE_AUTO_CANDIDATES", (int)M));
+    E.auto_max_swaps = (uint32_t)std::max(1, envi("ANQI_EDGE_AUTO_MAX_SWAPS", (int)M));
     E.lambda = envd("ANQI_EDGE_RESID_LAMBDA", 1.0);
     {
         const char* kv = std::getenv("ANQI_EDGE_KS");
@@ -222,7 +225,8 @@ int configure_edge_policy(uint32_t n, uint32_t M, const std::string& folder, anq
         }
     }
     if (E.nks == 0) return 0;
-    if (E.budget == 0 && pol != "ball_rng") return 0;
+    const bool bubble_auto = pol == "bubble_auto" || pol == "bubble_auto_full";
+    if (E.budget == 0 && pol != "ball_rng" && !bubble_auto) return 0;
     {
         std::ifstream mf(folder + "_anqi_meta.txt");
         double mstar = 0.0, scale = 1.0;
@@ -240,9 +244,16 @@ int configure_edge_policy(uint32_t n, uint32_t M, const std::string& folder, anq
         const char* occ = std::getenv("ANQI_EDGE_BALL_OCCLUSION");
         E.occlude_all = !(occ && std::string(occ) == "any");
     }
-    std::fprintf(stderr, "[edge-policy] %s enabled ks=%s budget=%u lambda=%.4g scale2=%.6g nk=%u%s%s\n", pol.c_str(),
+    E.bubble_auto = bubble_auto;
+    E.bubble_full = pol == "bubble_auto_full";
+    std::string auto_note;
+    if (bubble_auto)
+        auto_note = " auto_candidates=" + (E.bubble_full ? std::string("all") : std::to_string(E.auto_candidates)) +
+                    " auto_max_swaps=" + std::to_string(E.auto_max_swaps);
+    std::fprintf(stderr, "[edge-policy] %s enabled ks=%s budget=%u lambda=%.4g scale2=%.6g nk=%u%s%s%s\n", pol.c_str(),
                  std::getenv("ANQI_EDGE_KS") ? std::getenv("ANQI_EDGE_KS") : "10,50,100", E.budget, E.lambda,
-                 E.scale2, E.nk, E.ball ? " ball_occlusion=" : "", E.ball ? (E.occlude_all ? "all" : "any") : "");
+                 E.scale2, E.nk, E.ball ? " ball_occlusion=" : "", E.ball ? (E.occlude_all ? "all" : "any") : "",
+                 auto_note.c_str());
     return 0;
 }
 
@@ -324,13 +335,6 @@ void check_configuration() {
     if (std::getenv("ANQI_SPECIFIC")) {   // rknn_query.cpp:1806-1811
         std::fprintf(stderr, "[rknn] independent postfilter verifier requires a shared graph; unset ANQI_SPECIFIC\n");
         std::exit(2);
-    }
-    {
-        const char* pol = std::getenv("ANQI_EDGE_POLICY");
-        if (pol && (std::string(pol) == "bubble_auto" || std::string(pol) == "bubble_auto_full")) {
-            std::fprintf(stderr, "[rknn] ANQI_EDGE_POLICY=%s is not supported by the GPU build\n", pol);
-            std::exit(2);
-        }
     }
     for (const Pin& p : pins()) {
         const char* v = std::getenv(p.name);

@@ -1014,6 +1014,10 @@ static bool edge_policy_dev(anqi_ctx* ctx, const anqi_edge_policy* e, uint32_t M
     out.mu = e->mu; out.resid = e->resid; out.nk = e->nk; out.nks = e->nks;
     out.budget = std::min(e->budget, M);
     for (uint32_t t = 0; t < e->nks; t++) out.ks[t] = e->ks[t];
+    out.bubble_auto = e->bubble_auto && e->ball;
+    out.bubble_full = e->bubble_full;
+    out.auto_candidates = std::max(1u, e->auto_candidates);
+    out.auto_max_swaps = std::max(1u, e->auto_max_swaps);
     return true;
 }
 

@@ -8,9 +8,39 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
-// scratch bytes per pool entry: two sorted orders of (u64 key, u32 id, f32 d)
-__host__ __device__ __forceinline__ size_t policy_scratch_bytes(uint32_t cap2) {
-    return (size_t)cap2 * (2 * 8 + 2 * 4 + 4);
+__host__ __device__ __forceinline__ size_t pol_r16(size_t b) { return (b + 15) & ~size_t(15); }
+__host__ __device__ __forceinline__ uint32_t pol_p2(uint32_t v) {
+    uint32_t p = 64;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// The bubble optimizer's sizes for pools of up to pmax entries: universe
+// (the selected ids and the alternatives), targets, signature words.
+struct BubbleDims {
+    uint32_t nu, nt, nt2, words;
+};
+__host__ __device__ __forceinline__ BubbleDims bubble_dims(const EdgePolicyDev& E, uint32_t pmax, uint32_t M) {
+    BubbleDims b;
+    b.nu = M + (E.bubble_full ? pmax : (E.auto_candidates < pmax ? E.auto_candidates : pmax));
+    b.nt = E.bubble_full ? pmax : b.nu;
+    b.nt2 = pol_p2(b.nt);
+    b.words = (b.nt + 63) / 64;
+    return b;
+}
+__host__ __device__ __forceinline__ size_t bubble_scratch_bytes(const EdgePolicyDev& E, uint32_t pmax, uint32_t M) {
+    if (!E.bubble_auto) return 0;
+    const BubbleDims b = bubble_dims(E, pmax, M);
+    return pol_r16(4 * (size_t)b.nu) + pol_r16(4 * (size_t)b.nt2) + pol_r16(4 * (size_t)b.nt2) +
+           pol_r16(8 * (size_t)b.nt2) + pol_r16(8 * (size_t)b.nu) + pol_r16(16 * (size_t)b.nu * b.words) +
+           pol_r16(8 * (size_t)b.nt) + pol_r16(4 * (size_t)M) + pol_r16(b.nu) + 3 * pol_r16(8 * (size_t)b.words);
+}
+
+// scratch bytes of a policy prune over pools of up to pmax entries (cap2 the
+// sorted size): two sorted orders of (u64 key, u32 id, f32 d), and the
+// bubble optimizer's arrays
+__host__ __device__ __forceinline__ size_t policy_scratch_bytes(uint32_t cap2, const EdgePolicyDev& E, uint32_t M) {
+    return pol_r16((size_t)cap2 * (2 * 8 + 2 * 4 + 4)) + bubble_scratch_bytes(E, cap2, M);
 }
 
 namespace edgepol {
@@ -204,6 +234,213 @@ __device__ void cover(const float* x, const float* nrm, uint32_t n, int d, uint3
     (void)cap2;
 }
 
+// optimize_verification_constrained_bubble (:521-723) over the selected ids
+// S (the plain prune's): universe = S + the first alternatives of the ball
+// order (bid, w of them) not in it; targets = the pool, or the universe's own
+// pool entries, by (d, id); greedy swaps (at most min(max_swaps,
+// alternatives)) that keep every target some selected edge verifies
+// verified and maximise (route coverage gain, power key gain), the first
+// such (candidate, slot) pair in scan order.  S's ids are replaced in place.
+__device__ void bubble(const float* x, const float* nrm, int d, uint32_t M, float alpha, const EdgePolicyDev& E,
+                       uint32_t base, const float* pd, const uint32_t* pi, uint32_t w, const uint32_t* bid,
+                       char* scr, uint32_t pmax, Sel& S) {
+    const uint32_t lane = (uint32_t)__lane_id();
+    const int l = (int)(lane & 15), g = (int)(lane >> 4);
+    if (!E.bubble_auto || S.deg == 0 || w == 0) return;
+    const BubbleDims B = bubble_dims(E, pmax, M);
+    size_t o = 0;
+    auto take = [&](size_t bytes) { char* p = scr + o; o += pol_r16(bytes); return p; };
+    uint32_t* U = (uint32_t*)take(4 * (size_t)B.nu);
+    uint32_t* tid = (uint32_t*)take(4 * (size_t)B.nt2);
+    float* td = (float*)take(4 * (size_t)B.nt2);
+    uint64_t* tk = (uint64_t*)take(8 * (size_t)B.nt2);
+    double* pk = (double*)take(8 * (size_t)B.nu);
+    uint64_t* ver = (uint64_t*)take(16 * (size_t)B.nu * B.words);
+    uint64_t* rou = ver + (size_t)B.nu * B.words;
+    int* vc = (int*)take(8 * (size_t)B.nt);
+    int* rc = vc + B.nt;
+    uint32_t* sel = (uint32_t*)take(4 * (size_t)M);
+    uint8_t* is_sel = (uint8_t*)take(B.nu);
+    uint64_t* unc = (uint64_t*)take(8 * (size_t)B.words);
+    uint64_t* rc1 = (uint64_t*)take(8 * (size_t)B.words);
+    uint64_t* need = (uint64_t*)take(8 * (size_t)B.words);   // required, verified by exactly one
+    // universe
+    uint32_t nu = S.deg;
+    for (uint32_t t = lane; t < S.deg; t += 64) U[t] = S.id[t];
+    __syncthreads();
+    const uint32_t limit = E.bubble_full ? w : E.auto_candidates;
+    uint32_t alts = 0;
+    for (uint32_t t = 0; t < w && alts < limit; t++) {
+        const uint32_t c = bid[t];
+        int in = 0;
+        for (uint32_t u0 = 0; u0 < nu && !in; u0 += 64) in = __any(u0 + lane < nu && U[u0 + lane] == c);
+        if (in) continue;
+        if (lane == 0) U[nu] = c;
+        nu++;
+        alts++;
+        __syncthreads();
+    }
+    if (alts == 0) return;
+    // targets
+    uint32_t nt = 0;
+    if (E.bubble_full) {
+        for (uint32_t t = lane; t < w; t += 64) { tid[t] = pi[t]; td[t] = pd[t]; }
+        nt = w;
+    } else {
+        for (uint32_t u = lane; u < nu; u += 64) {
+            float dv = 0.0f;
+            for (uint32_t t = 0; t < w; t++)
+                if (pi[t] == U[u]) { dv = pd[t]; break; }
+            tid[u] = U[u];
+            td[u] = dv;
+            tk[u] = dkey((double)dv);
+        }
+        nt = nu;
+        __syncthreads();
+        sort_keys(tk, tid, td, nt, pol_p2(nt));
+    }
+    __syncthreads();
+    const uint32_t words = (nt + 63) / 64;
+    // signatures: verify / route bits of every (universe, target) pair; power keys
+    for (uint32_t u = 0; u < nu; u++) {
+        const uint32_t id = U[u];
+        QRow QU;
+        qrow_load(QU, x + (size_t)id * d, d, l);
+        const float nu_ = nrm[id];
+        if (lane == 0) {
+            float db = 0.0f;
+            for (uint32_t t = 0; t < w; t++)
+                if (pi[t] == id) { db = pd[t]; break; }
+            pk[u] = fma(-E.lambda, sort_radj(E, id), (double)db);
+        }
+        // 64 targets per word: each 16-lane group takes every fourth one,
+        // the groups' bits are merged in registers and one lane stores the
+        // word (plain stores, as every later read of them)
+        for (uint32_t t0 = 0; t0 < nt; t0 += 64) {
+            uint64_t vb = 0, rb = 0;
+            for (uint32_t s4 = 0; s4 < 64; s4 += 4) {
+                const uint32_t t = t0 + s4 + (uint32_t)g;
+                if (t < nt) {
+                    const uint32_t tg = tid[t];
+                    float dsc = 0.0f;
+                    if (tg != id) {
+                        const float ip = qrow_ip(QU, x + (size_t)tg * d, d, l);
+                        dsc = d2_combine(ip, nu_, nrm[tg]);
+                    }
+                    if (tg == id || alpha * dsc < td[t]) vb |= 1ull << (s4 + (uint32_t)g);
+                    if (ball_occ(E, alpha, base, id, td[t], dsc)) rb |= 1ull << (s4 + (uint32_t)g);
+                }
+            }
+            vb |= __shfl_xor(vb, 16);
+            vb |= __shfl_xor(vb, 32);
+            rb |= __shfl_xor(rb, 16);
+            rb |= __shfl_xor(rb, 32);
+            if (lane == 0) {
+                ver[(size_t)u * B.words + (t0 >> 6)] = vb;
+                rou[(size_t)u * B.words + (t0 >> 6)] = rb;
+            }
+        }
+        __syncthreads();
+    }
+    // the selected edges, in their order
+    for (uint32_t u = lane; u < nu; u += 64) is_sel[u] = 0;
+    __syncthreads();
+    uint32_t ns = 0;
+    for (uint32_t s = 0; s < S.deg; s++) {
+        const uint32_t id = S.id[s];
+        uint32_t found = U32_NONE;
+        for (uint32_t u0 = 0; u0 < nu && found == U32_NONE; u0 += 64) {
+            const uint64_t m = __ballot(u0 + lane < nu && U[u0 + lane] == id);
+            if (m) found = u0 + (uint32_t)(__ffsll((long long)m) - 1);
+        }
+        if (found != U32_NONE) {
+            if (lane == 0) { sel[ns] = found; is_sel[found] = 1; }
+            ns++;
+        }
+        __syncthreads();
+    }
+    if (ns == 0) return;
+    auto bit = [&](const uint64_t* b, uint32_t u, uint32_t t) -> int {
+        return (int)((b[(size_t)u * B.words + (t >> 6)] >> (t & 63u)) & 1ull);
+    };
+    for (uint32_t t = lane; t < nt; t += 64) {
+        int a = 0, r = 0;
+        for (uint32_t s = 0; s < ns; s++) { a += bit(ver, sel[s], t); r += bit(rou, sel[s], t); }
+        vc[t] = a;
+        rc[t] = r;
+    }
+    __syncthreads();
+    // required targets: verified before any swap (fixed from here on)
+    uint64_t* req = tk;   // the target keys are no longer needed
+    for (uint32_t wd = lane; wd < words; wd += 64) {
+        uint64_t m = 0;
+        for (uint32_t b2 = 0; b2 < 64 && wd * 64 + b2 < nt; b2++)
+            if (vc[wd * 64 + b2] > 0) m |= 1ull << b2;
+        req[wd] = m;
+    }
+    __syncthreads();
+    const uint32_t max_swaps = min(E.auto_max_swaps, alts);
+    for (uint32_t swaps = 0; swaps < max_swaps; swaps++) {
+        for (uint32_t wd = lane; wd < words; wd += 64) {
+            uint64_t a = 0, b1 = 0, c1 = 0;
+            for (uint32_t b2 = 0; b2 < 64 && wd * 64 + b2 < nt; b2++) {
+                const uint32_t t = wd * 64 + b2;
+                if (rc[t] == 0) a |= 1ull << b2;
+                if (rc[t] == 1) b1 |= 1ull << b2;
+                if (vc[t] == 1) c1 |= 1ull << b2;
+            }
+            unc[wd] = a;
+            rc1[wd] = b1;
+            need[wd] = c1 & req[wd];
+        }
+        __syncthreads();
+        // every (candidate q, slot) pair in scan order, one per lane at a time
+        long best_rg = LONG_MIN;
+        double best_pg = -INFINITY;
+        uint32_t best_p = U32_NONE;
+        const uint32_t npairs = nu * ns;
+        for (uint32_t p = lane; p < npairs; p += 64) {
+            const uint32_t q = p / ns, s = p % ns;
+            if (is_sel[q]) continue;
+            const uint32_t old = sel[s];
+            bool feasible = true;
+            long newly = 0, lost = 0;
+            for (uint32_t wd = 0; wd < words; wd++) {
+                const uint64_t ov = ver[(size_t)old * B.words + wd], qv = ver[(size_t)q * B.words + wd];
+                const uint64_t orr = rou[(size_t)old * B.words + wd], qr = rou[(size_t)q * B.words + wd];
+                if (need[wd] & ov & ~qv) { feasible = false; break; }
+                newly += __popcll(qr & unc[wd]);
+                lost += __popcll(orr & rc1[wd] & ~qr);
+            }
+            if (!feasible) continue;
+            const long rg = newly - lost;
+            const double pg = pk[old] - pk[q];
+            if (rg > best_rg || (rg == best_rg && pg > best_pg)) { best_rg = rg; best_pg = pg; best_p = p; }
+        }
+        // the wave's best: largest (gain, power gain), then the first pair
+        for (int off = 32; off > 0; off >>= 1) {
+            const long org = __shfl_xor(best_rg, off);
+            const double opg = __shfl_xor(best_pg, off);
+            const uint32_t op = __shfl_xor(best_p, off);
+            const bool take_o = op != U32_NONE &&
+                                (best_p == U32_NONE || org > best_rg || (org == best_rg && opg > best_pg) ||
+                                 (org == best_rg && opg == best_pg && op < best_p));
+            if (take_o) { best_rg = org; best_pg = opg; best_p = op; }
+        }
+        if (best_p == U32_NONE || best_rg < 0 || (best_rg == 0 && best_pg <= 1e-12)) break;
+        const uint32_t bq = best_p / ns, bs = best_p % ns, old = sel[bs];
+        for (uint32_t t = lane; t < nt; t += 64) {
+            vc[t] += bit(ver, bq, t) - bit(ver, old, t);
+            rc[t] += bit(rou, bq, t) - bit(rou, old, t);
+        }
+        __syncthreads();
+        if (lane == 0) { is_sel[old] = 0; is_sel[bq] = 1; sel[bs] = bq; }
+        __syncthreads();
+    }
+    for (uint32_t s = lane; s < ns; s += 64) { S.id[s] = U[sel[s]]; S.out[s] = U[sel[s]]; }
+    __syncthreads();
+}
+
 }  // namespace edgepol
 
 // prune_pool_to_ids (:772-829) under an edge policy, over a (dist, id)-sorted
@@ -220,6 +457,7 @@ __device__ uint32_t policy_select(const float* x, const float* nrm, uint32_t n, 
     uint32_t* lid = (uint32_t*)(bk + cap2);
     uint32_t* bid = lid + cap2;
     float* bd = (float*)(bid + cap2);
+    char* bscr = scratch + pol_r16((size_t)cap2 * (2 * 8 + 2 * 4 + 4));
     Sel S{selv, sel, selnrm, out, 0};
     const uint32_t bud = min(E.budget, M);
     auto build_lsg = [&]() {
@@ -234,7 +472,11 @@ __device__ uint32_t policy_select(const float* x, const float* nrm, uint32_t n, 
         }
         sort_keys(bk, bid, bd, w, cap2);
     };
-    if (E.lsg && E.ball && E.lsg_alpha != 0.0) {
+    if (E.bubble_auto) {
+        build_ball();
+        extend<0>(x, nrm, d, alpha, E, base, pi, pd, nullptr, w, M, S);
+        bubble(x, nrm, d, M, alpha, E, base, pd, pi, w, bid, bscr, cap2, S);
+    } else if (E.lsg && E.ball && E.lsg_alpha != 0.0) {
         build_lsg();
         build_ball();
         const uint32_t cap = (M - bud) ? (M - bud) : M;

@@ -528,7 +528,7 @@ bool gather_register_lists(const GatherParams& P, bool graph) { return gather_rk
 size_t gather_lds_bytes(const GatherParams& P, bool graph, bool rk) {
     size_t pool = graph ? 2 * rnd16(4 * (size_t)P.pmax) : (rk ? rnd16(gather_xbytes(P, graph) * (size_t)P.K) : 4 * rnd16(4 * (size_t)P.K));
     return gather_region0(P, graph, rk) + pool + 2 * rnd16(4 * (size_t)gather_tcap(P, graph)) + (rk ? 0 : rnd16(8 * 64)) +
-           ((graph && P.pol.on) ? rnd16(policy_scratch_bytes(std::max<uint32_t>(P.pmax, 64))) : 0);
+           ((graph && P.pol.on) ? rnd16(policy_scratch_bytes(std::max<uint32_t>(P.pmax, 64), P.pol, P.M)) : 0);
 }
 
 // rk: register lists; the queue's exchange area is qd0 (explore entries), the
@@ -567,7 +567,7 @@ __device__ __forceinline__ GatherLds gather_carve(char* base, const GatherParams
     s.tid = (uint32_t*)take(4 * (size_t)gather_tcap(P, graph));
     s.tdv = (float*)take(4 * (size_t)gather_tcap(P, graph));
     s.cs = rk ? nullptr : (uint64_t*)take(8 * 64);
-    s.pscratch = (graph && P.pol.on) ? take(policy_scratch_bytes(max(P.pmax, 64u))) : nullptr;
+    s.pscratch = (graph && P.pol.on) ? take(policy_scratch_bytes(max(P.pmax, 64u), P.pol, P.M)) : nullptr;
     return s;
 }
 
@@ -1033,7 +1033,7 @@ hipError_t launch_gather(const GatherParams& P, bool graph, bool gws, hipStream_
 // ---- commit: cand = O_i u R_i, prune to M (include/index.h:907-930) -----------------
 size_t commit_lds_bytes(const PruneParams& P) {
     return 2 * rnd16(4 * (size_t)P.pmax) + rnd16(4 * (size_t)P.M * P.d) + rnd16(4 * (size_t)P.M) +
-           rnd16(4 * 64) + (P.pol.on ? rnd16(policy_scratch_bytes(std::max<uint32_t>(P.pmax, 64))) : 0);
+           rnd16(4 * 64) + (P.pol.on ? rnd16(policy_scratch_bytes(std::max<uint32_t>(P.pmax, 64), P.pol, P.M)) : 0);
 }
 
 // MC = (d >> 4) for the batched distance pass (0: one row per group at a time).

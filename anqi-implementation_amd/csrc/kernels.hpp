@@ -188,6 +188,8 @@ struct EdgePolicyDev {
     const float* resid;         // n x nk residuals
     uint32_t nk, nks, budget;
     int32_t ks[16];             // 1-based ranks
+    int bubble_auto, bubble_full;              // optimize_verification_constrained_bubble
+    uint32_t auto_candidates, auto_max_swaps;
 };
 
 struct GatherParams {

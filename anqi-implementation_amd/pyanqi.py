@@ -43,7 +43,8 @@ class EdgePolicy(ctypes.Structure):
     """include/anqi.h anqi_edge_policy (device pointers for mu and resid)."""
     _fields_ = [("resid_cover", I32), ("ball", I32), ("ball_pure", I32), ("occlude_all", I32), ("lsg", I32),
                 ("lsg_alpha", F64), ("mu", P), ("resid", P), ("nk", U32), ("ks", I32 * 16), ("nks", U32),
-                ("budget", U32), ("lambda_", F64), ("scale2", F64)]
+                ("budget", U32), ("lambda_", F64), ("scale2", F64), ("bubble_auto", I32), ("bubble_full", I32),
+                ("auto_candidates", U32), ("auto_max_swaps", U32)]
 
 
 class GraphParams(ctypes.Structure):

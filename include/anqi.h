@@ -269,10 +269,10 @@ int anqi_tree_free(anqi_ctx* ctx, anqi_tree* t);
  * configure_edge_residual_cover (:305-438) leaves it: every prune of the build
  * (the gathered pools, commit_edges and the Vamana passes) follows
  * prune_pool_to_ids's branch for these flags.  All flags zero: the plain
- * alpha-RNG prune.  bubble_auto / bubble_auto_full have no form here. */
+ * alpha-RNG prune. */
 typedef struct {
     int resid_cover;    /* "residual_cover" */
-    int ball;           /* "ball_*", "bubble_*" (but bubble_auto*), "lsg_bubble_hybrid" */
+    int ball;           /* "ball_*", "bubble_*", "lsg_bubble_hybrid" */
     int ball_pure;      /* "ball_rng", "bubble_rng" */
     int occlude_all;    /* ANQI_EDGE_BALL_OCCLUSION other than "any" */
     int lsg;            /* "lsg_rng", "lsg_bubble_hybrid" */
@@ -285,6 +285,10 @@ typedef struct {
     uint32_t budget;    /* min(ANQI_EDGE_RESID_BUDGET, M), 16 */
     double lambda;      /* ANQI_EDGE_RESID_LAMBDA, 1 */
     double scale2;      /* the lift scale squared (_anqi_meta.txt), 1 when absent */
+    int bubble_auto;    /* "bubble_auto", "bubble_auto_full" (with ball set) */
+    int bubble_full;    /* "bubble_auto_full": every pool entry an alternative and a target */
+    uint32_t auto_candidates;  /* ANQI_EDGE_AUTO_CANDIDATES, M */
+    uint32_t auto_max_swaps;   /* ANQI_EDGE_AUTO_MAX_SWAPS, M */
 } anqi_edge_policy;
 
 typedef struct {

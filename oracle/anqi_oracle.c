@@ -55,6 +55,7 @@
 #include "anqi_oracle.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1419,6 +1420,105 @@ static uint32_t pol_cover(const or_edge_policy* E, uint32_t n, uint32_t base, co
     return deg;
 }
 
+/* optimize_verification_constrained_bubble (:521-723): with the plain
+ * prune's ids selected, greedy swaps against the top alternatives in the ball
+ * order (or the whole pool) that keep every target some selected edge
+ * verifies (alpha d2(s, t) < d2(base, t), or s == t) verified, and raise
+ * the number of targets a selected edge ball-occludes; ties by the power key
+ * gain.  ids (deg of them) are replaced in place. */
+static void pol_bubble(const float* x, const float* nrm, uint32_t d, float alpha, const or_edge_policy* E,
+                       uint32_t base, const float* pd, const uint32_t* pi, uint32_t w, const pol_key* bp,
+                       uint32_t* ids, uint32_t deg) {
+    if (!E->bubble_auto || deg == 0 || w == 0) return;
+    uint32_t* U = (uint32_t*)malloc(sizeof(uint32_t) * (deg + w));
+    uint32_t nu = 0;
+    for (uint32_t t = 0; t < deg; t++) U[nu++] = ids[t];
+    const uint32_t limit = E->bubble_full ? w : E->auto_candidates;
+    uint32_t alts = 0;
+    for (uint32_t t = 0; t < w; t++) {
+        if (pol_has(U, nu, bp[t].id)) continue;
+        U[nu++] = bp[t].id;
+        if (++alts >= limit) break;
+    }
+    if (alts == 0) { free(U); return; }
+    /* targets: the whole pool, or the universe's own entries, by (d, id) */
+    uint32_t nt = 0;
+    float* td = (float*)malloc(sizeof(float) * (w + nu));
+    uint32_t* tid = (uint32_t*)malloc(sizeof(uint32_t) * (w + nu));
+    if (E->bubble_full) {
+        for (uint32_t t = 0; t < w; t++) { td[nt] = pd[t]; tid[nt] = pi[t]; nt++; }
+    } else {
+        for (uint32_t u = 0; u < nu; u++)
+            for (uint32_t t = 0; t < w; t++)
+                if (pi[t] == U[u]) { td[nt] = pd[t]; tid[nt] = pi[t]; nt++; break; }
+        or_sort_pairs(td, tid, nt);
+    }
+    uint8_t* ver = (uint8_t*)calloc((size_t)nu * nt, 1);
+    uint8_t* rou = (uint8_t*)calloc((size_t)nu * nt, 1);
+    double* pk = (double*)malloc(sizeof(double) * nu);
+    for (uint32_t u = 0; u < nu; u++) {
+        float db = 0.0f;
+        for (uint32_t t = 0; t < w; t++) if (pi[t] == U[u]) { db = pd[t]; break; }
+        pk[u] = fma(-E->lambda, pol_sort_radj(E, U[u]), (double)db);
+        for (uint32_t t = 0; t < nt; t++) {
+            const float dsc = U[u] != tid[t] ? pol_d2(x, nrm, d, U[u], tid[t]) : 0.0f;
+            ver[(size_t)u * nt + t] = U[u] == tid[t] || alpha * dsc < td[t];
+            rou[(size_t)u * nt + t] = (uint8_t)pol_ball_occ(E, alpha, base, U[u], td[t], dsc);
+        }
+    }
+    uint32_t* sel = (uint32_t*)malloc(sizeof(uint32_t) * deg);
+    uint8_t* is_sel = (uint8_t*)calloc(nu, 1);
+    uint32_t ns = 0;
+    for (uint32_t t = 0; t < deg; t++)
+        for (uint32_t u = 0; u < nu; u++)
+            if (U[u] == ids[t]) { sel[ns++] = u; is_sel[u] = 1; break; }
+    int* vc = (int*)calloc(nt, sizeof(int));
+    int* rc = (int*)calloc(nt, sizeof(int));
+    uint8_t* req = (uint8_t*)calloc(nt, 1);
+    for (uint32_t s2 = 0; s2 < ns; s2++)
+        for (uint32_t t = 0; t < nt; t++) { vc[t] += ver[(size_t)sel[s2] * nt + t]; rc[t] += rou[(size_t)sel[s2] * nt + t]; }
+    for (uint32_t t = 0; t < nt; t++) req[t] = vc[t] > 0;
+    uint32_t swaps = 0;
+    const uint32_t max_swaps = E->auto_max_swaps < alts ? E->auto_max_swaps : alts;
+    while (swaps < max_swaps) {
+        long best_rg = LONG_MIN;
+        double best_pg = -INFINITY;
+        uint32_t bq = UINT32_MAX, bs = UINT32_MAX;
+        for (uint32_t q = 0; q < nu; q++) {
+            if (is_sel[q]) continue;
+            long newly = 0;
+            for (uint32_t t = 0; t < nt; t++) newly += rc[t] == 0 && rou[(size_t)q * nt + t];
+            for (uint32_t s2 = 0; s2 < ns; s2++) {
+                const uint32_t old = sel[s2];
+                int feasible = 1;
+                long lost = 0;
+                for (uint32_t t = 0; t < nt; t++) {
+                    const int ov = ver[(size_t)old * nt + t], qv = ver[(size_t)q * nt + t];
+                    if (req[t] && ov && vc[t] == 1 && !qv) { feasible = 0; break; }
+                    const int orr = rou[(size_t)old * nt + t], qr = rou[(size_t)q * nt + t];
+                    if (orr && rc[t] == 1 && !qr) lost++;
+                }
+                if (!feasible) continue;
+                const long rg = newly - lost;
+                const double pg = pk[old] - pk[q];
+                if (rg > best_rg || (rg == best_rg && pg > best_pg)) { best_rg = rg; best_pg = pg; bq = q; bs = s2; }
+            }
+        }
+        if (bq == UINT32_MAX || best_rg < 0 || (best_rg == 0 && best_pg <= 1e-12)) break;
+        const uint32_t old = sel[bs];
+        for (uint32_t t = 0; t < nt; t++) {
+            vc[t] += (int)ver[(size_t)bq * nt + t] - (int)ver[(size_t)old * nt + t];
+            rc[t] += (int)rou[(size_t)bq * nt + t] - (int)rou[(size_t)old * nt + t];
+        }
+        is_sel[old] = 0;
+        is_sel[bq] = 1;
+        sel[bs] = bq;
+        swaps++;
+    }
+    for (uint32_t s2 = 0; s2 < ns; s2++) ids[s2] = U[sel[s2]];
+    free(U); free(td); free(tid); free(ver); free(rou); free(pk); free(sel); free(is_sel); free(vc); free(rc); free(req);
+}
+
 /* prune_pool_to_ids (:772-829) under an edge policy; the sort_unique step as
  * in or_prune.  Policies whose configuration left every flag off prune like
  * or_prune. */
@@ -1463,7 +1563,10 @@ uint32_t or_prune_policy(const float* x, const float* nrm, uint32_t d, uint32_t 
     qsort(lp, w, sizeof(pol_key), pol_key_cmp);
     if (E->ball) qsort(bp, w, sizeof(pol_key), pol_key_cmp);
     const uint32_t bud = E->budget < M ? E->budget : M;
-    if (E->lsg && E->ball && E->lsg_alpha != 0.0) {
+    if (E->bubble_auto) {
+        deg = pol_rng(x, nrm, d, alpha, pool_d, pool_id, w, M, out, deg);
+        pol_bubble(x, nrm, d, alpha, E, base, pool_d, pool_id, w, bp, out, deg);
+    } else if (E->lsg && E->ball && E->lsg_alpha != 0.0) {
         uint32_t cap = M - bud;
         if (cap == 0) cap = M;
         deg = pol_rng_lsg(x, nrm, d, alpha, E, lp, w, cap, out, deg);

@@ -221,6 +221,8 @@ typedef struct {
     uint32_t nks;
     uint32_t budget;
     double lambda, scale2;
+    int bubble_auto, bubble_full;              /* bubble_auto, bubble_auto_full */
+    uint32_t auto_candidates, auto_max_swaps;  /* ANQI_EDGE_AUTO_CANDIDATES / _MAX_SWAPS */
 } or_edge_policy;
 uint32_t or_prune_policy(const float* x, const float* nrm, uint32_t d, uint32_t base, uint32_t n, float* pool_d,
                          uint32_t* pool_id, uint32_t m, uint32_t M, float alpha, const or_edge_policy* E,

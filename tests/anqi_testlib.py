@@ -364,7 +364,8 @@ class OrEdgePolicy(ctypes.Structure):
                 ("occlude_all", ctypes.c_int), ("lsg", ctypes.c_int), ("lsg_alpha", ctypes.c_double),
                 ("mu", ctypes.c_void_p), ("resid", ctypes.c_void_p), ("nk", ctypes.c_uint32),
                 ("ks", ctypes.c_void_p), ("nks", ctypes.c_uint32), ("budget", ctypes.c_uint32),
-                ("lambda_", ctypes.c_double), ("scale2", ctypes.c_double)]
+                ("lambda_", ctypes.c_double), ("scale2", ctypes.c_double), ("bubble_auto", ctypes.c_int),
+                ("bubble_full", ctypes.c_int), ("auto_candidates", ctypes.c_uint32), ("auto_max_swaps", ctypes.c_uint32)]
 
 
 def edge_policy_config(env, n, M, folder):
@@ -378,7 +379,7 @@ def edge_policy_config(env, n, M, folder):
     if not pol:
         return None
     f = dict(resid_cover=0, ball=0, ball_pure=0, occlude_all=1, lsg=0, lsg_alpha=1.0, mu=None, resid=None, nk=100,
-             ks=[], budget=16, lam=1.0, scale2=1.0)
+             ks=[], budget=16, lam=1.0, scale2=1.0, bubble_auto=0, bubble_full=0, auto_candidates=M, auto_max_swaps=M)
     if pol in ("lsg_rng", "lsg_bubble_hybrid"):
         f["lsg_alpha"] = float(env.get("ANQI_LSG_ALPHA", "1.0") or 0)
         f["mu"] = _np.fromfile(env["ANQI_LSG_MU_FILE"], dtype=_np.float32)
@@ -393,6 +394,8 @@ def edge_policy_config(env, n, M, folder):
     f["nk"] = nk
     f["budget"] = min(max(0, int(env.get("ANQI_EDGE_RESID_BUDGET", "16") or 0)), M)
     f["lam"] = float(env.get("ANQI_EDGE_RESID_LAMBDA", "1.0") or 0)
+    f["auto_candidates"] = max(1, int(env.get("ANQI_EDGE_AUTO_CANDIDATES", str(M)) or 0))
+    f["auto_max_swaps"] = max(1, int(env.get("ANQI_EDGE_AUTO_MAX_SWAPS", str(M)) or 0))
     ks = [int(t) for t in env.get("ANQI_EDGE_KS", "10,50,100").split(",") if t and int(t) > 0]
     f["ks"] = [k for k in ks if 1 <= k <= nk]
     if not f["ks"]:
@@ -409,6 +412,8 @@ def edge_policy_config(env, n, M, folder):
                               dtype=_np.float32)
     assert f["resid"].size == n * nk
     f["resid_cover"] = 1
+    f["bubble_auto"] = int(auto)
+    f["bubble_full"] = int(pol == "bubble_auto_full")
     if pol != "residual_cover":
         f["resid_cover"] = 0
         f["ball"] = 1
@@ -423,6 +428,7 @@ def or_edge_policy(f):
     ks = _np.array(f["ks"] or [0], dtype=_np.int32)
     E = OrEdgePolicy(f["resid_cover"], f["ball"], f["ball_pure"], f["occlude_all"], f["lsg"], f["lsg_alpha"],
                      ptr(f["mu"]) if f["mu"] is not None else None, ptr(f["resid"]) if f["resid"] is not None else None,
-                     f["nk"], ptr(ks), len(f["ks"]), f["budget"], f["lam"], f["scale2"])
+                     f["nk"], ptr(ks), len(f["ks"]), f["budget"], f["lam"], f["scale2"], f["bubble_auto"],
+                     f["bubble_full"], f["auto_candidates"], f["auto_max_swaps"])
     E._keep = (f, ks)
     return E

@@ -397,6 +397,8 @@ SAMPLE_VARIANTS = {
                           "ANQI_VAMANA_STYLE": "batch", "ANQI_VAMANA_L": "64"},
     "edge_lsg": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_EDGE_POLICY": "lsg_bubble_hybrid",
                  "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin", "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin"},
+    "edge_bubble_auto": {"ANQI_REUSE_ORIG_KNN": "1", "ANQI_EDGE_POLICY": "bubble_auto", "ANQI_EDGE_AUTO_CANDIDATES": "24",
+                         "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin"},
 }
 
 

@@ -93,8 +93,7 @@ def test_rknn_query_rejects_unpinned_configuration():
     instead of silently computing the pinned one."""
     pfx = os.path.join(GOLDEN, "sample", "sample")
     for env in ({"ANQI_RADIUS_MODE": "approx"}, {"ANQI_VERIFIER_PLACEMENT": "integrated"},
-                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_EDGE_POLICY": "bubble_auto"},
-                {"ANQI_EDGE_POLICY": "bubble_auto_full"}, {"ANQI_VAMANA": "0", "ANQI_VAMANA_L": "2000"},
+                {"ANQI_RECHECK": "1"}, {"ANQI_RANKM": "4"}, {"ANQI_VAMANA": "0", "ANQI_VAMANA_L": "2000"},
                 {"ANQI_RADIUS_RELAX": "0.5"}):
         r = run("rknn_query", ["16", pfx, "48", "500", "64", "600", "10"], env)
         assert r.returncode == 2, (env, r.returncode, r.stderr)
@@ -219,9 +218,13 @@ def test_u8_base_cache_key_and_bad_files_equal_reference(tmp_path):
      "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin", "ANQI_LSG_ALPHA": "0.75", "ANQI_REUSE_ORIG_KNN": "1"},
     {"ANQI_EDGE_POLICY": "lsg_rng", "ANQI_LSG_MU_FILE": "{dir}/edge_mu.bin", "ANQI_VAMANA": "1"},
     {"ANQI_EDGE_POLICY": "rng"},
+    {"ANQI_EDGE_POLICY": "bubble_auto", "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin",
+     "ANQI_EDGE_AUTO_CANDIDATES": "24", "ANQI_EDGE_RESID_BUDGET": "0"},
+    {"ANQI_EDGE_POLICY": "bubble_auto_full", "ANQI_EDGE_RESID_FILE": "{dir}/edge_resid.bin",
+     "ANQI_EDGE_AUTO_MAX_SWAPS": "3", "ANQI_REUSE_ORIG_KNN": "1"},
     {"ANQI_GRAPH_RADIUS_SOURCE_PFX": "{golden}/sample"},
 ], ids=["reuse", "vamana", "vamana_batch", "vamana_old", "edge_ball", "edge_lsg_hybrid", "edge_lsg", "edge_rng",
-        "radius_source_pfx"])
+        "edge_bubble_auto", "edge_bubble_auto_full", "radius_source_pfx"])
 def test_variant_cache_names_equal_reference(tmp_path, extra):
     """The build variants (ANQI_REUSE_ORIG_KNN, ANQI_VAMANA* in both styles)
     key the graph cache as the reference does (src/rknn_query.cpp:301-341,

@@ -256,7 +256,7 @@ def test_ingestion_options_give_the_same_answers(tmp_path):
 
 
 @pytest.mark.parametrize("variant", ["reuse", "vamana_global", "vamana_batch", "single_entry", "edge_ball",
-                                     "edge_cover_vamana", "edge_lsg"])
+                                     "edge_cover_vamana", "edge_lsg", "edge_bubble_auto"])
 def test_build_variants_equal_reference(tmp_path, variant):
     """Build variants of the reference (tests/golden/sample_variants, its own
     rknn_query on the sample): ANQI_REUSE_ORIG_KNN (no lifted LocalJoin,

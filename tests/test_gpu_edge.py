@@ -48,6 +48,8 @@ def device_policy(ctx, f):
     for t, k in enumerate(f["ks"]):
         E.ks[t] = k
     E.budget = f["budget"]
+    E.bubble_auto, E.bubble_full = f["bubble_auto"], f["bubble_full"]
+    E.auto_candidates, E.auto_max_swaps = f["auto_candidates"], f["auto_max_swaps"]
     return E, keep
 
 
@@ -65,12 +67,15 @@ def inputs(data):
 
 CASES = [("residual_cover", None), ("residual_cover_ks", "batch"), ("ball_rng", None), ("ball_hybrid", "global"),
          ("ball_hybrid_any", None), ("bubble_multik", None), ("bubble_rng_nk", "batch"), ("lsg_rng", None),
-         ("lsg_rng_half", None), ("lsg_bubble_hybrid", "global")]
+         ("lsg_rng_half", None), ("lsg_bubble_hybrid", "global"), ("bubble_auto", None),
+         ("bubble_auto_opts", "batch"), ("bubble_auto_full", None)]
 
 
 @pytest.mark.parametrize("data", ["sample", "mixture"])
 @pytest.mark.parametrize("name,vamana", CASES, ids=[f"{a}-{b}" for a, b in CASES])
 def test_policy_graph_equals_oracle(ctx, data, name, vamana):
+    if name == "bubble_auto_full" and data != "sample":
+        pytest.skip("the full bubble optimizer's oracle is cubic in the pool: sample only")
     x, ids = inputs(data)
     n = x.shape[0]
     with tempfile.TemporaryDirectory() as td:
@@ -96,4 +101,10 @@ def test_policy_graph_equals_oracle(ctx, data, name, vamana):
             say("lsg alpha 0.5:", diff)
             assert int(diff.split()[-2]) <= max(1, n // 100), diff
         else:
+            if diff:
+                for i in range(n):
+                    a = got[0][i, :got[1][i]].tolist()
+                    b = want[0][i, :want[1][i]].tolist()
+                    if a != b:
+                        say("row", i, "gpu", a, "oracle", b)
             assert diff == "", (data, name, vamana)

@@ -30,6 +30,10 @@ POLICIES = {
     "lsg_rng": {"ANQI_EDGE_POLICY": "lsg_rng"},
     "lsg_rng_half": {"ANQI_EDGE_POLICY": "lsg_rng", "ANQI_LSG_ALPHA": "0.5"},
     "lsg_bubble_hybrid": {"ANQI_EDGE_POLICY": "lsg_bubble_hybrid", "ANQI_EDGE_RESID_BUDGET": "12"},
+    "bubble_auto": {"ANQI_EDGE_POLICY": "bubble_auto"},
+    "bubble_auto_opts": {"ANQI_EDGE_POLICY": "bubble_auto", "ANQI_EDGE_AUTO_CANDIDATES": "20",
+                         "ANQI_EDGE_AUTO_MAX_SWAPS": "5", "ANQI_EDGE_RESID_BUDGET": "0", "ANQI_EDGE_BALL_OCCLUSION": "any"},
+    "bubble_auto_full": {"ANQI_EDGE_POLICY": "bubble_auto_full", "ANQI_EDGE_KS": "10,100"},
 }
 
 
@@ -150,6 +154,7 @@ def test_policies_change_the_prune():
 
 
 GRAPH_CASES = [
+    ("bubble_auto", None), ("bubble_auto_full", "global"),
     ("residual_cover", None), ("ball_hybrid", None), ("bubble_rng_nk", None), ("lsg_bubble_hybrid", None),
     ("lsg_rng", "batch"), ("ball_rng", "global"), ("residual_cover_ks", "batch"),
 ]
