@@ -151,6 +151,72 @@ __device__ __forceinline__ float qrow_l2(const QRow& q, const float* __restrict_
     return res;
 }
 
+// qrow_load / qrow_ip / qrow_l2 for rows of d = 16 MC + r (r < 16) with MC
+// known at compile time: every element of the row is requested before the
+// first one is used, so a row costs one memory round trip (with a run-time
+// chunk count each chunk's load sits in its own branch next to its FMA and
+// the loads go out one after another).  Same FMA order, reduction tree and
+// tail term as the run-time forms, hence the same bits.  MC = 0: the
+// run-time forms.
+template <int MC>
+__device__ __forceinline__ void qrow_load_mc(QRow& q, const float* __restrict__ x, int d, int l) {
+    if constexpr (MC == 0) {
+        qrow_load(q, x, d, l);
+    } else {
+        const int r = d - 16 * MC;
+#pragma unroll
+        for (int t = 0; t < QMAX_CHUNKS; t++) q.v[t] = (t < MC) ? x[16 * t + l] : 0.0f;
+        q.tail = (l < r) ? x[16 * MC + l] : 0.0f;
+    }
+}
+
+template <int MC>
+__device__ __forceinline__ float qrow_ip_mc(const QRow& q, const float* __restrict__ y, int d, int l) {
+    if constexpr (MC == 0) {
+        return qrow_ip(q, y, d, l);
+    } else {
+        const int r = d - 16 * MC;
+        float yv[MC];
+#pragma unroll
+        for (int t = 0; t < MC; t++) yv[t] = y[16 * t + l];
+        const float yt = (l < r) ? y[16 * MC + l] : 0.0f;
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < MC; t++) acc = fmaf(q.v[t], yv[t], acc);
+        float res = tree16(acc);
+        if (r) {
+            float p = (l < r) ? q.tail * yt : 0.0f;
+            res += tree16(p);
+        }
+        return res;
+    }
+}
+
+template <int MC>
+__device__ __forceinline__ float qrow_l2_mc(const QRow& q, const float* __restrict__ y, int d, int l) {
+    if constexpr (MC == 0) {
+        return qrow_l2(q, y, d, l);
+    } else {
+        const int r = d - 16 * MC;
+        float yv[MC];
+#pragma unroll
+        for (int t = 0; t < MC; t++) yv[t] = y[16 * t + l];
+        const float yt = (l < r) ? y[16 * MC + l] : 0.0f;
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < MC; t++) {
+            const float a = q.v[t] - yv[t];
+            acc = fmaf(a, a, acc);
+        }
+        float res = tree16(acc);
+        if (r) {
+            const float a = (l < r) ? q.tail - yt : 0.0f;
+            res += tree16(a * a);
+        }
+        return res;
+    }
+}
+
 // Batched row distances: for rows ids[s], s < k, group g (16 lanes) takes
 // s = g, g+4, ..., g + 4 (RB - 1) of each 4 RB-row pass and issues all RB
 // rows' loads before reducing any, so the fetches overlap; the last pass

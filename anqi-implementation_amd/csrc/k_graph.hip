@@ -595,7 +595,10 @@ __device__ void lds_sort_pairs(float* pd, uint32_t* pi, uint32_t m, uint32_t cap
 }
 
 // alpha-RNG selection over a (dist, id)-sorted, duplicate-free pool
-// (include/index.h:267-281); selected vectors are staged in LDS.
+// (include/index.h:267-281); selected vectors are staged in LDS.  MC = d >> 4
+// (or 0): each candidate row and each selected vector is requested whole
+// before its dot product starts.
+template <int MC>
 __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uint32_t* pi, uint32_t m,
                                float* selv, uint32_t* sel, uint32_t* out) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
@@ -606,14 +609,14 @@ __device__ uint32_t rng_select(const GatherParams& P, const float* pd, const uin
         uint32_t cid = pi[t];
         float d2c = pd[t];
         QRow QC;
-        qrow_load(QC, P.x + (size_t)cid * d, d, l);
+        qrow_load_mc<MC>(QC, P.x + (size_t)cid * d, d, l);
         float nc = P.nrm[cid];
         int occ = 0;
         for (uint32_t s0 = 0; s0 < deg; s0 += 4) {
             uint32_t s = s0 + (uint32_t)g;
             int o = 0;
             if (s < deg) {
-                float ip = qrow_ip(QC, selv + (size_t)s * d, d, l);
+                float ip = qrow_ip_mc<MC>(QC, selv + (size_t)s * d, d, l);
                 float dsc = d2_combine(ip, selnrm[s], nc);
                 o = (P.alpha * dsc < d2c);
             }
@@ -968,7 +971,7 @@ __device__ __forceinline__ void gather_row(const GatherParams& P, GatherLds S, u
         deg = policy_select(P.x, P.nrm, P.n, d, P.M, P.alpha, P.pol, i, S.pd, S.pi, w, S.pscratch, cap2, S.selv, S.sel,
                             pol_nrm, P.out_ids + (size_t)i * P.M);
     } else {
-        deg = rng_select(P, S.pd, S.pi, w, S.selv, S.sel, P.out_ids + (size_t)i * P.M);
+        deg = rng_select<MC>(P, S.pd, S.pi, w, S.selv, S.sel, P.out_ids + (size_t)i * P.M);
     }
     if (lane == 0) { P.out_cnt[i] = deg; P.status[i] = 0; }
 }
@@ -1099,7 +1102,7 @@ __global__ void __launch_bounds__(64) k_commit(PruneParams P) {
     } else {
         GatherParams G{};
         G.x = P.x; G.nrm = P.nrm; G.n = P.n; G.d = P.d; G.M = P.M; G.alpha = P.alpha;
-        dg = rng_select(G, pd, pi, w, selv, sel, P.links + (size_t)i * P.M);
+        dg = rng_select<MC>(G, pd, pi, w, selv, sel, P.links + (size_t)i * P.M);
     }
     for (uint32_t t = dg + (uint32_t)lane; t < P.M; t += 64) P.links[(size_t)i * P.M + t] = U32_NONE;
     if (lane == 0) { P.deg[i] = dg; P.status[i] = 0; }

@@ -467,15 +467,20 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     if (P.entries) {
         ep = P.entries[qi];
     } else {
+        // a node's children, offset and hyperplane are requested together
         uint32_t node = P.t_root;
-        while (P.t_left[node] != U32_NONE) {
-            float mg = qrow_ip(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l) - P.t_off[node];
-            node = (mg < -1e-6f) ? P.t_left[node] : (mg > 1e-6f ? P.t_right[node] : P.t_left[node]);
+        for (;;) {
+            const uint32_t lc = P.t_left[node], rc = P.t_right[node];
+            const float off = P.t_off[node];
+            const float ip = qrow_ip_mc<MC>(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l);
+            if (lc == U32_NONE) break;
+            const float mg = ip - off;
+            node = (mg < -1e-6f) ? lc : (mg > 1e-6f ? rc : lc);
         }
         ep = P.t_entry[node];
     }
-    const float dep = (MC > 0 && P.lmain) ? qrow_l2(QL, P.lmain + (size_t)ep * P.d, P.d, l) + P.ltail[ep]
-                                          : qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
+    const float dep = (MC > 0 && P.lmain) ? qrow_l2_mc<MC>(QL, P.lmain + (size_t)ep * P.d, P.d, l) + P.ltail[ep]
+                                          : qrow_l2_mc<MC>(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
     uint32_t nd = 1, rcnt = 0, bsize = 1, nx = 0;
     int buf = 0, ovf = 0;
     if (lane == 0) {
@@ -698,6 +703,8 @@ __global__ void __launch_bounds__(64) k_range_search_persistent(SearchParams P) 
 // (rptrees.h:108-123, ties go left).  Queries beyond nq (or *nq_dev) exit.
 // key (optional): the lifted distance from the query to its entry, the
 // scheduling key of k_entry_order.
+// MC = d / 16 = dl / 16 (the search's chunk count) or 0 (run-time form).
+template <int MC>
 __global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_t* __restrict__ out,
                                                          float* __restrict__ key) {
     const int lane = __lane_id(), l = lane & 15, g = lane >> 4;
@@ -713,18 +720,27 @@ __global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_
     QRow QL;
 #pragma unroll
     for (int t = 0; t < QMAX_CHUNKS; t++) QL.v[t] = 0.0f;
-    for (uint32_t t = 0; t < m; t++) {
-        const float v = lifted_q(16u * t + (uint32_t)l);
+    if constexpr (MC > 0) {
+        // 16 t + l < 16 MC <= d: the query's elements, all requested at once
+        float qv[MC];
 #pragma unroll
-        for (int u = 0; u < QMAX_CHUNKS; u++)
-            if ((uint32_t)u == t) QL.v[u] = v;
+        for (int t = 0; t < MC; t++) qv[t] = q[16 * t + l];
+#pragma unroll
+        for (int t = 0; t < MC; t++) QL.v[t] = sd_to_float(sd_mul(sd_from_float(qv[t]), P.scale_bits));
+    } else {
+        for (uint32_t t = 0; t < m; t++) {
+            const float v = lifted_q(16u * t + (uint32_t)l);
+#pragma unroll
+            for (int u = 0; u < QMAX_CHUNKS; u++)
+                if ((uint32_t)u == t) QL.v[u] = v;
+        }
     }
     QL.tail = ((uint32_t)l < r) ? lifted_q(16u * m + (uint32_t)l) : 0.0f;
     uint32_t node = P.t_root;
     for (;;) {
         const uint32_t left = P.t_left[node], right = P.t_right[node];
         const float off = P.t_off[node];
-        const float ip = qrow_ip(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l);
+        const float ip = qrow_ip_mc<MC>(QL, P.t_hyper + (size_t)node * P.dl, P.dl, l);
         if (left == U32_NONE) break;
         const float mg = ip - off;
         node = (mg < -1e-6f) ? left : (mg > 1e-6f ? right : left);
@@ -733,8 +749,8 @@ __global__ void __launch_bounds__(64) k_tree_entry_batch(SearchParams P, uint32_
     if (l == 0) out[qi] = ep;
     if (key) {
         const float dep = (P.d % 16 == 0 && P.lmain && P.ltail)
-                              ? qrow_l2(QL, P.lmain + (size_t)ep * P.d, P.d, l) + P.ltail[ep]
-                              : qrow_l2(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
+                              ? qrow_l2_mc<MC>(QL, P.lmain + (size_t)ep * P.d, P.d, l) + P.ltail[ep]
+                              : qrow_l2_mc<MC>(QL, P.lifted + (size_t)ep * P.dl, P.dl, l);
         if (l == 0) key[qi] = dep;
     }
 }
@@ -771,7 +787,11 @@ __global__ void __launch_bounds__(1024) k_entry_order(const float* __restrict__ 
 hipError_t launch_tree_entry_batch(const SearchParams& P, uint32_t max_nq, uint32_t* out, float* key,
                                    hipStream_t s) {
     if (max_nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tree_entry_batch, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out, key);
+    const uint32_t m = P.dl >> 4;
+    const int mc = ((P.d >> 4) == m && (m == 8 || m == 6)) ? (int)m : 0;
+    if (mc == 8) hipLaunchKernelGGL(k_tree_entry_batch<8>, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out, key);
+    else if (mc == 6) hipLaunchKernelGGL(k_tree_entry_batch<6>, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out, key);
+    else hipLaunchKernelGGL(k_tree_entry_batch<0>, dim3((max_nq + 3) / 4), dim3(64), 0, s, P, out, key);
     return hipGetLastError();
 }
 
