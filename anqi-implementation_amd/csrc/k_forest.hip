@@ -155,7 +155,9 @@ __device__ void wave_sort_lds(float* pd, uint32_t* pi, uint32_t m, uint32_t cap2
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   \
     } while (0)
 
-// 4 wavefronts per leaf; wavefront w handles rows w, w+4, ...
+// 4 wavefronts per leaf; wavefront w handles rows w, w+4, ...  MC = d >> 4
+// (or 0): a member's row is requested whole before its dot product starts.
+template <int MC>
 __global__ void __launch_bounds__(256) k_leaf_knn(LeafParams P) {
     extern __shared__ __align__(16) char lds[];
     const uint32_t leaf = blockIdx.x;
@@ -181,7 +183,7 @@ __global__ void __launch_bounds__(256) k_leaf_knn(LeafParams P) {
         float* rkd = P.keys + (size_t)r * P.K;
         uint64_t thr = cnt >= P.K ? pkey(rkd[P.K - 1], rid[P.K - 1]) : ~0ull;
         QRow QR;
-        qrow_load(QR, P.x + (size_t)r * d, d, l);
+        qrow_load_mc<MC>(QR, P.x + (size_t)r * d, d, l);
         float nr = P.nrm[r];
         uint32_t nc = 0;
         if (P.ipm) {
@@ -207,7 +209,7 @@ __global__ void __launch_bounds__(256) k_leaf_knn(LeafParams P) {
             uint32_t c = 0;
             if (b < size && b != a) {
                 c = mem[b];
-                float ip = qrow_ip(QR, P.x + (size_t)c * d, d, l);
+                float ip = qrow_ip_mc<MC>(QR, P.x + (size_t)c * d, d, l);
                 dv = d2_combine(ip, nr, P.nrm[c]);
                 ok = (l == 0) && (pkey(dv, c) < thr);
             }
@@ -274,8 +276,13 @@ hipError_t launch_leaf_knn(const LeafParams& P, bool mfma, hipStream_t s) {
     (void)mfma;
     if (P.nleaves == 0) return hipSuccess;
     size_t lds = leaf_lds_bytes(P);
-    if (hipError_t e = set_lds_limit((const void*)k_leaf_knn, lds); e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_leaf_knn, dim3(P.nleaves), dim3(256), lds, s, P);
+    const uint32_t mc = P.d >> 4;
+    const void* f = mc == 8 ? (const void*)k_leaf_knn<8> : mc == 6 ? (const void*)k_leaf_knn<6>
+                                                                   : (const void*)k_leaf_knn<0>;
+    if (hipError_t e = set_lds_limit(f, lds); e != hipSuccess) return e;
+    if (mc == 8) hipLaunchKernelGGL(k_leaf_knn<8>, dim3(P.nleaves), dim3(256), lds, s, P);
+    else if (mc == 6) hipLaunchKernelGGL(k_leaf_knn<6>, dim3(P.nleaves), dim3(256), lds, s, P);
+    else hipLaunchKernelGGL(k_leaf_knn<0>, dim3(P.nleaves), dim3(256), lds, s, P);
     return hipGetLastError();
 }
 

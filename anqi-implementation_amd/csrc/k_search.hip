@@ -551,8 +551,8 @@ __device__ void search_one(const SearchParams& P, SearchLds& S, uint32_t qi) {
     }
     // (3) postfilter in base space
     QRow QB;
-    qrow_load(QB, q, P.d, l);
-    float qn = qrow_ip(QB, q, P.d, l);
+    qrow_load_mc<MC>(QB, q, P.d, l);
+    float qn = qrow_ip_mc<MC>(QB, q, P.d, l);
     uint32_t npred = 0;
     uint32_t* out = P.cand_ids + (size_t)qi * P.out_cap;
     uint8_t* acc = P.accept + (size_t)qi * P.out_cap;
