@@ -1133,6 +1133,8 @@ hipError_t launch_commit(const PruneParams& P, bool gws, hipStream_t s) {
 
 // The search-tree entry of each row (rptrees.h:108-123, ties go left), the
 // row itself descending: four rows per wavefront, one per 16-lane group.
+// MC = dl >> 4 (or 0): each level's loads are issued together.
+template <int MC>
 __global__ void __launch_bounds__(64) k_tree_entry_rows(const float* __restrict__ x, uint32_t n, uint32_t dl,
                                                         const uint32_t* __restrict__ left,
                                                         const uint32_t* __restrict__ right,
@@ -1144,11 +1146,11 @@ __global__ void __launch_bounds__(64) k_tree_entry_rows(const float* __restrict_
     const uint32_t i = blockIdx.x * 4u + (uint32_t)g;
     if (i >= n) return;                         // whole 16-lane groups
     QRow QL;
-    qrow_load(QL, x + (size_t)i * dl, (int)dl, l);
+    qrow_load_mc<MC>(QL, x + (size_t)i * dl, (int)dl, l);
     uint32_t node = root;
     for (;;) {
         const uint32_t lc = left[node], rc = right[node];
-        const float ip = qrow_ip(QL, hyper + (size_t)node * dl, (int)dl, l);
+        const float ip = qrow_ip_mc<MC>(QL, hyper + (size_t)node * dl, (int)dl, l);
         if (lc == U32_NONE) break;
         const float mg = ip - off[node];
         node = (mg < -1e-6f) ? lc : (mg > 1e-6f ? rc : lc);
@@ -1160,8 +1162,13 @@ hipError_t launch_tree_entry_rows(const float* x, uint32_t n, uint32_t dl, const
                                   const uint32_t* right, const float* off, const float* hyper,
                                   const uint32_t* entry, uint32_t root, uint32_t* out, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tree_entry_rows, dim3((n + 3) / 4), dim3(64), 0, s, x, n, dl, left, right, off, hyper,
-                       entry, root, out);
+    const dim3 grid((n + 3) / 4);
+    if ((dl >> 4) == 8)
+        hipLaunchKernelGGL(k_tree_entry_rows<8>, grid, dim3(64), 0, s, x, n, dl, left, right, off, hyper, entry, root, out);
+    else if ((dl >> 4) == 6)
+        hipLaunchKernelGGL(k_tree_entry_rows<6>, grid, dim3(64), 0, s, x, n, dl, left, right, off, hyper, entry, root, out);
+    else
+        hipLaunchKernelGGL(k_tree_entry_rows<0>, grid, dim3(64), 0, s, x, n, dl, left, right, off, hyper, entry, root, out);
     return hipGetLastError();
 }
 
